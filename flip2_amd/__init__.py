@@ -1,0 +1,401 @@
+"""Python bindings of libparticles.so, the MI355X build of flip2's per-step
+particle pipeline (binning, radix sort, scans, sparse-voxel P2G, pressure
+solve, G2P/advection).
+
+The library is C++/HIP behind the C ABI in include/flip2.h; this module only
+marshals numpy arrays through ctypes.  There is no fallback: if the built
+library is missing or the GPU is absent, calls raise.
+
+Reference interfaces mirrored here (paths relative to the reference repo):
+  - class Particles (engine/particles.hu:11-45)
+  - cudaParallelPrefixSum (engine/algorithms/parallelPrefixSumKernels.hu:53-54)
+  - cudaRadixSortUint (engine/algorithms/radixSortKernels.hu:16-17)
+  - the ctypes load of libparticles.so in plugin/plugin.py:4-5
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REPO_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(REPO_ROOT, "build", "engine", "libparticles.so")
+HEADER_PATH = os.path.join(REPO_ROOT, "include", "flip2.h")
+
+_lib = None
+
+
+class Flip2Error(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"flip2 error {code}: {message}")
+        self.code = code
+
+
+class GridC(ctypes.Structure):
+    """flip2_grid: same layout as the reference's Grid (engine/grid.hpp:35-37)."""
+
+    _fields_ = [
+        ("negX", ctypes.c_float),
+        ("negY", ctypes.c_float),
+        ("negZ", ctypes.c_float),
+        ("sizeX", ctypes.c_uint32),
+        ("sizeY", ctypes.c_uint32),
+        ("sizeZ", ctypes.c_uint32),
+        ("cellSize", ctypes.c_float),
+    ]
+
+
+class RcStatsC(ctypes.Structure):
+    _fields_ = [
+        ("numUsedGridNodes", ctypes.c_uint32),
+        ("numUsedVoxels", ctypes.c_uint32),
+        ("lastSorIterations", ctypes.c_uint32),
+        ("lastSolveAttempts", ctypes.c_uint32),
+        ("lastResidual", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("frameDt", ctypes.c_double),
+        ("elapsedTime", ctypes.c_double),
+        ("elapsedTimeThisFrame", ctypes.c_double),
+        ("outOfTileEvents", ctypes.c_uint32),
+        ("relocatedParticles", ctypes.c_uint32),
+        ("substeps", ctypes.c_uint32),
+    ]
+
+
+def lib() -> ctypes.CDLL:
+    """Load build/engine/libparticles.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing; build it with `make -C flip2_amd/csrc` "
+                "or __graft_entry__.build()")
+        l = ctypes.CDLL(LIB_PATH)
+        l.flip2_last_error.restype = ctypes.c_char_p
+        l.flip2_scan_workspace_bytes.restype = ctypes.c_size_t
+        l.flip2_sort_workspace_bytes.restype = ctypes.c_size_t
+        l.flip2_scan_workspace_bytes.argtypes = [ctypes.c_uint32]
+        l.flip2_sort_workspace_bytes.argtypes = [ctypes.c_uint32] * 3
+        _declare(l)
+        _lib = l
+    return _lib
+
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_f64 = ctypes.c_double
+_sz = ctypes.c_size_t
+
+
+def _declare(l: ctypes.CDLL) -> None:
+    sig = {
+        "flip2_malloc": [ctypes.POINTER(_vp), _sz],
+        "flip2_free": [_vp],
+        "flip2_memcpy_htod": [_vp, _vp, _sz],
+        "flip2_memcpy_dtoh": [_vp, _vp, _sz],
+        "flip2_memcpy_dtod": [_vp, _vp, _sz],
+        "flip2_memset": [_vp, ctypes.c_int, _sz],
+        "flip2_stream_create": [ctypes.POINTER(_vp)],
+        "flip2_stream_destroy": [_vp],
+        "flip2_stream_synchronize": [_vp],
+        "flip2_timer_create": [ctypes.POINTER(_vp)],
+        "flip2_timer_destroy": [_vp],
+        "flip2_timer_start": [_vp, _vp],
+        "flip2_timer_stop": [_vp, _vp],
+        "flip2_timer_elapsed_ms": [_vp, ctypes.POINTER(ctypes.c_float)],
+        "flip2_scan_u32": [_vp, _vp, _u32, ctypes.c_int, _vp, _vp, _sz, _vp],
+        "flip2_sort_pairs_u32": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _sz, _vp],
+        "flip2_find_grid_cell": [_vp, _vp, _vp, _u32, GridC, _vp, _vp],
+        "flip2_philox4x32_10": [_vp, _u32, _vp],
+        "flip2_particles_create": [_u32, ctypes.POINTER(_vp)],
+        "flip2_particles_destroy": [_vp],
+        "flip2_set_domain": [_vp, _f64, _f64, _f64, _u32, _u32, _u32, _f64],
+        "flip2_set_positions": [_vp, _vp, _vp, _vp],
+        "flip2_set_velocities": [_vp, _vp, _vp, _vp],
+        "flip2_get_positions": [_vp, _vp, _vp, _vp],
+        "flip2_get_velocities": [_vp, _vp, _vp, _vp],
+        "flip2_get_courant_dt": [_vp, ctypes.POINTER(_f64)],
+        "flip2_apply_gravity": [_vp, _f64],
+        "flip2_remove_gravity": [_vp, _f64],
+        "flip2_gs_solve": [_vp, _f64, _f64, _u32, ctypes.POINTER(_f64), ctypes.POINTER(_u32)],
+        "flip2_gs_iterations": [_vp, _f64, _u32],
+        "flip2_solve_frame": [_vp, _f64],
+        "flip2_write_positions_to_file": [_vp, ctypes.c_char_p],
+        "flip2_get_stats": [_vp, ctypes.POINTER(RcStatsC)],
+        "flip2_get_array": [_vp, ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_sz)],
+        "flip2_set_array": [_vp, ctypes.c_char_p, _vp, _sz],
+        "flip2_set_time": [_vp, _f64, _f64, _f64, _f64],
+        "flip2_set_solver_limits": [_vp, _f64, _u32],
+    }
+    for name in ("align_particles_to_grid", "sort_particles", "generate_voxels", "particle_vel_to_voxels",
+                 "calc_div_u", "pressure_solve", "update_voxel_velocities", "voxel_vels_to_particles",
+                 "advect_particles", "move_solids", "move_solid_particles", "initialize"):
+        sig["flip2_" + name] = [_vp]
+    for name, args in sig.items():
+        fn = getattr(l, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise Flip2Error(rc, lib().flip2_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# --------------------------------------------------------------------------
+# Device memory
+# --------------------------------------------------------------------------
+class DeviceBuffer:
+    """Device allocation owned from Python (flip2_malloc / flip2_free)."""
+
+    def __init__(self, nbytes: int):
+        p = _vp()
+        check(lib().flip2_malloc(ctypes.byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().flip2_memcpy_htod(b.ptr, _ptr(a), a.nbytes))
+        return b
+
+    def to_array(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().flip2_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if a.nbytes:
+            check(lib().flip2_memcpy_htod(self.ptr, _ptr(a), a.nbytes))
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().flip2_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(stream: Optional[int] = None) -> None:
+    check(lib().flip2_stream_synchronize(stream))
+
+
+# --------------------------------------------------------------------------
+# Primitives (host-array conveniences over the device entry points)
+# --------------------------------------------------------------------------
+def scan_u32(values: np.ndarray, exclusive: bool = False) -> Tuple[np.ndarray, int]:
+    """Prefix scan mod 2^32 on the GPU; returns (scan, total)."""
+    values = np.ascontiguousarray(values, dtype=np.uint32)
+    n = values.size
+    d_in = DeviceBuffer.from_array(values)
+    d_out = DeviceBuffer(4 * n)
+    d_tot = DeviceBuffer(4)
+    ws_bytes = lib().flip2_scan_workspace_bytes(n)
+    d_ws = DeviceBuffer(ws_bytes)
+    check(lib().flip2_scan_u32(d_in.ptr, d_out.ptr, n, int(exclusive), d_tot.ptr, d_ws.ptr, ws_bytes, None))
+    synchronize()
+    return d_out.to_array(np.uint32, n), int(d_tot.to_array(np.uint32, 1)[0]) if n else 0
+
+
+def sort_pairs_u32(keys: np.ndarray, values: Optional[np.ndarray] = None, begin_bit: int = 0,
+                   end_bit: int = 32) -> Tuple[np.ndarray, np.ndarray]:
+    """Stable radix sort on the GPU; values=None returns the permutation."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = keys.size
+    d_k = DeviceBuffer.from_array(keys)
+    d_v = DeviceBuffer.from_array(np.ascontiguousarray(values, dtype=np.uint32)) if values is not None else None
+    d_ko, d_vo, d_ka, d_va = (DeviceBuffer(4 * n) for _ in range(4))
+    ws_bytes = lib().flip2_sort_workspace_bytes(n, begin_bit, end_bit)
+    d_ws = DeviceBuffer(ws_bytes)
+    check(lib().flip2_sort_pairs_u32(d_k.ptr, d_v.ptr if d_v else None, d_ko.ptr, d_vo.ptr, d_ka.ptr, d_va.ptr, n,
+                                     begin_bit, end_bit, d_ws.ptr, ws_bytes, None))
+    synchronize()
+    return d_ko.to_array(np.uint32, n), d_vo.to_array(np.uint32, n)
+
+
+def find_grid_cell(px, py, pz, grid: Sequence) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """rootCell on the GPU: returns clamped (px, py, pz) and keys."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (px, py, pz)]
+    n = arrs[0].size
+    d = [DeviceBuffer.from_array(a) for a in arrs]
+    dk = DeviceBuffer(4 * n)
+    g = GridC(*grid)
+    check(lib().flip2_find_grid_cell(d[0].ptr, d[1].ptr, d[2].ptr, n, g, dk.ptr, None))
+    synchronize()
+    return tuple(b.to_array(np.float64, n) for b in d) + (dk.to_array(np.uint32, n),)
+
+
+def philox4x32_10(ctr_key: np.ndarray) -> np.ndarray:
+    """Philox4x32-10 on the GPU for rows (c0, c1, c2, c3, k0, k1)."""
+    a = np.ascontiguousarray(ctr_key, dtype=np.uint32).reshape(-1, 6)
+    out = np.empty((a.shape[0], 4), dtype=np.uint32)
+    check(lib().flip2_philox4x32_10(_ptr(a), a.shape[0], _ptr(out)))
+    return out
+
+
+# --------------------------------------------------------------------------
+# Reference-compatible particle system
+# --------------------------------------------------------------------------
+_ARRAY_DTYPES = {
+    "px": np.float64, "py": np.float64, "pz": np.float64,
+    "vx": np.float64, "vy": np.float64, "vz": np.float64,
+    "gridCell": np.uint32, "reorderedGridIndices": np.uint32, "uniqueGridNodeIndices": np.uint32,
+    "gridNodeIndicesToFirstParticleIndex": np.uint32, "yDimNumUsedGridNodes": np.uint32,
+    "nodeIndexUsedVoxels": np.uint32, "voxelIDsUsed": np.uint32, "solids": np.int8,
+    "voxelsUx": np.float64, "voxelsUy": np.float64, "voxelsUz": np.float64,
+    "divU": np.float64, "p": np.float64, "residuals": np.float64,
+}
+
+
+class Particles:
+    """class Particles (engine/particles.hu:11-45) backed by the HIP build."""
+
+    def __init__(self, size: int):
+        h = _vp()
+        check(lib().flip2_particles_create(int(size), ctypes.byref(h)))
+        self._h = h.value
+        self.size = int(size)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().flip2_particles_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # particles.hu:11-45 -------------------------------------------------
+    def setDomain(self, nx, ny, nz, x, y, z, cellSize):
+        check(lib().flip2_set_domain(self._h, nx, ny, nz, x, y, z, cellSize))
+
+    def alignParticlesToGrid(self):
+        check(lib().flip2_align_particles_to_grid(self._h))
+
+    def sortParticles(self):
+        check(lib().flip2_sort_particles(self._h))
+
+    def generateVoxels(self):
+        check(lib().flip2_generate_voxels(self._h))
+
+    def particleVelToVoxels(self):
+        check(lib().flip2_particle_vel_to_voxels(self._h))
+
+    def pressureSolve(self):
+        check(lib().flip2_pressure_solve(self._h))
+
+    def updateVoxelVelocities(self):
+        check(lib().flip2_update_voxel_velocities(self._h))
+
+    def advectParticles(self):
+        check(lib().flip2_advect_particles(self._h))
+
+    def solveFrame(self, fps):
+        check(lib().flip2_solve_frame(self._h, float(fps)))
+
+    def initialize(self):
+        check(lib().flip2_initialize(self._h))
+
+    def voxelVelsToParticles(self):
+        check(lib().flip2_voxel_vels_to_particles(self._h))
+
+    def moveSolidParticles(self):
+        check(lib().flip2_move_solid_particles(self._h))
+
+    def writePositionsToFile(self, fileName: str):
+        check(lib().flip2_write_positions_to_file(self._h, fileName.encode()))
+
+    def getCourantDt(self) -> float:
+        d = _f64()
+        check(lib().flip2_get_courant_dt(self._h, ctypes.byref(d)))
+        return d.value
+
+    # stage access --------------------------------------------------------
+    def applyGravity(self, dt):
+        check(lib().flip2_apply_gravity(self._h, dt))
+
+    def removeGravity(self, dt):
+        check(lib().flip2_remove_gravity(self._h, dt))
+
+    def calcDivU(self):
+        check(lib().flip2_calc_div_u(self._h))
+
+    def gsSolve(self, dt, threshold=0.01, maxIterations=16384):
+        r = _f64()
+        it = _u32()
+        check(lib().flip2_gs_solve(self._h, dt, threshold, maxIterations, ctypes.byref(r), ctypes.byref(it)))
+        return r.value, it.value
+
+    def gsIterations(self, dt, count):
+        check(lib().flip2_gs_iterations(self._h, dt, count))
+
+    def moveSolids(self):
+        check(lib().flip2_move_solids(self._h))
+
+    def setSolverLimits(self, threshold=0.01, maxIterations=16384):
+        check(lib().flip2_set_solver_limits(self._h, threshold, maxIterations))
+
+    def setTime(self, dt, frameDt, elapsedTime=0.0, elapsedTimeThisFrame=0.0):
+        check(lib().flip2_set_time(self._h, dt, frameDt, elapsedTime, elapsedTimeThisFrame))
+
+    def set_positions(self, px, py, pz):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (px, py, pz)]
+        assert all(v.size == self.size for v in a)
+        check(lib().flip2_set_positions(self._h, *(_ptr(v) for v in a)))
+
+    def set_velocities(self, vx, vy, vz):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz)]
+        assert all(v.size == self.size for v in a)
+        check(lib().flip2_set_velocities(self._h, *(_ptr(v) for v in a)))
+
+    def positions(self):
+        out = [np.empty(self.size) for _ in range(3)]
+        check(lib().flip2_get_positions(self._h, *(_ptr(v) for v in out)))
+        return out
+
+    def velocities(self):
+        out = [np.empty(self.size) for _ in range(3)]
+        check(lib().flip2_get_velocities(self._h, *(_ptr(v) for v in out)))
+        return out
+
+    def get(self, name: str) -> np.ndarray:
+        cnt = _sz()
+        check(lib().flip2_get_array(self._h, name.encode(), None, 0, ctypes.byref(cnt)))
+        out = np.empty(cnt.value, dtype=_ARRAY_DTYPES[name])
+        check(lib().flip2_get_array(self._h, name.encode(), _ptr(out) if out.size else None, out.size,
+                                    ctypes.byref(cnt)))
+        return out
+
+    def set(self, name: str, values: np.ndarray) -> None:
+        a = np.ascontiguousarray(values, dtype=_ARRAY_DTYPES[name])
+        check(lib().flip2_set_array(self._h, name.encode(), _ptr(a) if a.size else None, a.size))
+
+    def stats(self) -> dict:
+        s = RcStatsC()
+        check(lib().flip2_get_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in RcStatsC._fields_}

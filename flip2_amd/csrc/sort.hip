@@ -1,0 +1,303 @@
+// Stable LSD radix sort of u32 keys with u32 values ("onesweep" form),
+// written for CDNA wave64.  Replaces the reference's 32 passes of 1-bit
+// splits (engine/algorithms/radixSortKernels.cu:85-123), each of which runs a
+// flag kernel, two multi-level scans, a split kernel and two gathers.
+//
+// Per sort:
+//   1. one histogram kernel counts the 8-bit digits of every pass at once;
+//   2. a small kernel turns the counts into per-pass digit offsets;
+//   3. one kernel per pass: each 4096-key tile ranks its keys by digit with
+//      64-lane ballot matching (stable within the wave), publishes its digit
+//      counts, resolves its global digit bases by per-digit decoupled
+//      look-back over earlier tiles, stages the keys in LDS in sorted order
+//      and writes them out in digit runs.
+// A stable LSD sort has a unique output, so keys and the permutation match
+// the reference's bit for bit.
+#include "sort.hpp"
+
+#include "scan_impl.hpp"
+
+namespace flip2 {
+
+namespace {
+
+constexpr int kSortThreads = 256;             // 4 waves
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortItems = 16;                // keys per lane
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr int kRadix = 256;
+constexpr int kHistThreads = 256;
+constexpr int kHistItems = 32;                // keys per thread in the histogram kernel
+constexpr uint32_t kDigitFlagShift = 30;
+constexpr uint32_t kDigitValueMask = (1u << kDigitFlagShift) - 1u;
+constexpr uint32_t kDigitAggregate = 1u << kDigitFlagShift;
+constexpr uint32_t kDigitInclusive = 2u << kDigitFlagShift;
+constexpr int kMaxPasses = 4;
+
+struct SortLayout {
+    uint32_t passes;
+    uint32_t tiles;
+    size_t histOff, offsetOff, counterOff, statusOff, total;
+};
+
+SortLayout sort_layout(uint32_t n, uint32_t beginBit, uint32_t endBit) {
+    SortLayout L{};
+    L.passes = (endBit > beginBit) ? ceil_div(endBit - beginBit, 8) : 0;
+    L.tiles = ceil_div(n ? n : 1, kSortTile);
+    size_t off = 0;
+    L.histOff = off; off += (size_t)kMaxPasses * kRadix * sizeof(uint32_t);
+    L.offsetOff = off; off += (size_t)kMaxPasses * kRadix * sizeof(uint32_t);
+    L.counterOff = off; off += 256;
+    L.statusOff = off; off += (size_t)L.passes * L.tiles * kRadix * sizeof(uint32_t);
+    L.total = off + 256;
+    return L;
+}
+
+__device__ inline uint32_t digit_of(uint32_t key, uint32_t shift, uint32_t mask) {
+    return (key >> shift) & mask;
+}
+
+__global__ void sort_clear_kernel(uint32_t* words, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) words[i] = 0;
+}
+
+// Digit histograms of every pass.  LDS counters per pass, merged with one
+// global atomic per non-empty bin per workgroup.
+__global__ void __launch_bounds__(kHistThreads)
+sort_histogram_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t beginBit, uint32_t endBit,
+                      uint32_t passes, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kHistThreads) s_hist[i] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (uint32_t)(kHistThreads * kHistItems);
+    for (int k = 0; k < kHistItems; ++k) {
+        const uint32_t i = base + (uint32_t)k * kHistThreads + threadIdx.x;
+        if (i < n) {
+            const uint32_t key = keys[i];
+            for (uint32_t p = 0; p < passes; ++p) {
+                const uint32_t shift = beginBit + 8u * p;
+                const uint32_t bits = min(8u, endBit - shift);
+                const uint32_t d = digit_of(key, shift, (1u << bits) - 1u);
+                atomicAdd(&s_hist[p * kRadix + d], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kHistThreads) {
+        const uint32_t c = s_hist[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// Exclusive scan of each pass's 256 digit counts (one workgroup, one wave
+// per pass).
+__global__ void sort_offsets_kernel(const uint32_t* __restrict__ hist, uint32_t passes,
+                                    uint32_t* __restrict__ offsets) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t p = threadIdx.x >> 6;
+    if (p >= passes) return;
+    uint32_t carry = 0;
+    for (int chunk = 0; chunk < kRadix / 64; ++chunk) {
+        const uint32_t d = (uint32_t)chunk * 64u + (uint32_t)lane;
+        const uint32_t c = hist[p * kRadix + d];
+        const uint32_t incl = wave_inclusive_scan(c);
+        offsets[p * kRadix + d] = carry + incl - c;
+        carry += __shfl(incl, 63, 64);
+    }
+}
+
+__device__ inline uint32_t digit_status_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void digit_status_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
+                 uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n,
+                 uint32_t shift, uint32_t mask, const uint32_t* __restrict__ digitOffsets,
+                 uint32_t* __restrict__ status, uint32_t* __restrict__ counter) {
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_wave[kSortWaves][kRadix];  // counts, then exclusive offsets across waves
+    __shared__ uint32_t s_tileExcl[kRadix];
+    __shared__ uint32_t s_globalBase[kRadix];
+    __shared__ uint32_t s_scanTmp[kSortWaves];
+    __shared__ uint32_t s_tile;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(counter, 1u);
+    for (int i = tid; i < kSortWaves * kRadix; i += kSortThreads) (&s_wave[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t tileBase = tile * (uint32_t)kSortTile;
+    const uint32_t waveBase = tileBase + (uint32_t)wave * (64u * kSortItems);
+
+    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t i = waveBase + (uint32_t)k * 64u + (uint32_t)lane;
+        const bool valid = i < n;
+        key[k] = valid ? keysIn[i] : 0xffffffffu;
+        val[k] = valid ? (valsIn ? valsIn[i] : i) : 0u;
+    }
+
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t i = waveBase + (uint32_t)k * 64u + (uint32_t)lane;
+        const bool valid = i < n;
+        const uint32_t d = digit_of(key[k], shift, mask);
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bm = __ballot((d >> b) & 1u);
+            m &= ((d >> b) & 1u) ? bm : ~bm;
+        }
+        const uint64_t lower = m & ltMask;
+        const uint32_t before = s_wave[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && lower == 0) s_wave[wave][d] = before + (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        rank[k] = before + (uint32_t)__popcll(lower);
+    }
+    __syncthreads();
+
+    // Thread d: tile count of digit d and the wave-exclusive offsets.
+    const uint32_t d = (uint32_t)tid;
+    uint32_t tileCount = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+        const uint32_t c = s_wave[w][d];
+        s_wave[w][d] = tileCount;
+        tileCount += c;
+    }
+    uint32_t* myStatus = status + (size_t)tile * kRadix + d;
+    digit_status_store(myStatus, (tile == 0 ? kDigitInclusive : kDigitAggregate) | tileCount);
+
+    // Exclusive scan of the tile counts over the 256 digits.
+    {
+        const uint32_t incl = wave_inclusive_scan(tileCount);
+        if (lane == 63) s_scanTmp[wave] = incl;
+        __syncthreads();
+        uint32_t waveOff = 0;
+        for (int w = 0; w < wave; ++w) waveOff += s_scanTmp[w];
+        s_tileExcl[d] = waveOff + incl - tileCount;
+    }
+
+    // Per-digit look-back over the earlier tiles.
+    uint32_t prefix = 0;
+    if (tile > 0) {
+        int64_t j = (int64_t)tile - 1;
+        for (;;) {
+            const uint32_t* p = status + (size_t)j * kRadix + d;
+            uint32_t s = digit_status_load(p);
+            while ((s >> kDigitFlagShift) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                s = digit_status_load(p);
+            }
+            prefix += s & kDigitValueMask;
+            if ((s >> kDigitFlagShift) == 2u || j == 0) break;
+            --j;
+        }
+        digit_status_store(myStatus, kDigitInclusive | (prefix + tileCount));
+    }
+    __syncthreads();
+    s_globalBase[d] = digitOffsets[d] + prefix - s_tileExcl[d];
+
+    // Stage in LDS in tile-sorted order.
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t i = waveBase + (uint32_t)k * 64u + (uint32_t)lane;
+        if (i < n) {
+            const uint32_t dk = digit_of(key[k], shift, mask);
+            const uint32_t pos = s_tileExcl[dk] + s_wave[wave][dk] + rank[k];
+            s_keys[pos] = key[k];
+            s_vals[pos] = val[k];
+        }
+    }
+    __syncthreads();
+
+    const uint32_t valid = (n > tileBase) ? min((uint32_t)kSortTile, n - tileBase) : 0u;
+    for (uint32_t j = (uint32_t)tid; j < valid; j += kSortThreads) {
+        const uint32_t k = s_keys[j];
+        const uint32_t o = s_globalBase[digit_of(k, shift, mask)] + j;
+        if (o < n) {
+            keysOut[o] = k;
+            valsOut[o] = s_vals[j];
+        }
+    }
+}
+
+}  // namespace
+
+size_t sort_workspace_bytes(uint32_t n, uint32_t beginBit, uint32_t endBit) {
+    if (endBit > 32) endBit = 32;
+    return sort_layout(n, beginBit, endBit).total;
+}
+
+void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* keysOut, uint32_t* valsOut,
+                    uint32_t* keysAlt, uint32_t* valsAlt, uint32_t n, uint32_t beginBit, uint32_t endBit,
+                    void* ws, size_t wsBytes, hipStream_t stream) {
+    FLIP2_REQUIRE(endBit <= 32 && beginBit <= endBit, kErrArgument, "sort: bad bit range");
+    FLIP2_REQUIRE(n < (1u << kDigitFlagShift), kErrRange, "sort: n must be below 2^30");
+    FLIP2_REQUIRE(keysIn && keysOut && valsOut, kErrArgument, "sort: null buffer");
+    FLIP2_REQUIRE(keysIn != keysOut && keysIn != keysAlt && (!valsIn || (valsIn != valsOut && valsIn != valsAlt)),
+                  kErrArgument, "sort: inputs must not alias outputs");
+    if (n == 0) return;
+    const SortLayout L = sort_layout(n, beginBit, endBit);
+    FLIP2_REQUIRE(ws && wsBytes >= L.total, kErrArgument, "sort: workspace too small");
+    if (L.passes == 0) {
+        FLIP2_HIP(hipMemcpyAsync(keysOut, keysIn, (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+        if (valsIn) {
+            FLIP2_HIP(hipMemcpyAsync(valsOut, valsIn, (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+        } else {
+            // identity permutation via a one-pass "sort" on an all-zero digit
+            // range is overkill; the caller asked for no bits, so write iota.
+            FLIP2_REQUIRE(false, kErrArgument, "sort: empty bit range needs explicit values");
+        }
+        return;
+    }
+    FLIP2_REQUIRE(L.passes == 1 || (keysAlt && valsAlt), kErrArgument, "sort: alternate buffers required");
+    char* w = reinterpret_cast<char*>(ws);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w + L.histOff);
+    uint32_t* offsets = reinterpret_cast<uint32_t*>(w + L.offsetOff);
+    uint32_t* counters = reinterpret_cast<uint32_t*>(w + L.counterOff);
+    uint32_t* status = reinterpret_cast<uint32_t*>(w + L.statusOff);
+
+    const uint32_t clearWords = (uint32_t)((L.total - 256) / 4);
+    hipLaunchKernelGGL(sort_clear_kernel, dim3(ceil_div(clearWords, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<uint32_t*>(w), clearWords);
+    FLIP2_LAUNCH_CHECK();
+    const uint32_t histBlocks = ceil_div(n, (uint64_t)kHistThreads * kHistItems);
+    hipLaunchKernelGGL(sort_histogram_kernel, dim3(histBlocks), dim3(kHistThreads), 0, stream, keysIn, n,
+                       beginBit, endBit, L.passes, hist);
+    FLIP2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sort_offsets_kernel, dim3(1), dim3(64 * kMaxPasses), 0, stream, hist, L.passes,
+                       offsets);
+    FLIP2_LAUNCH_CHECK();
+
+    // Ping-pong so that the last pass writes keysOut/valsOut.
+    const uint32_t* srcK = keysIn;
+    const uint32_t* srcV = valsIn;
+    for (uint32_t p = 0; p < L.passes; ++p) {
+        const bool toOut = ((L.passes - 1 - p) % 2) == 0;
+        uint32_t* dstK = toOut ? keysOut : keysAlt;
+        uint32_t* dstV = toOut ? valsOut : valsAlt;
+        const uint32_t shift = beginBit + 8u * p;
+        const uint32_t bits = std::min(8u, endBit - shift);
+        const uint32_t mask = (1u << bits) - 1u;
+        hipLaunchKernelGGL(sort_pass_kernel, dim3(L.tiles), dim3(kSortThreads), 0, stream, srcK, srcV, dstK,
+                           dstV, n, shift, mask, offsets + p * kRadix,
+                           status + (size_t)p * L.tiles * kRadix, counters + p);
+        FLIP2_LAUNCH_CHECK();
+        srcK = dstK;
+        srcV = dstV;
+    }
+}
+
+}  // namespace flip2

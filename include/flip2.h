@@ -1,0 +1,173 @@
+/*
+ * flip2 per-step particle pipeline on MI355X (gfx950): C ABI.
+ *
+ * Shipped as build/engine/libparticles.so, the path the reference's Blender
+ * add-on opens (plugin/plugin.py:4-5).  All functions return 0 on success and
+ * a negative status on failure; flip2_last_error() gives the message of the
+ * last failure on the calling thread.  Nothing here calls exit().
+ *
+ * Pointers named d_* are device pointers (e.g. from flip2_malloc); all other
+ * array pointers are host memory that is copied and not retained.  `stream`
+ * is a hipStream_t (NULL = the default stream).
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef FLIP2_H
+#define FLIP2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLIP2_OK 0
+#define FLIP2_ERR_ARGUMENT (-1)
+#define FLIP2_ERR_HIP (-2)
+#define FLIP2_ERR_STATE (-3)
+#define FLIP2_ERR_NOMEM (-4)
+#define FLIP2_ERR_RANGE (-5)
+#define FLIP2_ERR_INTERNAL (-6)
+
+/* Same layout as class Grid (engine/grid.hpp:8-38). */
+typedef struct flip2_grid {
+    float negX, negY, negZ;
+    uint32_t sizeX, sizeY, sizeZ;
+    float cellSize;
+} flip2_grid;
+
+/* Message of the last failed call on this thread ("" if none).
+ * Replaces gpuAssert's stderr print + exit (engine/typedefs.h:12-20). */
+const char* flip2_last_error(void);
+int flip2_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Device runtime (replaces engine/cudaDeviceManager.hu:26-107 and the    */
+/* allocation/copy surface of engine/cudaVec.cu:20-163)                   */
+/* ---------------------------------------------------------------------- */
+int flip2_device_count(int* count);
+/* Device name and total memory (replaces Particles::setupCudaDevices,
+ * engine/particles.cu:771-782, which prints them). */
+int flip2_device_info(int device, char* name, size_t nameLen, uint64_t* totalBytes);
+int flip2_malloc(void** d_ptr, size_t bytes);                     /* cudaVec.cu:89-96   */
+int flip2_free(void* d_ptr);                                      /* cudaVec.cu:125-134 */
+int flip2_memcpy_htod(void* d_dst, const void* src, size_t bytes); /* cudaVec.cu:49-52  */
+int flip2_memcpy_dtoh(void* dst, const void* d_src, size_t bytes); /* cudaVec.cu:54-60  */
+int flip2_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+int flip2_memset(void* d_ptr, int value, size_t bytes);           /* zeroArray, cudaVec.hu:9-15 */
+int flip2_stream_create(void** stream);
+int flip2_stream_destroy(void* stream);
+int flip2_stream_synchronize(void* stream);
+/* Event timer on a stream (the reference has none). */
+int flip2_timer_create(void** timer);
+int flip2_timer_destroy(void* timer);
+int flip2_timer_start(void* timer, void* stream);
+int flip2_timer_stop(void* timer, void* stream);
+int flip2_timer_elapsed_ms(void* timer, float* ms);   /* waits for the stop event */
+
+/* ---------------------------------------------------------------------- */
+/* Parallel primitives on device pointers                                 */
+/* ---------------------------------------------------------------------- */
+
+/* u32 prefix scan, out-of-place or in place (d_in == d_out), mod 2^32.
+ * exclusive=0 is the reference's inclusive scan.  d_total (may be NULL)
+ * receives the grand total on the device.  Workspace from
+ * flip2_scan_workspace_bytes; d_ws == NULL uses an internal workspace.
+ * Replaces cudaParallelPrefixSum (parallelPrefixSumKernels.cu:65-77,
+ * parallelPrefixSumKernels.hu:53-54). */
+size_t flip2_scan_workspace_bytes(uint32_t n);
+int flip2_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, int exclusive, uint32_t* d_total,
+                   void* d_ws, size_t ws_bytes, void* stream);
+
+/* Stable LSD radix sort of u32 keys with u32 values over key bits
+ * [begin_bit, end_bit).  d_vals_in == NULL sorts the permutation (values
+ * 0..n-1), which is the reference's reorderedGridIndices.  d_keys_alt /
+ * d_vals_alt are ping-pong buffers of n elements (unused when one 8-bit pass
+ * suffices).  Inputs are left unchanged and must not alias the outputs.
+ * Replaces cudaRadixSortUint (radixSortKernels.cu:85-123, radixSortKernels.hu:16-17)
+ * and cudaSortParticlesByGridNode (particleToGridFunctions.cu:270-301). */
+size_t flip2_sort_workspace_bytes(uint32_t n, uint32_t begin_bit, uint32_t end_bit);
+int flip2_sort_pairs_u32(const uint32_t* d_keys_in, const uint32_t* d_vals_in, uint32_t* d_keys_out,
+                         uint32_t* d_vals_out, uint32_t* d_keys_alt, uint32_t* d_vals_alt, uint32_t n,
+                         uint32_t begin_bit, uint32_t end_bit, void* d_ws, size_t ws_bytes, void* stream);
+
+/* Clamp FP64 positions into the domain (written back) and compute u32 cell
+ * keys x + y*sx + z*sx*sy.  Replaces cudaFindGridCell / rootCell
+ * (particleToGridFunctions.cu:19-45, 216-219; particleToGridFunctions.hu:10). */
+int flip2_find_grid_cell(double* d_px, double* d_py, double* d_pz, uint32_t n, flip2_grid grid, uint32_t* d_keys,
+                         void* stream);
+
+/* Philox4x32-10 on the device: in = count x (c0,c1,c2,c3,k0,k1), out = count x 4
+ * (host arrays).  Known-answer check of the generator used by moveSolids. */
+int flip2_philox4x32_10(const uint32_t* in, uint32_t count, uint32_t* out);
+
+/* ---------------------------------------------------------------------- */
+/* Reference-compatible particle system: class Particles                  */
+/* (engine/particles.hu:11-95), FP64, same stage semantics                */
+/* ---------------------------------------------------------------------- */
+typedef struct flip2_particles flip2_particles;
+
+typedef struct flip2_rc_stats {
+    uint32_t numUsedGridNodes;
+    uint32_t numUsedVoxels;
+    uint32_t lastSorIterations;
+    uint32_t lastSolveAttempts;
+    double lastResidual;
+    double dt;
+    double frameDt;
+    double elapsedTime;
+    double elapsedTimeThisFrame;
+    uint32_t outOfTileEvents;
+    uint32_t relocatedParticles;
+    uint32_t substeps;
+} flip2_rc_stats;
+
+int flip2_particles_create(uint32_t n, flip2_particles** out);      /* Particles::Particles, particles.cu:15-40 */
+int flip2_particles_destroy(flip2_particles* p);                     /* Particles::~Particles, particles.cu:830 */
+int flip2_set_domain(flip2_particles* p, double nx, double ny, double nz, uint32_t x, uint32_t y, uint32_t z,
+                     double cell_size);                                /* particles.cu:42-46 */
+int flip2_set_positions(flip2_particles* p, const double* px, const double* py, const double* pz);
+int flip2_set_velocities(flip2_particles* p, const double* vx, const double* vy, const double* vz);
+int flip2_get_positions(flip2_particles* p, double* px, double* py, double* pz);
+int flip2_get_velocities(flip2_particles* p, double* vx, double* vy, double* vz);
+int flip2_align_particles_to_grid(flip2_particles* p);   /* particles.cu:48-50   */
+int flip2_sort_particles(flip2_particles* p);            /* particles.cu:52-78   */
+int flip2_generate_voxels(flip2_particles* p);           /* particles.cu:323-368 */
+int flip2_particle_vel_to_voxels(flip2_particles* p);    /* particles.cu:500-505 */
+int flip2_get_courant_dt(flip2_particles* p, double* dt); /* particles.cu:493-498 */
+int flip2_apply_gravity(flip2_particles* p, double dt);  /* voxelSolveFunctions.cu:177-180 */
+int flip2_remove_gravity(flip2_particles* p, double dt); /* voxelSolveFunctions.cu:182-185 */
+int flip2_calc_div_u(flip2_particles* p);                /* voxelSolveFunctions.cu:245-271 */
+/* cudaGSiteration (voxelSolveFunctions.cu:391-412). */
+int flip2_gs_solve(flip2_particles* p, double dt, double threshold, uint32_t max_iterations, double* residual,
+                   uint32_t* iterations);
+/* `count` red/black iterations without convergence checks. */
+int flip2_gs_iterations(flip2_particles* p, double dt, uint32_t count);
+int flip2_pressure_solve(flip2_particles* p);            /* particles.cu:507-555 */
+int flip2_update_voxel_velocities(flip2_particles* p);   /* particles.cu:557-560 */
+int flip2_voxel_vels_to_particles(flip2_particles* p);   /* particles.cu:742-749 */
+int flip2_advect_particles(flip2_particles* p);          /* particles.cu:751-760 */
+int flip2_move_solids(flip2_particles* p);               /* moveSolids kernel, particles.cu:674-740 */
+int flip2_move_solid_particles(flip2_particles* p);      /* particles.cu:762-769 */
+int flip2_initialize(flip2_particles* p);                /* particles.cu:805-812 */
+int flip2_solve_frame(flip2_particles* p, double fps);   /* particles.cu:784-803 */
+int flip2_write_positions_to_file(flip2_particles* p, const char* path); /* particles.cu:817-828 */
+int flip2_get_stats(flip2_particles* p, flip2_rc_stats* stats);
+/* Internal arrays by their reference member names (engine/particles.hu:47-68),
+ * the access the friend harness engine/testing.h:27-75 has.  *count receives
+ * the element count; data is copied when host != NULL and capacity suffices. */
+int flip2_get_array(flip2_particles* p, const char* name, void* host, size_t capacity, size_t* count);
+int flip2_set_array(flip2_particles* p, const char* name, const void* host, size_t count);
+/* SOR threshold and iteration cap used by flip2_pressure_solve; the defaults
+ * are the reference's constants 0.01 and 16384 (particles.cu:509-510). */
+int flip2_set_solver_limits(flip2_particles* p, double threshold, uint32_t max_iterations);
+int flip2_set_time(flip2_particles* p, double dt, double frame_dt, double elapsed_time,
+                   double elapsed_time_this_frame);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLIP2_H */

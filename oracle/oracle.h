@@ -1,0 +1,142 @@
+/*
+ * CPU oracle for the flip2 per-step particle pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this code; it is the checker that the HIP path is compared against
+ * (tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py).
+ *
+ * Two families live here:
+ *   - "rc_*": a serial FP64 restatement of the reference implementation
+ *     (Aidan-F-Ryan/flip2, engine/particles.cu and the engine/algorithms sources),
+ *     including the behaviours the reference's outputs depend on.  Each
+ *     function cites the reference file:line it follows.
+ *   - "ns_*" (ns_oracle.c): the north-star method (trilinear MAC P2G,
+ *     Jacobi / brick-MIC(0) PCG, FLIP 0.95, RK2, conv3d U-Net) in FP64.
+ *
+ * Pinning (what checks each function against something other than itself) is
+ * listed per function in tests/test_oracle_*.py.  Functions with no such
+ * check are marked "parity unpinned" below.
+ *
+ * Build: oracle/Makefile -> oracle/liboracle.so (gcc -O2 -ffp-contract=off).
+ * FMA contraction is disabled so every product/sum is an individually rounded
+ * IEEE operation; where the reference calls fma() explicitly, so does this.
+ */
+#ifndef FLIP2_ORACLE_H
+#define FLIP2_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order and types as the reference's Grid (engine/grid.hpp:35-37). */
+typedef struct orc_grid {
+    float negX, negY, negZ;
+    uint32_t sizeX, sizeY, sizeZ;
+    float cellSize;
+} orc_grid;
+
+/* ------------------------------------------------------------------------ */
+/* Primitives                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* rootCell (engine/algorithms/particleToGridFunctions.cu:19-45): clamp
+ * positions into the box (written back) and compute the u32 cell key. */
+void orc_find_grid_cell(double* px, double* py, double* pz, uint32_t n,
+                        orc_grid g, uint32_t* keys);
+
+/* Inclusive u32 scan mod 2^32 (parallelPrefixSumKernels.cu:5-77), and the
+ * exclusive form used by the per-node helper (parallelPrefixSumKernels.hu:16-48). */
+void orc_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, int exclusive);
+
+/* Stable LSD sort of u32 keys by 1-bit splits over bits [0, nbits)
+ * (radixSortKernels.cu:6-123).  keys is sorted in place; perm[i] receives the
+ * original index of the i-th sorted key. */
+void orc_radix_sort_u32(uint32_t* keys, uint32_t* perm, uint32_t n, uint32_t nbits);
+
+/* Key/value form: values travel with the keys (stable). */
+void orc_radix_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t nbits);
+
+/* getMaxFromArray + CudaVec::getMax with abs=true (engine/cudaVec.cu:173-233):
+ * the signed element of largest magnitude among the elements the reference's
+ * tree samples (indices 0 and 256 of each 512-tile, recursively). */
+double orc_sampled_absmax(const double* a, uint32_t n);
+
+/* Philox4x32-10 (Salmon et al., SC'11), used for the relocation jitter in
+ * moveSolids and for the seeded synthetic scenes. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ------------------------------------------------------------------------ */
+/* Reference-compatible particle system (class Particles, particles.hu)    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct orc_particles {
+    uint32_t n;
+    double *px, *py, *pz, *vx, *vy, *vz;
+    uint32_t* gridCell;                 /* keys, sorted in place */
+    uint32_t* reorderedGridIndices;     /* sort permutation */
+    uint32_t* uniqueGridNodeIndices;    /* inclusive scan of head flags */
+    uint32_t numUsedGridNodes;          /* B */
+    uint32_t* gridNodeIndicesToFirstParticleIndex; /* [B] */
+    uint32_t yDimRows;                  /* allocated rows of the row table */
+    uint32_t* yDimNumUsedGridNodes;     /* row table */
+    uint32_t* nodeIndexUsedVoxels;      /* [B] inclusive voxel offsets */
+    uint32_t numUsedVoxels;             /* V */
+    uint32_t* voxelIDsUsed;             /* [V] */
+    int8_t* solids;                     /* [V] */
+    double *voxelsUx, *voxelsUy, *voxelsUz, *divU, *p, *residuals; /* [V] */
+    orc_grid grid;
+    double radius;          /* 2 */
+    uint32_t refinementLevel; /* 1 */
+    uint32_t numVoxels1D;   /* 8 */
+    uint32_t numVoxelsPerNode; /* 512 */
+    double frameDt, dt, elapsedTime, elapsedTimeThisFrame;
+    /* statistics of the last pressure solve */
+    uint32_t lastSorIterations;   /* total red/black iterations, all attempts */
+    uint32_t lastSolveAttempts;   /* number of SOR runs (1 + dt halvings) */
+    double lastResidual;
+    uint32_t outOfTileEvents;     /* stencil voxels outside the 8^3 tile (reference printf) */
+    uint64_t moveSolidsCounter;   /* Philox counter word for moveSolids */
+    uint32_t relocatedParticles;
+    double sorThreshold;          /* 0.01 in the reference (particles.cu:509) */
+    uint32_t sorMaxIterations;    /* 16384 in the reference (particles.cu:510) */
+} orc_particles;
+
+orc_particles* orc_particles_create(uint32_t n);
+void orc_particles_destroy(orc_particles* s);
+void orc_set_domain(orc_particles* s, double nx, double ny, double nz,
+                    uint32_t x, uint32_t y, uint32_t z, double cellSize);
+
+void orc_align_particles_to_grid(orc_particles* s);     /* particles.cu:48-50   */
+void orc_sort_particles(orc_particles* s);              /* particles.cu:52-78   */
+void orc_generate_voxels(orc_particles* s);             /* particles.cu:323-368 */
+void orc_particle_vel_to_voxels(orc_particles* s);      /* particles.cu:500-505 */
+double orc_get_courant_dt(orc_particles* s);            /* particles.cu:493-498 */
+void orc_apply_gravity(orc_particles* s, double dt);    /* voxelSolveFunctions.cu:177-180 */
+void orc_remove_gravity(orc_particles* s, double dt);   /* voxelSolveFunctions.cu:182-185 */
+void orc_calc_div_u(orc_particles* s);                  /* voxelSolveFunctions.cu:245-271 */
+/* cudaGSiteration (voxelSolveFunctions.cu:391-412); returns the terminating
+ * sampled residual, *iters receives the number of red/black iterations. */
+double orc_gs_solve(orc_particles* s, double dt, double threshold, uint32_t maxIterations,
+                    uint32_t* iters);
+/* Run exactly `count` red/black iterations (no convergence logic). */
+void orc_gs_iterations(orc_particles* s, double dt, uint32_t count);
+void orc_pressure_solve(orc_particles* s);              /* particles.cu:507-555 */
+void orc_update_voxel_velocities(orc_particles* s);     /* particles.cu:557-560 */
+void orc_voxel_vels_to_particles(orc_particles* s);     /* particles.cu:742-749 */
+void orc_advect_particles(orc_particles* s);            /* particles.cu:751-760 */
+void orc_move_solid_particles_only(orc_particles* s);   /* particles.cu:674-740 (kernel only) */
+void orc_move_solid_particles(orc_particles* s);        /* particles.cu:762-769 */
+void orc_initialize(orc_particles* s);                  /* particles.cu:805-812 */
+void orc_solve_frame(orc_particles* s, double fps);     /* particles.cu:784-803 */
+
+/* Resize voxel arrays to V (used by stage tests that inject state). */
+void orc_resize_voxels(orc_particles* s, uint32_t V);
+void orc_resize_nodes(orc_particles* s, uint32_t B);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,200 @@
+"""GPU parity of the reference-compatible pipeline (class Particles) against
+the FP64 oracle, stage by stage and over whole frames, on seeded scenes.
+
+Integer outputs (keys, permutation, node and voxel tables, solid flags) must
+be bit-exact.  FP64 outputs must agree to 1e-5 relative in the L-infinity
+norm (max |gpu - oracle| <= 1e-5 * max |oracle|), the tolerance the path is
+specified to; in practice they agree to a few ulp.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from flip2_amd import scenes
+
+RTOL = 1e-5
+INT_ARRAYS = ["gridCell", "reorderedGridIndices", "uniqueGridNodeIndices", "gridNodeIndicesToFirstParticleIndex",
+              "yDimNumUsedGridNodes", "nodeIndexUsedVoxels", "voxelIDsUsed", "solids"]
+
+SCENES = {
+    # name: (domain (nx,ny,nz,x,y,z,h), block cells, origin cell, velocity amplitude, seed, SOR cap)
+    "block16": ((0.0, 0.0, 0.0, 16, 16, 16, 1.0 / 16), (6, 8, 16), (2, 0, 0), 0.5, 3, 2049),
+    "still32": ((0.0, 0.0, 0.0, 32, 32, 32, 1.0 / 32), (10, 16, 32), (0, 0, 0), 0.0, 4, 2049),
+    # fast particles: the reference's dt-halving retry loop runs ~46 times here
+    "offset20": ((-10.0, -10.0, -10.0, 20, 20, 20, 0.25), (7, 9, 5), (3, 1, 6), 2.0, 5, 129),
+}
+
+
+def build_scene(name):
+    dom, cells, origin, amp, seed, _ = SCENES[name]
+    nx, ny, nz, x, y, z, h = dom
+    mask = np.zeros((x, y, z), dtype=bool)
+    mask[origin[0]:origin[0] + cells[0], origin[1]:origin[1] + cells[1], origin[2]:origin[2] + cells[2]] = True
+    px, py, pz = scenes.jittered_cells((x, y, z), h, seed, origin=(nx, ny, nz), cell_mask=mask)
+    pos = [a.astype(np.float64) for a in (px, py, pz)]
+    if amp:
+        vel = [a.astype(np.float64) for a in scenes.uniform_velocities(px.size, seed + 100, amp)]
+    else:
+        vel = [np.zeros(px.size) for _ in range(3)]
+    return dom, pos, vel
+
+
+def make_pair(flip2, oracle, name):
+    dom, pos, vel = build_scene(name)
+    n = pos[0].size
+    g = flip2.Particles(n)
+    o = oracle.OracleParticles(n)
+    for s in (g, o):
+        # the reference's 16384-iteration cap makes oracle frames take minutes;
+        # a lower cap (2049: residual checks at iterations 0, 1024, 2048) runs the
+        # same control logic
+        s.setSolverLimits(0.01, SCENES[name][5])
+        s.setDomain(*dom)
+        s.set_positions(*pos)
+        s.set_velocities(*vel)
+    return g, o
+
+
+def close(a, b, name=""):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    assert a.shape == b.shape, (name, a.shape, b.shape)
+    if a.size == 0:
+        return
+    scale = max(np.max(np.abs(b)), 1e-300)
+    err = np.max(np.abs(a - b))
+    assert err <= RTOL * scale, f"{name}: max abs err {err:.3e} vs scale {scale:.3e}"
+
+
+def exact(g, o, names):
+    for n in names:
+        a, b = g.get(n), o.get(n)
+        assert a.shape == b.shape, (n, a.shape, b.shape)
+        assert np.array_equal(a, b), f"{n} differs at {np.flatnonzero(a != b)[:10]}"
+
+
+def inject(g, o, names):
+    for n in names:
+        g.set(n, o.get(n))
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_initialize_bit_exact(flip2, oracle, name):
+    g, o = make_pair(flip2, oracle, name)
+    g.initialize(); o.initialize()
+    assert g.stats()["numUsedGridNodes"] == o.stats()["numUsedGridNodes"]
+    assert g.stats()["numUsedVoxels"] == o.stats()["numUsedVoxels"]
+    exact(g, o, INT_ARRAYS + ["px", "py", "pz", "vx", "vy", "vz"])
+    assert np.all(g.get("p") == 0.0)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_p2g(flip2, oracle, name):
+    g, o = make_pair(flip2, oracle, name)
+    g.initialize(); o.initialize()
+    g.particleVelToVoxels(); o.particleVelToVoxels()
+    for c in ("voxelsUx", "voxelsUy", "voxelsUz"):
+        close(g.get(c), o.get(c), c)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_courant_gravity_divergence(flip2, oracle, name):
+    g, o = make_pair(flip2, oracle, name)
+    g.initialize(); o.initialize()
+    o.particleVelToVoxels()
+    inject(g, o, ["voxelsUx", "voxelsUy", "voxelsUz"])
+    assert g.getCourantDt() == o.getCourantDt()
+    dt = 1.0 / 240.0
+    g.applyGravity(dt); o.applyGravity(dt)
+    close(g.get("voxelsUy"), o.get("voxelsUy"), "Uy+g")
+    g.calcDivU(); o.calcDivU()
+    close(g.get("divU"), o.get("divU"), "divU")
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("iters", [1, 7, 1024])
+def test_sor_iterations(flip2, oracle, name, iters):
+    g, o = make_pair(flip2, oracle, name)
+    g.initialize(); o.initialize()
+    o.particleVelToVoxels()
+    o.applyGravity(1.0 / 240.0)
+    o.calcDivU()
+    inject(g, o, ["divU"])
+    dt = 1.0 / 240.0
+    g.gsIterations(dt, iters); o.gsIterations(dt, iters)
+    close(g.get("p"), o.get("p"), "p")
+    close(g.get("residuals"), o.get("residuals"), "residuals")
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_gs_solve_and_pressure_solve(flip2, oracle, name):
+    g, o = make_pair(flip2, oracle, name)
+    g.initialize(); o.initialize()
+    o.particleVelToVoxels()
+    inject(g, o, ["voxelsUx", "voxelsUy", "voxelsUz"])
+    g.setTime(1.0 / 240.0, 1.0 / 24.0); o.setTime(1.0 / 240.0, 1.0 / 24.0)
+    g.pressureSolve(); o.pressureSolve()
+    sg, so = g.stats(), o.stats()
+    assert sg["dt"] == so["dt"]
+    assert sg["lastSorIterations"] == so["lastSorIterations"]
+    assert sg["lastSolveAttempts"] == so["lastSolveAttempts"]
+    close([sg["lastResidual"]], [so["lastResidual"]], "residual")
+    close(g.get("p"), o.get("p"), "p")
+    close(g.get("divU"), o.get("divU"), "divU")
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_update_g2p_advect_move(flip2, oracle, name):
+    g, o = make_pair(flip2, oracle, name)
+    g.initialize(); o.initialize()
+    o.particleVelToVoxels()
+    o.setTime(1.0 / 240.0, 1.0 / 24.0)
+    o.pressureSolve()
+    g.setTime(o.stats()["dt"], 1.0 / 24.0)
+    inject(g, o, ["voxelsUx", "voxelsUy", "voxelsUz", "p"])
+    g.updateVoxelVelocities(); o.updateVoxelVelocities()
+    for c in ("voxelsUx", "voxelsUy", "voxelsUz"):
+        close(g.get(c), o.get(c), c)
+    inject(g, o, ["voxelsUx", "voxelsUy", "voxelsUz"])
+    g.voxelVelsToParticles(); o.voxelVelsToParticles()
+    for c in ("vx", "vy", "vz"):
+        close(g.get(c), o.get(c), c)
+    inject(g, o, ["vx", "vy", "vz"])
+    g.advectParticles(); o.advectParticles()
+    for c in ("px", "py", "pz"):
+        close(g.get(c), o.get(c), c)
+    inject(g, o, ["px", "py", "pz"])
+    g.moveSolids(); o.moveSolids()
+    assert g.stats()["relocatedParticles"] == o.stats()["relocatedParticles"]
+    for c in ("px", "py", "pz"):
+        close(g.get(c), o.get(c), c)
+
+
+# offset20 is left out: there the reference's retry loop drives dt to ~1e-15,
+# so one frame needs ~1e14 substeps (particles.cu:528-549, 788).
+@pytest.mark.parametrize("name", ["block16", "still32"])
+def test_solve_frames(flip2, oracle, name):
+    g, o = make_pair(flip2, oracle, name)
+    g.initialize(); o.initialize()
+    for frame in range(2):
+        g.solveFrame(24.0); o.solveFrame(24.0)
+        sg, so = g.stats(), o.stats()
+        for k in ("numUsedGridNodes", "numUsedVoxels", "lastSorIterations", "lastSolveAttempts"):
+            assert sg[k] == so[k], (frame, k, sg[k], so[k])
+        close([sg["elapsedTime"]], [so["elapsedTime"]], "elapsed")
+        for c in ("px", "py", "pz", "vx", "vy", "vz"):
+            close(g.get(c), o.get(c), f"frame {frame} {c}")
+        exact(g, o, ["gridCell", "voxelIDsUsed", "solids", "nodeIndexUsedVoxels"])
+
+
+def test_write_positions_csv(flip2, tmp_path):
+    dom, pos, vel = build_scene("block16")
+    g = flip2.Particles(pos[0].size)
+    g.setDomain(*dom)
+    g.set_positions(*pos)
+    path = tmp_path / "frame0"
+    g.writePositionsToFile(str(path))
+    rows = np.loadtxt(path, delimiter=",")
+    assert rows.shape == (pos[0].size, 3)
+    assert np.allclose(rows[:, 0], pos[0], rtol=1e-5)
