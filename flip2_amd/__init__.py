@@ -48,6 +48,37 @@ class GridC(ctypes.Structure):
     ]
 
 
+class FluidConfigC(ctypes.Structure):
+    _fields_ = [
+        ("grid", GridC),
+        ("dt", ctypes.c_float),
+        ("gravity", ctypes.c_float),
+        ("flip", ctypes.c_float),
+        ("preconditioner", ctypes.c_int),
+        ("tolerance", ctypes.c_double),
+        ("max_iterations", ctypes.c_uint32),
+        ("check_every", ctypes.c_uint32),
+        ("margin", ctypes.c_float),
+    ]
+
+
+class SolveStatsC(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("refinements", ctypes.c_uint32),
+        ("rel_residual", ctypes.c_double),
+        ("b_norm", ctypes.c_double),
+        ("active_bricks", ctypes.c_uint32),
+        ("fluid_cells", ctypes.c_uint32),
+        ("solve_ms", ctypes.c_double),
+        ("converged", ctypes.c_int),
+        ("used_fallback", ctypes.c_int),
+    ]
+
+
+PREC = {"jacobi": 0, "mic0": 1, "learned": 2}
+
+
 class RcStatsC(ctypes.Structure):
     _fields_ = [
         ("numUsedGridNodes", ctypes.c_uint32),
@@ -129,6 +160,26 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_set_array": [_vp, ctypes.c_char_p, _vp, _sz],
         "flip2_set_time": [_vp, _f64, _f64, _f64, _f64],
         "flip2_set_solver_limits": [_vp, _f64, _u32],
+        "flip2_ns_bin": [_vp, _vp, _vp, _u32, GridC, _vp, _vp],
+        "flip2_ns_p2g": [_vp, _vp, GridC] + [_vp] * 7,
+        "flip2_p2g_pipeline_create": [_u32, GridC, ctypes.POINTER(_vp)],
+        "flip2_p2g_pipeline_destroy": [_vp],
+        "flip2_p2g_pipeline_run": [_vp, _u32] + [_vp] * 10,
+        "flip2_p2g_pipeline_get": [_vp, ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_sz)],
+        "flip2_p2g_pipeline_enable_timing": [_vp, ctypes.c_int],
+        "flip2_fluid_create": [_u32, ctypes.POINTER(FluidConfigC), ctypes.POINTER(_vp)],
+        "flip2_fluid_destroy": [_vp],
+        "flip2_fluid_set_config": [_vp, ctypes.POINTER(FluidConfigC)],
+        "flip2_fluid_set_particles": [_vp, _u32] + [_vp] * 6,
+        "flip2_fluid_get_particles": [_vp] * 7,
+        "flip2_fluid_set_solid": [_vp, _vp],
+        "flip2_fluid_step": [_vp],
+        "flip2_fluid_stage": [_vp, ctypes.c_char_p],
+        "flip2_fluid_solve_with": [_vp, ctypes.c_int, ctypes.POINTER(SolveStatsC)],
+        "flip2_fluid_last_solve": [_vp, ctypes.POINTER(SolveStatsC)],
+        "flip2_fluid_get_array": [_vp, ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_sz)],
+        "flip2_p2g_pipeline_timing": [_vp, ctypes.c_int, ctypes.c_char_p, _sz, ctypes.POINTER(_f64),
+                                      ctypes.POINTER(ctypes.c_int)],
     }
     for name in ("align_particles_to_grid", "sort_particles", "generate_voxels", "particle_vel_to_voxels",
                  "calc_div_u", "pressure_solve", "update_voxel_velocities", "voxel_vels_to_particles",
@@ -248,6 +299,166 @@ def philox4x32_10(ctr_key: np.ndarray) -> np.ndarray:
     out = np.empty((a.shape[0], 4), dtype=np.uint32)
     check(lib().flip2_philox4x32_10(_ptr(a), a.shape[0], _ptr(out)))
     return out
+
+
+# --------------------------------------------------------------------------
+# Particle -> grid transfer on 8^3 bricks (north-star method)
+# --------------------------------------------------------------------------
+_PIPE_DTYPES = {"keys": np.uint32, "sortedKeys": np.uint32, "perm": np.uint32, "counts": np.uint32,
+                "cellStart": np.uint32, "x": np.float32, "y": np.float32, "z": np.float32,
+                "vx": np.float32, "vy": np.float32, "vz": np.float32}
+
+
+class P2GPipeline:
+    """bin -> key/value radix sort -> gather -> per-cell counts -> exclusive
+    scan -> trilinear MAC P2G, on a brick-tiled grid (sizes multiples of 8)."""
+
+    def __init__(self, max_particles: int, grid: Sequence):
+        h = _vp()
+        self.grid = GridC(*grid)
+        check(lib().flip2_p2g_pipeline_create(int(max_particles), self.grid, ctypes.byref(h)))
+        self._h = h.value
+        self.num_cells = self.grid.sizeX * self.grid.sizeY * self.grid.sizeZ
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().flip2_p2g_pipeline_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, n: int, state: Sequence[DeviceBuffer], out: Sequence[DeviceBuffer], stream=None):
+        """state: device x, y, z, vx, vy, vz (FP32); out: device U, V, W."""
+        check(lib().flip2_p2g_pipeline_run(self._h, int(n), *(b.ptr for b in state), *(b.ptr for b in out), stream))
+
+    def get(self, name: str) -> np.ndarray:
+        cnt = _sz()
+        check(lib().flip2_p2g_pipeline_get(self._h, name.encode(), None, 0, ctypes.byref(cnt)))
+        out = np.empty(cnt.value, dtype=_PIPE_DTYPES[name])
+        check(lib().flip2_p2g_pipeline_get(self._h, name.encode(), _ptr(out) if out.size else None, out.size,
+                                           ctypes.byref(cnt)))
+        return out
+
+    def enable_timing(self, capacity: int) -> None:
+        check(lib().flip2_p2g_pipeline_enable_timing(self._h, int(capacity)))
+
+    def timing(self) -> Tuple[dict, int]:
+        """({kernel: total ms}, timed runs)."""
+        res = {}
+        runs = ctypes.c_int()
+        k = 0
+        while True:
+            name = ctypes.create_string_buffer(64)
+            ms = _f64()
+            rc = lib().flip2_p2g_pipeline_timing(self._h, k, name, 64, ctypes.byref(ms), ctypes.byref(runs))
+            if rc != 0:
+                break
+            res[name.value.decode()] = ms.value
+            k += 1
+        return res, runs.value
+
+
+def transfer(x, y, z, vx, vy, vz, grid: Sequence):
+    """Host convenience: run the transfer pipeline once; returns (U, V, W, pipeline)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z, vx, vy, vz)]
+    n = arrs[0].size
+    pl = P2GPipeline(max(n, 1), grid)
+    state = [DeviceBuffer.from_array(a) for a in arrs]
+    out = [DeviceBuffer(4 * pl.num_cells) for _ in range(3)]
+    pl.run(n, state, out)
+    synchronize()
+    return tuple(b.to_array(np.float32, pl.num_cells) for b in out) + (pl,)
+
+
+# --------------------------------------------------------------------------
+# North-star FLIP step
+# --------------------------------------------------------------------------
+_FLUID_DTYPES = {"U": np.float32, "V": np.float32, "W": np.float32, "U0": np.float32, "V0": np.float32,
+                 "W0": np.float32, "counts": np.uint32, "cellStart": np.uint32, "activeBricks": np.uint32,
+                 "brickSlot": np.int32, "cellType": np.uint8, "coef": np.uint16, "b": np.float64, "p": np.float64}
+
+
+def _stats_dict(s: SolveStatsC) -> dict:
+    return {f: getattr(s, f) for f, _ in SolveStatsC._fields_}
+
+
+class FluidSim:
+    """Mixed-precision FLIP step on 8^3 bricks (see include/flip2.h)."""
+
+    def __init__(self, max_particles: int, grid: Sequence, dt=1.0 / 240.0, gravity=-9.81, flip=0.95,
+                 preconditioner="jacobi", tolerance=1e-6, max_iterations=4000, check_every=8, margin=1e-3):
+        cfg = FluidConfigC()
+        cfg.grid = GridC(*grid)
+        cfg.dt, cfg.gravity, cfg.flip = dt, gravity, flip
+        cfg.preconditioner = PREC[preconditioner] if isinstance(preconditioner, str) else int(preconditioner)
+        cfg.tolerance, cfg.max_iterations, cfg.check_every, cfg.margin = tolerance, max_iterations, check_every, margin
+        self.cfg = cfg
+        h = _vp()
+        check(lib().flip2_fluid_create(int(max_particles), ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h.value
+        self.grid = tuple(grid)
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().flip2_fluid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_preconditioner(self, name):
+        self.cfg.preconditioner = PREC[name] if isinstance(name, str) else int(name)
+        check(lib().flip2_fluid_set_config(self._h, ctypes.byref(self.cfg)))
+
+    def set_particles(self, x, y, z, vx, vy, vz):
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, z, vx, vy, vz)]
+        self.n = a[0].size
+        check(lib().flip2_fluid_set_particles(self._h, self.n, *(_ptr(v) for v in a)))
+
+    def particles(self):
+        out = [np.empty(self.n, np.float32) for _ in range(6)]
+        check(lib().flip2_fluid_get_particles(self._h, *(_ptr(v) for v in out)))
+        return out
+
+    def set_solid(self, mask):
+        if mask is None:
+            check(lib().flip2_fluid_set_solid(self._h, None))
+        else:
+            m = np.ascontiguousarray(mask, dtype=np.uint8)
+            check(lib().flip2_fluid_set_solid(self._h, _ptr(m)))
+
+    def step(self):
+        check(lib().flip2_fluid_step(self._h))
+
+    def stage(self, name: str):
+        check(lib().flip2_fluid_stage(self._h, name.encode()))
+
+    def solve_with(self, preconditioner) -> dict:
+        s = SolveStatsC()
+        p = PREC[preconditioner] if isinstance(preconditioner, str) else int(preconditioner)
+        check(lib().flip2_fluid_solve_with(self._h, p, ctypes.byref(s)))
+        return _stats_dict(s)
+
+    def last_solve(self) -> dict:
+        s = SolveStatsC()
+        check(lib().flip2_fluid_last_solve(self._h, ctypes.byref(s)))
+        return _stats_dict(s)
+
+    def get(self, name: str) -> np.ndarray:
+        cnt = _sz()
+        check(lib().flip2_fluid_get_array(self._h, name.encode(), None, 0, ctypes.byref(cnt)))
+        out = np.empty(cnt.value, dtype=_FLUID_DTYPES[name])
+        check(lib().flip2_fluid_get_array(self._h, name.encode(), _ptr(out) if out.size else None, out.size,
+                                          ctypes.byref(cnt)))
+        return out
 
 
 # --------------------------------------------------------------------------

@@ -7,10 +7,23 @@
 #include <string>
 
 #include "common.hpp"
+#include "fluid.hpp"
+#include "ns_kernels.hpp"
+#include "p2g_pipeline.hpp"
 #include "particles.hpp"
 #include "rc_kernels.hpp"
 #include "scan_impl.hpp"
 #include "sort.hpp"
+
+struct flip2_p2g_pipeline {
+    flip2::P2GPipeline impl;
+    flip2_p2g_pipeline(uint32_t n, const flip2::Grid& g) : impl(n, g) {}
+};
+
+struct flip2_fluid {
+    flip2::FluidSim impl;
+    flip2_fluid(uint32_t n, const flip2::FluidConfig& c) : impl(n, c) {}
+};
 
 struct flip2_particles {
     flip2::Particles impl;
@@ -68,6 +81,34 @@ struct InternalWorkspace {
     }
 };
 InternalWorkspace g_ws;
+
+flip2::FluidConfig to_config(const flip2_fluid_config& c) {
+    flip2::FluidConfig r;
+    r.grid = to_grid(c.grid);
+    r.dt = c.dt;
+    r.gravity = c.gravity;
+    r.flip = c.flip;
+    r.preconditioner = c.preconditioner;
+    r.tolerance = c.tolerance;
+    r.maxIterations = c.max_iterations;
+    r.checkEvery = c.check_every ? c.check_every : 8;
+    r.margin = c.margin;
+    FLIP2_REQUIRE(c.dt > 0.0f && c.tolerance > 0.0 && c.max_iterations > 0, flip2::kErrArgument, "invalid config");
+    FLIP2_REQUIRE(c.preconditioner >= 0 && c.preconditioner <= 2, flip2::kErrArgument, "unknown preconditioner");
+    return r;
+}
+
+void to_stats(const flip2::SolveStats& s, flip2_solve_stats* o) {
+    o->iterations = s.iterations;
+    o->refinements = s.refinements;
+    o->rel_residual = s.relResidual;
+    o->b_norm = s.bNorm;
+    o->active_bricks = s.activeBricks;
+    o->fluid_cells = s.fluidCells;
+    o->solve_ms = s.solveMs;
+    o->converged = s.converged ? 1 : 0;
+    o->used_fallback = s.usedFallback ? 1 : 0;
+}
 
 #define NEED(p) FLIP2_REQUIRE((p) != nullptr, flip2::kErrArgument, #p " is NULL")
 
@@ -403,6 +444,173 @@ int flip2_set_time(flip2_particles* p, double dt, double frame_dt, double elapse
     return guarded([&] {
         NEED(p);
         p->impl.setTime(dt, frame_dt, elapsed_time, elapsed_time_this_frame);
+    });
+}
+
+int flip2_ns_bin(const float* d_x, const float* d_y, const float* d_z, uint32_t n, flip2_grid grid, uint32_t* d_keys,
+                 void* stream) {
+    return guarded([&] {
+        FLIP2_REQUIRE(n == 0 || (d_x && d_y && d_z && d_keys), flip2::kErrArgument, "null buffer");
+        FLIP2_REQUIRE(grid.sizeX % 8 == 0 && grid.sizeY % 8 == 0 && grid.sizeZ % 8 == 0 && grid.sizeX && grid.sizeY &&
+                          grid.sizeZ && grid.cellSize > 0.0f,
+                      flip2::kErrArgument, "grid sizes must be positive multiples of 8");
+        flip2::ns::launch_bin(d_x, d_y, d_z, n, to_grid(grid), d_keys, as_stream(stream));
+    });
+}
+
+int flip2_ns_p2g(const float* d_records, const uint32_t* d_cell_start, flip2_grid grid, float* d_u, float* d_v,
+                 float* d_w, float* d_wu, float* d_wv, float* d_ww, void* stream) {
+    return guarded([&] {
+        FLIP2_REQUIRE(d_records && d_cell_start && d_u && d_v && d_w, flip2::kErrArgument, "null buffer");
+        FLIP2_REQUIRE((reinterpret_cast<uintptr_t>(d_records) & 15u) == 0, flip2::kErrArgument,
+                      "records must be 16-byte aligned");
+        FLIP2_REQUIRE(grid.sizeX % 8 == 0 && grid.sizeY % 8 == 0 && grid.sizeZ % 8 == 0 && grid.sizeX && grid.sizeY &&
+                          grid.sizeZ && grid.cellSize > 0.0f,
+                      flip2::kErrArgument, "grid sizes must be positive multiples of 8");
+        flip2::ns::launch_p2g(reinterpret_cast<const float4*>(d_records), d_cell_start, to_grid(grid), d_u, d_v, d_w,
+                              d_wu, d_wv, d_ww, as_stream(stream));
+    });
+}
+
+int flip2_p2g_pipeline_create(uint32_t max_particles, flip2_grid grid, flip2_p2g_pipeline** out) {
+    return guarded([&] {
+        NEED(out);
+        *out = new flip2_p2g_pipeline(max_particles, to_grid(grid));
+    });
+}
+int flip2_p2g_pipeline_destroy(flip2_p2g_pipeline* pl) {
+    return guarded([&] {
+        if (pl) FLIP2_HIP(hipDeviceSynchronize());
+        delete pl;
+    });
+}
+int flip2_p2g_pipeline_run(flip2_p2g_pipeline* pl, uint32_t n, const float* d_x, const float* d_y, const float* d_z,
+                           const float* d_vx, const float* d_vy, const float* d_vz, float* d_u, float* d_v,
+                           float* d_w, void* stream) {
+    return guarded([&] {
+        NEED(pl);
+        pl->impl.run(n, d_x, d_y, d_z, d_vx, d_vy, d_vz, d_u, d_v, d_w, as_stream(stream));
+    });
+}
+int flip2_p2g_pipeline_enable_timing(flip2_p2g_pipeline* pl, int capacity) {
+    return guarded([&] {
+        NEED(pl);
+        pl->impl.enableTiming(capacity);
+        pl->impl.resetTiming();
+    });
+}
+int flip2_p2g_pipeline_timing(flip2_p2g_pipeline* pl, int k, char* name, size_t name_len, double* total_ms,
+                              int* runs) {
+    return guarded([&] {
+        NEED(pl);
+        pl->impl.finishTiming();
+        FLIP2_REQUIRE(k >= 0 && k < pl->impl.timedKernels(), flip2::kErrRange, "no such timed kernel");
+        if (name && name_len) {
+            std::strncpy(name, pl->impl.timedKernelName(k), name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (total_ms) *total_ms = pl->impl.timedKernelMs(k);
+        if (runs) *runs = pl->impl.timedRuns();
+    });
+}
+int flip2_p2g_pipeline_get(flip2_p2g_pipeline* pl, const char* name, void* host, size_t capacity, size_t* count) {
+    return guarded([&] {
+        NEED(pl); NEED(name);
+        FLIP2_HIP(hipDeviceSynchronize());
+        const size_t c = pl->impl.get(name, host, capacity, nullptr);
+        if (count) *count = c;
+    });
+}
+
+void flip2_fluid_default_config(flip2_fluid_config* c) {
+    if (!c) return;
+    flip2::FluidConfig d;
+    c->grid = flip2_grid{0.f, 0.f, 0.f, 64, 64, 64, 1.0f / 64.0f};
+    c->dt = d.dt;
+    c->gravity = d.gravity;
+    c->flip = d.flip;
+    c->preconditioner = d.preconditioner;
+    c->tolerance = d.tolerance;
+    c->max_iterations = d.maxIterations;
+    c->check_every = d.checkEvery;
+    c->margin = d.margin;
+}
+int flip2_fluid_create(uint32_t max_particles, const flip2_fluid_config* cfg, flip2_fluid** out) {
+    return guarded([&] {
+        NEED(cfg); NEED(out);
+        *out = new flip2_fluid(max_particles, to_config(*cfg));
+    });
+}
+int flip2_fluid_destroy(flip2_fluid* f) {
+    return guarded([&] { delete f; });
+}
+int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg) {
+    return guarded([&] {
+        NEED(f); NEED(cfg);
+        f->impl.setConfig(to_config(*cfg));
+    });
+}
+int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const float* y, const float* z,
+                              const float* vx, const float* vy, const float* vz) {
+    return guarded([&] {
+        NEED(f);
+        FLIP2_REQUIRE(n == 0 || (x && y && z && vx && vy && vz), flip2::kErrArgument, "null particle array");
+        f->impl.setParticles(n, x, y, z, vx, vy, vz);
+    });
+}
+int flip2_fluid_get_particles(flip2_fluid* f, float* x, float* y, float* z, float* vx, float* vy, float* vz) {
+    return guarded([&] {
+        NEED(f);
+        f->impl.getParticles(x, y, z, vx, vy, vz);
+    });
+}
+int flip2_fluid_set_solid(flip2_fluid* f, const uint8_t* mask) {
+    return guarded([&] {
+        NEED(f);
+        f->impl.setSolidMask(mask);
+    });
+}
+int flip2_fluid_step(flip2_fluid* f) {
+    return guarded([&] {
+        NEED(f);
+        f->impl.step();
+        f->impl.synchronize();
+    });
+}
+int flip2_fluid_stage(flip2_fluid* f, const char* stage) {
+    return guarded([&] {
+        NEED(f); NEED(stage);
+        const std::string s(stage);
+        if (s == "transfer") f->impl.transfer();
+        else if (s == "prepare") f->impl.prepareFaces();
+        else if (s == "system") f->impl.buildSystem();
+        else if (s == "solve") f->impl.solve();
+        else if (s == "project") f->impl.project();
+        else if (s == "g2p") f->impl.gridToParticles();
+        else if (s == "advect") f->impl.advect();
+        else throw flip2::Error(flip2::kErrArgument, "unknown stage " + s);
+        f->impl.synchronize();
+    });
+}
+int flip2_fluid_solve_with(flip2_fluid* f, int preconditioner, flip2_solve_stats* stats) {
+    return guarded([&] {
+        NEED(f);
+        const flip2::SolveStats s = f->impl.solveWith(preconditioner);
+        f->impl.synchronize();
+        if (stats) to_stats(s, stats);
+    });
+}
+int flip2_fluid_last_solve(flip2_fluid* f, flip2_solve_stats* stats) {
+    return guarded([&] {
+        NEED(f); NEED(stats);
+        to_stats(f->impl.lastSolve(), stats);
+    });
+}
+int flip2_fluid_get_array(flip2_fluid* f, const char* name, void* host, size_t capacity, size_t* count) {
+    return guarded([&] {
+        NEED(f); NEED(name);
+        const size_t c = f->impl.getArray(name, host, capacity);
+        if (count) *count = c;
     });
 }
 

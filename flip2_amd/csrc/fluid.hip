@@ -1,0 +1,312 @@
+// North-star FLIP step driver (see fluid.hpp).
+#include "fluid.hpp"
+
+#include "scan_impl.hpp"
+
+#include <chrono>
+#include <cmath>
+
+namespace flip2 {
+
+FluidSim::FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device)
+    : cfg_(cfg), dm_(device), maxParticles_(maxParticles) {
+    const Grid& g = cfg.grid;
+    FLIP2_REQUIRE(g.sizeX % 8 == 0 && g.sizeY % 8 == 0 && g.sizeZ % 8 == 0 && g.sizeX && g.sizeY && g.sizeZ,
+                  kErrArgument, "grid sizes must be positive multiples of 8");
+    FLIP2_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    pipe_.reset(new P2GPipeline(maxParticles, g));
+    numCells_ = g.sizeX * g.sizeY * g.sizeZ;
+    numBricks_ = numCells_ / 512u;
+    rec_.resizeAsync(2 * maxParticles, stream_);
+    for (auto* a : {&U_, &V_, &W_, &U0_, &V0_, &W0_}) a->resizeAsync(numCells_, stream_);
+    brickFlag_.resizeAsync(numBricks_ + 1, stream_);
+    brickExcl_.resizeAsync(numBricks_ + 1, stream_);
+    brickSlot_.resizeAsync(numBricks_, stream_);
+    activeBricks_.resizeAsync(numBricks_, stream_);
+    counter_.resizeAsync(4, stream_);
+    counter_.zeroDeviceAsync(stream_);
+    total_.resizeAsync(4, stream_);
+    scal_.resizeAsync(16, stream_);
+    hist_.resizeAsync(4 * (cfg.maxIterations + 2), stream_);
+    ws_.resizeAsync((uint32_t)scan_workspace_bytes(numBricks_ + 1), stream_);
+    synchronize();
+}
+
+FluidSim::~FluidSim() {
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+void FluidSim::synchronize() { FLIP2_HIP(hipStreamSynchronize(stream_)); }
+
+void FluidSim::setConfig(const FluidConfig& cfg) {
+    FLIP2_REQUIRE(cfg.grid.sizeX == cfg_.grid.sizeX && cfg.grid.sizeY == cfg_.grid.sizeY &&
+                      cfg.grid.sizeZ == cfg_.grid.sizeZ,
+                  kErrArgument, "the grid cannot change");
+    cfg_ = cfg;
+    if (hist_.size() < 4 * (cfg.maxIterations + 2)) hist_.resizeAsync(4 * (cfg.maxIterations + 2), stream_);
+}
+
+void FluidSim::setParticles(uint32_t n, const float* x, const float* y, const float* z, const float* vx,
+                            const float* vy, const float* vz) {
+    FLIP2_REQUIRE(n <= maxParticles_, kErrRange, "more particles than the simulation was created for");
+    n_ = n;
+    std::vector<float4> r(2 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        r[2 * (size_t)i] = make_float4(x[i], y[i], z[i], vx[i]);
+        r[2 * (size_t)i + 1] = make_float4(vy[i], vz[i], 0.f, 0.f);
+    }
+    if (n) rec_.uploadFrom(r.data(), 2 * n, stream_);
+    synchronize();
+}
+
+void FluidSim::getParticles(float* x, float* y, float* z, float* vx, float* vy, float* vz) {
+    std::vector<float4> r(2 * (size_t)n_);
+    if (n_) rec_.downloadTo(r.data(), 2 * n_, stream_);
+    synchronize();
+    float* dst[6] = {x, y, z, vx, vy, vz};
+    for (uint32_t i = 0; i < n_; ++i) {
+        const float* f = reinterpret_cast<const float*>(&r[2 * (size_t)i]);
+        for (int c = 0; c < 6; ++c)
+            if (dst[c]) dst[c][i] = f[c];
+    }
+}
+
+void FluidSim::setSolidMask(const uint8_t* mask) {
+    const Grid& g = cfg_.grid;
+    std::vector<uint8_t> tiled(numCells_, 0);
+    const uint32_t nbX = g.sizeX / 8, nbY = g.sizeY / 8;
+    bool any = false;
+    for (uint32_t k = 0; k < g.sizeZ; ++k)
+        for (uint32_t j = 0; j < g.sizeY; ++j)
+            for (uint32_t i = 0; i < g.sizeX; ++i) {
+                const uint8_t v = mask ? mask[((size_t)k * g.sizeY + j) * g.sizeX + i] : 0;
+                const uint32_t t = (((k >> 3) * nbY + (j >> 3)) * nbX + (i >> 3)) * 512u + (k & 7u) * 64u +
+                                   (j & 7u) * 8u + (i & 7u);
+                tiled[t] = v ? 1 : 0;
+                any |= v != 0;
+            }
+    hasSolid_ = any;
+    if (any) {
+        solid_.resizeAsync(numCells_, stream_);
+        solid_.uploadFrom(tiled.data(), numCells_, stream_);
+    }
+    synchronize();
+}
+
+void FluidSim::ensureSlots(uint32_t numActive) {
+    if (numActive <= slotCapacity_) return;
+    const uint32_t cap = numActive + numActive / 4 + 8;
+    const uint32_t cells = cap * 512u;
+    nbrSlot_.resizeAsync(cap * 6u, stream_);
+    cellType_.resizeAsync(cells, stream_);
+    coef_.resizeAsync(cells, stream_);
+    for (auto* a : {&b64_, &x64_, &r64_}) a->resizeAsync(cells, stream_);
+    for (auto* a : {&r_, &z_, &p_, &q_, &e_, &precon_}) a->resizeAsync(cells, stream_);
+    partials_.resizeAsync(cap * 2u + 64u, stream_);
+    slotCapacity_ = cap;
+}
+
+double FluidSim::readScalar(const double* dev) {
+    double v = 0.0;
+    FLIP2_HIP(hipMemcpyAsync(&v, dev, sizeof(v), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    return v;
+}
+
+void FluidSim::readHistory(uint32_t first, uint32_t count, double* host) {
+    FLIP2_HIP(hipMemcpyAsync(host, hist_.devPtr() + 4u * first, 4u * count * sizeof(double), hipMemcpyDeviceToHost,
+                             stream_));
+    synchronize();
+}
+
+void FluidSim::transfer() {
+    pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_);
+    rec_.swapDevice(pipe_->sortedVec());
+}
+
+void FluidSim::prepareFaces() {
+    ns::launch_prepare_faces(U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
+                             hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.gravity * cfg_.dt, stream_);
+}
+
+void FluidSim::buildSystem() {
+    const uint8_t* solid = hasSolid_ ? solid_.devPtr() : nullptr;
+    ns::launch_classify_bricks(pipe_->counts(), solid, numBricks_, brickFlag_.devPtr(), stream_);
+    scan_u32(brickFlag_.devPtr(), brickExcl_.devPtr(), numBricks_, true, ws_.devPtr(), ws_.size(), total_.devPtr(),
+             stream_);
+    uint32_t numActive = 0;
+    FLIP2_HIP(hipMemcpyAsync(&numActive, total_.devPtr(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    numActive_ = numActive;
+    ensureSlots(std::max<uint32_t>(numActive, 1));
+    ns::launch_build_slots(brickFlag_.devPtr(), brickExcl_.devPtr(), numBricks_, brickSlot_.devPtr(),
+                           activeBricks_.devPtr(), stream_);
+    ns::launch_brick_setup(numActive, activeBricks_.devPtr(), brickSlot_.devPtr(), pipe_->counts(), solid, cfg_.grid,
+                           nbrSlot_.devPtr(), cellType_.devPtr(), stream_);
+    const double hOverDt = (double)cfg_.grid.cellSize / (double)cfg_.dt;
+    ns::launch_assemble(numActive, activeBricks_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(), solid, cfg_.grid,
+                        hOverDt, U_.devPtr(), V_.devPtr(), W_.devPtr(), coef_.devPtr(), b64_.devPtr(), reduce(),
+                        scal_.devPtr() + 0, stream_);
+}
+
+SolveStats FluidSim::solve() { return solveWith(cfg_.preconditioner); }
+
+// Mixed-precision PCG with iterative refinement: FP64 b, x and true
+// residual; each refinement solves A e = r in FP32 (operator and
+// preconditioner) with FP64 dot products, to a relative reduction that lets
+// the FP64 residual reach the tolerance.
+SolveStats FluidSim::solveWith(int prec) {
+    SolveStats st;
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t na = numActive_;
+    st.activeBricks = na;
+    const uint32_t cells = na * 512u;
+    if (na == 0) {
+        st.converged = true;
+        stats_ = st;
+        return st;
+    }
+    const double bb = readScalar(scal_.devPtr() + 0);
+    st.bNorm = std::sqrt(bb);
+    ns::launch_fill_f64(x64_.devPtr(), cells, 0.0, stream_);
+    if (bb == 0.0) {
+        st.converged = true;
+        stats_ = st;
+        return st;
+    }
+    FLIP2_HIP(hipMemcpyAsync(r64_.devPtr(), b64_.devPtr(), (size_t)cells * sizeof(double), hipMemcpyDeviceToDevice,
+                             stream_));
+    FLIP2_REQUIRE(prec != kPrecLearned || learned_, kErrState, "no learned preconditioner installed");
+    if (prec == kPrecMIC0) ns::launch_mic_factor(na, coef_.devPtr(), precon_.devPtr(), stream_);
+    const bool jac = prec == kPrecJacobi;
+    double rel = 1.0;
+    uint32_t total = 0;
+    const uint32_t cap = cfg_.maxIterations;
+    std::vector<double> h(4 * (cfg_.checkEvery + 1));
+    double* H = hist_.devPtr();
+    for (uint32_t refine = 0; refine < 12 && total < cap; ++refine) {
+        const double innerTol = std::max(0.5 * cfg_.tolerance / rel, 2e-5);
+        const double ref = rel * st.bNorm;  // norm of the FP32 right-hand side of this refinement
+        ns::launch_refine_start(na, r64_.devPtr(), r_.devPtr(), e_.devPtr(), coef_.devPtr(), reduce(),
+                                scal_.devPtr() + 1, stream_);
+        if (jac) {
+            ns::launch_jacobi_init(na, r_.devPtr(), coef_.devPtr(), z_.devPtr(), p_.devPtr(), reduce(), H, stream_);
+        } else {
+            if (prec == kPrecMIC0) {
+                ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), H,
+                                     stream_);
+            } else {
+                learned_(na, r_.devPtr(), z_.devPtr(), stream_);
+                ns::launch_dot(na, r_.devPtr(), z_.devPtr(), reduce(), H, stream_);
+            }
+            ns::launch_direction(na, H, H, z_.devPtr(), p_.devPtr(), true, stream_);
+        }
+        uint32_t it = 0;
+        bool innerDone = false;
+        while (!innerDone && total + it < cap) {
+            const uint32_t batch = std::min(cfg_.checkEvery, cap - total - it);
+            const uint32_t first = it;
+            for (uint32_t k = 0; k < batch; ++k, ++it) {
+                double* hk = H + 4u * it;
+                double* hn = H + 4u * (it + 1);
+                ns::launch_matvec_dot(na, p_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), q_.devPtr(), reduce(), hk + 2,
+                                      stream_);
+                ns::launch_update(na, hk, hk + 2, p_.devPtr(), q_.devPtr(), coef_.devPtr(), e_.devPtr(), r_.devPtr(),
+                                  z_.devPtr(), jac, reduce(), hn, stream_);
+                if (prec == kPrecMIC0) {
+                    ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), hn,
+                                         stream_);
+                } else if (prec == kPrecLearned) {
+                    learned_(na, r_.devPtr(), z_.devPtr(), stream_);
+                    ns::launch_dot(na, r_.devPtr(), z_.devPtr(), reduce(), hn, stream_);
+                }
+                ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
+            }
+            readHistory(first + 1, batch, h.data());
+            for (uint32_t k = 0; k < batch; ++k) {
+                const double rr = h[4 * k + 1];
+                if (!(rr == rr) || std::sqrt(std::max(rr, 0.0)) <= innerTol * ref) {
+                    innerDone = true;
+                    break;
+                }
+            }
+        }
+        total += it;
+        ns::launch_accumulate(na, e_.devPtr(), x64_.devPtr(), coef_.devPtr(), stream_);
+        ns::launch_true_residual(na, b64_.devPtr(), x64_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), r64_.devPtr(),
+                                 reduce(), scal_.devPtr() + 2, stream_);
+        const double rr = readScalar(scal_.devPtr() + 2);
+        rel = std::sqrt(std::max(rr, 0.0)) / st.bNorm;
+        st.refinements = refine + 1;
+        if (rel <= cfg_.tolerance) {
+            st.converged = true;
+            break;
+        }
+        if (!(rel == rel)) break;
+    }
+    st.iterations = total;
+    st.relResidual = rel;
+    st.solveMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats_ = st;
+    return st;
+}
+
+void FluidSim::project() {
+    ns::launch_velocity_update(numActive_, activeBricks_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(),
+                               hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, x64_.devPtr(),
+                               cfg_.dt / cfg_.grid.cellSize, U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_);
+}
+
+void FluidSim::gridToParticles() {
+    ns::launch_g2p_flip(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(),
+                        W0_.devPtr(), cfg_.grid, cfg_.flip, stream_);
+}
+
+void FluidSim::advect() {
+    ns::launch_advect_rk2(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(),
+                          hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.dt, cfg_.margin * cfg_.grid.cellSize,
+                          stream_);
+}
+
+void FluidSim::step() {
+    transfer();
+    prepareFaces();
+    buildSystem();
+    solve();
+    project();
+    gridToParticles();
+    advect();
+}
+
+size_t FluidSim::getArray(const std::string& name, void* host, size_t capacity) {
+    synchronize();
+    auto out = [&](auto& v, uint32_t count) -> size_t {
+        using T = typename std::remove_reference<decltype(*v.devPtr())>::type;
+        if (host) {
+            FLIP2_REQUIRE(capacity >= count, kErrRange, "buffer too small");
+            v.downloadTo(reinterpret_cast<T*>(host), count, stream_);
+            synchronize();
+        }
+        return count;
+    };
+    const uint32_t cells = numActive_ * 512u;
+    if (name == "U") return out(U_, numCells_);
+    if (name == "V") return out(V_, numCells_);
+    if (name == "W") return out(W_, numCells_);
+    if (name == "U0") return out(U0_, numCells_);
+    if (name == "V0") return out(V0_, numCells_);
+    if (name == "W0") return out(W0_, numCells_);
+    if (name == "activeBricks") return out(activeBricks_, numActive_);
+    if (name == "brickSlot") return out(brickSlot_, numBricks_);
+    if (name == "cellType") return out(cellType_, cells);
+    if (name == "coef") return out(coef_, cells);
+    if (name == "b") return out(b64_, cells);
+    if (name == "p") return out(x64_, cells);
+    if (name == "counts" || name == "cellStart") return pipe_->get(name, host, capacity, stream_);
+    throw Error(kErrArgument, "unknown array " + name);
+}
+
+}  // namespace flip2

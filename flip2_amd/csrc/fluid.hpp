@@ -1,0 +1,128 @@
+// North-star FLIP fluid step on 8^3 bricks (FP32 particle state):
+//   transfer (bin -> sort -> gather -> counts -> scan -> P2G)
+//   -> wall faces + gravity (pre-projection field kept for FLIP)
+//   -> active bricks, operator and FP64 right-hand side
+//   -> mixed-precision PCG (Jacobi, brick MIC(0) or a learned preconditioner)
+//   -> pressure-gradient update -> FLIP G2P -> RK2 advection.
+#pragma once
+
+#include "common.hpp"
+#include "device.hpp"
+#include "ns_solver.hpp"
+#include "p2g_pipeline.hpp"
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+namespace flip2 {
+
+enum Preconditioner : int { kPrecJacobi = 0, kPrecMIC0 = 1, kPrecLearned = 2 };
+
+struct FluidConfig {
+    Grid grid;
+    float dt = 1.0f / 240.0f;
+    float gravity = -9.81f;
+    float flip = 0.95f;
+    int preconditioner = kPrecJacobi;
+    double tolerance = 1e-6;       // ||b - A p||_2 / ||b||_2, FP64
+    uint32_t maxIterations = 4000; // inner iterations over all refinements
+    uint32_t checkEvery = 8;       // iterations between host residual checks
+    float margin = 1e-3f;          // particles are kept this far (x h) inside the box
+};
+
+struct SolveStats {
+    uint32_t iterations = 0;       // inner PCG iterations (all refinements)
+    uint32_t refinements = 0;      // outer iterative-refinement steps
+    double relResidual = 0.0;      // final FP64 ||b - A p|| / ||b||
+    double bNorm = 0.0;
+    uint32_t activeBricks = 0;
+    uint32_t fluidCells = 0;
+    double solveMs = 0.0;          // wall time of the solve (host clock, synchronised)
+    bool converged = false;
+    bool usedFallback = false;
+};
+
+// A preconditioner supplied from outside the solver (the learned one):
+// z = M^-1 r on the active-brick vectors, FP32.
+using PrecondFn = std::function<void(uint32_t numActive, const float* r, float* z, hipStream_t s)>;
+
+class FluidSim {
+public:
+    FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device = 0);
+    ~FluidSim();
+    FluidSim(const FluidSim&) = delete;
+    FluidSim& operator=(const FluidSim&) = delete;
+
+    void setParticles(uint32_t n, const float* x, const float* y, const float* z, const float* vx, const float* vy,
+                      const float* vz);
+    void getParticles(float* x, float* y, float* z, float* vx, float* vy, float* vz);
+    // Static solid cells, host array of n^3 bytes indexed x + y*nx + z*nx*ny.
+    void setSolidMask(const uint8_t* mask);
+    void setConfig(const FluidConfig& cfg);
+    const FluidConfig& config() const { return cfg_; }
+    const float4* recordsDev() const { return rec_.devPtr(); }
+
+    void step();                   // one full step
+    // Stages (the step is these in order)
+    void transfer();
+    void prepareFaces();
+    void buildSystem();
+    SolveStats solve();
+    void project();
+    void gridToParticles();
+    void advect();
+
+    // Solve A x = b for the current system with a caller-given preconditioner
+    // choice (used for preconditioner comparisons on one system).
+    SolveStats solveWith(int preconditioner);
+    void setLearnedPreconditioner(PrecondFn fn) { learned_ = std::move(fn); }
+
+    const SolveStats& lastSolve() const { return stats_; }
+    uint32_t numParticles() const { return n_; }
+    uint32_t numActive() const { return numActive_; }
+    hipStream_t stream() const { return stream_; }
+    void synchronize();
+
+    // Named arrays for checks: "U","V","W","U0","V0","W0" (n^3 f32),
+    // "counts","cellStart" (n^3+1 u32), "activeBricks" (u32), "cellType" (u8,
+    // slots*512), "coef" (u16), "b" and "p" (f64, slots*512), "brickSlot" (i32).
+    size_t getArray(const std::string& name, void* host, size_t capacity);
+    // Active-brick vectors for external preconditioners / training data.
+    const uint16_t* coefDev() const { return coef_.devPtr(); }
+    const int32_t* nbrSlotDev() const { return nbrSlot_.devPtr(); }
+    const uint32_t* activeBricksDev() const { return activeBricks_.devPtr(); }
+    const uint8_t* cellTypeDev() const { return cellType_.devPtr(); }
+
+private:
+    void ensureSlots(uint32_t numActive);
+    ns::Reduce reduce() { return ns::Reduce{partials_.devPtr(), counter_.devPtr()}; }
+    double readScalar(const double* dev);
+    void readHistory(uint32_t first, uint32_t count, double* host);
+
+    FluidConfig cfg_;
+    DeviceManager dm_;
+    hipStream_t stream_ = nullptr;
+    uint32_t maxParticles_, n_ = 0, numCells_, numBricks_;
+    uint32_t numActive_ = 0;
+    std::unique_ptr<P2GPipeline> pipe_;
+    DeviceVec<float4> rec_;  // particle records (2 x float4 per particle)
+    DeviceVec<float> U_, V_, W_, U0_, V0_, W0_;
+    DeviceVec<uint8_t> solid_;
+    bool hasSolid_ = false;
+    // active bricks
+    DeviceVec<uint32_t> brickFlag_, brickExcl_, activeBricks_;
+    DeviceVec<int32_t> brickSlot_, nbrSlot_;
+    DeviceVec<uint8_t> cellType_;
+    DeviceVec<uint16_t> coef_;
+    DeviceVec<double> b64_, x64_, r64_;
+    DeviceVec<float> r_, z_, p_, q_, e_, precon_;
+    DeviceVec<double> partials_, hist_, scal_;
+    DeviceVec<uint32_t> counter_, total_;
+    DeviceVec<uint8_t> ws_;
+    uint32_t slotCapacity_ = 0;
+    SolveStats stats_;
+    PrecondFn learned_;
+};
+
+}  // namespace flip2

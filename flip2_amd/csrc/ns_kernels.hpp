@@ -1,0 +1,36 @@
+// North-star transfer kernels on 8^3 bricks (ns_kernels.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace flip2 {
+namespace ns {
+
+struct Float6 {
+    const float* p[6];  // x, y, z, vx, vy, vz
+};
+using Float6In = Float6;
+struct Float6Out {
+    float* p[6];
+};
+
+// Brick-tiled cell keys of FP32 positions, clamped into the grid.
+void launch_bin(const float* x, const float* y, const float* z, uint32_t n, Grid g, uint32_t* keys, hipStream_t s);
+// Particle records: 2 x float4 per particle, (x, y, z, vx), (vy, vz, 0, 0).
+void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, hipStream_t s);
+void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, hipStream_t s);
+// dst[i] = src[perm[i]] (records)
+void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, hipStream_t s);
+void launch_pack(Float6 src, uint32_t n, float4* rec, hipStream_t s);
+void launch_unpack(const float4* rec, uint32_t n, Float6Out dst, hipStream_t s);
+// counts[0..numCells] zeroed, then counts[key] = run length of key in the
+// sorted keys (counts[numCells] stays 0, so an exclusive scan over
+// numCells + 1 entries ends with n).
+void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, hipStream_t s);
+// Trilinear MAC P2G from records sorted by cell key, one workgroup per
+// brick.  WU/WV/WW (weight sums) may be null.
+void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, float* U, float* V, float* W, float* WU,
+                float* WV, float* WW, hipStream_t s);
+
+}  // namespace ns
+}  // namespace flip2

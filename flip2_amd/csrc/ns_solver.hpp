@@ -1,0 +1,63 @@
+// North-star pressure projection, G2P and advection kernels (ns_solver.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace flip2 {
+namespace ns {
+
+constexpr uint8_t kAir = 0;
+constexpr uint8_t kFluid = 1;
+constexpr uint8_t kSolid = 2;
+
+// Deterministic grid-wide reduction scratch: one double per workgroup per
+// quantity, and a completion counter (must start at 0; the last workgroup
+// resets it).
+struct Reduce {
+    double* partials;
+    uint32_t* counter;
+};
+
+void launch_classify_bricks(const uint32_t* counts, const uint8_t* solid, uint32_t numBricks, uint32_t* flag,
+                            hipStream_t s);
+void launch_build_slots(const uint32_t* flag, const uint32_t* excl, uint32_t numBricks, int32_t* brickSlot,
+                        uint32_t* activeBricks, hipStream_t s);
+void launch_brick_setup(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
+                        const uint32_t* counts, const uint8_t* solid, Grid g, int32_t* nbrSlot, uint8_t* cellType,
+                        hipStream_t s);
+void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot, const uint8_t* cellType,
+                     const uint8_t* solid, Grid g, double hOverDt, const float* U, const float* V, const float* W,
+                     uint16_t* coef, double* b64, Reduce red, double* out, hipStream_t s);
+void launch_refine_start(uint32_t numActive, const double* r64, float* r, float* e, const uint16_t* coef, Reduce red,
+                         double* out, hipStream_t s);
+void launch_jacobi_init(uint32_t numActive, const float* r, const uint16_t* coef, float* z, float* p, Reduce red,
+                        double* out, hipStream_t s);
+void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
+                       Reduce red, double* out, hipStream_t s);
+void launch_update(uint32_t numActive, const double* rzOld, const double* pq, const float* p, const float* q,
+                   const uint16_t* coef, float* e, float* r, float* z, bool jacobi, Reduce red, double* out,
+                   hipStream_t s);
+void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, double* out, hipStream_t s);
+void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
+                      bool first, hipStream_t s);
+void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s);
+void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
+                          const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s);
+void launch_fill_f32(float* a, uint32_t count, float v, hipStream_t s);
+void launch_fill_f64(double* a, uint32_t count, double v, hipStream_t s);
+void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, hipStream_t s);
+void launch_mic_apply(uint32_t numActive, const float* r, const uint16_t* coef, const float* precon, float* z,
+                      Reduce red, double* out, hipStream_t s);
+void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
+                            const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,
+                            float* U, float* V, float* W, hipStream_t s);
+void launch_prepare_faces(float* U, float* V, float* W, float* U0, float* V0, float* W0, const uint8_t* solid, Grid g,
+                          float gdt, hipStream_t s);
+// rec: particle records, 2 x float4 per particle (x, y, z, vx), (vy, vz, 0, 0).
+void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
+                     const float* V0, const float* W0, Grid g, float flip, hipStream_t s);
+void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
+                       Grid g, float dt, float margin, hipStream_t s);
+
+}  // namespace ns
+}  // namespace flip2

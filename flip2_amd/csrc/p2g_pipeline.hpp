@@ -1,0 +1,65 @@
+// Particle->grid transfer pipeline on 8^3 bricks: bin -> key/value radix
+// sort -> gather -> per-cell counts -> exclusive scan -> P2G.
+#pragma once
+
+#include "common.hpp"
+#include "device.hpp"
+#include "ns_kernels.hpp"
+
+#include <string>
+#include <vector>
+
+namespace flip2 {
+
+class P2GPipeline {
+public:
+    P2GPipeline(uint32_t maxParticles, const Grid& g);
+    ~P2GPipeline();
+    P2GPipeline(const P2GPipeline&) = delete;
+    P2GPipeline& operator=(const P2GPipeline&) = delete;
+    // Inputs: device FP32 particle state (unsorted).  Outputs: MAC face
+    // velocities U, V, W (n^3 each, brick-tiled) and, inside the pipeline, the
+    // sorted state, sort permutation and per-cell start offsets.
+    void run(uint32_t n, const float* x, const float* y, const float* z, const float* vx, const float* vy,
+             const float* vz, float* U, float* V, float* W, hipStream_t s);
+    // Same, from particle records (2 x float4 per particle).
+    void runRecords(uint32_t n, const float4* rec, float* U, float* V, float* W, hipStream_t s);
+    size_t get(const std::string& name, void* host, size_t capacity, hipStream_t s);
+    // Per-kernel timing of subsequent runs (HIP events on the run's stream):
+    // ms[k] accumulates over runs for kernels "bin", "sort_setup" (clear +
+    // histogram + offsets), "sort_pass0".. , "gather", "counts", "scan", "p2g".
+    // Event sets for `capacity` runs are recorded without host waits; they are
+    // read when all sets are used or at finishTiming().
+    void enableTiming(int capacity);
+    void finishTiming();
+    int timedKernels() const { return (int)timingNames_.size(); }
+    const char* timedKernelName(int k) const { return timingNames_[k].c_str(); }
+    double timedKernelMs(int k) const { return timingMs_[k]; }
+    int timedRuns() const { return timedRuns_; }
+    void resetTiming();
+    const uint32_t* cellStart() const { return cellStart_.devPtr(); }
+    const uint32_t* counts() const { return counts_.devPtr(); }
+    // Sorted particle records of the last run; a caller may swap them into
+    // its own state (DeviceVec::swapDevice).
+    DeviceVec<float4>& sortedVec() { return sorted_; }
+    const float4* sortedRecords() const { return sorted_.devPtr(); }
+    uint32_t numCells() const { return numCells_; }
+    const Grid& grid() const { return grid_; }
+    uint32_t keyBits() const { return keyBits_; }
+
+private:
+    Grid grid_;
+    uint32_t maxParticles_, numCells_, keyBits_, lastN_ = 0;
+    DeviceVec<uint32_t> keys_, keysSorted_, perm_, keysAlt_, permAlt_, counts_, cellStart_;
+    DeviceVec<float4> packed_, sorted_;
+    DeviceVec<uint8_t> ws_;
+    bool timing_ = false;
+    int pending_ = 0;
+    std::vector<std::vector<hipEvent_t>> events_;
+    std::vector<std::string> timingNames_;
+    std::vector<double> timingMs_;
+    int timedRuns_ = 0;
+    void runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W, hipStream_t s);
+};
+
+}  // namespace flip2

@@ -327,7 +327,14 @@ __device__ inline void add_neighbor(uint32_t nb, const uint32_t* nodeOff, const 
     const uint32_t n2 = N1 * N1;
     const uint32_t vs = (nb == 0) ? 0u : nodeOff[nb - 1];
     const uint32_t ve = nodeOff[nb];
-    for (uint32_t k = vs + threadIdx.x; k < ve; k += blockDim.x) {
+    // In the z-negative branch the target index does not depend on the
+    // source x (Q17), so several sources share a target: that phase runs on
+    // one thread in source order.  Every other phase maps sources to
+    // distinct targets.
+    const bool serial = zo < 0;
+    if (serial && threadIdx.x != 0) return;
+    const uint32_t step = serial ? 1u : blockDim.x;
+    for (uint32_t k = vs + (serial ? 0u : threadIdx.x); k < ve; k += step) {
         const uint32_t id = voxelIDs[k];
         const uint32_t sx = id % N1, sy = (id % n2) / N1, sz = id / n2;
         uint32_t tx = sx, ty = sy, tz = sz;

@@ -235,6 +235,10 @@ sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict
 
 }  // namespace
 
+uint32_t sort_passes(uint32_t beginBit, uint32_t endBit) {
+    return (endBit > beginBit) ? ceil_div(endBit - beginBit, 8) : 0;
+}
+
 size_t sort_workspace_bytes(uint32_t n, uint32_t beginBit, uint32_t endBit) {
     if (endBit > 32) endBit = 32;
     return sort_layout(n, beginBit, endBit).total;
@@ -242,7 +246,7 @@ size_t sort_workspace_bytes(uint32_t n, uint32_t beginBit, uint32_t endBit) {
 
 void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* keysOut, uint32_t* valsOut,
                     uint32_t* keysAlt, uint32_t* valsAlt, uint32_t n, uint32_t beginBit, uint32_t endBit,
-                    void* ws, size_t wsBytes, hipStream_t stream) {
+                    void* ws, size_t wsBytes, hipStream_t stream, hipEvent_t* passEvents) {
     FLIP2_REQUIRE(endBit <= 32 && beginBit <= endBit, kErrArgument, "sort: bad bit range");
     FLIP2_REQUIRE(n < (1u << kDigitFlagShift), kErrRange, "sort: n must be below 2^30");
     FLIP2_REQUIRE(keysIn && keysOut && valsOut, kErrArgument, "sort: null buffer");
@@ -280,6 +284,7 @@ void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* ke
     hipLaunchKernelGGL(sort_offsets_kernel, dim3(1), dim3(64 * kMaxPasses), 0, stream, hist, L.passes,
                        offsets);
     FLIP2_LAUNCH_CHECK();
+    if (passEvents) FLIP2_HIP(hipEventRecord(passEvents[0], stream));
 
     // Ping-pong so that the last pass writes keysOut/valsOut.
     const uint32_t* srcK = keysIn;
@@ -295,6 +300,7 @@ void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* ke
                            dstV, n, shift, mask, offsets + p * kRadix,
                            status + (size_t)p * L.tiles * kRadix, counters + p);
         FLIP2_LAUNCH_CHECK();
+        if (passEvents) FLIP2_HIP(hipEventRecord(passEvents[p + 1], stream));
         srcK = dstK;
         srcV = dstV;
     }

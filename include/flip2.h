@@ -166,6 +166,105 @@ int flip2_set_solver_limits(flip2_particles* p, double threshold, uint32_t max_i
 int flip2_set_time(flip2_particles* p, double dt, double frame_dt, double elapsed_time,
                    double elapsed_time_this_frame);
 
+/* ---------------------------------------------------------------------- */
+/* Particle -> grid transfer on 8^3 bricks (north-star method; FP32)      */
+/* ---------------------------------------------------------------------- */
+/* Grids for these entry points have sizes that are multiples of 8.  Cells
+ * and MAC faces are stored brick-tiled: cell (i,j,k) at
+ * ((k/8*nbY + j/8)*nbX + i/8)*512 + (k%8)*64 + (j%8)*8 + i%8, u(i,j,k) on the
+ * low-x face of cell (i,j,k).  The reference keys cells x + y*sx + z*sx*sy
+ * (particleToGridFunctions.cu:40-43) and keeps a node list instead of bricks. */
+
+/* Brick-tiled cell keys of FP32 positions, clamped into the grid. */
+int flip2_ns_bin(const float* d_x, const float* d_y, const float* d_z, uint32_t n, flip2_grid grid, uint32_t* d_keys,
+                 void* stream);
+/* Trilinear MAC P2G onto bricks from particle records sorted by cell key
+ * (8 floats per particle: x, y, z, vx, vy, vz, 0, 0; 16-byte aligned), with
+ * d_cell_start[c] the first particle of cell c (n^3 + 1 entries).  d_wu,
+ * d_wv, d_ww (weight sums) may be NULL.  Replaces the reference's per-node
+ * P2G (gatherParticleVelsToVoxels, particles.cu:379-488) for the north-star
+ * method. */
+int flip2_ns_p2g(const float* d_records, const uint32_t* d_cell_start, flip2_grid grid, float* d_u, float* d_v,
+                 float* d_w, float* d_wu, float* d_wv, float* d_ww, void* stream);
+
+/* The whole transfer as one object: bin -> key/value sort -> gather of the
+ * particle state -> per-cell counts -> exclusive scan -> P2G.  Mirrors the
+ * reference's alignParticlesToGrid / sortParticles / generateVoxels /
+ * particleVelToVoxels sequence (particles.cu:48-78, 323-368, 500-505). */
+typedef struct flip2_p2g_pipeline flip2_p2g_pipeline;
+int flip2_p2g_pipeline_create(uint32_t max_particles, flip2_grid grid, flip2_p2g_pipeline** out);
+int flip2_p2g_pipeline_destroy(flip2_p2g_pipeline* pl);
+int flip2_p2g_pipeline_run(flip2_p2g_pipeline* pl, uint32_t n, const float* d_x, const float* d_y, const float* d_z,
+                           const float* d_vx, const float* d_vy, const float* d_vz, float* d_u, float* d_v,
+                           float* d_w, void* stream);
+/* Intermediates of the last run: "keys", "sortedKeys", "perm", "counts",
+ * "cellStart", "x", "y", "z", "vx", "vy", "vz" (sorted state). */
+int flip2_p2g_pipeline_get(flip2_p2g_pipeline* pl, const char* name, void* host, size_t capacity, size_t* count);
+/* Per-kernel HIP-event timing of the following runs (capacity = runs recorded
+ * between host reads; 0 disables).  flip2_p2g_pipeline_timing returns, for
+ * kernel k, its name and the summed milliseconds over *runs timed runs
+ * (FLIP2_ERR_RANGE past the last kernel). */
+int flip2_p2g_pipeline_enable_timing(flip2_p2g_pipeline* pl, int capacity);
+int flip2_p2g_pipeline_timing(flip2_p2g_pipeline* pl, int k, char* name, size_t name_len, double* total_ms,
+                              int* runs);
+
+/* ---------------------------------------------------------------------- */
+/* North-star FLIP step on 8^3 bricks: transfer -> wall faces + gravity   */
+/* -> active-brick Poisson system -> mixed-precision PCG (FP32 operator   */
+/* and preconditioner, FP64 dots, FP64 true residual and iterative        */
+/* refinement) -> pressure-gradient update -> FLIP G2P -> RK2 advection.  */
+/* The reference's step is Particles::solveFrame (particles.cu:784-803)   */
+/* with a brick-local SOR; this is the configs' plain method.             */
+/* ---------------------------------------------------------------------- */
+#define FLIP2_PREC_JACOBI 0
+#define FLIP2_PREC_MIC0 1
+#define FLIP2_PREC_LEARNED 2
+
+typedef struct flip2_fluid_config {
+    flip2_grid grid;          /* sizes multiples of 8 */
+    float dt;                 /* fixed step */
+    float gravity;            /* added to v faces: gravity * dt */
+    float flip;               /* FLIP blend (0.95) */
+    int preconditioner;       /* FLIP2_PREC_* */
+    double tolerance;         /* ||b - A p||_2 / ||b||_2 in FP64 */
+    uint32_t max_iterations;  /* inner PCG iterations per solve */
+    uint32_t check_every;     /* iterations between residual checks */
+    float margin;             /* particles kept margin*h inside the box */
+} flip2_fluid_config;
+
+typedef struct flip2_solve_stats {
+    uint32_t iterations;
+    uint32_t refinements;
+    double rel_residual;
+    double b_norm;
+    uint32_t active_bricks;
+    uint32_t fluid_cells;
+    double solve_ms;
+    int converged;
+    int used_fallback;
+} flip2_solve_stats;
+
+typedef struct flip2_fluid flip2_fluid;
+void flip2_fluid_default_config(flip2_fluid_config* cfg);
+int flip2_fluid_create(uint32_t max_particles, const flip2_fluid_config* cfg, flip2_fluid** out);
+int flip2_fluid_destroy(flip2_fluid* f);
+int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg);
+int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const float* y, const float* z,
+                              const float* vx, const float* vy, const float* vz);
+int flip2_fluid_get_particles(flip2_fluid* f, float* x, float* y, float* z, float* vx, float* vy, float* vz);
+/* Static solid cells: n^3 bytes indexed x + y*nx + z*nx*ny (NULL clears). */
+int flip2_fluid_set_solid(flip2_fluid* f, const uint8_t* mask);
+int flip2_fluid_step(flip2_fluid* f);
+/* One stage: "transfer", "prepare", "system", "solve", "project", "g2p", "advect". */
+int flip2_fluid_stage(flip2_fluid* f, const char* stage);
+/* Solve the current system with the given preconditioner (the system must
+ * have been built by the "system" stage). */
+int flip2_fluid_solve_with(flip2_fluid* f, int preconditioner, flip2_solve_stats* stats);
+int flip2_fluid_last_solve(flip2_fluid* f, flip2_solve_stats* stats);
+/* "U","V","W","U0","V0","W0","counts","cellStart","activeBricks","brickSlot",
+ * "cellType","coef","b","p". */
+int flip2_fluid_get_array(flip2_fluid* f, const char* name, void* host, size_t capacity, size_t* count);
+
 #ifdef __cplusplus
 }
 #endif

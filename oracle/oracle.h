@@ -132,6 +132,33 @@ void orc_move_solid_particles(orc_particles* s);        /* particles.cu:762-769 
 void orc_initialize(orc_particles* s);                  /* particles.cu:805-812 */
 void orc_solve_frame(orc_particles* s, double fps);     /* particles.cu:784-803 */
 
+/* ------------------------------------------------------------------------ */
+/* North-star method (ns_oracle.c)                                          */
+/* ------------------------------------------------------------------------ */
+uint32_t orc_ns_tiled_index(uint32_t i, uint32_t j, uint32_t k, orc_grid g);
+void orc_ns_bin(const float* x, const float* y, const float* z, uint32_t n, orc_grid g, uint32_t* keys);
+void orc_ns_p2g(const float* x, const float* y, const float* z, const float* vx, const float* vy, const float* vz,
+                uint32_t n, orc_grid g, double* U, double* V, double* W, double* WU, double* WV, double* WW);
+
+void orc_ns_transfer_step(const float* x, const float* y, const float* z, const float* vx, const float* vy,
+                          const float* vz, uint32_t n, orc_grid g, uint32_t keyBits, uint32_t* keys, uint32_t* perm,
+                          float* sorted, uint32_t* counts, uint32_t* cellStart, double* U, double* V, double* W);
+
+void orc_ns_prepare_faces(double* U, double* V, double* W, double* U0, double* V0, double* W0, const uint8_t* solid,
+                          orc_grid g, double gdt);
+void orc_ns_assemble(const uint32_t* counts, const uint8_t* solid, orc_grid g, double hOverDt, const double* U,
+                     const double* V, const double* W, uint8_t* type, uint16_t* coef, double* b);
+void orc_ns_apply_A(const uint16_t* coef, orc_grid g, const double* x, double* y);
+uint32_t orc_ns_pcg_jacobi(const uint16_t* coef, orc_grid g, const double* b, double* x, double tol, uint32_t maxIt,
+                           double* relOut);
+void orc_ns_velocity_update(const uint8_t* type, const uint8_t* solid, orc_grid g, const double* p, double dtOverH,
+                            double* U, double* V, double* W);
+void orc_ns_g2p_flip(uint32_t n, const float* x, const float* y, const float* z, double* vx, double* vy, double* vz,
+                     const double* U, const double* V, const double* W, const double* U0, const double* V0,
+                     const double* W0, orc_grid g, double flip);
+void orc_ns_advect_rk2(uint32_t n, double* x, double* y, double* z, const double* U, const double* V, const double* W,
+                       const uint8_t* solid, orc_grid g, double dt, double margin);
+
 /* Resize voxel arrays to V (used by stage tests that inject state). */
 void orc_resize_voxels(orc_particles* s, uint32_t V);
 void orc_resize_nodes(orc_particles* s, uint32_t B);

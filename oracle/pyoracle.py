@@ -95,6 +95,19 @@ def lib() -> ctypes.CDLL:
         l.orc_solve_frame.argtypes = [P, ctypes.c_double]
         l.orc_resize_voxels.argtypes = [P, ctypes.c_uint32]
         l.orc_resize_nodes.argtypes = [P, ctypes.c_uint32]
+        l.orc_ns_tiled_index.argtypes = [ctypes.c_uint32] * 3 + [OrcGrid]
+        l.orc_ns_tiled_index.restype = ctypes.c_uint32
+        l.orc_ns_bin.argtypes = [vp, vp, vp, ctypes.c_uint32, OrcGrid, vp]
+        l.orc_ns_p2g.argtypes = [vp] * 6 + [ctypes.c_uint32, OrcGrid] + [vp] * 6
+        l.orc_ns_prepare_faces.argtypes = [vp] * 7 + [OrcGrid, ctypes.c_double]
+        l.orc_ns_assemble.argtypes = [vp, vp, OrcGrid, ctypes.c_double, vp, vp, vp, vp, vp, vp]
+        l.orc_ns_apply_A.argtypes = [vp, OrcGrid, vp, vp]
+        l.orc_ns_pcg_jacobi.argtypes = [vp, OrcGrid, vp, vp, ctypes.c_double, ctypes.c_uint32, vp]
+        l.orc_ns_pcg_jacobi.restype = ctypes.c_uint32
+        l.orc_ns_velocity_update.argtypes = [vp, vp, OrcGrid, vp, ctypes.c_double, vp, vp, vp]
+        l.orc_ns_g2p_flip.argtypes = [ctypes.c_uint32] + [vp] * 12 + [OrcGrid, ctypes.c_double]
+        l.orc_ns_advect_rk2.argtypes = [ctypes.c_uint32] + [vp] * 7 + [OrcGrid, ctypes.c_double, ctypes.c_double]
+        l.orc_ns_transfer_step.argtypes = [vp] * 6 + [ctypes.c_uint32, OrcGrid, ctypes.c_uint32] + [vp] * 8
         _lib = l
     return _lib
 
@@ -142,6 +155,124 @@ def philox4x32_10(ctr, key):
     out = np.empty(4, dtype=np.uint32)
     lib().orc_philox4x32_10(_p(c), _p(k), _p(out))
     return out
+
+
+def ns_bin(x, y, z, grid):
+    a = [np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, z)]
+    keys = np.empty(a[0].size, dtype=np.uint32)
+    lib().orc_ns_bin(_p(a[0]), _p(a[1]), _p(a[2]), a[0].size, OrcGrid(*grid), _p(keys))
+    return keys
+
+
+def ns_p2g(x, y, z, vx, vy, vz, grid, weights=False):
+    """FP64 trilinear MAC P2G onto the brick-tiled grid: (U, V, W[, WU, WV, WW])."""
+    a = [np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, z, vx, vy, vz)]
+    g = OrcGrid(*grid)
+    cells = g.sizeX * g.sizeY * g.sizeZ
+    out = [np.empty(cells) for _ in range(6 if weights else 3)]
+    ptrs = [_p(o) for o in out] + ([None] * 3 if not weights else [])
+    lib().orc_ns_p2g(*(_p(v) for v in a), a[0].size, g, *ptrs)
+    return tuple(out)
+
+
+class NsTransferStep:
+    """Serial transfer step (bin, sort, gather, counts, scan, P2G) with
+    preallocated scratch, for timing the CPU baseline."""
+
+    def __init__(self, x, y, z, vx, vy, vz, grid):
+        self.a = [np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, z, vx, vy, vz)]
+        self.g = OrcGrid(*grid)
+        self.n = self.a[0].size
+        cells = self.g.sizeX * self.g.sizeY * self.g.sizeZ
+        self.key_bits = max(1, int(cells - 1).bit_length())
+        self.keys = np.empty(self.n, np.uint32)
+        self.perm = np.empty(self.n, np.uint32)
+        self.sorted = np.empty(6 * self.n, np.float32)
+        self.counts = np.empty(cells + 1, np.uint32)
+        self.start = np.empty(cells + 1, np.uint32)
+        self.out = [np.empty(cells) for _ in range(3)]
+
+    def run(self):
+        lib().orc_ns_transfer_step(*(_p(v) for v in self.a), self.n, self.g, self.key_bits, _p(self.keys),
+                                   _p(self.perm), _p(self.sorted), _p(self.counts), _p(self.start),
+                                   *(_p(o) for o in self.out))
+
+
+def _f64(a):
+    return np.array(a, dtype=np.float64, copy=True)
+
+
+def ns_prepare_faces(U, V, W, grid, gdt, solid=None):
+    F = [_f64(a) for a in (U, V, W)]
+    F0 = [np.empty_like(a) for a in F]
+    s = None if solid is None else np.ascontiguousarray(solid, np.uint8)
+    lib().orc_ns_prepare_faces(*(_p(a) for a in F), *(_p(a) for a in F0), None if s is None else _p(s),
+                               OrcGrid(*grid), gdt)
+    return F, F0
+
+
+def ns_assemble(counts, grid, h_over_dt, U, V, W, solid=None):
+    g = OrcGrid(*grid)
+    cells = g.sizeX * g.sizeY * g.sizeZ
+    c = np.ascontiguousarray(counts[:cells], np.uint32)
+    F = [np.ascontiguousarray(a, np.float64) for a in (U, V, W)]
+    ty = np.empty(cells, np.uint8)
+    coef = np.empty(cells, np.uint16)
+    b = np.empty(cells)
+    s = None if solid is None else np.ascontiguousarray(solid, np.uint8)
+    lib().orc_ns_assemble(_p(c), None if s is None else _p(s), g, h_over_dt, *(_p(a) for a in F), _p(ty), _p(coef),
+                          _p(b))
+    return ty, coef, b
+
+
+def ns_apply_A(coef, grid, x):
+    coef = np.ascontiguousarray(coef, np.uint16)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    lib().orc_ns_apply_A(_p(coef), OrcGrid(*grid), _p(x), _p(y))
+    return y
+
+
+def ns_pcg_jacobi(coef, grid, b, tol=1e-12, max_it=100000):
+    coef = np.ascontiguousarray(coef, np.uint16)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.empty_like(b)
+    rel = ctypes.c_double()
+    it = lib().orc_ns_pcg_jacobi(_p(coef), OrcGrid(*grid), _p(b), _p(x), tol, max_it, ctypes.byref(rel))
+    return x, it, rel.value
+
+
+def ns_velocity_update(types, grid, p, dt_over_h, U, V, W):
+    F = [_f64(a) for a in (U, V, W)]
+    lib().orc_ns_velocity_update(_p(np.ascontiguousarray(types, np.uint8)), None, OrcGrid(*grid),
+                                 _p(np.ascontiguousarray(p, np.float64)), dt_over_h, *(_p(a) for a in F))
+    return F
+
+
+def ns_g2p_flip(x, y, z, vx, vy, vz, U, V, W, U0, V0, W0, grid, flip=0.95):
+    pos = [np.ascontiguousarray(a, np.float32) for a in (x, y, z)]
+    vel = [_f64(a) for a in (vx, vy, vz)]
+    F = [np.ascontiguousarray(a, np.float64) for a in (U, V, W, U0, V0, W0)]
+    lib().orc_ns_g2p_flip(pos[0].size, *(_p(a) for a in pos), *(_p(a) for a in vel), *(_p(a) for a in F),
+                          OrcGrid(*grid), flip)
+    return vel
+
+
+def ns_advect_rk2(x, y, z, U, V, W, grid, dt, margin, solid=None):
+    pos = [_f64(a) for a in (x, y, z)]
+    F = [np.ascontiguousarray(a, np.float64) for a in (U, V, W)]
+    s = None if solid is None else np.ascontiguousarray(solid, np.uint8)
+    lib().orc_ns_advect_rk2(pos[0].size, *(_p(a) for a in pos), *(_p(a) for a in F), None if s is None else _p(s),
+                            OrcGrid(*grid), dt, margin)
+    return pos
+
+
+def ns_tiled_index(i, j, k, grid):
+    """Brick-tiled index of cell (i, j, k) (numpy arrays accepted)."""
+    g = OrcGrid(*grid)
+    nbx, nby = g.sizeX >> 3, g.sizeY >> 3
+    i, j, k = (np.asarray(v, dtype=np.int64) for v in (i, j, k))
+    return ((((k >> 3) * nby + (j >> 3)) * nbx + (i >> 3)) * 512 + (k & 7) * 64 + (j & 7) * 8 + (i & 7))
 
 
 _RC_ARRAYS = {

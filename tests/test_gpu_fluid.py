@@ -1,0 +1,152 @@
+"""GPU checks of the north-star FLIP step (mixed-precision PCG on bricks).
+
+Stages are compared with the FP64 oracle on the same inputs (the GPU's own
+inputs of each stage), 1e-5 normwise relative for transfers, faces, G2P and
+advection.  The solve is judged by its FP64 true residual (<= 1e-6 relative,
+recomputed here by the oracle's operator) and, against a tightly converged
+FP64 solve of the same system, by the field error that residual permits
+(stated per test).  Closed form: a still full-width slab has hydrostatic
+pressure p_j = |g| h (J - j) and zero velocity after projection.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from flip2_amd import scenes
+
+RTOL = 1e-5
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def dense_from_slots(sim, arr, cells):
+    ab = sim.get("activeBricks").astype(np.int64)
+    out = np.zeros(cells, dtype=arr.dtype)
+    idx = (ab[:, None] * 512 + np.arange(512)[None, :]).reshape(-1)
+    out[idx] = arr[:idx.size]
+    return out
+
+
+def dam_break_sim(flip2, n=32, precond="jacobi", seed=1):
+    x, y, z = scenes.dam_break(n, seed=seed)
+    zero = np.zeros_like(x)
+    grid = (0.0, 0.0, 0.0, n, n, n, 1.0 / n)
+    sim = flip2.FluidSim(x.size, grid, preconditioner=precond)
+    sim.set_particles(x, y, z, zero, zero, zero)
+    return sim, grid, (x, y, z)
+
+
+def test_stages_match_oracle(flip2, oracle):
+    n = 32
+    sim, grid, (x, y, z) = dam_break_sim(flip2, n)
+    cells = n ** 3
+    dt, h = 1.0 / 240.0, 1.0 / n
+    # transfer: P2G of the unsorted input
+    sim.stage("transfer")
+    P = [sim.get(c).astype(np.float64) for c in ("U", "V", "W")]
+    Po = oracle.ns_p2g(x, y, z, np.zeros_like(x), np.zeros_like(x), np.zeros_like(x), grid)
+    for a, b in zip(P, Po):
+        assert np.max(np.abs(a - b)) <= 1e-6  # all-zero velocities: exact zero field
+    # prepare: walls + gravity, saved field
+    sim.stage("prepare")
+    F, F0 = oracle.ns_prepare_faces(*P, grid, -9.81 * dt)
+    for name, ref in zip(("U", "V", "W", "U0", "V0", "W0"), F + F0):
+        assert rel_err(sim.get(name), ref) <= RTOL or np.max(np.abs(ref)) == 0.0, name
+    # system
+    sim.stage("system")
+    counts = sim.get("counts")
+    Fg = [sim.get(c).astype(np.float64) for c in ("U", "V", "W")]
+    ty, coef, b = oracle.ns_assemble(counts, grid, h / dt, *Fg)
+    assert np.array_equal(dense_from_slots(sim, sim.get("coef"), cells), coef)
+    assert rel_err(dense_from_slots(sim, sim.get("b"), cells), b) <= RTOL
+    # solve: FP64 true residual of the GPU pressure
+    sim.stage("solve")
+    st = sim.last_solve()
+    assert st["converged"], st
+    p = dense_from_slots(sim, sim.get("p"), cells)
+    r = b - oracle.ns_apply_A(coef, grid, p)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-6
+    pref, _, rel = oracle.ns_pcg_jacobi(coef, grid, b, tol=1e-12)
+    assert rel <= 1e-12
+    assert rel_err(p, pref) <= 2e-3  # residual 1e-6 x condition number of the 32^3 Poisson operator
+    # project
+    sim.stage("project")
+    Fp = oracle.ns_velocity_update(ty, grid, p, dt / h, *Fg)
+    for name, ref in zip(("U", "V", "W"), Fp):
+        assert rel_err(sim.get(name), ref) <= RTOL, name
+    # divergence of the projected field, relative to before
+    _, _, b_after = oracle.ns_assemble(counts, grid, h / dt, *[sim.get(c).astype(np.float64) for c in ("U", "V", "W")])
+    assert np.linalg.norm(b_after) <= 2e-6 * np.linalg.norm(b)
+    # g2p (state is in sorted order after the transfer)
+    xs, ys, zs, vxs, vys, vzs = sim.particles()
+    Fn = [sim.get(c) for c in ("U", "V", "W", "U0", "V0", "W0")]
+    vo = oracle.ns_g2p_flip(xs, ys, zs, vxs, vys, vzs, *Fn, grid, 0.95)
+    sim.stage("g2p")
+    _, _, _, vx2, vy2, vz2 = sim.particles()
+    for a, ref in zip((vx2, vy2, vz2), vo):
+        assert rel_err(a, ref) <= RTOL
+    # advect
+    xo = oracle.ns_advect_rk2(xs, ys, zs, *Fn[:3], grid, dt, 1e-3 * h)
+    sim.stage("advect")
+    x2, y2, z2, _, _, _ = sim.particles()
+    for a, ref in zip((x2, y2, z2), xo):
+        assert rel_err(a, ref) <= RTOL
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "mic0"])
+def test_hydrostatic_slab_closed_form(flip2, precond):
+    n = 32
+    h = 1.0 / n
+    J = n // 2
+    x, y, z = scenes.jittered_cells((n, n, n), h, seed=9, keep=lambda a, b, c: b < np.float32(0.5))
+    zero = np.zeros_like(x)
+    grid = (0.0, 0.0, 0.0, n, n, n, h)
+    sim = flip2.FluidSim(x.size, grid, preconditioner=precond)
+    sim.set_particles(x, y, z, zero, zero, zero)
+    for s in ("transfer", "prepare", "system", "solve"):
+        sim.stage(s)
+    assert sim.last_solve()["converged"]
+    p = dense_from_slots(sim, sim.get("p"), n ** 3)
+    from oracle import pyoracle
+    jj = np.arange(n)
+    i, j, k = np.meshgrid(jj, jj, jj, indexing="ij")
+    idx = pyoracle.ns_tiled_index(i, j, k, grid)
+    expect = np.where(j < J, 9.81 * h * (J - j), 0.0)
+    got = p[idx]
+    assert rel_err(got, expect) <= 1e-4
+    sim.stage("project")
+    V = sim.get("V")
+    assert np.max(np.abs(V[idx][j <= J])) <= 1e-4 * 9.81 / 240.0
+
+
+def test_mic0_fewer_iterations_than_jacobi(flip2):
+    sim, grid, _ = dam_break_sim(flip2, 64)
+    for s in ("transfer", "prepare", "system"):
+        sim.stage(s)
+    sj = sim.solve_with("jacobi")
+    sm = sim.solve_with("mic0")
+    assert sj["converged"] and sm["converged"]
+    assert sm["rel_residual"] <= 1e-6 and sj["rel_residual"] <= 1e-6
+    assert sm["iterations"] < sj["iterations"], (sm, sj)
+
+
+def test_dam_break_steps(flip2):
+    sim, grid, (x, y, z) = dam_break_sim(flip2, 64)
+    n0 = x.size
+    y0 = float(np.mean(y))
+    for _ in range(10):
+        sim.step()
+        st = sim.last_solve()
+        assert st["converged"] and st["rel_residual"] <= 1e-6, st
+    xs, ys, zs, vx, vy, vz = sim.particles()
+    assert xs.size == n0
+    for a in (xs, ys, zs):
+        assert np.all(a > 0.0) and np.all(a < 1.0)
+    assert float(np.mean(ys)) < y0          # the column falls
+    assert float(np.mean(vy)) < 0.0
+    assert np.all(np.isfinite(vx)) and np.all(np.isfinite(vy)) and np.all(np.isfinite(vz))
