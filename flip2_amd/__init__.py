@@ -180,6 +180,21 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_get_array": [_vp, ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_sz)],
         "flip2_p2g_pipeline_timing": [_vp, ctypes.c_int, ctypes.c_char_p, _sz, ctypes.POINTER(_f64),
                                       ctypes.POINTER(ctypes.c_int)],
+        "flip2_unet_create": [ctypes.c_int, ctypes.POINTER(_vp)],
+        "flip2_unet_destroy": [_vp],
+        "flip2_unet_num_params": [_vp, ctypes.POINTER(_sz)],
+        "flip2_unet_init": [_vp, ctypes.c_uint64],
+        "flip2_unet_set_params": [_vp, _vp, _sz],
+        "flip2_unet_get_params": [_vp, _vp, _sz],
+        "flip2_unet_apply_dense": [_vp, _vp, _vp, _vp],
+        "flip2_unet_loss_grad": [_vp, _vp, _vp, ctypes.POINTER(_f64), _vp],
+        "flip2_unet_add_frame": [_vp, _vp],
+        "flip2_unet_num_frames": [_vp, ctypes.POINTER(ctypes.c_int)],
+        "flip2_unet_clear_frames": [_vp],
+        "flip2_unet_train": [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                             ctypes.POINTER(_f64)],
+        "flip2_unet_eval": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_f64)],
+        "flip2_fluid_set_unet": [_vp, _vp, _u32],
     }
     for name in ("align_particles_to_grid", "sort_particles", "generate_voxels", "particle_vel_to_voxels",
                  "calc_div_u", "pressure_solve", "update_voxel_velocities", "voxel_vels_to_particles",
@@ -452,6 +467,11 @@ class FluidSim:
         check(lib().flip2_fluid_last_solve(self._h, ctypes.byref(s)))
         return _stats_dict(s)
 
+    def set_unet(self, net: Optional[UNet], fallback_iterations: int = 0) -> None:
+        """Attach `net` as the "learned" preconditioner (None detaches)."""
+        check(lib().flip2_fluid_set_unet(self._h, net._h if net is not None else None, int(fallback_iterations)))
+        self._unet = net
+
     def get(self, name: str) -> np.ndarray:
         cnt = _sz()
         check(lib().flip2_fluid_get_array(self._h, name.encode(), None, 0, ctypes.byref(cnt)))
@@ -459,6 +479,90 @@ class FluidSim:
         check(lib().flip2_fluid_get_array(self._h, name.encode(), _ptr(out) if out.size else None, out.size,
                                           ctypes.byref(cnt)))
         return out
+
+
+class UNet:
+    """Learned preconditioner network on a dense n^3 grid (see include/flip2.h)."""
+
+    def __init__(self, n: int, seed: Optional[int] = None):
+        h = _vp()
+        check(lib().flip2_unet_create(int(n), ctypes.byref(h)))
+        self._h = h.value
+        self.n = int(n)
+        cnt = _sz()
+        check(lib().flip2_unet_num_params(self._h, ctypes.byref(cnt)))
+        self.num_params = cnt.value
+        if seed is not None:
+            check(lib().flip2_unet_init(self._h, int(seed)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().flip2_unet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, seed: int) -> None:
+        check(lib().flip2_unet_init(self._h, int(seed)))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.num_params, np.float32)
+        check(lib().flip2_unet_get_params(self._h, _ptr(out), out.size))
+        return out
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().flip2_unet_set_params(self._h, _ptr(a), a.size))
+
+    def _dense(self, b, coef):
+        nv = self.n ** 3
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+        cc = np.ascontiguousarray(coef, dtype=np.uint16).reshape(-1)
+        if bb.size != nv or cc.size != nv:
+            raise ValueError(f"expected {nv} values")
+        return bb, cc
+
+    def apply_dense(self, b, coef) -> np.ndarray:
+        bb, cc = self._dense(b, coef)
+        z = np.empty(bb.size, np.float32)
+        check(lib().flip2_unet_apply_dense(self._h, _ptr(bb), _ptr(cc), _ptr(z)))
+        return z
+
+    def loss_grad(self, b, coef, grad: bool = True):
+        bb, cc = self._dense(b, coef)
+        loss = _f64()
+        g = np.empty(self.num_params, np.float32) if grad else None
+        check(lib().flip2_unet_loss_grad(self._h, _ptr(bb), _ptr(cc), ctypes.byref(loss),
+                                         _ptr(g) if grad else None))
+        return loss.value, g
+
+    def add_frame(self, sim: "FluidSim") -> None:
+        check(lib().flip2_unet_add_frame(self._h, sim._h))
+
+    def num_frames(self) -> int:
+        c = ctypes.c_int()
+        check(lib().flip2_unet_num_frames(self._h, ctypes.byref(c)))
+        return c.value
+
+    def clear_frames(self) -> None:
+        check(lib().flip2_unet_clear_frames(self._h))
+
+    def train(self, epochs: int, lr: float, first: int = 0, last: Optional[int] = None, seed: int = 0) -> float:
+        last = self.num_frames() if last is None else last
+        loss = _f64()
+        check(lib().flip2_unet_train(self._h, int(epochs), float(lr), int(first), int(last), int(seed),
+                                     ctypes.byref(loss)))
+        return loss.value
+
+    def eval(self, first: int = 0, last: Optional[int] = None) -> float:
+        last = self.num_frames() if last is None else last
+        loss = _f64()
+        check(lib().flip2_unet_eval(self._h, int(first), int(last), ctypes.byref(loss)))
+        return loss.value
 
 
 # --------------------------------------------------------------------------

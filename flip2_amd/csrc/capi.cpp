@@ -8,6 +8,7 @@
 
 #include "common.hpp"
 #include "fluid.hpp"
+#include "unet.hpp"
 #include "ns_kernels.hpp"
 #include "p2g_pipeline.hpp"
 #include "particles.hpp"
@@ -23,6 +24,40 @@ struct flip2_p2g_pipeline {
 struct flip2_fluid {
     flip2::FluidSim impl;
     flip2_fluid(uint32_t n, const flip2::FluidConfig& c) : impl(n, c) {}
+};
+
+struct flip2_unet {
+    flip2::UNet net;
+    hipStream_t stream = nullptr;
+    struct Frame {
+        flip2::DeviceVec<float> b, x;
+        flip2::DeviceVec<uint16_t> coef;
+    };
+    std::vector<std::unique_ptr<Frame>> frames;
+    flip2::DeviceVec<float> res, dOut, bTmp;
+    flip2::DeviceVec<uint16_t> coefTmp;
+    flip2::DeviceVec<double> red;
+    explicit flip2_unet(int n) : net(n, nullptr) {
+        const uint32_t nv = (uint32_t)n * n * n;
+        res.resizeAsync(nv, nullptr);
+        dOut.resizeAsync(nv, nullptr);
+        red.resizeAsync(8, nullptr);
+    }
+    int n() const { return net.gridSize(); }
+    // loss of one dense system; gradients accumulated when `grad`
+    double lossGrad(const float* bD, const uint16_t* coefD, bool grad) {
+        const int nn = n();
+        flip2::nn::launch_dense_input(bD, coefD, nn, net.input(), net.mask(), net.scale(), red.devPtr() + 2, nullptr);
+        net.forward(nullptr);
+        flip2::nn::launch_residual(coefD, net.output(), bD, res.devPtr(), nn, red.devPtr(), nullptr);
+        if (grad) {
+            flip2::nn::launch_loss_grad(coefD, res.devPtr(), red.devPtr(), dOut.devPtr(), nn, nullptr);
+            net.backward(dOut.devPtr(), nullptr);
+        }
+        double h[2];
+        FLIP2_HIP(hipMemcpy(h, red.devPtr(), sizeof(h), hipMemcpyDeviceToHost));
+        return h[1] > 0.0 ? h[0] / h[1] : 0.0;
+    }
 };
 
 struct flip2_particles {
@@ -611,6 +646,164 @@ int flip2_fluid_get_array(flip2_fluid* f, const char* name, void* host, size_t c
         NEED(f); NEED(name);
         const size_t c = f->impl.getArray(name, host, capacity);
         if (count) *count = c;
+    });
+}
+
+int flip2_unet_create(int n, flip2_unet** out) {
+    return guarded([&] {
+        NEED(out);
+        *out = new flip2_unet(n);
+    });
+}
+int flip2_unet_destroy(flip2_unet* net) {
+    return guarded([&] {
+        if (net) FLIP2_HIP(hipDeviceSynchronize());
+        delete net;
+    });
+}
+int flip2_unet_num_params(flip2_unet* net, size_t* count) {
+    return guarded([&] {
+        NEED(net); NEED(count);
+        *count = net->net.numParams();
+    });
+}
+int flip2_unet_init(flip2_unet* net, uint64_t seed) {
+    return guarded([&] {
+        NEED(net);
+        net->net.initParams(seed);
+    });
+}
+int flip2_unet_set_params(flip2_unet* net, const float* host, size_t count) {
+    return guarded([&] {
+        NEED(net); NEED(host);
+        net->net.setParams(host, count);
+    });
+}
+int flip2_unet_get_params(flip2_unet* net, float* host, size_t count) {
+    return guarded([&] {
+        NEED(net); NEED(host);
+        net->net.getParams(host, count);
+    });
+}
+namespace {
+void upload_dense(flip2_unet* net, const float* b, const uint16_t* coef) {
+    const uint32_t nv = (uint32_t)net->n() * net->n() * net->n();
+    if (net->bTmp.size() < nv) net->bTmp.resizeAsync(nv, nullptr);
+    if (net->coefTmp.size() < nv) net->coefTmp.resizeAsync(nv, nullptr);
+    net->bTmp.uploadFrom(b, nv, nullptr);
+    net->coefTmp.uploadFrom(coef, nv, nullptr);
+}
+}  // namespace
+int flip2_unet_apply_dense(flip2_unet* net, const float* b, const uint16_t* coef, float* z) {
+    return guarded([&] {
+        NEED(net); NEED(b); NEED(coef); NEED(z);
+        upload_dense(net, b, coef);
+        const int nn = net->n();
+        flip2::nn::launch_dense_input(net->bTmp.devPtr(), net->coefTmp.devPtr(), nn, net->net.input(),
+                                      net->net.mask(), net->net.scale(), net->red.devPtr() + 2, nullptr);
+        net->net.forward(nullptr);
+        FLIP2_HIP(hipMemcpy(z, net->net.output(), (size_t)nn * nn * nn * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+int flip2_unet_loss_grad(flip2_unet* net, const float* b, const uint16_t* coef, double* loss, float* grad) {
+    return guarded([&] {
+        NEED(net); NEED(b); NEED(coef);
+        upload_dense(net, b, coef);
+        net->net.zeroGrad(nullptr);
+        const double l = net->lossGrad(net->bTmp.devPtr(), net->coefTmp.devPtr(), grad != nullptr);
+        if (loss) *loss = l;
+        if (grad)
+            FLIP2_HIP(hipMemcpy(grad, net->net.gradDev(), net->net.numParams() * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+int flip2_unet_add_frame(flip2_unet* net, flip2_fluid* f) {
+    return guarded([&] {
+        NEED(net); NEED(f);
+        flip2::FluidSim& sim = f->impl;
+        FLIP2_REQUIRE(sim.gridN() == net->n(), flip2::kErrArgument, "grid size differs from the network's");
+        sim.synchronize();
+        const uint32_t nv = (uint32_t)net->n() * net->n() * net->n();
+        std::unique_ptr<flip2_unet::Frame> fr(new flip2_unet::Frame);
+        fr->b.resizeAsync(nv, nullptr);
+        fr->x.resizeAsync(nv, nullptr);
+        fr->coef.resizeAsync(nv, nullptr);
+        FLIP2_HIP(hipMemset(fr->b.devPtr(), 0, fr->b.bytes()));
+        FLIP2_HIP(hipMemset(fr->x.devPtr(), 0, fr->x.bytes()));
+        FLIP2_HIP(hipMemset(fr->coef.devPtr(), 0, fr->coef.bytes()));
+        flip2::nn::launch_dense_system(sim.numActive(), sim.activeBricksDev(), sim.coefDev(), sim.rhsDev(),
+                                       sim.solutionDev(), net->n(), fr->b.devPtr(), fr->x.devPtr(), fr->coef.devPtr(),
+                                       nullptr);
+        FLIP2_HIP(hipDeviceSynchronize());
+        net->frames.push_back(std::move(fr));
+    });
+}
+int flip2_unet_num_frames(flip2_unet* net, int* count) {
+    return guarded([&] {
+        NEED(net); NEED(count);
+        *count = (int)net->frames.size();
+    });
+}
+int flip2_unet_clear_frames(flip2_unet* net) {
+    return guarded([&] {
+        NEED(net);
+        net->frames.clear();
+    });
+}
+int flip2_unet_train(flip2_unet* net, int epochs, float lr, int first, int last, uint64_t seed, double* mean_loss) {
+    return guarded([&] {
+        NEED(net);
+        FLIP2_REQUIRE(first >= 0 && last <= (int)net->frames.size() && first < last, flip2::kErrRange,
+                      "bad frame range");
+        std::vector<int> order;
+        for (int i = first; i < last; ++i) order.push_back(i);
+        uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+        double mean = 0.0;
+        for (int e = 0; e < epochs; ++e) {
+            for (size_t i = order.size(); i > 1; --i) {  // Fisher-Yates with an LCG
+                st = st * 6364136223846793005ull + 1442695040888963407ull;
+                std::swap(order[i - 1], order[(size_t)((st >> 33) % i)]);
+            }
+            double sum = 0.0;
+            for (int fi : order) {
+                flip2_unet::Frame& fr = *net->frames[fi];
+                net->net.zeroGrad(nullptr);
+                sum += net->lossGrad(fr.b.devPtr(), fr.coef.devPtr(), true);
+                net->net.adamStep(lr, nullptr);
+            }
+            mean = sum / (double)order.size();
+        }
+        FLIP2_HIP(hipDeviceSynchronize());
+        if (mean_loss) *mean_loss = mean;
+    });
+}
+int flip2_unet_eval(flip2_unet* net, int first, int last, double* mean_loss) {
+    return guarded([&] {
+        NEED(net);
+        FLIP2_REQUIRE(first >= 0 && last <= (int)net->frames.size() && first < last, flip2::kErrRange,
+                      "bad frame range");
+        double sum = 0.0;
+        for (int i = first; i < last; ++i)
+            sum += net->lossGrad(net->frames[i]->b.devPtr(), net->frames[i]->coef.devPtr(), false);
+        if (mean_loss) *mean_loss = sum / (double)(last - first);
+    });
+}
+int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iterations) {
+    return guarded([&] {
+        NEED(f);
+        flip2::FluidSim& sim = f->impl;
+        flip2::FluidConfig c = sim.config();
+        c.learnedFallbackIterations = fallback_iterations;
+        sim.setConfig(c);
+        if (!net) {
+            sim.setLearnedPreconditioner(nullptr);
+            return;
+        }
+        FLIP2_REQUIRE(sim.gridN() == net->n(), flip2::kErrArgument, "grid size differs from the network's");
+        flip2::UNet* u = &net->net;
+        flip2::FluidSim* s = &sim;
+        sim.setLearnedPreconditioner([u, s](uint32_t na, const float* r, float* z, hipStream_t st) {
+            u->applyActive(na, s->activeBricksDev(), s->cellTypeDev(), r, z, s->gridN(), st);
+        });
     });
 }
 

@@ -159,6 +159,20 @@ SolveStats FluidSim::solve() { return solveWith(cfg_.preconditioner); }
 // preconditioner) with FP64 dot products, to a relative reduction that lets
 // the FP64 residual reach the tolerance.
 SolveStats FluidSim::solveWith(int prec) {
+    if (prec == kPrecLearned && cfg_.learnedFallbackIterations > 0) {
+        SolveStats st = solveOnce(prec, cfg_.learnedFallbackIterations);
+        if (st.converged) return st;
+        const double ms = st.solveMs;
+        SolveStats fb = solveOnce(kPrecJacobi, cfg_.maxIterations);
+        fb.usedFallback = true;
+        fb.solveMs += ms;
+        stats_ = fb;
+        return fb;
+    }
+    return solveOnce(prec, cfg_.maxIterations);
+}
+
+SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     SolveStats st;
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t na = numActive_;
@@ -184,7 +198,7 @@ SolveStats FluidSim::solveWith(int prec) {
     const bool jac = prec == kPrecJacobi;
     double rel = 1.0;
     uint32_t total = 0;
-    const uint32_t cap = cfg_.maxIterations;
+    const uint32_t cap = std::min(capIn, cfg_.maxIterations);
     std::vector<double> h(4 * (cfg_.checkEvery + 1));
     double* H = hist_.devPtr();
     for (uint32_t refine = 0; refine < 12 && total < cap; ++refine) {
@@ -222,8 +236,12 @@ SolveStats FluidSim::solveWith(int prec) {
                 } else if (prec == kPrecLearned) {
                     learned_(na, r_.devPtr(), z_.devPtr(), stream_);
                     ns::launch_dot(na, r_.devPtr(), z_.devPtr(), reduce(), hn, stream_);
+                    ns::launch_dot(na, z_.devPtr(), q_.devPtr(), reduce(), hn + 3, stream_);
                 }
-                ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
+                if (prec == kPrecLearned)
+                    ns::launch_direction_flex(na, hn + 3, hk + 2, z_.devPtr(), p_.devPtr(), stream_);
+                else
+                    ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
             }
             readHistory(first + 1, batch, h.data());
             for (uint32_t k = 0; k < batch; ++k) {

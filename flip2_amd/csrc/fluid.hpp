@@ -29,6 +29,9 @@ struct FluidConfig {
     uint32_t maxIterations = 4000; // inner iterations over all refinements
     uint32_t checkEvery = 8;       // iterations between host residual checks
     float margin = 1e-3f;          // particles are kept this far (x h) inside the box
+    // Learned preconditioner: fall back to Jacobi-PCG when flexible CG needs
+    // more iterations than this (0: no fallback).
+    uint32_t learnedFallbackIterations = 0;
 };
 
 struct SolveStats {
@@ -76,6 +79,7 @@ public:
     // Solve A x = b for the current system with a caller-given preconditioner
     // choice (used for preconditioner comparisons on one system).
     SolveStats solveWith(int preconditioner);
+    SolveStats solveOnce(int preconditioner, uint32_t cap);
     void setLearnedPreconditioner(PrecondFn fn) { learned_ = std::move(fn); }
 
     const SolveStats& lastSolve() const { return stats_; }
@@ -93,6 +97,9 @@ public:
     const int32_t* nbrSlotDev() const { return nbrSlot_.devPtr(); }
     const uint32_t* activeBricksDev() const { return activeBricks_.devPtr(); }
     const uint8_t* cellTypeDev() const { return cellType_.devPtr(); }
+    const double* rhsDev() const { return b64_.devPtr(); }
+    const double* solutionDev() const { return x64_.devPtr(); }
+    int gridN() const { return (int)cfg_.grid.sizeX; }
 
 private:
     void ensureSlots(uint32_t numActive);

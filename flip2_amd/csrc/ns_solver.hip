@@ -396,6 +396,16 @@ direction_kernel(const double* __restrict__ rzNew, const double* __restrict__ rz
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) p[base + l] = fmaf(beta, p[base + l], z[base + l]);
 }
 
+// Flexible CG (Notay's FCG(1)) direction: beta = -(z . q) / (p . q),
+// p = z + beta p, for a preconditioner that is not a fixed SPD matrix.
+__global__ void __launch_bounds__(kWG)
+direction_flex_kernel(const double* __restrict__ zq, const double* __restrict__ pq, const float* __restrict__ z,
+                      float* __restrict__ p) {
+    const float beta = (*pq != 0.0) ? (float)(-*zq / *pq) : 0.f;
+    const uint32_t base = blockIdx.x * 512u;
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) p[base + l] = fmaf(beta, p[base + l], z[base + l]);
+}
+
 // x64 += e; r64 = b - A x64 (FP64); ||r64||^2 into out[0].
 __global__ void __launch_bounds__(kWG)
 accumulate_kernel(const float* __restrict__ e, double* __restrict__ x64, const uint16_t* __restrict__ coef) {
@@ -863,6 +873,11 @@ void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, 
 void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
                       bool first, hipStream_t s) {
     hipLaunchKernelGGL(direction_kernel, dim3(numActive), dim3(kWG), 0, s, rzNew, rzOld, z, p, first ? 1 : 0);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_direction_flex(uint32_t numActive, const double* zq, const double* pq, const float* z, float* p,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(direction_flex_kernel, dim3(numActive), dim3(kWG), 0, s, zq, pq, z, p);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s) {

@@ -40,6 +40,8 @@ void launch_update(uint32_t numActive, const double* rzOld, const double* pq, co
 void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, double* out, hipStream_t s);
 void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
                       bool first, hipStream_t s);
+void launch_direction_flex(uint32_t numActive, const double* zq, const double* pq, const float* z, float* p,
+                           hipStream_t s);
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s);
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s);

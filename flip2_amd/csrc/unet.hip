@@ -1,0 +1,427 @@
+// Learned preconditioner host side (see unet.hpp).
+#include "unet.hpp"
+
+#include "unet_kernels.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace flip2 {
+
+namespace nn {
+
+namespace {
+__device__ inline uint16_t to_bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline void dense_coords(uint32_t brick, uint32_t l, int n, int& x, int& y, int& z) {
+    const uint32_t nb = (uint32_t)n >> 3;
+    x = (int)((brick % nb) * 8u + (l & 7u));
+    y = (int)(((brick / nb) % nb) * 8u + ((l >> 3) & 7u));
+    z = (int)((brick / (nb * nb)) * 8u + (l >> 6));
+}
+
+inline dim3 g1(long long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+}  // namespace
+
+__global__ void zero_red_kernel(double* red, int count) {
+    if ((int)threadIdx.x < count) red[threadIdx.x] = 0.0;
+}
+
+__global__ void clear_input_kernel(uint16_t* __restrict__ in0, uint8_t* __restrict__ mask, long long nvox) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    in0[v * 16] = 0;
+    in0[v * 16 + 1] = 0;
+    mask[v] = 0;
+}
+
+// red[0] += sum r^2, red[1] += fluid count (active layout)
+__global__ void active_norm_kernel(const uint8_t* __restrict__ cellType, const float* __restrict__ r, uint32_t cells,
+                                   double* __restrict__ red) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, c = 0.0;
+    if (i < cells && cellType[i] == 1) {
+        a = (double)r[i] * (double)r[i];
+        c = 1.0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        c += __shfl_xor(c, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (a != 0.0 || c != 0.0)) {
+        atomicAdd(&red[0], a);
+        atomicAdd(&red[1], c);
+    }
+}
+
+__device__ inline float rms_scale(const double* red) {
+    const double cnt = red[1] > 0.0 ? red[1] : 1.0;
+    const double s = sqrt(red[0] / cnt);
+    return s > 0.0 ? (float)s : 1.0f;
+}
+
+__global__ void scatter_input_kernel(const uint32_t* __restrict__ activeBricks, const uint8_t* __restrict__ cellType,
+                                     const float* __restrict__ r, uint32_t cells, int n, uint16_t* __restrict__ in0,
+                                     uint8_t* __restrict__ mask, float* __restrict__ scale, const double* red) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float s = rms_scale(red);
+    if (i == 0) scale[0] = s;
+    if (i >= cells || cellType[i] != 1) return;
+    int x, y, z;
+    dense_coords(activeBricks[i >> 9], i & 511u, n, x, y, z);
+    const long long v = ((long long)z * n + y) * n + x;
+    in0[v * 16] = to_bf(r[i] / s);
+    in0[v * 16 + 1] = to_bf(1.0f);
+    mask[v] = 1;
+}
+
+__global__ void gather_output_kernel(const uint32_t* __restrict__ activeBricks, const uint8_t* __restrict__ cellType,
+                                     const float* __restrict__ out, uint32_t cells, int n, float* __restrict__ z) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    float v = 0.f;
+    if (cellType[i] == 1) {
+        int x, y, zz;
+        dense_coords(activeBricks[i >> 9], i & 511u, n, x, y, zz);
+        v = out[((long long)zz * n + y) * n + x];
+    }
+    z[i] = v;
+}
+
+__global__ void dense_system_kernel(const uint32_t* __restrict__ activeBricks, const uint16_t* __restrict__ coef,
+                                    const double* __restrict__ b, const double* __restrict__ x, uint32_t cells, int n,
+                                    float* __restrict__ bD, float* __restrict__ xD, uint16_t* __restrict__ coefD) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const uint16_t cf = coef[i];
+    if (!cf) return;
+    int xx, yy, zz;
+    dense_coords(activeBricks[i >> 9], i & 511u, n, xx, yy, zz);
+    const long long v = ((long long)zz * n + yy) * n + xx;
+    bD[v] = (float)b[i];
+    if (xD) xD[v] = (float)x[i];
+    coefD[v] = cf;
+}
+
+__device__ inline float dense_A(const uint16_t* coefD, const float* x, int n, long long v) {
+    const uint16_t cf = coefD[v];
+    if (!cf) return 0.f;
+    const long long st[3] = {1, n, (long long)n * n};
+    float s = (float)(cf & 7u) * x[v];
+    const uint32_t m = cf >> 3;
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+        if ((m >> d) & 1u) s -= x[v + ((d & 1) ? st[d >> 1] : -st[d >> 1])];
+    return s;
+}
+
+__global__ void dense_apply_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ x, float* __restrict__ y,
+                                   int n) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long long)n * n * n) return;
+    y[v] = dense_A(coefD, x, n, v);
+}
+
+__global__ void residual_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ z,
+                                const float* __restrict__ bD, float* __restrict__ res, int n, double* __restrict__ red) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, c = 0.0;
+    if (v < (long long)n * n * n) {
+        float rv = 0.f;
+        if (coefD[v]) {
+            rv = dense_A(coefD, z, n, v) - bD[v];
+            c = (double)bD[v] * (double)bD[v];
+        }
+        res[v] = rv;
+        a = (double)rv * (double)rv;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        c += __shfl_xor(c, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (a != 0.0 || c != 0.0)) {
+        atomicAdd(&red[0], a);
+        atomicAdd(&red[1], c);
+    }
+}
+
+__global__ void loss_grad_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ res,
+                                 const double* __restrict__ red, float* __restrict__ dOut, int n) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long long)n * n * n) return;
+    const float k = red[1] > 0.0 ? (float)(2.0 / red[1]) : 0.f;
+    dOut[v] = k * dense_A(coefD, res, n, v);
+}
+
+__global__ void dense_norm_kernel(const float* __restrict__ bD, const uint16_t* __restrict__ coefD, long long nvox,
+                                  double* __restrict__ red) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, c = 0.0;
+    if (v < nvox && coefD[v]) {
+        a = (double)bD[v] * (double)bD[v];
+        c = 1.0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        c += __shfl_xor(c, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (a != 0.0 || c != 0.0)) {
+        atomicAdd(&red[0], a);
+        atomicAdd(&red[1], c);
+    }
+}
+
+__global__ void dense_input_kernel(const float* __restrict__ bD, const uint16_t* __restrict__ coefD, long long nvox,
+                                   uint16_t* __restrict__ in0, uint8_t* __restrict__ mask, float* __restrict__ scale,
+                                   const double* __restrict__ red) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const float s = rms_scale(red);
+    if (v == 0) scale[0] = s;
+    if (v >= nvox) return;
+    const bool f = coefD[v] != 0;
+    in0[v * 16] = f ? to_bf(bD[v] / s) : (uint16_t)0;
+    in0[v * 16 + 1] = f ? to_bf(1.0f) : (uint16_t)0;
+    mask[v] = f ? 1 : 0;
+}
+
+void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
+                          int n, uint16_t* in0, uint8_t* mask, float* scale, double* red, hipStream_t s) {
+    const long long nvox = (long long)n * n * n;
+    const uint32_t cells = numActive * 512u;
+    hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
+    hipLaunchKernelGGL(clear_input_kernel, g1(nvox, 256), dim3(256), 0, s, in0, mask, nvox);
+    hipLaunchKernelGGL(active_norm_kernel, g1(cells, 256), dim3(256), 0, s, cellType, r, cells, red);
+    hipLaunchKernelGGL(scatter_input_kernel, g1(cells ? cells : 1, 256), dim3(256), 0, s, activeBricks, cellType, r,
+                       cells, n, in0, mask, scale, red);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_gather_output(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* out,
+                          int n, float* z, hipStream_t s) {
+    const uint32_t cells = numActive * 512u;
+    if (!cells) return;
+    hipLaunchKernelGGL(gather_output_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, cellType, out, cells, n, z);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_dense_system(uint32_t numActive, const uint32_t* activeBricks, const uint16_t* coef, const double* b,
+                         const double* x, int n, float* bD, float* xD, uint16_t* coefD, hipStream_t s) {
+    const uint32_t cells = numActive * 512u;
+    if (!cells) return;
+    hipLaunchKernelGGL(dense_system_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, coef, b, x, cells, n, bD, xD,
+                       coefD);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_dense_apply_A(const uint16_t* coefD, const float* x, float* y, int n, hipStream_t s) {
+    hipLaunchKernelGGL(dense_apply_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, x, y, n);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_residual(const uint16_t* coefD, const float* z, const float* bD, float* res, int n, double* red,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
+    hipLaunchKernelGGL(residual_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, z, bD, res, n, red);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_loss_grad(const uint16_t* coefD, const float* res, const double* red, float* dOut, int n, hipStream_t s) {
+    hipLaunchKernelGGL(loss_grad_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, res, red, dOut, n);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t* in0, uint8_t* mask, float* scale,
+                        double* red, hipStream_t s) {
+    const long long nvox = (long long)n * n * n;
+    hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
+    hipLaunchKernelGGL(dense_norm_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, red);
+    hipLaunchKernelGGL(dense_input_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, in0, mask, scale, red);
+    FLIP2_LAUNCH_CHECK();
+}
+
+}  // namespace nn
+
+// ---------------------------------------------------------------------------
+// UNet
+// ---------------------------------------------------------------------------
+namespace {
+// SplitMix64 + Box-Muller for deterministic initialisation.
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform()); }
+};
+}  // namespace
+
+UNet::UNet(int n, hipStream_t stream) : n_(n), n1_(n / 2), n2_(n / 4), stream_(stream) {
+    FLIP2_REQUIRE(n >= 8 && n % 8 == 0, kErrArgument, "U-Net grid size must be a positive multiple of 8");
+    const int spec[10][3] = {{16, 16, 0}, {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2},
+                             {64, 64, 2}, {96, 32, 1}, {32, 32, 1}, {48, 16, 0}, {16, 16, 0}};
+    size_t off = 0;
+    for (auto& l : spec) {
+        UNetLayer L{l[0], l[1], l[2], off, 0};
+        off += (size_t)27 * l[0] * l[1];
+        L.b = off;
+        off += (size_t)l[1];
+        layers_.push_back(L);
+    }
+    headW_ = off;
+    off += 16;
+    headB_ = off;
+    off += 1;
+    numParams_ = off;
+    for (auto* a : {&params_, &grad_, &adamM_, &adamV_}) a->resizeAsync((uint32_t)numParams_, stream_);
+    paramsBf_.resizeAsync((uint32_t)numParams_, stream_);
+    const long long N0 = (long long)n_ * n_ * n_, N1 = (long long)n1_ * n1_ * n1_, N2 = (long long)n2_ * n2_ * n2_;
+    in0_.resizeAsync((uint32_t)(N0 * 16), stream_);
+    FLIP2_HIP(hipMemsetAsync(in0_.devPtr(), 0, in0_.bytes(), stream_));
+    a0_.resizeAsync((uint32_t)(N0 * 16), stream_);
+    b0_.resizeAsync((uint32_t)(N0 * 16), stream_);
+    p0_.resizeAsync((uint32_t)(N1 * 16), stream_);
+    a1_.resizeAsync((uint32_t)(N1 * 32), stream_);
+    b1_.resizeAsync((uint32_t)(N1 * 32), stream_);
+    p1_.resizeAsync((uint32_t)(N2 * 32), stream_);
+    a2_.resizeAsync((uint32_t)(N2 * 64), stream_);
+    b2_.resizeAsync((uint32_t)(N2 * 64), stream_);
+    u1_.resizeAsync((uint32_t)(N1 * 96), stream_);
+    c1_.resizeAsync((uint32_t)(N1 * 32), stream_);
+    d1_.resizeAsync((uint32_t)(N1 * 32), stream_);
+    u0_.resizeAsync((uint32_t)(N0 * 48), stream_);
+    c0_.resizeAsync((uint32_t)(N0 * 16), stream_);
+    d0_.resizeAsync((uint32_t)(N0 * 16), stream_);
+    mask_.resizeAsync((uint32_t)N0, stream_);
+    scale_.resizeAsync(4, stream_);
+    out_.resizeAsync((uint32_t)N0, stream_);
+    red_.resizeAsync(8, stream_);
+    initParams(1234);
+}
+
+UNet::~UNet() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    else (void)hipDeviceSynchronize();
+}
+
+void UNet::initParams(uint64_t seed) {
+    std::vector<float> p(numParams_, 0.f);
+    Rng rng{seed};
+    for (const auto& L : layers_) {
+        const double std = std::sqrt(2.0 / (27.0 * L.cin));
+        for (size_t i = 0; i < (size_t)27 * L.cin * L.cout; ++i) p[L.w + i] = (float)(std * rng.normal());
+    }
+    for (int c = 0; c < 16; ++c) p[headW_ + c] = (float)(0.1 * rng.normal());
+    setParams(p.data(), p.size());
+    step_ = 0;
+    FLIP2_HIP(hipMemsetAsync(adamM_.devPtr(), 0, adamM_.bytes(), stream_));
+    FLIP2_HIP(hipMemsetAsync(adamV_.devPtr(), 0, adamV_.bytes(), stream_));
+}
+
+void UNet::setParams(const float* host, size_t count) {
+    FLIP2_REQUIRE(count == numParams_, kErrRange, "parameter count mismatch");
+    params_.uploadFrom(host, (uint32_t)count, stream_);
+    nn::launch_to_bf16(params_.devPtr(), paramsBf_.devPtr(), (long long)numParams_, stream_);
+    FLIP2_HIP(hipStreamSynchronize(stream_));
+}
+
+void UNet::getParams(float* host, size_t count) {
+    FLIP2_REQUIRE(count >= numParams_, kErrRange, "buffer too small");
+    params_.downloadTo(host, (uint32_t)numParams_, stream_);
+    FLIP2_HIP(hipStreamSynchronize(stream_));
+}
+
+void UNet::conv(int layer, const uint16_t* X, uint16_t* Y, hipStream_t s) {
+    const UNetLayer& L = layers_[layer];
+    const int n = L.level == 0 ? n_ : (L.level == 1 ? n1_ : n2_);
+    nn::launch_conv3_fwd(X, paramsBf_.devPtr() + L.w, params_.devPtr() + L.b, Y, n, L.cin, L.cout, true, s);
+}
+
+void UNet::forward(hipStream_t s) {
+    conv(0, in0_.devPtr(), a0_.devPtr(), s);
+    conv(1, a0_.devPtr(), b0_.devPtr(), s);
+    nn::launch_avgpool(b0_.devPtr(), p0_.devPtr(), n_, 16, s);
+    conv(2, p0_.devPtr(), a1_.devPtr(), s);
+    conv(3, a1_.devPtr(), b1_.devPtr(), s);
+    nn::launch_avgpool(b1_.devPtr(), p1_.devPtr(), n1_, 32, s);
+    conv(4, p1_.devPtr(), a2_.devPtr(), s);
+    conv(5, a2_.devPtr(), b2_.devPtr(), s);
+    nn::launch_upcat(b2_.devPtr(), b1_.devPtr(), u1_.devPtr(), n1_, 64, 32, s);
+    conv(6, u1_.devPtr(), c1_.devPtr(), s);
+    conv(7, c1_.devPtr(), d1_.devPtr(), s);
+    nn::launch_upcat(d1_.devPtr(), b0_.devPtr(), u0_.devPtr(), n_, 32, 16, s);
+    conv(8, u0_.devPtr(), c0_.devPtr(), s);
+    conv(9, c0_.devPtr(), d0_.devPtr(), s);
+    nn::launch_head_fwd(d0_.devPtr(), params_.devPtr() + headW_, params_.devPtr() + headB_, mask_.devPtr(),
+                        scale_.devPtr(), out_.devPtr(), (long long)n_ * n_ * n_, 16, s);
+}
+
+// dY: gradient w.r.t. this layer's (post-ReLU) output Y; turned into the
+// pre-activation gradient in place; weight/bias gradients accumulated; dX
+// (if not null) receives the gradient w.r.t. the layer input X.
+void UNet::convBackward(int layer, float* dY, const uint16_t* Y, const uint16_t* X, float* dX, hipStream_t s) {
+    const UNetLayer& L = layers_[layer];
+    const int n = L.level == 0 ? n_ : (L.level == 1 ? n1_ : n2_);
+    const long long nvox = (long long)n * n * n;
+    nn::launch_relu_mask(dY, Y, nvox * L.cout, s);
+    nn::launch_conv3_wgrad(dY, X, grad_.devPtr() + L.w, grad_.devPtr() + L.b, n, L.cin, L.cout, s);
+    if (dX) nn::launch_conv3_dgrad(dY, params_.devPtr() + L.w, dX, n, L.cin, L.cout, s);
+}
+
+void UNet::backward(const float* dOut, hipStream_t s) {
+    const long long N0 = (long long)n_ * n_ * n_, N1 = (long long)n1_ * n1_ * n1_, N2 = (long long)n2_ * n2_ * n2_;
+    auto ensure = [&](DeviceVec<float>& v, long long count) {
+        if (v.size() < count) v.resizeAsync((uint32_t)count, s);
+    };
+    ensure(gd0_, N0 * 16); ensure(gc0_, N0 * 16); ensure(gu0_, N0 * 48); ensure(gb0_, N0 * 16); ensure(ga0_, N0 * 16);
+    ensure(gd1_, N1 * 32); ensure(gc1_, N1 * 32); ensure(gu1_, N1 * 96); ensure(gb1_, N1 * 32); ensure(ga1_, N1 * 32);
+    ensure(gp0_, N1 * 16); ensure(gb2_, N2 * 64); ensure(ga2_, N2 * 64); ensure(gp1_, N2 * 32);
+
+    nn::launch_head_bwd(dOut, d0_.devPtr(), params_.devPtr() + headW_, mask_.devPtr(), scale_.devPtr(), gd0_.devPtr(),
+                        grad_.devPtr() + headW_, grad_.devPtr() + headB_, N0, 16, s);
+    convBackward(9, gd0_.devPtr(), d0_.devPtr(), c0_.devPtr(), gc0_.devPtr(), s);
+    convBackward(8, gc0_.devPtr(), c0_.devPtr(), u0_.devPtr(), gu0_.devPtr(), s);
+    nn::launch_zero_f32(gd1_.devPtr(), N1 * 32, s);
+    nn::launch_zero_f32(gb0_.devPtr(), N0 * 16, s);
+    nn::launch_upcat_bwd(gu0_.devPtr(), gd1_.devPtr(), gb0_.devPtr(), n_, 32, 16, s);
+    convBackward(7, gd1_.devPtr(), d1_.devPtr(), c1_.devPtr(), gc1_.devPtr(), s);
+    convBackward(6, gc1_.devPtr(), c1_.devPtr(), u1_.devPtr(), gu1_.devPtr(), s);
+    nn::launch_zero_f32(gb2_.devPtr(), N2 * 64, s);
+    nn::launch_zero_f32(gb1_.devPtr(), N1 * 32, s);
+    nn::launch_upcat_bwd(gu1_.devPtr(), gb2_.devPtr(), gb1_.devPtr(), n1_, 64, 32, s);
+    convBackward(5, gb2_.devPtr(), b2_.devPtr(), a2_.devPtr(), ga2_.devPtr(), s);
+    convBackward(4, ga2_.devPtr(), a2_.devPtr(), p1_.devPtr(), gp1_.devPtr(), s);
+    nn::launch_avgpool_bwd(gp1_.devPtr(), gb1_.devPtr(), n1_, 32, s);
+    convBackward(3, gb1_.devPtr(), b1_.devPtr(), a1_.devPtr(), ga1_.devPtr(), s);
+    convBackward(2, ga1_.devPtr(), a1_.devPtr(), p0_.devPtr(), gp0_.devPtr(), s);
+    nn::launch_avgpool_bwd(gp0_.devPtr(), gb0_.devPtr(), n_, 16, s);
+    convBackward(1, gb0_.devPtr(), b0_.devPtr(), a0_.devPtr(), ga0_.devPtr(), s);
+    convBackward(0, ga0_.devPtr(), a0_.devPtr(), in0_.devPtr(), nullptr, s);
+}
+
+void UNet::zeroGrad(hipStream_t s) { nn::launch_zero_f32(grad_.devPtr(), (long long)numParams_, s); }
+
+void UNet::adamStep(float lr, hipStream_t s) {
+    ++step_;
+    nn::launch_adam(params_.devPtr(), grad_.devPtr(), adamM_.devPtr(), adamV_.devPtr(), paramsBf_.devPtr(),
+                    (int)numParams_, lr, 0.9f, 0.999f, 1e-8f, step_, s);
+}
+
+void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
+                       float* z, int gridN, hipStream_t s) {
+    FLIP2_REQUIRE(gridN == n_, kErrArgument, "U-Net grid size differs from the simulation grid");
+    nn::launch_scatter_input(numActive, activeBricks, cellType, r, n_, in0_.devPtr(), mask_.devPtr(), scale_.devPtr(),
+                             red_.devPtr(), s);
+    forward(s);
+    nn::launch_gather_output(numActive, activeBricks, cellType, out_.devPtr(), n_, z, s);
+}
+
+}  // namespace flip2

@@ -1,0 +1,35 @@
+// Kernels of the learned preconditioner's U-Net (unet_kernels.hip).
+// Activations are BF16 channels-last ([z][y][x][C]); gradients FP32.
+#pragma once
+
+#include "common.hpp"
+
+namespace flip2 {
+namespace nn {
+
+// Y = act(conv3x3x3(X, W) + bias), MFMA BF16 with FP32 accumulation.
+// W: [27][cout][cin] BF16 (tap = (dz*3 + dy)*3 + dx, offsets d-1).
+void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin, int cout,
+                      bool relu, hipStream_t s);
+// dX = conv3x3x3^T(dP) with FP32 weights (same layout).
+void launch_conv3_dgrad(const float* dP, const float* W32, float* dX, int n, int cin, int cout, hipStream_t s);
+// dW += sum_v dP (x) X(v + tap), dB += sum_v dP  (FP32 atomics).
+void launch_conv3_wgrad(const float* dP, const uint16_t* X, float* dW, float* dB, int n, int cin, int cout,
+                        hipStream_t s);
+void launch_relu_mask(float* g, const uint16_t* act, long long count, hipStream_t s);
+void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s);
+void launch_avgpool_bwd(const float* dP, float* dX, int n, int C, hipStream_t s);  // dX += ...
+void launch_upcat(const uint16_t* A, const uint16_t* B, uint16_t* U, int n, int Ca, int Cb, hipStream_t s);
+void launch_upcat_bwd(const float* dU, float* dA, float* dB, int n, int Ca, int Cb, hipStream_t s);  // +=
+void launch_head_fwd(const uint16_t* X, const float* w, const float* b, const uint8_t* mask, const float* scale,
+                     float* out, long long nvox, int C, hipStream_t s);
+void launch_head_bwd(const float* dOut, const uint16_t* X, const float* w, const uint8_t* mask, const float* scale,
+                     float* dX, float* dw, float* db, long long nvox, int C, hipStream_t s);
+void launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wbf, int count, float lr, float b1, float b2,
+                 float eps, int t, hipStream_t s);
+void launch_to_bf16(const float* a, uint16_t* b, long long count, hipStream_t s);
+void launch_bf16_to_f32(const uint16_t* a, float* b, long long count, hipStream_t s);
+void launch_zero_f32(float* a, long long count, hipStream_t s);
+
+}  // namespace nn
+}  // namespace flip2

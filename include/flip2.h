@@ -265,6 +265,44 @@ int flip2_fluid_last_solve(flip2_fluid* f, flip2_solve_stats* stats);
  * "cellType","coef","b","p". */
 int flip2_fluid_get_array(flip2_fluid* f, const char* name, void* host, size_t capacity, size_t* count);
 
+/* ---------------------------------------------------------------------- */
+/* Learned preconditioner: 3-level 3D U-Net (16/32/64 channels, 3^3       */
+/* convolutions, BF16 weights/activations, FP32 accumulation on MFMA) on  */
+/* the dense n^3 grid of a fluid simulation; trained on the GPU with the  */
+/* library's own forward/backward kernels and Adam on (b, p) pairs that   */
+/* the solver's PCG solves produce; used inside flexible CG.  No          */
+/* counterpart in the reference (its solve is a brick-local SOR).         */
+/* ---------------------------------------------------------------------- */
+typedef struct flip2_unet flip2_unet;
+int flip2_unet_create(int n, flip2_unet** out);
+int flip2_unet_destroy(flip2_unet* net);
+int flip2_unet_num_params(flip2_unet* net, size_t* count);
+int flip2_unet_init(flip2_unet* net, uint64_t seed);
+/* FP32 master parameters: per conv layer W[27][cout][cin] then b[cout], in
+ * network order, then the 1x1 head w[16], b[1]. */
+int flip2_unet_set_params(flip2_unet* net, const float* host, size_t count);
+int flip2_unet_get_params(flip2_unet* net, float* host, size_t count);
+/* z = net(b) for a dense system (host arrays of n^3: b, operator
+ * coefficients u16 diag | fluid-neighbour mask << 3 in [z][y][x] order). */
+int flip2_unet_apply_dense(flip2_unet* net, const float* b, const uint16_t* coef, float* z);
+/* Loss ||A z - b||^2 / ||b||^2 with z = net(b) and its parameter gradient
+ * (host array of num_params floats, may be NULL). */
+int flip2_unet_loss_grad(flip2_unet* net, const float* b, const uint16_t* coef, double* loss, float* grad);
+/* Training data: the current system and solution of a fluid simulation
+ * (after its "solve" stage), kept on the device. */
+int flip2_unet_add_frame(flip2_unet* net, flip2_fluid* f);
+int flip2_unet_num_frames(flip2_unet* net, int* count);
+int flip2_unet_clear_frames(flip2_unet* net);
+/* `epochs` passes of Adam over the captured frames (frames [first, last)),
+ * learning rate lr; *mean_loss = mean loss of the last epoch. */
+int flip2_unet_train(flip2_unet* net, int epochs, float lr, int first, int last, uint64_t seed, double* mean_loss);
+/* Mean loss over frames [first, last) without training. */
+int flip2_unet_eval(flip2_unet* net, int first, int last, double* mean_loss);
+/* Use the network as the FLIP2_PREC_LEARNED preconditioner of a fluid
+ * simulation (NULL detaches); fallback_iterations > 0 reruns a solve with
+ * Jacobi-PCG when flexible CG needs more iterations than that. */
+int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iterations);
+
 #ifdef __cplusplus
 }
 #endif

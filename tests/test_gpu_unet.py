@@ -1,0 +1,111 @@
+"""GPU checks of the learned preconditioner's U-Net.
+
+Forward pass (BF16 MFMA convolutions, FP32 accumulation) against the FP64
+oracle that rounds every stored activation and weight to bfloat16 as the HIP
+path does: 1e-2 normwise relative (bf16 keeps 8 significant bits, and
+summation order flips an occasional last bit of an activation).  Parameter
+gradients (FP32 backward kernels on the bf16 activations) against the
+oracle's FP64 gradient of the same bf16-rounded forward: 2e-2 normwise
+relative per layer.  Training: Adam on the library's own kernels lowers the
+loss, and the trained network inside flexible CG reaches the 1e-6 FP64
+residual.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import unet_oracle as uo
+from tests.unet_cases import synthetic_system
+from flip2_amd import scenes
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def biased_params(net, seed):
+    p = net.params().astype(np.float64)
+    layout, hw, hb, _ = uo.param_layout()
+    rng = np.random.default_rng(seed)
+    for w, bo, cin, cout, _ in layout:
+        p[bo:bo + cout] = rng.normal(0.0, 0.05, cout)
+    p[hb] = 0.01
+    net.set_params(p.astype(np.float32))
+    return net.params()
+
+
+@pytest.mark.parametrize("n", [8, 16, 24])
+def test_forward_matches_oracle(flip2, n):
+    net = flip2.UNet(n, seed=5)
+    p = biased_params(net, 1 + n)
+    b, coef = synthetic_system(n, 4 + n)
+    z = net.apply_dense(b, coef).reshape(n, n, n)
+    zo = uo.Net(n, p, bf16=True).forward(b, coef)
+    assert rel(z, zo) <= 1e-2, rel(z, zo)
+    assert np.all(z[coef == 0] == 0.0)
+    # and within bf16 precision of the exact FP64 network
+    z64 = uo.Net(n, p, bf16=False).forward(b, coef)
+    assert rel(z, z64) <= 5e-2
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_gradient_matches_oracle(flip2, n):
+    net = flip2.UNet(n, seed=7)
+    p = biased_params(net, 30 + n)
+    b, coef = synthetic_system(n, 9 + n)
+    loss, g = net.loss_grad(b, coef)
+    Lo, go = uo.Net(n, p, bf16=True).loss_grad(b, coef)
+    assert abs(loss - Lo) <= 1e-2 * Lo, (loss, Lo)
+    layout, hw, hb, total = uo.param_layout()
+    for i, (w, bo, cin, cout, _) in enumerate(layout):
+        assert rel(g[w:bo], go[w:bo]) <= 2e-2, (i, "W", rel(g[w:bo], go[w:bo]))
+        assert rel(g[bo:bo + cout], go[bo:bo + cout]) <= 2e-2, (i, "b", rel(g[bo:bo + cout], go[bo:bo + cout]))
+    assert rel(g[hw:hw + 16], go[hw:hw + 16]) <= 2e-2
+    assert abs(g[hb] - go[hb]) <= 2e-2 * max(abs(go[hb]), 1e-6)
+
+
+def test_loss_grad_is_repeatable(flip2):
+    n = 16
+    net = flip2.UNet(n, seed=3)
+    b, coef = synthetic_system(n, 21)
+    l1, g1 = net.loss_grad(b, coef)
+    l2, g2 = net.loss_grad(b, coef)
+    assert l1 == l2
+    # weight gradients are FP32 atomics: order may differ in the last bits
+    assert rel(g1, g2.astype(np.float64)) <= 1e-5
+
+
+def capture_frames(flip2, sim, net, steps):
+    for _ in range(steps):
+        for st in ("transfer", "prepare", "system", "solve"):
+            sim.stage(st)
+        assert sim.last_solve()["converged"]
+        net.add_frame(sim)
+        for st in ("project", "g2p", "advect"):
+            sim.stage(st)
+
+
+def test_training_and_learned_fcg(flip2):
+    n = 32
+    x, y, z = scenes.dam_break(n, seed=1)
+    zero = np.zeros_like(x)
+    grid = (0.0, 0.0, 0.0, n, n, n, 1.0 / n)
+    sim = flip2.FluidSim(x.size, grid, preconditioner="jacobi")
+    sim.set_particles(x, y, z, zero, zero, zero)
+    net = flip2.UNet(n, seed=2)
+    capture_frames(flip2, sim, net, 24)
+    assert net.num_frames() == 24
+    before = net.eval(20, 24)
+    net.train(60, 1e-3, 0, 20, seed=1)
+    after = net.eval(20, 24)
+    assert after < 0.5 * before, (before, after)
+    # next system: learned FCG vs Jacobi-PCG on the same right-hand side
+    for st in ("transfer", "prepare", "system"):
+        sim.stage(st)
+    sim.set_unet(net)
+    jac = sim.solve_with("jacobi")
+    lrn = sim.solve_with("learned")
+    assert jac["converged"] and lrn["converged"], (jac, lrn)
+    assert lrn["rel_residual"] <= 1e-6
+    assert lrn["iterations"] < jac["iterations"], (lrn, jac)
