@@ -173,6 +173,7 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_set_particles": [_vp, _u32] + [_vp] * 6,
         "flip2_fluid_get_particles": [_vp] * 7,
         "flip2_fluid_set_solid": [_vp, _vp],
+        "flip2_fluid_set_solid_box": [_vp, _vp, _vp, _vp],
         "flip2_fluid_step": [_vp],
         "flip2_fluid_stage": [_vp, ctypes.c_char_p],
         "flip2_fluid_solve_with": [_vp, ctypes.c_int, ctypes.POINTER(SolveStatsC)],
@@ -449,6 +450,14 @@ class FluidSim:
         else:
             m = np.ascontiguousarray(mask, dtype=np.uint8)
             check(lib().flip2_fluid_set_solid(self._h, _ptr(m)))
+
+    def set_solid_box(self, lo=None, hi=None, vel=(0.0, 0.0, 0.0)):
+        """Moving solid box [lo, hi) with face velocity vel; lo=None removes it."""
+        if lo is None:
+            check(lib().flip2_fluid_set_solid_box(self._h, None, None, None))
+            return
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (lo, hi, vel)]
+        check(lib().flip2_fluid_set_solid_box(self._h, *(_ptr(v) for v in a)))
 
     def step(self):
         check(lib().flip2_fluid_step(self._h))

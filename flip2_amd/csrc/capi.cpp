@@ -605,6 +605,12 @@ int flip2_fluid_set_solid(flip2_fluid* f, const uint8_t* mask) {
         f->impl.setSolidMask(mask);
     });
 }
+int flip2_fluid_set_solid_box(flip2_fluid* f, const float* lo, const float* hi, const float* vel) {
+    return guarded([&] {
+        NEED(f);
+        f->impl.setSolidBox(lo, hi, vel);
+    });
+}
 int flip2_fluid_step(flip2_fluid* f) {
     return guarded([&] {
         NEED(f);

@@ -88,12 +88,35 @@ void FluidSim::setSolidMask(const uint8_t* mask) {
                 tiled[t] = v ? 1 : 0;
                 any |= v != 0;
             }
-    hasSolid_ = any;
+    hasStatic_ = any;
     if (any) {
-        solid_.resizeAsync(numCells_, stream_);
-        solid_.uploadFrom(tiled.data(), numCells_, stream_);
+        solidStatic_.resizeAsync(numCells_, stream_);
+        solidStatic_.uploadFrom(tiled.data(), numCells_, stream_);
     }
+    rebuildSolid();
     synchronize();
+}
+
+void FluidSim::setSolidBox(const float* lo, const float* hi, const float* vel) {
+    box_ = ns::SolidBox{};
+    if (lo) {
+        FLIP2_REQUIRE(hi != nullptr, kErrArgument, "solid box needs both corners");
+        for (int d = 0; d < 3; ++d) {
+            box_.lo[d] = lo[d];
+            box_.hi[d] = hi[d];
+            box_.vel[d] = vel ? vel[d] : 0.f;
+        }
+        box_.active = 1;
+    }
+    rebuildSolid();
+}
+
+void FluidSim::rebuildSolid() {
+    hasSolid_ = hasStatic_ || box_.active;
+    if (!hasSolid_) return;
+    if (solid_.size() < numCells_) solid_.resizeAsync(numCells_, stream_);
+    ns::launch_rasterize_solid(hasStatic_ ? solidStatic_.devPtr() : nullptr, box_, cfg_.grid, solid_.devPtr(),
+                               stream_);
 }
 
 void FluidSim::ensureSlots(uint32_t numActive) {
@@ -129,7 +152,7 @@ void FluidSim::transfer() {
 
 void FluidSim::prepareFaces() {
     ns::launch_prepare_faces(U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
-                             hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.gravity * cfg_.dt, stream_);
+                             hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.gravity * cfg_.dt, box_, stream_);
 }
 
 void FluidSim::buildSystem() {
@@ -286,7 +309,7 @@ void FluidSim::gridToParticles() {
 void FluidSim::advect() {
     ns::launch_advect_rk2(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(),
                           hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.dt, cfg_.margin * cfg_.grid.cellSize,
-                          stream_);
+                          box_, stream_);
 }
 
 void FluidSim::step() {

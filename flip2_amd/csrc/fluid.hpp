@@ -62,6 +62,9 @@ public:
     void getParticles(float* x, float* y, float* z, float* vx, float* vy, float* vz);
     // Static solid cells, host array of n^3 bytes indexed x + y*nx + z*nx*ny.
     void setSolidMask(const uint8_t* mask);
+    // A moving solid box (cells whose centre lies in [lo, hi)), its faces
+    // carrying velocity vel; replaces the previous box.  Null lo removes it.
+    void setSolidBox(const float* lo, const float* hi, const float* vel);
     void setConfig(const FluidConfig& cfg);
     const FluidConfig& config() const { return cfg_; }
     const float4* recordsDev() const { return rec_.devPtr(); }
@@ -115,8 +118,10 @@ private:
     std::unique_ptr<P2GPipeline> pipe_;
     DeviceVec<float4> rec_;  // particle records (2 x float4 per particle)
     DeviceVec<float> U_, V_, W_, U0_, V0_, W0_;
-    DeviceVec<uint8_t> solid_;
-    bool hasSolid_ = false;
+    DeviceVec<uint8_t> solid_, solidStatic_;  // combined mask, static part
+    bool hasSolid_ = false, hasStatic_ = false;
+    ns::SolidBox box_{};
+    void rebuildSolid();
     // active bricks
     DeviceVec<uint32_t> brickFlag_, brickExcl_, activeBricks_;
     DeviceVec<int32_t> brickSlot_, nbrSlot_;

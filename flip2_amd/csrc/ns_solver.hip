@@ -645,12 +645,35 @@ __device__ inline bool cell_solid(int i, int j, int k, Grid g, const uint8_t* so
     return solid && solid[tiled((uint32_t)i, (uint32_t)j, (uint32_t)k, nbX, nbY)];
 }
 
-// Faces touching a solid cell or the domain boundary take the solid velocity
-// (0); the others keep the P2G value, v faces get + g dt.  The result is
-// also saved as the pre-projection field for the FLIP update.
+__device__ inline bool in_box(int i, int j, int k, Grid g, const SolidBox& box) {
+    if (!box.active) return false;
+    const float c[3] = {g.negX + ((float)i + 0.5f) * g.cellSize, g.negY + ((float)j + 0.5f) * g.cellSize,
+                        g.negZ + ((float)k + 0.5f) * g.cellSize};
+    return c[0] >= box.lo[0] && c[0] < box.hi[0] && c[1] >= box.lo[1] && c[1] < box.hi[1] && c[2] >= box.lo[2] &&
+           c[2] < box.hi[2];
+}
+
+__device__ inline bool in_domain(int i, int j, int k, Grid g) {
+    return i >= 0 && j >= 0 && k >= 0 && i < (int)g.sizeX && j < (int)g.sizeY && k < (int)g.sizeZ;
+}
+
+__global__ void rasterize_solid_kernel(const uint8_t* __restrict__ staticMask, SolidBox box, Grid g,
+                                       uint8_t* __restrict__ solid, uint32_t numCells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= numCells) return;
+    int gc[3];
+    cell_coords(c >> 9, c & 511u, g.sizeX >> 3, g.sizeY >> 3, gc);
+    solid[c] = ((staticMask && staticMask[c]) || in_box(gc[0], gc[1], gc[2], g, box)) ? 1 : 0;
+}
+
+// Faces touching the domain boundary take velocity 0, faces touching a solid
+// cell the solid's velocity (the box velocity for box cells, else 0); the
+// others keep the P2G value, v faces get + g dt.  The result is also saved
+// as the pre-projection field for the FLIP update.
 __global__ void prepare_faces_kernel(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
                                      float* __restrict__ U0, float* __restrict__ V0, float* __restrict__ W0,
-                                     const uint8_t* __restrict__ solid, Grid g, float gdt, uint32_t numCells) {
+                                     const uint8_t* __restrict__ solid, Grid g, float gdt, SolidBox box,
+                                     uint32_t numCells) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= numCells) return;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
@@ -663,8 +686,13 @@ __global__ void prepare_faces_kernel(float* __restrict__ U, float* __restrict__ 
     for (int axis = 0; axis < 3; ++axis) {
         int m[3] = {gc[0], gc[1], gc[2]};
         m[axis] -= 1;
-        const bool wall = here || cell_solid(m[0], m[1], m[2], g, solid, nbX, nbY);
-        float v = wall ? 0.f : F[axis][c];
+        const bool nbSolid = cell_solid(m[0], m[1], m[2], g, solid, nbX, nbY);
+        const bool wall = here || nbSolid;
+        float vs = 0.f;  // solid velocity of this face
+        if (wall && in_domain(m[0], m[1], m[2], g) &&
+            ((here && in_box(gc[0], gc[1], gc[2], g, box)) || (nbSolid && in_box(m[0], m[1], m[2], g, box))))
+            vs = box.vel[axis];
+        float v = wall ? vs : F[axis][c];
         F0[axis][c] = v;
         if (axis == 1 && !wall) v += gdt;
         F[axis][c] = v;
@@ -767,11 +795,12 @@ __global__ void g2p_flip_kernel(uint32_t n, float4* __restrict__ rec, const floa
 }
 
 // Midpoint (RK2) advection through the grid velocity, then clamping into
-// [lo, hi] per axis and out of solid cells of the static mask (particles in
-// a solid cell return to their start point).
+// [lo, hi] per axis and out of solid cells: a particle that ends inside the
+// solid box is pushed out through the box face nearest to it, one in another
+// solid cell returns to its start point.
 __global__ void advect_rk2_kernel(uint32_t n, float4* __restrict__ rec, const float* __restrict__ U,
                                   const float* __restrict__ V, const float* __restrict__ W,
-                                  const uint8_t* __restrict__ solid, Grid g, float dt, float margin) {
+                                  const uint8_t* __restrict__ solid, Grid g, float dt, float margin, SolidBox box) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
@@ -794,10 +823,22 @@ __global__ void advect_rk2_kernel(uint32_t n, float4* __restrict__ rec, const fl
         const float u2 = interp(F[c], c, q, g, nbX, nbY);
         xn[c] = fminf(fmaxf(fmaf(dt, u2, x0[c]), lo[c]), hi[c]);
     }
+    if (box.active && xn[0] >= box.lo[0] && xn[0] < box.hi[0] && xn[1] >= box.lo[1] && xn[1] < box.hi[1] &&
+        xn[2] >= box.lo[2] && xn[2] < box.hi[2]) {
+        int best = -1;
+        float bestD = 3.4e38f, target = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float dl = xn[c] - box.lo[c], dh = box.hi[c] - xn[c];
+            if (box.lo[c] - margin >= lo[c] && dl < bestD) { bestD = dl; best = c; target = box.lo[c] - margin; }
+            if (box.hi[c] + margin <= hi[c] && dh < bestD) { bestD = dh; best = c; target = box.hi[c] + margin; }
+        }
+        if (best >= 0) xn[best] = target;
+    }
     if (solid) {
         to_cell_units(xn[0], xn[1], xn[2], g, q);
         const int ci = (int)floor(q[0]), cj = (int)floor(q[1]), ck = (int)floor(q[2]);
-        if (cell_solid(ci, cj, ck, g, solid, nbX, nbY)) {
+        if (cell_solid(ci, cj, ck, g, solid, nbX, nbY) && !in_box(ci, cj, ck, g, box)) {
             xn[0] = x0[0]; xn[1] = x0[1]; xn[2] = x0[2];
         }
     }
@@ -919,10 +960,15 @@ void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, co
     FLIP2_LAUNCH_CHECK();
 }
 void launch_prepare_faces(float* U, float* V, float* W, float* U0, float* V0, float* W0, const uint8_t* solid, Grid g,
-                          float gdt, hipStream_t s) {
+                          float gdt, SolidBox box, hipStream_t s) {
     const uint32_t cells = g.sizeX * g.sizeY * g.sizeZ;
-    hipLaunchKernelGGL(prepare_faces_kernel, g1(cells, 256), dim3(256), 0, s, U, V, W, U0, V0, W0, solid, g, gdt,
+    hipLaunchKernelGGL(prepare_faces_kernel, g1(cells, 256), dim3(256), 0, s, U, V, W, U0, V0, W0, solid, g, gdt, box,
                        cells);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_rasterize_solid(const uint8_t* staticMask, SolidBox box, Grid g, uint8_t* solid, hipStream_t s) {
+    const uint32_t cells = g.sizeX * g.sizeY * g.sizeZ;
+    hipLaunchKernelGGL(rasterize_solid_kernel, g1(cells, 256), dim3(256), 0, s, staticMask, box, g, solid, cells);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
@@ -932,9 +978,9 @@ void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, co
     FLIP2_LAUNCH_CHECK();
 }
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
-                       Grid g, float dt, float margin, hipStream_t s) {
+                       Grid g, float dt, float margin, SolidBox box, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(advect_rk2_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin);
+    hipLaunchKernelGGL(advect_rk2_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
     FLIP2_LAUNCH_CHECK();
 }
 

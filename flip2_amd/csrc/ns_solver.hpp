@@ -18,6 +18,13 @@ struct Reduce {
     uint32_t* counter;
 };
 
+// A moving solid box (e.g. a wave paddle): cells whose centre lies in
+// [lo, hi) are solid and the faces they touch carry the box velocity.
+struct SolidBox {
+    float lo[3], hi[3], vel[3];
+    int active;
+};
+
 void launch_classify_bricks(const uint32_t* counts, const uint8_t* solid, uint32_t numBricks, uint32_t* flag,
                             hipStream_t s);
 void launch_build_slots(const uint32_t* flag, const uint32_t* excl, uint32_t numBricks, int32_t* brickSlot,
@@ -54,12 +61,14 @@ void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, co
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,
                             float* U, float* V, float* W, hipStream_t s);
 void launch_prepare_faces(float* U, float* V, float* W, float* U0, float* V0, float* W0, const uint8_t* solid, Grid g,
-                          float gdt, hipStream_t s);
+                          float gdt, SolidBox box, hipStream_t s);
+// solid = staticMask | box (brick-tiled bytes); staticMask may be null.
+void launch_rasterize_solid(const uint8_t* staticMask, SolidBox box, Grid g, uint8_t* solid, hipStream_t s);
 // rec: particle records, 2 x float4 per particle (x, y, z, vx), (vy, vz, 0, 0).
 void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
                      const float* V0, const float* W0, Grid g, float flip, hipStream_t s);
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
-                       Grid g, float dt, float margin, hipStream_t s);
+                       Grid g, float dt, float margin, SolidBox box, hipStream_t s);
 
 }  // namespace ns
 }  // namespace flip2

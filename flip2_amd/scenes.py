@@ -132,3 +132,80 @@ def rc_block_scene(n: int, h: float, count_cells: Tuple[int, int, int], seed: in
     else:
         vel = [np.zeros(x.size) for _ in range(3)]
     return pos, vel
+
+
+def _sphere_cells(n: int, center, radius: float) -> np.ndarray:
+    """Bool [i, j, k] mask of cells whose 8 jitter sub-cells may fall in the sphere
+    (cell centre within radius + half a cell diagonal)."""
+    h = 1.0 / n
+    lo = [max(0, int((c - radius) / h) - 1) for c in center]
+    hi = [min(n, int((c + radius) / h) + 2) for c in center]
+    m = np.zeros((n, n, n), dtype=bool)
+    ii, jj, kk = np.meshgrid(*(np.arange(l, u) for l, u in zip(lo, hi)), indexing="ij")
+    d2 = sum(((a + 0.5) * h - c) ** 2 for a, c in zip((ii, jj, kk), center))
+    m[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = d2 <= (radius + 0.87 * h) ** 2
+    return m
+
+
+def dam_break_sphere(n: int = 128, seed: int = 5, center=(0.6, 0.2, 0.5), radius: float = 0.1):
+    """Dam break (x < 0.4, y < 0.6) with a static solid sphere.  Returns the
+    particles and the solid mask (n^3 bytes indexed x + y*n + z*n*n, cells
+    whose centre lies inside the sphere)."""
+    x, y, z = dam_break(n, seed)
+    c = np.float32(center)
+    r2 = np.float32(radius * radius)
+    out = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 > r2
+    h = 1.0 / n
+    a = (np.arange(n) + 0.5) * h
+    zz, yy, xx = np.meshgrid(a, a, a, indexing="ij")
+    solid = ((xx - center[0]) ** 2 + (yy - center[1]) ** 2 + (zz - center[2]) ** 2 <= radius * radius)
+    return (x[out], y[out], z[out]), solid.astype(np.uint8).reshape(-1)
+
+
+def colliding_spheres(n: int = 512, seed: int = 4, radius: float = 0.15, speed: float = 2.0):
+    """Two fluid spheres at (0.3, .5, .5) moving +x and (0.7, .5, .5) moving -x."""
+    xs, ys, zs, us = [], [], [], []
+    h = 1.0 / n
+    for cx, u in ((0.3, speed), (0.7, -speed)):
+        center = (cx, 0.5, 0.5)
+        mask = _sphere_cells(n, center, radius)
+        c = np.float32(center)
+        r2 = np.float32(radius * radius)
+        x, y, z = jittered_cells((n, n, n), h, seed, cell_mask=mask,
+                                 keep=lambda x, y, z: (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r2)
+        xs.append(x); ys.append(y); zs.append(z)
+        us.append(np.full(x.size, np.float32(u)))
+    x, y, z, vx = (np.concatenate(a) for a in (xs, ys, zs, us))
+    zero = np.zeros_like(x)
+    return (x, y, z), (vx, zero, zero.copy())
+
+
+def wave_surface(x, z, seed: int, depth: float = 0.35, amp: float = 0.02, modes: int = 4):
+    """Seeded surface height depth + amp * sum_k a_k sin(2 pi (f_k x + g_k z) + phi_k)."""
+    u = uniform01(seed, 2, modes)
+    y = np.full(np.shape(x), depth, dtype=np.float64)
+    for k in range(modes):
+        a = 0.5 + 0.5 * float(u[k, 0])
+        f = 1 + int(3 * float(u[k, 1]))
+        g = 1 + int(3 * float(u[k, 2]))
+        phi = 2 * np.pi * float(u[k, 3])
+        y += amp * a * np.sin(2 * np.pi * (f * np.asarray(x, np.float64) + g * np.asarray(z, np.float64)) + phi) / modes
+    return y
+
+
+def wave_tank(n: int = 256, seed: int = 6, depth: float = 0.35, paddle_x: float = 0.05):
+    """Wave tank: fluid below a seeded perturbed surface around `depth`, right
+    of the paddle plane x = paddle_x (the paddle's rest position)."""
+    h = 1.0 / n
+    top = int(np.ceil((depth + 0.03) / h)) + 1
+    mask = np.zeros((n, n, n), dtype=bool)
+    mask[:, :min(n, top), :] = True
+    x, y, z = jittered_cells((n, n, n), h, seed, cell_mask=mask)
+    keep = (y < wave_surface(x, z, seed, depth).astype(np.float32)) & (x >= np.float32(paddle_x))
+    return x[keep], y[keep], z[keep]
+
+
+def paddle_position(t: float, rest: float = 0.05, amp: float = 0.05, freq: float = 1.0):
+    """Paddle plane x(t) = rest + amp sin(2 pi f t) and its velocity."""
+    w = 2 * np.pi * freq
+    return rest + amp * np.sin(w * t), amp * w * np.cos(w * t)

@@ -254,6 +254,12 @@ int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const 
 int flip2_fluid_get_particles(flip2_fluid* f, float* x, float* y, float* z, float* vx, float* vy, float* vz);
 /* Static solid cells: n^3 bytes indexed x + y*nx + z*nx*ny (NULL clears). */
 int flip2_fluid_set_solid(flip2_fluid* f, const uint8_t* mask);
+/* A moving solid box (e.g. a wave paddle): cells whose centre lies in
+ * [lo, hi) are solid in addition to the static mask, and the faces they touch
+ * carry velocity vel (may be NULL: at rest).  Call before each step that
+ * moves it; lo == NULL removes the box.  No counterpart in the reference
+ * (its solids are the low domain walls). */
+int flip2_fluid_set_solid_box(flip2_fluid* f, const float* lo, const float* hi, const float* vel);
 int flip2_fluid_step(flip2_fluid* f);
 /* One stage: "transfer", "prepare", "system", "solve", "project", "g2p", "advect". */
 int flip2_fluid_stage(flip2_fluid* f, const char* stage);
