@@ -190,7 +190,9 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_unet_apply_dense": [_vp, _vp, _vp, _vp],
         "flip2_unet_loss_grad": [_vp, _vp, _vp, ctypes.POINTER(_f64), _vp],
         "flip2_unet_add_frame": [_vp, _vp],
+        "flip2_unet_set_smoothing": [_vp, ctypes.c_float],
         "flip2_unet_num_frames": [_vp, ctypes.POINTER(ctypes.c_int)],
+        "flip2_unet_capture_residuals": [_vp, _vp, _vp, ctypes.c_int],
         "flip2_unet_clear_frames": [_vp],
         "flip2_unet_train": [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                              ctypes.POINTER(_f64)],
@@ -549,8 +551,18 @@ class UNet:
                                          _ptr(g) if grad else None))
         return loss.value, g
 
+    def set_smoothing(self, omega: float) -> None:
+        check(lib().flip2_unet_set_smoothing(self._h, float(omega)))
+
     def add_frame(self, sim: "FluidSim") -> None:
         check(lib().flip2_unet_add_frame(self._h, sim._h))
+
+    def capture_residuals(self, sim: Optional["FluidSim"], iterations=()) -> None:
+        """Add the residuals after the given inner iteration counts of every
+        following solve of `sim` as training frames (empty stops)."""
+        its = np.ascontiguousarray(iterations, dtype=np.uint32)
+        check(lib().flip2_unet_capture_residuals(self._h, sim._h, _ptr(its) if its.size else None, int(its.size)))
+        sim._capture_net = self if its.size else None
 
     def num_frames(self) -> int:
         c = ctypes.c_int()

@@ -28,30 +28,58 @@ struct flip2_fluid {
 
 struct flip2_unet {
     flip2::UNet net;
-    hipStream_t stream = nullptr;
-    struct Frame {
-        flip2::DeviceVec<float> b, x;
+    float omega = 2.0f / 3.0f;  // damped-Jacobi smoothing around the network (0: none)
+    struct Frame {  // a dense right-hand side (b or a solver residual) and its operator
+        flip2::DeviceVec<float> b;
         flip2::DeviceVec<uint16_t> coef;
     };
     std::vector<std::unique_ptr<Frame>> frames;
-    flip2::DeviceVec<float> res, dOut, bTmp;
+    Frame& newFrame(hipStream_t st = nullptr) {
+        const uint32_t nv = (uint32_t)n() * n() * n();
+        std::unique_ptr<Frame> fr(new Frame);
+        fr->b.resizeAsync(nv, st);
+        fr->coef.resizeAsync(nv, st);
+        FLIP2_HIP(hipMemsetAsync(fr->b.devPtr(), 0, fr->b.bytes(), st));
+        FLIP2_HIP(hipMemsetAsync(fr->coef.devPtr(), 0, fr->coef.bytes(), st));
+        frames.push_back(std::move(fr));
+        return *frames.back();
+    }
+    flip2::DeviceVec<float> res, dOut, bTmp, r1, zS, gS;
     flip2::DeviceVec<uint16_t> coefTmp;
     flip2::DeviceVec<double> red;
     explicit flip2_unet(int n) : net(n, nullptr) {
         const uint32_t nv = (uint32_t)n * n * n;
-        res.resizeAsync(nv, nullptr);
-        dOut.resizeAsync(nv, nullptr);
+        for (auto* v : {&res, &dOut, &r1, &zS, &gS}) v->resizeAsync(nv, nullptr);
         red.resizeAsync(8, nullptr);
     }
     int n() const { return net.gridSize(); }
-    // loss of one dense system; gradients accumulated when `grad`
+    // z = M(b) on the dense grid: the network alone or wrapped in smoothing.
+    const float* apply(const float* bD, const uint16_t* coefD) {
+        const int nn = n();
+        const float* in = bD;
+        if (omega > 0.f) {
+            flip2::nn::launch_dense_presmooth(coefD, bD, omega, r1.devPtr(), nn, nullptr);
+            in = r1.devPtr();
+        }
+        flip2::nn::launch_dense_input(in, coefD, nn, net.input(), net.mask(), net.scale(), red.devPtr() + 2, nullptr);
+        net.forward(nullptr);
+        if (omega <= 0.f) return net.output();
+        flip2::nn::launch_dense_postsmooth(coefD, bD, net.output(), omega, zS.devPtr(), nn, nullptr);
+        return zS.devPtr();
+    }
+    // loss ||A M(b) - b||^2 / ||b||^2 of one dense system; parameter
+    // gradients accumulated when `grad`
     double lossGrad(const float* bD, const uint16_t* coefD, bool grad) {
         const int nn = n();
-        flip2::nn::launch_dense_input(bD, coefD, nn, net.input(), net.mask(), net.scale(), red.devPtr() + 2, nullptr);
-        net.forward(nullptr);
-        flip2::nn::launch_residual(coefD, net.output(), bD, res.devPtr(), nn, red.devPtr(), nullptr);
+        const float* z = apply(bD, coefD);
+        flip2::nn::launch_residual(coefD, z, bD, res.devPtr(), nn, red.devPtr(), nullptr);
         if (grad) {
-            flip2::nn::launch_loss_grad(coefD, res.devPtr(), red.devPtr(), dOut.devPtr(), nn, nullptr);
+            if (omega > 0.f) {
+                flip2::nn::launch_loss_grad(coefD, res.devPtr(), red.devPtr(), gS.devPtr(), nn, nullptr);
+                flip2::nn::launch_dense_presmooth(coefD, gS.devPtr(), omega, dOut.devPtr(), nn, nullptr);
+            } else {
+                flip2::nn::launch_loss_grad(coefD, res.devPtr(), red.devPtr(), dOut.devPtr(), nn, nullptr);
+            }
             net.backward(dOut.devPtr(), nullptr);
         }
         double h[2];
@@ -582,7 +610,11 @@ int flip2_fluid_destroy(flip2_fluid* f) {
 int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg) {
     return guarded([&] {
         NEED(f); NEED(cfg);
-        f->impl.setConfig(to_config(*cfg));
+        // the learned-preconditioner settings belong to flip2_fluid_set_unet
+        flip2::FluidConfig c = to_config(*cfg);
+        c.learnedFallbackIterations = f->impl.config().learnedFallbackIterations;
+        c.learnedSmoothing = f->impl.config().learnedSmoothing;
+        f->impl.setConfig(c);
     });
 }
 int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const float* y, const float* z,
@@ -705,10 +737,8 @@ int flip2_unet_apply_dense(flip2_unet* net, const float* b, const uint16_t* coef
         NEED(net); NEED(b); NEED(coef); NEED(z);
         upload_dense(net, b, coef);
         const int nn = net->n();
-        flip2::nn::launch_dense_input(net->bTmp.devPtr(), net->coefTmp.devPtr(), nn, net->net.input(),
-                                      net->net.mask(), net->net.scale(), net->red.devPtr() + 2, nullptr);
-        net->net.forward(nullptr);
-        FLIP2_HIP(hipMemcpy(z, net->net.output(), (size_t)nn * nn * nn * sizeof(float), hipMemcpyDeviceToHost));
+        const float* zd = net->apply(net->bTmp.devPtr(), net->coefTmp.devPtr());
+        FLIP2_HIP(hipMemcpy(z, zd, (size_t)nn * nn * nn * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
 int flip2_unet_loss_grad(flip2_unet* net, const float* b, const uint16_t* coef, double* loss, float* grad) {
@@ -722,25 +752,42 @@ int flip2_unet_loss_grad(flip2_unet* net, const float* b, const uint16_t* coef, 
             FLIP2_HIP(hipMemcpy(grad, net->net.gradDev(), net->net.numParams() * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
+int flip2_unet_set_smoothing(flip2_unet* net, float omega) {
+    return guarded([&] {
+        NEED(net);
+        FLIP2_REQUIRE(omega >= 0.0f && omega < 1.0f, flip2::kErrArgument, "smoothing weight must lie in [0, 1)");
+        net->omega = omega;
+    });
+}
 int flip2_unet_add_frame(flip2_unet* net, flip2_fluid* f) {
     return guarded([&] {
         NEED(net); NEED(f);
         flip2::FluidSim& sim = f->impl;
         FLIP2_REQUIRE(sim.gridN() == net->n(), flip2::kErrArgument, "grid size differs from the network's");
         sim.synchronize();
-        const uint32_t nv = (uint32_t)net->n() * net->n() * net->n();
-        std::unique_ptr<flip2_unet::Frame> fr(new flip2_unet::Frame);
-        fr->b.resizeAsync(nv, nullptr);
-        fr->x.resizeAsync(nv, nullptr);
-        fr->coef.resizeAsync(nv, nullptr);
-        FLIP2_HIP(hipMemset(fr->b.devPtr(), 0, fr->b.bytes()));
-        FLIP2_HIP(hipMemset(fr->x.devPtr(), 0, fr->x.bytes()));
-        FLIP2_HIP(hipMemset(fr->coef.devPtr(), 0, fr->coef.bytes()));
-        flip2::nn::launch_dense_system(sim.numActive(), sim.activeBricksDev(), sim.coefDev(), sim.rhsDev(),
-                                       sim.solutionDev(), net->n(), fr->b.devPtr(), fr->x.devPtr(), fr->coef.devPtr(),
-                                       nullptr);
+        flip2_unet::Frame& fr = net->newFrame();
+        flip2::nn::launch_dense_system(sim.numActive(), sim.activeBricksDev(), sim.coefDev(), sim.rhsDev(), nullptr,
+                                       net->n(), fr.b.devPtr(), nullptr, fr.coef.devPtr(), nullptr);
         FLIP2_HIP(hipDeviceSynchronize());
-        net->frames.push_back(std::move(fr));
+    });
+}
+int flip2_unet_capture_residuals(flip2_unet* net, flip2_fluid* f, const uint32_t* iterations, int count) {
+    return guarded([&] {
+        NEED(net); NEED(f);
+        flip2::FluidSim& sim = f->impl;
+        if (count <= 0) {
+            sim.setResidualCapture({}, nullptr);
+            return;
+        }
+        NEED(iterations);
+        FLIP2_REQUIRE(sim.gridN() == net->n(), flip2::kErrArgument, "grid size differs from the network's");
+        std::vector<uint32_t> its(iterations, iterations + count);
+        flip2::FluidSim* sp = &sim;
+        sim.setResidualCapture(std::move(its), [net, sp](uint32_t na, const float* r, hipStream_t st) {
+            flip2_unet::Frame& fr = net->newFrame(st);
+            flip2::nn::launch_dense_vector(na, sp->activeBricksDev(), sp->coefDev(), r, net->n(), fr.b.devPtr(),
+                                           fr.coef.devPtr(), st);
+        });
     });
 }
 int flip2_unet_num_frames(flip2_unet* net, int* count) {
@@ -799,6 +846,7 @@ int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iter
         flip2::FluidSim& sim = f->impl;
         flip2::FluidConfig c = sim.config();
         c.learnedFallbackIterations = fallback_iterations;
+        c.learnedSmoothing = net ? net->omega : 0.0f;
         sim.setConfig(c);
         if (!net) {
             sim.setLearnedPreconditioner(nullptr);

@@ -127,9 +127,23 @@ void FluidSim::ensureSlots(uint32_t numActive) {
     cellType_.resizeAsync(cells, stream_);
     coef_.resizeAsync(cells, stream_);
     for (auto* a : {&b64_, &x64_, &r64_}) a->resizeAsync(cells, stream_);
-    for (auto* a : {&r_, &z_, &p_, &q_, &e_, &precon_}) a->resizeAsync(cells, stream_);
+    for (auto* a : {&r_, &z_, &p_, &q_, &e_, &precon_, &aux0_, &aux1_}) a->resizeAsync(cells, stream_);
     partials_.resizeAsync(cap * 2u + 64u, stream_);
     slotCapacity_ = cap;
+}
+
+// z = M(r) for the learned preconditioner: the network alone, or wrapped in
+// a damped-Jacobi pre-/post-smoothing step.
+void FluidSim::applyLearned(uint32_t na) {
+    const float w = cfg_.learnedSmoothing;
+    if (w > 0.f) {
+        ns::launch_presmooth(na, r_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), w, aux0_.devPtr(), stream_);
+        learned_(na, aux0_.devPtr(), aux1_.devPtr(), stream_);
+        ns::launch_postsmooth(na, r_.devPtr(), aux1_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), w, z_.devPtr(),
+                              stream_);
+    } else {
+        learned_(na, r_.devPtr(), z_.devPtr(), stream_);
+    }
 }
 
 double FluidSim::readScalar(const double* dev) {
@@ -229,6 +243,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
         const double ref = rel * st.bNorm;  // norm of the FP32 right-hand side of this refinement
         ns::launch_refine_start(na, r64_.devPtr(), r_.devPtr(), e_.devPtr(), coef_.devPtr(), reduce(),
                                 scal_.devPtr() + 1, stream_);
+        if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
         if (jac) {
             ns::launch_jacobi_init(na, r_.devPtr(), coef_.devPtr(), z_.devPtr(), p_.devPtr(), reduce(), H, stream_);
         } else {
@@ -236,7 +251,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                 ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), H,
                                      stream_);
             } else {
-                learned_(na, r_.devPtr(), z_.devPtr(), stream_);
+                applyLearned(na);
                 ns::launch_dot(na, r_.devPtr(), z_.devPtr(), reduce(), H, stream_);
             }
             ns::launch_direction(na, H, H, z_.devPtr(), p_.devPtr(), true, stream_);
@@ -257,7 +272,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                     ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), hn,
                                          stream_);
                 } else if (prec == kPrecLearned) {
-                    learned_(na, r_.devPtr(), z_.devPtr(), stream_);
+                    applyLearned(na);
                     ns::launch_dot(na, r_.devPtr(), z_.devPtr(), reduce(), hn, stream_);
                     ns::launch_dot(na, z_.devPtr(), q_.devPtr(), reduce(), hn + 3, stream_);
                 }
@@ -265,6 +280,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                     ns::launch_direction_flex(na, hn + 3, hk + 2, z_.devPtr(), p_.devPtr(), stream_);
                 else
                     ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
+                if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
             }
             readHistory(first + 1, batch, h.data());
             for (uint32_t k = 0; k < batch; ++k) {

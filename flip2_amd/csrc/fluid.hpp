@@ -32,6 +32,9 @@ struct FluidConfig {
     // Learned preconditioner: fall back to Jacobi-PCG when flexible CG needs
     // more iterations than this (0: no fallback).
     uint32_t learnedFallbackIterations = 0;
+    // Damped-Jacobi weight of the smoothing step before and after the
+    // learned correction (0: the network alone is the preconditioner).
+    float learnedSmoothing = 0.0f;
 };
 
 struct SolveStats {
@@ -49,6 +52,9 @@ struct SolveStats {
 // A preconditioner supplied from outside the solver (the learned one):
 // z = M^-1 r on the active-brick vectors, FP32.
 using PrecondFn = std::function<void(uint32_t numActive, const float* r, float* z, hipStream_t s)>;
+// Receives the residual r_k = b - A x_k (FP32, active-brick layout) of a
+// solve's first refinement after k inner iterations (training data).
+using CaptureFn = std::function<void(uint32_t numActive, const float* r, hipStream_t s)>;
 
 class FluidSim {
 public:
@@ -84,6 +90,12 @@ public:
     SolveStats solveWith(int preconditioner);
     SolveStats solveOnce(int preconditioner, uint32_t cap);
     void setLearnedPreconditioner(PrecondFn fn) { learned_ = std::move(fn); }
+    // Residuals after the listed inner iteration counts (0 = b) of every
+    // following solve go to fn; an empty list stops the capture.
+    void setResidualCapture(std::vector<uint32_t> iterations, CaptureFn fn) {
+        captureIts_ = std::move(iterations);
+        capture_ = std::move(fn);
+    }
 
     const SolveStats& lastSolve() const { return stats_; }
     uint32_t numParticles() const { return n_; }
@@ -128,13 +140,22 @@ private:
     DeviceVec<uint8_t> cellType_;
     DeviceVec<uint16_t> coef_;
     DeviceVec<double> b64_, x64_, r64_;
-    DeviceVec<float> r_, z_, p_, q_, e_, precon_;
+    DeviceVec<float> r_, z_, p_, q_, e_, precon_, aux0_, aux1_;
+    void applyLearned(uint32_t na);
     DeviceVec<double> partials_, hist_, scal_;
     DeviceVec<uint32_t> counter_, total_;
     DeviceVec<uint8_t> ws_;
     uint32_t slotCapacity_ = 0;
     SolveStats stats_;
     PrecondFn learned_;
+    CaptureFn capture_;
+    std::vector<uint32_t> captureIts_;
+    bool captureAt(uint32_t k) const {
+        if (!capture_) return false;
+        for (uint32_t v : captureIts_)
+            if (v == k) return true;
+        return false;
+    }
 };
 
 }  // namespace flip2

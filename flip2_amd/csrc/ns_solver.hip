@@ -262,6 +262,32 @@ __device__ inline float apply_A(uint32_t slot, uint32_t l, uint16_t cf, const fl
     return s;
 }
 
+// y = A x with x given by a functor of the linear (slot * 512 + local) index.
+template <class X>
+__device__ inline float apply_A_with(uint32_t slot, uint32_t l, uint16_t cf, const int32_t* nbrSlot, X x) {
+    if (!cf) return 0.f;
+    const uint32_t diag = cf & 7u, mask = cf >> 3;
+    float s = (float)diag * x(slot * 512u + l);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        if (!((mask >> d) & 1u)) continue;
+        const int axis = d >> 1;
+        const bool up = d & 1;
+        const uint32_t lc = axis == 0 ? (l & 7u) : axis == 1 ? ((l >> 3) & 7u) : (l >> 6);
+        const uint32_t stride = axis == 0 ? 1u : axis == 1 ? 8u : 64u;
+        uint32_t ns = slot, nl;
+        if (up) {
+            if (lc < 7u) nl = l + stride;
+            else { ns = (uint32_t)nbrSlot[slot * 6u + d]; nl = l - 7u * stride; }
+        } else {
+            if (lc > 0u) nl = l - stride;
+            else { ns = (uint32_t)nbrSlot[slot * 6u + d]; nl = l + 7u * stride; }
+        }
+        s -= x(ns * 512u + nl);
+    }
+    return s;
+}
+
 __device__ inline double apply_A64(uint32_t slot, uint32_t l, uint16_t cf, const double* x, const int32_t* nbrSlot) {
     if (!cf) return 0.0;
     const uint32_t diag = cf & 7u, mask = cf >> 3;
@@ -385,6 +411,44 @@ dot_kernel(const float* __restrict__ a, const float* __restrict__ b, double* __r
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) s += (double)a[base + l] * (double)b[base + l];
     const double t = block_sum(s, s_tmp);
     finish_reduction(&t, 1, partials, counter, out);
+}
+
+// Damped-Jacobi smoothing around an external (learned) correction:
+//   presmooth:  r1 = r - A (w D^-1 r)
+//   postsmooth: z  = x + w D^-1 (r - A x),  x = w D^-1 r + y
+__global__ void __launch_bounds__(kWG)
+presmooth_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
+                 float omega, float* __restrict__ r1) {
+    const uint32_t slot = blockIdx.x;
+    auto jac = [&](uint32_t i) {
+        const uint16_t c = coef[i];
+        return c ? omega * r[i] / (float)(c & 7u) : 0.f;
+    };
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+        const uint32_t v = slot * 512u + l;
+        const uint16_t cf = coef[v];
+        r1[v] = cf ? r[v] - apply_A_with(slot, l, cf, nbrSlot, jac) : 0.f;
+    }
+}
+
+__global__ void __launch_bounds__(kWG)
+postsmooth_kernel(const float* __restrict__ r, const float* __restrict__ y, const uint16_t* __restrict__ coef,
+                  const int32_t* __restrict__ nbrSlot, float omega, float* __restrict__ z) {
+    const uint32_t slot = blockIdx.x;
+    auto xf = [&](uint32_t i) {
+        const uint16_t c = coef[i];
+        return c ? omega * r[i] / (float)(c & 7u) + y[i] : 0.f;
+    };
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+        const uint32_t v = slot * 512u + l;
+        const uint16_t cf = coef[v];
+        if (!cf) {
+            z[v] = 0.f;
+            continue;
+        }
+        const float xv = xf(v);
+        z[v] = xv + omega * (r[v] - apply_A_with(slot, l, cf, nbrSlot, xf)) / (float)(cf & 7u);
+    }
 }
 
 // p = z + beta p, beta = rzNew / rzOld (p = z when first != 0).
@@ -914,6 +978,16 @@ void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, 
 void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
                       bool first, hipStream_t s) {
     hipLaunchKernelGGL(direction_kernel, dim3(numActive), dim3(kWG), 0, s, rzNew, rzOld, z, p, first ? 1 : 0);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot, float omega,
+                      float* r1, hipStream_t s) {
+    hipLaunchKernelGGL(presmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, nbrSlot, omega, r1);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const uint16_t* coef, const int32_t* nbrSlot,
+                       float omega, float* z, hipStream_t s) {
+    hipLaunchKernelGGL(postsmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, y, coef, nbrSlot, omega, z);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_direction_flex(uint32_t numActive, const double* zq, const double* pq, const float* z, float* p,

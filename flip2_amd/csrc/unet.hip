@@ -189,6 +189,63 @@ __global__ void dense_input_kernel(const float* __restrict__ bD, const uint16_t*
     mask[v] = f ? 1 : 0;
 }
 
+__device__ inline float dense_jac(const uint16_t* coefD, const float* r, float omega, long long v) {
+    const uint16_t c = coefD[v];
+    return c ? omega * r[v] / (float)(c & 7u) : 0.f;
+}
+
+// r1 = r - A (w D^-1 r) on fluid cells, 0 elsewhere
+__global__ void dense_presmooth_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ r, float omega,
+                                       float* __restrict__ r1, int n) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long long)n * n * n) return;
+    const uint16_t cf = coefD[v];
+    if (!cf) {
+        r1[v] = 0.f;
+        return;
+    }
+    const long long st[3] = {1, n, (long long)n * n};
+    float s = (float)(cf & 7u) * dense_jac(coefD, r, omega, v);
+    const uint32_t m = cf >> 3;
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+        if ((m >> d) & 1u) s -= dense_jac(coefD, r, omega, v + ((d & 1) ? st[d >> 1] : -st[d >> 1]));
+    r1[v] = r[v] - s;
+}
+
+// z = x + w D^-1 (r - A x), x = w D^-1 r + y
+__global__ void dense_postsmooth_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ r,
+                                        const float* __restrict__ y, float omega, float* __restrict__ z, int n) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long long)n * n * n) return;
+    const uint16_t cf = coefD[v];
+    if (!cf) {
+        z[v] = 0.f;
+        return;
+    }
+    const long long st[3] = {1, n, (long long)n * n};
+    auto xf = [&](long long u) { return coefD[u] ? dense_jac(coefD, r, omega, u) + y[u] : 0.f; };
+    const float xv = xf(v);
+    float s = (float)(cf & 7u) * xv;
+    const uint32_t m = cf >> 3;
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+        if ((m >> d) & 1u) s -= xf(v + ((d & 1) ? st[d >> 1] : -st[d >> 1]));
+    z[v] = xv + omega * (r[v] - s) / (float)(cf & 7u);
+}
+
+void launch_dense_presmooth(const uint16_t* coefD, const float* r, float omega, float* r1, int n, hipStream_t s) {
+    hipLaunchKernelGGL(dense_presmooth_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, r, omega, r1, n);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_dense_postsmooth(const uint16_t* coefD, const float* r, const float* y, float omega, float* z, int n,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(dense_postsmooth_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, r, y, omega, z,
+                       n);
+    FLIP2_LAUNCH_CHECK();
+}
+
 void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
                           int n, uint16_t* in0, uint8_t* mask, float* scale, double* red, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
@@ -206,6 +263,28 @@ void launch_gather_output(uint32_t numActive, const uint32_t* activeBricks, cons
     const uint32_t cells = numActive * 512u;
     if (!cells) return;
     hipLaunchKernelGGL(gather_output_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, cellType, out, cells, n, z);
+    FLIP2_LAUNCH_CHECK();
+}
+
+__global__ void dense_vector_kernel(const uint32_t* __restrict__ activeBricks, const uint16_t* __restrict__ coef,
+                                    const float* __restrict__ v, uint32_t cells, int n, float* __restrict__ vD,
+                                    uint16_t* __restrict__ coefD) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const uint16_t cf = coef[i];
+    if (!cf) return;
+    int xx, yy, zz;
+    dense_coords(activeBricks[i >> 9], i & 511u, n, xx, yy, zz);
+    const long long d = ((long long)zz * n + yy) * n + xx;
+    vD[d] = v[i];
+    coefD[d] = cf;
+}
+
+void launch_dense_vector(uint32_t numActive, const uint32_t* activeBricks, const uint16_t* coef, const float* v, int n,
+                         float* vD, uint16_t* coefD, hipStream_t s) {
+    const uint32_t cells = numActive * 512u;
+    if (!cells) return;
+    hipLaunchKernelGGL(dense_vector_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, coef, v, cells, n, vD, coefD);
     FLIP2_LAUNCH_CHECK();
 }
 

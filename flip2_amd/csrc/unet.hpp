@@ -90,6 +90,10 @@ void launch_gather_output(uint32_t numActive, const uint32_t* activeBricks, cons
 // (u16 diag | mask << 3, brick-tiled order in, dense [z][y][x] out).
 void launch_dense_system(uint32_t numActive, const uint32_t* activeBricks, const uint16_t* coef, const double* b,
                          const double* x, int n, float* bD, float* xD, uint16_t* coefD, hipStream_t s);
+// Dense copy of an FP32 active-brick vector and the operator coefficients
+// (fluid cells only; the caller zeroes vD and coefD).
+void launch_dense_vector(uint32_t numActive, const uint32_t* activeBricks, const uint16_t* coef, const float* v, int n,
+                         float* vD, uint16_t* coefD, hipStream_t s);
 // y = A x on the dense grid (coefficient mask bits: -x, +x, -y, +y, -z, +z).
 void launch_dense_apply_A(const uint16_t* coefD, const float* x, float* y, int n, hipStream_t s);
 // Training loss pieces: res = A z - b; returns via red[0] = ||res||^2,
@@ -97,6 +101,12 @@ void launch_dense_apply_A(const uint16_t* coefD, const float* x, float* y, int n
 void launch_residual(const uint16_t* coefD, const float* z, const float* bD, float* res, int n, double* red,
                      hipStream_t s);
 void launch_loss_grad(const uint16_t* coefD, const float* res, const double* red, float* dOut, int n, hipStream_t s);
+// Damped-Jacobi smoothing around the network on the dense grid:
+// r1 = r - A (w D^-1 r);  z = x + w D^-1 (r - A x) with x = w D^-1 r + y.
+// The adjoint of y -> z is g -> g - A (w D^-1 g), i.e. the presmooth map.
+void launch_dense_presmooth(const uint16_t* coefD, const float* r, float omega, float* r1, int n, hipStream_t s);
+void launch_dense_postsmooth(const uint16_t* coefD, const float* r, const float* y, float omega, float* z, int n,
+                             hipStream_t s);
 // in0/mask/scale from a dense right-hand side (training input).
 void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t* in0, uint8_t* mask, float* scale,
                         double* red, hipStream_t s);

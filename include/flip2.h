@@ -248,6 +248,8 @@ typedef struct flip2_fluid flip2_fluid;
 void flip2_fluid_default_config(flip2_fluid_config* cfg);
 int flip2_fluid_create(uint32_t max_particles, const flip2_fluid_config* cfg, flip2_fluid** out);
 int flip2_fluid_destroy(flip2_fluid* f);
+/* Replaces the configuration (the grid must not change); the learned
+ * preconditioner's settings made by flip2_fluid_set_unet are kept. */
 int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg);
 int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const float* y, const float* z,
                               const float* vx, const float* vy, const float* vz);
@@ -294,9 +296,19 @@ int flip2_unet_apply_dense(flip2_unet* net, const float* b, const uint16_t* coef
 /* Loss ||A z - b||^2 / ||b||^2 with z = net(b) and its parameter gradient
  * (host array of num_params floats, may be NULL). */
 int flip2_unet_loss_grad(flip2_unet* net, const float* b, const uint16_t* coef, double* loss, float* grad);
+/* The preconditioner the network defines: with omega > 0 (default 2/3) it is
+ * wrapped in damped-Jacobi smoothing, z1 = w D^-1 r, y = net(r - A z1),
+ * z = (z1 + y) + w D^-1 (r - A (z1 + y)); omega = 0 uses the network alone.
+ * Training, apply_dense and the solver use the same definition. */
+int flip2_unet_set_smoothing(flip2_unet* net, float omega);
 /* Training data: the current system and solution of a fluid simulation
  * (after its "solve" stage), kept on the device. */
 int flip2_unet_add_frame(flip2_unet* net, flip2_fluid* f);
+/* Residual capture: every following solve of `f` (any preconditioner) adds
+ * its residual r_k = b - A x_k after each listed inner iteration count k of
+ * its first refinement (0 = b itself) as a training frame.  count = 0 stops
+ * the capture.  The network must outlive the capture. */
+int flip2_unet_capture_residuals(flip2_unet* net, flip2_fluid* f, const uint32_t* iterations, int count);
 int flip2_unet_num_frames(flip2_unet* net, int* count);
 int flip2_unet_clear_frames(flip2_unet* net);
 /* `epochs` passes of Adam over the captured frames (frames [first, last)),

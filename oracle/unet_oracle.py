@@ -194,10 +194,13 @@ class Net:
     def loss_grad(self, b, coef):
         """(loss, gradient w.r.t. the flat parameter vector)."""
         L = self.loss(b, coef)
-        g = np.zeros(self.total)
         if self.bb <= 0:
-            return L, g
-        dz = 2.0 * apply_A(coef, self.res) / self.bb
+            return L, np.zeros(self.total)
+        return L, self._backward(2.0 * apply_A(coef, self.res) / self.bb)
+
+    def _backward(self, dz):
+        """Parameter gradient for dL/dz (z = the last forward's output)."""
+        g = np.zeros(self.total)
         t = self.t
         dh = np.where(self.mask, dz * self.s, 0.0)
         g[self.hw:self.hw + 16] = (t["d0"] * dh[..., None]).reshape(-1, 16).sum(axis=0)
@@ -230,7 +233,7 @@ class Net:
         grad["b0"] = grad["b0"] + pool_backward(grad["p0"])
         back(1, "a0", "b0")
         back(0, "x0", "a0")
-        return L, g
+        return g
 
 
 def init_like_library(seed: int = 1234):
@@ -243,3 +246,50 @@ def init_like_library(seed: int = 1234):
         p[w:w + 27 * cin * cout] = rng.normal(0.0, np.sqrt(2.0 / (27 * cin)), 27 * cin * cout)
     p[hw:hw + 16] = rng.normal(0.0, 0.1, 16)
     return p.astype(np.float32)
+
+
+def jacobi(coef, r, omega):
+    """omega D^-1 r on fluid cells (D = operator diagonal)."""
+    coef = np.asarray(coef).astype(np.int64)
+    d = (coef & 7).astype(np.float64)
+    return np.where(coef != 0, omega * np.asarray(r, np.float64) / np.where(d > 0, d, 1.0), 0.0)
+
+
+class SmoothedNet(Net):
+    """Two-level preconditioner around the network N:
+         z1 = w D^-1 r,  r1 = r - A z1,  y = N(r1),
+         z  = (z1 + y) + w D^-1 (r - A (z1 + y))
+    i.e. a damped-Jacobi pre- and post-smoothing step with the network as the
+    coarse correction.  loss = ||A z - r||^2 / ||r||^2; the gradient reaches
+    the network through dL/dy = (I - w A D^-1) dL/dz (A symmetric)."""
+
+    def __init__(self, n, params, omega=2.0 / 3.0, bf16=False):
+        super().__init__(n, params, bf16)
+        self.omega = omega
+
+    def apply(self, r, coef):
+        coef = np.asarray(coef)
+        r64 = np.where(coef != 0, np.asarray(r, np.float32).astype(np.float64), 0.0)
+        z1 = jacobi(coef, r64, self.omega)
+        r1 = np.where(coef != 0, r64 - apply_A(coef, z1), 0.0)
+        y = self.forward(r1.astype(np.float32), coef)
+        x = z1 + y
+        z = x + jacobi(coef, r64 - apply_A(coef, x), self.omega)
+        self.r64 = r64
+        return np.where(coef != 0, z, 0.0)
+
+    def loss(self, r, coef):
+        z = self.apply(r, coef)
+        res = np.where(np.asarray(coef) != 0, apply_A(coef, z) - self.r64, 0.0)
+        rr = float((self.r64 ** 2).sum())
+        self.res, self.bb = res, rr
+        return float((res ** 2).sum()) / rr if rr > 0 else 0.0
+
+    def loss_grad(self, r, coef):
+        L = self.loss(r, coef)
+        g = np.zeros(self.total)
+        if self.bb <= 0:
+            return L, g
+        gz = 2.0 * apply_A(coef, self.res) / self.bb
+        gy = gz - apply_A(coef, jacobi(coef, gz, self.omega))
+        return L, self._backward(gy)

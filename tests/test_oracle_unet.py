@@ -121,3 +121,37 @@ def test_bf16_forward_close_to_fp64():
     z64 = uo.Net(n, p).forward(b, coef)
     z16 = uo.Net(n, p, bf16=True).forward(b, coef)
     assert np.linalg.norm(z16 - z64) <= 3e-2 * np.linalg.norm(z64)
+
+
+def test_smoothed_net_gradient_matches_finite_differences():
+    n = 8
+    b, coef = synthetic_system(n, 5)
+    p = uo.init_like_library(17).astype(np.float64)
+    layout, hw, hb, total = uo.param_layout()
+    rng = np.random.default_rng(18)
+    for w, bo, cin, cout, _ in layout:
+        p[bo:bo + cout] = rng.normal(0.0, 0.05, cout)
+    L, g = uo.SmoothedNet(n, p).loss_grad(b, coef)
+    idx = [hw + 1, hb] + [int(rng.integers(w, bo)) for w, bo, _, _, _ in layout]
+    eps = 1e-6
+    for i in idx:
+        pp = p.copy()
+        pp[i] += eps
+        lp = uo.SmoothedNet(n, pp).loss(b, coef)
+        pp[i] -= 2 * eps
+        lm = uo.SmoothedNet(n, pp).loss(b, coef)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-8, (i, fd, g[i])
+
+
+def test_smoothed_net_with_zero_network_is_two_jacobi_sweeps():
+    # N = 0 (all parameters zero): z = two damped-Jacobi steps from 0
+    n = 8
+    b, coef = synthetic_system(n, 6)
+    net = uo.SmoothedNet(n, np.zeros(uo.param_layout()[3]), omega=0.5)
+    z = net.apply(b, coef)
+    r = np.where(coef != 0, b.astype(np.float64), 0.0)
+    d = np.where(coef != 0, (coef & 7).astype(np.float64), 1.0)
+    x1 = 0.5 * r / d
+    x2 = x1 + 0.5 * (r - uo.apply_A(coef, x1)) / d
+    assert np.allclose(z, np.where(coef != 0, x2, 0.0), rtol=1e-14, atol=1e-14)
