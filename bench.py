@@ -8,7 +8,8 @@ U[-1,1)^3 (seed 3), a 256^3 grid stored as 8^3 bricks.  One step:
   payload = particle index) -> gather of the six particle arrays -> per-cell
   counts -> exclusive scan over the 16,777,216 cell counts -> trilinear MAC P2G
   onto the bricks.
-Inputs are resident in device memory before the timed region; every step
+Inputs (the particle state as 32-byte records, the layout the FLIP step
+keeps) are resident in device memory before the timed region; every step
 redoes the whole transfer from the same unsorted input.
 
 Prints one JSON line.  value = particles x steps / wall seconds of the timed
@@ -83,6 +84,8 @@ def main():
     ap.add_argument("--particles", type=int, default=1 << 24)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB --steps times (reference traffic for counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this pipeline has no multi-GPU "
@@ -92,18 +95,33 @@ def main():
     import flip2_amd as F
     from flip2_amd import scenes
 
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src, dst = F.DeviceBuffer(nbytes), F.DeviceBuffer(nbytes)
+        F.check(F.lib().flip2_memset(src.ptr, 1, nbytes))
+        for _ in range(args.steps):
+            F.check(F.lib().flip2_copy(dst.ptr, src.ptr, nbytes, None))
+        F.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "launches": args.steps}), flush=True)
+        return
+
     n, gn = args.particles, args.grid
     grid = (0.0, 0.0, 0.0, gn, gn, gn, 1.0 / gn)
     print(f"[bench] generating {n} particles", file=sys.stderr, flush=True)
     x, y, z = scenes.uniform_random(n, seed=2)
     vx, vy, vz = scenes.uniform_velocities(n, seed=3)
-    state = [F.DeviceBuffer.from_array(a) for a in (x, y, z, vx, vy, vz)]
+    # the particle state as the FLIP step keeps it: 32-byte records
+    rec = np.zeros((n, 8), np.float32)
+    for col, a in enumerate((x, y, z, vx, vy, vz)):
+        rec[:, col] = a
+    records = F.DeviceBuffer.from_array(rec)
+    del rec
     cells = gn ** 3
     out = [F.DeviceBuffer(4 * cells) for _ in range(3)]
     pl = F.P2GPipeline(n, grid)
     print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        pl.run(n, state, out)
+        pl.run_records(n, records, out)
     F.synchronize()
     occupied = int(np.count_nonzero(pl.get("counts")))
 
@@ -112,7 +130,7 @@ def main():
     F.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        pl.run(n, state, out)
+        pl.run_records(n, records, out)
     F.synchronize()
     wall = time.perf_counter() - t0
     kernels, runs = pl.timing()

@@ -129,6 +129,7 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_memcpy_dtoh": [_vp, _vp, _sz],
         "flip2_memcpy_dtod": [_vp, _vp, _sz],
         "flip2_memset": [_vp, ctypes.c_int, _sz],
+        "flip2_copy": [_vp, _vp, _sz, _vp],
         "flip2_stream_create": [ctypes.POINTER(_vp)],
         "flip2_stream_destroy": [_vp],
         "flip2_stream_synchronize": [_vp],
@@ -167,6 +168,7 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_p2g_pipeline_run": [_vp, _u32] + [_vp] * 10,
         "flip2_p2g_pipeline_get": [_vp, ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_sz)],
         "flip2_p2g_pipeline_enable_timing": [_vp, ctypes.c_int],
+        "flip2_p2g_pipeline_run_records": [_vp, _u32, _vp, _vp, _vp, _vp, _vp],
         "flip2_fluid_create": [_u32, ctypes.POINTER(FluidConfigC), ctypes.POINTER(_vp)],
         "flip2_fluid_destroy": [_vp],
         "flip2_fluid_set_config": [_vp, ctypes.POINTER(FluidConfigC)],
@@ -352,6 +354,10 @@ class P2GPipeline:
     def run(self, n: int, state: Sequence[DeviceBuffer], out: Sequence[DeviceBuffer], stream=None):
         """state: device x, y, z, vx, vy, vz (FP32); out: device U, V, W."""
         check(lib().flip2_p2g_pipeline_run(self._h, int(n), *(b.ptr for b in state), *(b.ptr for b in out), stream))
+
+    def run_records(self, n: int, records: DeviceBuffer, out: Sequence[DeviceBuffer], stream=None):
+        """records: device particle records (8 floats each); out: device U, V, W."""
+        check(lib().flip2_p2g_pipeline_run_records(self._h, int(n), records.ptr, *(b.ptr for b in out), stream))
 
     def get(self, name: str) -> np.ndarray:
         cnt = _sz()

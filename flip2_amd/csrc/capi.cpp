@@ -219,6 +219,12 @@ int flip2_memcpy_dtoh(void* dst, const void* d_src, size_t bytes) {
 int flip2_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes) {
     return guarded([&] { if (bytes) FLIP2_HIP(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice)); });
 }
+int flip2_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        FLIP2_REQUIRE(bytes == 0 || (dst && src), flip2::kErrArgument, "null buffer");
+        flip2::launch_copy(dst, src, bytes, as_stream(stream));
+    });
+}
 int flip2_memset(void* d_ptr, int value, size_t bytes) {
     return guarded([&] { if (bytes) FLIP2_HIP(hipMemset(d_ptr, value, bytes)); });
 }
@@ -530,8 +536,14 @@ int flip2_ns_p2g(const float* d_records, const uint32_t* d_cell_start, flip2_gri
         FLIP2_REQUIRE(grid.sizeX % 8 == 0 && grid.sizeY % 8 == 0 && grid.sizeZ % 8 == 0 && grid.sizeX && grid.sizeY &&
                           grid.sizeZ && grid.cellSize > 0.0f,
                       flip2::kErrArgument, "grid sizes must be positive multiples of 8");
-        flip2::ns::launch_p2g(reinterpret_cast<const float4*>(d_records), d_cell_start, to_grid(grid), d_u, d_v, d_w,
-                              d_wu, d_wv, d_ww, as_stream(stream));
+        const flip2::Grid g = to_grid(grid);
+        const hipStream_t st = as_stream(stream);
+        const float4* rec = reinterpret_cast<const float4*>(d_records);
+        uint32_t* stats = nullptr;
+        FLIP2_HIP(hipMallocAsync(reinterpret_cast<void**>(&stats), 2 * sizeof(uint32_t), st));
+        flip2::ns::launch_p2g_stats(rec, d_cell_start, g.sizeX * g.sizeY * g.sizeZ, stats, st);
+        flip2::ns::launch_p2g(rec, d_cell_start, g, stats, d_u, d_v, d_w, d_wu, d_wv, d_ww, st);
+        FLIP2_HIP(hipFreeAsync(stats, st));
     });
 }
 
@@ -553,6 +565,16 @@ int flip2_p2g_pipeline_run(flip2_p2g_pipeline* pl, uint32_t n, const float* d_x,
     return guarded([&] {
         NEED(pl);
         pl->impl.run(n, d_x, d_y, d_z, d_vx, d_vy, d_vz, d_u, d_v, d_w, as_stream(stream));
+    });
+}
+int flip2_p2g_pipeline_run_records(flip2_p2g_pipeline* pl, uint32_t n, const float* d_records, float* d_u, float* d_v,
+                                   float* d_w, void* stream) {
+    return guarded([&] {
+        NEED(pl);
+        FLIP2_REQUIRE(n == 0 || d_records, flip2::kErrArgument, "null records");
+        FLIP2_REQUIRE((reinterpret_cast<uintptr_t>(d_records) & 15u) == 0, flip2::kErrArgument,
+                      "records must be 16-byte aligned");
+        pl->impl.runRecords(n, reinterpret_cast<const float4*>(d_records), d_u, d_v, d_w, as_stream(stream));
     });
 }
 int flip2_p2g_pipeline_enable_timing(flip2_p2g_pipeline* pl, int capacity) {

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -65,5 +66,8 @@ constexpr double kRcRadius = 2.0;
 constexpr uint32_t kRcRefinement = 1;
 
 inline unsigned ceil_div(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Stream-ordered device copy by a kernel (util.hip).
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 }  // namespace flip2

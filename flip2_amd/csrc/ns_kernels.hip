@@ -4,6 +4,14 @@
 //   counts of the sorted keys, and trilinear MAC P2G with one workgroup per
 //   brick accumulating in an LDS tile.
 //
+// P2G accumulates in 64-bit fixed point with integer LDS atomics: on gfx950
+// an LDS float atomic add costs tens of times an integer one, and integer
+// sums are exact, so the result does not depend on the order in which
+// particles arrive (it is deterministic).  The scale is chosen per launch
+// from the largest particle speed and the largest per-cell particle count
+// (both gathered by the earlier kernels of the step) so that no sum can
+// exceed 2^62; at 8 particles per cell and |v| <= 1 the resolution is 2^-55.
+//
 // Brick-tiled layout: cell (i,j,k) of an n^3 grid (n a multiple of 8) has
 //   index ((bk*nbY + bj)*nbX + bi)*512 + (k&7)*64 + (j&7)*8 + (i&7)
 // so a brick's 512 cells, and after sorting by that index its particles,
@@ -46,8 +54,10 @@ __global__ void bin_kernel(const float* __restrict__ x, const float* __restrict_
 }
 
 // Cell keys plus the packed particle record from SoA arrays.
-__global__ void bin_pack_kernel(Float6 s, uint32_t n, Grid g, uint32_t* __restrict__ keys, float4* __restrict__ rec) {
+__global__ void bin_pack_kernel(Float6 s, uint32_t n, Grid g, uint32_t* __restrict__ keys, float4* __restrict__ rec,
+                                uint32_t* __restrict__ stats) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && stats) { stats[0] = 0u; stats[1] = 0u; }
     if (p >= n) return;
     const float x = s.p[0][p], y = s.p[1][p], z = s.p[2][p];
     keys[p] = cell_key(x, y, z, g);
@@ -56,24 +66,49 @@ __global__ void bin_pack_kernel(Float6 s, uint32_t n, Grid g, uint32_t* __restri
 }
 
 // Cell keys from records.
-__global__ void bin_rec_kernel(const float4* __restrict__ rec, uint32_t n, Grid g, uint32_t* __restrict__ keys) {
+__global__ void bin_rec_kernel(const float4* __restrict__ rec, uint32_t n, Grid g, uint32_t* __restrict__ keys,
+                               uint32_t* __restrict__ stats) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && stats) { stats[0] = 0u; stats[1] = 0u; }
     if (p >= n) return;
     const float4 a = rec[2 * p];
     keys[p] = cell_key(a.x, a.y, a.z, g);
 }
 
-// dst[i] = src[perm[i]] (records).
+// Block maximum of a non-negative u32 (bit pattern of a non-negative float
+// orders like the float), folded into *target with at most one atomic per
+// block, and none once *target is already larger (the common case after the
+// first blocks, which keeps the single address uncontended).
+__device__ inline void block_max_to(uint32_t v, uint32_t* target) {
+    __shared__ uint32_t s_part[16];
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) s_part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (uint32_t w = 0; w < (blockDim.x + 63u) >> 6; ++w) m = max(m, s_part[w]);
+        if (m > __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(target, m);
+    }
+}
+
+// dst[i] = src[perm[i]] (records); stats[0] = max |velocity component|
+// (float bits) when stats is given.
 __global__ void gather_rec_kernel(const uint32_t* __restrict__ perm, uint32_t n, const float4* __restrict__ src,
-                                  float4* __restrict__ dst) {
+                                  float4* __restrict__ dst, uint32_t* __restrict__ stats) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t j = perm[i];
-    if (j >= n) return;
-    const float4 a = src[2 * j];
-    const float4 b = src[2 * j + 1];
-    dst[2 * i] = a;
-    dst[2 * i + 1] = b;
+    float vmax = 0.f;
+    if (i < n) {
+        const uint32_t j = perm[i];
+        if (j < n) {
+            const float4 a = src[2 * j];
+            const float4 b = src[2 * j + 1];
+            dst[2 * i] = a;
+            dst[2 * i + 1] = b;
+            vmax = fmaxf(fabsf(a.w), fmaxf(fabsf(b.x), fabsf(b.y)));
+        }
+    }
+    if (stats) block_max_to(__float_as_uint(vmax), &stats[0]);
 }
 
 // SoA -> records and back.
@@ -99,15 +134,46 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ a, uint32_t count) {
 
 // counts[key] = length of key's run in the sorted key array (written by the
 // run's first element; cells without particles keep the zero fill).
+// counts[key] = run length; stats[1] = largest run (atomically) when given.
 __global__ void run_length_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t numCells,
-                                  uint32_t* __restrict__ counts) {
+                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ stats) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = keys[i];
-    if (i > 0 && keys[i - 1] == k) return;
-    uint32_t e = i + 1;
-    while (e < n && keys[e] == k) ++e;
-    if (k < numCells) counts[k] = e - i;
+    uint32_t run = 0;
+    if (i < n) {
+        const uint32_t k = keys[i];
+        if (i == 0 || keys[i - 1] != k) {
+            uint32_t e = i + 1;
+            while (e < n && keys[e] == k) ++e;
+            if (k < numCells) counts[k] = e - i;
+            run = e - i;
+        }
+    }
+    if (stats) block_max_to(run, &stats[1]);
+}
+
+// Statistics for a P2G call whose inputs did not come through the pipeline:
+// stats[0] = max |velocity component| (float bits), stats[1] = max cell count.
+__global__ void p2g_stats_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart,
+                                 uint32_t numCells, uint32_t* __restrict__ stats) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t n = cellStart[numCells];
+    float vmax = 0.f;
+    uint32_t cmax = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 a = rec[2 * i];
+        const float4 b = rec[2 * i + 1];
+        vmax = fmaxf(vmax, fmaxf(fabsf(a.w), fmaxf(fabsf(b.x), fabsf(b.y))));
+    }
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < numCells; c += stride)
+        cmax = max(cmax, cellStart[c + 1] - cellStart[c]);
+    for (int d = 32; d >= 1; d >>= 1) {
+        vmax = fmaxf(vmax, __shfl_xor(vmax, d, 64));
+        cmax = max(cmax, (uint32_t)__shfl_xor(cmax, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (vmax > 0.f) atomicMax(&stats[0], __float_as_uint(vmax));
+        if (cmax) atomicMax(&stats[1], cmax);
+    }
 }
 
 namespace {
@@ -145,101 +211,186 @@ __device__ inline uint32_t compact3(uint32_t v) {
 
 }  // namespace
 
-// P2G of one brick: particles of the brick's cells and of the one-cell shell
-// around it scatter trilinear weights into the brick's owned faces (the low
-// faces of its 512 cells, per component) held in LDS; each owned face is
-// written once: u = sum(w v) / sum(w), 0 where no particle reaches it.
-// Workgroups are dealt to the 8 XCDs round-robin; each XCD gets a contiguous
-// range of bricks, in Z-order when the brick grid is a power-of-two cube, so
-// the shell cells a workgroup reads were loaded by its neighbours into the
-// same L2.
+// The one-cell shell around a brick, as 222 runs of cells that are
+// contiguous in the tiled order (so each run is one contiguous range of
+// sorted particles): the two z-face layers (64 cells each), the y faces (8
+// runs of 8 cells each), the four yz edges (8 cells), and the x-side cells,
+// which are single cells.  Run r's cells are returned as a tiled range.
+__device__ inline bool shell_run(int r, const int bb[3], uint32_t nbX, uint32_t nbY, uint32_t nbZ, uint32_t& c0,
+                                 uint32_t& c1) {
+    int acc = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (!dx && !dy && !dz) continue;
+                const int nr = dx ? (dy ? 1 : 8) * (dz ? 1 : 8) : (dy && !dz ? 8 : 1);
+                if (r >= acc + nr) {
+                    acc += nr;
+                    continue;
+                }
+                const int k = r - acc;
+                const int bx = bb[0] + dx, by = bb[1] + dy, bz = bb[2] + dz;
+                if (bx < 0 || by < 0 || bz < 0 || bx >= (int)nbX || by >= (int)nbY || bz >= (int)nbZ) return false;
+                const uint32_t base = (((uint32_t)bz * nbY + (uint32_t)by) * nbX + (uint32_t)bx) * 512u;
+                const int ci = dx < 0 ? 7 : 0, cj = dy < 0 ? 7 : 0, ck = dz < 0 ? 7 : 0;
+                uint32_t lo, len;
+                if (!dx && !dy) { lo = ck * 64; len = 64; }
+                else if (!dx && !dz) { lo = k * 64 + cj * 8; len = 8; }
+                else if (!dx) { lo = ck * 64 + cj * 8; len = 8; }
+                else {
+                    const int j2 = dy ? cj : (k & 7);
+                    const int k2 = dz ? ck : (dy ? k : (k >> 3));
+                    lo = k2 * 64 + j2 * 8 + ci;
+                    len = 1;
+                }
+                c0 = base + lo;
+                c1 = c0 + len;
+                return true;
+            }
+    return false;
+}
+
+constexpr int kShellRuns = 222;
+
+// P2G of one brick: the particles of the brick's cells and of the one-cell
+// shell around it (run 0 = the brick, runs 1..222 = the shell) are handed out
+// one per thread; each scatters trilinear weights into the brick's owned
+// faces (the low faces of its 512 cells, per component) held in LDS as
+// 64-bit fixed-point sums.  Each owned face is written once:
+// u = sum(w v) / sum(w), 0 where no particle reaches it.  Workgroups are
+// dealt to the 8 XCDs round-robin; each XCD gets a contiguous range of
+// bricks, in Z-order when the brick grid is a power-of-two cube, so the
+// shell cells a workgroup reads were loaded by its neighbours into the same
+// L2.
 __global__ void __launch_bounds__(256)
 p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart, Grid g,
-                 uint32_t numBricks, int morton, float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
-                 float* __restrict__ WU, float* __restrict__ WV, float* __restrict__ WW) {
-    __shared__ float s_acc[3][512];
-    __shared__ float s_wgt[3][512];
+                 uint32_t numBricks, int morton, const uint32_t* __restrict__ stats, float* __restrict__ U,
+                 float* __restrict__ V, float* __restrict__ W, float* __restrict__ WU, float* __restrict__ WV,
+                 float* __restrict__ WW) {
+    __shared__ unsigned long long s_w[3][512];
+    __shared__ unsigned long long s_m[3][512];
+    __shared__ uint32_t s_first[kShellRuns + 1];
+    __shared__ uint32_t s_off[kShellRuns + 2];
+    __shared__ uint32_t s_wave[4];
     const uint32_t tid = threadIdx.x;
     const uint32_t perXcd = (numBricks + 7u) >> 3;
     const uint32_t order = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
     if (order >= numBricks) return;
-    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
-    uint32_t bi, bj, bk;
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
+    int bb[3];
     if (morton) {
-        bi = compact3(order);
-        bj = compact3(order >> 1);
-        bk = compact3(order >> 2);
+        bb[0] = (int)compact3(order);
+        bb[1] = (int)compact3(order >> 1);
+        bb[2] = (int)compact3(order >> 2);
     } else {
-        bi = order % nbX;
-        bj = (order / nbX) % nbY;
-        bk = order / (nbX * nbY);
+        bb[0] = (int)(order % nbX);
+        bb[1] = (int)((order / nbX) % nbY);
+        bb[2] = (int)(order / (nbX * nbY));
     }
-    const uint32_t brick = (bk * nbY + bj) * nbX + bi;
-    const int lo[3] = {(int)(bi * 8u), (int)(bj * 8u), (int)(bk * 8u)};
-    for (uint32_t t = tid; t < 512u; t += 256u) {
-        s_acc[0][t] = 0.f; s_acc[1][t] = 0.f; s_acc[2][t] = 0.f;
-        s_wgt[0][t] = 0.f; s_wgt[1][t] = 0.f; s_wgt[2][t] = 0.f;
+    const uint32_t brick = ((uint32_t)bb[2] * nbY + (uint32_t)bb[1]) * nbX + (uint32_t)bb[0];
+    for (uint32_t t = tid; t < 1536u; t += 256u) {
+        (&s_w[0][0])[t] = 0ull;
+        (&s_m[0][0])[t] = 0ull;
     }
+    // run table and its exclusive scan
+    uint32_t first = 0, cnt = 0;
+    if (tid <= (uint32_t)kShellRuns) {
+        uint32_t c0 = brick * 512u, c1 = c0 + 512u;
+        const bool ok = tid == 0 || shell_run((int)tid - 1, bb, nbX, nbY, nbZ, c0, c1);
+        if (ok) {
+            first = cellStart[c0];
+            cnt = cellStart[c1] - first;
+        }
+        s_first[tid] = first;
+    }
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= (uint32_t)d) incl += o;
+    }
+    if (lane == 63u) s_wave[wave] = incl;
     __syncthreads();
-    const double inv = 1.0 / (double)g.cellSize;
-    const double neg[3] = {(double)g.negX, (double)g.negY, (double)g.negZ};
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < wave; ++w) base += s_wave[w];
+    const uint32_t total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    if (tid <= (uint32_t)kShellRuns) s_off[tid] = base + incl - cnt;
+    if (tid == 0) s_off[kShellRuns + 1] = total;
+    __syncthreads();
 
-    auto scatter = [&](uint32_t p) {
-        const float4 a = rec[2 * p];
-        const float4 b = rec[2 * p + 1];
-        const double q[3] = {((double)a.x - neg[0]) * inv, ((double)a.y - neg[1]) * inv, ((double)a.z - neg[2]) * inv};
-        const float vel[3] = {a.w, b.x, b.y};
+    // fixed-point scales: a face sums at most 27 cells' particles with w <= 1
+    const uint32_t maxCount = stats[1] > 0u ? stats[1] : 1u;
+    const float vmax = __uint_as_float(stats[0]);
+    const int ew = ilogbf(27.0f * (float)maxCount) + 1;
+    const float scaleW = ldexpf(1.0f, 62 - ew);
+    const int em = vmax > 0.f ? ilogbf(27.0f * (float)maxCount * vmax) + 1 : ew;
+    const float scaleM = ldexpf(1.0f, min(62 - em, 126));
+
+    const double inv = 1.0 / (double)g.cellSize;
+    const double org[3] = {(double)g.negX + (double)(bb[0] * 8) * (double)g.cellSize,
+                           (double)g.negY + (double)(bb[1] * 8) * (double)g.cellSize,
+                           (double)g.negZ + (double)(bb[2] * 8) * (double)g.cellSize};
+    for (uint32_t sIdx = tid; sIdx < total; sIdx += 256u) {
+        int a = 0, b = kShellRuns + 1;  // last run with s_off <= sIdx
+        while (b - a > 1) {
+            const int m = (a + b) >> 1;
+            if (s_off[m] <= sIdx) a = m;
+            else b = m;
+        }
+        const uint32_t p = s_first[a] + (sIdx - s_off[a]);
+        const float4 r0 = rec[2 * p];
+        const float4 r1 = rec[2 * p + 1];
+        // position in cells relative to the brick's low corner
+        const float q[3] = {(float)(((double)r0.x - org[0]) * inv), (float)(((double)r0.y - org[1]) * inv),
+                            (float)(((double)r0.z - org[2]) * inv)};
+        const float vel[3] = {r0.w, r1.x, r1.y};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const Stencil st = mac_stencil(q, c);
+            int bs[3];
+            float t1[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float v = (d == c) ? q[d] : q[d] - 0.5f;
+                const float f = floorf(v);
+                bs[d] = (int)f;
+                t1[d] = v - f;
+            }
+            const float mv = vel[c] * scaleM;
 #pragma unroll
             for (int ax = 0; ax < 2; ++ax) {
-                const int li = st.base[0] + ax - lo[0];
+                const int li = bs[0] + ax;
                 if ((unsigned)li >= 8u) continue;
-                const float wa = ax ? st.w1[0] : st.w0[0];
+                const float wa = ax ? t1[0] : 1.f - t1[0];
 #pragma unroll
                 for (int by = 0; by < 2; ++by) {
-                    const int lj = st.base[1] + by - lo[1];
+                    const int lj = bs[1] + by;
                     if ((unsigned)lj >= 8u) continue;
-                    const float wab = wa * (by ? st.w1[1] : st.w0[1]);
+                    const float wab = wa * (by ? t1[1] : 1.f - t1[1]);
 #pragma unroll
                     for (int bz = 0; bz < 2; ++bz) {
-                        const int lk = st.base[2] + bz - lo[2];
+                        const int lk = bs[2] + bz;
                         if ((unsigned)lk >= 8u) continue;
-                        const float wt = wab * (bz ? st.w1[2] : st.w0[2]);
+                        const float wt = wab * (bz ? t1[2] : 1.f - t1[2]);
                         const int node = lk * 64 + lj * 8 + li;
-                        atomicAdd(&s_wgt[c][node], wt);
-                        atomicAdd(&s_acc[c][node], wt * vel[c]);
+                        atomicAdd(&s_w[c][node], (unsigned long long)__float2ull_rn(wt * scaleW));
+                        atomicAdd(&s_m[c][node], (unsigned long long)__float2ll_rn(wt * mv));
                     }
                 }
             }
         }
-    };
-
-    // the brick's own particles: one contiguous range
-    const uint32_t pBegin = cellStart[brick * 512u];
-    const uint32_t pEnd = cellStart[brick * 512u + 512u];
-    for (uint32_t p = pBegin + tid; p < pEnd; p += 256u) scatter(p);
-
-    // shell cells: the 10^3 block around the brick minus the brick
-    for (uint32_t t = tid; t < 1000u; t += 256u) {
-        const int sx = (int)(t % 10u) - 1, sy = (int)((t / 10u) % 10u) - 1, sz = (int)(t / 100u) - 1;
-        if (sx >= 0 && sx < 8 && sy >= 0 && sy < 8 && sz >= 0 && sz < 8) continue;
-        const int ci = lo[0] + sx, cj = lo[1] + sy, ck = lo[2] + sz;
-        if (ci < 0 || cj < 0 || ck < 0 || ci >= (int)g.sizeX || cj >= (int)g.sizeY || ck >= (int)g.sizeZ) continue;
-        const uint32_t cell = tiled_index((uint32_t)ci, (uint32_t)cj, (uint32_t)ck, nbX, nbY);
-        const uint32_t b0 = cellStart[cell], b1 = cellStart[cell + 1];
-        for (uint32_t p = b0; p < b1; ++p) scatter(p);
     }
     __syncthreads();
     float* out[3] = {U, V, W};
     float* outW[3] = {WU, WV, WW};
+    const double iw = 1.0 / (double)scaleW, im = 1.0 / (double)scaleM;
     for (uint32_t t = tid; t < 512u; t += 256u) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float wsum = s_wgt[c][t];
-            out[c][brick * 512u + t] = wsum > 0.f ? s_acc[c][t] / wsum : 0.f;
-            if (outW[c]) outW[c][brick * 512u + t] = wsum;
+            const double ws = (double)s_w[c][t] * iw;
+            const double ms = (double)(long long)s_m[c][t] * im;
+            out[c][brick * 512u + t] = ws > 0.0 ? (float)(ms / ws) : 0.f;
+            if (outW[c]) outW[c][brick * 512u + t] = (float)ws;
         }
     }
 }
@@ -255,21 +406,22 @@ void launch_bin(const float* x, const float* y, const float* z, uint32_t n, Grid
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, hipStream_t s) {
+void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, uint32_t* stats, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(bin_pack_kernel, grid1d(n, 256), dim3(256), 0, s, src, n, g, keys, rec);
+    hipLaunchKernelGGL(bin_pack_kernel, grid1d(n, 256), dim3(256), 0, s, src, n, g, keys, rec, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, hipStream_t s) {
+void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint32_t* stats, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(bin_rec_kernel, grid1d(n, 256), dim3(256), 0, s, rec, n, g, keys);
+    hipLaunchKernelGGL(bin_rec_kernel, grid1d(n, 256), dim3(256), 0, s, rec, n, g, keys, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, hipStream_t s) {
+void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, uint32_t* stats,
+                       hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(gather_rec_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, src, dst);
+    hipLaunchKernelGGL(gather_rec_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, src, dst, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -285,24 +437,33 @@ void launch_unpack(const float4* rec, uint32_t n, Float6Out dst, hipStream_t s) 
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, hipStream_t s) {
+void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, uint32_t* stats,
+                        hipStream_t s) {
     hipLaunchKernelGGL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
     FLIP2_LAUNCH_CHECK();
     if (!n) return;
-    hipLaunchKernelGGL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts);
+    hipLaunchKernelGGL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, float* U, float* V, float* W, float* WU,
-                float* WV, float* WW, hipStream_t s) {
+void launch_p2g_stats(const float4* sortedRec, const uint32_t* cellStart, uint32_t numCells, uint32_t* stats,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, stats, 2u);
+    FLIP2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(p2g_stats_kernel, dim3(1024), dim3(256), 0, s, sortedRec, cellStart, numCells, stats);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, const uint32_t* stats, float* U, float* V,
+                float* W, float* WU, float* WV, float* WW, hipStream_t s) {
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
     const uint32_t numBricks = nbX * nbY * nbZ;
     if (!numBricks) return;
     const bool pow2 = (nbX & (nbX - 1)) == 0;
     const int morton = (nbX == nbY && nbY == nbZ && pow2 && nbX <= 1024) ? 1 : 0;
     const uint32_t blocks = ((numBricks + 7u) >> 3) * 8u;
-    hipLaunchKernelGGL(p2g_brick_kernel, dim3(blocks), dim3(256), 0, s, sortedRec, cellStart, g, numBricks, morton, U,
-                       V, W, WU, WV, WW);
+    hipLaunchKernelGGL(p2g_brick_kernel, dim3(blocks), dim3(256), 0, s, sortedRec, cellStart, g, numBricks, morton,
+                       stats, U, V, W, WU, WV, WW);
     FLIP2_LAUNCH_CHECK();
 }
 

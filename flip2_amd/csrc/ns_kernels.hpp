@@ -17,20 +17,30 @@ struct Float6Out {
 // Brick-tiled cell keys of FP32 positions, clamped into the grid.
 void launch_bin(const float* x, const float* y, const float* z, uint32_t n, Grid g, uint32_t* keys, hipStream_t s);
 // Particle records: 2 x float4 per particle, (x, y, z, vx), (vy, vz, 0, 0).
-void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, hipStream_t s);
-void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, hipStream_t s);
+// stats (2 x u32, may be null) are the P2G statistics of the step: the bin
+// kernels zero them, the gather records max |velocity component| (float
+// bits) in stats[0], the cell counts the largest count in stats[1].
+void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, uint32_t* stats, hipStream_t s);
+void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint32_t* stats, hipStream_t s);
 // dst[i] = src[perm[i]] (records)
-void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, hipStream_t s);
+void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, uint32_t* stats,
+                       hipStream_t s);
 void launch_pack(Float6 src, uint32_t n, float4* rec, hipStream_t s);
 void launch_unpack(const float4* rec, uint32_t n, Float6Out dst, hipStream_t s);
 // counts[0..numCells] zeroed, then counts[key] = run length of key in the
 // sorted keys (counts[numCells] stays 0, so an exclusive scan over
 // numCells + 1 entries ends with n).
-void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, hipStream_t s);
+void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, uint32_t* stats,
+                        hipStream_t s);
+// The statistics for a P2G whose inputs did not come through the pipeline.
+// (the particle count is cellStart[numCells])
+void launch_p2g_stats(const float4* sortedRec, const uint32_t* cellStart, uint32_t numCells, uint32_t* stats,
+                      hipStream_t s);
 // Trilinear MAC P2G from records sorted by cell key, one workgroup per
-// brick.  WU/WV/WW (weight sums) may be null.
-void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, float* U, float* V, float* W, float* WU,
-                float* WV, float* WW, hipStream_t s);
+// brick, 64-bit fixed-point accumulation scaled from stats.  WU/WV/WW
+// (weight sums) may be null.
+void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, const uint32_t* stats, float* U, float* V,
+                float* W, float* WU, float* WV, float* WW, hipStream_t s);
 
 }  // namespace ns
 }  // namespace flip2

@@ -25,6 +25,8 @@ P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g) : grid_(g), maxPa
     sorted_.resizeAsync(2 * maxParticles, nullptr);
     const size_t ws = std::max(sort_workspace_bytes(maxParticles, 0, keyBits_), scan_workspace_bytes(numCells_ + 1));
     ws_.resizeAsync((uint32_t)ws, nullptr);
+    stats_.resizeAsync(2u, nullptr);
+    stats_.zeroDevice();
 }
 
 void P2GPipeline::run(uint32_t n, const float* x, const float* y, const float* z, const float* vx, const float* vy,
@@ -51,23 +53,24 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
     if (timed) FLIP2_HIP(hipEventRecord(ev[0], s));
     const float4* records = rec;
     if (soa) {
-        ns::launch_bin_pack(*soa, n, grid_, keys_.devPtr(), packed_.devPtr(), s);
+        ns::launch_bin_pack(*soa, n, grid_, keys_.devPtr(), packed_.devPtr(), stats_.devPtr(), s);
         records = packed_.devPtr();
     } else {
-        ns::launch_bin_rec(rec, n, grid_, keys_.devPtr(), s);
+        ns::launch_bin_rec(rec, n, grid_, keys_.devPtr(), stats_.devPtr(), s);
     }
     if (timed) FLIP2_HIP(hipEventRecord(ev[1], s));
     if (n) {
         sort_pairs_u32(keys_.devPtr(), nullptr, keysSorted_.devPtr(), perm_.devPtr(), keysAlt_.devPtr(),
                        permAlt_.devPtr(), n, 0, keyBits_, ws_.devPtr(), ws_.size(), s, timed ? ev + 2 : nullptr);
-        ns::launch_gather_rec(perm_.devPtr(), n, records, sorted_.devPtr(), s);
+        ns::launch_gather_rec(perm_.devPtr(), n, records, sorted_.devPtr(), stats_.devPtr(), s);
     }
     if (timed) FLIP2_HIP(hipEventRecord(ev[3 + passes], s));
-    ns::launch_cell_counts(keysSorted_.devPtr(), n, numCells_, counts_.devPtr(), s);
+    ns::launch_cell_counts(keysSorted_.devPtr(), n, numCells_, counts_.devPtr(), stats_.devPtr(), s);
     if (timed) FLIP2_HIP(hipEventRecord(ev[4 + passes], s));
     scan_u32(counts_.devPtr(), cellStart_.devPtr(), numCells_ + 1, true, ws_.devPtr(), ws_.size(), nullptr, s);
     if (timed) FLIP2_HIP(hipEventRecord(ev[5 + passes], s));
-    ns::launch_p2g(sorted_.devPtr(), cellStart_.devPtr(), grid_, U, V, W, nullptr, nullptr, nullptr, s);
+    ns::launch_p2g(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), U, V, W, nullptr, nullptr, nullptr,
+                   s);
     if (timed) {
         FLIP2_HIP(hipEventRecord(ev[6 + passes], s));
         ++pending_;

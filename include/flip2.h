@@ -57,6 +57,9 @@ int flip2_memcpy_htod(void* d_dst, const void* src, size_t bytes); /* cudaVec.cu
 int flip2_memcpy_dtoh(void* dst, const void* d_src, size_t bytes); /* cudaVec.cu:54-60  */
 int flip2_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
 int flip2_memset(void* d_ptr, int value, size_t bytes);           /* zeroArray, cudaVec.hu:9-15 */
+/* Stream-ordered device-to-device copy done by a kernel (16-byte vectors
+ * when both pointers are 16-byte aligned); stream NULL = default stream. */
+int flip2_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
 int flip2_stream_create(void** stream);
 int flip2_stream_destroy(void* stream);
 int flip2_stream_synchronize(void* stream);
@@ -197,6 +200,10 @@ int flip2_p2g_pipeline_destroy(flip2_p2g_pipeline* pl);
 int flip2_p2g_pipeline_run(flip2_p2g_pipeline* pl, uint32_t n, const float* d_x, const float* d_y, const float* d_z,
                            const float* d_vx, const float* d_vy, const float* d_vz, float* d_u, float* d_v,
                            float* d_w, void* stream);
+/* Same from particle records (8 floats per particle: x, y, z, vx, vy, vz, 0,
+ * 0; 16-byte aligned), the layout the FLIP step keeps its particles in. */
+int flip2_p2g_pipeline_run_records(flip2_p2g_pipeline* pl, uint32_t n, const float* d_records, float* d_u, float* d_v,
+                                   float* d_w, void* stream);
 /* Intermediates of the last run: "keys", "sortedKeys", "perm", "counts",
  * "cellStart", "x", "y", "z", "vx", "vy", "vz" (sorted state). */
 int flip2_p2g_pipeline_get(flip2_p2g_pipeline* pl, const char* name, void* host, size_t capacity, size_t* count);

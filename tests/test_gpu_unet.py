@@ -104,22 +104,18 @@ def test_loss_grad_is_repeatable(flip2):
     b, coef = synthetic_system(n, 21)
     l1, g1 = net.loss_grad(b, coef)
     l2, g2 = net.loss_grad(b, coef)
-    assert l1 == l2
-    # weight gradients are FP32 atomics: order may differ in the last bits
+    # the loss and the weight gradients are sums with atomics: the order may
+    # differ in the last bits
+    assert abs(l1 - l2) <= 1e-12 * abs(l1)
     assert rel(g1, g2.astype(np.float64)) <= 1e-5
 
 
-def capture_frames(flip2, sim, net, steps):
-    for _ in range(steps):
-        for st in ("transfer", "prepare", "system", "solve"):
-            sim.stage(st)
-        assert sim.last_solve()["converged"]
-        net.add_frame(sim)
-        for st in ("project", "g2p", "advect"):
-            sim.stage(st)
-
-
 def test_training_and_learned_fcg(flip2):
+    """Train on the residuals r_k = b - A x_k that Jacobi-PCG produces in its
+    first iterations (k = 0, 2, ..., 64) over 12 dam-break steps; the trained
+    network, wrapped in damped-Jacobi smoothing, preconditions flexible CG
+    to the 1e-6 FP64 residual in fewer iterations than Jacobi-PCG on the
+    next step's system."""
     n = 32
     x, y, z = scenes.dam_break(n, seed=1)
     zero = np.zeros_like(x)
@@ -127,13 +123,19 @@ def test_training_and_learned_fcg(flip2):
     sim = flip2.FluidSim(x.size, grid, preconditioner="jacobi")
     sim.set_particles(x, y, z, zero, zero, zero)
     net = flip2.UNet(n, seed=2)
-    capture_frames(flip2, sim, net, 24)
-    assert net.num_frames() == 24
-    before = net.eval(20, 24)
-    net.train(60, 1e-3, 0, 20, seed=1)
-    after = net.eval(20, 24)
-    assert after < 0.5 * before, (before, after)
-    # next system: learned FCG vs Jacobi-PCG on the same right-hand side
+    its = [0, 2, 4, 8, 16, 32, 64]
+    net.capture_residuals(sim, its)
+    for _ in range(12):
+        sim.step()
+        assert sim.last_solve()["converged"]
+    net.capture_residuals(sim, [])
+    nf = net.num_frames()
+    assert nf == 12 * len(its)
+    hold = 2 * len(its)
+    before = net.eval(nf - hold, nf)
+    net.train(12, 1e-3, 0, nf - hold, seed=1)
+    after = net.eval(nf - hold, nf)
+    assert after < 0.25 * before, (before, after)
     for st in ("transfer", "prepare", "system"):
         sim.stage(st)
     sim.set_unet(net)
@@ -142,3 +144,7 @@ def test_training_and_learned_fcg(flip2):
     assert jac["converged"] and lrn["converged"], (jac, lrn)
     assert lrn["rel_residual"] <= 1e-6
     assert lrn["iterations"] < jac["iterations"], (lrn, jac)
+    # the fallback: a cap below what FCG needs hands the solve to Jacobi-PCG
+    sim.set_unet(net, fallback_iterations=2)
+    fb = sim.solve_with("learned")
+    assert fb["used_fallback"] and fb["converged"] and fb["rel_residual"] <= 1e-6, fb
