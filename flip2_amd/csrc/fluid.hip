@@ -186,7 +186,7 @@ void FluidSim::buildSystem() {
     const double hOverDt = (double)cfg_.grid.cellSize / (double)cfg_.dt;
     ns::launch_assemble(numActive, activeBricks_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(), solid, cfg_.grid,
                         hOverDt, U_.devPtr(), V_.devPtr(), W_.devPtr(), coef_.devPtr(), b64_.devPtr(), reduce(),
-                        scal_.devPtr() + 0, stream_);
+                        scal_.devPtr() + 4, stream_);
 }
 
 SolveStats FluidSim::solve() { return solveWith(cfg_.preconditioner); }
@@ -220,7 +220,11 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
         stats_ = st;
         return st;
     }
-    const double bb = readScalar(scal_.devPtr() + 0);
+    double sys[2];  // ||b||^2, fluid cells
+    FLIP2_HIP(hipMemcpyAsync(sys, scal_.devPtr() + 4, sizeof(sys), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    const double bb = sys[0];
+    st.fluidCells = (uint32_t)sys[1];
     st.bNorm = std::sqrt(bb);
     ns::launch_fill_f64(x64_.devPtr(), cells, 0.0, stream_);
     if (bb == 0.0) {

@@ -190,7 +190,7 @@ __device__ inline void cell_coords(uint32_t b, uint32_t l, uint32_t nbX, uint32_
 
 // Operator coefficients (per fluid cell: number of non-solid neighbours and a
 // mask of fluid neighbours) and the FP64 right-hand side -(h/dt) D_c;
-// ||b||^2 reduced into scal[0].
+// ||b||^2 reduced into scal[0], the number of fluid cells into scal[1].
 __global__ void __launch_bounds__(kWG)
 assemble_kernel(const uint32_t* __restrict__ activeBricks, const int32_t* __restrict__ nbrSlot,
                 const uint8_t* __restrict__ cellType, const uint8_t* __restrict__ solid, Grid g, double hOverDt,
@@ -201,12 +201,13 @@ assemble_kernel(const uint32_t* __restrict__ activeBricks, const int32_t* __rest
     const uint32_t slot = blockIdx.x;
     const uint32_t b = activeBricks[slot];
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
-    double bb = 0.0;
+    double bb = 0.0, fluid = 0.0;
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
         const uint32_t v = slot * 512u + l;
         uint16_t cf = 0;
         double rhs = 0.0;
         if (cellType[v] == kFluid) {
+            fluid += 1.0;
             int gc[3];
             cell_coords(b, l, nbX, nbY, gc);
             uint32_t diag = 0, mask = 0;
@@ -231,8 +232,8 @@ assemble_kernel(const uint32_t* __restrict__ activeBricks, const int32_t* __rest
         b64[v] = rhs;
         bb += rhs * rhs;
     }
-    const double t = block_sum(bb, s_tmp);
-    finish_reduction(&t, 1, partials, counter, scal);
+    const double t[2] = {block_sum(bb, s_tmp), block_sum(fluid, s_tmp)};
+    finish_reduction(t, 2, partials, counter, scal);
 }
 
 namespace {

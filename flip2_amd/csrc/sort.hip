@@ -6,7 +6,7 @@
 // Per sort:
 //   1. one histogram kernel counts the 8-bit digits of every pass at once;
 //   2. a small kernel turns the counts into per-pass digit offsets;
-//   3. one kernel per pass: each 4096-key tile ranks its keys by digit with
+//   3. one kernel per pass: each tile (256 lanes x 16..32 keys) ranks its keys by digit with
 //      64-lane ballot matching (stable within the wave), publishes its digit
 //      counts, resolves its global digit bases by per-digit decoupled
 //      look-back over earlier tiles, stages the keys in LDS in sorted order
@@ -23,8 +23,8 @@ namespace {
 
 constexpr int kSortThreads = 256;             // 4 waves
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortItems = 16;                // keys per lane
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr int kMinSortItems = 16;             // keys per lane of the smallest tile
+constexpr int kMinSortTile = kSortThreads * kMinSortItems;  // 4096
 constexpr int kRadix = 256;
 constexpr int kHistThreads = 256;
 constexpr int kHistItems = 32;                // keys per thread in the histogram kernel
@@ -43,7 +43,7 @@ struct SortLayout {
 SortLayout sort_layout(uint32_t n, uint32_t beginBit, uint32_t endBit) {
     SortLayout L{};
     L.passes = (endBit > beginBit) ? ceil_div(endBit - beginBit, 8) : 0;
-    L.tiles = ceil_div(n ? n : 1, kSortTile);
+    L.tiles = ceil_div(n ? n : 1, kMinSortTile);  // enough for every tile size
     size_t off = 0;
     L.histOff = off; off += (size_t)kMaxPasses * kRadix * sizeof(uint32_t);
     L.offsetOff = off; off += (size_t)kMaxPasses * kRadix * sizeof(uint32_t);
@@ -114,13 +114,14 @@ __device__ inline void digit_status_store(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <int ITEMS>
 __global__ void __launch_bounds__(kSortThreads)
 sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
                  uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n,
                  uint32_t shift, uint32_t mask, const uint32_t* __restrict__ digitOffsets,
                  uint32_t* __restrict__ status, uint32_t* __restrict__ counter) {
-    __shared__ uint32_t s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_keys[(ITEMS * kSortThreads)];
+    __shared__ uint32_t s_vals[(ITEMS * kSortThreads)];
     __shared__ uint32_t s_wave[kSortWaves][kRadix];  // counts, then exclusive offsets across waves
     __shared__ uint32_t s_tileExcl[kRadix];
     __shared__ uint32_t s_globalBase[kRadix];
@@ -134,12 +135,12 @@ sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict
     for (int i = tid; i < kSortWaves * kRadix; i += kSortThreads) (&s_wave[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t tileBase = tile * (uint32_t)kSortTile;
-    const uint32_t waveBase = tileBase + (uint32_t)wave * (64u * kSortItems);
+    const uint32_t tileBase = tile * (uint32_t)(ITEMS * kSortThreads);
+    const uint32_t waveBase = tileBase + (uint32_t)wave * (64u * ITEMS);
 
-    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = waveBase + (uint32_t)k * 64u + (uint32_t)lane;
         const bool valid = i < n;
         key[k] = valid ? keysIn[i] : 0xffffffffu;
@@ -148,7 +149,7 @@ sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict
 
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = waveBase + (uint32_t)k * 64u + (uint32_t)lane;
         const bool valid = i < n;
         const uint32_t d = digit_of(key[k], shift, mask);
@@ -211,7 +212,7 @@ sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict
 
     // Stage in LDS in tile-sorted order.
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = waveBase + (uint32_t)k * 64u + (uint32_t)lane;
         if (i < n) {
             const uint32_t dk = digit_of(key[k], shift, mask);
@@ -222,7 +223,7 @@ sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict
     }
     __syncthreads();
 
-    const uint32_t valid = (n > tileBase) ? min((uint32_t)kSortTile, n - tileBase) : 0u;
+    const uint32_t valid = (n > tileBase) ? min((uint32_t)(ITEMS * kSortThreads), n - tileBase) : 0u;
     for (uint32_t j = (uint32_t)tid; j < valid; j += kSortThreads) {
         const uint32_t k = s_keys[j];
         const uint32_t o = s_globalBase[digit_of(k, shift, mask)] + j;
@@ -246,7 +247,9 @@ size_t sort_workspace_bytes(uint32_t n, uint32_t beginBit, uint32_t endBit) {
 
 void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* keysOut, uint32_t* valsOut,
                     uint32_t* keysAlt, uint32_t* valsAlt, uint32_t n, uint32_t beginBit, uint32_t endBit,
-                    void* ws, size_t wsBytes, hipStream_t stream, hipEvent_t* passEvents) {
+                    void* ws, size_t wsBytes, hipStream_t stream, hipEvent_t* passEvents, int itemsPerThread) {
+    const int items = itemsPerThread ? itemsPerThread : 16;
+    FLIP2_REQUIRE(items == 16 || items == 24 || items == 32, kErrArgument, "sort: 16, 24 or 32 keys per lane");
     FLIP2_REQUIRE(endBit <= 32 && beginBit <= endBit, kErrArgument, "sort: bad bit range");
     FLIP2_REQUIRE(n < (1u << kDigitFlagShift), kErrRange, "sort: n must be below 2^30");
     FLIP2_REQUIRE(keysIn && keysOut && valsOut, kErrArgument, "sort: null buffer");
@@ -296,9 +299,16 @@ void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* ke
         const uint32_t shift = beginBit + 8u * p;
         const uint32_t bits = std::min(8u, endBit - shift);
         const uint32_t mask = (1u << bits) - 1u;
-        hipLaunchKernelGGL(sort_pass_kernel, dim3(L.tiles), dim3(kSortThreads), 0, stream, srcK, srcV, dstK,
-                           dstV, n, shift, mask, offsets + p * kRadix,
-                           status + (size_t)p * L.tiles * kRadix, counters + p);
+        uint32_t* st = status + (size_t)p * L.tiles * kRadix;
+        if (items == 32)
+            hipLaunchKernelGGL(sort_pass_kernel<32>, dim3(ceil_div(n, 32u * kSortThreads)), dim3(kSortThreads), 0,
+                               stream, srcK, srcV, dstK, dstV, n, shift, mask, offsets + p * kRadix, st, counters + p);
+        else if (items == 24)
+            hipLaunchKernelGGL(sort_pass_kernel<24>, dim3(ceil_div(n, 24u * kSortThreads)), dim3(kSortThreads), 0,
+                               stream, srcK, srcV, dstK, dstV, n, shift, mask, offsets + p * kRadix, st, counters + p);
+        else
+            hipLaunchKernelGGL(sort_pass_kernel<16>, dim3(ceil_div(n, 16u * kSortThreads)), dim3(kSortThreads), 0,
+                               stream, srcK, srcV, dstK, dstV, n, shift, mask, offsets + p * kRadix, st, counters + p);
         FLIP2_LAUNCH_CHECK();
         if (passEvents) FLIP2_HIP(hipEventRecord(passEvents[p + 1], stream));
         srcK = dstK;
