@@ -76,6 +76,145 @@ def cpu_baseline(target_seconds=10.0):
                       f"in a 128^3 grid, {el:.1f} s on 1 host thread"}
 
 
+def _fluid_step_stats(sim, steps):
+    """Run `steps` FLIP steps; return wall seconds and per-step solve stats."""
+    import flip2_amd as F
+    F.synchronize()
+    solves = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+        solves.append(sim.last_solve())
+    F.synchronize()
+    return time.perf_counter() - t0, solves
+
+
+def _summary(solves):
+    it = [s["iterations"] for s in solves]
+    ms = [s["solve_ms"] for s in solves]
+    return {"mean_iterations": round(float(np.mean(it)), 1), "mean_solve_ms": round(float(np.mean(ms)), 3),
+            "max_rel_residual": float(max(s["rel_residual"] for s in solves)),
+            "all_converged": bool(all(s["converged"] for s in solves)),
+            "fluid_cells": int(solves[-1]["fluid_cells"])}
+
+
+def config_run(args):
+    """Whole FLIP steps on the configurations the solver targets (one JSON line)."""
+    import flip2_amd as F
+    from flip2_amd import scenes
+
+    cfg = args.config
+    if cfg == "c1":
+        n, steps, prec = 64, 10, "jacobi"
+        x, y, z = scenes.dam_break(n, seed=1)
+        vel = None
+        solid = None
+        desc = "dam break x<0.4, y<0.6 on a 64^3 MAC grid, 8 jittered particles per cell, 10 steps, Jacobi-PCG 1e-6"
+    elif cfg == "c3":
+        n, steps, prec = args.grid_c3, 3, "mic0"
+        (x, y, z), vel = scenes.colliding_spheres(n, seed=4)
+        solid = None
+        desc = (f"two fluid spheres r=0.15 at (0.3,.5,.5) +2 m/s and (0.7,.5,.5) -2 m/s, {n}^3 as 8^3 bricks, "
+                "8 ppc, MIC(0)-PCG 1e-6 (FP32 matvec/preconditioner, FP64 dots and refinement)")
+    else:
+        raise SystemExit(f"unknown config {cfg}")
+    grid = (0.0, 0.0, 0.0, n, n, n, 1.0 / n)
+    print(f"[bench] {cfg}: {x.size} particles", file=sys.stderr, flush=True)
+    sim = F.FluidSim(x.size, grid, preconditioner=prec)
+    zero = np.zeros_like(x)
+    if vel is None:
+        sim.set_particles(x, y, z, zero, zero, zero)
+    else:
+        sim.set_particles(x, y, z, *vel)
+    if solid is not None:
+        sim.set_solid(solid)
+    _fluid_step_stats(sim, 1)  # warm-up step (allocation, first sort)
+    wall, solves = _fluid_step_stats(sim, steps)
+    out = {"metric": "particle-steps/s", "config": cfg, "workload": desc, "particles": int(x.size), "grid": n,
+           "steps": steps, "value": x.size * steps / wall, "ms_per_step": round(wall / steps * 1e3, 3),
+           "preconditioner": prec, "solve": _summary(solves)}
+    print(json.dumps(out), flush=True)
+
+
+def learned_run(args):
+    """Learned preconditioner vs plain PCG, ms per frame of the pressure solve
+    to the 1e-6 FP64 relative residual (c4: train then compare; c5: apply a
+    saved network to the 256^3 wave tank with a moving paddle)."""
+    import flip2_amd as F
+    from flip2_amd import scenes
+
+    its = [0, 2, 4, 8, 16, 32, 64, 128]
+    if args.config == "c4":
+        n = args.grid_c4
+        (x, y, z), solid = scenes.dam_break_sphere(n, seed=5)
+        grid = (0.0, 0.0, 0.0, n, n, n, 1.0 / n)
+        sim = F.FluidSim(x.size, grid, preconditioner="jacobi")
+        sim.set_particles(x, y, z, *(np.zeros_like(x),) * 3)
+        sim.set_solid(solid)
+        net = F.UNet(n, seed=2)
+        print(f"[bench] c4: {x.size} particles; capturing {args.frames} frames", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        net.capture_residuals(sim, its)
+        for _ in range(args.frames):
+            sim.step()
+        net.capture_residuals(sim, [])
+        nf = net.num_frames()
+        capture_s = time.perf_counter() - t0
+        print(f"[bench] training {args.epochs} epochs on {nf} frames", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        losses = []
+        for e in range(args.epochs):
+            losses.append(net.train(1, 1e-3 if e < args.epochs - 1 else 3e-4, 0, nf, seed=e))
+            print(f"[bench] epoch {e + 1}: loss {losses[-1]:.4f}", file=sys.stderr, flush=True)
+        train_s = time.perf_counter() - t0
+        if args.save_net:
+            np.save(args.save_net, net.params(), allow_pickle=False)
+        desc = (f"dam break + static sphere r=0.1 at (0.6,0.2,0.5), {n}^3, 8 ppc; U-Net 16/32/64, BF16/FP32, "
+                f"trained on {nf} residual frames of {args.frames} Jacobi-PCG solves ({args.epochs} epochs)")
+        extra = {"capture_s": round(capture_s, 1), "train_s": round(train_s, 1),
+                 "final_loss": round(float(losses[-1]), 5) if losses else None}
+    elif args.config == "c5":
+        n = args.grid_c5
+        x, y, z = scenes.wave_tank(n, seed=6)
+        grid = (0.0, 0.0, 0.0, n, n, n, 1.0 / n)
+        sim = F.FluidSim(x.size, grid, preconditioner="jacobi")
+        sim.set_particles(x, y, z, *(np.zeros_like(x),) * 3)
+        net = F.UNet(n, seed=2)
+        if not args.load_net:
+            raise SystemExit("c5 applies a network trained on c4: pass --load-net")
+        net.set_params(np.load(args.load_net, allow_pickle=False))
+        desc = (f"wave tank depth 0.35 with paddle x = 0.05 + 0.05 sin(2 pi t), {n}^3, 8 ppc; network trained "
+                "on c4, no retraining")
+        extra = {}
+    else:
+        raise SystemExit(f"unknown config {args.config}")
+    sim.set_unet(net, fallback_iterations=args.fallback)
+    res = {"jacobi": [], "mic0": [], "learned": []}
+    t_sim = 0.0
+    dt = 1.0 / 240.0
+    for s in range(args.compare_frames):
+        if args.config == "c5":
+            xp, up = scenes.paddle_position(t_sim)
+            sim.set_solid_box((0.0, 0.0, 0.0), (xp, 1.0, 1.0), (up, 0.0, 0.0))
+        for st in ("transfer", "prepare", "system"):
+            sim.stage(st)
+        for name in ("jacobi", "mic0", "learned"):
+            res[name].append(sim.solve_with(name))
+        sim.solve_with("jacobi")
+        for st in ("project", "g2p", "advect"):
+            sim.stage(st)
+        t_sim += dt
+    summ = {k: _summary(v) for k, v in res.items()}
+    summ["learned"]["used_fallback"] = int(sum(s["used_fallback"] for s in res["learned"]))
+    out = {"metric": "ms/frame pressure solve @1e-6 rel. residual, learned vs PCG", "config": args.config,
+           "workload": desc, "particles": int(x.size), "grid": n, "frames": args.compare_frames,
+           "solves": summ,
+           "speedup_vs_jacobi": round(summ["jacobi"]["mean_solve_ms"] / summ["learned"]["mean_solve_ms"], 3),
+           "speedup_vs_mic0": round(summ["mic0"]["mean_solve_ms"] / summ["learned"]["mean_solve_ms"], 3)}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,11 +225,28 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB --steps times (reference traffic for counters)")
+    ap.add_argument("--config", choices=["c1", "c3", "c4", "c5"], default=None,
+                    help="whole FLIP steps (c1, c3) or the learned-vs-PCG solve comparison (c4, c5) instead of "
+                         "the transfer microbench")
+    ap.add_argument("--grid-c3", type=int, default=512)
+    ap.add_argument("--grid-c4", type=int, default=128)
+    ap.add_argument("--grid-c5", type=int, default=256)
+    ap.add_argument("--frames", type=int, default=200, help="c4: frames of Jacobi-PCG solves captured for training")
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--compare-frames", type=int, default=8)
+    ap.add_argument("--fallback", type=int, default=0, help="learned: Jacobi-PCG fallback iteration cap (0: none)")
+    ap.add_argument("--save-net", default=None)
+    ap.add_argument("--load-net", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this pipeline has no multi-GPU "
               "path: see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+
+    if args.config in ("c1", "c3"):
+        return config_run(args)
+    if args.config in ("c4", "c5"):
+        return learned_run(args)
 
     import flip2_amd as F
     from flip2_amd import scenes

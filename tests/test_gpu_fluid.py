@@ -150,3 +150,47 @@ def test_dam_break_steps(flip2):
     assert float(np.mean(ys)) < y0          # the column falls
     assert float(np.mean(vy)) < 0.0
     assert np.all(np.isfinite(vx)) and np.all(np.isfinite(vy)) and np.all(np.isfinite(vz))
+
+
+def test_moving_solid_box_boundary(flip2, oracle):
+    """A solid box moving at u_p (a paddle): faces between box cells and fluid
+    cells carry u_p before and after the projection, domain walls stay 0,
+    and a particle that ends inside the box is pushed out of it."""
+    n = 32
+    h = 1.0 / n
+    x, y, z = scenes.jittered_cells((n, n, n), h, seed=12,
+                                    keep=lambda a, b, c: (a >= np.float32(0.25)) & (a < np.float32(0.75))
+                                    & (b < np.float32(0.5)))
+    # one particle inside the box region, at rest
+    x = np.append(x, np.float32(0.2)); y = np.append(y, np.float32(0.1)); z = np.append(z, np.float32(0.5))
+    zero = np.zeros_like(x)
+    grid = (0.0, 0.0, 0.0, n, n, n, h)
+    sim = flip2.FluidSim(x.size, grid, preconditioner="mic0")
+    sim.set_particles(x, y, z, zero, zero, zero)
+    up = 0.5
+    sim.set_solid_box((0.0, 0.0, 0.0), (0.25, 1.0, 1.0), (up, 0.0, 0.0))
+    sim.stage("transfer")
+    sim.stage("prepare")
+    U = sim.get("U")
+    jj = np.arange(n)
+    i, j, k = np.meshgrid(jj, jj, jj, indexing="ij")
+    idx = oracle.ns_tiled_index(i, j, k, grid)
+    Ud = U[idx]  # [i, j, k]
+    assert np.all(Ud[8, :, :] == np.float32(up))      # box | fluid-or-air faces
+    assert np.all(Ud[1:8, :, :] == np.float32(up))    # faces between box cells
+    assert np.all(Ud[0, :, :] == 0.0)                 # domain wall
+    for s in ("system", "solve"):
+        sim.stage(s)
+    assert sim.last_solve()["converged"]
+    sim.stage("project")
+    U2 = sim.get("U")[idx]
+    assert np.all(U2[8, :, :] == np.float32(up))
+    sim.stage("g2p")
+    sim.stage("advect")
+    px, py, pz, _, _, _ = sim.particles()
+    assert np.all(px >= np.float32(0.25)), float(px.min())
+    # without the box the wall faces inside x < 0.25 are ordinary faces again
+    sim.set_solid_box(None)
+    sim.stage("transfer")
+    sim.stage("prepare")
+    assert np.all(sim.get("U")[idx][1:8, :, :] == 0.0)  # no particles there: P2G gives 0
