@@ -450,9 +450,11 @@ void UNet::convBackward(int layer, float* dY, const uint16_t* Y, const uint16_t*
     const UNetLayer& L = layers_[layer];
     const int n = L.level == 0 ? n_ : (L.level == 1 ? n1_ : n2_);
     const long long nvox = (long long)n * n * n;
-    nn::launch_relu_mask(dY, Y, nvox * L.cout, s);
-    nn::launch_conv3_wgrad(dY, X, grad_.devPtr() + L.w, grad_.devPtr() + L.b, n, L.cin, L.cout, s);
-    if (dX) nn::launch_conv3_dgrad(dY, params_.devPtr() + L.w, dX, n, L.cin, L.cout, s);
+    if (dPb_.size() < (uint32_t)(nvox * L.cout)) dPb_.resizeAsync((uint32_t)(nvox * L.cout), s);
+    if (WT_.size() < (uint32_t)(27 * L.cin * L.cout)) WT_.resizeAsync((uint32_t)(27 * L.cin * L.cout), s);
+    nn::launch_relu_mask_bf16(dY, Y, dPb_.devPtr(), nvox * L.cout, s);
+    nn::launch_conv3_wgrad(dPb_.devPtr(), dY, X, grad_.devPtr() + L.w, grad_.devPtr() + L.b, n, L.cin, L.cout, s);
+    if (dX) nn::launch_conv3_dgrad(dPb_.devPtr(), params_.devPtr() + L.w, WT_.devPtr(), dX, n, L.cin, L.cout, s);
 }
 
 void UNet::backward(const float* dOut, hipStream_t s) {

@@ -75,6 +75,7 @@ private:
     // gradients (FP32)
     DeviceVec<float> gd0_, gc0_, gu0_, gd1_, gc1_, gu1_, gb2_, ga2_, gp1_, gb1_, ga1_, gp0_, gb0_, ga0_;
     DeviceVec<double> red_;
+    DeviceVec<uint16_t> dPb_, WT_;  // BF16 gradient of the current layer, transposed weights
 };
 
 // Dense-grid helpers shared by the trainer and the preconditioner apply.

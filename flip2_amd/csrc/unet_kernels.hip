@@ -7,9 +7,10 @@
 // with M = 16 output channels, N = 16 voxels, K = 16 input channels per
 // instruction.  A wave owns 64 consecutive voxels and all output channels.
 //
-// Backward kernels (data and weight gradients) run in FP32 on the vector
-// ALUs; the weight gradient reduces over voxels with one FP32 atomic add per
-// weight per workgroup.
+// Backward: the data gradient is the same kernel on the BF16 gradient with
+// transposed, mirrored weights (FP32 output); the weight gradient is a GEMM
+// over the voxels on MFMA (BF16 operands, FP32 accumulation, FP32 atomics per
+// chunk); bias gradients are FP32 column sums.
 #include "unet_kernels.hpp"
 
 namespace flip2 {
@@ -29,10 +30,12 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b <<
 // ---------------------------------------------------------------------------
 // conv3x3x3 forward, zero padding
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT>
+// MODE 0: Y (BF16) = act(conv + bias); MODE 1: Yf (FP32) = conv (no bias, no
+// activation), used for the data gradient with transposed, mirrored weights.
+template <int CIN, int COUT, int MODE = 0>
 __global__ void __launch_bounds__(256)
 conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
-                 uint16_t* __restrict__ Y, int n, int relu) {
+                 uint16_t* __restrict__ Y, float* __restrict__ Yf, int n, int relu) {
     constexpr int MB = COUT / 16;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -96,6 +99,11 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt
             if (!valid[j]) continue;
             const long long v = v0 + 16 * j + col;
             const int co = 16 * m + 4 * kq;
+            if (MODE == 1) {
+                *reinterpret_cast<float4*>(Yf + v * COUT + co) =
+                    make_float4(acc[m][j][0], acc[m][j][1], acc[m][j][2], acc[m][j][3]);
+                continue;
+            }
             uint16_t o[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -111,105 +119,151 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt
     }
 }
 
-// ---------------------------------------------------------------------------
-// FP32 backward of conv3x3x3
-// ---------------------------------------------------------------------------
-
-// dX[v][ci] = sum_tap sum_co W[tap][co][ci] * dP[v - off(tap)][co]
-// (dP = gradient w.r.t. the pre-activation of this layer's output).
-// One thread per (voxel, ci); weights of the workgroup's channel staged in LDS.
-__global__ void __launch_bounds__(256)
-conv3_dgrad_kernel(const float* __restrict__ dP, const float* __restrict__ W32, float* __restrict__ dX, int n, int cin,
-                   int cout) {
-    extern __shared__ float s_w[];  // [27][cout] for this block's ci
-    const int ci = blockIdx.y;
-    for (int t = threadIdx.x; t < 27 * cout; t += blockDim.x) {
-        const int tap = t / cout, co = t % cout;
-        s_w[t] = W32[((size_t)tap * cout + co) * cin + ci];
-    }
-    __syncthreads();
-    const long long nvox = (long long)n * n * n;
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox) return;
-    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((long long)n * n));
-    float acc = 0.f;
-    for (int tap = 0; tap < 27; ++tap) {
-        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-        // output voxel u with u + off = v  ->  u = v - off
-        const int xo = x - dx, yo = y - dy, zo = z - dz;
-        if (xo < 0 || yo < 0 || zo < 0 || xo >= n || yo >= n || zo >= n) continue;
-        const float* g = dP + (((long long)zo * n + yo) * n + xo) * cout;
-        const float* w = s_w + tap * cout;
-        for (int co = 0; co < cout; ++co) acc = fmaf(w[co], g[co], acc);
-    }
-    dX[v * cin + ci] = acc;
+// g <- g * (act > 0) in place, and a BF16 copy of the result for the
+// MFMA gradient kernels.
+__global__ void relu_mask_bf16_kernel(float* __restrict__ g, const uint16_t* __restrict__ act,
+                                      uint16_t* __restrict__ gb, long long count) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float v = g[i];
+    if (!(bf2f(act[i]) > 0.f)) v = 0.f;
+    g[i] = v;
+    gb[i] = f2bf(v);
 }
 
-// dW[tap][co][ci] += sum_v dP[v][co] * X[v + off(tap)][ci]; dB[co] += sum_v dP.
-// Workgroup: one tap, a chunk of voxels; thread t handles (co, ci) pairs
-// t, t+256, ...; voxels staged through LDS in batches of 16.
+// dB[c] += sum_v g[v][c]: grid-stride partial sums per workgroup, one atomic
+// per channel per workgroup.
+__global__ void __launch_bounds__(256) bias_grad_kernel(const float* __restrict__ g, long long nvox, int C,
+                                                        float* __restrict__ dB) {
+    __shared__ float s_acc[256];
+    const int c = threadIdx.x % C;
+    const int lanesPerC = 256 / C;  // C divides 256 (16, 32 or 64)
+    const int sub = threadIdx.x / C;
+    float acc = 0.f;
+    for (long long v = (long long)blockIdx.x * lanesPerC + sub; v < nvox; v += (long long)gridDim.x * lanesPerC)
+        acc += g[v * C + c];
+    s_acc[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.f;
+        for (int k = 0; k < lanesPerC; ++k) t += s_acc[k * C + threadIdx.x];
+        atomicAdd(&dB[threadIdx.x], t);
+    }
+}
+
+// WT[t][ci][co] = bf16(W[26 - t][co][ci]): the data gradient is a forward
+// convolution of dP with the transposed weights and mirrored taps.
+__global__ void transpose_weights_kernel(const float* __restrict__ W, uint16_t* __restrict__ WT, int cin, int cout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 27 * cin * cout) return;
+    const int t = i / (cin * cout), r = i % (cin * cout);
+    const int ci = r / cout, co = r % cout;
+    WT[i] = f2bf(W[((26 - t) * cout + co) * cin + ci]);
+}
+
+// dW[tap][co][ci] += sum_v dP[v][co] X[v + off(tap)][ci] on MFMA: a GEMM with
+// M = COUT, N = CIN and the voxels as the reduction dimension.  Workgroup
+// (chunk, tap): 128-voxel slices of its chunk are staged in LDS transposed
+// ([channel][voxel], rows padded against bank conflicts) so each lane's four
+// consecutive voxels are one 8-byte read; each wave owns some of the 16x16
+// output tiles; the chunk's partial sums go to dW with FP32 atomics.
+template <int CIN, int COUT>
 __global__ void __launch_bounds__(256)
-conv3_wgrad_kernel(const float* __restrict__ dP, const uint16_t* __restrict__ X, float* __restrict__ dW,
-                   float* __restrict__ dB, int n, int cin, int cout, int chunk) {
-    extern __shared__ float s_mem[];
-    constexpr int B = 16;
-    float* s_g = s_mem;              // [B][cout]
-    float* s_x = s_mem + B * cout;   // [B][cin]
+conv3_wgrad_mfma_kernel(const uint16_t* __restrict__ dP, const uint16_t* __restrict__ X, float* __restrict__ dW,
+                        int n, long long chunk) {
+    constexpr int KV = 128, KP = KV + 4;
+    constexpr int MT = COUT / 16, NT = CIN / 16, T = MT * NT;
+    // with fewer than 4 output tiles the waves also split the reduction
+    constexpr int KSPLIT = T >= 4 ? 1 : (T == 1 ? 4 : (T == 2 ? 2 : 1));
+    constexpr int TW = KSPLIT > 1 ? 1 : (T + 3) / 4;
+    __shared__ __align__(16) uint16_t sP[COUT * KP];
+    __shared__ __align__(16) uint16_t sX[CIN * KP];
     const int tap = blockIdx.y;
     const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
     const long long nvox = (long long)n * n * n;
     const long long vBeg = (long long)blockIdx.x * chunk;
     const long long vEnd = vBeg + chunk < nvox ? vBeg + chunk : nvox;
-    const int pairs = cin * cout;
-    constexpr int MAXP = 24;  // pairs per thread (cin*cout <= 6144)
-    float acc[MAXP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    v4f acc[TW];
 #pragma unroll
-    for (int k = 0; k < MAXP; ++k) acc[k] = 0.f;
-    float bacc = 0.f;
-    for (long long vb = vBeg; vb < vEnd; vb += B) {
+    for (int t = 0; t < TW; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (long long vb = vBeg; vb < vEnd; vb += KV) {
         __syncthreads();
-        for (int t = threadIdx.x; t < B * cout; t += blockDim.x) {
-            const long long v = vb + t / cout;
-            s_g[t] = v < vEnd ? dP[v * cout + t % cout] : 0.f;
+        for (int i = threadIdx.x; i < KV * COUT; i += 256) {
+            const int k = i / COUT, c = i % COUT;
+            const long long v = vb + k;
+            sP[c * KP + k] = v < vEnd ? dP[v * COUT + c] : (uint16_t)0;
         }
-        for (int t = threadIdx.x; t < B * cin; t += blockDim.x) {
-            const long long v = vb + t / cin;
-            float xv = 0.f;
+        for (int i = threadIdx.x; i < KV * CIN; i += 256) {
+            const int k = i / CIN, c = i % CIN;
+            const long long v = vb + k;
+            uint16_t xv = 0;
             if (v < vEnd) {
                 const int x = (int)(v % n) + dx, y = (int)((v / n) % n) + dy, z = (int)(v / ((long long)n * n)) + dz;
                 if (x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n)
-                    xv = bf2f(X[(((long long)z * n + y) * n + x) * cin + t % cin]);
+                    xv = X[(((long long)z * n + y) * n + x) * CIN + c];
             }
-            s_x[t] = xv;
+            sX[c * KP + k] = xv;
         }
         __syncthreads();
+        if (KSPLIT > 1) {
+            const int t = wave % T, ks = wave / T;
+            const int m = t / NT, nn = t % NT;
+            if (ks < KSPLIT) {
 #pragma unroll
-        for (int k = 0; k < MAXP; ++k) {
-            const int pidx = threadIdx.x + k * 256;
-            if (pidx >= pairs) break;
-            const int co = pidx / cin, ci = pidx % cin;
-            float a = acc[k];
-            for (int b = 0; b < B; ++b) a = fmaf(s_g[b * cout + co], s_x[b * cin + ci], a);
-            acc[k] = a;
+                for (int k0 = 16 * ks; k0 < KV; k0 += 16 * KSPLIT) {
+                    const v4s a = *reinterpret_cast<const v4s*>(&sP[(16 * m + col) * KP + k0 + 4 * kq]);
+                    const v4s b = *reinterpret_cast<const v4s*>(&sX[(16 * nn + col) * KP + k0 + 4 * kq]);
+                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[0], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k0 = 0; k0 < KV; k0 += 16) {
+#pragma unroll
+                for (int tt = 0; tt < TW; ++tt) {
+                    const int t = wave + 4 * tt;
+                    if (t >= T) break;
+                    const int m = t / NT, nn = t % NT;
+                    const v4s a = *reinterpret_cast<const v4s*>(&sP[(16 * m + col) * KP + k0 + 4 * kq]);
+                    const v4s b = *reinterpret_cast<const v4s*>(&sX[(16 * nn + col) * KP + k0 + 4 * kq]);
+                    acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tt], 0, 0, 0);
+                }
+            }
         }
-        if (tap == 13 && (int)threadIdx.x < cout)
-            for (int b = 0; b < B; ++b) bacc += s_g[b * cout + threadIdx.x];
+    }
+    if (KSPLIT > 1) {
+        // sum the KSPLIT partial tiles through LDS (reusing sP), then one
+        // wave per tile adds the result to dW
+        __syncthreads();
+        float* red = reinterpret_cast<float*>(sP);  // 4 waves x 256 floats <= sizeof(sP)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave * 256 + (4 * kq + r) * 16 + col] = acc[0][r];
+        __syncthreads();
+        if (wave < T) {
+            const int t = wave, m = t / NT, nn = t % NT;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = 0.f;
+                for (int ks = 0; ks < KSPLIT; ++ks) v += red[(t + ks * T) * 256 + (4 * kq + r) * 16 + col];
+                const int co = 16 * m + 4 * kq + r, ci = 16 * nn + col;
+                atomicAdd(&dW[((size_t)tap * COUT + co) * CIN + ci], v);
+            }
+        }
+        return;
     }
 #pragma unroll
-    for (int k = 0; k < MAXP; ++k) {
-        const int pidx = threadIdx.x + k * 256;
-        if (pidx >= pairs) break;
-        const int co = pidx / cin, ci = pidx % cin;
-        atomicAdd(&dW[((size_t)tap * cout + co) * cin + ci], acc[k]);
+    for (int tt = 0; tt < TW; ++tt) {
+        const int t = wave + 4 * tt;
+        if (t >= T) break;
+        const int m = t / NT, nn = t % NT;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = 16 * m + 4 * kq + r, ci = 16 * nn + col;
+            atomicAdd(&dW[((size_t)tap * COUT + co) * CIN + ci], acc[tt][r]);
+        }
     }
-    if (tap == 13 && (int)threadIdx.x < cout) atomicAdd(&dB[threadIdx.x], bacc);
-}
-
-// g <- g * (act > 0)  (ReLU derivative from the stored BF16 activation)
-__global__ void relu_mask_kernel(float* __restrict__ g, const uint16_t* __restrict__ act, long long count) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    if (!(bf2f(act[i]) > 0.f)) g[i] = 0.f;
 }
 
 // ---------------------------------------------------------------------------
@@ -371,11 +425,12 @@ void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, u
                       bool relu, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
     const dim3 grid((unsigned)((nvox + 255) / 256));
-#define CONV_CASE(CI, CO)                                                                                  \
-    if (cin == CI && cout == CO) {                                                                         \
-        hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO>), grid, dim3(256), 0, s, X, W, bias, Y, n, relu ? 1 : 0); \
-        FLIP2_LAUNCH_CHECK();                                                                              \
-        return;                                                                                            \
+#define CONV_CASE(CI, CO)                                                                                   \
+    if (cin == CI && cout == CO) {                                                                          \
+        hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, 0>), grid, dim3(256), 0, s, X, W, bias, Y, (float*)nullptr, \
+                           n, relu ? 1 : 0);                                                                \
+        FLIP2_LAUNCH_CHECK();                                                                               \
+        return;                                                                                             \
     }
     CONV_CASE(16, 16)
     CONV_CASE(16, 32)
@@ -390,26 +445,66 @@ void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, u
     throw Error(kErrArgument, "conv3: unsupported channel pair");
 }
 
-void launch_conv3_dgrad(const float* dP, const float* W32, float* dX, int n, int cin, int cout, hipStream_t s) {
-    const long long nvox = (long long)n * n * n;
-    hipLaunchKernelGGL(conv3_dgrad_kernel, dim3((unsigned)((nvox + 255) / 256), cin), dim3(256),
-                       27 * cout * sizeof(float), s, dP, W32, dX, n, cin, cout);
-    FLIP2_LAUNCH_CHECK();
-}
-
-void launch_conv3_wgrad(const float* dP, const uint16_t* X, float* dW, float* dB, int n, int cin, int cout,
+void launch_conv3_dgrad(const uint16_t* dPb, const float* W32, uint16_t* WT, float* dX, int n, int cin, int cout,
                         hipStream_t s) {
-    FLIP2_REQUIRE(cin * cout <= 24 * 256, kErrArgument, "wgrad: too many channel pairs");
     const long long nvox = (long long)n * n * n;
-    const int chunk = nvox >= (1 << 20) ? 8192 : (nvox >= 4096 ? 1024 : 64);
-    const unsigned blocks = (unsigned)((nvox + chunk - 1) / chunk);
-    hipLaunchKernelGGL(conv3_wgrad_kernel, dim3(blocks, 27), dim3(256), 16 * (cin + cout) * sizeof(float), s, dP, X, dW,
-                       dB, n, cin, cout, chunk);
+    const int nw = 27 * cin * cout;
+    hipLaunchKernelGGL(transpose_weights_kernel, g1(nw, 256), dim3(256), 0, s, W32, WT, cin, cout);
     FLIP2_LAUNCH_CHECK();
+    const dim3 grid((unsigned)((nvox + 255) / 256));
+    // the forward kernel with the roles of the channel counts swapped
+#define DGRAD_CASE(CI, CO)                                                                                      \
+    if (cin == CI && cout == CO) {                                                                              \
+        hipLaunchKernelGGL((conv3_fwd_kernel<CO, CI, 1>), grid, dim3(256), 0, s, dPb, WT, (const float*)nullptr, \
+                           (uint16_t*)nullptr, dX, n, 0);                                                       \
+        FLIP2_LAUNCH_CHECK();                                                                                   \
+        return;                                                                                                 \
+    }
+    DGRAD_CASE(16, 16)
+    DGRAD_CASE(16, 32)
+    DGRAD_CASE(32, 32)
+    DGRAD_CASE(32, 64)
+    DGRAD_CASE(64, 64)
+    DGRAD_CASE(96, 32)
+    DGRAD_CASE(48, 16)
+#undef DGRAD_CASE
+    throw Error(kErrArgument, "conv3 dgrad: unsupported channel pair");
 }
 
-void launch_relu_mask(float* g, const uint16_t* act, long long count, hipStream_t s) {
-    hipLaunchKernelGGL(relu_mask_kernel, g1(count, 256), dim3(256), 0, s, g, act, count);
+void launch_conv3_wgrad(const uint16_t* dPb, const float* dP, const uint16_t* X, float* dW, float* dB, int n, int cin,
+                        int cout, hipStream_t s) {
+    const long long nvox = (long long)n * n * n;
+    // bias: column sums of dP
+    const int bblocks = (int)std::min<long long>(1024, (nvox + 255) / 256);
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(bblocks), dim3(256), 0, s, dP, nvox, cout, dB);
+    FLIP2_LAUNCH_CHECK();
+    // chunks: enough workgroups to fill the GPU, few enough that the atomics
+    // (27 * cout * cin per chunk) stay around 4M
+    long long chunks = std::max<long long>(1, 4000000LL / (27LL * cin * cout));
+    chunks = std::min<long long>(chunks, (nvox + 127) / 128);
+    long long chunk = (nvox + chunks - 1) / chunks;
+    chunk = (chunk + 127) / 128 * 128;
+    chunks = (nvox + chunk - 1) / chunk;
+#define WGRAD_CASE(CI, CO)                                                                                      \
+    if (cin == CI && cout == CO) {                                                                              \
+        hipLaunchKernelGGL((conv3_wgrad_mfma_kernel<CI, CO>), dim3((unsigned)chunks, 27), dim3(256), 0, s, dPb, X, \
+                           dW, n, chunk);                                                                       \
+        FLIP2_LAUNCH_CHECK();                                                                                   \
+        return;                                                                                                 \
+    }
+    WGRAD_CASE(16, 16)
+    WGRAD_CASE(16, 32)
+    WGRAD_CASE(32, 32)
+    WGRAD_CASE(32, 64)
+    WGRAD_CASE(64, 64)
+    WGRAD_CASE(96, 32)
+    WGRAD_CASE(48, 16)
+#undef WGRAD_CASE
+    throw Error(kErrArgument, "conv3 wgrad: unsupported channel pair");
+}
+
+void launch_relu_mask_bf16(float* g, const uint16_t* act, uint16_t* gb, long long count, hipStream_t s) {
+    hipLaunchKernelGGL(relu_mask_bf16_kernel, g1(count, 256), dim3(256), 0, s, g, act, gb, count);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s) {

@@ -11,12 +11,15 @@ namespace nn {
 // W: [27][cout][cin] BF16 (tap = (dz*3 + dy)*3 + dx, offsets d-1).
 void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin, int cout,
                       bool relu, hipStream_t s);
-// dX = conv3x3x3^T(dP) with FP32 weights (same layout).
-void launch_conv3_dgrad(const float* dP, const float* W32, float* dX, int n, int cin, int cout, hipStream_t s);
-// dW += sum_v dP (x) X(v + tap), dB += sum_v dP  (FP32 atomics).
-void launch_conv3_wgrad(const float* dP, const uint16_t* X, float* dW, float* dB, int n, int cin, int cout,
+// dX (FP32) = conv3x3x3^T(dP) from the BF16 gradient dPb; WT (27*cin*cout
+// BF16 scratch) receives the transposed, mirrored weights.
+void launch_conv3_dgrad(const uint16_t* dPb, const float* W32, uint16_t* WT, float* dX, int n, int cin, int cout,
                         hipStream_t s);
-void launch_relu_mask(float* g, const uint16_t* act, long long count, hipStream_t s);
+// dW += sum_v dP (x) X(v + tap) (MFMA on dPb and X), dB += sum_v dP (FP32).
+void launch_conv3_wgrad(const uint16_t* dPb, const float* dP, const uint16_t* X, float* dW, float* dB, int n, int cin,
+                        int cout, hipStream_t s);
+// g <- g * (act > 0) in place; gb = BF16 copy.
+void launch_relu_mask_bf16(float* g, const uint16_t* act, uint16_t* gb, long long count, hipStream_t s);
 void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s);
 void launch_avgpool_bwd(const float* dP, float* dX, int n, int C, hipStream_t s);  // dX += ...
 void launch_upcat(const uint16_t* A, const uint16_t* B, uint16_t* U, int n, int Ca, int Cb, hipStream_t s);
