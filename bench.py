@@ -34,6 +34,11 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s; ms/frame pressure solve @1e-6 rel. residual, learned vs PCG"
 PEAK_HBM_GBPS = 8000.0
 
+# HBM bytes per dispatch of the default workload (2^24 particles, 256^3),
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs scaled by the same
+# counters on a plain copy of known size (profiles/transfer_16M_256_pmc.md).
+TRAFFIC_BYTES = {"bin": 604.0e6, "sort_pass": 272.6e6, "gather": 2747.9e6, "scan": 137.1e6, "p2g": 962.1e6}
+
 
 def algorithmic_bytes(name, n, cells, occupied, passes):
     """Bytes a kernel must move at minimum (reads + writes), per launch."""
@@ -319,7 +324,10 @@ def main():
                                "+ exclusive scan over 16,777,216 counts -> trilinear MAC P2G",
                    "particles": n, "grid": gn, "brick": 8, "seed": 2},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None},
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                     "traffic": (int(TRAFFIC_BYTES[dom]) if (n == 1 << 24 and gn == 256 and dom in TRAFFIC_BYTES)
+                                 else None),
+                     "alg_bytes": table[dom]["alg_bytes"]},
         "kernels": table,
         "cpu_baseline": None,
     }

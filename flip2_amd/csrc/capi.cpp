@@ -540,7 +540,7 @@ int flip2_ns_p2g(const float* d_records, const uint32_t* d_cell_start, flip2_gri
         const hipStream_t st = as_stream(stream);
         const float4* rec = reinterpret_cast<const float4*>(d_records);
         uint32_t* stats = nullptr;
-        FLIP2_HIP(hipMallocAsync(reinterpret_cast<void**>(&stats), 2 * sizeof(uint32_t), st));
+        FLIP2_HIP(hipMallocAsync(reinterpret_cast<void**>(&stats), sizeof(uint32_t), st));
         flip2::ns::launch_p2g_stats(rec, d_cell_start, g.sizeX * g.sizeY * g.sizeZ, stats, st);
         flip2::ns::launch_p2g(rec, d_cell_start, g, stats, d_u, d_v, d_w, d_wu, d_wv, d_ww, st);
         FLIP2_HIP(hipFreeAsync(stats, st));

@@ -7,10 +7,11 @@
 // P2G accumulates in 64-bit fixed point with integer LDS atomics: on gfx950
 // an LDS float atomic add costs tens of times an integer one, and integer
 // sums are exact, so the result does not depend on the order in which
-// particles arrive (it is deterministic).  The scale is chosen per launch
-// from the largest particle speed and the largest per-cell particle count
-// (both gathered by the earlier kernels of the step) so that no sum can
-// exceed 2^62; at 8 particles per cell and |v| <= 1 the resolution is 2^-55.
+// particles arrive (it is deterministic).  The scale is chosen per brick
+// from the number of particles the brick's workgroup scatters (a face's
+// weight sum cannot exceed it) and the largest particle speed of the step
+// (reduced by the gather kernel), so that no sum can exceed 2^62; at ~1000
+// particles per brick neighbourhood and |v| <= 1 the resolution is ~2^-52.
 //
 // Brick-tiled layout: cell (i,j,k) of an n^3 grid (n a multiple of 8) has
 //   index ((bk*nbY + bj)*nbX + bi)*512 + (k&7)*64 + (j&7)*8 + (i&7)
@@ -57,7 +58,7 @@ __global__ void bin_kernel(const float* __restrict__ x, const float* __restrict_
 __global__ void bin_pack_kernel(Float6 s, uint32_t n, Grid g, uint32_t* __restrict__ keys, float4* __restrict__ rec,
                                 uint32_t* __restrict__ stats) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0 && stats) { stats[0] = 0u; stats[1] = 0u; }
+    if (p == 0 && stats) stats[0] = 0u;
     if (p >= n) return;
     const float x = s.p[0][p], y = s.p[1][p], z = s.p[2][p];
     keys[p] = cell_key(x, y, z, g);
@@ -69,7 +70,7 @@ __global__ void bin_pack_kernel(Float6 s, uint32_t n, Grid g, uint32_t* __restri
 __global__ void bin_rec_kernel(const float4* __restrict__ rec, uint32_t n, Grid g, uint32_t* __restrict__ keys,
                                uint32_t* __restrict__ stats) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0 && stats) { stats[0] = 0u; stats[1] = 0u; }
+    if (p == 0 && stats) stats[0] = 0u;
     if (p >= n) return;
     const float4 a = rec[2 * p];
     keys[p] = cell_key(a.x, a.y, a.z, g);
@@ -94,18 +95,29 @@ __device__ inline void block_max_to(uint32_t v, uint32_t* target) {
 
 // dst[i] = src[perm[i]] (records); stats[0] = max |velocity component|
 // (float bits) when stats is given.
+// Each thread moves 4 records, so 8 independent random 16-byte loads per
+// lane are in flight (the reads through the permutation are the cost).
 __global__ void gather_rec_kernel(const uint32_t* __restrict__ perm, uint32_t n, const float4* __restrict__ src,
                                   float4* __restrict__ dst, uint32_t* __restrict__ stats) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4u;
     float vmax = 0.f;
-    if (i < n) {
-        const uint32_t j = perm[i];
-        if (j < n) {
-            const float4 a = src[2 * j];
-            const float4 b = src[2 * j + 1];
-            dst[2 * i] = a;
-            dst[2 * i + 1] = b;
-            vmax = fmaxf(fabsf(a.w), fmaxf(fabsf(b.x), fabsf(b.y)));
+    uint32_t j[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) j[k] = (i0 + k < n) ? perm[i0 + k] : n;
+    float4 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (j[k] < n) {
+            a[k] = src[2 * j[k]];
+            b[k] = src[2 * j[k] + 1];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (j[k] < n) {
+            dst[2 * (i0 + k)] = a[k];
+            dst[2 * (i0 + k) + 1] = b[k];
+            vmax = fmaxf(vmax, fmaxf(fabsf(a[k].w), fmaxf(fabsf(b[k].x), fabsf(b[k].y))));
         }
     }
     if (stats) block_max_to(__float_as_uint(vmax), &stats[0]);
@@ -134,46 +146,31 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ a, uint32_t count) {
 
 // counts[key] = length of key's run in the sorted key array (written by the
 // run's first element; cells without particles keep the zero fill).
-// counts[key] = run length; stats[1] = largest run (atomically) when given.
+// counts[key] = run length of key in the sorted keys.
 __global__ void run_length_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t numCells,
-                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ stats) {
+                                  uint32_t* __restrict__ counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t run = 0;
-    if (i < n) {
-        const uint32_t k = keys[i];
-        if (i == 0 || keys[i - 1] != k) {
-            uint32_t e = i + 1;
-            while (e < n && keys[e] == k) ++e;
-            if (k < numCells) counts[k] = e - i;
-            run = e - i;
-        }
-    }
-    if (stats) block_max_to(run, &stats[1]);
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (i > 0 && keys[i - 1] == k) return;
+    uint32_t e = i + 1;
+    while (e < n && keys[e] == k) ++e;
+    if (k < numCells) counts[k] = e - i;
 }
 
 // Statistics for a P2G call whose inputs did not come through the pipeline:
-// stats[0] = max |velocity component| (float bits), stats[1] = max cell count.
+// stats[0] = max |velocity component| (float bits).
 __global__ void p2g_stats_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart,
                                  uint32_t numCells, uint32_t* __restrict__ stats) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n = cellStart[numCells];
     float vmax = 0.f;
-    uint32_t cmax = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float4 a = rec[2 * i];
         const float4 b = rec[2 * i + 1];
         vmax = fmaxf(vmax, fmaxf(fabsf(a.w), fmaxf(fabsf(b.x), fabsf(b.y))));
     }
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < numCells; c += stride)
-        cmax = max(cmax, cellStart[c + 1] - cellStart[c]);
-    for (int d = 32; d >= 1; d >>= 1) {
-        vmax = fmaxf(vmax, __shfl_xor(vmax, d, 64));
-        cmax = max(cmax, (uint32_t)__shfl_xor(cmax, d, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (vmax > 0.f) atomicMax(&stats[0], __float_as_uint(vmax));
-        if (cmax) atomicMax(&stats[1], cmax);
-    }
+    block_max_to(__float_as_uint(vmax), &stats[0]);
 }
 
 namespace {
@@ -319,12 +316,13 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
     if (tid == 0) s_off[kShellRuns + 1] = total;
     __syncthreads();
 
-    // fixed-point scales: a face sums at most 27 cells' particles with w <= 1
-    const uint32_t maxCount = stats[1] > 0u ? stats[1] : 1u;
+    // fixed-point scales: a face's weight sum is at most the number of
+    // particles this workgroup scatters (each w <= 1)
+    const float bound = (float)(total > 0u ? total : 1u);
     const float vmax = __uint_as_float(stats[0]);
-    const int ew = ilogbf(27.0f * (float)maxCount) + 1;
+    const int ew = ilogbf(bound) + 1;
     const float scaleW = ldexpf(1.0f, 62 - ew);
-    const int em = vmax > 0.f ? ilogbf(27.0f * (float)maxCount * vmax) + 1 : ew;
+    const int em = vmax > 0.f ? ilogbf(bound * vmax) + 1 : ew;
     const float scaleM = ldexpf(1.0f, min(62 - em, 126));
 
     const double inv = 1.0 / (double)g.cellSize;
@@ -421,7 +419,7 @@ void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint3
 void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, uint32_t* stats,
                        hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(gather_rec_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, src, dst, stats);
+    hipLaunchKernelGGL(gather_rec_kernel, grid1d(ceil_div(n, 4u), 256), dim3(256), 0, s, perm, n, src, dst, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -437,18 +435,17 @@ void launch_unpack(const float4* rec, uint32_t n, Float6Out dst, hipStream_t s) 
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, uint32_t* stats,
-                        hipStream_t s) {
+void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, hipStream_t s) {
     hipLaunchKernelGGL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
     FLIP2_LAUNCH_CHECK();
     if (!n) return;
-    hipLaunchKernelGGL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts, stats);
+    hipLaunchKernelGGL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_p2g_stats(const float4* sortedRec, const uint32_t* cellStart, uint32_t numCells, uint32_t* stats,
                       hipStream_t s) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, stats, 2u);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, stats, 1u);
     FLIP2_LAUNCH_CHECK();
     hipLaunchKernelGGL(p2g_stats_kernel, dim3(1024), dim3(256), 0, s, sortedRec, cellStart, numCells, stats);
     FLIP2_LAUNCH_CHECK();

@@ -17,9 +17,8 @@ struct Float6Out {
 // Brick-tiled cell keys of FP32 positions, clamped into the grid.
 void launch_bin(const float* x, const float* y, const float* z, uint32_t n, Grid g, uint32_t* keys, hipStream_t s);
 // Particle records: 2 x float4 per particle, (x, y, z, vx), (vy, vz, 0, 0).
-// stats (2 x u32, may be null) are the P2G statistics of the step: the bin
-// kernels zero them, the gather records max |velocity component| (float
-// bits) in stats[0], the cell counts the largest count in stats[1].
+// stats (u32, may be null) is the P2G statistic of the step: the bin kernels
+// zero it, the gather records max |velocity component| (float bits) in it.
 void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, uint32_t* stats, hipStream_t s);
 void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint32_t* stats, hipStream_t s);
 // dst[i] = src[perm[i]] (records)
@@ -30,8 +29,7 @@ void launch_unpack(const float4* rec, uint32_t n, Float6Out dst, hipStream_t s);
 // counts[0..numCells] zeroed, then counts[key] = run length of key in the
 // sorted keys (counts[numCells] stays 0, so an exclusive scan over
 // numCells + 1 entries ends with n).
-void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, uint32_t* stats,
-                        hipStream_t s);
+void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, hipStream_t s);
 // The statistics for a P2G whose inputs did not come through the pipeline.
 // (the particle count is cellStart[numCells])
 void launch_p2g_stats(const float4* sortedRec, const uint32_t* cellStart, uint32_t numCells, uint32_t* stats,

@@ -25,7 +25,7 @@ P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g) : grid_(g), maxPa
     sorted_.resizeAsync(2 * maxParticles, nullptr);
     const size_t ws = std::max(sort_workspace_bytes(maxParticles, 0, keyBits_), scan_workspace_bytes(numCells_ + 1));
     ws_.resizeAsync((uint32_t)ws, nullptr);
-    stats_.resizeAsync(2u, nullptr);
+    stats_.resizeAsync(1u, nullptr);
     stats_.zeroDevice();
 }
 
@@ -65,7 +65,7 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
         ns::launch_gather_rec(perm_.devPtr(), n, records, sorted_.devPtr(), stats_.devPtr(), s);
     }
     if (timed) FLIP2_HIP(hipEventRecord(ev[3 + passes], s));
-    ns::launch_cell_counts(keysSorted_.devPtr(), n, numCells_, counts_.devPtr(), stats_.devPtr(), s);
+    ns::launch_cell_counts(keysSorted_.devPtr(), n, numCells_, counts_.devPtr(), s);
     if (timed) FLIP2_HIP(hipEventRecord(ev[4 + passes], s));
     scan_u32(counts_.devPtr(), cellStart_.devPtr(), numCells_ + 1, true, ws_.devPtr(), ws_.size(), nullptr, s);
     if (timed) FLIP2_HIP(hipEventRecord(ev[5 + passes], s));

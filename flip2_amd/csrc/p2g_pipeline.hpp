@@ -53,7 +53,7 @@ private:
     DeviceVec<uint32_t> keys_, keysSorted_, perm_, keysAlt_, permAlt_, counts_, cellStart_;
     DeviceVec<float4> packed_, sorted_;
     DeviceVec<uint8_t> ws_;
-    DeviceVec<uint32_t> stats_;  // P2G statistics of the step (max |v| bits, max cell count)
+    DeviceVec<uint32_t> stats_;  // P2G statistic of the step: max |velocity component| (float bits)
     bool timing_ = false;
     int pending_ = 0;
     std::vector<std::vector<hipEvent_t>> events_;

@@ -70,15 +70,26 @@ sort_histogram_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t be
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kHistThreads) s_hist[i] = 0;
     __syncthreads();
+    // each thread reads kHistItems keys as 16-byte vectors (rows of 256 x 4)
     const uint32_t base = blockIdx.x * (uint32_t)(kHistThreads * kHistItems);
-    for (int k = 0; k < kHistItems; ++k) {
-        const uint32_t i = base + (uint32_t)k * kHistThreads + threadIdx.x;
-        if (i < n) {
-            const uint32_t key = keys[i];
+    const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0;
+    for (int k = 0; k < kHistItems / 4; ++k) {
+        const uint32_t i = base + ((uint32_t)k * kHistThreads + threadIdx.x) * 4u;
+        uint32_t key4[4];
+        if (vec && i + 4 <= n) {
+            const uint4 v = *reinterpret_cast<const uint4*>(keys + i);
+            key4[0] = v.x; key4[1] = v.y; key4[2] = v.z; key4[3] = v.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) key4[e] = (i + e < n) ? keys[i + e] : 0u;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (i + e >= n) break;
             for (uint32_t p = 0; p < passes; ++p) {
                 const uint32_t shift = beginBit + 8u * p;
                 const uint32_t bits = min(8u, endBit - shift);
-                const uint32_t d = digit_of(key, shift, (1u << bits) - 1u);
+                const uint32_t d = digit_of(key4[e], shift, (1u << bits) - 1u);
                 atomicAdd(&s_hist[p * kRadix + d], 1u);
             }
         }
@@ -190,20 +201,32 @@ sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict
         s_tileExcl[d] = waveOff + incl - tileCount;
     }
 
-    // Per-digit look-back over the earlier tiles.
+    // Per-digit look-back over the earlier tiles, four tiles per step (their
+    // status loads are independent, so they are in flight together).
     uint32_t prefix = 0;
     if (tile > 0) {
         int64_t j = (int64_t)tile - 1;
         for (;;) {
-            const uint32_t* p = status + (size_t)j * kRadix + d;
-            uint32_t s = digit_status_load(p);
-            while ((s >> kDigitFlagShift) == 0) {
-                __builtin_amdgcn_s_sleep(1);
-                s = digit_status_load(p);
+            uint32_t st[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                st[q] = (j - q >= 0) ? digit_status_load(status + (size_t)(j - q) * kRadix + d) : kDigitInclusive;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                while ((st[q] >> kDigitFlagShift) == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    st[q] = digit_status_load(status + (size_t)(j - q) * kRadix + d);
+                }
             }
-            prefix += s & kDigitValueMask;
-            if ((s >> kDigitFlagShift) == 2u || j == 0) break;
-            --j;
+            bool done = false;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (done) break;
+                prefix += st[q] & kDigitValueMask;
+                done = (st[q] >> kDigitFlagShift) == 2u;
+            }
+            if (done) break;
+            j -= 4;
         }
         digit_status_store(myStatus, kDigitInclusive | (prefix + tileCount));
     }

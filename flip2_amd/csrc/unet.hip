@@ -40,23 +40,50 @@ __global__ void clear_input_kernel(uint16_t* __restrict__ in0, uint8_t* __restri
     mask[v] = 0;
 }
 
-// red[0] += sum r^2, red[1] += fluid count (active layout)
-__global__ void active_norm_kernel(const uint8_t* __restrict__ cellType, const float* __restrict__ r, uint32_t cells,
-                                   double* __restrict__ red) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double a = 0.0, c = 0.0;
-    if (i < cells && cellType[i] == 1) {
-        a = (double)r[i] * (double)r[i];
-        c = 1.0;
-    }
+// The reductions below run grid-stride over a bounded number of workgroups
+// and add one partial pair per workgroup: atomics on one address serialize,
+// so their count, not the data, would otherwise set the time.
+constexpr int kRedBlocks = 512;
+
+__device__ inline void block_add2(double a, double c, double* red) {
+    __shared__ double s_a[4], s_c[4];
     for (int d = 32; d >= 1; d >>= 1) {
         a += __shfl_xor(a, d, 64);
         c += __shfl_xor(c, d, 64);
     }
-    if ((threadIdx.x & 63) == 0 && (a != 0.0 || c != 0.0)) {
-        atomicAdd(&red[0], a);
-        atomicAdd(&red[1], c);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_a[wave] = a;
+        s_c[wave] = c;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tc = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            ta += s_a[w];
+            tc += s_c[w];
+        }
+        if (ta != 0.0 || tc != 0.0) {
+            atomicAdd(&red[0], ta);
+            atomicAdd(&red[1], tc);
+        }
+    }
+}
+
+inline dim3 red_grid(long long n) { return dim3((unsigned)std::min<long long>(kRedBlocks, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); }
+
+// red[0] += sum r^2, red[1] += fluid count (active layout)
+__global__ void __launch_bounds__(256) active_norm_kernel(const uint8_t* __restrict__ cellType,
+                                                          const float* __restrict__ r, uint32_t cells,
+                                                          double* __restrict__ red) {
+    double a = 0.0, c = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
+        if (cellType[i] == 1) {
+            a += (double)r[i] * (double)r[i];
+            c += 1.0;
+        }
+    }
+    block_add2(a, c, red);
 }
 
 __device__ inline float rms_scale(const double* red) {
@@ -127,27 +154,22 @@ __global__ void dense_apply_kernel(const uint16_t* __restrict__ coefD, const flo
     y[v] = dense_A(coefD, x, n, v);
 }
 
-__global__ void residual_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ z,
-                                const float* __restrict__ bD, float* __restrict__ res, int n, double* __restrict__ red) {
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) residual_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ z,
+                                                       const float* __restrict__ bD, float* __restrict__ res, int n,
+                                                       double* __restrict__ red) {
     double a = 0.0, c = 0.0;
-    if (v < (long long)n * n * n) {
+    const long long nvox = (long long)n * n * n;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long long)gridDim.x * blockDim.x) {
         float rv = 0.f;
         if (coefD[v]) {
             rv = dense_A(coefD, z, n, v) - bD[v];
-            c = (double)bD[v] * (double)bD[v];
+            c += (double)bD[v] * (double)bD[v];
         }
         res[v] = rv;
-        a = (double)rv * (double)rv;
+        a += (double)rv * (double)rv;
     }
-    for (int d = 32; d >= 1; d >>= 1) {
-        a += __shfl_xor(a, d, 64);
-        c += __shfl_xor(c, d, 64);
-    }
-    if ((threadIdx.x & 63) == 0 && (a != 0.0 || c != 0.0)) {
-        atomicAdd(&red[0], a);
-        atomicAdd(&red[1], c);
-    }
+    block_add2(a, c, red);
 }
 
 __global__ void loss_grad_kernel(const uint16_t* __restrict__ coefD, const float* __restrict__ res,
@@ -158,22 +180,18 @@ __global__ void loss_grad_kernel(const uint16_t* __restrict__ coefD, const float
     dOut[v] = k * dense_A(coefD, res, n, v);
 }
 
-__global__ void dense_norm_kernel(const float* __restrict__ bD, const uint16_t* __restrict__ coefD, long long nvox,
-                                  double* __restrict__ red) {
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) dense_norm_kernel(const float* __restrict__ bD,
+                                                         const uint16_t* __restrict__ coefD, long long nvox,
+                                                         double* __restrict__ red) {
     double a = 0.0, c = 0.0;
-    if (v < nvox && coefD[v]) {
-        a = (double)bD[v] * (double)bD[v];
-        c = 1.0;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long long)gridDim.x * blockDim.x) {
+        if (coefD[v]) {
+            a += (double)bD[v] * (double)bD[v];
+            c += 1.0;
+        }
     }
-    for (int d = 32; d >= 1; d >>= 1) {
-        a += __shfl_xor(a, d, 64);
-        c += __shfl_xor(c, d, 64);
-    }
-    if ((threadIdx.x & 63) == 0 && (a != 0.0 || c != 0.0)) {
-        atomicAdd(&red[0], a);
-        atomicAdd(&red[1], c);
-    }
+    block_add2(a, c, red);
 }
 
 __global__ void dense_input_kernel(const float* __restrict__ bD, const uint16_t* __restrict__ coefD, long long nvox,
@@ -252,7 +270,7 @@ void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, cons
     const uint32_t cells = numActive * 512u;
     hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
     hipLaunchKernelGGL(clear_input_kernel, g1(nvox, 256), dim3(256), 0, s, in0, mask, nvox);
-    hipLaunchKernelGGL(active_norm_kernel, g1(cells, 256), dim3(256), 0, s, cellType, r, cells, red);
+    hipLaunchKernelGGL(active_norm_kernel, red_grid(cells), dim3(256), 0, s, cellType, r, cells, red);
     hipLaunchKernelGGL(scatter_input_kernel, g1(cells ? cells : 1, 256), dim3(256), 0, s, activeBricks, cellType, r,
                        cells, n, in0, mask, scale, red);
     FLIP2_LAUNCH_CHECK();
@@ -305,7 +323,7 @@ void launch_dense_apply_A(const uint16_t* coefD, const float* x, float* y, int n
 void launch_residual(const uint16_t* coefD, const float* z, const float* bD, float* res, int n, double* red,
                      hipStream_t s) {
     hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
-    hipLaunchKernelGGL(residual_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, z, bD, res, n, red);
+    hipLaunchKernelGGL(residual_kernel, red_grid((long long)n * n * n), dim3(256), 0, s, coefD, z, bD, res, n, red);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -318,7 +336,7 @@ void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t*
                         double* red, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
     hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
-    hipLaunchKernelGGL(dense_norm_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, red);
+    hipLaunchKernelGGL(dense_norm_kernel, red_grid(nvox), dim3(256), 0, s, bD, coefD, nvox, red);
     hipLaunchKernelGGL(dense_input_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, in0, mask, scale, red);
     FLIP2_LAUNCH_CHECK();
 }

@@ -5,7 +5,8 @@
 // BF16 weights and activations, FP32 accumulation, bias + ReLU fused in the
 // epilogue.  D[cout][voxel] = sum_{tap,cin} W[tap][cout][cin] X[voxel+tap][cin]
 // with M = 16 output channels, N = 16 voxels, K = 16 input channels per
-// instruction.  A wave owns 64 consecutive voxels and all output channels.
+// instruction.  A wave owns 64 voxels (an 8 x 8 slice of a 4 x 8 x 8 block
+// whose input, with halo, is staged in LDS) and all output channels.
 //
 // Backward: the data gradient is the same kernel on the BF16 gradient with
 // transposed, mirrored weights (FP32 output); the weight gradient is a GEMM
@@ -32,55 +33,59 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b <<
 // ---------------------------------------------------------------------------
 // MODE 0: Y (BF16) = act(conv + bias); MODE 1: Yf (FP32) = conv (no bias, no
 // activation), used for the data gradient with transposed, mirrored weights.
+//
+// A workgroup computes a 4 (z) x 8 (y) x 8 (x) block of output voxels, wave w
+// the z-slice w as four 16-voxel MFMA columns (two x-rows each).  The input
+// block with its one-voxel halo (6 x 10 x 10 voxels) is staged in LDS 16
+// channels at a time, so each input voxel is fetched from memory once per
+// workgroup instead of once per tap; the 27 taps then read their B fragments
+// (4 channels = 8 bytes per lane) from LDS.  Weights (A fragments) are read
+// from global memory, where every wave of the GPU shares them through the
+// caches.
+constexpr int kTX = 8, kTY = 8, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHalo = kHX * kHY * kHZ;  // 600 voxels
+
 template <int CIN, int COUT, int MODE = 0>
 __global__ void __launch_bounds__(256)
 conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                  uint16_t* __restrict__ Y, float* __restrict__ Yf, int n, int relu) {
     constexpr int MB = COUT / 16;
+    __shared__ __align__(16) uint16_t sIn[kHalo * 16];  // [halo voxel][16 channels]
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const long long nvox = (long long)n * n * n;
-    const long long v0 = ((long long)blockIdx.x * 4 + wave) * 64;
-    if (v0 >= nvox) return;
     const int col = lane & 15, kq = lane >> 4;
-    int vx[4], vy[4], vz[4];
-    bool valid[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const long long v = v0 + 16 * j + col;
-        valid[j] = v < nvox;
-        const long long vv = valid[j] ? v : 0;
-        vx[j] = (int)(vv % n);
-        vy[j] = (int)((vv / n) % n);
-        vz[j] = (int)(vv / ((long long)n * n));
-    }
+    const int bx = blockIdx.x * kTX, by = blockIdx.y * kTY, bz = blockIdx.z * kTZ;
+    // output voxel of lane column `col` in MFMA column j: (x, y) = (col & 7, 2j + (col >> 3)), z = wave
+    const int lx = col & 7, lyo = col >> 3;
+
     v4f acc[MB][4];
 #pragma unroll
     for (int m = 0; m < MB; ++m)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    for (int tap = 0; tap < 27; ++tap) {
-        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-        long long src[4];
-        bool inb[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int xs = vx[j] + dx, ys = vy[j] + dy, zs = vz[j] + dz;
-            inb[j] = valid[j] && xs >= 0 && ys >= 0 && zs >= 0 && xs < n && ys < n && zs < n;
-            src[j] = inb[j] ? (((long long)zs * n + ys) * n + xs) * CIN : 0;
+#pragma unroll 1
+    for (int c0 = 0; c0 < CIN; c0 += 16) {
+        __syncthreads();
+        // stage 600 voxels x 16 channels (32 bytes = 2 x uint4 each)
+        for (int t = threadIdx.x; t < kHalo * 2; t += 256) {
+            const int hv = t >> 1, half = t & 1;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n)
+                v = *reinterpret_cast<const uint4*>(X + (((long long)z * n + y) * n + x) * CIN + c0 + 8 * half);
+            *reinterpret_cast<uint4*>(&sIn[hv * 16 + 8 * half]) = v;
         }
-#pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 16) {
+        __syncthreads();
+#pragma unroll 3
+        for (int tap = 0; tap < 27; ++tap) {
+            const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // offsets + 1
             v4s b[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                if (inb[j]) {
-                    const uint2 raw = *reinterpret_cast<const uint2*>(X + src[j] + c0 + 4 * kq);
-                    b[j] = __builtin_bit_cast(v4s, raw);
-                } else {
-                    b[j] = v4s{0, 0, 0, 0};
-                }
+                const int hv = ((wave + dz) * kHY + (2 * j + lyo + dy)) * kHX + (lx + dx);
+                b[j] = *reinterpret_cast<const v4s*>(&sIn[hv * 16 + 4 * kq]);
             }
 #pragma unroll
             for (int m = 0; m < MB; ++m) {
@@ -92,12 +97,14 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt
             }
         }
     }
+    const int z = bz + wave;
 #pragma unroll
-    for (int m = 0; m < MB; ++m) {
+    for (int j = 0; j < 4; ++j) {
+        const int x = bx + lx, y = by + 2 * j + lyo;
+        if (x >= n || y >= n || z >= n) continue;
+        const long long v = ((long long)z * n + y) * n + x;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (!valid[j]) continue;
-            const long long v = v0 + 16 * j + col;
+        for (int m = 0; m < MB; ++m) {
             const int co = 16 * m + 4 * kq;
             if (MODE == 1) {
                 *reinterpret_cast<float4*>(Yf + v * COUT + co) =
@@ -107,9 +114,9 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt
             uint16_t o[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float y = acc[m][j][r] + bias[co + r];
-                if (relu) y = fmaxf(y, 0.f);
-                o[r] = f2bf(y);
+                float yv = acc[m][j][r] + bias[co + r];
+                if (relu) yv = fmaxf(yv, 0.f);
+                o[r] = f2bf(yv);
             }
             uint2 packed;
             packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
@@ -424,7 +431,8 @@ inline dim3 g1(long long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs))
 void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin, int cout,
                       bool relu, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
-    const dim3 grid((unsigned)((nvox + 255) / 256));
+    (void)nvox;
+    const dim3 grid((unsigned)((n + kTX - 1) / kTX), (unsigned)((n + kTY - 1) / kTY), (unsigned)((n + kTZ - 1) / kTZ));
 #define CONV_CASE(CI, CO)                                                                                   \
     if (cin == CI && cout == CO) {                                                                          \
         hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, 0>), grid, dim3(256), 0, s, X, W, bias, Y, (float*)nullptr, \
@@ -451,7 +459,8 @@ void launch_conv3_dgrad(const uint16_t* dPb, const float* W32, uint16_t* WT, flo
     const int nw = 27 * cin * cout;
     hipLaunchKernelGGL(transpose_weights_kernel, g1(nw, 256), dim3(256), 0, s, W32, WT, cin, cout);
     FLIP2_LAUNCH_CHECK();
-    const dim3 grid((unsigned)((nvox + 255) / 256));
+    (void)nvox;
+    const dim3 grid((unsigned)((n + kTX - 1) / kTX), (unsigned)((n + kTY - 1) / kTY), (unsigned)((n + kTZ - 1) / kTZ));
     // the forward kernel with the roles of the channel counts swapped
 #define DGRAD_CASE(CI, CO)                                                                                      \
     if (cin == CI && cout == CO) {                                                                              \

@@ -112,7 +112,7 @@ def test_loss_grad_is_repeatable(flip2):
 
 def test_training_and_learned_fcg(flip2):
     """Train on the residuals r_k = b - A x_k that Jacobi-PCG produces in its
-    first iterations (k = 0, 2, ..., 64) over 12 dam-break steps; the trained
+    first iterations (k = 0, 2, ..., 64) over 24 dam-break steps; the trained
     network, wrapped in damped-Jacobi smoothing, preconditions flexible CG
     to the 1e-6 FP64 residual in fewer iterations than Jacobi-PCG on the
     next step's system."""
@@ -125,15 +125,15 @@ def test_training_and_learned_fcg(flip2):
     net = flip2.UNet(n, seed=2)
     its = [0, 2, 4, 8, 16, 32, 64]
     net.capture_residuals(sim, its)
-    for _ in range(12):
+    for _ in range(24):
         sim.step()
         assert sim.last_solve()["converged"]
     net.capture_residuals(sim, [])
     nf = net.num_frames()
-    assert nf == 12 * len(its)
+    assert nf == 24 * len(its)
     hold = 2 * len(its)
     before = net.eval(nf - hold, nf)
-    net.train(12, 1e-3, 0, nf - hold, seed=1)
+    net.train(20, 1e-3, 0, nf - hold, seed=1)
     after = net.eval(nf - hold, nf)
     assert after < 0.25 * before, (before, after)
     for st in ("transfer", "prepare", "system"):
