@@ -23,8 +23,6 @@ FluidSim::FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device)
     brickExcl_.resizeAsync(numBricks_ + 1, stream_);
     brickSlot_.resizeAsync(numBricks_, stream_);
     activeBricks_.resizeAsync(numBricks_, stream_);
-    counter_.resizeAsync(4, stream_);
-    counter_.zeroDeviceAsync(stream_);
     total_.resizeAsync(4, stream_);
     scal_.resizeAsync(16, stream_);
     hist_.resizeAsync(4 * (cfg.maxIterations + 2), stream_);

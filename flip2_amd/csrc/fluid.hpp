@@ -118,7 +118,7 @@ public:
 
 private:
     void ensureSlots(uint32_t numActive);
-    ns::Reduce reduce() { return ns::Reduce{partials_.devPtr(), counter_.devPtr()}; }
+    ns::Reduce reduce() { return ns::Reduce{partials_.devPtr()}; }
     double readScalar(const double* dev);
     void readHistory(uint32_t first, uint32_t count, double* host);
 
@@ -143,7 +143,7 @@ private:
     DeviceVec<float> r_, z_, p_, q_, e_, precon_, aux0_, aux1_;
     void applyLearned(uint32_t na);
     DeviceVec<double> partials_, hist_, scal_;
-    DeviceVec<uint32_t> counter_, total_;
+    DeviceVec<uint32_t> total_;
     DeviceVec<uint8_t> ws_;
     uint32_t slotCapacity_ = 0;
     SolveStats stats_;

@@ -95,29 +95,21 @@ __device__ inline void block_max_to(uint32_t v, uint32_t* target) {
 
 // dst[i] = src[perm[i]] (records); stats[0] = max |velocity component|
 // (float bits) when stats is given.
-// Each thread moves 4 records, so 8 independent random 16-byte loads per
-// lane are in flight (the reads through the permutation are the cost).
+// The reads through the permutation fetch a whole 128-byte line per 32-byte
+// record; with one record per lane the kernel runs at the HBM rate of that
+// traffic (several records per lane measured slower).
 __global__ void gather_rec_kernel(const uint32_t* __restrict__ perm, uint32_t n, const float4* __restrict__ src,
                                   float4* __restrict__ dst, uint32_t* __restrict__ stats) {
-    const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float vmax = 0.f;
-    uint32_t j[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) j[k] = (i0 + k < n) ? perm[i0 + k] : n;
-    float4 a[4], b[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (j[k] < n) {
-            a[k] = src[2 * j[k]];
-            b[k] = src[2 * j[k] + 1];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (j[k] < n) {
-            dst[2 * (i0 + k)] = a[k];
-            dst[2 * (i0 + k) + 1] = b[k];
-            vmax = fmaxf(vmax, fmaxf(fabsf(a[k].w), fmaxf(fabsf(b[k].x), fabsf(b[k].y))));
+    if (i < n) {
+        const uint32_t j = perm[i];
+        if (j < n) {
+            const float4 a = src[2 * j];
+            const float4 b = src[2 * j + 1];
+            dst[2 * i] = a;
+            dst[2 * i + 1] = b;
+            vmax = fmaxf(fabsf(a.w), fmaxf(fabsf(b.x), fabsf(b.y)));
         }
     }
     if (stats) block_max_to(__float_as_uint(vmax), &stats[0]);
@@ -419,7 +411,7 @@ void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint3
 void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, uint32_t* stats,
                        hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(gather_rec_kernel, grid1d(ceil_div(n, 4u), 256), dim3(256), 0, s, perm, n, src, dst, stats);
+    hipLaunchKernelGGL(gather_rec_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, src, dst, stats);
     FLIP2_LAUNCH_CHECK();
 }
 

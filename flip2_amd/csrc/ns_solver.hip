@@ -49,32 +49,14 @@ __device__ inline double block_sum(double v, double* s_tmp) {
     return r;
 }
 
-// Each workgroup stores its partial (of `count` quantities); the last one to
-// finish sums all partials in workgroup order and writes out[q].
-__device__ inline void finish_reduction(const double* mine, int count, double* partials, uint32_t* counter,
-                                        double* out) {
-    __shared__ bool s_last;
-    if (threadIdx.x == 0) {
+// Each workgroup stores its partial (of `count` quantities); a one-workgroup
+// follow-up kernel sums them in workgroup order (deterministic).  No
+// workgroup waits for the others: on a chip whose L2 caches are per XCD, the
+// release fence a "last workgroup finishes" scheme needs in every workgroup
+// writes back L2 and costs more than the kernel's own work.
+__device__ inline void store_partials(const double* mine, int count, double* partials) {
+    if (threadIdx.x == 0)
         for (int q = 0; q < count; ++q) partials[(size_t)q * gridDim.x + blockIdx.x] = mine[q];
-        __threadfence();
-        const uint32_t done = atomicAdd(counter, 1u);
-        s_last = (done == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    __shared__ double s_tmp[16];
-    for (int q = 0; q < count; ++q) {
-        double acc = 0.0;
-        // fixed-order chunks per thread, then a fixed tree
-        const uint32_t per = (gridDim.x + blockDim.x - 1) / blockDim.x;
-        const uint32_t beg = threadIdx.x * per;
-        for (uint32_t b = beg; b < beg + per && b < gridDim.x; ++b)
-            acc += __hip_atomic_load(&partials[(size_t)q * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double t = block_sum(acc, s_tmp);
-        if (threadIdx.x == 0) out[q] = t;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
 }
 
 }  // namespace
@@ -195,8 +177,7 @@ __global__ void __launch_bounds__(kWG)
 assemble_kernel(const uint32_t* __restrict__ activeBricks, const int32_t* __restrict__ nbrSlot,
                 const uint8_t* __restrict__ cellType, const uint8_t* __restrict__ solid, Grid g, double hOverDt,
                 const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                uint16_t* __restrict__ coef, double* __restrict__ b64, double* __restrict__ partials,
-                uint32_t* __restrict__ counter, double* __restrict__ scal) {
+                uint16_t* __restrict__ coef, double* __restrict__ b64, double* __restrict__ partials) {
     __shared__ double s_tmp[16];
     const uint32_t slot = blockIdx.x;
     const uint32_t b = activeBricks[slot];
@@ -233,7 +214,7 @@ assemble_kernel(const uint32_t* __restrict__ activeBricks, const int32_t* __rest
         bb += rhs * rhs;
     }
     const double t[2] = {block_sum(bb, s_tmp), block_sum(fluid, s_tmp)};
-    finish_reduction(t, 2, partials, counter, scal);
+    store_partials(t, 2, partials);
 }
 
 namespace {
@@ -316,10 +297,23 @@ __device__ inline double apply_A64(uint32_t slot, uint32_t l, uint16_t cf, const
 }  // namespace
 
 // r32 = (float) r64; e = 0; rr into scal[out].
+// out[q] = sum over workgroups of partials[q * blocks + b], fixed order.
+__global__ void __launch_bounds__(kWG)
+reduce_partials_kernel(const double* __restrict__ partials, int count, uint32_t blocks, double* __restrict__ out) {
+    __shared__ double s_tmp[16];
+    const uint32_t per = (blocks + blockDim.x - 1) / blockDim.x;
+    const uint32_t beg = threadIdx.x * per;
+    for (int q = 0; q < count; ++q) {
+        double acc = 0.0;
+        for (uint32_t b = beg; b < beg + per && b < blocks; ++b) acc += partials[(size_t)q * blocks + b];
+        const double t = block_sum(acc, s_tmp);
+        if (threadIdx.x == 0) out[q] = t;
+    }
+}
+
 __global__ void __launch_bounds__(kWG)
 refine_start_kernel(const double* __restrict__ r64, float* __restrict__ r, float* __restrict__ e,
-                    const uint16_t* __restrict__ coef, double* __restrict__ partials, uint32_t* __restrict__ counter,
-                    double* __restrict__ out) {
+                    const uint16_t* __restrict__ coef, double* __restrict__ partials) {
     __shared__ double s_tmp[16];
     const uint32_t base = blockIdx.x * 512u;
     double rr = 0.0;
@@ -330,14 +324,13 @@ refine_start_kernel(const double* __restrict__ r64, float* __restrict__ r, float
         rr += (double)v * (double)v;
     }
     const double t = block_sum(rr, s_tmp);
-    finish_reduction(&t, 1, partials, counter, out);
+    store_partials(&t, 1, partials);
 }
 
 // Jacobi z = r / diag, p = z (first iteration), rz into out[0].
 __global__ void __launch_bounds__(kWG)
 jacobi_init_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, float* __restrict__ z,
-                   float* __restrict__ p, double* __restrict__ partials, uint32_t* __restrict__ counter,
-                   double* __restrict__ out) {
+                   float* __restrict__ p, double* __restrict__ partials) {
     __shared__ double s_tmp[16];
     const uint32_t base = blockIdx.x * 512u;
     double rz = 0.0;
@@ -349,14 +342,13 @@ jacobi_init_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coe
         rz += (double)r[base + l] * (double)zv;
     }
     const double t = block_sum(rz, s_tmp);
-    finish_reduction(&t, 1, partials, counter, out);
+    store_partials(&t, 1, partials);
 }
 
 // q = A p; pq into out[0].
 __global__ void __launch_bounds__(kWG)
 matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
-                  float* __restrict__ q, double* __restrict__ partials, uint32_t* __restrict__ counter,
-                  double* __restrict__ out) {
+                  float* __restrict__ q, double* __restrict__ partials) {
     __shared__ double s_tmp[16];
     const uint32_t slot = blockIdx.x;
     double pq = 0.0;
@@ -367,7 +359,7 @@ matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef
         pq += (double)p[v] * (double)qv;
     }
     const double t = block_sum(pq, s_tmp);
-    finish_reduction(&t, 1, partials, counter, out);
+    store_partials(&t, 1, partials);
 }
 
 // alpha = rz / pq; e += alpha p; r -= alpha q; with Jacobi also z = r/diag
@@ -375,8 +367,7 @@ matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef
 __global__ void __launch_bounds__(kWG)
 update_kernel(const double* __restrict__ rzOld, const double* __restrict__ pq, const float* __restrict__ p,
               const float* __restrict__ q, const uint16_t* __restrict__ coef, float* __restrict__ e,
-              float* __restrict__ r, float* __restrict__ z, int jacobi, double* __restrict__ partials,
-              uint32_t* __restrict__ counter, double* __restrict__ out /* [rz_new, rr_new] */) {
+              float* __restrict__ r, float* __restrict__ z, int jacobi, double* __restrict__ partials /* [rz_new, rr_new] per workgroup */) {
     __shared__ double s_tmp[16];
     const double den = *pq;
     const float alpha = (den != 0.0) ? (float)(*rzOld / den) : 0.f;
@@ -399,19 +390,18 @@ update_kernel(const double* __restrict__ rzOld, const double* __restrict__ pq, c
     double vals[2];
     vals[0] = block_sum(rz, s_tmp);
     vals[1] = block_sum(rr, s_tmp);
-    finish_reduction(vals, 2, partials, counter, out);
+    store_partials(vals, 2, partials);
 }
 
 // rz = r . z (after a separate preconditioner apply).
 __global__ void __launch_bounds__(kWG)
-dot_kernel(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ partials,
-           uint32_t* __restrict__ counter, double* __restrict__ out) {
+dot_kernel(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ partials) {
     __shared__ double s_tmp[16];
     const uint32_t base = blockIdx.x * 512u;
     double s = 0.0;
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) s += (double)a[base + l] * (double)b[base + l];
     const double t = block_sum(s, s_tmp);
-    finish_reduction(&t, 1, partials, counter, out);
+    store_partials(&t, 1, partials);
 }
 
 // Damped-Jacobi smoothing around an external (learned) correction:
@@ -482,8 +472,7 @@ accumulate_kernel(const float* __restrict__ e, double* __restrict__ x64, const u
 __global__ void __launch_bounds__(kWG)
 true_residual_kernel(const double* __restrict__ b64, const double* __restrict__ x64,
                      const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
-                     double* __restrict__ r64, double* __restrict__ partials, uint32_t* __restrict__ counter,
-                     double* __restrict__ out) {
+                     double* __restrict__ r64, double* __restrict__ partials) {
     __shared__ double s_tmp[16];
     const uint32_t slot = blockIdx.x;
     double rr = 0.0;
@@ -496,7 +485,7 @@ true_residual_kernel(const double* __restrict__ b64, const double* __restrict__ 
         rr += rv * rv;
     }
     const double t = block_sum(rr, s_tmp);
-    finish_reduction(&t, 1, partials, counter, out);
+    store_partials(&t, 1, partials);
 }
 
 __global__ void fill_f32_kernel(float* __restrict__ a, uint32_t count, float v) {
@@ -592,8 +581,7 @@ mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon)
 // sweeps); rz into out[0].
 __global__ void __launch_bounds__(64)
 mic_apply_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, const float* __restrict__ precon,
-                 float* __restrict__ z, double* __restrict__ partials, uint32_t* __restrict__ counter,
-                 double* __restrict__ out) {
+                 float* __restrict__ z, double* __restrict__ partials) {
     __shared__ float s_q[512];
     __shared__ float s_pc[512];
     __shared__ uint16_t s_cf[512];
@@ -654,7 +642,7 @@ mic_apply_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef,
         rz += (double)r[base + l] * (double)s_q[l];
     }
     const double t = block_sum(rz, s_tmp);
-    finish_reduction(&t, 1, partials, counter, out);
+    store_partials(&t, 1, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -918,6 +906,11 @@ __global__ void advect_rk2_kernel(uint32_t n, float4* __restrict__ rec, const fl
 // ---------------------------------------------------------------------------
 namespace {
 inline dim3 g1(uint32_t n, uint32_t bs) { return dim3(ceil_div(n ? n : 1, bs)); }
+// out[0..count) = sums of the workgroup partials of the kernel just launched
+void reduce_to(const Reduce& red, int count, uint32_t blocks, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kWG), 0, s, red.partials, count, blocks, out);
+    FLIP2_LAUNCH_CHECK();
+}
 }  // namespace
 
 void launch_classify_bricks(const uint32_t* counts, const uint8_t* solid, uint32_t numBricks, uint32_t* flag,
@@ -944,37 +937,40 @@ void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int
                      uint16_t* coef, double* b64, Reduce red, double* out, hipStream_t s) {
     if (!numActive) return;
     hipLaunchKernelGGL(assemble_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, nbrSlot, cellType, solid, g,
-                       hOverDt, U, V, W, coef, b64, red.partials, red.counter, out);
+                       hOverDt, U, V, W, coef, b64, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 2, numActive, out, s);
 }
 void launch_refine_start(uint32_t numActive, const double* r64, float* r, float* e, const uint16_t* coef, Reduce red,
                          double* out, hipStream_t s) {
-    hipLaunchKernelGGL(refine_start_kernel, dim3(numActive), dim3(kWG), 0, s, r64, r, e, coef, red.partials,
-                       red.counter, out);
+    hipLaunchKernelGGL(refine_start_kernel, dim3(numActive), dim3(kWG), 0, s, r64, r, e, coef, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 1, numActive, out, s);
 }
 void launch_jacobi_init(uint32_t numActive, const float* r, const uint16_t* coef, float* z, float* p, Reduce red,
                         double* out, hipStream_t s) {
-    hipLaunchKernelGGL(jacobi_init_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, z, p, red.partials, red.counter,
-                       out);
+    hipLaunchKernelGGL(jacobi_init_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, z, p, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 1, numActive, out, s);
 }
 void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
                        Reduce red, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(matvec_dot_kernel, dim3(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials,
-                       red.counter, out);
+    hipLaunchKernelGGL(matvec_dot_kernel, dim3(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 1, numActive, out, s);
 }
 void launch_update(uint32_t numActive, const double* rzOld, const double* pq, const float* p, const float* q,
                    const uint16_t* coef, float* e, float* r, float* z, bool jacobi, Reduce red, double* out,
                    hipStream_t s) {
     hipLaunchKernelGGL(update_kernel, dim3(numActive), dim3(kWG), 0, s, rzOld, pq, p, q, coef, e, r, z,
-                       jacobi ? 1 : 0, red.partials, red.counter, out);
+                       jacobi ? 1 : 0, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 2, numActive, out, s);
 }
 void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(dot_kernel, dim3(numActive), dim3(kWG), 0, s, a, b, red.partials, red.counter, out);
+    hipLaunchKernelGGL(dot_kernel, dim3(numActive), dim3(kWG), 0, s, a, b, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 1, numActive, out, s);
 }
 void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
                       bool first, hipStream_t s) {
@@ -1003,8 +999,9 @@ void launch_accumulate(uint32_t numActive, const float* e, double* x64, const ui
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s) {
     hipLaunchKernelGGL(true_residual_kernel, dim3(numActive), dim3(kWG), 0, s, b64, x64, coef, nbrSlot, r64,
-                       red.partials, red.counter, out);
+                       red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 1, numActive, out, s);
 }
 void launch_fill_f32(float* a, uint32_t count, float v, hipStream_t s) {
     if (!count) return;
@@ -1022,9 +1019,9 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
 }
 void launch_mic_apply(uint32_t numActive, const float* r, const uint16_t* coef, const float* precon, float* z,
                       Reduce red, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(mic_apply_kernel, dim3(numActive), dim3(64), 0, s, r, coef, precon, z, red.partials,
-                       red.counter, out);
+    hipLaunchKernelGGL(mic_apply_kernel, dim3(numActive), dim3(64), 0, s, r, coef, precon, z, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 1, numActive, out, s);
 }
 void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,

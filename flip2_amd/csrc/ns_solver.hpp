@@ -11,11 +11,9 @@ constexpr uint8_t kFluid = 1;
 constexpr uint8_t kSolid = 2;
 
 // Deterministic grid-wide reduction scratch: one double per workgroup per
-// quantity, and a completion counter (must start at 0; the last workgroup
-// resets it).
+// quantity (a follow-up kernel sums them in workgroup order).
 struct Reduce {
     double* partials;
-    uint32_t* counter;
 };
 
 // A moving solid box (e.g. a wave paddle): cells whose centre lies in
