@@ -200,6 +200,8 @@ def _declare(l: ctypes.CDLL) -> None:
                              ctypes.POINTER(_f64)],
         "flip2_unet_eval": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_f64)],
         "flip2_fluid_set_unet": [_vp, _vp, _u32],
+        "flip2_fluid_set_flex_directions": [_vp, _u32],
+        "flip2_fluid_set_learned_graph": [_vp, ctypes.c_int],
     }
     for name in ("align_particles_to_grid", "sort_particles", "generate_voxels", "particle_vel_to_voxels",
                  "calc_div_u", "pressure_solve", "update_voxel_velocities", "voxel_vels_to_particles",
@@ -488,6 +490,14 @@ class FluidSim:
         """Attach `net` as the "learned" preconditioner (None detaches)."""
         check(lib().flip2_fluid_set_unet(self._h, net._h if net is not None else None, int(fallback_iterations)))
         self._unet = net
+
+    def set_learned_graph(self, enable: bool) -> None:
+        """Replay the learned preconditioner from a HIP graph (default) or not."""
+        check(lib().flip2_fluid_set_learned_graph(self._h, 1 if enable else 0))
+
+    def set_flex_directions(self, directions: int) -> None:
+        """Earlier directions (1..16) flexible CG orthogonalises against."""
+        check(lib().flip2_fluid_set_flex_directions(self._h, int(directions)))
 
     def get(self, name: str) -> np.ndarray:
         cnt = _sz()

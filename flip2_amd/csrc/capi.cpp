@@ -54,15 +54,17 @@ struct flip2_unet {
     }
     int n() const { return net.gridSize(); }
     // z = M(b) on the dense grid: the network alone or wrapped in smoothing.
-    const float* apply(const float* bD, const uint16_t* coefD) {
+    const float* apply(const float* bD, const uint16_t* coefD, bool keepActivations = false) {
         const int nn = n();
         const float* in = bD;
         if (omega > 0.f) {
             flip2::nn::launch_dense_presmooth(coefD, bD, omega, r1.devPtr(), nn, nullptr);
             in = r1.devPtr();
         }
-        flip2::nn::launch_dense_input(in, coefD, nn, net.input(), net.mask(), net.scale(), red.devPtr() + 2, nullptr);
-        net.forward(nullptr);
+        flip2::nn::launch_dense_input(in, coefD, nn, net.input(), net.inputCompact(), net.mask(), net.scale(),
+                                      red.devPtr() + 2, nullptr);
+        net.setInputMasks(nullptr, !keepActivations);
+        net.forward(nullptr, keepActivations);
         if (omega <= 0.f) return net.output();
         flip2::nn::launch_dense_postsmooth(coefD, bD, net.output(), omega, zS.devPtr(), nn, nullptr);
         return zS.devPtr();
@@ -71,7 +73,7 @@ struct flip2_unet {
     // gradients accumulated when `grad`
     double lossGrad(const float* bD, const uint16_t* coefD, bool grad) {
         const int nn = n();
-        const float* z = apply(bD, coefD);
+        const float* z = apply(bD, coefD, grad);
         flip2::nn::launch_residual(coefD, z, bD, res.devPtr(), nn, red.devPtr(), nullptr);
         if (grad) {
             if (omega > 0.f) {
@@ -636,6 +638,8 @@ int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg) {
         flip2::FluidConfig c = to_config(*cfg);
         c.learnedFallbackIterations = f->impl.config().learnedFallbackIterations;
         c.learnedSmoothing = f->impl.config().learnedSmoothing;
+        c.learnedDirections = f->impl.config().learnedDirections;
+        c.learnedGraph = f->impl.config().learnedGraph;
         f->impl.setConfig(c);
     });
 }
@@ -862,6 +866,24 @@ int flip2_unet_eval(flip2_unet* net, int first, int last, double* mean_loss) {
         if (mean_loss) *mean_loss = sum / (double)(last - first);
     });
 }
+int flip2_fluid_set_flex_directions(flip2_fluid* f, uint32_t directions) {
+    return guarded([&] {
+        NEED(f);
+        FLIP2_REQUIRE(directions >= 1 && directions <= (uint32_t)flip2::ns::kMaxFlexDirections, flip2::kErrArgument,
+                      "flexible CG keeps 1 to 16 directions");
+        flip2::FluidConfig c = f->impl.config();
+        c.learnedDirections = directions;
+        f->impl.setConfig(c);
+    });
+}
+int flip2_fluid_set_learned_graph(flip2_fluid* f, int enable) {
+    return guarded([&] {
+        NEED(f);
+        flip2::FluidConfig c = f->impl.config();
+        c.learnedGraph = enable != 0;
+        f->impl.setConfig(c);
+    });
+}
 int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iterations) {
     return guarded([&] {
         NEED(f);
@@ -877,8 +899,8 @@ int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iter
         FLIP2_REQUIRE(sim.gridN() == net->n(), flip2::kErrArgument, "grid size differs from the network's");
         flip2::UNet* u = &net->net;
         flip2::FluidSim* s = &sim;
-        sim.setLearnedPreconditioner([u, s](uint32_t na, const float* r, float* z, hipStream_t st) {
-            u->applyActive(na, s->activeBricksDev(), s->cellTypeDev(), r, z, s->gridN(), st);
+        sim.setLearnedPreconditioner([u, s](uint32_t na, const float* r, float* z, bool newSystem, hipStream_t st) {
+            u->applyActive(na, s->activeBricksDev(), s->cellTypeDev(), r, z, s->gridN(), newSystem, st);
         });
     });
 }

@@ -31,6 +31,7 @@ FluidSim::FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device)
 }
 
 FluidSim::~FluidSim() {
+    dropLearnedGraph();
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
@@ -126,22 +127,67 @@ void FluidSim::ensureSlots(uint32_t numActive) {
     coef_.resizeAsync(cells, stream_);
     for (auto* a : {&b64_, &x64_, &r64_}) a->resizeAsync(cells, stream_);
     for (auto* a : {&r_, &z_, &p_, &q_, &e_, &precon_, &aux0_, &aux1_}) a->resizeAsync(cells, stream_);
-    partials_.resizeAsync(cap * 2u + 64u, stream_);
+    partials_.resizeAsync(cap * (uint32_t)ns::kMaxFlexDirections + 64u, stream_);
     slotCapacity_ = cap;
+    ringSlots_ = 0;
+}
+
+void FluidSim::ensureRing(uint32_t slots) {
+    if (ringSlots_ >= slots) return;
+    const size_t cells = (size_t)slotCapacity_ * 512u;
+    pRing_.resizeAsync((uint32_t)(cells * slots), stream_);
+    qRing_.resizeAsync((uint32_t)(cells * slots), stream_);
+    zq_.resizeAsync(ns::kMaxFlexDirections, stream_);
+    ringSlots_ = slots;
 }
 
 // z = M(r) for the learned preconditioner: the network alone, or wrapped in
 // a damped-Jacobi pre-/post-smoothing step.
-void FluidSim::applyLearned(uint32_t na) {
+void FluidSim::applyLearnedEager(uint32_t na, bool newSystem) {
     const float w = cfg_.learnedSmoothing;
     if (w > 0.f) {
         ns::launch_presmooth(na, r_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), w, aux0_.devPtr(), stream_);
-        learned_(na, aux0_.devPtr(), aux1_.devPtr(), stream_);
+        learned_(na, aux0_.devPtr(), aux1_.devPtr(), newSystem, stream_);
         ns::launch_postsmooth(na, r_.devPtr(), aux1_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), w, z_.devPtr(),
                               stream_);
     } else {
-        learned_(na, r_.devPtr(), z_.devPtr(), stream_);
+        learned_(na, r_.devPtr(), z_.devPtr(), newSystem, stream_);
     }
+}
+
+void FluidSim::dropLearnedGraph() {
+    if (learnedGraphExec_) (void)hipGraphExecDestroy(learnedGraphExec_);
+    if (learnedGraph_) (void)hipGraphDestroy(learnedGraph_);
+    learnedGraphExec_ = nullptr;
+    learnedGraph_ = nullptr;
+}
+
+// The first apply of a solve runs eagerly (it rebuilds the network's masks);
+// the second is captured into a HIP graph (kernels only: no memset nodes)
+// that the remaining applies of the solve replay, saving the launch gaps of
+// its ~20 kernels.
+void FluidSim::applyLearned(uint32_t na) {
+    if (learnedFresh_ || !cfg_.learnedGraph) {
+        const bool fresh = learnedFresh_;
+        learnedFresh_ = false;
+        dropLearnedGraph();
+        applyLearnedEager(na, fresh);
+        return;
+    }
+    if (!learnedGraphExec_) {
+        FLIP2_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            applyLearnedEager(na, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FLIP2_HIP(hipStreamEndCapture(stream_, &learnedGraph_));
+        FLIP2_HIP(hipGraphInstantiate(&learnedGraphExec_, learnedGraph_, nullptr, nullptr, 0));
+    }
+    FLIP2_HIP(hipGraphLaunch(learnedGraphExec_, stream_));
 }
 
 double FluidSim::readScalar(const double* dev) {
@@ -233,8 +279,31 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     FLIP2_HIP(hipMemcpyAsync(r64_.devPtr(), b64_.devPtr(), (size_t)cells * sizeof(double), hipMemcpyDeviceToDevice,
                              stream_));
     FLIP2_REQUIRE(prec != kPrecLearned || learned_, kErrState, "no learned preconditioner installed");
+    learnedFresh_ = true;
     if (prec == kPrecMIC0) ns::launch_mic_factor(na, coef_.devPtr(), precon_.devPtr(), stream_);
     const bool jac = prec == kPrecJacobi;
+    const bool flex = prec == kPrecLearned;
+    // Flexible CG: direction of inner iteration j in ring slot j % slots
+    // (one more slot than directions kept, so a new direction never
+    // overwrites one it is orthogonalised against).
+    const uint32_t keep = std::max(1u, std::min<uint32_t>(cfg_.learnedDirections, ns::kMaxFlexDirections));
+    const uint32_t slots = keep + 1;
+    const size_t cellsCap = (size_t)slotCapacity_ * 512u;
+    if (flex) ensureRing(slots);
+    auto ringP = [&](uint32_t j) { return pRing_.devPtr() + (j % slots) * cellsCap; };
+    auto ringQ = [&](uint32_t j) { return qRing_.devPtr() + (j % slots) * cellsCap; };
+    // the directions of iterations it, it-1, ... (at most `keep`)
+    auto kept = [&](uint32_t it, double* Hb) {
+        ns::FlexDirections d{};
+        d.count = (int)std::min(keep, it + 1);
+        for (int i = 0; i < d.count; ++i) {
+            const uint32_t j = it - (uint32_t)i;
+            d.p[i] = ringP(j);
+            d.q[i] = ringQ(j);
+            d.pq[i] = Hb + 4u * j + 2;
+        }
+        return d;
+    };
     double rel = 1.0;
     uint32_t total = 0;
     const uint32_t cap = std::min(capIn, cfg_.maxIterations);
@@ -248,14 +317,14 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
         if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
         if (jac) {
             ns::launch_jacobi_init(na, r_.devPtr(), coef_.devPtr(), z_.devPtr(), p_.devPtr(), reduce(), H, stream_);
+        } else if (flex) {
+            // p_0 = z_0; H[0] = p_0 . r_0 (the step-length numerator)
+            applyLearned(na);
+            ns::FlexDirections none{};
+            ns::launch_flex_direction(na, z_.devPtr(), none, zq_.devPtr(), r_.devPtr(), ringP(0), reduce(), H,
+                                      stream_);
         } else {
-            if (prec == kPrecMIC0) {
-                ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), H,
-                                     stream_);
-            } else {
-                applyLearned(na);
-                ns::launch_dot(na, r_.devPtr(), z_.devPtr(), reduce(), H, stream_);
-            }
+            ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), H, stream_);
             ns::launch_direction(na, H, H, z_.devPtr(), p_.devPtr(), true, stream_);
         }
         uint32_t it = 0;
@@ -266,22 +335,24 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             for (uint32_t k = 0; k < batch; ++k, ++it) {
                 double* hk = H + 4u * it;
                 double* hn = H + 4u * (it + 1);
-                ns::launch_matvec_dot(na, p_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), q_.devPtr(), reduce(), hk + 2,
-                                      stream_);
-                ns::launch_update(na, hk, hk + 2, p_.devPtr(), q_.devPtr(), coef_.devPtr(), e_.devPtr(), r_.devPtr(),
-                                  z_.devPtr(), jac, reduce(), hn, stream_);
+                float* p = flex ? ringP(it) : p_.devPtr();
+                float* q = flex ? ringQ(it) : q_.devPtr();
+                ns::launch_matvec_dot(na, p, coef_.devPtr(), nbrSlot_.devPtr(), q, reduce(), hk + 2, stream_);
+                ns::launch_update(na, hk, hk + 2, p, q, coef_.devPtr(), e_.devPtr(), r_.devPtr(), z_.devPtr(), jac,
+                                  reduce(), hn, stream_);
                 if (prec == kPrecMIC0) {
                     ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), hn,
                                          stream_);
-                } else if (prec == kPrecLearned) {
+                } else if (flex) {
+                    // p = z - sum_i (z . q_i / p_i . q_i) p_i over the kept
+                    // directions; hn[0] = p . r
                     applyLearned(na);
-                    ns::launch_dot(na, r_.devPtr(), z_.devPtr(), reduce(), hn, stream_);
-                    ns::launch_dot(na, z_.devPtr(), q_.devPtr(), reduce(), hn + 3, stream_);
+                    const ns::FlexDirections d = kept(it, H);
+                    ns::launch_flex_dots(na, z_.devPtr(), d, reduce(), zq_.devPtr(), stream_);
+                    ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(it + 1), reduce(),
+                                              hn, stream_);
                 }
-                if (prec == kPrecLearned)
-                    ns::launch_direction_flex(na, hn + 3, hk + 2, z_.devPtr(), p_.devPtr(), stream_);
-                else
-                    ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
+                if (!flex) ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
                 if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
             }
             readHistory(first + 1, batch, h.data());

@@ -35,6 +35,12 @@ struct FluidConfig {
     // Damped-Jacobi weight of the smoothing step before and after the
     // learned correction (0: the network alone is the preconditioner).
     float learnedSmoothing = 0.0f;
+    // Flexible CG keeps this many earlier directions for the learned
+    // preconditioner (1 = FCG(1); at most ns::kMaxFlexDirections).
+    uint32_t learnedDirections = 8;
+    // Replay the learned preconditioner's apply from a HIP graph captured
+    // once per solve.
+    bool learnedGraph = true;
 };
 
 struct SolveStats {
@@ -50,8 +56,11 @@ struct SolveStats {
 };
 
 // A preconditioner supplied from outside the solver (the learned one):
-// z = M^-1 r on the active-brick vectors, FP32.
-using PrecondFn = std::function<void(uint32_t numActive, const float* r, float* z, hipStream_t s)>;
+// z = M^-1 r on the active-brick vectors, FP32.  newSystem is set on the
+// first apply of a solve (the fluid cells may have changed); the other
+// applies of a solve launch the same kernels on the same buffers, so the
+// solver may replay them from a HIP graph.
+using PrecondFn = std::function<void(uint32_t numActive, const float* r, float* z, bool newSystem, hipStream_t s)>;
 // Receives the residual r_k = b - A x_k (FP32, active-brick layout) of a
 // solve's first refinement after k inner iterations (training data).
 using CaptureFn = std::function<void(uint32_t numActive, const float* r, hipStream_t s)>;
@@ -89,7 +98,11 @@ public:
     // choice (used for preconditioner comparisons on one system).
     SolveStats solveWith(int preconditioner);
     SolveStats solveOnce(int preconditioner, uint32_t cap);
-    void setLearnedPreconditioner(PrecondFn fn) { learned_ = std::move(fn); }
+    void setLearnedPreconditioner(PrecondFn fn) {
+        dropLearnedGraph();
+        learnedFresh_ = true;
+        learned_ = std::move(fn);
+    }
     // Residuals after the listed inner iteration counts (0 = b) of every
     // following solve go to fn; an empty list stops the capture.
     void setResidualCapture(std::vector<uint32_t> iterations, CaptureFn fn) {
@@ -141,7 +154,16 @@ private:
     DeviceVec<uint16_t> coef_;
     DeviceVec<double> b64_, x64_, r64_;
     DeviceVec<float> r_, z_, p_, q_, e_, precon_, aux0_, aux1_;
+    DeviceVec<float> pRing_, qRing_;  // flexible CG directions, learnedDirections + 1 slots
+    DeviceVec<double> zq_;
+    uint32_t ringSlots_ = 0;
+    void ensureRing(uint32_t slots);
     void applyLearned(uint32_t na);
+    void applyLearnedEager(uint32_t na, bool newSystem);
+    void dropLearnedGraph();
+    bool learnedFresh_ = true;  // the next learned apply starts a new system
+    hipGraph_t learnedGraph_ = nullptr;
+    hipGraphExec_t learnedGraphExec_ = nullptr;
     DeviceVec<double> partials_, hist_, scal_;
     DeviceVec<uint32_t> total_;
     DeviceVec<uint8_t> ws_;

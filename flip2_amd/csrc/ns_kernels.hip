@@ -321,7 +321,16 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
     const double org[3] = {(double)g.negX + (double)(bb[0] * 8) * (double)g.cellSize,
                            (double)g.negY + (double)(bb[1] * 8) * (double)g.cellSize,
                            (double)g.negZ + (double)(bb[2] * 8) * (double)g.cellSize};
-    for (uint32_t sIdx = tid; sIdx < total; sIdx += 256u) {
+    // Wave w takes the contiguous quarter [w Q, w Q + Q) of the particles
+    // and its lanes stride through it R apart, so the lanes of one atomic
+    // instruction sit in different cells: consecutive particles (same or
+    // adjacent cells) would hit the same LDS words and serialise.
+    const uint32_t Q = (total + 3u) >> 2;
+    const uint32_t R = (Q + 63u) >> 6;
+    const uint32_t qBeg = wave * Q, qEnd = min(total, qBeg + Q);
+    for (uint32_t i = 0; i < R; ++i) {
+        const uint32_t sIdx = qBeg + lane * R + i;
+        if (sIdx >= qEnd) continue;
         int a = 0, b = kShellRuns + 1;  // last run with s_off <= sIdx
         while (b - a > 1) {
             const int m = (a + b) >> 1;

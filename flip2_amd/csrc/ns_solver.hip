@@ -451,14 +451,47 @@ direction_kernel(const double* __restrict__ rzNew, const double* __restrict__ rz
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) p[base + l] = fmaf(beta, p[base + l], z[base + l]);
 }
 
-// Flexible CG (Notay's FCG(1)) direction: beta = -(z . q) / (p . q),
-// p = z + beta p, for a preconditioner that is not a fixed SPD matrix.
 __global__ void __launch_bounds__(kWG)
-direction_flex_kernel(const double* __restrict__ zq, const double* __restrict__ pq, const float* __restrict__ z,
-                      float* __restrict__ p) {
-    const float beta = (*pq != 0.0) ? (float)(-*zq / *pq) : 0.f;
+flex_dots_kernel(const float* __restrict__ z, FlexDirections d, double* __restrict__ partials) {
+    __shared__ double s_tmp[16];
     const uint32_t base = blockIdx.x * 512u;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) p[base + l] = fmaf(beta, p[base + l], z[base + l]);
+    double acc[kMaxFlexDirections];
+#pragma unroll
+    for (int i = 0; i < kMaxFlexDirections; ++i) acc[i] = 0.0;
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+        const double zv = (double)z[base + l];
+#pragma unroll
+        for (int i = 0; i < kMaxFlexDirections; ++i)
+            if (i < d.count) acc[i] += zv * (double)d.q[i][base + l];
+    }
+    double vals[kMaxFlexDirections];
+    for (int i = 0; i < d.count; ++i) vals[i] = block_sum(acc[i], s_tmp);
+    store_partials(vals, d.count, partials);
+}
+
+__global__ void __launch_bounds__(kWG)
+flex_direction_kernel(const float* __restrict__ z, FlexDirections d, const double* __restrict__ zq,
+                      const float* __restrict__ r, float* __restrict__ p, double* __restrict__ partials) {
+    __shared__ double s_tmp[16];
+    float beta[kMaxFlexDirections];
+#pragma unroll
+    for (int i = 0; i < kMaxFlexDirections; ++i) {
+        const double den = i < d.count ? *d.pq[i] : 0.0;
+        beta[i] = den != 0.0 ? (float)(-zq[i] / den) : 0.f;
+    }
+    const uint32_t base = blockIdx.x * 512u;
+    double pr = 0.0;
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+        const uint32_t v = base + l;
+        float pv = z[v];
+#pragma unroll
+        for (int i = 0; i < kMaxFlexDirections; ++i)
+            if (i < d.count) pv = fmaf(beta[i], d.p[i][v], pv);
+        p[v] = pv;
+        pr += (double)pv * (double)r[v];
+    }
+    const double t = block_sum(pr, s_tmp);
+    store_partials(&t, 1, partials);
 }
 
 // x64 += e; r64 = b - A x64 (FP64); ||r64||^2 into out[0].
@@ -987,11 +1020,25 @@ void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const
     hipLaunchKernelGGL(postsmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, y, coef, nbrSlot, omega, z);
     FLIP2_LAUNCH_CHECK();
 }
-void launch_direction_flex(uint32_t numActive, const double* zq, const double* pq, const float* z, float* p,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(direction_flex_kernel, dim3(numActive), dim3(kWG), 0, s, zq, pq, z, p);
+void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& d, Reduce red, double* out,
+                      hipStream_t s) {
+    FLIP2_REQUIRE(d.count >= 0 && d.count <= kMaxFlexDirections, kErrArgument, "flexible CG: too many directions");
+    if (d.count == 0) return;
+    hipLaunchKernelGGL(flex_dots_kernel, dim3(numActive), dim3(kWG), 0, s, z, d, red.partials);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(red, d.count, numActive, out, s);
 }
+
+void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirections& d, const double* zq,
+                           const float* r, float* p, Reduce red, double* pr, hipStream_t s) {
+    FLIP2_REQUIRE(d.count >= 0 && d.count <= kMaxFlexDirections, kErrArgument, "flexible CG: too many directions");
+    for (int i = 0; i < d.count; ++i)
+        FLIP2_REQUIRE(d.p[i] != p, kErrArgument, "flexible CG: output aliases a kept direction");
+    hipLaunchKernelGGL(flex_direction_kernel, dim3(numActive), dim3(kWG), 0, s, z, d, zq, r, p, red.partials);
+    FLIP2_LAUNCH_CHECK();
+    reduce_to(red, 1, numActive, pr, s);
+}
+
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s) {
     hipLaunchKernelGGL(accumulate_kernel, dim3(numActive), dim3(kWG), 0, s, e, x64, coef);
     FLIP2_LAUNCH_CHECK();

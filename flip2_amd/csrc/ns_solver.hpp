@@ -45,8 +45,21 @@ void launch_update(uint32_t numActive, const double* rzOld, const double* pq, co
 void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, double* out, hipStream_t s);
 void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
                       bool first, hipStream_t s);
-void launch_direction_flex(uint32_t numActive, const double* zq, const double* pq, const float* z, float* p,
-                           hipStream_t s);
+// Flexible CG with the last `count` directions kept (Notay's truncated FCG):
+// p_i, q_i = A p_i and pq_i = p_i . q_i (device scalar) of each kept direction.
+constexpr int kMaxFlexDirections = 16;
+struct FlexDirections {
+    const float* p[kMaxFlexDirections];
+    const float* q[kMaxFlexDirections];
+    const double* pq[kMaxFlexDirections];
+    int count;
+};
+// out[i] = z . q_i for the kept directions.
+void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& d, Reduce red, double* out,
+                      hipStream_t s);
+// p = z - sum_i (zq[i] / pq_i) p_i (p must not be one of the p_i); pr = p . r.
+void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirections& d, const double* zq,
+                           const float* r, float* p, Reduce red, double* pr, hipStream_t s);
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s);
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s);

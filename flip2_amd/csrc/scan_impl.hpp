@@ -3,10 +3,12 @@
 // (engine/algorithms/parallelPrefixSumKernels.cu:5-77), which launches two
 // kernels per level and allocates its block sums per call.
 //
-// Tile = 256 threads x 16 u32 = 4096 elements.  Wave w of a tile owns the
-// 1024 elements [w*1024, w*1024+1024) as 4 rows of 64 lanes x uint4, so every
+// Tile = 256 threads x 32 u32 = 8192 elements.  Wave w of a tile owns the
+// 2048 elements [w*2048, w*2048+2048) as 8 rows of 64 lanes x uint4, so every
 // load and store is a coalesced 1 KiB row.  Tile ids come from an atomic
-// counter, so a tile only ever waits on tiles that started before it.
+// counter, so a tile only ever waits on tiles that started before it.  The
+// look-back uses all 256 lanes: one status word of an earlier tile per lane,
+// so an inclusive prefix advances 256 tiles per status round trip.
 #pragma once
 
 #include "common.hpp"
@@ -14,8 +16,8 @@
 namespace flip2 {
 
 constexpr int kScanThreads = 256;
-constexpr int kScanRows = 4;                      // uint4 rows per wave
-constexpr int kScanTile = kScanThreads * 4 * kScanRows;  // 4096
+constexpr int kScanRows = 8;                      // uint4 rows per wave
+constexpr int kScanTile = kScanThreads * 4 * kScanRows;  // 8192
 
 // Look-back status word: bits 32..33 hold the flag, bits 0..31 the value.
 constexpr uint64_t kFlagAggregate = 1ull << 32;
@@ -101,6 +103,8 @@ scan_kernel(Input input, uint32_t* __restrict__ out, uint32_t n, int exclusive, 
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave_total[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
+    __shared__ int s_firstIncl[kScanThreads / 64];
+    __shared__ uint32_t s_lookSum[kScanThreads / 64];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -133,37 +137,46 @@ scan_kernel(Input input, uint32_t* __restrict__ out, uint32_t n, int exclusive, 
         aggregate += t;
     }
 
-    if (wave == 0) {
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            if (lane == 0) status_store(&status[0], kFlagInclusive | aggregate);
-        } else {
-            if (lane == 0) status_store(&status[tile], kFlagAggregate | aggregate);
-            int64_t base = (int64_t)tile - 1;
-            for (;;) {
-                const int64_t idx = base - lane;
-                uint64_t s = (idx >= 0) ? status_load(&status[idx]) : kFlagInclusive;
-                while (__ballot((s >> 32) == 0) != 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if ((s >> 32) == 0) s = status_load(&status[idx]);
-                }
-                const uint64_t inclMask = __ballot((s >> 32) == 2);
-                if (inclMask) {
-                    const int first = __builtin_ctzll(inclMask);
-                    const uint32_t val = (lane <= first) ? (uint32_t)s : 0u;
-                    prefix += wave_sum(val);
-                    break;
-                }
-                prefix += wave_sum((uint32_t)s);
-                base -= 64;
-            }
-            if (lane == 0) status_store(&status[tile], kFlagInclusive | (uint32_t)(prefix + aggregate));
+    // Look-back: lane t of the workgroup reads the status of tile (base - t).
+    // Every earlier tile publishes its aggregate without waiting on anything,
+    // so spinning until all 256 words are set always ends.
+    if (tile == 0) {
+        if (tid == 0) {
+            status_store(&status[0], kFlagInclusive | aggregate);
+            s_prefix = 0;
         }
-        if (lane == 0) {
+    } else {
+        if (tid == 0) status_store(&status[tile], kFlagAggregate | aggregate);
+        uint32_t prefix = 0;  // accumulated in thread 0
+        int64_t base = (int64_t)tile - 1;
+        for (;;) {
+            const int64_t idx = base - tid;
+            uint64_t s = (idx >= 0) ? status_load(&status[idx]) : kFlagInclusive;
+            while ((s >> 32) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                s = status_load(&status[idx]);
+            }
+            const uint64_t inclMask = __ballot((s >> 32) == 2);
+            if (lane == 0) s_firstIncl[wave] = inclMask ? __builtin_ctzll(inclMask) + wave * 64 : kScanThreads;
+            __syncthreads();
+            int first = kScanThreads;
+#pragma unroll
+            for (int w = 0; w < kScanThreads / 64; ++w) first = min(first, s_firstIncl[w]);
+            const uint32_t ws = wave_sum(tid <= first ? (uint32_t)s : 0u);
+            if (lane == 0) s_lookSum[wave] = ws;
+            __syncthreads();
+            if (tid == 0)
+                for (int w = 0; w < kScanThreads / 64; ++w) prefix += s_lookSum[w];
+            if (first < kScanThreads) break;
+            base -= kScanThreads;
+            __syncthreads();  // s_firstIncl / s_lookSum are rewritten next step
+        }
+        if (tid == 0) {
+            status_store(&status[tile], kFlagInclusive | (uint32_t)(prefix + aggregate));
             s_prefix = prefix;
-            if (total && (uint64_t)(tile + 1) * kScanTile >= n) *total = prefix + aggregate;
         }
     }
+    if (tid == 0 && total && (uint64_t)(tile + 1) * kScanTile >= n) *total = s_prefix + aggregate;
     __syncthreads();
     const uint32_t base = s_prefix + waveExcl;
 

@@ -4,7 +4,8 @@
 // flag kernel, two multi-level scans, a split kernel and two gathers.
 //
 // Per sort:
-//   1. one histogram kernel counts the 8-bit digits of every pass at once;
+//   1. one histogram kernel counts the 8-bit digits of every pass at once
+//      and clears the look-back words;
 //   2. a small kernel turns the counts into per-pass digit offsets;
 //   3. one kernel per pass: each tile (256 lanes x 16..32 keys) ranks its keys by digit with
 //      64-lane ballot matching (stable within the wave), publishes its digit
@@ -27,7 +28,9 @@ constexpr int kMinSortItems = 16;             // keys per lane of the smallest t
 constexpr int kMinSortTile = kSortThreads * kMinSortItems;  // 4096
 constexpr int kRadix = 256;
 constexpr int kHistThreads = 256;
-constexpr int kHistItems = 32;                // keys per thread in the histogram kernel
+constexpr int kHistItems = 32;                // keys per thread per chunk in the histogram kernel
+constexpr uint32_t kHistChunk = kHistThreads * kHistItems;  // 8192 keys
+constexpr int kLookback = 16;                 // earlier tiles whose status a digit loads per step
 constexpr uint32_t kDigitFlagShift = 30;
 constexpr uint32_t kDigitValueMask = (1u << kDigitFlagShift) - 1u;
 constexpr uint32_t kDigitAggregate = 1u << kDigitFlagShift;
@@ -36,7 +39,7 @@ constexpr int kMaxPasses = 4;
 
 struct SortLayout {
     uint32_t passes;
-    uint32_t tiles;
+    uint32_t tiles;       // tiles of the smallest tile size (sizes the status area)
     size_t histOff, offsetOff, counterOff, statusOff, total;
 };
 
@@ -57,40 +60,49 @@ __device__ inline uint32_t digit_of(uint32_t key, uint32_t shift, uint32_t mask)
     return (key >> shift) & mask;
 }
 
+// Clears the digit histograms and the tile counters.
 __global__ void sort_clear_kernel(uint32_t* words, uint32_t count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) words[i] = 0;
 }
 
-// Digit histograms of every pass.  LDS counters per pass, merged with one
-// global atomic per non-empty bin per workgroup.
+// Digit histograms of every pass.  A workgroup counts one chunk of 8192 keys
+// in LDS (all of a thread's loads issued before its counting) and adds its
+// counts to the global histogram, one atomic per non-empty bin.  The same
+// launch clears the look-back status words of the passes.
 __global__ void __launch_bounds__(kHistThreads)
 sort_histogram_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t beginBit, uint32_t endBit,
-                      uint32_t passes, uint32_t* __restrict__ hist) {
+                      uint32_t passes, uint32_t* __restrict__ hist, uint32_t* __restrict__ clearWords,
+                      uint32_t clearCount) {
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kHistThreads) s_hist[i] = 0;
-    __syncthreads();
-    // each thread reads kHistItems keys as 16-byte vectors (rows of 256 x 4)
-    const uint32_t base = blockIdx.x * (uint32_t)(kHistThreads * kHistItems);
-    const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0;
+    for (uint32_t i = blockIdx.x * kHistThreads + threadIdx.x; i < clearCount; i += gridDim.x * kHistThreads)
+        clearWords[i] = 0;
+    const uint32_t base = blockIdx.x * kHistChunk;
+    const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0 && base + kHistChunk <= n;
+    uint32_t key[kHistItems];
+#pragma unroll
     for (int k = 0; k < kHistItems / 4; ++k) {
         const uint32_t i = base + ((uint32_t)k * kHistThreads + threadIdx.x) * 4u;
-        uint32_t key4[4];
-        if (vec && i + 4 <= n) {
+        if (vec) {
             const uint4 v = *reinterpret_cast<const uint4*>(keys + i);
-            key4[0] = v.x; key4[1] = v.y; key4[2] = v.z; key4[3] = v.w;
+            key[4 * k] = v.x; key[4 * k + 1] = v.y; key[4 * k + 2] = v.z; key[4 * k + 3] = v.w;
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) key4[e] = (i + e < n) ? keys[i + e] : 0u;
+            for (int e = 0; e < 4; ++e) key[4 * k + e] = (i + e < n) ? keys[i + e] : 0u;
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kHistItems / 4; ++k) {
+        const uint32_t i = base + ((uint32_t)k * kHistThreads + threadIdx.x) * 4u;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             if (i + e >= n) break;
             for (uint32_t p = 0; p < passes; ++p) {
                 const uint32_t shift = beginBit + 8u * p;
                 const uint32_t bits = min(8u, endBit - shift);
-                const uint32_t d = digit_of(key4[e], shift, (1u << bits) - 1u);
-                atomicAdd(&s_hist[p * kRadix + d], 1u);
+                atomicAdd(&s_hist[p * kRadix + digit_of(key[4 * k + e], shift, (1u << bits) - 1u)], 1u);
             }
         }
     }
@@ -201,32 +213,32 @@ sort_pass_kernel(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict
         s_tileExcl[d] = waveOff + incl - tileCount;
     }
 
-    // Per-digit look-back over the earlier tiles, four tiles per step (their
-    // status loads are independent, so they are in flight together).
+    // Per-digit look-back over the earlier tiles, kLookback tiles per step:
+    // their status loads are issued together, then consumed nearest first
+    // until an inclusive prefix is found.  Inclusive prefixes then advance
+    // kLookback tiles per status round trip.
     uint32_t prefix = 0;
     if (tile > 0) {
         int64_t j = (int64_t)tile - 1;
         for (;;) {
-            uint32_t st[4];
+            uint32_t st[kLookback];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < kLookback; ++q)
                 st[q] = (j - q >= 0) ? digit_status_load(status + (size_t)(j - q) * kRadix + d) : kDigitInclusive;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                while ((st[q] >> kDigitFlagShift) == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    st[q] = digit_status_load(status + (size_t)(j - q) * kRadix + d);
-                }
-            }
             bool done = false;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (done) break;
-                prefix += st[q] & kDigitValueMask;
-                done = (st[q] >> kDigitFlagShift) == 2u;
+            for (int q = 0; q < kLookback; ++q) {
+                if (!done) {
+                    while ((st[q] >> kDigitFlagShift) == 0) {
+                        __builtin_amdgcn_s_sleep(1);
+                        st[q] = digit_status_load(status + (size_t)(j - q) * kRadix + d);
+                    }
+                    prefix += st[q] & kDigitValueMask;
+                    done = (st[q] >> kDigitFlagShift) == 2u;
+                }
             }
             if (done) break;
-            j -= 4;
+            j -= kLookback;
         }
         digit_status_store(myStatus, kDigitInclusive | (prefix + tileCount));
     }
@@ -301,16 +313,18 @@ void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* ke
     uint32_t* counters = reinterpret_cast<uint32_t*>(w + L.counterOff);
     uint32_t* status = reinterpret_cast<uint32_t*>(w + L.statusOff);
 
-    const uint32_t clearWords = (uint32_t)((L.total - 256) / 4);
-    hipLaunchKernelGGL(sort_clear_kernel, dim3(ceil_div(clearWords, 256)), dim3(256), 0, stream,
-                       reinterpret_cast<uint32_t*>(w), clearWords);
+    // Histograms and tile counters are cleared first; the status words of the
+    // passes (packed at this tile size) by the histogram kernel.
+    const uint32_t tiles = ceil_div(n, (uint32_t)items * kSortThreads);
+    const size_t passStride = (size_t)tiles * kRadix;
+    const uint32_t headWords = (uint32_t)(L.statusOff / 4);
+    hipLaunchKernelGGL(sort_clear_kernel, dim3(ceil_div(headWords, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<uint32_t*>(w), headWords);
     FLIP2_LAUNCH_CHECK();
-    const uint32_t histBlocks = ceil_div(n, (uint64_t)kHistThreads * kHistItems);
-    hipLaunchKernelGGL(sort_histogram_kernel, dim3(histBlocks), dim3(kHistThreads), 0, stream, keysIn, n,
-                       beginBit, endBit, L.passes, hist);
+    hipLaunchKernelGGL(sort_histogram_kernel, dim3(ceil_div(n, kHistChunk)), dim3(kHistThreads), 0, stream, keysIn, n,
+                       beginBit, endBit, L.passes, hist, status, (uint32_t)(L.passes * passStride));
     FLIP2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(sort_offsets_kernel, dim3(1), dim3(64 * kMaxPasses), 0, stream, hist, L.passes,
-                       offsets);
+    hipLaunchKernelGGL(sort_offsets_kernel, dim3(1), dim3(64 * kMaxPasses), 0, stream, hist, L.passes, offsets);
     FLIP2_LAUNCH_CHECK();
     if (passEvents) FLIP2_HIP(hipEventRecord(passEvents[0], stream));
 
@@ -324,7 +338,7 @@ void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* ke
         const uint32_t shift = beginBit + 8u * p;
         const uint32_t bits = std::min(8u, endBit - shift);
         const uint32_t mask = (1u << bits) - 1u;
-        uint32_t* st = status + (size_t)p * L.tiles * kRadix;
+        uint32_t* st = status + (size_t)p * passStride;
         if (items == 32)
             hipLaunchKernelGGL(sort_pass_kernel<32>, dim3(ceil_div(n, 32u * kSortThreads)), dim3(kSortThreads), 0,
                                stream, srcK, srcV, dstK, dstV, n, shift, mask, offsets + p * kRadix, st, counters + p);

@@ -32,11 +32,10 @@ __global__ void zero_red_kernel(double* red, int count) {
     if ((int)threadIdx.x < count) red[threadIdx.x] = 0.0;
 }
 
-__global__ void clear_input_kernel(uint16_t* __restrict__ in0, uint8_t* __restrict__ mask, long long nvox) {
+__global__ void clear_input_kernel(uint32_t* __restrict__ in0c, uint8_t* __restrict__ mask, long long nvox) {
     const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvox) return;
-    in0[v * 16] = 0;
-    in0[v * 16 + 1] = 0;
+    in0c[v] = 0u;
     mask[v] = 0;
 }
 
@@ -70,12 +69,54 @@ __device__ inline void block_add2(double a, double c, double* red) {
     }
 }
 
+// Fixed-order variant: each workgroup stores its pair in partials[2 b, 2 b + 1]
+// and sum_pairs_kernel adds them up in workgroup order (deterministic).
+__device__ inline void block_pair_partial(double a, double c, double* partials) {
+    __shared__ double s_a[4], s_c[4];
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        c += __shfl_xor(c, d, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_a[wave] = a;
+        s_c[wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tc = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            ta += s_a[w];
+            tc += s_c[w];
+        }
+        partials[2 * blockIdx.x] = ta;
+        partials[2 * blockIdx.x + 1] = tc;
+    }
+}
+
+// One wave: red[0..1] = sums of the pairs, lane l taking pairs l, l + 64, ...
+__global__ void sum_pairs_kernel(const double* __restrict__ partials, int blocks, double* __restrict__ red) {
+    double a = 0.0, c = 0.0;
+    for (int b = (int)threadIdx.x; b < blocks; b += 64) {
+        a += partials[2 * b];
+        c += partials[2 * b + 1];
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        c += __shfl_xor(c, d, 64);
+    }
+    if (threadIdx.x == 0) {
+        red[0] = a;
+        red[1] = c;
+    }
+}
+
 inline dim3 red_grid(long long n) { return dim3((unsigned)std::min<long long>(kRedBlocks, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); }
 
-// red[0] += sum r^2, red[1] += fluid count (active layout)
+// per workgroup: (sum r^2, fluid count) over its cells (active layout)
 __global__ void __launch_bounds__(256) active_norm_kernel(const uint8_t* __restrict__ cellType,
                                                           const float* __restrict__ r, uint32_t cells,
-                                                          double* __restrict__ red) {
+                                                          double* __restrict__ partials) {
     double a = 0.0, c = 0.0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
         if (cellType[i] == 1) {
@@ -83,7 +124,7 @@ __global__ void __launch_bounds__(256) active_norm_kernel(const uint8_t* __restr
             c += 1.0;
         }
     }
-    block_add2(a, c, red);
+    block_pair_partial(a, c, partials);
 }
 
 __device__ inline float rms_scale(const double* red) {
@@ -93,7 +134,7 @@ __device__ inline float rms_scale(const double* red) {
 }
 
 __global__ void scatter_input_kernel(const uint32_t* __restrict__ activeBricks, const uint8_t* __restrict__ cellType,
-                                     const float* __restrict__ r, uint32_t cells, int n, uint16_t* __restrict__ in0,
+                                     const float* __restrict__ r, uint32_t cells, int n, uint16_t* __restrict__ in0c,
                                      uint8_t* __restrict__ mask, float* __restrict__ scale, const double* red) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const float s = rms_scale(red);
@@ -102,8 +143,8 @@ __global__ void scatter_input_kernel(const uint32_t* __restrict__ activeBricks, 
     int x, y, z;
     dense_coords(activeBricks[i >> 9], i & 511u, n, x, y, z);
     const long long v = ((long long)z * n + y) * n + x;
-    in0[v * 16] = to_bf(r[i] / s);
-    in0[v * 16 + 1] = to_bf(1.0f);
+    in0c[v * 2] = to_bf(r[i] / s);
+    in0c[v * 2 + 1] = to_bf(1.0f);
     mask[v] = 1;
 }
 
@@ -195,15 +236,20 @@ __global__ void __launch_bounds__(256) dense_norm_kernel(const float* __restrict
 }
 
 __global__ void dense_input_kernel(const float* __restrict__ bD, const uint16_t* __restrict__ coefD, long long nvox,
-                                   uint16_t* __restrict__ in0, uint8_t* __restrict__ mask, float* __restrict__ scale,
-                                   const double* __restrict__ red) {
+                                   uint16_t* __restrict__ in0, uint16_t* __restrict__ in0c, uint8_t* __restrict__ mask,
+                                   float* __restrict__ scale, const double* __restrict__ red) {
     const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const float s = rms_scale(red);
     if (v == 0) scale[0] = s;
     if (v >= nvox) return;
     const bool f = coefD[v] != 0;
-    in0[v * 16] = f ? to_bf(bD[v] / s) : (uint16_t)0;
-    in0[v * 16 + 1] = f ? to_bf(1.0f) : (uint16_t)0;
+    const uint16_t c0 = f ? to_bf(bD[v] / s) : (uint16_t)0, c1 = f ? to_bf(1.0f) : (uint16_t)0;
+    if (in0) {
+        in0[v * 16] = c0;
+        in0[v * 16 + 1] = c1;
+    }
+    in0c[v * 2] = c0;
+    in0c[v * 2 + 1] = c1;
     mask[v] = f ? 1 : 0;
 }
 
@@ -265,14 +311,17 @@ void launch_dense_postsmooth(const uint16_t* coefD, const float* r, const float*
 }
 
 void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
-                          int n, uint16_t* in0, uint8_t* mask, float* scale, double* red, hipStream_t s) {
+                          int n, uint16_t* in0c, uint8_t* mask, float* scale, double* red, bool clear, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
     const uint32_t cells = numActive * 512u;
-    hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
-    hipLaunchKernelGGL(clear_input_kernel, g1(nvox, 256), dim3(256), 0, s, in0, mask, nvox);
-    hipLaunchKernelGGL(active_norm_kernel, red_grid(cells), dim3(256), 0, s, cellType, r, cells, red);
+    if (clear)
+        hipLaunchKernelGGL(clear_input_kernel, g1(nvox, 256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(in0c), mask,
+                           nvox);
+    const dim3 rg = red_grid(cells);
+    hipLaunchKernelGGL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
+    hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red);
     hipLaunchKernelGGL(scatter_input_kernel, g1(cells ? cells : 1, 256), dim3(256), 0, s, activeBricks, cellType, r,
-                       cells, n, in0, mask, scale, red);
+                       cells, n, in0c, mask, scale, red);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -332,12 +381,13 @@ void launch_loss_grad(const uint16_t* coefD, const float* res, const double* red
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t* in0, uint8_t* mask, float* scale,
-                        double* red, hipStream_t s) {
+void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t* in0, uint16_t* in0c, uint8_t* mask,
+                        float* scale, double* red, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
     hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
     hipLaunchKernelGGL(dense_norm_kernel, red_grid(nvox), dim3(256), 0, s, bD, coefD, nvox, red);
-    hipLaunchKernelGGL(dense_input_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, in0, mask, scale, red);
+    hipLaunchKernelGGL(dense_input_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, in0, in0c, mask, scale,
+                       red);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -383,6 +433,8 @@ UNet::UNet(int n, hipStream_t stream) : n_(n), n1_(n / 2), n2_(n / 4), stream_(s
     const long long N0 = (long long)n_ * n_ * n_, N1 = (long long)n1_ * n1_ * n1_, N2 = (long long)n2_ * n2_ * n2_;
     in0_.resizeAsync((uint32_t)(N0 * 16), stream_);
     FLIP2_HIP(hipMemsetAsync(in0_.devPtr(), 0, in0_.bytes(), stream_));
+    in0c_.resizeAsync((uint32_t)(N0 * 2), stream_);
+    FLIP2_HIP(hipMemsetAsync(in0c_.devPtr(), 0, in0c_.bytes(), stream_));
     a0_.resizeAsync((uint32_t)(N0 * 16), stream_);
     b0_.resizeAsync((uint32_t)(N0 * 16), stream_);
     p0_.resizeAsync((uint32_t)(N1 * 16), stream_);
@@ -398,9 +450,17 @@ UNet::UNet(int n, hipStream_t stream) : n_(n), n1_(n / 2), n2_(n / 4), stream_(s
     c0_.resizeAsync((uint32_t)(N0 * 16), stream_);
     d0_.resizeAsync((uint32_t)(N0 * 16), stream_);
     mask_.resizeAsync((uint32_t)N0, stream_);
+    mask1_.resizeAsync((uint32_t)N1, stream_);
+    mask2_.resizeAsync((uint32_t)N2, stream_);
+    const int levelN[3] = {n_, n1_, n2_};
+    for (int l = 0; l < 3; ++l) {
+        const uint32_t tiles = nn::conv_tiles(levelN[l]);
+        for (auto* f : {&tileFlags_[0][l], &tileFlags_[1][l], &tileOnes_[l]}) f->resizeAsync(tiles, stream_);
+        FLIP2_HIP(hipMemsetAsync(tileOnes_[l].devPtr(), 1, tiles, stream_));
+    }
     scale_.resizeAsync(4, stream_);
     out_.resizeAsync((uint32_t)N0, stream_);
-    red_.resizeAsync(8, stream_);
+    red_.resizeAsync(8 + 2 * nn::kRedBlocks, stream_);
     initParams(1234);
 }
 
@@ -436,27 +496,68 @@ void UNet::getParams(float* host, size_t count) {
     FLIP2_HIP(hipStreamSynchronize(stream_));
 }
 
-void UNet::conv(int layer, const uint16_t* X, uint16_t* Y, hipStream_t s) {
+void UNet::conv(int layer, const uint16_t* X, uint16_t* Y, const nn::ConvMask& cm, hipStream_t s) {
     const UNetLayer& L = layers_[layer];
     const int n = L.level == 0 ? n_ : (L.level == 1 ? n1_ : n2_);
-    nn::launch_conv3_fwd(X, paramsBf_.devPtr() + L.w, params_.devPtr() + L.b, Y, n, L.cin, L.cout, true, s);
+    nn::launch_conv3_fwd(X, paramsBf_.devPtr() + L.w, params_.devPtr() + L.b, Y, n, L.cin, L.cout, true, cm, s);
 }
 
-void UNet::forward(hipStream_t s) {
-    conv(0, in0_.devPtr(), a0_.devPtr(), s);
-    conv(1, a0_.devPtr(), b0_.devPtr(), s);
+void UNet::setInputMasks(hipStream_t s, bool sparse) {
+    // Level masks (every activation is zero outside its level's mask).  A
+    // sparse forward computes only the tiles that hold masked-in voxels; a
+    // tile computed by an earlier forward and not by this one is cleared.
+    nn::launch_mask_pool(mask_.devPtr(), n_, mask1_.devPtr(), s);
+    nn::launch_mask_pool(mask1_.devPtr(), n1_, mask2_.devPtr(), s);
+    const uint8_t* lm[3] = {mask_.devPtr(), mask1_.devPtr(), mask2_.devPtr()};
+    const int levelN[3] = {n_, n1_, n2_};
+    if (!sparse) {
+        for (int l = 0; l < 3; ++l) cm_[l] = nn::ConvMask{lm[l], nullptr, nullptr};
+        return;
+    }
+    flagSet_ ^= 1;
+    for (int l = 0; l < 3; ++l) {
+        nn::launch_tile_flags(lm[l], levelN[l], tileFlags_[flagSet_][l].devPtr(), s);
+        const uint8_t* prev = tilesDirty_ ? tileOnes_[l].devPtr() : tileFlags_[flagSet_ ^ 1][l].devPtr();
+        cm_[l] = nn::ConvMask{lm[l], tileFlags_[flagSet_][l].devPtr(), prev};
+    }
+    tilesDirty_ = false;
+}
+
+void UNet::forward(hipStream_t s, bool keepActivations) {
+    FLIP2_REQUIRE(!keepActivations || !cm_[0].tiles, kErrState, "U-Net: stored activations need a dense forward");
+    const nn::ConvMask* cm = cm_;
+    if (!cm[0].tiles) tilesDirty_ = true;  // a dense forward writes every tile
+    // the first layer reads the two non-zero input channels only
+    nn::launch_conv3_in2(in0c_.devPtr(), paramsBf_.devPtr() + layers_[0].w, params_.devPtr() + layers_[0].b,
+                         a0_.devPtr(), n_, true, cm[0], s);
+    conv(1, a0_.devPtr(), b0_.devPtr(), cm[0], s);
     nn::launch_avgpool(b0_.devPtr(), p0_.devPtr(), n_, 16, s);
-    conv(2, p0_.devPtr(), a1_.devPtr(), s);
-    conv(3, a1_.devPtr(), b1_.devPtr(), s);
+    conv(2, p0_.devPtr(), a1_.devPtr(), cm[1], s);
+    conv(3, a1_.devPtr(), b1_.devPtr(), cm[1], s);
     nn::launch_avgpool(b1_.devPtr(), p1_.devPtr(), n1_, 32, s);
-    conv(4, p1_.devPtr(), a2_.devPtr(), s);
-    conv(5, a2_.devPtr(), b2_.devPtr(), s);
+    conv(4, p1_.devPtr(), a2_.devPtr(), cm[2], s);
+    conv(5, a2_.devPtr(), b2_.devPtr(), cm[2], s);
+    if (!keepActivations) {
+        const UNetLayer& L6 = layers_[6];
+        const UNetLayer& L8 = layers_[8];
+        const UNetLayer& L9 = layers_[9];
+        nn::launch_conv3_fwd_upcat(b1_.devPtr(), b2_.devPtr(), paramsBf_.devPtr() + L6.w, params_.devPtr() + L6.b,
+                                   c1_.devPtr(), n1_, 64, 32, 32, true, cm[1], s);
+        conv(7, c1_.devPtr(), d1_.devPtr(), cm[1], s);
+        nn::launch_conv3_fwd_upcat(b0_.devPtr(), d1_.devPtr(), paramsBf_.devPtr() + L8.w, params_.devPtr() + L8.b,
+                                   c0_.devPtr(), n_, 32, 16, 16, true, cm[0], s);
+        const nn::HeadArgs head{params_.devPtr() + headW_, params_.devPtr() + headB_, mask_.devPtr(), scale_.devPtr(),
+                                out_.devPtr()};
+        nn::launch_conv3_fwd_head(c0_.devPtr(), paramsBf_.devPtr() + L9.w, params_.devPtr() + L9.b, nullptr, n_, 16,
+                                  true, head, cm[0], s);
+        return;
+    }
     nn::launch_upcat(b2_.devPtr(), b1_.devPtr(), u1_.devPtr(), n1_, 64, 32, s);
-    conv(6, u1_.devPtr(), c1_.devPtr(), s);
-    conv(7, c1_.devPtr(), d1_.devPtr(), s);
+    conv(6, u1_.devPtr(), c1_.devPtr(), cm[1], s);
+    conv(7, c1_.devPtr(), d1_.devPtr(), cm[1], s);
     nn::launch_upcat(d1_.devPtr(), b0_.devPtr(), u0_.devPtr(), n_, 32, 16, s);
-    conv(8, u0_.devPtr(), c0_.devPtr(), s);
-    conv(9, c0_.devPtr(), d0_.devPtr(), s);
+    conv(8, u0_.devPtr(), c0_.devPtr(), cm[0], s);
+    conv(9, c0_.devPtr(), d0_.devPtr(), cm[0], s);
     nn::launch_head_fwd(d0_.devPtr(), params_.devPtr() + headW_, params_.devPtr() + headB_, mask_.devPtr(),
                         scale_.devPtr(), out_.devPtr(), (long long)n_ * n_ * n_, 16, s);
 }
@@ -515,11 +616,12 @@ void UNet::adamStep(float lr, hipStream_t s) {
 }
 
 void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
-                       float* z, int gridN, hipStream_t s) {
+                       float* z, int gridN, bool newSystem, hipStream_t s) {
     FLIP2_REQUIRE(gridN == n_, kErrArgument, "U-Net grid size differs from the simulation grid");
-    nn::launch_scatter_input(numActive, activeBricks, cellType, r, n_, in0_.devPtr(), mask_.devPtr(), scale_.devPtr(),
-                             red_.devPtr(), s);
-    forward(s);
+    nn::launch_scatter_input(numActive, activeBricks, cellType, r, n_, in0c_.devPtr(), mask_.devPtr(), scale_.devPtr(),
+                             red_.devPtr(), newSystem, s);
+    if (newSystem) setInputMasks(s, true);
+    forward(s, false);
     nn::launch_gather_output(numActive, activeBricks, cellType, out_.devPtr(), n_, z, s);
 }
 

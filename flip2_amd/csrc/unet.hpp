@@ -14,6 +14,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "unet_kernels.hpp"
 
 #include <string>
 #include <vector>
@@ -39,26 +40,36 @@ public:
     void getParams(float* host, size_t count);
 
     // Input assembly on the dense grid: ch0 = r / s, ch1 = mask; out = z.
-    uint16_t* input() { return in0_.devPtr(); }
+    uint16_t* input() { return in0_.devPtr(); }          // [n^3][16] (read by the first layer's backward)
+    uint16_t* inputCompact() { return in0c_.devPtr(); }  // [n^3][2]: the channels the forward reads
     uint8_t* mask() { return mask_.devPtr(); }
     float* scale() { return scale_.devPtr(); }  // device scalar s
     float* output() { return out_.devPtr(); }   // FP32 [n^3]
 
-    void forward(hipStream_t s);
+    // keepActivations: store every activation the backward pass reads; without
+    // it the decoder reads its concatenated inputs in place and the output
+    // head runs in the last convolution (same values, less traffic).
+    void forward(hipStream_t s, bool keepActivations = true);
+    // Level masks (and, if sparse, tile flags) from mask(); call after the
+    // input assembly of a new system, before forward().  A forward that
+    // stores activations needs sparse = false.
+    void setInputMasks(hipStream_t s, bool sparse);
     // Accumulates parameter gradients for dL/dout (FP32 [n^3]).
     void backward(const float* dOut, hipStream_t s);
     void zeroGrad(hipStream_t s);
     void adamStep(float lr, hipStream_t s);
     const float* gradDev() const { return grad_.devPtr(); }
 
-    // Preconditioner apply on the solver's active-brick vectors.
+    // Preconditioner apply on the solver's active-brick vectors.  newSystem:
+    // the fluid cells may differ from the previous apply's (masks and tile
+    // flags are rebuilt); otherwise only r changed.
     void applyActive(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
-                     float* z, int gridN, hipStream_t s);
+                     float* z, int gridN, bool newSystem, hipStream_t s);
 
     const std::vector<UNetLayer>& layers() const { return layers_; }
 
 private:
-    void conv(int layer, const uint16_t* X, uint16_t* Y, hipStream_t s);
+    void conv(int layer, const uint16_t* X, uint16_t* Y, const nn::ConvMask& cm, hipStream_t s);
     void convBackward(int layer, float* dY, const uint16_t* Y, const uint16_t* X, float* dX, hipStream_t s);
 
     int n_, n1_, n2_;
@@ -69,8 +80,15 @@ private:
     DeviceVec<float> params_, grad_, adamM_, adamV_;
     DeviceVec<uint16_t> paramsBf_;
     // activations (BF16, channels-last)
-    DeviceVec<uint16_t> in0_, a0_, b0_, p0_, a1_, b1_, p1_, a2_, b2_, u1_, c1_, d1_, u0_, c0_, d0_;
+    DeviceVec<uint16_t> in0_, in0c_, a0_, b0_, p0_, a1_, b1_, p1_, a2_, b2_, u1_, c1_, d1_, u0_, c0_, d0_;
     DeviceVec<uint8_t> mask_;
+    // masks of levels 1 and 2; per level, tile flags of the current and the
+    // previous sparse forward (two sets, swapped each time) and all-ones flags
+    DeviceVec<uint8_t> mask1_, mask2_;
+    DeviceVec<uint8_t> tileFlags_[2][3], tileOnes_[3];
+    int flagSet_ = 0;
+    bool tilesDirty_ = true;  // a dense forward (or allocation) left values everywhere
+    nn::ConvMask cm_[3] = {};
     DeviceVec<float> scale_, out_;
     // gradients (FP32)
     DeviceVec<float> gd0_, gc0_, gu0_, gd1_, gc1_, gu1_, gb2_, ga2_, gp1_, gb1_, ga1_, gp0_, gb0_, ga0_;
@@ -80,10 +98,11 @@ private:
 
 // Dense-grid helpers shared by the trainer and the preconditioner apply.
 namespace nn {
-// Active-brick vector -> dense grid: in0 channels 0/1 (r/s, fluid mask),
+// Active-brick vector -> dense grid: in0c ([n^3][2]: r/s, fluid mask),
 // mask; s = RMS of r over fluid cells is computed on the device into *scale.
+// clear: zero in0c and mask everywhere first (the fluid cells changed).
 void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
-                          int n, uint16_t* in0, uint8_t* mask, float* scale, double* red, hipStream_t s);
+                          int n, uint16_t* in0c, uint8_t* mask, float* scale, double* red, bool clear, hipStream_t s);
 // z (active layout) = out (dense) on fluid cells, 0 elsewhere.
 void launch_gather_output(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* out,
                           int n, float* z, hipStream_t s);
@@ -108,9 +127,10 @@ void launch_loss_grad(const uint16_t* coefD, const float* res, const double* red
 void launch_dense_presmooth(const uint16_t* coefD, const float* r, float omega, float* r1, int n, hipStream_t s);
 void launch_dense_postsmooth(const uint16_t* coefD, const float* r, const float* y, float omega, float* z, int n,
                              hipStream_t s);
-// in0/mask/scale from a dense right-hand side (training input).
-void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t* in0, uint8_t* mask, float* scale,
-                        double* red, hipStream_t s);
+// Network input from a dense right-hand side: in0c ([n^3][2]) always, in0
+// ([n^3][16], for the first layer's weight gradient) unless null; mask; scale.
+void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t* in0, uint16_t* in0c, uint8_t* mask,
+                        float* scale, double* red, hipStream_t s);
 }  // namespace nn
 
 }  // namespace flip2

@@ -34,29 +34,110 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b <<
 // MODE 0: Y (BF16) = act(conv + bias); MODE 1: Yf (FP32) = conv (no bias, no
 // activation), used for the data gradient with transposed, mirrored weights.
 //
-// A workgroup computes a 4 (z) x 8 (y) x 8 (x) block of output voxels, wave w
-// the z-slice w as four 16-voxel MFMA columns (two x-rows each).  The input
-// block with its one-voxel halo (6 x 10 x 10 voxels) is staged in LDS 16
-// channels at a time, so each input voxel is fetched from memory once per
-// workgroup instead of once per tap; the 27 taps then read their B fragments
-// (4 channels = 8 bytes per lane) from LDS.  Weights (A fragments) are read
-// from global memory, where every wave of the GPU shares them through the
-// caches.
-constexpr int kTX = 8, kTY = 8, kTZ = 4;
-constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHalo = kHX * kHY * kHZ;  // 600 voxels
+// CA > 0: the input is the decoder's concatenation [up(Xc), X] read in
+// place: channels [0, CA) from the half-resolution tensor Xc (nearest
+// upsampling, voxel (x, y, z) reads (x/2, y/2, z/2)), channels [CA, CIN) from
+// X; the concatenated tensor is never written.
+//
+// head.out != nullptr (COUT == 16): the 1x1 output head is applied in the
+// epilogue, out[v] = mask[v] ? (b + sum_c w[c] Y[v][c]) * scale : 0 with the
+// BF16-rounded Y and the same summation order as head_fwd_kernel; Y may then
+// be null.
+//
+// Tile order: workgroups are dealt to the 8 XCDs round robin by linear id, so
+// the id is remapped to give each XCD one contiguous range of tiles (z-slabs)
+// and the halo rows that neighbouring tiles share are re-read from that XCD's
+// L2 instead of HBM.
+//
+// A workgroup computes a 16 (x) x 4 (y) x 4 (z) block of output voxels;
+// wave w owns the z-slice w as four MFMA columns, one 16-voxel x-row each.
+// The input block with its one-voxel halo (18 x 6 x 6 voxels) is staged in
+// LDS 16 channels at a time, as two planes of 8 channels ([chunk][voxel], 16
+// bytes per voxel), so each input voxel is fetched from memory once per
+// workgroup instead of once per tap.  MFMA is v_mfma_f32_16x16x32_bf16 with
+// K = two taps x 16 channels: lane (col, kq) holds tap 2p + (kq >> 1),
+// channels 8 (kq & 1) .. +7 of output-row voxel col.  A B fragment is one
+// 16-byte LDS read, and the 16 lanes of a quarter wave read 16 consecutive
+// voxels of one plane: 256 contiguous bytes, which cover the LDS banks once
+// (with [voxel][16 channels] rows and 8-byte reads, bank conflicts made LDS
+// the bound: ~16 stall cycles per read).  Weights (A fragments, 16 bytes per
+// lane) are read from global memory, where every wave of the GPU shares them
+// through the caches.
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHalo = kHX * kHY * kHZ;  // 648 voxels
+constexpr int kTapPairs = 14;  // taps 0..26 in pairs; tap 27 is zero
 
-template <int CIN, int COUT, int MODE = 0>
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+inline dim3 conv_grid(int n) {
+    return dim3((unsigned)((n + kTX - 1) / kTX), (unsigned)((n + kTY - 1) / kTY), (unsigned)((n + kTZ - 1) / kTZ));
+}
+
+// Workgroup id -> tile origin (and output-channel group mg when SPLIT
+// workgroups share a tile: grid.x = tiles in x * SPLIT), each XCD taking a
+// contiguous range of tiles.
+template <int SPLIT = 1>
+__device__ inline void conv_tile(int& bx, int& by, int& bz, int& mg) {
+    const uint32_t gx = gridDim.x, gxy = gridDim.x * gridDim.y, total = gxy * gridDim.z;
+    uint32_t lin = blockIdx.x + gx * blockIdx.y + gxy * blockIdx.z;
+    if ((total & 7u) == 0) lin = (lin & 7u) * (total >> 3) + (lin >> 3);
+    mg = (int)(lin % SPLIT);
+    const uint32_t tile = lin / SPLIT, tilesX = gx / SPLIT;
+    bx = (int)(tile % tilesX) * kTX;
+    by = (int)((tile / tilesX) % gridDim.y) * kTY;
+    bz = (int)(tile / (tilesX * gridDim.y)) * kTZ;
+}
+
+// Tiles without masked-in voxels: skipped, or cleared when they may hold
+// values from an earlier launch.  Returns true when the tile is not computed.
+// With SPLIT workgroups per tile each clears its own channel group.
+template <int COUT, int SPLIT = 1>
+__device__ inline bool skip_tile(const ConvMask& cm, int bx, int by, int bz, int mg, int n, uint16_t* Y,
+                                 float* headOut) {
+    if (!cm.tiles) return false;
+    const uint32_t tilesX = gridDim.x / SPLIT;
+    const uint32_t t = ((uint32_t)(bz / kTZ) * gridDim.y + (uint32_t)(by / kTY)) * tilesX + (uint32_t)(bx / kTX);
+    if (cm.tiles[t]) return false;
+    if (cm.prevTiles && cm.prevTiles[t]) {
+        const int x = bx + (threadIdx.x & 15), y = by + ((threadIdx.x >> 4) & 3), z = bz + (threadIdx.x >> 6);
+        if (x < n && y < n && z < n) {
+            const long long v = ((long long)z * n + y) * n + x;
+            if (Y) {
+                constexpr int CW = COUT / SPLIT;
+                uint4* dst = reinterpret_cast<uint4*>(Y + v * COUT + mg * CW);
+#pragma unroll
+                for (int q = 0; q < CW / 8; ++q) dst[q] = make_uint4(0u, 0u, 0u, 0u);
+            }
+            if (headOut) headOut[v] = 0.f;
+        }
+    }
+    return true;
+}
+
+__device__ inline int halo_index(int wave, int j, int col, int tap) {
+    const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // offsets + 1
+    return ((wave + dz) * kHY + (j + dy)) * kHX + (col + dx);
+}
+
+template <int CIN, int COUT, int MODE = 0, int CA = 0, int SPLIT = 1>
 __global__ void __launch_bounds__(256)
-conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
-                 uint16_t* __restrict__ Y, float* __restrict__ Yf, int n, int relu) {
-    constexpr int MB = COUT / 16;
-    __shared__ __align__(16) uint16_t sIn[kHalo * 16];  // [halo voxel][16 channels]
+conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc, const uint16_t* __restrict__ Wt,
+                 const float* __restrict__ bias, uint16_t* __restrict__ Y, float* __restrict__ Yf, int n, int relu,
+                 HeadArgs head, ConvMask cm) {
+    constexpr int MB = COUT / 16 / SPLIT;  // 16-channel output blocks of this workgroup
+    constexpr int CB = CIN - CA;           // channels read from X
+    static_assert(CA % 16 == 0 && CB % 16 == 0, "channel groups of 16");
+    static_assert(MB >= 1 && MB * 16 * SPLIT == COUT, "output channels split into whole 16-channel blocks");
+    __shared__ __align__(16) uint4 sIn[2][kHalo];  // [8-channel chunk][halo voxel]
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
-    const int bx = blockIdx.x * kTX, by = blockIdx.y * kTY, bz = blockIdx.z * kTZ;
-    // output voxel of lane column `col` in MFMA column j: (x, y) = (col & 7, 2j + (col >> 3)), z = wave
-    const int lx = col & 7, lyo = col >> 3;
+    const int chunk = kq & 1, tapOdd = kq >> 1;
+    int bx, by, bz, mg;
+    conv_tile<SPLIT>(bx, by, bz, mg);
+    if (skip_tile<COUT, SPLIT>(cm, bx, by, bz, mg, n, MODE == 0 ? Y : nullptr, MODE == 0 ? head.out : nullptr))
+        return;
+    const int m0 = mg * MB;  // first output block of this workgroup
 
     v4f acc[MB][4];
 #pragma unroll
@@ -67,45 +148,96 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt
 #pragma unroll 1
     for (int c0 = 0; c0 < CIN; c0 += 16) {
         __syncthreads();
-        // stage 600 voxels x 16 channels (32 bytes = 2 x uint4 each)
+        // stage 648 voxels x 16 channels as two 8-channel planes
         for (int t = threadIdx.x; t < kHalo * 2; t += 256) {
             const int hv = t >> 1, half = t & 1;
             const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
             const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n)
-                v = *reinterpret_cast<const uint4*>(X + (((long long)z * n + y) * n + x) * CIN + c0 + 8 * half);
-            *reinterpret_cast<uint4*>(&sIn[hv * 16 + 8 * half]) = v;
+            if (x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n) {
+                if (CA > 0 && c0 < CA) {
+                    const int nc = n >> 1;
+                    v = *reinterpret_cast<const uint4*>(
+                        Xc + (((long long)(z >> 1) * nc + (y >> 1)) * nc + (x >> 1)) * CA + c0 + 8 * half);
+                } else {
+                    v = *reinterpret_cast<const uint4*>(X + (((long long)z * n + y) * n + x) * CB + (c0 - CA) + 8 * half);
+                }
+            }
+            sIn[half][hv] = v;
         }
         __syncthreads();
-#pragma unroll 3
-        for (int tap = 0; tap < 27; ++tap) {
-            const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // offsets + 1
-            v4s b[4];
+#pragma unroll 2
+        for (int p = 0; p < kTapPairs; ++p) {
+            const int tap = 2 * p + tapOdd;
+            const bool live = tap < 27;
+            v8bf b[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int hv = ((wave + dz) * kHY + (2 * j + lyo + dy)) * kHX + (lx + dx);
-                b[j] = *reinterpret_cast<const v4s*>(&sIn[hv * 16 + 4 * kq]);
+                const uint4 raw = live ? sIn[chunk][halo_index(wave, j, col, tap)] : make_uint4(0u, 0u, 0u, 0u);
+                b[j] = __builtin_bit_cast(v8bf, raw);
             }
 #pragma unroll
             for (int m = 0; m < MB; ++m) {
-                const uint2 raw =
-                    *reinterpret_cast<const uint2*>(Wt + ((size_t)(tap * COUT + 16 * m + col) * CIN) + c0 + 4 * kq);
-                const v4s a = __builtin_bit_cast(v4s, raw);
+                const uint4 raw = live ? *reinterpret_cast<const uint4*>(
+                                             Wt + ((size_t)(tap * COUT + 16 * (m0 + m) + col) * CIN) + c0 + 8 * chunk)
+                                       : make_uint4(0u, 0u, 0u, 0u);
+                const v8bf a = __builtin_bit_cast(v8bf, raw);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[j], acc[m][j], 0, 0, 0);
+                for (int j = 0; j < 4; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
             }
         }
     }
-    const int z = bz + wave;
+    // lane (col, kq) holds output channels 16 m + 4 kq .. +3 of voxel
+    // (bx + col, by + j, bz + wave)
+    const int z = bz + wave, x = bx + col;
+    if (MODE == 0 && COUT == 16 && head.out) {
+        // fused head: gather the 16 channels of each voxel in its kq == 0 lane
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int y = by + j;
+            const bool inside = x < n && y < n && z < n;
+            const long long v = ((long long)z * n + y) * n + x;
+            const bool live = inside && (!cm.voxel || cm.voxel[v]);
+            float yv[4];
+            uint16_t o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float t = acc[0][j][r] + bias[4 * kq + r];
+                if (relu) t = fmaxf(t, 0.f);
+                o[r] = live ? f2bf(t) : (uint16_t)0;
+                yv[r] = bf2f(o[r]);
+            }
+            if (Y && inside) {
+                uint2 packed;
+                packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+                packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+                *reinterpret_cast<uint2*>(Y + v * COUT + 4 * kq) = packed;
+            }
+            float ch[16];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ch[r] = yv[r];
+#pragma unroll
+            for (int q = 1; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ch[4 * q + r] = __shfl(yv[r], col + 16 * q, 64);
+            if (kq == 0 && inside) {
+                float s = head.b[0];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) s = fmaf(ch[c], head.w[c], s);
+                head.out[v] = head.mask[v] ? s * head.scale[0] : 0.f;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int x = bx + lx, y = by + 2 * j + lyo;
+        const int y = by + j;
         if (x >= n || y >= n || z >= n) continue;
         const long long v = ((long long)z * n + y) * n + x;
+        const bool live = MODE == 1 || !cm.voxel || cm.voxel[v];
 #pragma unroll
         for (int m = 0; m < MB; ++m) {
-            const int co = 16 * m + 4 * kq;
+            const int co = 16 * (m0 + m) + 4 * kq;
             if (MODE == 1) {
                 *reinterpret_cast<float4*>(Yf + v * COUT + co) =
                     make_float4(acc[m][j][0], acc[m][j][1], acc[m][j][2], acc[m][j][3]);
@@ -116,7 +248,7 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt
             for (int r = 0; r < 4; ++r) {
                 float yv = acc[m][j][r] + bias[co + r];
                 if (relu) yv = fmaxf(yv, 0.f);
-                o[r] = f2bf(yv);
+                o[r] = live ? f2bf(yv) : (uint16_t)0;
             }
             uint2 packed;
             packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
@@ -124,6 +256,109 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt
             *reinterpret_cast<uint2*>(Y + v * COUT + co) = packed;
         }
     }
+}
+
+// First convolution, whose input has two channels (the scaled residual and
+// the fluid mask; the layer's other 14 input channels are zero): X2 is
+// [voxel][2] BF16 and the 27 taps x 2 channels form the reduction dimension
+// directly, k = 2 tap + channel, 54 of 64 slots in two K = 32 MFMA blocks
+// (instead of 27 blocks of 16 channels of which 14 are zero).  Lane (col, kq)
+// of K block kb holds taps 16 kb + 4 kq .. +3.  Same tiles, tile order and
+// epilogue as conv3_fwd_kernel (MODE 0, 16 output channels).
+__global__ void __launch_bounds__(256)
+conv3_in2_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+                 uint16_t* __restrict__ Y, int n, int relu, ConvMask cm) {
+    __shared__ uint32_t sIn[kHalo];  // [halo voxel] = 2 BF16 channels
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    int bx, by, bz, mg;
+    conv_tile(bx, by, bz, mg);
+    if (skip_tile<16>(cm, bx, by, bz, mg, n, Y, nullptr)) return;
+    for (int t = threadIdx.x; t < kHalo; t += 256) {
+        const int hx = t % kHX, hy = (t / kHX) % kHY, hz = t / (kHX * kHY);
+        const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
+        uint32_t v = 0u;
+        if (x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n)
+            v = reinterpret_cast<const uint32_t*>(X2)[((long long)z * n + y) * n + x];
+        sIn[t] = v;
+    }
+    // A fragments: W[tap][co = col][ci] for the lane's four taps, ci = 0, 1
+    v8bf a[2];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        const int t0 = 16 * kb + 4 * kq;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int tap = t0 + (e >> 1);
+            const uint16_t w = tap < 27 ? Wt[((size_t)tap * 16 + col) * 16 + (e & 1)] : (uint16_t)0;
+            a[kb][e] = __builtin_bit_cast(__bf16, w);
+        }
+    }
+    __syncthreads();
+    v4f acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        const int t0 = 16 * kb + 4 * kq;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t q[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int tap = t0 + e;
+                q[e] = tap < 27 ? sIn[halo_index(wave, j, col, tap)] : 0u;
+            }
+            const v8bf b = __builtin_bit_cast(v8bf, make_uint4(q[0], q[1], q[2], q[3]));
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], b, acc[j], 0, 0, 0);
+        }
+    }
+    const int z = bz + wave, x = bx + col;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = by + j;
+        if (x >= n || y >= n || z >= n) continue;
+        const long long v = ((long long)z * n + y) * n + x;
+        const bool live = !cm.voxel || cm.voxel[v];
+        const int co = 4 * kq;
+        uint16_t o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float yv = acc[j][r] + bias[co + r];
+            if (relu) yv = fmaxf(yv, 0.f);
+            o[r] = live ? f2bf(yv) : (uint16_t)0;
+        }
+        uint2 packed;
+        packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+        packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+        *reinterpret_cast<uint2*>(Y + v * 16 + co) = packed;
+    }
+}
+
+__global__ void mask_pool_kernel(const uint8_t* __restrict__ m, int n, uint8_t* __restrict__ out) {
+    const int nc = n >> 1;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nc * nc * nc) return;
+    const int x = (int)(i % nc), y = (int)((i / nc) % nc), z = (int)(i / ((long long)nc * nc));
+    uint8_t any = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        any |= m[((long long)(2 * z + (c >> 2)) * n + 2 * y + ((c >> 1) & 1)) * n + 2 * x + (c & 1)];
+    out[i] = any ? 1 : 0;
+}
+
+// One thread per convolution tile: any masked-in voxel in its 16x4x4 block.
+__global__ void tile_flags_kernel(const uint8_t* __restrict__ m, int n, uint32_t tilesX, uint32_t tilesY,
+                                  uint32_t tiles, uint8_t* __restrict__ flags) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) return;
+    const int bx = (int)(t % tilesX) * kTX, by = (int)((t / tilesX) % tilesY) * kTY, bz = (int)(t / (tilesX * tilesY)) * kTZ;
+    uint8_t any = 0;
+    for (int z = bz; z < min(bz + kTZ, n); ++z)
+        for (int y = by; y < min(by + kTY, n); ++y)
+            for (int x = bx; x < min(bx + kTX, n); ++x) any |= m[((long long)z * n + y) * n + x];
+    flags[t] = any ? 1 : 0;
 }
 
 // g <- g * (act > 0) in place, and a BF16 copy of the result for the
@@ -429,14 +664,24 @@ inline dim3 g1(long long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs))
 }
 
 void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin, int cout,
-                      bool relu, hipStream_t s) {
+                      bool relu, const ConvMask& cm, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
     (void)nvox;
-    const dim3 grid((unsigned)((n + kTX - 1) / kTX), (unsigned)((n + kTY - 1) / kTY), (unsigned)((n + kTZ - 1) / kTZ));
+    const dim3 grid = conv_grid(n);
+    const HeadArgs noHead{};
+    // Small grids (the coarse levels) split the output channels over 2 or 4
+    // workgroups per tile, so there are enough workgroups to fill the chip.
+    const uint32_t tiles = grid.x * grid.y * grid.z;
+    const int split = tiles >= 2048u ? 1 : (tiles >= 1024u ? 2 : 4);
+#define CONV_LAUNCH(CI, CO, SP)                                                                                     \
+    hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, 0, 0, SP>), dim3(grid.x * SP, grid.y, grid.z), dim3(256), 0, s, X, \
+                       (const uint16_t*)nullptr, W, bias, Y, (float*)nullptr, n, relu ? 1 : 0, noHead, cm)
 #define CONV_CASE(CI, CO)                                                                                   \
     if (cin == CI && cout == CO) {                                                                          \
-        hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, 0>), grid, dim3(256), 0, s, X, W, bias, Y, (float*)nullptr, \
-                           n, relu ? 1 : 0);                                                                \
+        const int sp = std::min(split, CO / 16);                                                            \
+        if (sp == 4) CONV_LAUNCH(CI, CO, (CO >= 64 ? 4 : 1));                                               \
+        else if (sp == 2) CONV_LAUNCH(CI, CO, (CO >= 32 ? 2 : 1));                                          \
+        else CONV_LAUNCH(CI, CO, 1);                                                                        \
         FLIP2_LAUNCH_CHECK();                                                                               \
         return;                                                                                             \
     }
@@ -450,7 +695,60 @@ void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, u
     CONV_CASE(32, 16)
     CONV_CASE(64, 32)
 #undef CONV_CASE
+#undef CONV_LAUNCH
     throw Error(kErrArgument, "conv3: unsupported channel pair");
+}
+
+void launch_conv3_fwd_upcat(const uint16_t* Xskip, const uint16_t* Xcoarse, const uint16_t* W, const float* bias,
+                            uint16_t* Y, int n, int ca, int cb, int cout, bool relu, const ConvMask& cm,
+                            hipStream_t s) {
+    FLIP2_REQUIRE(n % 2 == 0, kErrArgument, "conv3 with upsampled input: even grid size");
+    const dim3 grid = conv_grid(n);
+    if (ca == 64 && cb == 32 && cout == 32) {
+        hipLaunchKernelGGL((conv3_fwd_kernel<96, 32, 0, 64>), grid, dim3(256), 0, s, Xskip, Xcoarse, W, bias, Y,
+                           (float*)nullptr, n, relu ? 1 : 0, HeadArgs{}, cm);
+    } else if (ca == 32 && cb == 16 && cout == 16) {
+        hipLaunchKernelGGL((conv3_fwd_kernel<48, 16, 0, 32>), grid, dim3(256), 0, s, Xskip, Xcoarse, W, bias, Y,
+                           (float*)nullptr, n, relu ? 1 : 0, HeadArgs{}, cm);
+    } else {
+        throw Error(kErrArgument, "conv3 with upsampled input: unsupported channel counts");
+    }
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
+                      const ConvMask& cm, hipStream_t s) {
+    const dim3 grid = conv_grid(n);
+    hipLaunchKernelGGL(conv3_in2_kernel, grid, dim3(256), 0, s, X2, W, bias, Y, n, relu ? 1 : 0, cm);
+    FLIP2_LAUNCH_CHECK();
+}
+
+uint32_t conv_tiles(int n) {
+    const dim3 g = conv_grid(n);
+    return g.x * g.y * g.z;
+}
+
+void launch_mask_pool(const uint8_t* m, int n, uint8_t* out, hipStream_t s) {
+    const long long nc = n / 2;
+    hipLaunchKernelGGL(mask_pool_kernel, g1(nc * nc * nc, 256), dim3(256), 0, s, m, n, out);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_tile_flags(const uint8_t* m, int n, uint8_t* flags, hipStream_t s) {
+    const dim3 g = conv_grid(n);
+    const uint32_t tiles = g.x * g.y * g.z;
+    hipLaunchKernelGGL(tile_flags_kernel, g1(tiles, 256), dim3(256), 0, s, m, n, g.x, g.y, tiles, flags);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_conv3_fwd_head(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin,
+                           bool relu, const HeadArgs& head, const ConvMask& cm, hipStream_t s) {
+    FLIP2_REQUIRE(cin == 16 && head.out && head.w && head.b && head.mask && head.scale, kErrArgument,
+                  "conv3 with fused head: 16 input channels and all head arguments");
+    const dim3 grid = conv_grid(n);
+    hipLaunchKernelGGL((conv3_fwd_kernel<16, 16, 0>), grid, dim3(256), 0, s, X, (const uint16_t*)nullptr, W, bias, Y,
+                       (float*)nullptr, n, relu ? 1 : 0, head, cm);
+    FLIP2_LAUNCH_CHECK();
 }
 
 void launch_conv3_dgrad(const uint16_t* dPb, const float* W32, uint16_t* WT, float* dX, int n, int cin, int cout,
@@ -460,12 +758,12 @@ void launch_conv3_dgrad(const uint16_t* dPb, const float* W32, uint16_t* WT, flo
     hipLaunchKernelGGL(transpose_weights_kernel, g1(nw, 256), dim3(256), 0, s, W32, WT, cin, cout);
     FLIP2_LAUNCH_CHECK();
     (void)nvox;
-    const dim3 grid((unsigned)((n + kTX - 1) / kTX), (unsigned)((n + kTY - 1) / kTY), (unsigned)((n + kTZ - 1) / kTZ));
+    const dim3 grid = conv_grid(n);
     // the forward kernel with the roles of the channel counts swapped
 #define DGRAD_CASE(CI, CO)                                                                                      \
     if (cin == CI && cout == CO) {                                                                              \
-        hipLaunchKernelGGL((conv3_fwd_kernel<CO, CI, 1>), grid, dim3(256), 0, s, dPb, WT, (const float*)nullptr, \
-                           (uint16_t*)nullptr, dX, n, 0);                                                       \
+        hipLaunchKernelGGL((conv3_fwd_kernel<CO, CI, 1>), grid, dim3(256), 0, s, dPb, (const uint16_t*)nullptr, WT,  \
+                           (const float*)nullptr, (uint16_t*)nullptr, dX, n, 0, HeadArgs{}, ConvMask{});       \
         FLIP2_LAUNCH_CHECK();                                                                                   \
         return;                                                                                                 \
     }

@@ -7,10 +7,47 @@
 namespace flip2 {
 namespace nn {
 
+// Masking of a convolution's output (all fields may be null):
+//   voxel: the level's mask; Y = mask ? act(conv + bias) : 0;
+//   tiles: per 16x4x4 output tile (x fastest), non-zero where the tile holds
+//          a masked-in voxel; other tiles are not computed ...
+//   prevTiles: ... and, where this flag is set (the tile may still hold
+//          values from an earlier launch), are written with zeros.
+struct ConvMask {
+    const uint8_t* voxel;
+    const uint8_t* tiles;
+    const uint8_t* prevTiles;
+};
 // Y = act(conv3x3x3(X, W) + bias), MFMA BF16 with FP32 accumulation.
 // W: [27][cout][cin] BF16 (tap = (dz*3 + dy)*3 + dx, offsets d-1).
 void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin, int cout,
-                      bool relu, hipStream_t s);
+                      bool relu, const ConvMask& mask, hipStream_t s);
+// Arguments of the 1x1 output head fused into the last convolution.
+struct HeadArgs {
+    const float* w;
+    const float* b;
+    const uint8_t* mask;
+    const float* scale;
+    float* out;
+};
+// Decoder convolution reading concat(up2(Xcoarse) [ca channels], Xskip [cb
+// channels]) without materialising it; Y = ReLU(conv + bias).
+void launch_conv3_fwd_upcat(const uint16_t* Xskip, const uint16_t* Xcoarse, const uint16_t* W, const float* bias,
+                            uint16_t* Y, int n, int ca, int cb, int cout, bool relu, const ConvMask& mask,
+                            hipStream_t s);
+// First layer: 16 output channels from an input whose only non-zero
+// channels are the first two, given compactly as X2 [voxel][2] BF16; W is the
+// layer's [27][16][16] weights (input channels 2..15 unused).
+void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
+                      const ConvMask& mask, hipStream_t s);
+// 16 -> 16 convolution with the output head in its epilogue (Y may be null).
+void launch_conv3_fwd_head(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin,
+                           bool relu, const HeadArgs& head, const ConvMask& mask, hipStream_t s);
+// Coarse mask: out[v] = any of the 8 children of v in m (n^3 -> (n/2)^3).
+void launch_mask_pool(const uint8_t* m, int n, uint8_t* out, hipStream_t s);
+// Per-tile flags of a level mask for the convolution tiles (see ConvMask).
+void launch_tile_flags(const uint8_t* m, int n, uint8_t* flags, hipStream_t s);
+uint32_t conv_tiles(int n);
 // dX (FP32) = conv3x3x3^T(dP) from the BF16 gradient dPb; WT (27*cin*cout
 // BF16 scratch) receives the transposed, mirrored weights.
 void launch_conv3_dgrad(const uint16_t* dPb, const float* W32, uint16_t* WT, float* dX, int n, int cin, int cout,

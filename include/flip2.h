@@ -327,6 +327,15 @@ int flip2_unet_eval(flip2_unet* net, int first, int last, double* mean_loss);
  * simulation (NULL detaches); fallback_iterations > 0 reruns a solve with
  * Jacobi-PCG when flexible CG needs more iterations than that. */
 int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iterations);
+/* Number of earlier search directions (1..16, default 8) that flexible CG
+ * orthogonalises each new direction against when the learned
+ * preconditioner is used (1 is FCG(1)).  New in this library; the reference
+ * solver is red-black SOR (engine/algorithms/voxelSolveFunctions.cu:275-412). */
+int flip2_fluid_set_flex_directions(flip2_fluid* f, uint32_t directions);
+/* enable != 0 (default): the learned preconditioner's kernels are captured
+ * once per solve into a HIP graph and replayed; 0 launches them one by one.
+ * Same results either way.  No counterpart in the reference. */
+int flip2_fluid_set_learned_graph(flip2_fluid* f, int enable);
 
 #ifdef __cplusplus
 }

@@ -7,13 +7,18 @@ training loss can be checked:
 
   input  x[...,0] = bf16(b / s), x[...,1] = fluid mask, channels 2..15 zero,
          s = RMS of b over fluid cells (FP32)
-  a0 = relu(conv(x, 16->16))      b0 = relu(conv(a0, 16->16))        level 0
-  a1 = relu(conv(pool(b0), 16->32))  b1 = relu(conv(a1, 32->32))     level 1
-  a2 = relu(conv(pool(b1), 32->64))  b2 = relu(conv(a2, 64->64))     level 2
-  c1 = relu(conv([up(b2), b1], 96->32))  d1 = relu(conv(c1, 32->32))
-  c0 = relu(conv([up(d1), b0], 48->16))  d0 = relu(conv(c0, 16->16))
+  masks  M0 = fluid cells, M1 = any(M0) over 2x2x2 children, M2 likewise
+  a0 = M0 relu(conv(x, 16->16))      b0 = M0 relu(conv(a0, 16->16))        level 0
+  a1 = M1 relu(conv(pool(b0), 16->32))  b1 = M1 relu(conv(a1, 32->32))     level 1
+  a2 = M2 relu(conv(pool(b1), 32->64))  b2 = M2 relu(conv(a2, 64->64))     level 2
+  c1 = M1 relu(conv([up(b2), b1], 96->32))  d1 = M1 relu(conv(c1, 32->32))
+  c0 = M0 relu(conv([up(d1), b0], 48->16))  d0 = M0 relu(conv(c0, 16->16))
   z  = s * mask * (d0 . w + beta)
   loss = ||A z - b||^2 / ||b||^2 over fluid cells
+
+Every activation is zero outside its level's mask, so the network's output
+does not depend on anything outside the fluid and blocks of the grid with no
+fluid need not be computed at all (the HIP apply skips them).
 
 conv is a 3x3x3 cross-correlation with zero padding, weights W[tap][cout][cin]
 with tap = (dz*3 + dy)*3 + dx for offsets (dz-1, dy-1, dx-1); pool is the mean
@@ -105,6 +110,17 @@ def up(x):
     return np.repeat(np.repeat(np.repeat(x, 2, 0), 2, 1), 2, 2)
 
 
+def pool_any(m):
+    """Coarse mask: true where any of the 2x2x2 children is."""
+    n = m.shape[0] // 2
+    return np.asarray(m, bool).reshape(n, 2, n, 2, n, 2).any(axis=(1, 3, 5))
+
+
+def level_masks(mask):
+    m1 = pool_any(mask)
+    return [np.asarray(mask, bool), m1, pool_any(m1)]
+
+
 def up_backward(g):
     n = g.shape[0] // 2
     return g.reshape(n, 2, n, 2, n, 2, g.shape[-1]).sum(axis=(1, 3, 5))
@@ -163,6 +179,7 @@ class Net:
 
     def forward(self, b, coef):
         x0, mask, s = make_input(b, coef)
+        masks = level_masks(mask)
         t = {"x0": x0}
         order = [("x0", "a0", 0), ("a0", "b0", 1), ("p0", "a1", 2), ("a1", "b1", 3), ("p1", "a2", 4),
                  ("a2", "b2", 5), ("u1", "c1", 6), ("c1", "d1", 7), ("u0", "c0", 8), ("c0", "d0", 9)]
@@ -176,7 +193,8 @@ class Net:
             if src == "u0":
                 t["u0"] = np.concatenate([up(t["d1"]), t["b0"]], axis=-1)
             W, bias = self._W(i)
-            t[dst] = self._act(np.maximum(conv3(t[src], W, bias), 0.0))
+            lev = self.layout[i][4]
+            t[dst] = self._act(np.maximum(conv3(t[src], W, bias), 0.0) * masks[lev][..., None])
         hw = self.p[self.hw:self.hw + 16]
         hb = self.p[self.hb]
         z = np.where(mask, s * (t["d0"] @ hw + hb), 0.0)

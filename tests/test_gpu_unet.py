@@ -98,6 +98,21 @@ def test_smoothed_preconditioner_matches_oracle(flip2, n):
     check_gradients(g, go, GRAD_TOL)
 
 
+@pytest.mark.parametrize("n", [16, 24])
+def test_fused_decoder_matches_stored_activations(flip2, n):
+    """The preconditioner apply reads the decoder's concatenated inputs in
+    place and runs the output head inside the last convolution; the training
+    forward stores every activation.  Both compute the same BF16 values in
+    the same order (n=16 also takes the XCD tile remapping, n=24 does not)."""
+    net = flip2.UNet(n, seed=11)
+    net.set_smoothing(0.0)
+    biased_params(net, 70 + n)
+    b, coef = synthetic_system(n, 80 + n)
+    l_fused, _ = net.loss_grad(b, coef, grad=False)
+    l_stored, _ = net.loss_grad(b, coef, grad=True)
+    assert abs(l_fused - l_stored) <= 1e-12 * l_stored, (l_fused, l_stored)
+
+
 def test_loss_grad_is_repeatable(flip2):
     n = 16
     net = flip2.UNet(n, seed=3)
@@ -144,6 +159,15 @@ def test_training_and_learned_fcg(flip2):
     assert jac["converged"] and lrn["converged"], (jac, lrn)
     assert lrn["rel_residual"] <= 1e-6
     assert lrn["iterations"] < jac["iterations"], (lrn, jac)
+    # the applies replayed from the HIP graph compute what eager launches do
+    # (deterministic kernels: same iterations, same residual bit for bit);
+    # graph, eager, graph again, with the eager solves between the replays
+    sim.set_learned_graph(False)
+    eager = sim.solve_with("learned")
+    sim.set_learned_graph(True)
+    again = sim.solve_with("learned")
+    for r in (eager, again):
+        assert r["iterations"] == lrn["iterations"] and r["rel_residual"] == lrn["rel_residual"], (r, lrn)
     # the fallback: a cap below what FCG needs hands the solve to Jacobi-PCG
     sim.set_unet(net, fallback_iterations=2)
     fb = sim.solve_with("learned")
