@@ -328,52 +328,73 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
     const uint32_t Q = (total + 3u) >> 2;
     const uint32_t R = (Q + 63u) >> 6;
     const uint32_t qBeg = wave * Q, qEnd = min(total, qBeg + Q);
-    for (uint32_t i = 0; i < R; ++i) {
-        const uint32_t sIdx = qBeg + lane * R + i;
-        if (sIdx >= qEnd) continue;
-        int a = 0, b = kShellRuns + 1;  // last run with s_off <= sIdx
-        while (b - a > 1) {
-            const int m = (a + b) >> 1;
-            if (s_off[m] <= sIdx) a = m;
+    // A lane's particles are consecutive (sIdx = lBeg + i): one binary search
+    // for the first, then the run index only moves forward.  Records are
+    // loaded kPre at a time before any is scattered, so their memory
+    // latencies overlap.
+    constexpr uint32_t kPre = 4;
+    const uint32_t lBeg = qBeg + lane * R;
+    const uint32_t lCnt = lBeg < qEnd ? min(R, qEnd - lBeg) : 0u;
+    int run = 0;
+    if (lCnt) {
+        int b = kShellRuns + 1;  // last run with s_off <= lBeg
+        while (b - run > 1) {
+            const int m = (run + b) >> 1;
+            if (s_off[m] <= lBeg) run = m;
             else b = m;
         }
-        const uint32_t p = s_first[a] + (sIdx - s_off[a]);
-        const float4 r0 = rec[2 * p];
-        const float4 r1 = rec[2 * p + 1];
-        // position in cells relative to the brick's low corner
-        const float q[3] = {(float)(((double)r0.x - org[0]) * inv), (float)(((double)r0.y - org[1]) * inv),
-                            (float)(((double)r0.z - org[2]) * inv)};
-        const float vel[3] = {r0.w, r1.x, r1.y};
+    }
+    for (uint32_t i0 = 0; i0 < lCnt; i0 += kPre) {
+        float4 pr0[kPre], pr1[kPre];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            int bs[3];
-            float t1[3];
+        for (uint32_t k = 0; k < kPre; ++k) {
+            if (i0 + k >= lCnt) break;
+            const uint32_t sIdx = lBeg + i0 + k;
+            while (s_off[run + 1] <= sIdx) ++run;  // s_off[kShellRuns + 1] = total > sIdx
+            const uint32_t p = s_first[run] + (sIdx - s_off[run]);
+            pr0[k] = rec[2 * p];
+            pr1[k] = rec[2 * p + 1];
+        }
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const float v = (d == c) ? q[d] : q[d] - 0.5f;
-                const float f = floorf(v);
-                bs[d] = (int)f;
-                t1[d] = v - f;
-            }
-            const float mv = vel[c] * scaleM;
+        for (uint32_t k = 0; k < kPre; ++k) {
+            if (i0 + k >= lCnt) break;
+            const float4 r0 = pr0[k];
+            const float4 r1 = pr1[k];
+            // position in cells relative to the brick's low corner
+            const float q[3] = {(float)(((double)r0.x - org[0]) * inv), (float)(((double)r0.y - org[1]) * inv),
+                                (float)(((double)r0.z - org[2]) * inv)};
+            const float vel[3] = {r0.w, r1.x, r1.y};
 #pragma unroll
-            for (int ax = 0; ax < 2; ++ax) {
-                const int li = bs[0] + ax;
-                if ((unsigned)li >= 8u) continue;
-                const float wa = ax ? t1[0] : 1.f - t1[0];
+            for (int c = 0; c < 3; ++c) {
+                int bs[3];
+                float t1[3];
 #pragma unroll
-                for (int by = 0; by < 2; ++by) {
-                    const int lj = bs[1] + by;
-                    if ((unsigned)lj >= 8u) continue;
-                    const float wab = wa * (by ? t1[1] : 1.f - t1[1]);
+                for (int d = 0; d < 3; ++d) {
+                    const float v = (d == c) ? q[d] : q[d] - 0.5f;
+                    const float f = floorf(v);
+                    bs[d] = (int)f;
+                    t1[d] = v - f;
+                }
+                const float mv = vel[c] * scaleM;
 #pragma unroll
-                    for (int bz = 0; bz < 2; ++bz) {
-                        const int lk = bs[2] + bz;
-                        if ((unsigned)lk >= 8u) continue;
-                        const float wt = wab * (bz ? t1[2] : 1.f - t1[2]);
-                        const int node = lk * 64 + lj * 8 + li;
-                        atomicAdd(&s_w[c][node], (unsigned long long)__float2ull_rn(wt * scaleW));
-                        atomicAdd(&s_m[c][node], (unsigned long long)__float2ll_rn(wt * mv));
+                for (int ax = 0; ax < 2; ++ax) {
+                    const int li = bs[0] + ax;
+                    if ((unsigned)li >= 8u) continue;
+                    const float wa = ax ? t1[0] : 1.f - t1[0];
+#pragma unroll
+                    for (int by = 0; by < 2; ++by) {
+                        const int lj = bs[1] + by;
+                        if ((unsigned)lj >= 8u) continue;
+                        const float wab = wa * (by ? t1[1] : 1.f - t1[1]);
+#pragma unroll
+                        for (int bz = 0; bz < 2; ++bz) {
+                            const int lk = bs[2] + bz;
+                            if ((unsigned)lk >= 8u) continue;
+                            const float wt = wab * (bz ? t1[2] : 1.f - t1[2]);
+                            const int node = lk * 64 + lj * 8 + li;
+                            atomicAdd(&s_w[c][node], (unsigned long long)__float2ull_rn(wt * scaleW));
+                            atomicAdd(&s_m[c][node], (unsigned long long)__float2ll_rn(wt * mv));
+                        }
                     }
                 }
             }
