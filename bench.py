@@ -37,7 +37,7 @@ PEAK_HBM_GBPS = 8000.0
 # HBM bytes per dispatch of the default workload (2^24 particles, 256^3),
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs scaled by the same
 # counters on a plain copy of known size (profiles/transfer_16M_256_pmc.md).
-TRAFFIC_BYTES = {"bin": 604.0e6, "sort_pass": 272.6e6, "gather": 2747.9e6, "scan": 137.1e6, "p2g": 962.1e6}
+TRAFFIC_BYTES = {"bin": 604.0e6, "sort_pass": 284.6e6, "gather": 2748.0e6, "scan": 135.9e6, "p2g": 962.4e6}
 
 
 def algorithmic_bytes(name, n, cells, occupied, passes):

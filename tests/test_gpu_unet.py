@@ -113,6 +113,31 @@ def test_fused_decoder_matches_stored_activations(flip2, n):
     assert abs(l_fused - l_stored) <= 1e-12 * l_stored, (l_fused, l_stored)
 
 
+def test_sparse_apply_after_the_fluid_moved(flip2):
+    """The apply computes only tiles with fluid and clears tiles that the
+    previous apply computed: after a system whose fluid fills other tiles,
+    the output equals that of a fresh network (same parameters) on the new
+    system, bit for bit, and the masked (oracle) network within bf16."""
+    n = 32
+    net = flip2.UNet(n, seed=12)
+    net.set_smoothing(0.0)
+    p = biased_params(net, 90)
+    b1, coef1 = synthetic_system(n, 91, fill=0.9)   # fluid nearly everywhere
+    b2, coef2 = synthetic_system(n, 92, fill=0.25)  # a shallow pool
+    net.apply_dense(b1, coef1)
+    z2 = net.apply_dense(b2, coef2)
+    fresh = flip2.UNet(n, seed=13)
+    fresh.set_smoothing(0.0)
+    fresh.set_params(p)
+    z2f = fresh.apply_dense(b2, coef2)
+    assert np.array_equal(z2, z2f)
+    zo = uo.Net(n, p, bf16=True).forward(b2, coef2)
+    assert rel(z2.reshape(n, n, n), zo) <= 1e-2
+    # and back: the tiles of the first system are computed again
+    z1 = net.apply_dense(b1, coef1)
+    assert np.array_equal(z1, fresh.apply_dense(b1, coef1))
+
+
 def test_loss_grad_is_repeatable(flip2):
     n = 16
     net = flip2.UNet(n, seed=3)
