@@ -297,18 +297,16 @@ __device__ inline double apply_A64(uint32_t slot, uint32_t l, uint16_t cf, const
 }  // namespace
 
 // r32 = (float) r64; e = 0; rr into scal[out].
-// out[q] = sum over workgroups of partials[q * blocks + b], fixed order.
+// out[q] = sum over workgroups of partials[q * blocks + b], fixed order:
+// workgroup q sums quantity q, thread t the partials t, t + 256, ...
 __global__ void __launch_bounds__(kWG)
-reduce_partials_kernel(const double* __restrict__ partials, int count, uint32_t blocks, double* __restrict__ out) {
+reduce_partials_kernel(const double* __restrict__ partials, uint32_t blocks, double* __restrict__ out) {
     __shared__ double s_tmp[16];
-    const uint32_t per = (blocks + blockDim.x - 1) / blockDim.x;
-    const uint32_t beg = threadIdx.x * per;
-    for (int q = 0; q < count; ++q) {
-        double acc = 0.0;
-        for (uint32_t b = beg; b < beg + per && b < blocks; ++b) acc += partials[(size_t)q * blocks + b];
-        const double t = block_sum(acc, s_tmp);
-        if (threadIdx.x == 0) out[q] = t;
-    }
+    const double* p = partials + (size_t)blockIdx.x * blocks;
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) acc += p[b];
+    const double t = block_sum(acc, s_tmp);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
 __global__ void __launch_bounds__(kWG)
@@ -453,7 +451,6 @@ direction_kernel(const double* __restrict__ rzNew, const double* __restrict__ rz
 
 __global__ void __launch_bounds__(kWG)
 flex_dots_kernel(const float* __restrict__ z, FlexDirections d, double* __restrict__ partials) {
-    __shared__ double s_tmp[16];
     const uint32_t base = blockIdx.x * 512u;
     double acc[kMaxFlexDirections];
 #pragma unroll
@@ -464,9 +461,24 @@ flex_dots_kernel(const float* __restrict__ z, FlexDirections d, double* __restri
         for (int i = 0; i < kMaxFlexDirections; ++i)
             if (i < d.count) acc[i] += zv * (double)d.q[i][base + l];
     }
-    double vals[kMaxFlexDirections];
-    for (int i = 0; i < d.count; ++i) vals[i] = block_sum(acc[i], s_tmp);
-    store_partials(vals, d.count, partials);
+    // all d.count sums with one LDS exchange: wave sums by shuffles, then
+    // thread i adds the four waves' sums of quantity i
+    __shared__ double s_many[kWG / 64][kMaxFlexDirections];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kMaxFlexDirections; ++i) {
+        if (i < d.count) {
+            double v = acc[i];
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) s_many[wave][i] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < d.count) {
+        double t = 0.0;
+        for (int w = 0; w < kWG / 64; ++w) t += s_many[w][threadIdx.x];
+        partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
 }
 
 __global__ void __launch_bounds__(kWG)
@@ -941,7 +953,7 @@ namespace {
 inline dim3 g1(uint32_t n, uint32_t bs) { return dim3(ceil_div(n ? n : 1, bs)); }
 // out[0..count) = sums of the workgroup partials of the kernel just launched
 void reduce_to(const Reduce& red, int count, uint32_t blocks, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kWG), 0, s, red.partials, count, blocks, out);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(count), dim3(kWG), 0, s, red.partials, blocks, out);
     FLIP2_LAUNCH_CHECK();
 }
 }  // namespace

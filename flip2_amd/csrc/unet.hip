@@ -457,6 +457,14 @@ UNet::UNet(int n, hipStream_t stream) : n_(n), n1_(n / 2), n2_(n / 4), stream_(s
         const uint32_t tiles = nn::conv_tiles(levelN[l]);
         for (auto* f : {&tileFlags_[0][l], &tileFlags_[1][l], &tileOnes_[l]}) f->resizeAsync(tiles, stream_);
         FLIP2_HIP(hipMemsetAsync(tileOnes_[l].devPtr(), 1, tiles, stream_));
+        listCompute_[l].resizeAsync(tiles, stream_);
+        listClear_[l].resizeAsync(tiles, stream_);
+        listCounts_[l].resizeAsync(2, stream_);
+        nn::conv_tile_dims(levelN[l], tl_[l].tilesX, tl_[l].tilesY);
+        tl_[l].compute = listCompute_[l].devPtr();
+        tl_[l].clear = listClear_[l].devPtr();
+        tl_[l].numCompute = listCounts_[l].devPtr();
+        tl_[l].numClear = listCounts_[l].devPtr() + 1;
     }
     scale_.resizeAsync(4, stream_);
     out_.resizeAsync((uint32_t)N0, stream_);
@@ -519,6 +527,8 @@ void UNet::setInputMasks(hipStream_t s, bool sparse) {
         nn::launch_tile_flags(lm[l], levelN[l], tileFlags_[flagSet_][l].devPtr(), s);
         const uint8_t* prev = tilesDirty_ ? tileOnes_[l].devPtr() : tileFlags_[flagSet_ ^ 1][l].devPtr();
         cm_[l] = nn::ConvMask{lm[l], tileFlags_[flagSet_][l].devPtr(), prev};
+        nn::launch_tile_lists(tileFlags_[flagSet_][l].devPtr(), prev, nn::conv_tiles(levelN[l]),
+                              listCompute_[l].devPtr(), listClear_[l].devPtr(), listCounts_[l].devPtr(), s);
     }
     tilesDirty_ = false;
 }
@@ -530,6 +540,40 @@ void UNet::forward(hipStream_t s, bool keepActivations) {
     // the first layer reads the two non-zero input channels only
     nn::launch_conv3_in2(in0c_.devPtr(), paramsBf_.devPtr() + layers_[0].w, params_.devPtr() + layers_[0].b,
                          a0_.devPtr(), n_, true, cm[0], s);
+    if (cm[0].tiles) {
+        // The apply.  Level 0 (many tiles): one workgroup per tile of the
+        // grid, tiles without fluid skipped by their flag.  Levels 1 and 2
+        // (few tiles): a fixed grid of workgroups over the list of tiles with
+        // fluid, 2 or 4 workgroups per tile (output channels split), the
+        // next input stage prefetched during the current one's MFMAs.  Both
+        // measured faster than the other choice on their levels.
+        const int levelN[3] = {n_, n1_, n2_};
+        const int split[3] = {1, 2, 4};
+        auto lconv = [&](int layer, const uint16_t* X, const uint16_t* Xc, int ca, uint16_t* Y, const nn::HeadArgs& h) {
+            const UNetLayer& L = layers_[layer];
+            nn::launch_conv3_list(X, Xc, paramsBf_.devPtr() + L.w, params_.devPtr() + L.b, Y, levelN[L.level], L.cin,
+                                  ca, L.cout, split[L.level], true, h, cm[L.level].voxel, tl_[L.level], s);
+        };
+        const nn::HeadArgs none{};
+        conv(1, a0_.devPtr(), b0_.devPtr(), cm[0], s);
+        nn::launch_avgpool_list(b0_.devPtr(), p0_.devPtr(), n_, 16, tl_[1], s);
+        lconv(2, p0_.devPtr(), nullptr, 0, a1_.devPtr(), none);
+        lconv(3, a1_.devPtr(), nullptr, 0, b1_.devPtr(), none);
+        nn::launch_avgpool_list(b1_.devPtr(), p1_.devPtr(), n1_, 32, tl_[2], s);
+        lconv(4, p1_.devPtr(), nullptr, 0, a2_.devPtr(), none);
+        lconv(5, a2_.devPtr(), nullptr, 0, b2_.devPtr(), none);
+        lconv(6, b1_.devPtr(), b2_.devPtr(), 64, c1_.devPtr(), none);
+        lconv(7, c1_.devPtr(), nullptr, 0, d1_.devPtr(), none);
+        const UNetLayer& L8 = layers_[8];
+        const UNetLayer& L9 = layers_[9];
+        nn::launch_conv3_fwd_upcat(b0_.devPtr(), d1_.devPtr(), paramsBf_.devPtr() + L8.w, params_.devPtr() + L8.b,
+                                   c0_.devPtr(), n_, 32, 16, 16, true, cm[0], s);
+        const nn::HeadArgs head{params_.devPtr() + headW_, params_.devPtr() + headB_, mask_.devPtr(), scale_.devPtr(),
+                                out_.devPtr()};
+        nn::launch_conv3_fwd_head(c0_.devPtr(), paramsBf_.devPtr() + L9.w, params_.devPtr() + L9.b, nullptr, n_, 16,
+                                  true, head, cm[0], s);
+        return;
+    }
     conv(1, a0_.devPtr(), b0_.devPtr(), cm[0], s);
     nn::launch_avgpool(b0_.devPtr(), p0_.devPtr(), n_, 16, s);
     conv(2, p0_.devPtr(), a1_.devPtr(), cm[1], s);

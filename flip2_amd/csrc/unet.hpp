@@ -89,6 +89,8 @@ private:
     int flagSet_ = 0;
     bool tilesDirty_ = true;  // a dense forward (or allocation) left values everywhere
     nn::ConvMask cm_[3] = {};
+    DeviceVec<uint32_t> listCompute_[3], listClear_[3], listCounts_[3];
+    nn::TileList tl_[3] = {};
     DeviceVec<float> scale_, out_;
     // gradients (FP32)
     DeviceVec<float> gd0_, gc0_, gu0_, gd1_, gc1_, gu1_, gb2_, ga2_, gp1_, gb1_, ga1_, gp0_, gb0_, ga0_;

@@ -119,76 +119,59 @@ __device__ inline int halo_index(int wave, int j, int col, int tap) {
     return ((wave + dz) * kHY + (j + dy)) * kHX + (col + dx);
 }
 
-template <int CIN, int COUT, int MODE = 0, int CA = 0, int SPLIT = 1>
-__global__ void __launch_bounds__(256)
-conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc, const uint16_t* __restrict__ Wt,
-                 const float* __restrict__ bias, uint16_t* __restrict__ Y, float* __restrict__ Yf, int n, int relu,
-                 HeadArgs head, ConvMask cm) {
-    constexpr int MB = COUT / 16 / SPLIT;  // 16-channel output blocks of this workgroup
-    constexpr int CB = CIN - CA;           // channels read from X
-    static_assert(CA % 16 == 0 && CB % 16 == 0, "channel groups of 16");
-    static_assert(MB >= 1 && MB * 16 * SPLIT == COUT, "output channels split into whole 16-channel blocks");
-    __shared__ __align__(16) uint4 sIn[2][kHalo];  // [8-channel chunk][halo voxel]
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int col = lane & 15, kq = lane >> 4;
+// Global address of halo element t (voxel t >> 1, 8-channel half t & 1) of
+// input stage c0 for the tile at (bx, by, bz); null outside the grid.
+template <int CIN, int CA>
+__device__ inline const uint4* stage_src(const uint16_t* X, const uint16_t* Xc, int n, int bx, int by, int bz,
+                                         int c0, int t) {
+    constexpr int CB = CIN - CA;
+    const int hv = t >> 1, half = t & 1;
+    const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+    const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
+    if (x < 0 || y < 0 || z < 0 || x >= n || y >= n || z >= n) return nullptr;
+    if (CA > 0 && c0 < CA) {
+        const int nc = n >> 1;
+        return reinterpret_cast<const uint4*>(Xc + (((long long)(z >> 1) * nc + (y >> 1)) * nc + (x >> 1)) * CA + c0 +
+                                              8 * half);
+    }
+    return reinterpret_cast<const uint4*>(X + (((long long)z * n + y) * n + x) * CB + (c0 - CA) + 8 * half);
+}
+
+// The 27 taps of one 16-channel input stage held in LDS (two 8-channel
+// planes) into the accumulators of output blocks m0 .. m0 + MB - 1.
+template <int CIN, int COUT, int MB>
+__device__ inline void conv_stage_mfma(const uint4 (*sIn)[kHalo], const uint16_t* __restrict__ Wt, int c0, int m0,
+                                       int wave, int col, int kq, v4f (&acc)[MB][4]) {
     const int chunk = kq & 1, tapOdd = kq >> 1;
-    int bx, by, bz, mg;
-    conv_tile<SPLIT>(bx, by, bz, mg);
-    if (skip_tile<COUT, SPLIT>(cm, bx, by, bz, mg, n, MODE == 0 ? Y : nullptr, MODE == 0 ? head.out : nullptr))
-        return;
-    const int m0 = mg * MB;  // first output block of this workgroup
-
-    v4f acc[MB][4];
-#pragma unroll
-    for (int m = 0; m < MB; ++m)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
-
-#pragma unroll 1
-    for (int c0 = 0; c0 < CIN; c0 += 16) {
-        __syncthreads();
-        // stage 648 voxels x 16 channels as two 8-channel planes
-        for (int t = threadIdx.x; t < kHalo * 2; t += 256) {
-            const int hv = t >> 1, half = t & 1;
-            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-            const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n) {
-                if (CA > 0 && c0 < CA) {
-                    const int nc = n >> 1;
-                    v = *reinterpret_cast<const uint4*>(
-                        Xc + (((long long)(z >> 1) * nc + (y >> 1)) * nc + (x >> 1)) * CA + c0 + 8 * half);
-                } else {
-                    v = *reinterpret_cast<const uint4*>(X + (((long long)z * n + y) * n + x) * CB + (c0 - CA) + 8 * half);
-                }
-            }
-            sIn[half][hv] = v;
-        }
-        __syncthreads();
 #pragma unroll 2
-        for (int p = 0; p < kTapPairs; ++p) {
-            const int tap = 2 * p + tapOdd;
-            const bool live = tap < 27;
-            v8bf b[4];
+    for (int p = 0; p < kTapPairs; ++p) {
+        const int tap = 2 * p + tapOdd;
+        const bool live = tap < 27;
+        v8bf b[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint4 raw = live ? sIn[chunk][halo_index(wave, j, col, tap)] : make_uint4(0u, 0u, 0u, 0u);
-                b[j] = __builtin_bit_cast(v8bf, raw);
-            }
+        for (int j = 0; j < 4; ++j) {
+            const uint4 raw = live ? sIn[chunk][halo_index(wave, j, col, tap)] : make_uint4(0u, 0u, 0u, 0u);
+            b[j] = __builtin_bit_cast(v8bf, raw);
+        }
 #pragma unroll
-            for (int m = 0; m < MB; ++m) {
-                const uint4 raw = live ? *reinterpret_cast<const uint4*>(
-                                             Wt + ((size_t)(tap * COUT + 16 * (m0 + m) + col) * CIN) + c0 + 8 * chunk)
-                                       : make_uint4(0u, 0u, 0u, 0u);
-                const v8bf a = __builtin_bit_cast(v8bf, raw);
+        for (int m = 0; m < MB; ++m) {
+            const uint4 raw = live ? *reinterpret_cast<const uint4*>(
+                                         Wt + ((size_t)(tap * COUT + 16 * (m0 + m) + col) * CIN) + c0 + 8 * chunk)
+                                   : make_uint4(0u, 0u, 0u, 0u);
+            const v8bf a = __builtin_bit_cast(v8bf, raw);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
-            }
+            for (int j = 0; j < 4; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
         }
     }
-    // lane (col, kq) holds output channels 16 m + 4 kq .. +3 of voxel
-    // (bx + col, by + j, bz + wave)
+}
+
+// Epilogue: lane (col, kq) holds output channels 16 (m0 + m) + 4 kq .. +3 of
+// voxel (bx + col, by + j, bz + wave).
+template <int COUT, int MODE, int MB>
+__device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const float* __restrict__ bias, uint16_t* __restrict__ Y,
+                                     float* __restrict__ Yf, int n, int relu, const HeadArgs& head,
+                                     const uint8_t* __restrict__ vmask, int bx, int by, int bz, int m0, int wave,
+                                     int col, int kq) {
     const int z = bz + wave, x = bx + col;
     if (MODE == 0 && COUT == 16 && head.out) {
         // fused head: gather the 16 channels of each voxel in its kq == 0 lane
@@ -197,7 +180,7 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
             const int y = by + j;
             const bool inside = x < n && y < n && z < n;
             const long long v = ((long long)z * n + y) * n + x;
-            const bool live = inside && (!cm.voxel || cm.voxel[v]);
+            const bool live = inside && (!vmask || vmask[v]);
             float yv[4];
             uint16_t o[4];
 #pragma unroll
@@ -234,7 +217,7 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
         const int y = by + j;
         if (x >= n || y >= n || z >= n) continue;
         const long long v = ((long long)z * n + y) * n + x;
-        const bool live = MODE == 1 || !cm.voxel || cm.voxel[v];
+        const bool live = MODE == 1 || !vmask || vmask[v];
 #pragma unroll
         for (int m = 0; m < MB; ++m) {
             const int co = 16 * (m0 + m) + 4 * kq;
@@ -255,6 +238,171 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
             packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
             *reinterpret_cast<uint2*>(Y + v * COUT + co) = packed;
         }
+    }
+}
+
+template <int CIN, int COUT, int MODE = 0, int CA = 0, int SPLIT = 1>
+__global__ void __launch_bounds__(256)
+conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc, const uint16_t* __restrict__ Wt,
+                 const float* __restrict__ bias, uint16_t* __restrict__ Y, float* __restrict__ Yf, int n, int relu,
+                 HeadArgs head, ConvMask cm) {
+    constexpr int MB = COUT / 16 / SPLIT;  // 16-channel output blocks of this workgroup
+    constexpr int CB = CIN - CA;           // channels read from X
+    static_assert(CA % 16 == 0 && CB % 16 == 0, "channel groups of 16");
+    static_assert(MB >= 1 && MB * 16 * SPLIT == COUT, "output channels split into whole 16-channel blocks");
+    __shared__ __align__(16) uint4 sIn[2][kHalo];  // [8-channel chunk][halo voxel]
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    int bx, by, bz, mg;
+    conv_tile<SPLIT>(bx, by, bz, mg);
+    if (skip_tile<COUT, SPLIT>(cm, bx, by, bz, mg, n, MODE == 0 ? Y : nullptr, MODE == 0 ? head.out : nullptr))
+        return;
+    const int m0 = mg * MB;  // first output block of this workgroup
+
+    v4f acc[MB][4];
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 1
+    for (int c0 = 0; c0 < CIN; c0 += 16) {
+        __syncthreads();
+        // stage 648 voxels x 16 channels as two 8-channel planes
+        for (int t = threadIdx.x; t < kHalo * 2; t += 256) {
+            const uint4* src = stage_src<CIN, CA>(X, Xc, n, bx, by, bz, c0, t);
+            sIn[t & 1][t >> 1] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();
+        conv_stage_mfma<CIN, COUT, MB>(sIn, Wt, c0, m0, wave, col, kq, acc);
+    }
+    conv_epilogue<COUT, MODE, MB>(acc, bias, Y, Yf, n, relu, head, cm.voxel, bx, by, bz, m0, wave, col, kq);
+}
+
+// The same convolution over a list of tiles (the tiles of a level that hold
+// masked-in voxels): a fixed grid of workgroups loops over the list, and the
+// next input stage's global loads are issued into registers before the
+// current stage's MFMAs, then stored into the other of two LDS buffers, so
+// memory latency overlaps compute instead of waiting at every tile.  Each
+// XCD takes one contiguous part of the list (neighbouring tiles, shared halo
+// rows in its L2).  Tiles in `clear` (computed by an earlier forward, not by
+// this one) are zeroed first.
+constexpr int kStageLoads = (kHalo * 2 + 255) / 256;  // 16-byte loads per thread per stage (6)
+
+template <int CIN, int COUT, int CA = 0, int SPLIT = 1>
+__global__ void __launch_bounds__(256)
+conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc, const uint16_t* __restrict__ Wt,
+                  const float* __restrict__ bias, uint16_t* __restrict__ Y, int n, int relu, HeadArgs head,
+                  const uint8_t* __restrict__ vmask, TileList tl) {
+    constexpr int MB = COUT / 16 / SPLIT;
+    constexpr int STAGES = CIN / 16;
+    static_assert(MB >= 1 && MB * 16 * SPLIT == COUT, "output channels split into whole 16-channel blocks");
+    __shared__ __align__(16) uint4 sIn[2][2][kHalo];  // [buffer][8-channel chunk][halo voxel]
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const uint32_t tilesX = tl.tilesX, tilesXY = tl.tilesX * tl.tilesY;
+    // XCD x = blockIdx.x % 8 owns units [x U / 8, (x + 1) U / 8) of the U
+    // (tile, channel group) units; its workgroups stride through them.
+    const uint32_t xcd = blockIdx.x & 7u, perXcd = gridDim.x >> 3, w = blockIdx.x >> 3;
+
+    // clear the listed tiles (unit = tile x channel group)
+    {
+        const uint32_t U = *tl.numClear * SPLIT;
+        const uint32_t beg = (uint32_t)(((uint64_t)U * xcd) >> 3), end = (uint32_t)(((uint64_t)U * (xcd + 1)) >> 3);
+        for (uint32_t u = beg + w; u < end; u += perXcd) {
+            const uint32_t t = tl.clear[u / SPLIT];
+            const int mg = (int)(u % SPLIT);
+            const int bx = (int)(t % tilesX) * kTX, by = (int)((t % tilesXY) / tilesX) * kTY,
+                      bz = (int)(t / tilesXY) * kTZ;
+            const int x = bx + (threadIdx.x & 15), y = by + ((threadIdx.x >> 4) & 3), z = bz + (threadIdx.x >> 6);
+            if (x < n && y < n && z < n) {
+                const long long v = ((long long)z * n + y) * n + x;
+                if (Y) {
+                    constexpr int CW = COUT / SPLIT;
+                    uint4* dst = reinterpret_cast<uint4*>(Y + v * COUT + mg * CW);
+#pragma unroll
+                    for (int q = 0; q < CW / 8; ++q) dst[q] = make_uint4(0u, 0u, 0u, 0u);
+                }
+                if (head.out && mg == 0) head.out[v] = 0.f;
+            }
+        }
+    }
+
+    const uint32_t U = *tl.numCompute * SPLIT;
+    const uint32_t beg = (uint32_t)(((uint64_t)U * xcd) >> 3), end = (uint32_t)(((uint64_t)U * (xcd + 1)) >> 3);
+    uint32_t u = beg + w;
+    if (u >= end) return;
+    // step = (unit, stage); the loads of step s + 1 are in flight during step s
+    auto unit_origin = [&](uint32_t unit, int& bx, int& by, int& bz) {
+        const uint32_t t = tl.compute[unit / SPLIT];
+        bx = (int)(t % tilesX) * kTX;
+        by = (int)((t % tilesXY) / tilesX) * kTY;
+        bz = (int)(t / tilesXY) * kTZ;
+    };
+    int bx, by, bz;
+    unit_origin(u, bx, by, bz);
+    uint4 pre[kStageLoads];
+#pragma unroll
+    for (int k = 0; k < kStageLoads; ++k) {
+        const int t = (int)threadIdx.x + 256 * k;
+        const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, bx, by, bz, 0, t) : nullptr;
+        pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < kStageLoads; ++k) {
+        const int t = (int)threadIdx.x + 256 * k;
+        if (t < kHalo * 2) sIn[0][t & 1][t >> 1] = pre[k];
+    }
+    __syncthreads();
+    int buf = 0, stage = 0;
+    v4f acc[MB][4];
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (;;) {
+        // next step
+        uint32_t nu = u;
+        int ns = stage + 1;
+        if (ns == STAGES) {
+            ns = 0;
+            nu = u + perXcd;
+        }
+        const bool more = nu < end;
+        int nbx = bx, nby = by, nbz = bz;
+        if (more) {
+            if (nu != u) unit_origin(nu, nbx, nby, nbz);
+#pragma unroll
+            for (int k = 0; k < kStageLoads; ++k) {
+                const int t = (int)threadIdx.x + 256 * k;
+                const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, nbx, nby, nbz, 16 * ns, t) : nullptr;
+                pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        const int m0 = (int)(u % SPLIT) * MB;
+        conv_stage_mfma<CIN, COUT, MB>(sIn[buf], Wt, 16 * stage, m0, wave, col, kq, acc);
+        if (stage == STAGES - 1) {
+            conv_epilogue<COUT, 0, MB>(acc, bias, Y, nullptr, n, relu, head, vmask, bx, by, bz, m0, wave, col, kq);
+#pragma unroll
+            for (int m = 0; m < MB; ++m)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+        if (!more) break;
+#pragma unroll
+        for (int k = 0; k < kStageLoads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            if (t < kHalo * 2) sIn[buf ^ 1][t & 1][t >> 1] = pre[k];
+        }
+        __syncthreads();
+        buf ^= 1;
+        u = nu;
+        stage = ns;
+        bx = nbx;
+        by = nby;
+        bz = nbz;
     }
 }
 
@@ -359,6 +507,43 @@ __global__ void tile_flags_kernel(const uint8_t* __restrict__ m, int n, uint32_t
         for (int y = by; y < min(by + kTY, n); ++y)
             for (int x = bx; x < min(bx + kTX, n); ++x) any |= m[((long long)z * n + y) * n + x];
     flags[t] = any ? 1 : 0;
+}
+
+// One workgroup of 1024 threads: thread i owns a contiguous chunk of tiles,
+// so the lists come out in tile order.
+__global__ void __launch_bounds__(1024)
+tile_lists_kernel(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ prev, uint32_t tiles,
+                  uint32_t* __restrict__ compute, uint32_t* __restrict__ clear, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_a[1024], s_b[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (tiles + 1023u) / 1024u;
+    const uint32_t beg = min(tiles, tid * per), end = min(tiles, beg + per);
+    uint32_t na = 0, nb = 0;
+    for (uint32_t t = beg; t < end; ++t) {
+        const bool c = cur[t] != 0;
+        na += c ? 1u : 0u;
+        nb += (!c && prev && prev[t]) ? 1u : 0u;
+    }
+    s_a[tid] = na;
+    s_b[tid] = nb;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t a = tid >= d ? s_a[tid - d] : 0u, b = tid >= d ? s_b[tid - d] : 0u;
+        __syncthreads();
+        s_a[tid] += a;
+        s_b[tid] += b;
+        __syncthreads();
+    }
+    uint32_t oa = s_a[tid] - na, ob = s_b[tid] - nb;
+    for (uint32_t t = beg; t < end; ++t) {
+        const bool c = cur[t] != 0;
+        if (c) compute[oa++] = t;
+        else if (prev && prev[t]) clear[ob++] = t;
+    }
+    if (tid == 1023u) {
+        counts[0] = s_a[1023];
+        counts[1] = s_b[1023];
+    }
 }
 
 // g <- g * (act > 0) in place, and a BF16 copy of the result for the
@@ -527,6 +712,51 @@ __global__ void avgpool_kernel(const uint16_t* __restrict__ X, uint16_t* __restr
         s += bf2f(X[(((long long)zf * n + yf) * n + xf) * C + c]);
     }
     P[i] = f2bf(s * 0.125f);
+}
+
+// avgpool over a tile list of the coarse level (the tiles with fluid; tiles
+// in the clear list are zeroed): thread t of a workgroup owns coarse voxel t
+// of a 16x4x4 tile, 8 channels per 16-byte access, children summed in the
+// dense kernel's order (same values).
+template <int C>
+__global__ void __launch_bounds__(256)
+avgpool_list_kernel(const uint16_t* __restrict__ X, uint16_t* __restrict__ P, int n, TileList tl) {
+    const int nc = n / 2;
+    const uint32_t tilesX = tl.tilesX, tilesXY = tl.tilesX * tl.tilesY;
+    const uint32_t nClear = *tl.numClear, nComp = *tl.numCompute;
+    for (uint32_t u = blockIdx.x; u < nClear + nComp; u += gridDim.x) {
+        const bool clearing = u < nClear;
+        const uint32_t t = clearing ? tl.clear[u] : tl.compute[u - nClear];
+        const int x = (int)(t % tilesX) * kTX + (int)(threadIdx.x & 15);
+        const int y = (int)((t % tilesXY) / tilesX) * kTY + (int)((threadIdx.x >> 4) & 3);
+        const int z = (int)(t / tilesXY) * kTZ + (int)(threadIdx.x >> 6);
+        if (x >= nc || y >= nc || z >= nc) continue;
+        uint4* dst = reinterpret_cast<uint4*>(P + (((long long)z * nc + y) * nc + x) * C);
+#pragma unroll
+        for (int q = 0; q < C / 8; ++q) {
+            if (clearing) {
+                dst[q] = make_uint4(0u, 0u, 0u, 0u);
+                continue;
+            }
+            float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int xf = 2 * x + (k & 1), yf = 2 * y + ((k >> 1) & 1), zf = 2 * z + (k >> 2);
+                const uint4 v = reinterpret_cast<const uint4*>(X + (((long long)zf * n + yf) * n + xf) * C)[q];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    s[2 * e] += bf2f((uint16_t)(w[e] & 0xffffu));
+                    s[2 * e + 1] += bf2f((uint16_t)(w[e] >> 16));
+                }
+            }
+            uint32_t o[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                o[e] = (uint32_t)f2bf(s[2 * e] * 0.125f) | ((uint32_t)f2bf(s[2 * e + 1] * 0.125f) << 16);
+            dst[q] = make_uint4(o[0], o[1], o[2], o[3]);
+        }
+    }
 }
 
 __global__ void avgpool_bwd_kernel(const float* __restrict__ dP, float* __restrict__ dX, int n, int C) {
@@ -723,6 +953,42 @@ void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, 
     FLIP2_LAUNCH_CHECK();
 }
 
+void launch_tile_lists(const uint8_t* cur, const uint8_t* prev, uint32_t tiles, uint32_t* compute, uint32_t* clear,
+                       uint32_t* counts, hipStream_t s) {
+    hipLaunchKernelGGL(tile_lists_kernel, dim3(1), dim3(1024), 0, s, cur, prev, tiles, compute, clear, counts);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void conv_tile_dims(int n, uint32_t& tilesX, uint32_t& tilesY) {
+    const dim3 g = conv_grid(n);
+    tilesX = g.x;
+    tilesY = g.y;
+}
+
+void launch_conv3_list(const uint16_t* X, const uint16_t* Xcoarse, const uint16_t* W, const float* bias, uint16_t* Y,
+                       int n, int cin, int ca, int cout, int split, bool relu, const HeadArgs& head,
+                       const uint8_t* vmask, const TileList& tl, hipStream_t s) {
+    // three workgroups per CU fit the two LDS buffers (41.5 KB each)
+    const dim3 grid(256 * 3), block(256);
+    const int r = relu ? 1 : 0;
+#define LIST_CASE(CI, CA_, CO, SP)                                                                               \
+    if (cin == CI && ca == CA_ && cout == CO && split == SP) {                                                   \
+        hipLaunchKernelGGL((conv3_list_kernel<CI, CO, CA_, SP>), grid, block, 0, s, X, Xcoarse, W, bias, Y, n, r, \
+                           head, vmask, tl);                                                                      \
+        FLIP2_LAUNCH_CHECK();                                                                                    \
+        return;                                                                                                  \
+    }
+    LIST_CASE(16, 0, 16, 1)
+    LIST_CASE(16, 0, 32, 2)
+    LIST_CASE(32, 0, 32, 2)
+    LIST_CASE(32, 0, 64, 4)
+    LIST_CASE(64, 0, 64, 4)
+    LIST_CASE(96, 64, 32, 2)
+    LIST_CASE(48, 32, 16, 1)
+#undef LIST_CASE
+    throw Error(kErrArgument, "conv3 over a tile list: unsupported channels / split");
+}
+
 uint32_t conv_tiles(int n) {
     const dim3 g = conv_grid(n);
     return g.x * g.y * g.z;
@@ -817,6 +1083,12 @@ void launch_relu_mask_bf16(float* g, const uint16_t* act, uint16_t* gb, long lon
 void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s) {
     const long long total = (long long)(n / 2) * (n / 2) * (n / 2) * C;
     hipLaunchKernelGGL(avgpool_kernel, g1(total, 256), dim3(256), 0, s, X, P, n, C);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_avgpool_list(const uint16_t* X, uint16_t* P, int n, int C, const TileList& coarse, hipStream_t s) {
+    if (C == 16) hipLaunchKernelGGL(avgpool_list_kernel<16>, dim3(1024), dim3(256), 0, s, X, P, n, coarse);
+    else if (C == 32) hipLaunchKernelGGL(avgpool_list_kernel<32>, dim3(1024), dim3(256), 0, s, X, P, n, coarse);
+    else throw Error(kErrArgument, "avgpool over a tile list: 16 or 32 channels");
     FLIP2_LAUNCH_CHECK();
 }
 void launch_avgpool_bwd(const float* dP, float* dX, int n, int C, hipStream_t s) {

@@ -18,6 +18,15 @@ struct ConvMask {
     const uint8_t* tiles;
     const uint8_t* prevTiles;
 };
+// Tiles of a level to compute and to clear (device lists in x-fastest tile
+// order, with device counts), for the list-driven convolution.
+struct TileList {
+    const uint32_t* compute;
+    const uint32_t* numCompute;
+    const uint32_t* clear;
+    const uint32_t* numClear;
+    uint32_t tilesX, tilesY;
+};
 // Y = act(conv3x3x3(X, W) + bias), MFMA BF16 with FP32 accumulation.
 // W: [27][cout][cin] BF16 (tap = (dz*3 + dy)*3 + dx, offsets d-1).
 void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin, int cout,
@@ -43,6 +52,18 @@ void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, 
 // 16 -> 16 convolution with the output head in its epilogue (Y may be null).
 void launch_conv3_fwd_head(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin,
                            bool relu, const HeadArgs& head, const ConvMask& mask, hipStream_t s);
+// List-driven convolution (the preconditioner apply): computes the tiles in
+// tl.compute and zeroes those in tl.clear; vmask as ConvMask::voxel.  ca > 0:
+// input = concat(up2(Xcoarse) [ca], X [cin - ca]).  split: workgroups per
+// tile (output channels divided among them).  head as launch_conv3_fwd_head.
+void launch_conv3_list(const uint16_t* X, const uint16_t* Xcoarse, const uint16_t* W, const float* bias, uint16_t* Y,
+                       int n, int cin, int ca, int cout, int split, bool relu, const HeadArgs& head,
+                       const uint8_t* vmask, const TileList& tl, hipStream_t s);
+// Lists of the tiles with cur set (compute) and with cur clear but prev set
+// (clear); prev may be null.  counts[0], counts[1] receive their lengths.
+void launch_tile_lists(const uint8_t* cur, const uint8_t* prev, uint32_t tiles, uint32_t* compute, uint32_t* clear,
+                       uint32_t* counts, hipStream_t s);
+void conv_tile_dims(int n, uint32_t& tilesX, uint32_t& tilesY);
 // Coarse mask: out[v] = any of the 8 children of v in m (n^3 -> (n/2)^3).
 void launch_mask_pool(const uint8_t* m, int n, uint8_t* out, hipStream_t s);
 // Per-tile flags of a level mask for the convolution tiles (see ConvMask).
@@ -58,6 +79,9 @@ void launch_conv3_wgrad(const uint16_t* dPb, const float* dP, const uint16_t* X,
 // g <- g * (act > 0) in place; gb = BF16 copy.
 void launch_relu_mask_bf16(float* g, const uint16_t* act, uint16_t* gb, long long count, hipStream_t s);
 void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s);
+// The same over the tile list of the coarse level (n/2)^3: computes its
+// compute tiles, zeroes its clear tiles.
+void launch_avgpool_list(const uint16_t* X, uint16_t* P, int n, int C, const TileList& coarse, hipStream_t s);
 void launch_avgpool_bwd(const float* dP, float* dX, int n, int C, hipStream_t s);  // dX += ...
 void launch_upcat(const uint16_t* A, const uint16_t* B, uint16_t* U, int n, int Ca, int Cb, hipStream_t s);
 void launch_upcat_bwd(const float* dU, float* dA, float* dB, int n, int Ca, int Cb, hipStream_t s);  // +=
