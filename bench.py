@@ -75,10 +75,34 @@ def cpu_baseline(target_seconds=10.0):
         el = time.perf_counter() - t0
         if el >= target_seconds or reps >= 50:
             break
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": n * reps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": f"{reps} serial steps of the oracle's C transfer step (bin, 21-bit 1-bit-split radix sort, "
                       f"gather, counts, exclusive scan, FP64 trilinear MAC P2G) on 2^21 uniform-random particles "
-                      f"in a 128^3 grid, {el:.1f} s on 1 host thread"}
+                      f"in a 128^3 grid, {el:.1f} s on 1 host thread of {os.cpu_count()} ({model})"}
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Measured device-to-device copy rate (bytes read + written per second)
+    of the library's 16-byte copy kernel, as a practical HBM ceiling."""
+    import flip2_amd as F
+    src, dst = F.DeviceBuffer(nbytes), F.DeviceBuffer(nbytes)
+    F.check(F.lib().flip2_memset(src.ptr, 1, nbytes))
+    F.check(F.lib().flip2_copy(dst.ptr, src.ptr, nbytes, None))
+    F.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        F.check(F.lib().flip2_copy(dst.ptr, src.ptr, nbytes, None))
+    F.synchronize()
+    return 2.0 * nbytes * reps / (time.perf_counter() - t0) / 1e9
 
 
 def _fluid_step_stats(sim, steps):
@@ -141,6 +165,42 @@ def config_run(args):
     print(json.dumps(out), flush=True)
 
 
+def rc_run(args):
+    """The reference-compatible path (Particles::solveFrame: binning, 1-bit-split
+    order sort, voxelisation, FP64 P2G, brick-local red-black SOR, G2P, Euler)
+    on the C1 dam break: 10 frames at 240 frames/s (one JSON line)."""
+    import flip2_amd as F
+    from flip2_amd import scenes
+
+    n, frames, fps = 64, 10, 240.0
+    x, y, z = scenes.dam_break(n, seed=1)
+    p = F.Particles(x.size)
+    p.setDomain(0.0, 0.0, 0.0, n, n, n, 1.0 / n)
+    p.set_positions(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64))
+    zero = np.zeros(x.size)
+    p.set_velocities(zero, zero, zero)
+    p.initialize()
+    p.solveFrame(fps)  # warm-up frame
+    F.synchronize()
+    sor, subs = [], 0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        p.solveFrame(fps)
+        st = p.stats()
+        sor.append(st["lastSorIterations"])
+        subs += st["substeps"]
+    F.synchronize()
+    wall = time.perf_counter() - t0
+    st = p.stats()
+    out = {"metric": "particle-frames/s", "config": "c1rc",
+           "workload": "reference-compatible path (FP64, SOR pressure) on the C1 dam break: x<0.4, y<0.6, 64^3, "
+                       "8 jittered particles per cell, 10 frames at 240 frames/s",
+           "particles": int(x.size), "grid": n, "frames": frames, "value": x.size * frames / wall,
+           "ms_per_frame": round(wall / frames * 1e3, 3), "substeps": subs, "sor_iterations_last": sor,
+           "used_nodes": st["numUsedGridNodes"], "used_voxels": st["numUsedVoxels"]}
+    print(json.dumps(out), flush=True)
+
+
 def learned_run(args):
     """Learned preconditioner vs plain PCG, ms per frame of the pressure solve
     to the 1e-6 FP64 relative residual (c4: train then compare; c5: apply a
@@ -194,6 +254,8 @@ def learned_run(args):
     else:
         raise SystemExit(f"unknown config {args.config}")
     sim.set_unet(net, fallback_iterations=args.fallback)
+    if args.no_graph:
+        sim.set_learned_graph(False)  # one launch per kernel (for kernel tracing)
     res = {"jacobi": [], "mic0": [], "learned": []}
     t_sim = 0.0
     dt = 1.0 / 240.0
@@ -230,9 +292,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB --steps times (reference traffic for counters)")
-    ap.add_argument("--config", choices=["c1", "c3", "c4", "c5"], default=None,
-                    help="whole FLIP steps (c1, c3) or the learned-vs-PCG solve comparison (c4, c5) instead of "
-                         "the transfer microbench")
+    ap.add_argument("--config", choices=["c1", "c1rc", "c3", "c4", "c5"], default=None,
+                    help="whole FLIP steps (c1, c3), the reference-compatible path on C1 (c1rc) or the "
+                         "learned-vs-PCG solve comparison (c4, c5) instead of the transfer microbench")
     ap.add_argument("--grid-c3", type=int, default=512)
     ap.add_argument("--grid-c4", type=int, default=128)
     ap.add_argument("--grid-c5", type=int, default=256)
@@ -240,6 +302,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--compare-frames", type=int, default=8)
     ap.add_argument("--fallback", type=int, default=0, help="learned: Jacobi-PCG fallback iteration cap (0: none)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="learned: launch the preconditioner's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--save-net", default=None)
     ap.add_argument("--load-net", default=None)
     args = ap.parse_args()
@@ -250,6 +314,8 @@ def main():
 
     if args.config in ("c1", "c3"):
         return config_run(args)
+    if args.config == "c1rc":
+        return rc_run(args)
     if args.config in ("c4", "c5"):
         return learned_run(args)
 
@@ -327,7 +393,8 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBPS, 4),
                      "traffic": (int(TRAFFIC_BYTES[dom]) if (n == 1 << 24 and gn == 256 and dom in TRAFFIC_BYTES)
                                  else None),
-                     "alg_bytes": table[dom]["alg_bytes"]},
+                     "alg_bytes": table[dom]["alg_bytes"],
+                     "copy_GBps": round(copy_bandwidth(), 1)},
         "kernels": table,
         "cpu_baseline": None,
     }
