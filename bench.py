@@ -168,33 +168,38 @@ def config_run(args):
 def rc_run(args):
     """The reference-compatible path (Particles::solveFrame: binning, 1-bit-split
     order sort, voxelisation, FP64 P2G, brick-local red-black SOR, G2P, Euler)
-    on the C1 dam break: 10 frames at 240 frames/s (one JSON line)."""
+    on the C1 dam break: --rc-frames frames at 240 frames/s (one JSON line)."""
     import flip2_amd as F
     from flip2_amd import scenes
 
-    n, frames, fps = 64, 10, 240.0
+    n, frames, fps = 64, args.rc_frames, 240.0
     x, y, z = scenes.dam_break(n, seed=1)
+    print(f"[bench] c1rc: {x.size} particles", file=sys.stderr, flush=True)
     p = F.Particles(x.size)
     p.setDomain(0.0, 0.0, 0.0, n, n, n, 1.0 / n)
     p.set_positions(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64))
     zero = np.zeros(x.size)
     p.set_velocities(zero, zero, zero)
     p.initialize()
+    t0 = time.perf_counter()
     p.solveFrame(fps)  # warm-up frame
     F.synchronize()
+    print(f"[bench] c1rc: warm-up frame {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     sor, subs = [], 0
     t0 = time.perf_counter()
-    for _ in range(frames):
+    for f in range(frames):
         p.solveFrame(fps)
         st = p.stats()
         sor.append(st["lastSorIterations"])
         subs += st["substeps"]
+        print(f"[bench] c1rc: frame {f + 1}: {st['substeps']} substeps, last SOR {st['lastSorIterations']} "
+              f"iterations, {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     F.synchronize()
     wall = time.perf_counter() - t0
     st = p.stats()
     out = {"metric": "particle-frames/s", "config": "c1rc",
            "workload": "reference-compatible path (FP64, SOR pressure) on the C1 dam break: x<0.4, y<0.6, 64^3, "
-                       "8 jittered particles per cell, 10 frames at 240 frames/s",
+                       f"8 jittered particles per cell, {frames} frames at 240 frames/s",
            "particles": int(x.size), "grid": n, "frames": frames, "value": x.size * frames / wall,
            "ms_per_frame": round(wall / frames * 1e3, 3), "substeps": subs, "sor_iterations_last": sor,
            "used_nodes": st["numUsedGridNodes"], "used_voxels": st["numUsedVoxels"]}
@@ -296,6 +301,7 @@ def main():
                     help="whole FLIP steps (c1, c3), the reference-compatible path on C1 (c1rc) or the "
                          "learned-vs-PCG solve comparison (c4, c5) instead of the transfer microbench")
     ap.add_argument("--grid-c3", type=int, default=512)
+    ap.add_argument("--rc-frames", type=int, default=3, help="c1rc: timed frames")
     ap.add_argument("--grid-c4", type=int, default=128)
     ap.add_argument("--grid-c5", type=int, default=256)
     ap.add_argument("--frames", type=int, default=200, help="c4: frames of Jacobi-PCG solves captured for training")

@@ -136,18 +136,25 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ a, uint32_t count) {
     if (i < count) a[i] = 0u;
 }
 
-// counts[key] = length of key's run in the sorted key array (written by the
-// run's first element; cells without particles keep the zero fill).
-// counts[key] = run length of key in the sorted keys.
+// counts[0 .. numCells] from the sorted keys, every entry written once: the
+// first element of each run of equal keys writes its cell's run length and
+// zeros for the empty cells between the previous run's cell and its own (the
+// last run also for the cells after it, through counts[numCells]).  The
+// threads of a wave cover adjacent cell ranges, so the writes are contiguous
+// and no separate zero fill is needed.  Keys >= numCells count as numCells.
 __global__ void run_length_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t numCells,
                                   uint32_t* __restrict__ counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t k = keys[i];
-    if (i > 0 && keys[i - 1] == k) return;
+    const uint32_t k = min(keys[i], numCells);
+    const uint32_t prev = i > 0 ? min(keys[i - 1], numCells) : 0u;
+    if (i > 0 && prev == k) return;
+    for (uint32_t c = i > 0 ? prev + 1 : 0u; c < k; ++c) counts[c] = 0u;
     uint32_t e = i + 1;
-    while (e < n && keys[e] == k) ++e;
+    while (e < n && min(keys[e], numCells) == k) ++e;
     if (k < numCells) counts[k] = e - i;
+    if (e == n)
+        for (uint32_t c = k < numCells ? k + 1 : numCells; c <= numCells; ++c) counts[c] = 0u;
 }
 
 // Statistics for a P2G call whose inputs did not come through the pipeline:
@@ -458,9 +465,11 @@ void launch_unpack(const float4* rec, uint32_t n, Float6Out dst, hipStream_t s) 
 }
 
 void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, hipStream_t s) {
-    hipLaunchKernelGGL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
-    FLIP2_LAUNCH_CHECK();
-    if (!n) return;
+    if (!n) {
+        hipLaunchKernelGGL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
+        FLIP2_LAUNCH_CHECK();
+        return;
+    }
     hipLaunchKernelGGL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts);
     FLIP2_LAUNCH_CHECK();
 }

@@ -59,6 +59,13 @@ def test_transfer_dense_cluster(flip2, oracle):
     run_case(flip2, oracle, 200_000, 32, seed=7, lo=0.4, hi=0.45)
 
 
+def test_transfer_few_particles(flip2, oracle):
+    # one and two particles: a single run, empty cells before and after it
+    # through the count array's last entry
+    run_case(flip2, oracle, 1, 16, seed=8)
+    run_case(flip2, oracle, 2, 16, seed=9)
+
+
 def test_transfer_microbench_size(flip2, oracle):
     # 2^24 particles on 256^3: integer outputs bit-exact, faces vs FP64 oracle
     run_case(flip2, oracle, 1 << 24, 256, seed=2)
