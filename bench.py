@@ -52,7 +52,7 @@ def algorithmic_bytes(name, n, cells, occupied, passes):
     if name == "gather":
         return n * (4 + 24 + 24)
     if name == "counts":
-        return 4 * (cells + 1) + 4 * n + 4 * occupied
+        return 4 * n + 4 * (cells + 1)  # sorted keys read, every count written once
     if name == "scan":
         return 8 * (cells + 1)
     if name == "p2g":
@@ -234,7 +234,9 @@ def learned_run(args):
         t0 = time.perf_counter()
         losses = []
         for e in range(args.epochs):
-            losses.append(net.train(1, 1e-3 if e < args.epochs - 1 else 3e-4, 0, nf, seed=e))
+            # learning rate from 1e-3 down to 3e-4, geometrically over the epochs
+            lr = 1e-3 * 0.3 ** (e / max(1, args.epochs - 1))
+            losses.append(net.train(1, lr, 0, nf, seed=e))
             print(f"[bench] epoch {e + 1}: loss {losses[-1]:.4f}", file=sys.stderr, flush=True)
         train_s = time.perf_counter() - t0
         if args.save_net:
@@ -261,6 +263,7 @@ def learned_run(args):
     sim.set_unet(net, fallback_iterations=args.fallback)
     if args.no_graph:
         sim.set_learned_graph(False)  # one launch per kernel (for kernel tracing)
+    sim.set_flex_directions(args.flex)
     res = {"jacobi": [], "mic0": [], "learned": []}
     t_sim = 0.0
     dt = 1.0 / 240.0
@@ -308,6 +311,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--compare-frames", type=int, default=8)
     ap.add_argument("--fallback", type=int, default=0, help="learned: Jacobi-PCG fallback iteration cap (0: none)")
+    ap.add_argument("--flex", type=int, default=8, help="learned: earlier directions flexible CG keeps (1..16)")
     ap.add_argument("--no-graph", action="store_true",
                     help="learned: launch the preconditioner's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--save-net", default=None)
