@@ -1,12 +1,12 @@
 // Kernels of the learned preconditioner: a 3-level 3D U-Net on a dense
 // n^3 grid, channels-last activations ([z][y][x][C]).
 //
-// Forward convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x16_bf16):
+// Forward convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x32_bf16):
 // BF16 weights and activations, FP32 accumulation, bias + ReLU fused in the
 // epilogue.  D[cout][voxel] = sum_{tap,cin} W[tap][cout][cin] X[voxel+tap][cin]
-// with M = 16 output channels, N = 16 voxels, K = 16 input channels per
-// instruction.  A wave owns 64 voxels (an 8 x 8 slice of a 4 x 8 x 8 block
-// whose input, with halo, is staged in LDS) and all output channels.
+// with M = 16 output channels, N = 16 voxels, K = 2 taps x 16 input channels
+// per instruction.  A workgroup owns a 16 x 4 x 4 block of output voxels
+// (whose input, with halo, is staged in LDS), each wave a 16 x 4 slice.
 //
 // Backward: the data gradient is the same kernel on the BF16 gradient with
 // transposed, mirrored weights (FP32 output); the weight gradient is a GEMM
@@ -55,8 +55,8 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b <<
 // LDS 16 channels at a time, as two planes of 8 channels ([chunk][voxel], 16
 // bytes per voxel), so each input voxel is fetched from memory once per
 // workgroup instead of once per tap.  MFMA is v_mfma_f32_16x16x32_bf16 with
-// K = two taps x 16 channels: lane (col, kq) holds tap 2p + (kq >> 1),
-// channels 8 (kq & 1) .. +7 of output-row voxel col.  A B fragment is one
+// K = two taps x 16 channels (taps that share dy, so one B fragment serves
+// three output rows).  A B fragment is one
 // 16-byte LDS read, and the 16 lanes of a quarter wave read 16 consecutive
 // voxels of one plane: 256 contiguous bytes, which cover the LDS banks once
 // (with [voxel][16 channels] rows and 8-byte reads, bank conflicts made LDS
@@ -65,7 +65,7 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b <<
 // through the caches.
 constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kHalo = kHX * kHY * kHZ;  // 648 voxels
-constexpr int kTapPairs = 14;  // taps 0..26 in pairs; tap 27 is zero
+constexpr int kStageLoads = (kHalo * 2 + 255) / 256;  // 16-byte loads per thread per input stage (6)
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -139,28 +139,72 @@ __device__ inline const uint4* stage_src(const uint16_t* X, const uint16_t* Xc, 
 
 // The 27 taps of one 16-channel input stage held in LDS (two 8-channel
 // planes) into the accumulators of output blocks m0 .. m0 + MB - 1.
+// The K = 32 of an MFMA pairs two taps that share dy: the nine (dx, dz)
+// combinations s = dx + 3 dz form the pairs (2q, 2q + 1), q = 0..4 (s = 9 is
+// zero).  The B fragment of pair q at input row r (the wave's 4 output rows
+// read rows 0..5 of the halo) is the same for the output rows j = r - dy,
+// dy = 0..2, so six 16-byte LDS reads feed twelve MFMAs (per 16 output
+// channels): 30 reads per stage instead of one per MFMA.  Lane (col, kq)
+// holds combination 2q + (kq >> 1), channels 8 (kq & 1) .. +7.
 template <int CIN, int COUT, int MB>
 __device__ inline void conv_stage_mfma(const uint4 (*sIn)[kHalo], const uint16_t* __restrict__ Wt, int c0, int m0,
                                        int wave, int col, int kq, v4f (&acc)[MB][4]) {
-    const int chunk = kq & 1, tapOdd = kq >> 1;
-#pragma unroll 2
-    for (int p = 0; p < kTapPairs; ++p) {
-        const int tap = 2 * p + tapOdd;
-        const bool live = tap < 27;
-        v8bf b[4];
+    const int chunk = kq & 1, half = kq >> 1;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint4 raw = live ? sIn[chunk][halo_index(wave, j, col, tap)] : make_uint4(0u, 0u, 0u, 0u);
-            b[j] = __builtin_bit_cast(v8bf, raw);
+    for (int q = 0; q < 5; ++q) {
+        const int sc = 2 * q + half;
+        const bool live = sc < 9;
+        const int dx = sc % 3, dz = sc / 3;
+        v8bf b[kTY + 2];
+#pragma unroll
+        for (int r = 0; r < kTY + 2; ++r) {
+            const uint4 raw = live ? sIn[chunk][((wave + dz) * kHY + r) * kHX + col + dx] : make_uint4(0u, 0u, 0u, 0u);
+            b[r] = __builtin_bit_cast(v8bf, raw);
         }
 #pragma unroll
         for (int m = 0; m < MB; ++m) {
-            const uint4 raw = live ? *reinterpret_cast<const uint4*>(
-                                         Wt + ((size_t)(tap * COUT + 16 * (m0 + m) + col) * CIN) + c0 + 8 * chunk)
-                                   : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int tap = dx + 3 * dy + 9 * dz;
+                const uint4 raw = live ? *reinterpret_cast<const uint4*>(
+                                             Wt + ((size_t)(tap * COUT + 16 * (m0 + m) + col) * CIN) + c0 + 8 * chunk)
+                                       : make_uint4(0u, 0u, 0u, 0u);
+                const v8bf a = __builtin_bit_cast(v8bf, raw);
+#pragma unroll
+                for (int j = 0; j < kTY; ++j)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j + dy], acc[m][j], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// The same stage with the weights of one 16-channel output block in LDS,
+// sW[(tap * 2 + chunk) * 16 + cout] = 8 input channels (16 bytes): the A
+// fragments are LDS reads like the B fragments, so the MFMAs wait on LDS only
+// and never on the global loads of a prefetch in flight.
+constexpr int kWStage = 27 * 2 * 16;                         // 16-byte weight fragments per stage (864)
+constexpr int kWLoads = (kWStage + 255) / 256;               // per thread (4)
+__device__ inline void conv_stage_lds(const uint4 (*sIn)[kHalo], const uint4* sW, int wave, int col, int kq,
+                                      v4f (&acc)[4]) {
+    const int chunk = kq & 1, half = kq >> 1;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const int sc = 2 * q + half;
+        const bool live = sc < 9;
+        const int dx = sc % 3, dz = sc / 3;
+        v8bf b[kTY + 2];
+#pragma unroll
+        for (int r = 0; r < kTY + 2; ++r) {
+            const uint4 raw = live ? sIn[chunk][((wave + dz) * kHY + r) * kHX + col + dx] : make_uint4(0u, 0u, 0u, 0u);
+            b[r] = __builtin_bit_cast(v8bf, raw);
+        }
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+            const int tap = dx + 3 * dy + 9 * dz;
+            const uint4 raw = live ? sW[(tap * 2 + chunk) * 16 + col] : make_uint4(0u, 0u, 0u, 0u);
             const v8bf a = __builtin_bit_cast(v8bf, raw);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
+            for (int j = 0; j < kTY; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j + dy], acc[j], 0, 0, 0);
         }
     }
 }
@@ -266,30 +310,42 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
+    // Each stage (648 voxels x 16 channels, as two 8-channel planes) is
+    // loaded into registers while the previous stage's MFMAs run.
+    uint4 pre[kStageLoads];
+    auto load_stage = [&](int c0) {
+#pragma unroll
+        for (int k = 0; k < kStageLoads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, bx, by, bz, c0, t) : nullptr;
+            pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    load_stage(0);
 #pragma unroll 1
     for (int c0 = 0; c0 < CIN; c0 += 16) {
-        __syncthreads();
-        // stage 648 voxels x 16 channels as two 8-channel planes
-        for (int t = threadIdx.x; t < kHalo * 2; t += 256) {
-            const uint4* src = stage_src<CIN, CA>(X, Xc, n, bx, by, bz, c0, t);
-            sIn[t & 1][t >> 1] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+        if (c0) __syncthreads();  // every wave is done with the previous stage
+#pragma unroll
+        for (int k = 0; k < kStageLoads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            if (t < kHalo * 2) sIn[t & 1][t >> 1] = pre[k];
         }
         __syncthreads();
+        if (c0 + 16 < CIN) load_stage(c0 + 16);
         conv_stage_mfma<CIN, COUT, MB>(sIn, Wt, c0, m0, wave, col, kq, acc);
     }
     conv_epilogue<COUT, MODE, MB>(acc, bias, Y, Yf, n, relu, head, cm.voxel, bx, by, bz, m0, wave, col, kq);
 }
 
 // The same convolution over a list of tiles (the tiles of a level that hold
-// masked-in voxels): a fixed grid of workgroups loops over the list, and the
-// next input stage's global loads are issued into registers before the
-// current stage's MFMAs, then stored into the other of two LDS buffers, so
-// memory latency overlaps compute instead of waiting at every tile.  Each
-// XCD takes one contiguous part of the list (neighbouring tiles, shared halo
-// rows in its L2).  Tiles in `clear` (computed by an earlier forward, not by
-// this one) are zeroed first.
-constexpr int kStageLoads = (kHalo * 2 + 255) / 256;  // 16-byte loads per thread per stage (6)
-
+// masked-in voxels), for one 16-channel output block per workgroup: a fixed
+// grid of workgroups loops over the list, one (tile, channel group, input
+// stage) step at a time.  The next step's input halo and weights are loaded
+// into registers before the current step's MFMAs and stored into the other of
+// two LDS buffers after them, so memory latency overlaps compute.  Each XCD
+// takes one contiguous part of the list (neighbouring tiles, shared halo rows
+// in its L2).  Tiles in `clear` (computed by an earlier forward, not by this
+// one) are zeroed first.
 template <int CIN, int COUT, int CA = 0, int SPLIT = 1>
 __global__ void __launch_bounds__(256)
 conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc, const uint16_t* __restrict__ Wt,
@@ -297,8 +353,9 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
                   const uint8_t* __restrict__ vmask, TileList tl) {
     constexpr int MB = COUT / 16 / SPLIT;
     constexpr int STAGES = CIN / 16;
-    static_assert(MB >= 1 && MB * 16 * SPLIT == COUT, "output channels split into whole 16-channel blocks");
+    static_assert(MB == 1 && 16 * SPLIT == COUT, "one 16-channel output block per workgroup");
     __shared__ __align__(16) uint4 sIn[2][2][kHalo];  // [buffer][8-channel chunk][halo voxel]
+    __shared__ __align__(16) uint4 sW[2][kWStage];    // [buffer][(tap, chunk, cout)]
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
@@ -320,10 +377,9 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
             if (x < n && y < n && z < n) {
                 const long long v = ((long long)z * n + y) * n + x;
                 if (Y) {
-                    constexpr int CW = COUT / SPLIT;
-                    uint4* dst = reinterpret_cast<uint4*>(Y + v * COUT + mg * CW);
-#pragma unroll
-                    for (int q = 0; q < CW / 8; ++q) dst[q] = make_uint4(0u, 0u, 0u, 0u);
+                    uint4* dst = reinterpret_cast<uint4*>(Y + v * COUT + mg * 16);
+                    dst[0] = make_uint4(0u, 0u, 0u, 0u);
+                    dst[1] = make_uint4(0u, 0u, 0u, 0u);
                 }
                 if (head.out && mg == 0) head.out[v] = 0.f;
             }
@@ -334,34 +390,53 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
     const uint32_t beg = (uint32_t)(((uint64_t)U * xcd) >> 3), end = (uint32_t)(((uint64_t)U * (xcd + 1)) >> 3);
     uint32_t u = beg + w;
     if (u >= end) return;
-    // step = (unit, stage); the loads of step s + 1 are in flight during step s
     auto unit_origin = [&](uint32_t unit, int& bx, int& by, int& bz) {
         const uint32_t t = tl.compute[unit / SPLIT];
         bx = (int)(t % tilesX) * kTX;
         by = (int)((t % tilesXY) / tilesX) * kTY;
         bz = (int)(t / tilesXY) * kTZ;
     };
+    uint4 pre[kStageLoads], preW[kWLoads];
+    // loads of step (unit, stage) into pre / preW
+    auto load_step = [&](uint32_t unit, int bx, int by, int bz, int stage) {
+#pragma unroll
+        for (int k = 0; k < kStageLoads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, bx, by, bz, 16 * stage, t) : nullptr;
+            pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+        }
+        const int co0 = (int)(unit % SPLIT) * 16;
+#pragma unroll
+        for (int k = 0; k < kWLoads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            const int tap = t >> 5, chunk = (t >> 4) & 1, co = t & 15;
+            const uint4* src = t < kWStage ? reinterpret_cast<const uint4*>(Wt + (size_t)(tap * COUT + co0 + co) * CIN +
+                                                                            16 * stage + 8 * chunk)
+                                           : nullptr;
+            preW[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    auto store_step = [&](int b) {
+#pragma unroll
+        for (int k = 0; k < kStageLoads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            if (t < kHalo * 2) sIn[b][t & 1][t >> 1] = pre[k];
+        }
+#pragma unroll
+        for (int k = 0; k < kWLoads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            if (t < kWStage) sW[b][t] = preW[k];
+        }
+    };
     int bx, by, bz;
     unit_origin(u, bx, by, bz);
-    uint4 pre[kStageLoads];
-#pragma unroll
-    for (int k = 0; k < kStageLoads; ++k) {
-        const int t = (int)threadIdx.x + 256 * k;
-        const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, bx, by, bz, 0, t) : nullptr;
-        pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
-    }
-#pragma unroll
-    for (int k = 0; k < kStageLoads; ++k) {
-        const int t = (int)threadIdx.x + 256 * k;
-        if (t < kHalo * 2) sIn[0][t & 1][t >> 1] = pre[k];
-    }
+    load_step(u, bx, by, bz, 0);
+    store_step(0);
     __syncthreads();
     int buf = 0, stage = 0;
-    v4f acc[MB][4];
+    v4f acc[4];
 #pragma unroll
-    for (int m = 0; m < MB; ++m)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
     for (;;) {
         // next step
         uint32_t nu = u;
@@ -374,28 +449,18 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
         int nbx = bx, nby = by, nbz = bz;
         if (more) {
             if (nu != u) unit_origin(nu, nbx, nby, nbz);
-#pragma unroll
-            for (int k = 0; k < kStageLoads; ++k) {
-                const int t = (int)threadIdx.x + 256 * k;
-                const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, nbx, nby, nbz, 16 * ns, t) : nullptr;
-                pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
-            }
+            load_step(nu, nbx, nby, nbz, ns);
         }
-        const int m0 = (int)(u % SPLIT) * MB;
-        conv_stage_mfma<CIN, COUT, MB>(sIn[buf], Wt, 16 * stage, m0, wave, col, kq, acc);
+        conv_stage_lds(sIn[buf], sW[buf], wave, col, kq, acc);
         if (stage == STAGES - 1) {
-            conv_epilogue<COUT, 0, MB>(acc, bias, Y, nullptr, n, relu, head, vmask, bx, by, bz, m0, wave, col, kq);
+            const int m0 = (int)(u % SPLIT);
+            v4f (&accm)[1][4] = *reinterpret_cast<v4f(*)[1][4]>(&acc);
+            conv_epilogue<COUT, 0, 1>(accm, bias, Y, nullptr, n, relu, head, vmask, bx, by, bz, m0, wave, col, kq);
 #pragma unroll
-            for (int m = 0; m < MB; ++m)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[m][j] = v4f{0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
         }
         if (!more) break;
-#pragma unroll
-        for (int k = 0; k < kStageLoads; ++k) {
-            const int t = (int)threadIdx.x + 256 * k;
-            if (t < kHalo * 2) sIn[buf ^ 1][t & 1][t >> 1] = pre[k];
-        }
+        store_step(buf ^ 1);
         __syncthreads();
         buf ^= 1;
         u = nu;
@@ -968,8 +1033,8 @@ void conv_tile_dims(int n, uint32_t& tilesX, uint32_t& tilesY) {
 void launch_conv3_list(const uint16_t* X, const uint16_t* Xcoarse, const uint16_t* W, const float* bias, uint16_t* Y,
                        int n, int cin, int ca, int cout, int split, bool relu, const HeadArgs& head,
                        const uint8_t* vmask, const TileList& tl, hipStream_t s) {
-    // three workgroups per CU fit the two LDS buffers (41.5 KB each)
-    const dim3 grid(256 * 3), block(256);
+    // two workgroups per CU fit the double-buffered halo and weights (69 KB each)
+    const dim3 grid(256 * 2), block(256);
     const int r = relu ? 1 : 0;
 #define LIST_CASE(CI, CA_, CO, SP)                                                                               \
     if (cin == CI && ca == CA_ && cout == CO && split == SP) {                                                   \
