@@ -538,15 +538,15 @@ void UNet::forward(hipStream_t s, bool keepActivations) {
     const nn::ConvMask* cm = cm_;
     if (!cm[0].tiles) tilesDirty_ = true;  // a dense forward writes every tile
     // the first layer reads the two non-zero input channels only
-    nn::launch_conv3_in2(in0c_.devPtr(), paramsBf_.devPtr() + layers_[0].w, params_.devPtr() + layers_[0].b,
-                         a0_.devPtr(), n_, true, cm[0], s);
     if (cm[0].tiles) {
-        // The apply.  Level 0 (many tiles): one workgroup per tile of the
-        // grid, tiles without fluid skipped by their flag.  Levels 1 and 2
-        // (few tiles): a fixed grid of workgroups over the list of tiles with
-        // fluid, 2 or 4 workgroups per tile (output channels split), the
-        // next input stage prefetched during the current one's MFMAs.  Both
-        // measured faster than the other choice on their levels.
+        // The apply.  Every convolution runs on a fixed grid of workgroups
+        // over the list of its level's tiles with fluid (1, 2 or 4
+        // workgroups per tile on levels 0, 1, 2: output channels split), the
+        // next step's input halo and weights prefetched during the current
+        // step's MFMAs.  The decoder's concatenated inputs are read in place,
+        // and the output head runs in the last convolution.
+        nn::launch_conv3_in2_list(in0c_.devPtr(), paramsBf_.devPtr() + layers_[0].w, params_.devPtr() + layers_[0].b,
+                                  a0_.devPtr(), n_, true, cm[0].voxel, tl_[0], s);
         const int levelN[3] = {n_, n1_, n2_};
         const int split[3] = {1, 2, 4};
         auto lconv = [&](int layer, const uint16_t* X, const uint16_t* Xc, int ca, uint16_t* Y, const nn::HeadArgs& h) {
@@ -555,7 +555,7 @@ void UNet::forward(hipStream_t s, bool keepActivations) {
                                   ca, L.cout, split[L.level], true, h, cm[L.level].voxel, tl_[L.level], s);
         };
         const nn::HeadArgs none{};
-        conv(1, a0_.devPtr(), b0_.devPtr(), cm[0], s);
+        lconv(1, a0_.devPtr(), nullptr, 0, b0_.devPtr(), none);
         nn::launch_avgpool_list(b0_.devPtr(), p0_.devPtr(), n_, 16, tl_[1], s);
         lconv(2, p0_.devPtr(), nullptr, 0, a1_.devPtr(), none);
         lconv(3, a1_.devPtr(), nullptr, 0, b1_.devPtr(), none);
@@ -564,16 +564,14 @@ void UNet::forward(hipStream_t s, bool keepActivations) {
         lconv(5, a2_.devPtr(), nullptr, 0, b2_.devPtr(), none);
         lconv(6, b1_.devPtr(), b2_.devPtr(), 64, c1_.devPtr(), none);
         lconv(7, c1_.devPtr(), nullptr, 0, d1_.devPtr(), none);
-        const UNetLayer& L8 = layers_[8];
-        const UNetLayer& L9 = layers_[9];
-        nn::launch_conv3_fwd_upcat(b0_.devPtr(), d1_.devPtr(), paramsBf_.devPtr() + L8.w, params_.devPtr() + L8.b,
-                                   c0_.devPtr(), n_, 32, 16, 16, true, cm[0], s);
+        lconv(8, b0_.devPtr(), d1_.devPtr(), 32, c0_.devPtr(), none);
         const nn::HeadArgs head{params_.devPtr() + headW_, params_.devPtr() + headB_, mask_.devPtr(), scale_.devPtr(),
                                 out_.devPtr()};
-        nn::launch_conv3_fwd_head(c0_.devPtr(), paramsBf_.devPtr() + L9.w, params_.devPtr() + L9.b, nullptr, n_, 16,
-                                  true, head, cm[0], s);
+        lconv(9, c0_.devPtr(), nullptr, 0, nullptr, head);
         return;
     }
+    nn::launch_conv3_in2(in0c_.devPtr(), paramsBf_.devPtr() + layers_[0].w, params_.devPtr() + layers_[0].b,
+                         a0_.devPtr(), n_, true, cm[0], s);
     conv(1, a0_.devPtr(), b0_.devPtr(), cm[0], s);
     nn::launch_avgpool(b0_.devPtr(), p0_.devPtr(), n_, 16, s);
     conv(2, p0_.devPtr(), a1_.devPtr(), cm[1], s);

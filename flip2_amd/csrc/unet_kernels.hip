@@ -549,6 +549,140 @@ conv3_in2_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restrict__ W
     }
 }
 
+// The first convolution over the list of level-0 tiles with fluid (the
+// preconditioner apply): a fixed grid of workgroups loops over the list, the
+// next tile's halo (648 voxels x 2 channels, 4 bytes each) loaded into
+// registers during the current tile's MFMAs and stored into the other of two
+// LDS buffers; the weights (two A fragments per lane) are loaded once.  The
+// arithmetic is conv3_in2_kernel's, in the same order.  Tiles in `clear` are
+// zeroed first.
+constexpr int kIn2Loads = (kHalo + 255) / 256;  // 3
+__global__ void __launch_bounds__(256)
+conv3_in2_list_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restrict__ Wt,
+                      const float* __restrict__ bias, uint16_t* __restrict__ Y, int n, int relu,
+                      const uint8_t* __restrict__ vmask, TileList tl) {
+    __shared__ uint32_t sIn[2][kHalo];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const uint32_t tilesX = tl.tilesX, tilesXY = tl.tilesX * tl.tilesY;
+    const uint32_t xcd = blockIdx.x & 7u, perXcd = gridDim.x >> 3, w = blockIdx.x >> 3;
+    auto tile_origin = [&](uint32_t t, int& bx, int& by, int& bz) {
+        bx = (int)(t % tilesX) * kTX;
+        by = (int)((t % tilesXY) / tilesX) * kTY;
+        bz = (int)(t / tilesXY) * kTZ;
+    };
+    {
+        const uint32_t U = *tl.numClear;
+        const uint32_t beg = (uint32_t)(((uint64_t)U * xcd) >> 3), end = (uint32_t)(((uint64_t)U * (xcd + 1)) >> 3);
+        for (uint32_t u = beg + w; u < end; u += perXcd) {
+            int bx, by, bz;
+            tile_origin(tl.clear[u], bx, by, bz);
+            const int x = bx + (threadIdx.x & 15), y = by + ((threadIdx.x >> 4) & 3), z = bz + (threadIdx.x >> 6);
+            if (x < n && y < n && z < n) {
+                uint4* dst = reinterpret_cast<uint4*>(Y + (((long long)z * n + y) * n + x) * 16);
+                dst[0] = make_uint4(0u, 0u, 0u, 0u);
+                dst[1] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+    }
+    const uint32_t U = *tl.numCompute;
+    const uint32_t beg = (uint32_t)(((uint64_t)U * xcd) >> 3), end = (uint32_t)(((uint64_t)U * (xcd + 1)) >> 3);
+    uint32_t u = beg + w;
+    if (u >= end) return;
+    // A fragments: W[tap][co = col][ci = 0, 1] of the lane's four taps
+    v8bf a[2];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        uint32_t wq[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int tap = 16 * kb + 4 * kq + e;
+            wq[e] = tap < 27 ? *reinterpret_cast<const uint32_t*>(Wt + ((size_t)tap * 16 + col) * 16) : 0u;
+        }
+        a[kb] = __builtin_bit_cast(v8bf, make_uint4(wq[0], wq[1], wq[2], wq[3]));
+    }
+    uint32_t pre[kIn2Loads];
+    auto load_tile = [&](int bx, int by, int bz) {
+#pragma unroll
+        for (int k = 0; k < kIn2Loads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            const int hx = t % kHX, hy = (t / kHX) % kHY, hz = t / (kHX * kHY);
+            const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
+            const bool in = t < kHalo && x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n;
+            pre[k] = in ? reinterpret_cast<const uint32_t*>(X2)[((long long)z * n + y) * n + x] : 0u;
+        }
+    };
+    auto store_tile = [&](int b) {
+#pragma unroll
+        for (int k = 0; k < kIn2Loads; ++k) {
+            const int t = (int)threadIdx.x + 256 * k;
+            if (t < kHalo) sIn[b][t] = pre[k];
+        }
+    };
+    int bx, by, bz;
+    tile_origin(tl.compute[u], bx, by, bz);
+    load_tile(bx, by, bz);
+    store_tile(0);
+    __syncthreads();
+    int buf = 0;
+    for (;;) {
+        const uint32_t nu = u + perXcd;
+        const bool more = nu < end;
+        int nbx = bx, nby = by, nbz = bz;
+        if (more) {
+            tile_origin(tl.compute[nu], nbx, nby, nbz);
+            load_tile(nbx, nby, nbz);
+        }
+        v4f acc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            const int t0 = 16 * kb + 4 * kq;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint32_t q[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int tap = t0 + e;
+                    q[e] = tap < 27 ? sIn[buf][halo_index(wave, j, col, tap)] : 0u;
+                }
+                const v8bf b = __builtin_bit_cast(v8bf, make_uint4(q[0], q[1], q[2], q[3]));
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], b, acc[j], 0, 0, 0);
+            }
+        }
+        const int z = bz + wave, x = bx + col;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int y = by + j;
+            if (x >= n || y >= n || z >= n) continue;
+            const long long v = ((long long)z * n + y) * n + x;
+            const bool live = !vmask || vmask[v];
+            const int co = 4 * kq;
+            uint16_t o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float yv = acc[j][r] + bias[co + r];
+                if (relu) yv = fmaxf(yv, 0.f);
+                o[r] = live ? f2bf(yv) : (uint16_t)0;
+            }
+            uint2 packed;
+            packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+            packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+            *reinterpret_cast<uint2*>(Y + v * 16 + co) = packed;
+        }
+        if (!more) break;
+        store_tile(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+        u = nu;
+        bx = nbx;
+        by = nby;
+        bz = nbz;
+    }
+}
+
 __global__ void mask_pool_kernel(const uint8_t* __restrict__ m, int n, uint8_t* __restrict__ out) {
     const int nc = n >> 1;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1015,6 +1149,14 @@ void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, 
                       const ConvMask& cm, hipStream_t s) {
     const dim3 grid = conv_grid(n);
     hipLaunchKernelGGL(conv3_in2_kernel, grid, dim3(256), 0, s, X2, W, bias, Y, n, relu ? 1 : 0, cm);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_conv3_in2_list(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
+                           const uint8_t* vmask, const TileList& tl, hipStream_t s) {
+    // four workgroups per CU (the register budget's limit)
+    hipLaunchKernelGGL(conv3_in2_list_kernel, dim3(256 * 4), dim3(256), 0, s, X2, W, bias, Y, n, relu ? 1 : 0, vmask,
+                       tl);
     FLIP2_LAUNCH_CHECK();
 }
 

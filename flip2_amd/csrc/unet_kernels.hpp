@@ -49,6 +49,10 @@ void launch_conv3_fwd_upcat(const uint16_t* Xskip, const uint16_t* Xcoarse, cons
 // layer's [27][16][16] weights (input channels 2..15 unused).
 void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
                       const ConvMask& mask, hipStream_t s);
+// The same over the list of tiles with fluid (tl), zeroing the tiles in tl's
+// clear list; vmask: the level's voxel mask (outputs outside it are zero).
+void launch_conv3_in2_list(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
+                           const uint8_t* vmask, const TileList& tl, hipStream_t s);
 // 16 -> 16 convolution with the output head in its epilogue (Y may be null).
 void launch_conv3_fwd_head(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin,
                            bool relu, const HeadArgs& head, const ConvMask& mask, hipStream_t s);
