@@ -178,14 +178,12 @@ __device__ inline void conv_stage_mfma(const uint4 (*sIn)[kHalo], const uint16_t
     }
 }
 
-// The same stage with the weights of one 16-channel output block in LDS,
-// sW[(tap * 2 + chunk) * 16 + cout] = 8 input channels (16 bytes): the A
-// fragments are LDS reads like the B fragments, so the MFMAs wait on LDS only
-// and never on the global loads of a prefetch in flight.
-constexpr int kWStage = 27 * 2 * 16;                         // 16-byte weight fragments per stage (864)
-constexpr int kWLoads = (kWStage + 255) / 256;               // per thread (4)
-__device__ inline void conv_stage_lds(const uint4 (*sIn)[kHalo], const uint4* sW, int wave, int col, int kq,
-                                      v4f (&acc)[4]) {
+// The same stage for one 16-channel output block, with the A fragment of
+// (pair q, dy) supplied by getA(q, dy): from LDS when the weights change
+// between steps, from registers when they do not.
+template <class GetA>
+__device__ inline void conv_stage_rows(const uint4 (*sIn)[kHalo], GetA getA, int wave, int col, int kq,
+                                       v4f (&acc)[4]) {
     const int chunk = kq & 1, half = kq >> 1;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -200,22 +198,64 @@ __device__ inline void conv_stage_lds(const uint4 (*sIn)[kHalo], const uint4* sW
         }
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy) {
-            const int tap = dx + 3 * dy + 9 * dz;
-            const uint4 raw = live ? sW[(tap * 2 + chunk) * 16 + col] : make_uint4(0u, 0u, 0u, 0u);
-            const v8bf a = __builtin_bit_cast(v8bf, raw);
+            const v8bf a = getA(q, dy);
 #pragma unroll
             for (int j = 0; j < kTY; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j + dy], acc[j], 0, 0, 0);
         }
     }
 }
 
+// Weights of one input stage and 16-channel output block in LDS:
+// sW[(tap * 2 + chunk) * 16 + cout] = 8 input channels (16 bytes); A
+// fragments read from there make the MFMAs wait on LDS only, never on the
+// global loads of a prefetch in flight.
+constexpr int kWStage = 27 * 2 * 16;            // 16-byte weight fragments per stage (864)
+constexpr int kWLoads = (kWStage + 255) / 256;  // per thread (4)
+
+// Tap of lane half `half` in pair q at row offset dy (27 = none).
+__device__ inline int pair_tap(int q, int dy, int half) {
+    const int sc = 2 * q + half;
+    return sc < 9 ? (sc % 3) + 3 * dy + 9 * (sc / 3) : 27;
+}
+
+// Inputs of the epilogue (MODE 0) for lane (col, kq): the bias of its
+// channels and the masks of its four voxels (bx + col, by + j, bz + wave).
+// Loaded before the MFMAs, so their latency overlaps them and a prefetch
+// issued later is not waited for.
+template <int MB>
+struct EpiIn {
+    float bias[MB][4];
+    bool live[4];      // inside the grid and in the voxel mask
+    bool headLive[4];  // head.mask of the voxel (fused head only)
+};
+
+template <int COUT, int MB>
+__device__ inline EpiIn<MB> epi_load(const float* __restrict__ bias, const uint8_t* __restrict__ vmask,
+                                     const HeadArgs& head, int n, int bx, int by, int bz, int m0, int wave, int col,
+                                     int kq) {
+    EpiIn<MB> e;
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) e.bias[m][r] = bias[16 * (m0 + m) + 4 * kq + r];
+    const int z = bz + wave, x = bx + col;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int y = by + j;
+        const bool inside = x < n && y < n && z < n;
+        const long long v = ((long long)z * n + y) * n + x;
+        e.live[j] = inside && (!vmask || vmask[v]);
+        e.headLive[j] = COUT == 16 && head.out && inside && head.mask[v];
+    }
+    return e;
+}
+
 // Epilogue: lane (col, kq) holds output channels 16 (m0 + m) + 4 kq .. +3 of
 // voxel (bx + col, by + j, bz + wave).
 template <int COUT, int MODE, int MB>
-__device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const float* __restrict__ bias, uint16_t* __restrict__ Y,
-                                     float* __restrict__ Yf, int n, int relu, const HeadArgs& head,
-                                     const uint8_t* __restrict__ vmask, int bx, int by, int bz, int m0, int wave,
-                                     int col, int kq) {
+__device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e, uint16_t* __restrict__ Y,
+                                     float* __restrict__ Yf, int n, int relu, const HeadArgs& head, int bx, int by,
+                                     int bz, int m0, int wave, int col, int kq) {
     const int z = bz + wave, x = bx + col;
     if (MODE == 0 && COUT == 16 && head.out) {
         // fused head: gather the 16 channels of each voxel in its kq == 0 lane
@@ -224,14 +264,13 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const float* __res
             const int y = by + j;
             const bool inside = x < n && y < n && z < n;
             const long long v = ((long long)z * n + y) * n + x;
-            const bool live = inside && (!vmask || vmask[v]);
             float yv[4];
             uint16_t o[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float t = acc[0][j][r] + bias[4 * kq + r];
+                float t = acc[0][j][r] + e.bias[0][r];
                 if (relu) t = fmaxf(t, 0.f);
-                o[r] = live ? f2bf(t) : (uint16_t)0;
+                o[r] = e.live[j] ? f2bf(t) : (uint16_t)0;
                 yv[r] = bf2f(o[r]);
             }
             if (Y && inside) {
@@ -251,7 +290,7 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const float* __res
                 float s = head.b[0];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) s = fmaf(ch[c], head.w[c], s);
-                head.out[v] = head.mask[v] ? s * head.scale[0] : 0.f;
+                head.out[v] = e.headLive[j] ? s * head.scale[0] : 0.f;
             }
         }
         return;
@@ -261,7 +300,6 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const float* __res
         const int y = by + j;
         if (x >= n || y >= n || z >= n) continue;
         const long long v = ((long long)z * n + y) * n + x;
-        const bool live = MODE == 1 || !vmask || vmask[v];
 #pragma unroll
         for (int m = 0; m < MB; ++m) {
             const int co = 16 * (m0 + m) + 4 * kq;
@@ -273,9 +311,9 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const float* __res
             uint16_t o[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float yv = acc[m][j][r] + bias[co + r];
+                float yv = acc[m][j][r] + e.bias[m][r];
                 if (relu) yv = fmaxf(yv, 0.f);
-                o[r] = live ? f2bf(yv) : (uint16_t)0;
+                o[r] = e.live[j] ? f2bf(yv) : (uint16_t)0;
             }
             uint2 packed;
             packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
@@ -303,6 +341,8 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
     if (skip_tile<COUT, SPLIT>(cm, bx, by, bz, mg, n, MODE == 0 ? Y : nullptr, MODE == 0 ? head.out : nullptr))
         return;
     const int m0 = mg * MB;  // first output block of this workgroup
+    EpiIn<MB> epi{};
+    if (MODE == 0) epi = epi_load<COUT, MB>(bias, cm.voxel, head, n, bx, by, bz, m0, wave, col, kq);
 
     v4f acc[MB][4];
 #pragma unroll
@@ -334,7 +374,7 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
         if (c0 + 16 < CIN) load_stage(c0 + 16);
         conv_stage_mfma<CIN, COUT, MB>(sIn, Wt, c0, m0, wave, col, kq, acc);
     }
-    conv_epilogue<COUT, MODE, MB>(acc, bias, Y, Yf, n, relu, head, cm.voxel, bx, by, bz, m0, wave, col, kq);
+    conv_epilogue<COUT, MODE, MB>(acc, epi, Y, Yf, n, relu, head, bx, by, bz, m0, wave, col, kq);
 }
 
 // The same convolution over a list of tiles (the tiles of a level that hold
@@ -354,8 +394,12 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
     constexpr int MB = COUT / 16 / SPLIT;
     constexpr int STAGES = CIN / 16;
     static_assert(MB == 1 && 16 * SPLIT == COUT, "one 16-channel output block per workgroup");
+    // one input stage and one channel group: the weights are the same at
+    // every step and stay in registers
+    constexpr bool kFixedW = STAGES == 1 && SPLIT == 1;
+    constexpr int kWBuf = kFixedW ? 1 : 2;
     __shared__ __align__(16) uint4 sIn[2][2][kHalo];  // [buffer][8-channel chunk][halo voxel]
-    __shared__ __align__(16) uint4 sW[2][kWStage];    // [buffer][(tap, chunk, cout)]
+    __shared__ __align__(16) uint4 sW[kWBuf][kFixedW ? 1 : kWStage];  // [buffer][(tap, chunk, cout)]
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
@@ -405,6 +449,7 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
             const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, bx, by, bz, 16 * stage, t) : nullptr;
             pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
         }
+        if (kFixedW) return;
         const int co0 = (int)(unit % SPLIT) * 16;
 #pragma unroll
         for (int k = 0; k < kWLoads; ++k) {
@@ -422,21 +467,36 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
             const int t = (int)threadIdx.x + 256 * k;
             if (t < kHalo * 2) sIn[b][t & 1][t >> 1] = pre[k];
         }
+        if (kFixedW) return;
 #pragma unroll
         for (int k = 0; k < kWLoads; ++k) {
             const int t = (int)threadIdx.x + 256 * k;
             if (t < kWStage) sW[b][t] = preW[k];
         }
     };
+    v8bf aw[kFixedW ? 15 : 1];
+    if (kFixedW) {
+        const int chunk = kq & 1, half = kq >> 1;
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int tap = pair_tap(q, dy, half);
+                const uint4 raw = tap < 27 ? *reinterpret_cast<const uint4*>(Wt + (size_t)(tap * COUT + col) * CIN +
+                                                                             8 * chunk)
+                                           : make_uint4(0u, 0u, 0u, 0u);
+                aw[kFixedW ? q * 3 + dy : 0] = __builtin_bit_cast(v8bf, raw);
+            }
+    }
     int bx, by, bz;
     unit_origin(u, bx, by, bz);
     load_step(u, bx, by, bz, 0);
     store_step(0);
     __syncthreads();
     int buf = 0, stage = 0;
-    v4f acc[4];
+    v4f acc[1][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) acc[0][j] = v4f{0.f, 0.f, 0.f, 0.f};
     for (;;) {
         // next step
         uint32_t nu = u;
@@ -446,18 +506,32 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
             nu = u + perXcd;
         }
         const bool more = nu < end;
+        const int m0 = (int)(u % SPLIT);
+        EpiIn<1> epi{};
+        if (stage == STAGES - 1) epi = epi_load<COUT, 1>(bias, vmask, head, n, bx, by, bz, m0, wave, col, kq);
         int nbx = bx, nby = by, nbz = bz;
         if (more) {
             if (nu != u) unit_origin(nu, nbx, nby, nbz);
             load_step(nu, nbx, nby, nbz, ns);
         }
-        conv_stage_lds(sIn[buf], sW[buf], wave, col, kq, acc);
+        if constexpr (kFixedW) {
+            conv_stage_rows(sIn[buf], [&](int q, int dy) { return aw[q * 3 + dy]; }, wave, col, kq, acc[0]);
+        } else {
+            const uint4* w = sW[buf];
+            const int chunk = kq & 1, half = kq >> 1;
+            conv_stage_rows(
+                sIn[buf],
+                [&](int q, int dy) {
+                    const int tap = pair_tap(q, dy, half);
+                    return __builtin_bit_cast(v8bf, tap < 27 ? w[(tap * 2 + chunk) * 16 + col]
+                                                             : make_uint4(0u, 0u, 0u, 0u));
+                },
+                wave, col, kq, acc[0]);
+        }
         if (stage == STAGES - 1) {
-            const int m0 = (int)(u % SPLIT);
-            v4f (&accm)[1][4] = *reinterpret_cast<v4f(*)[1][4]>(&acc);
-            conv_epilogue<COUT, 0, 1>(accm, bias, Y, nullptr, n, relu, head, vmask, bx, by, bz, m0, wave, col, kq);
+            conv_epilogue<COUT, 0, 1>(acc, epi, Y, nullptr, n, relu, head, bx, by, bz, m0, wave, col, kq);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < 4; ++j) acc[0][j] = v4f{0.f, 0.f, 0.f, 0.f};
         }
         if (!more) break;
         store_step(buf ^ 1);
@@ -629,6 +703,8 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restric
     for (;;) {
         const uint32_t nu = u + perXcd;
         const bool more = nu < end;
+        // this tile's voxel masks before the next tile's loads
+        const EpiIn<1> epi = epi_load<16, 1>(bias, vmask, HeadArgs{}, n, bx, by, bz, 0, wave, col, kq);
         int nbx = bx, nby = by, nbz = bz;
         if (more) {
             tile_origin(tl.compute[nu], nbx, nby, nbz);
@@ -658,19 +734,17 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restric
             const int y = by + j;
             if (x >= n || y >= n || z >= n) continue;
             const long long v = ((long long)z * n + y) * n + x;
-            const bool live = !vmask || vmask[v];
-            const int co = 4 * kq;
             uint16_t o[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float yv = acc[j][r] + bias[co + r];
+                float yv = acc[j][r] + epi.bias[0][r];
                 if (relu) yv = fmaxf(yv, 0.f);
-                o[r] = live ? f2bf(yv) : (uint16_t)0;
+                o[r] = epi.live[j] ? f2bf(yv) : (uint16_t)0;
             }
             uint2 packed;
             packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
             packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-            *reinterpret_cast<uint2*>(Y + v * 16 + co) = packed;
+            *reinterpret_cast<uint2*>(Y + v * 16 + 4 * kq) = packed;
         }
         if (!more) break;
         store_tile(buf ^ 1);
