@@ -549,25 +549,26 @@ void UNet::forward(hipStream_t s, bool keepActivations) {
                                   a0_.devPtr(), n_, true, cm[0].voxel, tl_[0], s);
         const int levelN[3] = {n_, n1_, n2_};
         const int split[3] = {1, 2, 4};
-        auto lconv = [&](int layer, const uint16_t* X, const uint16_t* Xc, int ca, uint16_t* Y, const nn::HeadArgs& h) {
+        auto lconv = [&](int layer, const uint16_t* X, const uint16_t* Xc, int ca, uint16_t* Y, uint16_t* pooled,
+                         const nn::HeadArgs& h) {
             const UNetLayer& L = layers_[layer];
-            nn::launch_conv3_list(X, Xc, paramsBf_.devPtr() + L.w, params_.devPtr() + L.b, Y, levelN[L.level], L.cin,
-                                  ca, L.cout, split[L.level], true, h, cm[L.level].voxel, tl_[L.level], s);
+            nn::launch_conv3_list(X, Xc, paramsBf_.devPtr() + L.w, params_.devPtr() + L.b, Y, pooled, levelN[L.level],
+                                  L.cin, ca, L.cout, split[L.level], true, h, cm[L.level].voxel, tl_[L.level], s);
         };
         const nn::HeadArgs none{};
-        lconv(1, a0_.devPtr(), nullptr, 0, b0_.devPtr(), none);
-        nn::launch_avgpool_list(b0_.devPtr(), p0_.devPtr(), n_, 16, tl_[1], s);
-        lconv(2, p0_.devPtr(), nullptr, 0, a1_.devPtr(), none);
-        lconv(3, a1_.devPtr(), nullptr, 0, b1_.devPtr(), none);
-        nn::launch_avgpool_list(b1_.devPtr(), p1_.devPtr(), n1_, 32, tl_[2], s);
-        lconv(4, p1_.devPtr(), nullptr, 0, a2_.devPtr(), none);
-        lconv(5, a2_.devPtr(), nullptr, 0, b2_.devPtr(), none);
-        lconv(6, b1_.devPtr(), b2_.devPtr(), 64, c1_.devPtr(), none);
-        lconv(7, c1_.devPtr(), nullptr, 0, d1_.devPtr(), none);
-        lconv(8, b0_.devPtr(), d1_.devPtr(), 32, c0_.devPtr(), none);
+        // the encoder's second convolution on levels 0 and 1 also writes the
+        // pooled input of the next level
+        lconv(1, a0_.devPtr(), nullptr, 0, b0_.devPtr(), p0_.devPtr(), none);
+        lconv(2, p0_.devPtr(), nullptr, 0, a1_.devPtr(), nullptr, none);
+        lconv(3, a1_.devPtr(), nullptr, 0, b1_.devPtr(), p1_.devPtr(), none);
+        lconv(4, p1_.devPtr(), nullptr, 0, a2_.devPtr(), nullptr, none);
+        lconv(5, a2_.devPtr(), nullptr, 0, b2_.devPtr(), nullptr, none);
+        lconv(6, b1_.devPtr(), b2_.devPtr(), 64, c1_.devPtr(), nullptr, none);
+        lconv(7, c1_.devPtr(), nullptr, 0, d1_.devPtr(), nullptr, none);
+        lconv(8, b0_.devPtr(), d1_.devPtr(), 32, c0_.devPtr(), nullptr, none);
         const nn::HeadArgs head{params_.devPtr() + headW_, params_.devPtr() + headB_, mask_.devPtr(), scale_.devPtr(),
                                 out_.devPtr()};
-        lconv(9, c0_.devPtr(), nullptr, 0, nullptr, head);
+        lconv(9, c0_.devPtr(), nullptr, 0, nullptr, nullptr, head);
         return;
     }
     nn::launch_conv3_in2(in0c_.devPtr(), paramsBf_.devPtr() + layers_[0].w, params_.devPtr() + layers_[0].b,

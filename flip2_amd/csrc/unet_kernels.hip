@@ -377,6 +377,59 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
     conv_epilogue<COUT, MODE, MB>(acc, epi, Y, Yf, n, relu, head, bx, by, bz, m0, wave, col, kq);
 }
 
+// 2x2x2 average pooling fused into a list convolution's epilogue: the
+// tile's 16 x 4 x 4 output voxels are the children of 8 x 2 x 2 voxels of the
+// next level, whose pooled values (the BF16 outputs averaged, children summed
+// in avgpool_kernel's order) go to P ([voxel][PC] BF16, channels c0 ..
+// c0 + 15).  Each pooled voxel belongs to one tile, so the pooled tensor is
+// complete once every computed tile has pooled and every cleared tile has
+// zeroed its part.  Contains a workgroup barrier.
+__device__ inline void pool_tile(const v4f (&acc)[4], const EpiIn<1>& e, int relu, uint16_t (*sPool)[kTY][kTX][16],
+                                 uint16_t* __restrict__ P, int n, int PC, int c0, int bx, int by, int bz, int wave,
+                                 int col, int kq) {
+#pragma unroll
+    for (int j = 0; j < kTY; ++j) {
+        uint16_t o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float t = acc[j][r] + e.bias[0][r];
+            if (relu) t = fmaxf(t, 0.f);
+            o[r] = e.live[j] ? f2bf(t) : (uint16_t)0;
+        }
+        uint2 packed;
+        packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+        packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+        *reinterpret_cast<uint2*>(&sPool[wave][j][col][4 * kq]) = packed;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int cv = (int)threadIdx.x >> 1, h = (int)threadIdx.x & 1;  // coarse voxel of the tile, channel half
+        const int cx = cv & 7, cy = (cv >> 3) & 1, cz = cv >> 4;
+        const int nc = n >> 1;
+        const int X = (bx >> 1) + cx, Yc = (by >> 1) + cy, Z = (bz >> 1) + cz;
+        if (X < nc && Yc < nc && Z < nc) {
+            float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint4 v =
+                    *reinterpret_cast<const uint4*>(&sPool[2 * cz + (k >> 2)][2 * cy + ((k >> 1) & 1)][2 * cx + (k & 1)][8 * h]);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    sum[2 * q] += bf2f((uint16_t)(w[q] & 0xffffu));
+                    sum[2 * q + 1] += bf2f((uint16_t)(w[q] >> 16));
+                }
+            }
+            uint32_t o[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                o[q] = (uint32_t)f2bf(sum[2 * q] * 0.125f) | ((uint32_t)f2bf(sum[2 * q + 1] * 0.125f) << 16);
+            *reinterpret_cast<uint4*>(P + (((long long)Z * nc + Yc) * nc + X) * PC + c0 + 8 * h) =
+                make_uint4(o[0], o[1], o[2], o[3]);
+        }
+    }
+}
+
 // The same convolution over a list of tiles (the tiles of a level that hold
 // masked-in voxels), for one 16-channel output block per workgroup: a fixed
 // grid of workgroups loops over the list, one (tile, channel group, input
@@ -385,12 +438,14 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
 // two LDS buffers after them, so memory latency overlaps compute.  Each XCD
 // takes one contiguous part of the list (neighbouring tiles, shared halo rows
 // in its L2).  Tiles in `clear` (computed by an earlier forward, not by this
-// one) are zeroed first.
+// one) are zeroed first.  P != null: the output is also average-pooled into
+// P (the next level's input, [voxel][COUT] at half resolution; see
+// pool_tile).
 template <int CIN, int COUT, int CA = 0, int SPLIT = 1>
 __global__ void __launch_bounds__(256)
 conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc, const uint16_t* __restrict__ Wt,
-                  const float* __restrict__ bias, uint16_t* __restrict__ Y, int n, int relu, HeadArgs head,
-                  const uint8_t* __restrict__ vmask, TileList tl) {
+                  const float* __restrict__ bias, uint16_t* __restrict__ Y, uint16_t* __restrict__ P, int n, int relu,
+                  HeadArgs head, const uint8_t* __restrict__ vmask, TileList tl) {
     constexpr int MB = COUT / 16 / SPLIT;
     constexpr int STAGES = CIN / 16;
     static_assert(MB == 1 && 16 * SPLIT == COUT, "one 16-channel output block per workgroup");
@@ -400,6 +455,7 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
     constexpr int kWBuf = kFixedW ? 1 : 2;
     __shared__ __align__(16) uint4 sIn[2][2][kHalo];  // [buffer][8-channel chunk][halo voxel]
     __shared__ __align__(16) uint4 sW[kWBuf][kFixedW ? 1 : kWStage];  // [buffer][(tap, chunk, cout)]
+    __shared__ __align__(16) uint16_t sPool[kTZ][kTY][kTX][16];       // outputs to pool
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
@@ -426,6 +482,14 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
                     dst[1] = make_uint4(0u, 0u, 0u, 0u);
                 }
                 if (head.out && mg == 0) head.out[v] = 0.f;
+            }
+            if (P && threadIdx.x < 64) {
+                const int cv = (int)threadIdx.x >> 1, h = (int)threadIdx.x & 1;
+                const int nc = n >> 1;
+                const int X2 = (bx >> 1) + (cv & 7), Y2 = (by >> 1) + ((cv >> 3) & 1), Z2 = (bz >> 1) + (cv >> 4);
+                if (X2 < nc && Y2 < nc && Z2 < nc)
+                    *reinterpret_cast<uint4*>(P + (((long long)Z2 * nc + Y2) * nc + X2) * COUT + mg * 16 + 8 * h) =
+                        make_uint4(0u, 0u, 0u, 0u);
             }
         }
     }
@@ -530,6 +594,7 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
         }
         if (stage == STAGES - 1) {
             conv_epilogue<COUT, 0, 1>(acc, epi, Y, nullptr, n, relu, head, bx, by, bz, m0, wave, col, kq);
+            if (P) pool_tile(acc[0], epi, relu, sPool, P, n, COUT, 16 * m0, bx, by, bz, wave, col, kq);
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[0][j] = v4f{0.f, 0.f, 0.f, 0.f};
         }
@@ -987,51 +1052,6 @@ __global__ void avgpool_kernel(const uint16_t* __restrict__ X, uint16_t* __restr
     P[i] = f2bf(s * 0.125f);
 }
 
-// avgpool over a tile list of the coarse level (the tiles with fluid; tiles
-// in the clear list are zeroed): thread t of a workgroup owns coarse voxel t
-// of a 16x4x4 tile, 8 channels per 16-byte access, children summed in the
-// dense kernel's order (same values).
-template <int C>
-__global__ void __launch_bounds__(256)
-avgpool_list_kernel(const uint16_t* __restrict__ X, uint16_t* __restrict__ P, int n, TileList tl) {
-    const int nc = n / 2;
-    const uint32_t tilesX = tl.tilesX, tilesXY = tl.tilesX * tl.tilesY;
-    const uint32_t nClear = *tl.numClear, nComp = *tl.numCompute;
-    for (uint32_t u = blockIdx.x; u < nClear + nComp; u += gridDim.x) {
-        const bool clearing = u < nClear;
-        const uint32_t t = clearing ? tl.clear[u] : tl.compute[u - nClear];
-        const int x = (int)(t % tilesX) * kTX + (int)(threadIdx.x & 15);
-        const int y = (int)((t % tilesXY) / tilesX) * kTY + (int)((threadIdx.x >> 4) & 3);
-        const int z = (int)(t / tilesXY) * kTZ + (int)(threadIdx.x >> 6);
-        if (x >= nc || y >= nc || z >= nc) continue;
-        uint4* dst = reinterpret_cast<uint4*>(P + (((long long)z * nc + y) * nc + x) * C);
-#pragma unroll
-        for (int q = 0; q < C / 8; ++q) {
-            if (clearing) {
-                dst[q] = make_uint4(0u, 0u, 0u, 0u);
-                continue;
-            }
-            float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int xf = 2 * x + (k & 1), yf = 2 * y + ((k >> 1) & 1), zf = 2 * z + (k >> 2);
-                const uint4 v = reinterpret_cast<const uint4*>(X + (((long long)zf * n + yf) * n + xf) * C)[q];
-                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    s[2 * e] += bf2f((uint16_t)(w[e] & 0xffffu));
-                    s[2 * e + 1] += bf2f((uint16_t)(w[e] >> 16));
-                }
-            }
-            uint32_t o[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                o[e] = (uint32_t)f2bf(s[2 * e] * 0.125f) | ((uint32_t)f2bf(s[2 * e + 1] * 0.125f) << 16);
-            dst[q] = make_uint4(o[0], o[1], o[2], o[3]);
-        }
-    }
-}
-
 __global__ void avgpool_bwd_kernel(const float* __restrict__ dP, float* __restrict__ dX, int n, int C) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)n * n * n * C;
@@ -1247,14 +1267,15 @@ void conv_tile_dims(int n, uint32_t& tilesX, uint32_t& tilesY) {
 }
 
 void launch_conv3_list(const uint16_t* X, const uint16_t* Xcoarse, const uint16_t* W, const float* bias, uint16_t* Y,
-                       int n, int cin, int ca, int cout, int split, bool relu, const HeadArgs& head,
+                       uint16_t* pooled, int n, int cin, int ca, int cout, int split, bool relu, const HeadArgs& head,
                        const uint8_t* vmask, const TileList& tl, hipStream_t s) {
+    FLIP2_REQUIRE(!pooled || (n % 2 == 0 && !head.out), kErrArgument, "pooled list convolution: even n, no head");
     // two workgroups per CU fit the double-buffered halo and weights (69 KB each)
     const dim3 grid(256 * 2), block(256);
     const int r = relu ? 1 : 0;
 #define LIST_CASE(CI, CA_, CO, SP)                                                                               \
     if (cin == CI && ca == CA_ && cout == CO && split == SP) {                                                   \
-        hipLaunchKernelGGL((conv3_list_kernel<CI, CO, CA_, SP>), grid, block, 0, s, X, Xcoarse, W, bias, Y, n, r, \
+        hipLaunchKernelGGL((conv3_list_kernel<CI, CO, CA_, SP>), grid, block, 0, s, X, Xcoarse, W, bias, Y, pooled, n, r, \
                            head, vmask, tl);                                                                      \
         FLIP2_LAUNCH_CHECK();                                                                                    \
         return;                                                                                                  \
@@ -1364,12 +1385,6 @@ void launch_relu_mask_bf16(float* g, const uint16_t* act, uint16_t* gb, long lon
 void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s) {
     const long long total = (long long)(n / 2) * (n / 2) * (n / 2) * C;
     hipLaunchKernelGGL(avgpool_kernel, g1(total, 256), dim3(256), 0, s, X, P, n, C);
-    FLIP2_LAUNCH_CHECK();
-}
-void launch_avgpool_list(const uint16_t* X, uint16_t* P, int n, int C, const TileList& coarse, hipStream_t s) {
-    if (C == 16) hipLaunchKernelGGL(avgpool_list_kernel<16>, dim3(1024), dim3(256), 0, s, X, P, n, coarse);
-    else if (C == 32) hipLaunchKernelGGL(avgpool_list_kernel<32>, dim3(1024), dim3(256), 0, s, X, P, n, coarse);
-    else throw Error(kErrArgument, "avgpool over a tile list: 16 or 32 channels");
     FLIP2_LAUNCH_CHECK();
 }
 void launch_avgpool_bwd(const float* dP, float* dX, int n, int C, hipStream_t s) {

@@ -60,8 +60,10 @@ void launch_conv3_fwd_head(const uint16_t* X, const uint16_t* W, const float* bi
 // tl.compute and zeroes those in tl.clear; vmask as ConvMask::voxel.  ca > 0:
 // input = concat(up2(Xcoarse) [ca], X [cin - ca]).  split: workgroups per
 // tile (output channels divided among them).  head as launch_conv3_fwd_head.
+// pooled != null: Y is also 2x2x2 average-pooled into pooled ([voxel][cout]
+// at n / 2), the same values as launch_avgpool on the whole of Y.
 void launch_conv3_list(const uint16_t* X, const uint16_t* Xcoarse, const uint16_t* W, const float* bias, uint16_t* Y,
-                       int n, int cin, int ca, int cout, int split, bool relu, const HeadArgs& head,
+                       uint16_t* pooled, int n, int cin, int ca, int cout, int split, bool relu, const HeadArgs& head,
                        const uint8_t* vmask, const TileList& tl, hipStream_t s);
 // Lists of the tiles with cur set (compute) and with cur clear but prev set
 // (clear); prev may be null.  counts[0], counts[1] receive their lengths.
@@ -83,9 +85,6 @@ void launch_conv3_wgrad(const uint16_t* dPb, const float* dP, const uint16_t* X,
 // g <- g * (act > 0) in place; gb = BF16 copy.
 void launch_relu_mask_bf16(float* g, const uint16_t* act, uint16_t* gb, long long count, hipStream_t s);
 void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s);
-// The same over the tile list of the coarse level (n/2)^3: computes its
-// compute tiles, zeroes its clear tiles.
-void launch_avgpool_list(const uint16_t* X, uint16_t* P, int n, int C, const TileList& coarse, hipStream_t s);
 void launch_avgpool_bwd(const float* dP, float* dX, int n, int C, hipStream_t s);  // dX += ...
 void launch_upcat(const uint16_t* A, const uint16_t* B, uint16_t* U, int n, int Ca, int Cb, hipStream_t s);
 void launch_upcat_bwd(const float* dU, float* dA, float* dB, int n, int Ca, int Cb, hipStream_t s);  // +=
