@@ -213,7 +213,7 @@ def learned_run(args):
     import flip2_amd as F
     from flip2_amd import scenes
 
-    its = [0, 2, 4, 8, 16, 32, 64, 128]
+    its = [int(v) for v in args.capture_its.split(",")]
     if args.config == "c4":
         n = args.grid_c4
         (x, y, z), solid = scenes.dam_break_sphere(n, seed=5)
@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--grid-c5", type=int, default=256)
     ap.add_argument("--frames", type=int, default=200, help="c4: frames of Jacobi-PCG solves captured for training")
     ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--capture-its", default="0,2,4,8,16,32,64,128",
+                    help="c4: inner iterations of each Jacobi-PCG solve whose residuals become training frames")
     ap.add_argument("--compare-frames", type=int, default=8)
     ap.add_argument("--fallback", type=int, default=0, help="learned: Jacobi-PCG fallback iteration cap (0: none)")
     ap.add_argument("--flex", type=int, default=8, help="learned: earlier directions flexible CG keeps (1..16)")

@@ -201,6 +201,7 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_unet_eval": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_f64)],
         "flip2_fluid_set_unet": [_vp, _vp, _u32],
         "flip2_fluid_set_flex_directions": [_vp, _u32],
+        "flip2_fluid_set_flex_check_every": [_vp, _u32],
         "flip2_fluid_set_learned_graph": [_vp, ctypes.c_int],
     }
     for name in ("align_particles_to_grid", "sort_particles", "generate_voxels", "particle_vel_to_voxels",
@@ -360,6 +361,10 @@ class P2GPipeline:
     def run_records(self, n: int, records: DeviceBuffer, out: Sequence[DeviceBuffer], stream=None):
         """records: device particle records (8 floats each); out: device U, V, W."""
         check(lib().flip2_p2g_pipeline_run_records(self._h, int(n), records.ptr, *(b.ptr for b in out), stream))
+
+    def set_flex_check_every(self, iterations: int) -> None:
+        """Iterations (1..1024) between residual checks in flexible CG."""
+        check(lib().flip2_fluid_set_flex_check_every(self._h, int(iterations)))
 
     def get(self, name: str) -> np.ndarray:
         cnt = _sz()

@@ -639,6 +639,7 @@ int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg) {
         c.learnedFallbackIterations = f->impl.config().learnedFallbackIterations;
         c.learnedSmoothing = f->impl.config().learnedSmoothing;
         c.learnedDirections = f->impl.config().learnedDirections;
+        c.learnedCheckEvery = f->impl.config().learnedCheckEvery;
         c.learnedGraph = f->impl.config().learnedGraph;
         f->impl.setConfig(c);
     });
@@ -873,6 +874,16 @@ int flip2_fluid_set_flex_directions(flip2_fluid* f, uint32_t directions) {
                       "flexible CG keeps 1 to 16 directions");
         flip2::FluidConfig c = f->impl.config();
         c.learnedDirections = directions;
+        f->impl.setConfig(c);
+    });
+}
+int flip2_fluid_set_flex_check_every(flip2_fluid* f, uint32_t iterations) {
+    return guarded([&] {
+        NEED(f);
+        FLIP2_REQUIRE(iterations >= 1 && iterations <= 1024, flip2::kErrArgument,
+                      "flexible CG checks its residual every 1 to 1024 iterations");
+        flip2::FluidConfig c = f->impl.config();
+        c.learnedCheckEvery = iterations;
         f->impl.setConfig(c);
     });
 }

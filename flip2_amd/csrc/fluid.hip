@@ -307,7 +307,8 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     double rel = 1.0;
     uint32_t total = 0;
     const uint32_t cap = std::min(capIn, cfg_.maxIterations);
-    std::vector<double> h(4 * (cfg_.checkEvery + 1));
+    const uint32_t every = std::max(1u, flex ? cfg_.learnedCheckEvery : cfg_.checkEvery);
+    std::vector<double> h(4 * (every + 1));
     double* H = hist_.devPtr();
     for (uint32_t refine = 0; refine < 12 && total < cap; ++refine) {
         const double innerTol = std::max(0.5 * cfg_.tolerance / rel, 2e-5);
@@ -330,7 +331,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
         uint32_t it = 0;
         bool innerDone = false;
         while (!innerDone && total + it < cap) {
-            const uint32_t batch = std::min(cfg_.checkEvery, cap - total - it);
+            const uint32_t batch = std::min(every, cap - total - it);
             const uint32_t first = it;
             for (uint32_t k = 0; k < batch; ++k, ++it) {
                 double* hk = H + 4u * it;

@@ -38,6 +38,10 @@ struct FluidConfig {
     // Flexible CG keeps this many earlier directions for the learned
     // preconditioner (1 = FCG(1); at most ns::kMaxFlexDirections).
     uint32_t learnedDirections = 8;
+    // Iterations between host residual checks in flexible CG (its
+    // iterations are costly: checking after each one stops at the first
+    // that converges, where checkEvery's batches would overshoot).
+    uint32_t learnedCheckEvery = 1;
     // Replay the learned preconditioner's apply from a HIP graph captured
     // once per solve.
     bool learnedGraph = true;

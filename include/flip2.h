@@ -332,6 +332,10 @@ int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iter
  * preconditioner is used (1 is FCG(1)).  New in this library; the reference
  * solver is red-black SOR (engine/algorithms/voxelSolveFunctions.cu:275-412). */
 int flip2_fluid_set_flex_directions(flip2_fluid* f, uint32_t directions);
+/* Iterations (1..1024, default 1) between the host's residual checks in
+ * flexible CG; the other solvers check every config.check_every
+ * iterations.  No counterpart in the reference. */
+int flip2_fluid_set_flex_check_every(flip2_fluid* f, uint32_t iterations);
 /* enable != 0 (default): the learned preconditioner's kernels are captured
  * once per solve into a HIP graph and replayed; 0 launches them one by one.
  * Same results either way.  No counterpart in the reference. */
