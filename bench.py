@@ -224,16 +224,21 @@ def learned_run(args):
         net = F.UNet(n, seed=2)
         print(f"[bench] c4: {x.size} particles; capturing {args.frames} frames", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        net.capture_residuals(sim, its)
+        if not args.load_net:
+            net.capture_residuals(sim, its)
         for _ in range(args.frames):
             sim.step()
         net.capture_residuals(sim, [])
         nf = net.num_frames()
         capture_s = time.perf_counter() - t0
-        print(f"[bench] training {args.epochs} epochs on {nf} frames", file=sys.stderr, flush=True)
+        if args.load_net:
+            # a network trained by an earlier run of this configuration
+            net.set_params(np.load(args.load_net, allow_pickle=False))
+        epochs = 0 if args.load_net else args.epochs
+        print(f"[bench] training {epochs} epochs on {nf} frames", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         losses = []
-        for e in range(args.epochs):
+        for e in range(epochs):
             # learning rate from 1e-3 down to 3e-4, geometrically over the epochs
             lr = 1e-3 * 0.3 ** (e / max(1, args.epochs - 1))
             losses.append(net.train(1, lr, 0, nf, seed=e))
@@ -241,8 +246,10 @@ def learned_run(args):
         train_s = time.perf_counter() - t0
         if args.save_net:
             np.save(args.save_net, net.params(), allow_pickle=False)
+        trained = (f"network from {args.load_net}" if args.load_net else
+                   f"trained on {nf} residual frames of {args.frames} Jacobi-PCG solves ({args.epochs} epochs)")
         desc = (f"dam break + static sphere r=0.1 at (0.6,0.2,0.5), {n}^3, 8 ppc; U-Net 16/32/64, BF16/FP32, "
-                f"trained on {nf} residual frames of {args.frames} Jacobi-PCG solves ({args.epochs} epochs)")
+                f"{trained}; compared on the {args.compare_frames} frames after frame {args.frames}")
         extra = {"capture_s": round(capture_s, 1), "train_s": round(train_s, 1),
                  "final_loss": round(float(losses[-1]), 5) if losses else None}
     elif args.config == "c5":
@@ -273,6 +280,10 @@ def learned_run(args):
             sim.set_solid_box((0.0, 0.0, 0.0), (xp, 1.0, 1.0), (up, 0.0, 0.0))
         for st in ("transfer", "prepare", "system"):
             sim.stage(st)
+        if s == 0:
+            # untimed: first-use allocations (direction ring, network buffers)
+            for name in ("jacobi", "mic0", "learned"):
+                sim.solve_with(name)
         for name in ("jacobi", "mic0", "learned"):
             res[name].append(sim.solve_with(name))
         sim.solve_with("jacobi")
