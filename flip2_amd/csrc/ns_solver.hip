@@ -449,56 +449,63 @@ direction_kernel(const double* __restrict__ rzNew, const double* __restrict__ rz
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) p[base + l] = fmaf(beta, p[base + l], z[base + l]);
 }
 
+template <int N>  // kept directions (d.count == N)
 __global__ void __launch_bounds__(kWG)
 flex_dots_kernel(const float* __restrict__ z, FlexDirections d, double* __restrict__ partials) {
     const uint32_t base = blockIdx.x * 512u;
-    double acc[kMaxFlexDirections];
+    double acc[N];
 #pragma unroll
-    for (int i = 0; i < kMaxFlexDirections; ++i) acc[i] = 0.0;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
-        const double zv = (double)z[base + l];
+    for (int i = 0; i < N; ++i) acc[i] = 0.0;
 #pragma unroll
-        for (int i = 0; i < kMaxFlexDirections; ++i)
-            if (i < d.count) acc[i] += zv * (double)d.q[i][base + l];
+    for (int k = 0; k < 512 / kWG; ++k) {
+        const uint32_t v = base + threadIdx.x + k * kWG;
+        float q[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) q[i] = d.q[i][v];  // all loads issued before any is used
+        const double zv = (double)z[v];
+#pragma unroll
+        for (int i = 0; i < N; ++i) acc[i] += zv * (double)q[i];
     }
-    // all d.count sums with one LDS exchange: wave sums by shuffles, then
-    // thread i adds the four waves' sums of quantity i
-    __shared__ double s_many[kWG / 64][kMaxFlexDirections];
+    // all N sums with one LDS exchange: wave sums by shuffles, then thread i
+    // adds the four waves' sums of quantity i
+    __shared__ double s_many[kWG / 64][N];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < kMaxFlexDirections; ++i) {
-        if (i < d.count) {
-            double v = acc[i];
-            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0) s_many[wave][i] = v;
-        }
+    for (int i = 0; i < N; ++i) {
+        double v = acc[i];
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) s_many[wave][i] = v;
     }
     __syncthreads();
-    if ((int)threadIdx.x < d.count) {
+    if ((int)threadIdx.x < N) {
         double t = 0.0;
         for (int w = 0; w < kWG / 64; ++w) t += s_many[w][threadIdx.x];
         partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
     }
 }
 
+template <int N>  // kept directions (d.count == N, 0 for p = z)
 __global__ void __launch_bounds__(kWG)
 flex_direction_kernel(const float* __restrict__ z, FlexDirections d, const double* __restrict__ zq,
                       const float* __restrict__ r, float* __restrict__ p, double* __restrict__ partials) {
     __shared__ double s_tmp[16];
-    float beta[kMaxFlexDirections];
+    float beta[N > 0 ? N : 1];
 #pragma unroll
-    for (int i = 0; i < kMaxFlexDirections; ++i) {
-        const double den = i < d.count ? *d.pq[i] : 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double den = *d.pq[i];
         beta[i] = den != 0.0 ? (float)(-zq[i] / den) : 0.f;
     }
     const uint32_t base = blockIdx.x * 512u;
     double pr = 0.0;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
-        const uint32_t v = base + l;
+#pragma unroll
+    for (int k = 0; k < 512 / kWG; ++k) {
+        const uint32_t v = base + threadIdx.x + k * kWG;
+        float pd[N > 0 ? N : 1];
+#pragma unroll
+        for (int i = 0; i < N; ++i) pd[i] = d.p[i][v];
         float pv = z[v];
 #pragma unroll
-        for (int i = 0; i < kMaxFlexDirections; ++i)
-            if (i < d.count) pv = fmaf(beta[i], d.p[i][v], pv);
+        for (int i = 0; i < N; ++i) pv = fmaf(beta[i], pd[i], pv);
         p[v] = pv;
         pr += (double)pv * (double)r[v];
     }
@@ -1036,7 +1043,13 @@ void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& 
                       hipStream_t s) {
     FLIP2_REQUIRE(d.count >= 0 && d.count <= kMaxFlexDirections, kErrArgument, "flexible CG: too many directions");
     if (d.count == 0) return;
-    hipLaunchKernelGGL(flex_dots_kernel, dim3(numActive), dim3(kWG), 0, s, z, d, red.partials);
+    switch (d.count) {
+#define FLEX_CASE(N) \
+    case N: hipLaunchKernelGGL(flex_dots_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, red.partials); break;
+        FLEX_CASE(1) FLEX_CASE(2) FLEX_CASE(3) FLEX_CASE(4) FLEX_CASE(5) FLEX_CASE(6) FLEX_CASE(7) FLEX_CASE(8)
+        FLEX_CASE(9) FLEX_CASE(10) FLEX_CASE(11) FLEX_CASE(12) FLEX_CASE(13) FLEX_CASE(14) FLEX_CASE(15) FLEX_CASE(16)
+#undef FLEX_CASE
+    }
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, d.count, numActive, out, s);
 }
@@ -1046,7 +1059,16 @@ void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirecti
     FLIP2_REQUIRE(d.count >= 0 && d.count <= kMaxFlexDirections, kErrArgument, "flexible CG: too many directions");
     for (int i = 0; i < d.count; ++i)
         FLIP2_REQUIRE(d.p[i] != p, kErrArgument, "flexible CG: output aliases a kept direction");
-    hipLaunchKernelGGL(flex_direction_kernel, dim3(numActive), dim3(kWG), 0, s, z, d, zq, r, p, red.partials);
+    switch (d.count) {
+#define FLEX_CASE(N)                                                                                              \
+    case N:                                                                                                       \
+        hipLaunchKernelGGL(flex_direction_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, zq, r, p, red.partials); \
+        break;
+        FLEX_CASE(0) FLEX_CASE(1) FLEX_CASE(2) FLEX_CASE(3) FLEX_CASE(4) FLEX_CASE(5) FLEX_CASE(6) FLEX_CASE(7)
+        FLEX_CASE(8) FLEX_CASE(9) FLEX_CASE(10) FLEX_CASE(11) FLEX_CASE(12) FLEX_CASE(13) FLEX_CASE(14) FLEX_CASE(15)
+        FLEX_CASE(16)
+#undef FLEX_CASE
+    }
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, pr, s);
 }
