@@ -330,6 +330,16 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
         }
         uint32_t it = 0;
         bool innerDone = false;
+        // the residual norms of iterations first + 1 .. first + count (read
+        // back from the history) reached the inner tolerance
+        auto converged = [&](uint32_t first, uint32_t count) {
+            readHistory(first + 1, count, h.data());
+            for (uint32_t k = 0; k < count; ++k) {
+                const double rr = h[4 * k + 1];
+                if (!(rr == rr) || std::sqrt(std::max(rr, 0.0)) <= innerTol * ref) return true;
+            }
+            return false;
+        };
         while (!innerDone && total + it < cap) {
             const uint32_t batch = std::min(every, cap - total - it);
             const uint32_t first = it;
@@ -341,6 +351,14 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                 ns::launch_matvec_dot(na, p, coef_.devPtr(), nbrSlot_.devPtr(), q, reduce(), hk + 2, stream_);
                 ns::launch_update(na, hk, hk + 2, p, q, coef_.devPtr(), e_.devPtr(), r_.devPtr(), z_.devPtr(), jac,
                                   reduce(), hn, stream_);
+                if (flex && k + 1 == batch && converged(first, batch)) {
+                    // checked before the apply, the costly part of a
+                    // flexible-CG iteration: after convergence it is not needed
+                    innerDone = true;
+                    if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
+                    ++it;
+                    break;
+                }
                 if (prec == kPrecMIC0) {
                     ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), hn,
                                          stream_);
@@ -356,14 +374,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                 if (!flex) ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
                 if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
             }
-            readHistory(first + 1, batch, h.data());
-            for (uint32_t k = 0; k < batch; ++k) {
-                const double rr = h[4 * k + 1];
-                if (!(rr == rr) || std::sqrt(std::max(rr, 0.0)) <= innerTol * ref) {
-                    innerDone = true;
-                    break;
-                }
-            }
+            if (!flex && converged(first, batch)) innerDone = true;
         }
         total += it;
         ns::launch_accumulate(na, e_.devPtr(), x64_.devPtr(), coef_.devPtr(), stream_);
