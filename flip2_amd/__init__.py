@@ -362,10 +362,6 @@ class P2GPipeline:
         """records: device particle records (8 floats each); out: device U, V, W."""
         check(lib().flip2_p2g_pipeline_run_records(self._h, int(n), records.ptr, *(b.ptr for b in out), stream))
 
-    def set_flex_check_every(self, iterations: int) -> None:
-        """Iterations (1..1024) between residual checks in flexible CG."""
-        check(lib().flip2_fluid_set_flex_check_every(self._h, int(iterations)))
-
     def get(self, name: str) -> np.ndarray:
         cnt = _sz()
         check(lib().flip2_p2g_pipeline_get(self._h, name.encode(), None, 0, ctypes.byref(cnt)))
@@ -503,6 +499,10 @@ class FluidSim:
     def set_flex_directions(self, directions: int) -> None:
         """Earlier directions (1..16) flexible CG orthogonalises against."""
         check(lib().flip2_fluid_set_flex_directions(self._h, int(directions)))
+
+    def set_flex_check_every(self, iterations: int) -> None:
+        """Iterations (1..1024) between residual checks in flexible CG."""
+        check(lib().flip2_fluid_set_flex_check_every(self._h, int(iterations)))
 
     def get(self, name: str) -> np.ndarray:
         cnt = _sz()

@@ -911,7 +911,8 @@ int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iter
         flip2::UNet* u = &net->net;
         flip2::FluidSim* s = &sim;
         sim.setLearnedPreconditioner([u, s](uint32_t na, const float* r, float* z, bool newSystem, hipStream_t st) {
-            u->applyActive(na, s->activeBricksDev(), s->cellTypeDev(), r, z, s->gridN(), newSystem, st);
+            u->applyActive(na, s->activeBricksDev(), s->brickSlotDev(), s->cellTypeDev(), r, z, s->gridN(), newSystem,
+                           st);
         });
     });
 }

@@ -129,6 +129,7 @@ public:
     const int32_t* nbrSlotDev() const { return nbrSlot_.devPtr(); }
     const uint32_t* activeBricksDev() const { return activeBricks_.devPtr(); }
     const uint8_t* cellTypeDev() const { return cellType_.devPtr(); }
+    const int32_t* brickSlotDev() const { return brickSlot_.devPtr(); }
     const double* rhsDev() const { return b64_.devPtr(); }
     const double* solutionDev() const { return x64_.devPtr(); }
     int gridN() const { return (int)cfg_.grid.sizeX; }

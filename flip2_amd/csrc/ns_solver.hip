@@ -405,29 +405,39 @@ dot_kernel(const float* __restrict__ a, const float* __restrict__ b, double* __r
 // Damped-Jacobi smoothing around an external (learned) correction:
 //   presmooth:  r1 = r - A (w D^-1 r)
 //   postsmooth: z  = x + w D^-1 (r - A x),  x = w D^-1 r + y
+// The operand x of the slot's own 512 cells is evaluated once into LDS;
+// neighbours in other slots (the brick's faces) are evaluated from memory.
 __global__ void __launch_bounds__(kWG)
 presmooth_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
                  float omega, float* __restrict__ r1) {
+    __shared__ float sx[512];
     const uint32_t slot = blockIdx.x;
     auto jac = [&](uint32_t i) {
         const uint16_t c = coef[i];
         return c ? omega * r[i] / (float)(c & 7u) : 0.f;
     };
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) sx[l] = jac(slot * 512u + l);
+    __syncthreads();
+    auto xs = [&](uint32_t i) { return (i >> 9) == slot ? sx[i & 511u] : jac(i); };
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
         const uint32_t v = slot * 512u + l;
         const uint16_t cf = coef[v];
-        r1[v] = cf ? r[v] - apply_A_with(slot, l, cf, nbrSlot, jac) : 0.f;
+        r1[v] = cf ? r[v] - apply_A_with(slot, l, cf, nbrSlot, xs) : 0.f;
     }
 }
 
 __global__ void __launch_bounds__(kWG)
 postsmooth_kernel(const float* __restrict__ r, const float* __restrict__ y, const uint16_t* __restrict__ coef,
                   const int32_t* __restrict__ nbrSlot, float omega, float* __restrict__ z) {
+    __shared__ float sx[512];
     const uint32_t slot = blockIdx.x;
     auto xf = [&](uint32_t i) {
         const uint16_t c = coef[i];
         return c ? omega * r[i] / (float)(c & 7u) + y[i] : 0.f;
     };
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) sx[l] = xf(slot * 512u + l);
+    __syncthreads();
+    auto xs = [&](uint32_t i) { return (i >> 9) == slot ? sx[i & 511u] : xf(i); };
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
         const uint32_t v = slot * 512u + l;
         const uint16_t cf = coef[v];
@@ -435,8 +445,8 @@ postsmooth_kernel(const float* __restrict__ r, const float* __restrict__ y, cons
             z[v] = 0.f;
             continue;
         }
-        const float xv = xf(v);
-        z[v] = xv + omega * (r[v] - apply_A_with(slot, l, cf, nbrSlot, xf)) / (float)(cf & 7u);
+        const float xv = sx[l];
+        z[v] = xv + omega * (r[v] - apply_A_with(slot, l, cf, nbrSlot, xs)) / (float)(cf & 7u);
     }
 }
 

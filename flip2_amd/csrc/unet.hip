@@ -94,8 +94,12 @@ __device__ inline void block_pair_partial(double a, double c, double* partials) 
     }
 }
 
-// One wave: red[0..1] = sums of the pairs, lane l taking pairs l, l + 64, ...
-__global__ void sum_pairs_kernel(const double* __restrict__ partials, int blocks, double* __restrict__ red) {
+__device__ inline float rms_scale(const double* red);
+
+// One wave: red[0..1] = sums of the pairs, lane l taking pairs l, l + 64, ...;
+// scale != null: also *scale = RMS scale of the network input (rms_scale).
+__global__ void sum_pairs_kernel(const double* __restrict__ partials, int blocks, double* __restrict__ red,
+                                 float* __restrict__ scale = nullptr) {
     double a = 0.0, c = 0.0;
     for (int b = (int)threadIdx.x; b < blocks; b += 64) {
         a += partials[2 * b];
@@ -108,6 +112,7 @@ __global__ void sum_pairs_kernel(const double* __restrict__ partials, int blocks
     if (threadIdx.x == 0) {
         red[0] = a;
         red[1] = c;
+        if (scale) scale[0] = rms_scale(red);
     }
 }
 
@@ -146,19 +151,6 @@ __global__ void scatter_input_kernel(const uint32_t* __restrict__ activeBricks, 
     in0c[v * 2] = to_bf(r[i] / s);
     in0c[v * 2 + 1] = to_bf(1.0f);
     mask[v] = 1;
-}
-
-__global__ void gather_output_kernel(const uint32_t* __restrict__ activeBricks, const uint8_t* __restrict__ cellType,
-                                     const float* __restrict__ out, uint32_t cells, int n, float* __restrict__ z) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cells) return;
-    float v = 0.f;
-    if (cellType[i] == 1) {
-        int x, y, zz;
-        dense_coords(activeBricks[i >> 9], i & 511u, n, x, y, zz);
-        v = out[((long long)zz * n + y) * n + x];
-    }
-    z[i] = v;
 }
 
 __global__ void dense_system_kernel(const uint32_t* __restrict__ activeBricks, const uint16_t* __restrict__ coef,
@@ -325,11 +317,12 @@ void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, cons
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_gather_output(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* out,
-                          int n, float* z, hipStream_t s) {
+void launch_input_scale(uint32_t numActive, const uint8_t* cellType, const float* r, double* red, float* scale,
+                        hipStream_t s) {
     const uint32_t cells = numActive * 512u;
-    if (!cells) return;
-    hipLaunchKernelGGL(gather_output_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, cellType, out, cells, n, z);
+    const dim3 rg = red_grid(cells);
+    hipLaunchKernelGGL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
+    hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red, scale);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -545,8 +538,8 @@ void UNet::forward(hipStream_t s, bool keepActivations) {
         // next step's input halo and weights prefetched during the current
         // step's MFMAs.  The decoder's concatenated inputs are read in place,
         // and the output head runs in the last convolution.
-        nn::launch_conv3_in2_list(in0c_.devPtr(), paramsBf_.devPtr() + layers_[0].w, params_.devPtr() + layers_[0].b,
-                                  a0_.devPtr(), n_, true, cm[0].voxel, tl_[0], s);
+        nn::launch_conv3_in2_list(in0c_.devPtr(), slotIn_, paramsBf_.devPtr() + layers_[0].w,
+                                  params_.devPtr() + layers_[0].b, a0_.devPtr(), n_, true, cm[0].voxel, tl_[0], s);
         const int levelN[3] = {n_, n1_, n2_};
         const int split[3] = {1, 2, 4};
         auto lconv = [&](int layer, const uint16_t* X, const uint16_t* Xc, int ca, uint16_t* Y, uint16_t* pooled,
@@ -567,7 +560,7 @@ void UNet::forward(hipStream_t s, bool keepActivations) {
         lconv(7, c1_.devPtr(), nullptr, 0, d1_.devPtr(), nullptr, none);
         lconv(8, b0_.devPtr(), d1_.devPtr(), 32, c0_.devPtr(), nullptr, none);
         const nn::HeadArgs head{params_.devPtr() + headW_, params_.devPtr() + headB_, mask_.devPtr(), scale_.devPtr(),
-                                out_.devPtr()};
+                                slotOut_ ? nullptr : out_.devPtr(), slotOut_, slotIn_.brickSlot};
         lconv(9, c0_.devPtr(), nullptr, 0, nullptr, nullptr, head);
         return;
     }
@@ -658,14 +651,29 @@ void UNet::adamStep(float lr, hipStream_t s) {
                     (int)numParams_, lr, 0.9f, 0.999f, 1e-8f, step_, s);
 }
 
-void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
-                       float* z, int gridN, bool newSystem, hipStream_t s) {
-    FLIP2_REQUIRE(gridN == n_, kErrArgument, "U-Net grid size differs from the simulation grid");
-    nn::launch_scatter_input(numActive, activeBricks, cellType, r, n_, in0c_.devPtr(), mask_.devPtr(), scale_.devPtr(),
-                             red_.devPtr(), newSystem, s);
-    if (newSystem) setInputMasks(s, true);
-    forward(s, false);
-    nn::launch_gather_output(numActive, activeBricks, cellType, out_.devPtr(), n_, z, s);
+void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
+                       const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem, hipStream_t s) {
+    FLIP2_REQUIRE(gridN == n_ && n_ % 8 == 0, kErrArgument, "U-Net grid size differs from the simulation grid");
+    if (newSystem) {
+        // the dense fluid mask (level masks, tile lists, the head's mask)
+        nn::launch_scatter_input(numActive, activeBricks, cellType, r, n_, in0c_.devPtr(), mask_.devPtr(),
+                                 scale_.devPtr(), red_.devPtr(), true, s);
+        setInputMasks(s, true);
+        nn::launch_zero_f32(z, (long long)numActive * 512, s);
+    } else {
+        nn::launch_input_scale(numActive, cellType, r, red_.devPtr(), scale_.devPtr(), s);
+    }
+    slotIn_ = nn::SlotInput{r, cellType, brickSlot, scale_.devPtr()};
+    slotOut_ = z;
+    try {
+        forward(s, false);
+    } catch (...) {
+        slotIn_ = nn::SlotInput{};
+        slotOut_ = nullptr;
+        throw;
+    }
+    slotIn_ = nn::SlotInput{};
+    slotOut_ = nullptr;
 }
 
 }  // namespace flip2

@@ -60,11 +60,12 @@ public:
     void adamStep(float lr, hipStream_t s);
     const float* gradDev() const { return grad_.devPtr(); }
 
-    // Preconditioner apply on the solver's active-brick vectors.  newSystem:
-    // the fluid cells may differ from the previous apply's (masks and tile
-    // flags are rebuilt); otherwise only r changed.
-    void applyActive(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
-                     float* z, int gridN, bool newSystem, hipStream_t s);
+    // Preconditioner apply on the solver's active-brick vectors (brickSlot:
+    // brick -> slot, -1 inactive).  newSystem: the fluid cells may differ
+    // from the previous apply's (masks and tile flags are rebuilt, z is
+    // zeroed); otherwise only r changed.  z is written at fluid cells only.
+    void applyActive(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
+                     const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem, hipStream_t s);
 
     const std::vector<UNetLayer>& layers() const { return layers_; }
 
@@ -96,6 +97,10 @@ private:
     DeviceVec<float> gd0_, gc0_, gu0_, gd1_, gc1_, gu1_, gb2_, ga2_, gp1_, gb1_, ga1_, gp0_, gb0_, ga0_;
     DeviceVec<double> red_;
     DeviceVec<uint16_t> dPb_, WT_;  // BF16 gradient of the current layer, transposed weights
+    // set by applyActive for its forward: the first layer reads the solver's
+    // slots, the head writes them
+    nn::SlotInput slotIn_{};
+    float* slotOut_ = nullptr;
 };
 
 // Dense-grid helpers shared by the trainer and the preconditioner apply.
@@ -105,9 +110,10 @@ namespace nn {
 // clear: zero in0c and mask everywhere first (the fluid cells changed).
 void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* r,
                           int n, uint16_t* in0c, uint8_t* mask, float* scale, double* red, bool clear, hipStream_t s);
-// z (active layout) = out (dense) on fluid cells, 0 elsewhere.
-void launch_gather_output(uint32_t numActive, const uint32_t* activeBricks, const uint8_t* cellType, const float* out,
-                          int n, float* z, hipStream_t s);
+// *scale = s (as launch_scatter_input) without the scatter.
+void launch_input_scale(uint32_t numActive, const uint8_t* cellType, const float* r, double* red, float* scale,
+                        hipStream_t s);
+
 // Dense copies of a solver system: b, solution x, operator coefficients
 // (u16 diag | mask << 3, brick-tiled order in, dense [z][y][x] out).
 void launch_dense_system(uint32_t numActive, const uint32_t* activeBricks, const uint16_t* coef, const double* b,

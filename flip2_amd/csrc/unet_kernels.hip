@@ -225,8 +225,9 @@ __device__ inline int pair_tap(int q, int dy, int half) {
 template <int MB>
 struct EpiIn {
     float bias[MB][4];
-    bool live[4];      // inside the grid and in the voxel mask
-    bool headLive[4];  // head.mask of the voxel (fused head only)
+    bool live[4];          // inside the grid and in the voxel mask
+    bool headLive[4];      // head.mask of the voxel (fused head only)
+    uint32_t slotCell[4];  // head output index in the brick-slot layout (head.outSlots)
 };
 
 template <int COUT, int MB>
@@ -245,7 +246,13 @@ __device__ inline EpiIn<MB> epi_load(const float* __restrict__ bias, const uint8
         const bool inside = x < n && y < n && z < n;
         const long long v = ((long long)z * n + y) * n + x;
         e.live[j] = inside && (!vmask || vmask[v]);
-        e.headLive[j] = COUT == 16 && head.out && inside && head.mask[v];
+        e.headLive[j] = COUT == 16 && (head.out || head.outSlots) && inside && head.mask[v];
+        e.slotCell[j] = 0u;
+        if (COUT == 16 && head.outSlots && e.headLive[j]) {
+            const int nb = n >> 3;
+            const int32_t slot = head.brickSlot[((z >> 3) * nb + (y >> 3)) * nb + (x >> 3)];
+            e.slotCell[j] = (uint32_t)slot * 512u + (uint32_t)(((z & 7) * 8 + (y & 7)) * 8 + (x & 7));
+        }
     }
     return e;
 }
@@ -257,7 +264,7 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e
                                      float* __restrict__ Yf, int n, int relu, const HeadArgs& head, int bx, int by,
                                      int bz, int m0, int wave, int col, int kq) {
     const int z = bz + wave, x = bx + col;
-    if (MODE == 0 && COUT == 16 && head.out) {
+    if (MODE == 0 && COUT == 16 && (head.out || head.outSlots)) {
         // fused head: gather the 16 channels of each voxel in its kq == 0 lane
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -290,7 +297,8 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e
                 float s = head.b[0];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) s = fmaf(ch[c], head.w[c], s);
-                head.out[v] = e.headLive[j] ? s * head.scale[0] : 0.f;
+                if (!head.outSlots) head.out[v] = e.headLive[j] ? s * head.scale[0] : 0.f;
+                else if (e.headLive[j]) head.outSlots[e.slotCell[j]] = s * head.scale[0];
             }
         }
         return;
@@ -697,7 +705,7 @@ conv3_in2_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restrict__ W
 // zeroed first.
 constexpr int kIn2Loads = (kHalo + 255) / 256;  // 3
 __global__ void __launch_bounds__(256)
-conv3_in2_list_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restrict__ Wt,
+conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint16_t* __restrict__ Wt,
                       const float* __restrict__ bias, uint16_t* __restrict__ Y, int n, int relu,
                       const uint8_t* __restrict__ vmask, TileList tl) {
     __shared__ uint32_t sIn[2][kHalo];
@@ -742,7 +750,32 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restric
         a[kb] = __builtin_bit_cast(v8bf, make_uint4(wq[0], wq[1], wq[2], wq[3]));
     }
     uint32_t pre[kIn2Loads];
+    const uint32_t one = (uint32_t)f2bf(1.0f) << 16;
     auto load_tile = [&](int bx, int by, int bz) {
+        if (si.r) {
+            // brick slots first, then the residual and cell type of each voxel
+            int32_t slot[kIn2Loads];
+            uint32_t cell[kIn2Loads];
+#pragma unroll
+            for (int k = 0; k < kIn2Loads; ++k) {
+                const int t = (int)threadIdx.x + 256 * k;
+                const int hx = t % kHX, hy = (t / kHX) % kHY, hz = t / (kHX * kHY);
+                const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
+                const bool in = t < kHalo && x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n;
+                const int nb = n >> 3;
+                slot[k] = in ? si.brickSlot[((z >> 3) * nb + (y >> 3)) * nb + (x >> 3)] : -1;
+                cell[k] = in ? (uint32_t)(((z & 7) * 8 + (y & 7)) * 8 + (x & 7)) : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kIn2Loads; ++k) {
+                pre[k] = 0u;
+                if (slot[k] >= 0) {
+                    const uint32_t c = (uint32_t)slot[k] * 512u + cell[k];
+                    if (si.cellType[c] == 1) pre[k] = (uint32_t)f2bf(si.r[c] / si.scale[0]) | one;
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < kIn2Loads; ++k) {
             const int t = (int)threadIdx.x + 256 * k;
@@ -1246,11 +1279,12 @@ void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, 
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_conv3_in2_list(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
-                           const uint8_t* vmask, const TileList& tl, hipStream_t s) {
+void launch_conv3_in2_list(const uint16_t* X2, const SlotInput& slots, const uint16_t* W, const float* bias,
+                           uint16_t* Y, int n, bool relu, const uint8_t* vmask, const TileList& tl, hipStream_t s) {
+    FLIP2_REQUIRE(!slots.r || n % 8 == 0, kErrArgument, "slot input: n must be a multiple of the 8^3 bricks");
     // four workgroups per CU (the register budget's limit)
-    hipLaunchKernelGGL(conv3_in2_list_kernel, dim3(256 * 4), dim3(256), 0, s, X2, W, bias, Y, n, relu ? 1 : 0, vmask,
-                       tl);
+    hipLaunchKernelGGL(conv3_in2_list_kernel, dim3(256 * 4), dim3(256), 0, s, X2, slots, W, bias, Y, n, relu ? 1 : 0,
+                       vmask, tl);
     FLIP2_LAUNCH_CHECK();
 }
 

@@ -37,7 +37,19 @@ struct HeadArgs {
     const float* b;
     const uint8_t* mask;
     const float* scale;
-    float* out;
+    float* out;                // dense [n^3] output, or
+    float* outSlots;           // the solver's brick-slot layout (slot * 512 + cell), masked voxels only
+    const int32_t* brickSlot;  // brick -> slot (bricks of 8^3 voxels, n/8 per axis) for outSlots
+};
+// Input of the first convolution read from the solver's brick-slot layout:
+// voxel v of a brick with slot s >= 0 is r[s * 512 + local]; the residual
+// channel is bf16(r / scale[0]) and the mask channel 1 where cellType == 1
+// (fluid), both 0 elsewhere.
+struct SlotInput {
+    const float* r;
+    const uint8_t* cellType;
+    const int32_t* brickSlot;
+    const float* scale;
 };
 // Decoder convolution reading concat(up2(Xcoarse) [ca channels], Xskip [cb
 // channels]) without materialising it; Y = ReLU(conv + bias).
@@ -51,8 +63,9 @@ void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, 
                       const ConvMask& mask, hipStream_t s);
 // The same over the list of tiles with fluid (tl), zeroing the tiles in tl's
 // clear list; vmask: the level's voxel mask (outputs outside it are zero).
-void launch_conv3_in2_list(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
-                           const uint8_t* vmask, const TileList& tl, hipStream_t s);
+// slots.r != null: the input comes from slots instead of X2.
+void launch_conv3_in2_list(const uint16_t* X2, const SlotInput& slots, const uint16_t* W, const float* bias,
+                           uint16_t* Y, int n, bool relu, const uint8_t* vmask, const TileList& tl, hipStream_t s);
 // 16 -> 16 convolution with the output head in its epilogue (Y may be null).
 void launch_conv3_fwd_head(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* Y, int n, int cin,
                            bool relu, const HeadArgs& head, const ConvMask& mask, hipStream_t s);

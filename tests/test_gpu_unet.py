@@ -193,6 +193,13 @@ def test_training_and_learned_fcg(flip2):
     again = sim.solve_with("learned")
     for r in (eager, again):
         assert r["iterations"] == lrn["iterations"] and r["rel_residual"] == lrn["rel_residual"], (r, lrn)
+    # residual checks in batches of 8 iterations stop no earlier than checks
+    # after every iteration (the default), and reach the same tolerance
+    sim.set_flex_check_every(8)
+    batched = sim.solve_with("learned")
+    sim.set_flex_check_every(1)
+    assert batched["converged"] and batched["rel_residual"] <= 1e-6, batched
+    assert batched["iterations"] >= lrn["iterations"], (batched, lrn)
     # the fallback: a cap below what FCG needs hands the solve to Jacobi-PCG
     sim.set_unet(net, fallback_iterations=2)
     fb = sim.solve_with("learned")
