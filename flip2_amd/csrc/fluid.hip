@@ -310,6 +310,12 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     const uint32_t every = std::max(1u, flex ? cfg_.learnedCheckEvery : cfg_.checkEvery);
     std::vector<double> h(4 * (every + 1));
     double* H = hist_.devPtr();
+    // Flexible CG does not restart at a refinement: the recursive residual is
+    // replaced by the true one (FP64, rounded to FP32), the correction so far
+    // is added to x64, and the iteration continues with its kept directions
+    // (their A-conjugacy does not depend on the residual).  itCarry: the
+    // iteration count (ring slot and history index) it continues from.
+    uint32_t itCarry = 0;
     for (uint32_t refine = 0; refine < 12 && total < cap; ++refine) {
         const double innerTol = std::max(0.5 * cfg_.tolerance / rel, 2e-5);
         const double ref = rel * st.bNorm;  // norm of the FP32 right-hand side of this refinement
@@ -318,17 +324,26 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
         if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
         if (jac) {
             ns::launch_jacobi_init(na, r_.devPtr(), coef_.devPtr(), z_.devPtr(), p_.devPtr(), reduce(), H, stream_);
-        } else if (flex) {
+        } else if (flex && itCarry == 0) {
             // p_0 = z_0; H[0] = p_0 . r_0 (the step-length numerator)
             applyLearned(na);
             ns::FlexDirections none{};
             ns::launch_flex_direction(na, z_.devPtr(), none, zq_.devPtr(), r_.devPtr(), ringP(0), reduce(), H,
                                       stream_);
+        } else if (flex) {
+            // the apply and direction of iteration itCarry, on the replaced
+            // residual: p = z - sum_i (z . q_i / p_i . q_i) p_i
+            applyLearned(na);
+            const ns::FlexDirections d = kept(itCarry - 1, H);
+            ns::launch_flex_dots(na, z_.devPtr(), d, reduce(), zq_.devPtr(), stream_);
+            ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(itCarry), reduce(),
+                                      H + 4u * itCarry, stream_);
         } else {
             ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), H, stream_);
             ns::launch_direction(na, H, H, z_.devPtr(), p_.devPtr(), true, stream_);
         }
-        uint32_t it = 0;
+        const uint32_t it0 = flex ? itCarry : 0u;
+        uint32_t it = it0;
         bool innerDone = false;
         // the residual norms of iterations first + 1 .. first + count (read
         // back from the history) reached the inner tolerance
@@ -340,8 +355,8 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             }
             return false;
         };
-        while (!innerDone && total + it < cap) {
-            const uint32_t batch = std::min(every, cap - total - it);
+        while (!innerDone && total + (it - it0) < cap) {
+            const uint32_t batch = std::min(every, cap - total - (it - it0));
             const uint32_t first = it;
             for (uint32_t k = 0; k < batch; ++k, ++it) {
                 double* hk = H + 4u * it;
@@ -376,7 +391,8 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             }
             if (!flex && converged(first, batch)) innerDone = true;
         }
-        total += it;
+        total += it - it0;
+        if (flex) itCarry = it;
         ns::launch_accumulate(na, e_.devPtr(), x64_.devPtr(), coef_.devPtr(), stream_);
         ns::launch_true_residual(na, b64_.devPtr(), x64_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), r64_.devPtr(),
                                  reduce(), scal_.devPtr() + 2, stream_);
