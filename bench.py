@@ -234,20 +234,24 @@ def learned_run(args):
         if args.load_net:
             # a network trained by an earlier run of this configuration
             net.set_params(np.load(args.load_net, allow_pickle=False))
+        elif args.init_net:
+            # continue the training of an earlier run's network
+            net.set_params(np.load(args.init_net, allow_pickle=False))
         epochs = 0 if args.load_net else args.epochs
         print(f"[bench] training {epochs} epochs on {nf} frames", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         losses = []
         for e in range(epochs):
-            # learning rate from 1e-3 down to 3e-4, geometrically over the epochs
-            lr = 1e-3 * 0.3 ** (e / max(1, args.epochs - 1))
+            # learning rate from --lr down to 0.3 --lr, geometrically over the epochs
+            lr = args.lr * 0.3 ** (e / max(1, args.epochs - 1))
             losses.append(net.train(1, lr, 0, nf, seed=e))
             print(f"[bench] epoch {e + 1}: loss {losses[-1]:.4f}", file=sys.stderr, flush=True)
         train_s = time.perf_counter() - t0
         if args.save_net:
             np.save(args.save_net, net.params(), allow_pickle=False)
         trained = (f"network from {args.load_net}" if args.load_net else
-                   f"trained on {nf} residual frames of {args.frames} Jacobi-PCG solves ({args.epochs} epochs)")
+                   f"trained on {nf} residual frames of {args.frames} Jacobi-PCG solves ({args.epochs} epochs"
+                   + (f" after {args.init_net}" if args.init_net else "") + ")")
         desc = (f"dam break + static sphere r=0.1 at (0.6,0.2,0.5), {n}^3, 8 ppc; U-Net 16/32/64, BF16/FP32, "
                 f"{trained}; compared on the {args.compare_frames} frames after frame {args.frames}")
         extra = {"capture_s": round(capture_s, 1), "train_s": round(train_s, 1),
@@ -328,7 +332,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="learned: launch the preconditioner's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--save-net", default=None)
-    ap.add_argument("--load-net", default=None)
+    ap.add_argument("--load-net", default=None, help="c4/c5: use this saved network (c4: no training)")
+    ap.add_argument("--init-net", default=None, help="c4: start the training from this saved network")
+    ap.add_argument("--lr", type=float, default=1e-3, help="c4: first epoch's learning rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this pipeline has no multi-GPU "

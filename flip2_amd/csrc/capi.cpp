@@ -910,10 +910,11 @@ int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iter
         FLIP2_REQUIRE(sim.gridN() == net->n(), flip2::kErrArgument, "grid size differs from the network's");
         flip2::UNet* u = &net->net;
         flip2::FluidSim* s = &sim;
-        sim.setLearnedPreconditioner([u, s](uint32_t na, const float* r, float* z, bool newSystem, hipStream_t st) {
-            u->applyActive(na, s->activeBricksDev(), s->brickSlotDev(), s->cellTypeDev(), r, z, s->gridN(), newSystem,
-                           st);
-        });
+        sim.setLearnedPreconditioner(
+            [u, s](uint32_t na, const float* r, float* z, bool newSystem, const double* pairs, hipStream_t st) {
+                u->applyActive(na, s->activeBricksDev(), s->brickSlotDev(), s->cellTypeDev(), r, z, s->gridN(),
+                               newSystem, pairs, st);
+            });
     });
 }
 

@@ -146,12 +146,15 @@ void FluidSim::ensureRing(uint32_t slots) {
 void FluidSim::applyLearnedEager(uint32_t na, bool newSystem) {
     const float w = cfg_.learnedSmoothing;
     if (w > 0.f) {
-        ns::launch_presmooth(na, r_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), w, aux0_.devPtr(), stream_);
-        learned_(na, aux0_.devPtr(), aux1_.devPtr(), newSystem, stream_);
+        // the partial sums of the network input's scale come with the smoothing
+        double* pairs = partials_.devPtr();
+        ns::launch_presmooth(na, r_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(), w, aux0_.devPtr(),
+                             pairs, stream_);
+        learned_(na, aux0_.devPtr(), aux1_.devPtr(), newSystem, pairs, stream_);
         ns::launch_postsmooth(na, r_.devPtr(), aux1_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), w, z_.devPtr(),
                               stream_);
     } else {
-        learned_(na, r_.devPtr(), z_.devPtr(), newSystem, stream_);
+        learned_(na, r_.devPtr(), z_.devPtr(), newSystem, nullptr, stream_);
     }
 }
 

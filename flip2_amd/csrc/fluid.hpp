@@ -63,8 +63,10 @@ struct SolveStats {
 // z = M^-1 r on the active-brick vectors, FP32.  newSystem is set on the
 // first apply of a solve (the fluid cells may have changed); the other
 // applies of a solve launch the same kernels on the same buffers, so the
-// solver may replay them from a HIP graph.
-using PrecondFn = std::function<void(uint32_t numActive, const float* r, float* z, bool newSystem, hipStream_t s)>;
+// solver may replay them from a HIP graph.  normPairs: null, or per slot
+// (sum of r^2, count) over the fluid cells, already computed by the solver.
+using PrecondFn = std::function<void(uint32_t numActive, const float* r, float* z, bool newSystem,
+                                     const double* normPairs, hipStream_t s)>;
 // Receives the residual r_k = b - A x_k (FP32, active-brick layout) of a
 // solve's first refinement after k inner iterations (training data).
 using CaptureFn = std::function<void(uint32_t numActive, const float* r, hipStream_t s)>;

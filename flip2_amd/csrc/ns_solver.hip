@@ -407,10 +407,14 @@ dot_kernel(const float* __restrict__ a, const float* __restrict__ b, double* __r
 //   postsmooth: z  = x + w D^-1 (r - A x),  x = w D^-1 r + y
 // The operand x of the slot's own 512 cells is evaluated once into LDS;
 // neighbours in other slots (the brick's faces) are evaluated from memory.
+// pairs != null: workgroup b also stores (sum of r1^2, count) over the cells
+// with cellType 1 in pairs[2 b], pairs[2 b + 1].
 __global__ void __launch_bounds__(kWG)
 presmooth_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
-                 float omega, float* __restrict__ r1) {
+                 const uint8_t* __restrict__ cellType, float omega, float* __restrict__ r1,
+                 double* __restrict__ pairs) {
     __shared__ float sx[512];
+    __shared__ double s_tmp[16];
     const uint32_t slot = blockIdx.x;
     auto jac = [&](uint32_t i) {
         const uint16_t c = coef[i];
@@ -419,10 +423,24 @@ presmooth_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef,
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) sx[l] = jac(slot * 512u + l);
     __syncthreads();
     auto xs = [&](uint32_t i) { return (i >> 9) == slot ? sx[i & 511u] : jac(i); };
+    double ss = 0.0, cnt = 0.0;
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
         const uint32_t v = slot * 512u + l;
         const uint16_t cf = coef[v];
-        r1[v] = cf ? r[v] - apply_A_with(slot, l, cf, nbrSlot, xs) : 0.f;
+        const float rv = cf ? r[v] - apply_A_with(slot, l, cf, nbrSlot, xs) : 0.f;
+        r1[v] = rv;
+        if (pairs && cellType[v] == 1) {
+            ss += (double)rv * (double)rv;
+            cnt += 1.0;
+        }
+    }
+    if (pairs) {
+        const double a = block_sum(ss, s_tmp);
+        const double c = block_sum(cnt, s_tmp);
+        if (threadIdx.x == 0) {
+            pairs[2 * blockIdx.x] = a;
+            pairs[2 * blockIdx.x + 1] = c;
+        }
     }
 }
 
@@ -1039,9 +1057,10 @@ void launch_direction(uint32_t numActive, const double* rzNew, const double* rzO
     hipLaunchKernelGGL(direction_kernel, dim3(numActive), dim3(kWG), 0, s, rzNew, rzOld, z, p, first ? 1 : 0);
     FLIP2_LAUNCH_CHECK();
 }
-void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot, float omega,
-                      float* r1, hipStream_t s) {
-    hipLaunchKernelGGL(presmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, nbrSlot, omega, r1);
+void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot,
+                      const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s) {
+    hipLaunchKernelGGL(presmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, nbrSlot, cellType, omega, r1,
+                       pairs);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const uint16_t* coef, const int32_t* nbrSlot,

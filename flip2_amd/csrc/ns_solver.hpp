@@ -73,8 +73,10 @@ void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, co
                             float* U, float* V, float* W, hipStream_t s);
 // Damped-Jacobi smoothing around an external correction y (see ns_solver.hip):
 // r1 = r - A (w D^-1 r);  z = x + w D^-1 (r - A x) with x = w D^-1 r + y.
-void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot, float omega,
-                      float* r1, hipStream_t s);
+// pairs != null (2 numActive doubles): per slot (sum of r1^2, count) over the
+// cells with cellType 1, the input scale of the learned correction.
+void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot,
+                      const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s);
 void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const uint16_t* coef, const int32_t* nbrSlot,
                        float omega, float* z, hipStream_t s);
 void launch_prepare_faces(float* U, float* V, float* W, float* U0, float* V0, float* W0, const uint8_t* solid, Grid g,

@@ -326,6 +326,11 @@ void launch_input_scale(uint32_t numActive, const uint8_t* cellType, const float
     FLIP2_LAUNCH_CHECK();
 }
 
+void launch_pairs_scale(const double* pairs, uint32_t count, double* red, float* scale, hipStream_t s) {
+    hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, pairs, (int)count, red, scale);
+    FLIP2_LAUNCH_CHECK();
+}
+
 __global__ void dense_vector_kernel(const uint32_t* __restrict__ activeBricks, const uint16_t* __restrict__ coef,
                                     const float* __restrict__ v, uint32_t cells, int n, float* __restrict__ vD,
                                     uint16_t* __restrict__ coefD) {
@@ -652,7 +657,8 @@ void UNet::adamStep(float lr, hipStream_t s) {
 }
 
 void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
-                       const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem, hipStream_t s) {
+                       const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem,
+                       const double* normPairs, hipStream_t s) {
     FLIP2_REQUIRE(gridN == n_ && n_ % 8 == 0, kErrArgument, "U-Net grid size differs from the simulation grid");
     if (newSystem) {
         // the dense fluid mask (level masks, tile lists, the head's mask)
@@ -660,6 +666,8 @@ void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const i
                                  scale_.devPtr(), red_.devPtr(), true, s);
         setInputMasks(s, true);
         nn::launch_zero_f32(z, (long long)numActive * 512, s);
+    } else if (normPairs) {
+        nn::launch_pairs_scale(normPairs, numActive, red_.devPtr(), scale_.devPtr(), s);
     } else {
         nn::launch_input_scale(numActive, cellType, r, red_.devPtr(), scale_.devPtr(), s);
     }

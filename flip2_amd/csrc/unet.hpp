@@ -64,8 +64,10 @@ public:
     // brick -> slot, -1 inactive).  newSystem: the fluid cells may differ
     // from the previous apply's (masks and tile flags are rebuilt, z is
     // zeroed); otherwise only r changed.  z is written at fluid cells only.
+    // normPairs: null, or the per-slot (sum of r^2, fluid count) pairs of r.
     void applyActive(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
-                     const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem, hipStream_t s);
+                     const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem,
+                     const double* normPairs, hipStream_t s);
 
     const std::vector<UNetLayer>& layers() const { return layers_; }
 
@@ -113,6 +115,8 @@ void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, cons
 // *scale = s (as launch_scatter_input) without the scatter.
 void launch_input_scale(uint32_t numActive, const uint8_t* cellType, const float* r, double* red, float* scale,
                         hipStream_t s);
+// The same from `count` precomputed (sum of r^2, count) pairs.
+void launch_pairs_scale(const double* pairs, uint32_t count, double* red, float* scale, hipStream_t s);
 
 // Dense copies of a solver system: b, solution x, operator coefficients
 // (u16 diag | mask << 3, brick-tiled order in, dense [z][y][x] out).
