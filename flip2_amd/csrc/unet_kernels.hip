@@ -225,9 +225,15 @@ __device__ inline int pair_tap(int q, int dy, int half) {
 template <int MB>
 struct EpiIn {
     float bias[MB][4];
-    bool live[4];          // inside the grid and in the voxel mask
-    bool headLive[4];      // head.mask of the voxel (fused head only)
+    bool inside[4];        // the voxel lies inside the grid
+    // Loaded mask bytes, kept as data and tested only in the epilogue (a
+    // test here would wait for the loads, and with them for every store
+    // still in flight): bits 0-7 the voxel mask (1 when there is none),
+    // bits 8-15 head.mask (0 without a fused head).
+    uint32_t m[4];
     uint32_t slotCell[4];  // head output index in the brick-slot layout (head.outSlots)
+    __device__ bool live(int j) const { return inside[j] && (m[j] & 0xffu) != 0u; }          // in the voxel mask
+    __device__ bool headLive(int j) const { return inside[j] && (m[j] & 0xff00u) != 0u; }  // fused head's mask
 };
 
 template <int COUT, int MB>
@@ -239,28 +245,74 @@ __device__ inline EpiIn<MB> epi_load(const float* __restrict__ bias, const uint8
     for (int m = 0; m < MB; ++m)
 #pragma unroll
         for (int r = 0; r < 4; ++r) e.bias[m][r] = bias[16 * (m0 + m) + 4 * kq + r];
+    // Every load is issued unconditionally and only selects depend on the
+    // values: a load under a condition becomes a branch that waits for it on
+    // the spot, and that wait also drains the previous tile's output stores.
+    // Voxels outside the grid read voxel 0; an absent array is replaced by
+    // the bias array (always present), whose value is then ignored.
     const int z = bz + wave, x = bx + col;
+    const bool headOn = COUT == 16 && (head.out || head.outSlots);
+    const bool slotsOn = headOn && head.outSlots;
+    const uint8_t* vmBase = vmask ? vmask : reinterpret_cast<const uint8_t*>(bias);
+    const uint8_t* hmBase = headOn ? head.mask : reinterpret_cast<const uint8_t*>(bias);
+    const int32_t* slBase = slotsOn ? head.brickSlot : reinterpret_cast<const int32_t*>(bias);
+    const int nb = n >> 3;
+    uint8_t vm[4], hm[4];
+    int32_t slot[4];
+    // coordinates clamped into the grid (a voxel outside is never live), so
+    // no index computation depends on a per-lane condition either
+    const uint32_t xs = (uint32_t)min(x, n - 1), zs = (uint32_t)min(z, n - 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t ys = (uint32_t)min(by + j, n - 1);
+        const uint32_t v = (zs * (uint32_t)n + ys) * (uint32_t)n + xs;
+        const uint32_t b = ((zs >> 3) * (uint32_t)nb + (ys >> 3)) * (uint32_t)nb + (xs >> 3);
+        vm[j] = vmBase[vmask ? v : 0u];
+        hm[j] = hmBase[headOn ? v : 0u];
+        slot[j] = slBase[slotsOn ? b : 0u];
+    }
+    // Defaults for absent arrays are applied with bit operations, not
+    // selects: a select between a loaded value and a constant is turned
+    // back into a branch around the load.
+    const uint32_t vmSet = vmask ? 0u : 1u;
+    const uint32_t hmKeep = headOn ? 0xffu : 0u;
+    const uint32_t slKeep = slotsOn ? 0xffffffffu : 0u;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int y = by + j;
-        const bool inside = x < n && y < n && z < n;
-        const long long v = ((long long)z * n + y) * n + x;
-        e.live[j] = inside && (!vmask || vmask[v]);
-        e.headLive[j] = COUT == 16 && (head.out || head.outSlots) && inside && head.mask[v];
-        e.slotCell[j] = 0u;
-        if (COUT == 16 && head.outSlots && e.headLive[j]) {
-            const int nb = n >> 3;
-            const int32_t slot = head.brickSlot[((z >> 3) * nb + (y >> 3)) * nb + (x >> 3)];
-            e.slotCell[j] = (uint32_t)slot * 512u + (uint32_t)(((z & 7) * 8 + (y & 7)) * 8 + (x & 7));
-        }
+        e.inside[j] = x < n && y < n && z < n;
+        e.m[j] = ((uint32_t)vm[j] | vmSet) | (((uint32_t)hm[j] & hmKeep) << 8);
+        const uint32_t cell = (uint32_t)slot[j] * 512u + (uint32_t)(((z & 7) * 8 + (y & 7)) * 8 + (x & 7));
+        e.slotCell[j] = cell & slKeep;
     }
     return e;
+}
+
+// The fused 1x1 head's weights, bias and output scale, loaded once per
+// workgroup (before the first prefetch: a load issued inside the epilogue
+// would wait for the prefetch in flight).
+struct HeadW {
+    float w[16];
+    float b, scale;
+};
+
+template <int COUT>
+__device__ inline HeadW head_load(const HeadArgs& head) {
+    HeadW h{};
+    if (COUT == 16 && (head.out || head.outSlots)) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) h.w[c] = head.w[c];
+        h.b = head.b[0];
+        h.scale = head.scale[0];
+    }
+    return h;
 }
 
 // Epilogue: lane (col, kq) holds output channels 16 (m0 + m) + 4 kq .. +3 of
 // voxel (bx + col, by + j, bz + wave).
 template <int COUT, int MODE, int MB>
-__device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e, uint16_t* __restrict__ Y,
+__device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e, const HeadW& hw,
+                                     uint16_t* __restrict__ Y,
                                      float* __restrict__ Yf, int n, int relu, const HeadArgs& head, int bx, int by,
                                      int bz, int m0, int wave, int col, int kq) {
     const int z = bz + wave, x = bx + col;
@@ -277,7 +329,7 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e
             for (int r = 0; r < 4; ++r) {
                 float t = acc[0][j][r] + e.bias[0][r];
                 if (relu) t = fmaxf(t, 0.f);
-                o[r] = e.live[j] ? f2bf(t) : (uint16_t)0;
+                o[r] = e.live(j) ? f2bf(t) : (uint16_t)0;
                 yv[r] = bf2f(o[r]);
             }
             if (Y && inside) {
@@ -294,11 +346,11 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e
 #pragma unroll
                 for (int r = 0; r < 4; ++r) ch[4 * q + r] = __shfl(yv[r], col + 16 * q, 64);
             if (kq == 0 && inside) {
-                float s = head.b[0];
+                float s = hw.b;
 #pragma unroll
-                for (int c = 0; c < 16; ++c) s = fmaf(ch[c], head.w[c], s);
-                if (!head.outSlots) head.out[v] = e.headLive[j] ? s * head.scale[0] : 0.f;
-                else if (e.headLive[j]) head.outSlots[e.slotCell[j]] = s * head.scale[0];
+                for (int c = 0; c < 16; ++c) s = fmaf(ch[c], hw.w[c], s);
+                if (!head.outSlots) head.out[v] = e.headLive(j) ? s * hw.scale : 0.f;
+                else if (e.headLive(j)) head.outSlots[e.slotCell[j]] = s * hw.scale;
             }
         }
         return;
@@ -321,7 +373,7 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e
             for (int r = 0; r < 4; ++r) {
                 float yv = acc[m][j][r] + e.bias[m][r];
                 if (relu) yv = fmaxf(yv, 0.f);
-                o[r] = e.live[j] ? f2bf(yv) : (uint16_t)0;
+                o[r] = e.live(j) ? f2bf(yv) : (uint16_t)0;
             }
             uint2 packed;
             packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
@@ -350,7 +402,11 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
         return;
     const int m0 = mg * MB;  // first output block of this workgroup
     EpiIn<MB> epi{};
-    if (MODE == 0) epi = epi_load<COUT, MB>(bias, cm.voxel, head, n, bx, by, bz, m0, wave, col, kq);
+    HeadW hw{};
+    if (MODE == 0) {
+        epi = epi_load<COUT, MB>(bias, cm.voxel, head, n, bx, by, bz, m0, wave, col, kq);
+        hw = head_load<COUT>(head);
+    }
 
     v4f acc[MB][4];
 #pragma unroll
@@ -382,7 +438,7 @@ conv3_fwd_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc
         if (c0 + 16 < CIN) load_stage(c0 + 16);
         conv_stage_mfma<CIN, COUT, MB>(sIn, Wt, c0, m0, wave, col, kq, acc);
     }
-    conv_epilogue<COUT, MODE, MB>(acc, epi, Y, Yf, n, relu, head, bx, by, bz, m0, wave, col, kq);
+    conv_epilogue<COUT, MODE, MB>(acc, epi, hw, Y, Yf, n, relu, head, bx, by, bz, m0, wave, col, kq);
 }
 
 // 2x2x2 average pooling fused into a list convolution's epilogue: the
@@ -402,7 +458,7 @@ __device__ inline void pool_tile(const v4f (&acc)[4], const EpiIn<1>& e, int rel
         for (int r = 0; r < 4; ++r) {
             float t = acc[j][r] + e.bias[0][r];
             if (relu) t = fmaxf(t, 0.f);
-            o[r] = e.live[j] ? f2bf(t) : (uint16_t)0;
+            o[r] = e.live(j) ? f2bf(t) : (uint16_t)0;
         }
         uint2 packed;
         packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
@@ -564,6 +620,11 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
     unit_origin(u, bx, by, bz);
     load_step(u, bx, by, bz, 0);
     store_step(0);
+    // Tile of the next unit, fetched one unit ahead so that the prefetch of
+    // a new tile's halo never waits for its tile id (index 0 when there is
+    // none: the list is not empty here).
+    uint32_t tileNext = tl.compute[u + perXcd < end ? (u + perXcd) / SPLIT : 0u];
+    const HeadW hw = head_load<COUT>(head);
     __syncthreads();
     int buf = 0, stage = 0;
     v4f acc[1][4];
@@ -579,13 +640,21 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
         }
         const bool more = nu < end;
         const int m0 = (int)(u % SPLIT);
-        EpiIn<1> epi{};
-        if (stage == STAGES - 1) epi = epi_load<COUT, 1>(bias, vmask, head, n, bx, by, bz, m0, wave, col, kq);
+        // loaded at every step (only the last stage of a unit uses them): a
+        // conditional load would be tested, and waited for, on the spot
+        const EpiIn<1> epi = epi_load<COUT, 1>(bias, vmask, head, n, bx, by, bz, m0, wave, col, kq);
         int nbx = bx, nby = by, nbz = bz;
         if (more) {
-            if (nu != u) unit_origin(nu, nbx, nby, nbz);
+            if (nu != u) {
+                nbx = (int)(tileNext % tilesX) * kTX;
+                nby = (int)((tileNext % tilesXY) / tilesX) * kTY;
+                nbz = (int)(tileNext / tilesXY) * kTZ;
+            }
             load_step(nu, nbx, nby, nbz, ns);
         }
+        const bool newUnit = more && nu != u;
+        const uint32_t after = nu + perXcd;
+        const uint32_t tileAfter = tl.compute[newUnit && after < end ? after / SPLIT : 0u];
         if constexpr (kFixedW) {
             conv_stage_rows(sIn[buf], [&](int q, int dy) { return aw[q * 3 + dy]; }, wave, col, kq, acc[0]);
         } else {
@@ -600,15 +669,19 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
                 },
                 wave, col, kq, acc[0]);
         }
+        // The prefetched step goes into the other LDS buffer (free since the
+        // last barrier) before the epilogue's global stores are issued: the
+        // wait for the prefetch then does not also wait for those stores.
+        if (more) store_step(buf ^ 1);
         if (stage == STAGES - 1) {
-            conv_epilogue<COUT, 0, 1>(acc, epi, Y, nullptr, n, relu, head, bx, by, bz, m0, wave, col, kq);
+            conv_epilogue<COUT, 0, 1>(acc, epi, hw, Y, nullptr, n, relu, head, bx, by, bz, m0, wave, col, kq);
             if (P) pool_tile(acc[0], epi, relu, sPool, P, n, COUT, 16 * m0, bx, by, bz, wave, col, kq);
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[0][j] = v4f{0.f, 0.f, 0.f, 0.f};
         }
         if (!more) break;
-        store_step(buf ^ 1);
         __syncthreads();
+        if (newUnit) tileNext = tileAfter;
         buf ^= 1;
         u = nu;
         stage = ns;
@@ -749,30 +822,37 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
         }
         a[kb] = __builtin_bit_cast(v8bf, make_uint4(wq[0], wq[1], wq[2], wq[3]));
     }
+    // Prefetched halo: the packed input (dense input), or the cell type and
+    // residual of each voxel (slot input), converted when stored to LDS.
+    // Loads are unconditional (clamped index) with masks applied by bit
+    // operations: a load under a per-lane condition is waited for on the
+    // spot, and that wait also drains the previous tile's output stores.
     uint32_t pre[kIn2Loads];
+    float preR[kIn2Loads];
     const uint32_t one = (uint32_t)f2bf(1.0f) << 16;
+    const float scaleIn = si.r ? si.scale[0] : 1.f;
     auto load_tile = [&](int bx, int by, int bz) {
         if (si.r) {
             // brick slots first, then the residual and cell type of each voxel
             int32_t slot[kIn2Loads];
             uint32_t cell[kIn2Loads];
+            bool in[kIn2Loads];
 #pragma unroll
             for (int k = 0; k < kIn2Loads; ++k) {
                 const int t = (int)threadIdx.x + 256 * k;
                 const int hx = t % kHX, hy = (t / kHX) % kHY, hz = t / (kHX * kHY);
                 const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
-                const bool in = t < kHalo && x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n;
+                in[k] = t < kHalo && x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n;
                 const int nb = n >> 3;
-                slot[k] = in ? si.brickSlot[((z >> 3) * nb + (y >> 3)) * nb + (x >> 3)] : -1;
-                cell[k] = in ? (uint32_t)(((z & 7) * 8 + (y & 7)) * 8 + (x & 7)) : 0u;
+                slot[k] = si.brickSlot[in[k] ? ((z >> 3) * nb + (y >> 3)) * nb + (x >> 3) : 0];
+                cell[k] = in[k] ? (uint32_t)(((z & 7) * 8 + (y & 7)) * 8 + (x & 7)) : 0u;
             }
 #pragma unroll
             for (int k = 0; k < kIn2Loads; ++k) {
-                pre[k] = 0u;
-                if (slot[k] >= 0) {
-                    const uint32_t c = (uint32_t)slot[k] * 512u + cell[k];
-                    if (si.cellType[c] == 1) pre[k] = (uint32_t)f2bf(si.r[c] / si.scale[0]) | one;
-                }
+                const bool valid = in[k] && slot[k] >= 0;
+                const uint32_t c = valid ? (uint32_t)slot[k] * 512u + cell[k] : 0u;  // slot 0 exists
+                pre[k] = (uint32_t)si.cellType[c] & (valid ? 0xffu : 0u);
+                preR[k] = si.r[c];
             }
             return;
         }
@@ -782,20 +862,24 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
             const int hx = t % kHX, hy = (t / kHX) % kHY, hz = t / (kHX * kHY);
             const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
             const bool in = t < kHalo && x >= 0 && y >= 0 && z >= 0 && x < n && y < n && z < n;
-            pre[k] = in ? reinterpret_cast<const uint32_t*>(X2)[((long long)z * n + y) * n + x] : 0u;
+            pre[k] = reinterpret_cast<const uint32_t*>(X2)[in ? ((long long)z * n + y) * n + x : 0] &
+                     (in ? 0xffffffffu : 0u);
         }
     };
     auto store_tile = [&](int b) {
 #pragma unroll
         for (int k = 0; k < kIn2Loads; ++k) {
             const int t = (int)threadIdx.x + 256 * k;
-            if (t < kHalo) sIn[b][t] = pre[k];
+            const uint32_t v = si.r ? (pre[k] == 1u ? ((uint32_t)f2bf(preR[k] / scaleIn) | one) : 0u) : pre[k];
+            if (t < kHalo) sIn[b][t] = v;
         }
     };
     int bx, by, bz;
     tile_origin(tl.compute[u], bx, by, bz);
     load_tile(bx, by, bz);
     store_tile(0);
+    // next tile's id fetched one tile ahead (index 0 when there is none)
+    uint32_t tileNext = tl.compute[u + perXcd < end ? u + perXcd : 0u];
     __syncthreads();
     int buf = 0;
     for (;;) {
@@ -805,9 +889,10 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
         const EpiIn<1> epi = epi_load<16, 1>(bias, vmask, HeadArgs{}, n, bx, by, bz, 0, wave, col, kq);
         int nbx = bx, nby = by, nbz = bz;
         if (more) {
-            tile_origin(tl.compute[nu], nbx, nby, nbz);
+            tile_origin(tileNext, nbx, nby, nbz);
             load_tile(nbx, nby, nbz);
         }
+        const uint32_t tileAfter = tl.compute[more && nu + perXcd < end ? nu + perXcd : 0u];
         v4f acc[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -826,6 +911,8 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
                 acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kb], b, acc[j], 0, 0, 0);
             }
         }
+        // next tile into the other buffer before this tile's output stores
+        if (more) store_tile(buf ^ 1);
         const int z = bz + wave, x = bx + col;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -837,7 +924,7 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
             for (int r = 0; r < 4; ++r) {
                 float yv = acc[j][r] + epi.bias[0][r];
                 if (relu) yv = fmaxf(yv, 0.f);
-                o[r] = epi.live[j] ? f2bf(yv) : (uint16_t)0;
+                o[r] = epi.live(j) ? f2bf(yv) : (uint16_t)0;
             }
             uint2 packed;
             packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
@@ -845,8 +932,8 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
             *reinterpret_cast<uint2*>(Y + v * 16 + 4 * kq) = packed;
         }
         if (!more) break;
-        store_tile(buf ^ 1);
         __syncthreads();
+        tileNext = tileAfter;
         buf ^= 1;
         u = nu;
         bx = nbx;
