@@ -176,6 +176,9 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_get_particles": [_vp] * 7,
         "flip2_fluid_set_solid": [_vp, _vp],
         "flip2_fluid_set_solid_box": [_vp, _vp, _vp, _vp],
+        "flip2_fluid_num_particles": [_vp, _vp],
+        "flip2_fluid_save_snapshot": [_vp, ctypes.c_char_p],
+        "flip2_fluid_load_snapshot": [_vp, ctypes.c_char_p],
         "flip2_fluid_step": [_vp],
         "flip2_fluid_stage": [_vp, ctypes.c_char_p],
         "flip2_fluid_solve_with": [_vp, ctypes.c_int, ctypes.POINTER(SolveStatsC)],
@@ -450,7 +453,14 @@ class FluidSim:
         self.n = a[0].size
         check(lib().flip2_fluid_set_particles(self._h, self.n, *(_ptr(v) for v in a)))
 
+    def num_particles(self) -> int:
+        n = ctypes.c_uint32()
+        check(lib().flip2_fluid_num_particles(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     def particles(self):
+        # sized by the library's own count (a snapshot load changes it)
+        self.n = self.num_particles()
         out = [np.empty(self.n, np.float32) for _ in range(6)]
         check(lib().flip2_fluid_get_particles(self._h, *(_ptr(v) for v in out)))
         return out
@@ -469,6 +479,15 @@ class FluidSim:
             return
         a = [np.ascontiguousarray(v, dtype=np.float32) for v in (lo, hi, vel)]
         check(lib().flip2_fluid_set_solid_box(self._h, *(_ptr(v) for v in a)))
+
+    def save_snapshot(self, path: str) -> None:
+        """Write the state needed to resume the run (see include/flip2.h)."""
+        check(lib().flip2_fluid_save_snapshot(self._h, os.fspath(path).encode()))
+
+    def load_snapshot(self, path: str) -> None:
+        """Restore a state written by save_snapshot (same grid dimensions)."""
+        check(lib().flip2_fluid_load_snapshot(self._h, os.fspath(path).encode()))
+        self.n = self.num_particles()
 
     def step(self):
         check(lib().flip2_fluid_step(self._h))

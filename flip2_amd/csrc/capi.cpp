@@ -670,6 +670,27 @@ int flip2_fluid_set_solid_box(flip2_fluid* f, const float* lo, const float* hi, 
         f->impl.setSolidBox(lo, hi, vel);
     });
 }
+int flip2_fluid_num_particles(flip2_fluid* f, uint32_t* n) {
+    return guarded([&] {
+        NEED(f);
+        NEED(n);
+        *n = f->impl.numParticles();
+    });
+}
+int flip2_fluid_save_snapshot(flip2_fluid* f, const char* path) {
+    return guarded([&] {
+        NEED(f);
+        NEED(path);
+        f->impl.saveSnapshot(path);
+    });
+}
+int flip2_fluid_load_snapshot(flip2_fluid* f, const char* path) {
+    return guarded([&] {
+        NEED(f);
+        NEED(path);
+        f->impl.loadSnapshot(path);
+    });
+}
 int flip2_fluid_step(flip2_fluid* f) {
     return guarded([&] {
         NEED(f);

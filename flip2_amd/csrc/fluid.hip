@@ -1,4 +1,6 @@
 // North-star FLIP step driver (see fluid.hpp).
+#include <cstring>
+#include <cstdio>
 #include "fluid.hpp"
 
 #include "scan_impl.hpp"
@@ -108,6 +110,116 @@ void FluidSim::setSolidBox(const float* lo, const float* hi, const float* vel) {
         box_.active = 1;
     }
     rebuildSolid();
+}
+
+namespace {
+// Snapshot layout (little-endian): magic, version, Grid (28 bytes), dt,
+// gravity, flip, margin (float), tolerance (double), preconditioner,
+// maxIterations, checkEvery, n (uint32), 2 n float4 particle records,
+// static-mask flag (uint8) and, when set, the mask in the simulation's
+// brick-tiled cell order (one byte per cell), then the solid box.
+constexpr char kSnapMagic[8] = {'F', 'L', 'I', 'P', '2', 'N', 'S', '1'};
+constexpr uint32_t kSnapVersion = 1;
+
+struct SnapFile {
+    std::FILE* f = nullptr;
+    SnapFile(const std::string& path, const char* mode) : f(std::fopen(path.c_str(), mode)) {
+        FLIP2_REQUIRE(f != nullptr, kErrArgument, "snapshot: cannot open " + path);
+    }
+    ~SnapFile() {
+        if (f) std::fclose(f);
+    }
+    void put(const void* p, size_t bytes) {
+        FLIP2_REQUIRE(std::fwrite(p, 1, bytes, f) == bytes, kErrArgument, "snapshot: write failed");
+    }
+    void get(void* p, size_t bytes) {
+        FLIP2_REQUIRE(std::fread(p, 1, bytes, f) == bytes, kErrArgument, "snapshot: truncated file");
+    }
+    template <class T>
+    void put(const T& v) {
+        put(&v, sizeof(T));
+    }
+    template <class T>
+    T get() {
+        T v;
+        get(&v, sizeof(T));
+        return v;
+    }
+};
+}  // namespace
+
+void FluidSim::saveSnapshot(const std::string& path) {
+    std::vector<float4> rec(2 * (size_t)n_);
+    if (n_) rec_.downloadTo(rec.data(), 2 * n_, stream_);
+    std::vector<uint8_t> mask;
+    if (hasStatic_) {
+        mask.resize(numCells_);
+        solidStatic_.downloadTo(mask.data(), numCells_, stream_);
+    }
+    synchronize();
+    SnapFile out(path, "wb");
+    out.put(kSnapMagic, sizeof(kSnapMagic));
+    out.put(kSnapVersion);
+    out.put(cfg_.grid);
+    out.put(cfg_.dt);
+    out.put(cfg_.gravity);
+    out.put(cfg_.flip);
+    out.put(cfg_.margin);
+    out.put(cfg_.tolerance);
+    out.put((uint32_t)cfg_.preconditioner);
+    out.put(cfg_.maxIterations);
+    out.put(cfg_.checkEvery);
+    out.put(n_);
+    if (n_) out.put(rec.data(), rec.size() * sizeof(float4));
+    out.put((uint8_t)(hasStatic_ ? 1 : 0));
+    if (hasStatic_) out.put(mask.data(), mask.size());
+    out.put(box_);
+}
+
+void FluidSim::loadSnapshot(const std::string& path) {
+    SnapFile in(path, "rb");
+    char magic[sizeof(kSnapMagic)];
+    in.get(magic, sizeof(magic));
+    FLIP2_REQUIRE(std::memcmp(magic, kSnapMagic, sizeof(magic)) == 0, kErrArgument, "snapshot: not a snapshot file");
+    FLIP2_REQUIRE(in.get<uint32_t>() == kSnapVersion, kErrArgument, "snapshot: unsupported version");
+    FluidConfig cfg = cfg_;
+    cfg.grid = in.get<Grid>();
+    FLIP2_REQUIRE(cfg.grid.sizeX == cfg_.grid.sizeX && cfg.grid.sizeY == cfg_.grid.sizeY &&
+                      cfg.grid.sizeZ == cfg_.grid.sizeZ,
+                  kErrArgument, "snapshot: grid dimensions differ from this simulation's");
+    cfg.dt = in.get<float>();
+    cfg.gravity = in.get<float>();
+    cfg.flip = in.get<float>();
+    cfg.margin = in.get<float>();
+    cfg.tolerance = in.get<double>();
+    const uint32_t prec = in.get<uint32_t>();
+    FLIP2_REQUIRE(prec <= (uint32_t)kPrecLearned, kErrArgument, "snapshot: bad preconditioner");
+    cfg.preconditioner = (int)prec;
+    cfg.maxIterations = in.get<uint32_t>();
+    cfg.checkEvery = in.get<uint32_t>();
+    const uint32_t n = in.get<uint32_t>();
+    FLIP2_REQUIRE(n <= maxParticles_, kErrRange, "snapshot: more particles than the simulation was created for");
+    std::vector<float4> rec(2 * (size_t)n);
+    if (n) in.get(rec.data(), rec.size() * sizeof(float4));
+    const bool stat = in.get<uint8_t>() != 0;
+    std::vector<uint8_t> mask;
+    if (stat) {
+        mask.resize(numCells_);
+        in.get(mask.data(), mask.size());
+    }
+    const ns::SolidBox box = in.get<ns::SolidBox>();
+    // everything read: apply
+    setConfig(cfg);
+    n_ = n;
+    if (n) rec_.uploadFrom(rec.data(), 2 * n, stream_);
+    hasStatic_ = stat;
+    if (stat) {
+        solidStatic_.resizeAsync(numCells_, stream_);
+        solidStatic_.uploadFrom(mask.data(), numCells_, stream_);
+    }
+    box_ = box;
+    rebuildSolid();
+    synchronize();
 }
 
 void FluidSim::rebuildSolid() {

@@ -6,6 +6,7 @@
 //   -> pressure-gradient update -> FLIP G2P -> RK2 advection.
 #pragma once
 
+#include <string>
 #include "common.hpp"
 #include "device.hpp"
 #include "ns_solver.hpp"
@@ -87,6 +88,13 @@ public:
     // carrying velocity vel; replaces the previous box.  Null lo removes it.
     void setSolidBox(const float* lo, const float* hi, const float* vel);
     void setConfig(const FluidConfig& cfg);
+    // Binary snapshot of the simulation state (grid, physical settings,
+    // particle records in their current order, static solid mask, solid
+    // box) for resuming a run: loading it into a simulation of the same grid
+    // and stepping gives the same particles, bit for bit, as stepping the
+    // saved one.  The learned preconditioner's settings are not part of it.
+    void saveSnapshot(const std::string& path);
+    void loadSnapshot(const std::string& path);
     const FluidConfig& config() const { return cfg_; }
     const float4* recordsDev() const { return rec_.devPtr(); }
 

@@ -261,6 +261,10 @@ int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg);
 int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const float* y, const float* z,
                               const float* vx, const float* vy, const float* vz);
 int flip2_fluid_get_particles(flip2_fluid* f, float* x, float* y, float* z, float* vx, float* vy, float* vz);
+/* Number of particles the simulation holds (set by flip2_fluid_set_particles
+ * or flip2_fluid_load_snapshot): the length of each flip2_fluid_get_particles
+ * output array. */
+int flip2_fluid_num_particles(flip2_fluid* f, uint32_t* n);
 /* Static solid cells: n^3 bytes indexed x + y*nx + z*nx*ny (NULL clears). */
 int flip2_fluid_set_solid(flip2_fluid* f, const uint8_t* mask);
 /* A moving solid box (e.g. a wave paddle): cells whose centre lies in
@@ -269,6 +273,16 @@ int flip2_fluid_set_solid(flip2_fluid* f, const uint8_t* mask);
  * moves it; lo == NULL removes the box.  No counterpart in the reference
  * (its solids are the low domain walls). */
 int flip2_fluid_set_solid_box(flip2_fluid* f, const float* lo, const float* hi, const float* vel);
+/* Binary snapshot of the simulation state for resuming a run: grid, the
+ * physical settings of the configuration, the particle records in their
+ * current order, the static solid mask and the solid box.  Loading it into
+ * a simulation of the same grid dimensions (and at least as many particles)
+ * and stepping gives the same particles, bit for bit, as stepping the saved
+ * one; the learned preconditioner's settings are not part of it.  No
+ * counterpart in the reference, which writes positions only
+ * (writePositionsToFile, particles.cu:817-828) and cannot resume. */
+int flip2_fluid_save_snapshot(flip2_fluid* f, const char* path);
+int flip2_fluid_load_snapshot(flip2_fluid* f, const char* path);
 int flip2_fluid_step(flip2_fluid* f);
 /* One stage: "transfer", "prepare", "system", "solve", "project", "g2p", "advect". */
 int flip2_fluid_stage(flip2_fluid* f, const char* stage);
