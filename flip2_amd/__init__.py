@@ -177,6 +177,8 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_set_solid": [_vp, _vp],
         "flip2_fluid_set_solid_box": [_vp, _vp, _vp, _vp],
         "flip2_fluid_num_particles": [_vp, _vp],
+        "flip2_fluid_get_config": [_vp, ctypes.POINTER(FluidConfigC)],
+        "flip2_fluid_step_frame": [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _vp],
         "flip2_fluid_save_snapshot": [_vp, ctypes.c_char_p],
         "flip2_fluid_load_snapshot": [_vp, ctypes.c_char_p],
         "flip2_fluid_step": [_vp],
@@ -480,6 +482,18 @@ class FluidSim:
         a = [np.ascontiguousarray(v, dtype=np.float32) for v in (lo, hi, vel)]
         check(lib().flip2_fluid_set_solid_box(self._h, *(_ptr(v) for v in a)))
 
+    def step_frame(self, frame_dt: float, cfl: float = 0.5, max_substeps: int = 64) -> int:
+        """Advance by frame_dt in CFL-limited substeps; returns their number."""
+        k = ctypes.c_uint32()
+        check(lib().flip2_fluid_step_frame(self._h, float(frame_dt), float(cfl), int(max_substeps),
+                                           ctypes.byref(k)))
+        return int(k.value)
+
+    def set_dt(self, dt: float) -> None:
+        """Fixed step of step() (the configuration's dt)."""
+        self.cfg.dt = dt
+        check(lib().flip2_fluid_set_config(self._h, ctypes.byref(self.cfg)))
+
     def save_snapshot(self, path: str) -> None:
         """Write the state needed to resume the run (see include/flip2.h)."""
         check(lib().flip2_fluid_save_snapshot(self._h, os.fspath(path).encode()))
@@ -488,6 +502,9 @@ class FluidSim:
         """Restore a state written by save_snapshot (same grid dimensions)."""
         check(lib().flip2_fluid_load_snapshot(self._h, os.fspath(path).encode()))
         self.n = self.num_particles()
+        # the restored settings replace this wrapper's copy (set_dt and
+        # set_preconditioner write the whole configuration back)
+        check(lib().flip2_fluid_get_config(self._h, ctypes.byref(self.cfg)))
 
     def step(self):
         check(lib().flip2_fluid_step(self._h))

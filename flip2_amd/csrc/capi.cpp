@@ -163,6 +163,20 @@ flip2::FluidConfig to_config(const flip2_fluid_config& c) {
     return r;
 }
 
+void from_config(const flip2::FluidConfig& c, flip2_fluid_config* o) {
+    o->grid.negX = c.grid.negX; o->grid.negY = c.grid.negY; o->grid.negZ = c.grid.negZ;
+    o->grid.sizeX = c.grid.sizeX; o->grid.sizeY = c.grid.sizeY; o->grid.sizeZ = c.grid.sizeZ;
+    o->grid.cellSize = c.grid.cellSize;
+    o->dt = c.dt;
+    o->gravity = c.gravity;
+    o->flip = c.flip;
+    o->preconditioner = c.preconditioner;
+    o->tolerance = c.tolerance;
+    o->max_iterations = c.maxIterations;
+    o->check_every = c.checkEvery;
+    o->margin = c.margin;
+}
+
 void to_stats(const flip2::SolveStats& s, flip2_solve_stats* o) {
     o->iterations = s.iterations;
     o->refinements = s.refinements;
@@ -644,6 +658,12 @@ int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg) {
         f->impl.setConfig(c);
     });
 }
+int flip2_fluid_get_config(flip2_fluid* f, flip2_fluid_config* cfg) {
+    return guarded([&] {
+        NEED(f); NEED(cfg);
+        from_config(f->impl.config(), cfg);
+    });
+}
 int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const float* y, const float* z,
                               const float* vx, const float* vy, const float* vz) {
     return guarded([&] {
@@ -696,6 +716,14 @@ int flip2_fluid_step(flip2_fluid* f) {
         NEED(f);
         f->impl.step();
         f->impl.synchronize();
+    });
+}
+int flip2_fluid_step_frame(flip2_fluid* f, double frame_dt, double cfl, uint32_t max_substeps, uint32_t* substeps) {
+    return guarded([&] {
+        NEED(f);
+        const uint32_t k = f->impl.stepFrame(frame_dt, cfl, max_substeps);
+        f->impl.synchronize();
+        if (substeps) *substeps = k;
     });
 }
 int flip2_fluid_stage(flip2_fluid* f, const char* stage) {

@@ -1,4 +1,5 @@
 // North-star FLIP step driver (see fluid.hpp).
+#include <algorithm>
 #include <cstring>
 #include <cstdio>
 #include "fluid.hpp"
@@ -552,6 +553,41 @@ void FluidSim::step() {
     project();
     gridToParticles();
     advect();
+}
+
+uint32_t FluidSim::stepFrame(double frameDt, double cfl, uint32_t maxSubsteps) {
+    FLIP2_REQUIRE(frameDt > 0.0 && cfl > 0.0 && maxSubsteps >= 1, kErrArgument,
+                  "step_frame: frame dt and cfl must be positive, at least one substep");
+    const float saved = cfg_.dt;
+    const double h = (double)cfg_.grid.cellSize;
+    double t = 0.0;
+    uint32_t k = 0;
+    try {
+        while (frameDt - t > 1e-12 * frameDt && k < maxSubsteps) {
+            transfer();
+            uint32_t bits = 0;
+            FLIP2_HIP(hipMemcpyAsync(&bits, pipe_->maxSpeedBits(), sizeof(bits), hipMemcpyDeviceToHost, stream_));
+            synchronize();
+            float vmax;
+            std::memcpy(&vmax, &bits, sizeof(vmax));
+            double dt = frameDt - t;
+            if (vmax > 0.f && k + 1 < maxSubsteps) dt = std::min(dt, cfl * h / (double)vmax);
+            cfg_.dt = (float)dt;
+            prepareFaces();
+            buildSystem();
+            solve();
+            project();
+            gridToParticles();
+            advect();
+            t += dt;
+            ++k;
+        }
+    } catch (...) {
+        cfg_.dt = saved;
+        throw;
+    }
+    cfg_.dt = saved;
+    return k;
 }
 
 size_t FluidSim::getArray(const std::string& name, void* host, size_t capacity) {

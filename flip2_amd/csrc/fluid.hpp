@@ -99,6 +99,12 @@ public:
     const float4* recordsDev() const { return rec_.devPtr(); }
 
     void step();                   // one full step
+    // Advances by frameDt in substeps whose dt keeps every particle within
+    // cfl cells per step on each axis: dt = min(remaining, cfl h / max|v_c|)
+    // with the exact (unsampled) maximum over all particles' velocity
+    // components at the start of the substep; the maxSubsteps-th substep
+    // takes whatever time remains.  Returns the number of substeps.
+    uint32_t stepFrame(double frameDt, double cfl, uint32_t maxSubsteps);
     // Stages (the step is these in order)
     void transfer();
     void prepareFaces();

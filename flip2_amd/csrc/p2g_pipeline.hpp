@@ -39,6 +39,8 @@ public:
     void resetTiming();
     const uint32_t* cellStart() const { return cellStart_.devPtr(); }
     const uint32_t* counts() const { return counts_.devPtr(); }
+    // max |velocity component| of the last run's particles (float bits)
+    const uint32_t* maxSpeedBits() const { return stats_.devPtr(); }
     // Sorted particle records of the last run; a caller may swap them into
     // its own state (DeviceVec::swapDevice).
     DeviceVec<float4>& sortedVec() { return sorted_; }

@@ -258,6 +258,8 @@ int flip2_fluid_destroy(flip2_fluid* f);
 /* Replaces the configuration (the grid must not change); the learned
  * preconditioner's settings made by flip2_fluid_set_unet are kept. */
 int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg);
+/* The current configuration (after flip2_fluid_load_snapshot: the restored one). */
+int flip2_fluid_get_config(flip2_fluid* f, flip2_fluid_config* cfg);
 int flip2_fluid_set_particles(flip2_fluid* f, uint32_t n, const float* x, const float* y, const float* z,
                               const float* vx, const float* vy, const float* vz);
 int flip2_fluid_get_particles(flip2_fluid* f, float* x, float* y, float* z, float* vx, float* vy, float* vz);
@@ -284,6 +286,14 @@ int flip2_fluid_set_solid_box(flip2_fluid* f, const float* lo, const float* hi, 
 int flip2_fluid_save_snapshot(flip2_fluid* f, const char* path);
 int flip2_fluid_load_snapshot(flip2_fluid* f, const char* path);
 int flip2_fluid_step(flip2_fluid* f);
+/* Adaptive time step: advances by frame_dt in substeps with
+ * dt = min(remaining, cfl * h / max|v_c|), max|v_c| the exact maximum over
+ * all particles' velocity components at the start of each substep (the
+ * max_substeps-th substep takes the remaining time); *substeps (may be
+ * NULL) receives their number.  The configured dt is left unchanged.
+ * Replaces the reference's Courant step, dt = 0.7 h_sub / max|v| over a
+ * sampled maximum (getCourantDt, particles.cu:493-498; cudaVec.cu:173-233). */
+int flip2_fluid_step_frame(flip2_fluid* f, double frame_dt, double cfl, uint32_t max_substeps, uint32_t* substeps);
 /* One stage: "transfer", "prepare", "system", "solve", "project", "g2p", "advect". */
 int flip2_fluid_stage(flip2_fluid* f, const char* stage);
 /* Solve the current system with the given preconditioner (the system must
