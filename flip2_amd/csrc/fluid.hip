@@ -570,6 +570,7 @@ uint32_t FluidSim::stepFrame(double frameDt, double cfl, uint32_t maxSubsteps) {
             synchronize();
             float vmax;
             std::memcpy(&vmax, &bits, sizeof(vmax));
+            FLIP2_REQUIRE(std::isfinite(vmax), kErrState, "step_frame: a particle velocity is not finite");
             double dt = frameDt - t;
             if (vmax > 0.f && k + 1 < maxSubsteps) dt = std::min(dt, cfl * h / (double)vmax);
             cfg_.dt = (float)dt;
