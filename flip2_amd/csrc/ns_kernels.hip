@@ -136,25 +136,22 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ a, uint32_t count) {
     if (i < count) a[i] = 0u;
 }
 
-// counts[0 .. numCells] from the sorted keys, every entry written once: the
-// first element of each run of equal keys writes its cell's run length and
-// zeros for the empty cells between the previous run's cell and its own (the
-// last run also for the cells after it, through counts[numCells]).  The
-// threads of a wave cover adjacent cell ranges, so the writes are contiguous
-// and no separate zero fill is needed.  Keys >= numCells count as numCells.
+// counts[0 .. numCells] from the sorted keys, after a zero fill of the whole
+// array: the first element of each run of equal keys writes its cell's run
+// length.  (Filling the empty cells between runs from the run starts instead
+// serialises a sparse scene's long empty stretches -- tens of millions of
+// cells between two fluid bodies -- into single threads.)  Keys >= numCells
+// count as numCells, which keeps 0.
 __global__ void run_length_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t numCells,
                                   uint32_t* __restrict__ counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = min(keys[i], numCells);
-    const uint32_t prev = i > 0 ? min(keys[i - 1], numCells) : 0u;
-    if (i > 0 && prev == k) return;
-    for (uint32_t c = i > 0 ? prev + 1 : 0u; c < k; ++c) counts[c] = 0u;
+    if (k >= numCells) return;
+    if (i > 0 && min(keys[i - 1], numCells) == k) return;
     uint32_t e = i + 1;
     while (e < n && min(keys[e], numCells) == k) ++e;
-    if (k < numCells) counts[k] = e - i;
-    if (e == n)
-        for (uint32_t c = k < numCells ? k + 1 : numCells; c <= numCells; ++c) counts[c] = 0u;
+    counts[k] = e - i;
 }
 
 // Statistics for a P2G call whose inputs did not come through the pipeline:
@@ -470,6 +467,8 @@ void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCell
         FLIP2_LAUNCH_CHECK();
         return;
     }
+    hipLaunchKernelGGL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
+    FLIP2_LAUNCH_CHECK();
     hipLaunchKernelGGL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts);
     FLIP2_LAUNCH_CHECK();
 }
