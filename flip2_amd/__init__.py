@@ -208,6 +208,10 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_set_flex_directions": [_vp, _u32],
         "flip2_fluid_set_flex_check_every": [_vp, _u32],
         "flip2_fluid_set_learned_graph": [_vp, ctypes.c_int],
+        "flip2_fluid_enable_timing": [_vp, ctypes.c_char_p],
+        "flip2_fluid_disable_timing": [_vp],
+        "flip2_fluid_timing": [_vp, ctypes.c_int, ctypes.c_char_p, _sz, ctypes.POINTER(_f64),
+                               ctypes.POINTER(ctypes.c_uint64)],
     }
     for name in ("align_particles_to_grid", "sort_particles", "generate_voxels", "particle_vel_to_voxels",
                  "calc_div_u", "pressure_solve", "update_voxel_velocities", "voxel_vels_to_particles",
@@ -539,6 +543,28 @@ class FluidSim:
     def set_flex_check_every(self, iterations: int) -> None:
         """Iterations (1..1024) between residual checks in flexible CG."""
         check(lib().flip2_fluid_set_flex_check_every(self._h, int(iterations)))
+
+    def enable_timing(self, kernels: Optional[Sequence[str]] = None) -> None:
+        """Time every launch (or those of the named kernels) of the following
+        steps, stages and solves with HIP events on the simulation's stream."""
+        arg = ",".join(kernels).encode() if kernels else None
+        check(lib().flip2_fluid_enable_timing(self._h, arg))
+
+    def disable_timing(self) -> None:
+        check(lib().flip2_fluid_disable_timing(self._h))
+
+    def timing(self) -> dict:
+        """{kernel: (total ms, launches)} since enable_timing."""
+        out, k = {}, 0
+        while True:
+            name = ctypes.create_string_buffer(128)
+            ms, cnt = _f64(), ctypes.c_uint64()
+            rc = lib().flip2_fluid_timing(self._h, k, name, 128, ctypes.byref(ms), ctypes.byref(cnt))
+            if rc != 0:
+                break
+            out[name.value.decode()] = (ms.value, int(cnt.value))
+            k += 1
+        return out
 
     def get(self, name: str) -> np.ndarray:
         cnt = _sz()

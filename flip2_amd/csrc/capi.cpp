@@ -3,8 +3,10 @@
 #include "../../include/flip2.h"
 
 #include <cstring>
+#include <iterator>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "fluid.hpp"
@@ -23,7 +25,10 @@ struct flip2_p2g_pipeline {
 
 struct flip2_fluid {
     flip2::FluidSim impl;
+    flip2::KernelTimer timer;  // per-kernel event timing of the calls below (flip2_fluid_enable_timing)
+    bool timing = false;
     flip2_fluid(uint32_t n, const flip2::FluidConfig& c) : impl(n, c) {}
+    flip2::KernelTimer* activeTimer() { return timing ? &timer : nullptr; }
 };
 
 struct flip2_unet {
@@ -158,8 +163,7 @@ flip2::FluidConfig to_config(const flip2_fluid_config& c) {
     r.maxIterations = c.max_iterations;
     r.checkEvery = c.check_every ? c.check_every : 8;
     r.margin = c.margin;
-    FLIP2_REQUIRE(c.dt > 0.0f && c.tolerance > 0.0 && c.max_iterations > 0, flip2::kErrArgument, "invalid config");
-    FLIP2_REQUIRE(c.preconditioner >= 0 && c.preconditioner <= 2, flip2::kErrArgument, "unknown preconditioner");
+    // the simulation validates every field (FluidSim's constructor and setConfig)
     return r;
 }
 
@@ -714,6 +718,7 @@ int flip2_fluid_load_snapshot(flip2_fluid* f, const char* path) {
 int flip2_fluid_step(flip2_fluid* f) {
     return guarded([&] {
         NEED(f);
+        const flip2::KernelTimer::Install timed(f->activeTimer());
         f->impl.step();
         f->impl.synchronize();
     });
@@ -721,6 +726,7 @@ int flip2_fluid_step(flip2_fluid* f) {
 int flip2_fluid_step_frame(flip2_fluid* f, double frame_dt, double cfl, uint32_t max_substeps, uint32_t* substeps) {
     return guarded([&] {
         NEED(f);
+        const flip2::KernelTimer::Install timed(f->activeTimer());
         const uint32_t k = f->impl.stepFrame(frame_dt, cfl, max_substeps);
         f->impl.synchronize();
         if (substeps) *substeps = k;
@@ -729,6 +735,7 @@ int flip2_fluid_step_frame(flip2_fluid* f, double frame_dt, double cfl, uint32_t
 int flip2_fluid_stage(flip2_fluid* f, const char* stage) {
     return guarded([&] {
         NEED(f); NEED(stage);
+        const flip2::KernelTimer::Install timed(f->activeTimer());
         const std::string s(stage);
         if (s == "transfer") f->impl.transfer();
         else if (s == "prepare") f->impl.prepareFaces();
@@ -744,6 +751,7 @@ int flip2_fluid_stage(flip2_fluid* f, const char* stage) {
 int flip2_fluid_solve_with(flip2_fluid* f, int preconditioner, flip2_solve_stats* stats) {
     return guarded([&] {
         NEED(f);
+        const flip2::KernelTimer::Install timed(f->activeTimer());
         const flip2::SolveStats s = f->impl.solveWith(preconditioner);
         f->impl.synchronize();
         if (stats) to_stats(s, stats);
@@ -964,6 +972,50 @@ int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iter
                 u->applyActive(na, s->activeBricksDev(), s->brickSlotDev(), s->cellTypeDev(), r, z, s->gridN(),
                                newSystem, pairs, st);
             });
+    });
+}
+
+int flip2_fluid_enable_timing(flip2_fluid* f, const char* kernels) {
+    return guarded([&] {
+        NEED(f);
+        f->timer.reset();
+        std::vector<std::string> names;
+        if (kernels) {
+            std::string all(kernels), cur;
+            for (char c : all + ",") {
+                if (c == ',') {
+                    if (!cur.empty()) names.push_back(cur);
+                    cur.clear();
+                } else if (c != ' ') {
+                    cur.push_back(c);
+                }
+            }
+        }
+        f->timer.setFilter(names);
+        f->timing = true;
+    });
+}
+int flip2_fluid_disable_timing(flip2_fluid* f) {
+    return guarded([&] {
+        NEED(f);
+        f->timer.collect();
+        f->timing = false;
+    });
+}
+int flip2_fluid_timing(flip2_fluid* f, int k, char* name, size_t name_len, double* total_ms, uint64_t* launches) {
+    return guarded([&] {
+        NEED(f);
+        f->timer.collect();
+        const auto& t = f->timer.totals();
+        FLIP2_REQUIRE(k >= 0 && (size_t)k < t.size(), flip2::kErrRange, "no such timed kernel");
+        auto it = t.begin();
+        std::advance(it, k);
+        if (name && name_len) {
+            std::strncpy(name, it->first.c_str(), name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (total_ms) *total_ms = it->second.ms;
+        if (launches) *launches = it->second.launches;
     });
 }
 

@@ -8,6 +8,8 @@
 #include <stdexcept>
 #include <string>
 
+#include "timing.hpp"
+
 namespace flip2 {
 
 // Status codes returned through the C ABI (negative = error).

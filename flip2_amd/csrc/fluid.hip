@@ -11,8 +11,40 @@
 
 namespace flip2 {
 
+namespace {
+// Largest accepted inner-iteration cap and residual-check interval: the
+// solver keeps 4 doubles of history per inner iteration on the device and
+// reads (check interval + 1) x 4 of them back per check.
+constexpr uint32_t kMaxIterationsCap = 1u << 24;
+constexpr uint32_t kMaxCheckEvery = 1u << 16;
+
+// One validator for every way a configuration enters the simulation
+// (creation, set_config, snapshot load).
+void validate_config(const FluidConfig& c) {
+    FLIP2_REQUIRE(std::isfinite(c.dt) && c.dt > 0.0f, kErrArgument, "config: dt must be positive and finite");
+    FLIP2_REQUIRE(std::isfinite(c.gravity), kErrArgument, "config: gravity must be finite");
+    FLIP2_REQUIRE(std::isfinite(c.flip) && c.flip >= 0.0f && c.flip <= 1.0f, kErrArgument,
+                  "config: FLIP ratio must lie in [0, 1]");
+    FLIP2_REQUIRE(c.preconditioner >= kPrecJacobi && c.preconditioner <= kPrecLearned, kErrArgument,
+                  "config: unknown preconditioner");
+    FLIP2_REQUIRE(std::isfinite(c.tolerance) && c.tolerance > 0.0, kErrArgument,
+                  "config: tolerance must be positive and finite");
+    FLIP2_REQUIRE(c.maxIterations >= 1 && c.maxIterations <= kMaxIterationsCap, kErrArgument,
+                  "config: max_iterations must lie in [1, 2^24]");
+    FLIP2_REQUIRE(c.checkEvery >= 1 && c.checkEvery <= kMaxCheckEvery, kErrArgument,
+                  "config: check_every must lie in [1, 65536]");
+    FLIP2_REQUIRE(c.learnedCheckEvery >= 1 && c.learnedCheckEvery <= kMaxCheckEvery, kErrArgument,
+                  "config: the flexible-CG check interval must lie in [1, 65536]");
+    FLIP2_REQUIRE(std::isfinite(c.margin) && c.margin >= 0.0f && c.margin < 0.5f, kErrArgument,
+                  "config: margin must lie in [0, 0.5)");
+}
+
+// Device history entries the solver needs for a configuration.
+size_t history_entries(const FluidConfig& c) { return 4 * ((size_t)c.maxIterations + 2); }
+}  // namespace
+
 FluidSim::FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device)
-    : cfg_(cfg), dm_(device), maxParticles_(maxParticles) {
+    : cfg_((validate_config(cfg), cfg)), dm_(device), maxParticles_(maxParticles) {
     const Grid& g = cfg.grid;
     FLIP2_REQUIRE(g.sizeX % 8 == 0 && g.sizeY % 8 == 0 && g.sizeZ % 8 == 0 && g.sizeX && g.sizeY && g.sizeZ,
                   kErrArgument, "grid sizes must be positive multiples of 8");
@@ -28,7 +60,7 @@ FluidSim::FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device)
     activeBricks_.resizeAsync(numBricks_, stream_);
     total_.resizeAsync(4, stream_);
     scal_.resizeAsync(16, stream_);
-    hist_.resizeAsync(4 * (cfg.maxIterations + 2), stream_);
+    hist_.resizeAsync((uint32_t)history_entries(cfg), stream_);
     ws_.resizeAsync((uint32_t)scan_workspace_bytes(numBricks_ + 1), stream_);
     synchronize();
 }
@@ -47,8 +79,9 @@ void FluidSim::setConfig(const FluidConfig& cfg) {
     FLIP2_REQUIRE(cfg.grid.sizeX == cfg_.grid.sizeX && cfg.grid.sizeY == cfg_.grid.sizeY &&
                       cfg.grid.sizeZ == cfg_.grid.sizeZ,
                   kErrArgument, "the grid cannot change");
+    validate_config(cfg);
+    if (hist_.size() < history_entries(cfg)) hist_.resizeAsync((uint32_t)history_entries(cfg), stream_);
     cfg_ = cfg;
-    if (hist_.size() < 4 * (cfg.maxIterations + 2)) hist_.resizeAsync(4 * (cfg.maxIterations + 2), stream_);
 }
 
 void FluidSim::setParticles(uint32_t n, const float* x, const float* y, const float* z, const float* vx,
@@ -424,7 +457,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     uint32_t total = 0;
     const uint32_t cap = std::min(capIn, cfg_.maxIterations);
     const uint32_t every = std::max(1u, flex ? cfg_.learnedCheckEvery : cfg_.checkEvery);
-    std::vector<double> h(4 * (every + 1));
+    std::vector<double> h(4 * ((size_t)every + 1));
     double* H = hist_.devPtr();
     // Flexible CG does not restart at a refinement: the recursive residual is
     // replaced by the true one (FP64, rounded to FP32), the correction so far

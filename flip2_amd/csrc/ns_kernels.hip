@@ -80,6 +80,16 @@ __global__ void bin_rec_kernel(const float4* __restrict__ rec, uint32_t n, Grid 
 // orders like the float), folded into *target with at most one atomic per
 // block, and none once *target is already larger (the common case after the
 // first blocks, which keeps the single address uncontended).
+// Bits of max(|vx|, |vy|, |vz|) for the speed statistic.  fmaxf returns its
+// non-NaN operand, so a NaN component is mapped to the quiet-NaN pattern
+// 0x7fc00000 explicitly: as an unsigned integer it is larger than every
+// finite or infinite magnitude, so it wins the u32 maximum and the statistic
+// reads back as NaN.
+__device__ inline uint32_t speed_bits(float vx, float vy, float vz) {
+    if (isnan(vx) || isnan(vy) || isnan(vz)) return 0x7fc00000u;
+    return __float_as_uint(fmaxf(fabsf(vx), fmaxf(fabsf(vy), fabsf(vz))));
+}
+
 __device__ inline void block_max_to(uint32_t v, uint32_t* target) {
     __shared__ uint32_t s_part[16];
     for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
@@ -101,7 +111,7 @@ __device__ inline void block_max_to(uint32_t v, uint32_t* target) {
 __global__ void gather_rec_kernel(const uint32_t* __restrict__ perm, uint32_t n, const float4* __restrict__ src,
                                   float4* __restrict__ dst, uint32_t* __restrict__ stats) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    float vmax = 0.f;
+    uint32_t vmax = 0u;
     if (i < n) {
         const uint32_t j = perm[i];
         if (j < n) {
@@ -109,10 +119,10 @@ __global__ void gather_rec_kernel(const uint32_t* __restrict__ perm, uint32_t n,
             const float4 b = src[2 * j + 1];
             dst[2 * i] = a;
             dst[2 * i + 1] = b;
-            vmax = fmaxf(fabsf(a.w), fmaxf(fabsf(b.x), fabsf(b.y)));
+            vmax = speed_bits(a.w, b.x, b.y);
         }
     }
-    if (stats) block_max_to(__float_as_uint(vmax), &stats[0]);
+    if (stats) block_max_to(vmax, &stats[0]);
 }
 
 // SoA -> records and back.
@@ -160,13 +170,13 @@ __global__ void p2g_stats_kernel(const float4* __restrict__ rec, const uint32_t*
                                  uint32_t numCells, uint32_t* __restrict__ stats) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n = cellStart[numCells];
-    float vmax = 0.f;
+    uint32_t vmax = 0u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float4 a = rec[2 * i];
         const float4 b = rec[2 * i + 1];
-        vmax = fmaxf(vmax, fmaxf(fabsf(a.w), fmaxf(fabsf(b.x), fabsf(b.y))));
+        vmax = max(vmax, speed_bits(a.w, b.x, b.y));
     }
-    block_max_to(__float_as_uint(vmax), &stats[0]);
+    block_max_to(vmax, &stats[0]);
 }
 
 namespace {
@@ -426,58 +436,58 @@ inline dim3 grid1d(uint32_t n, uint32_t bs) { return dim3(ceil_div(n ? n : 1, bs
 
 void launch_bin(const float* x, const float* y, const float* z, uint32_t n, Grid g, uint32_t* keys, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(bin_kernel, grid1d(n, 256), dim3(256), 0, s, x, y, z, n, g, keys);
+    FLIP2_KERNEL(bin_kernel, grid1d(n, 256), dim3(256), 0, s, x, y, z, n, g, keys);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, uint32_t* stats, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(bin_pack_kernel, grid1d(n, 256), dim3(256), 0, s, src, n, g, keys, rec, stats);
+    FLIP2_KERNEL(bin_pack_kernel, grid1d(n, 256), dim3(256), 0, s, src, n, g, keys, rec, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint32_t* stats, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(bin_rec_kernel, grid1d(n, 256), dim3(256), 0, s, rec, n, g, keys, stats);
+    FLIP2_KERNEL(bin_rec_kernel, grid1d(n, 256), dim3(256), 0, s, rec, n, g, keys, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, uint32_t* stats,
                        hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(gather_rec_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, src, dst, stats);
+    FLIP2_KERNEL(gather_rec_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, src, dst, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_pack(Float6 src, uint32_t n, float4* rec, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(pack_kernel, grid1d(n, 256), dim3(256), 0, s, src, n, rec);
+    FLIP2_KERNEL(pack_kernel, grid1d(n, 256), dim3(256), 0, s, src, n, rec);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_unpack(const float4* rec, uint32_t n, Float6Out dst, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(unpack_kernel, grid1d(n, 256), dim3(256), 0, s, rec, n, dst);
+    FLIP2_KERNEL(unpack_kernel, grid1d(n, 256), dim3(256), 0, s, rec, n, dst);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_cell_counts(const uint32_t* sortedKeys, uint32_t n, uint32_t numCells, uint32_t* counts, hipStream_t s) {
     if (!n) {
-        hipLaunchKernelGGL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
+        FLIP2_KERNEL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
         FLIP2_LAUNCH_CHECK();
         return;
     }
-    hipLaunchKernelGGL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
+    FLIP2_KERNEL(zero_u32_kernel, grid1d(numCells + 1, 256), dim3(256), 0, s, counts, numCells + 1);
     FLIP2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts);
+    FLIP2_KERNEL(run_length_kernel, grid1d(n, 256), dim3(256), 0, s, sortedKeys, n, numCells, counts);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_p2g_stats(const float4* sortedRec, const uint32_t* cellStart, uint32_t numCells, uint32_t* stats,
                       hipStream_t s) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, stats, 1u);
+    FLIP2_KERNEL(zero_u32_kernel, dim3(1), dim3(64), 0, s, stats, 1u);
     FLIP2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(p2g_stats_kernel, dim3(1024), dim3(256), 0, s, sortedRec, cellStart, numCells, stats);
+    FLIP2_KERNEL(p2g_stats_kernel, dim3(1024), dim3(256), 0, s, sortedRec, cellStart, numCells, stats);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -489,7 +499,7 @@ void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, cons
     const bool pow2 = (nbX & (nbX - 1)) == 0;
     const int morton = (nbX == nbY && nbY == nbZ && pow2 && nbX <= 1024) ? 1 : 0;
     const uint32_t blocks = ((numBricks + 7u) >> 3) * 8u;
-    hipLaunchKernelGGL(p2g_brick_kernel, dim3(blocks), dim3(256), 0, s, sortedRec, cellStart, g, numBricks, morton,
+    FLIP2_KERNEL(p2g_brick_kernel, dim3(blocks), dim3(256), 0, s, sortedRec, cellStart, g, numBricks, morton,
                        stats, U, V, W, WU, WV, WW);
     FLIP2_LAUNCH_CHECK();
 }

@@ -988,19 +988,19 @@ namespace {
 inline dim3 g1(uint32_t n, uint32_t bs) { return dim3(ceil_div(n ? n : 1, bs)); }
 // out[0..count) = sums of the workgroup partials of the kernel just launched
 void reduce_to(const Reduce& red, int count, uint32_t blocks, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(count), dim3(kWG), 0, s, red.partials, blocks, out);
+    FLIP2_KERNEL(reduce_partials_kernel, dim3(count), dim3(kWG), 0, s, red.partials, blocks, out);
     FLIP2_LAUNCH_CHECK();
 }
 }  // namespace
 
 void launch_classify_bricks(const uint32_t* counts, const uint8_t* solid, uint32_t numBricks, uint32_t* flag,
                             hipStream_t s) {
-    hipLaunchKernelGGL(classify_bricks_kernel, dim3(numBricks), dim3(64), 0, s, counts, solid, flag);
+    FLIP2_KERNEL(classify_bricks_kernel, dim3(numBricks), dim3(64), 0, s, counts, solid, flag);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_build_slots(const uint32_t* flag, const uint32_t* excl, uint32_t numBricks, int32_t* brickSlot,
                         uint32_t* activeBricks, hipStream_t s) {
-    hipLaunchKernelGGL(build_slots_kernel, g1(numBricks, 256), dim3(256), 0, s, flag, excl, numBricks, brickSlot,
+    FLIP2_KERNEL(build_slots_kernel, g1(numBricks, 256), dim3(256), 0, s, flag, excl, numBricks, brickSlot,
                        activeBricks);
     FLIP2_LAUNCH_CHECK();
 }
@@ -1008,7 +1008,7 @@ void launch_brick_setup(uint32_t numActive, const uint32_t* activeBricks, const 
                         const uint32_t* counts, const uint8_t* solid, Grid g, int32_t* nbrSlot, uint8_t* cellType,
                         hipStream_t s) {
     if (!numActive) return;
-    hipLaunchKernelGGL(brick_setup_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, brickSlot, counts, solid, g,
+    FLIP2_KERNEL(brick_setup_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, brickSlot, counts, solid, g,
                        nbrSlot, cellType);
     FLIP2_LAUNCH_CHECK();
 }
@@ -1016,56 +1016,56 @@ void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int
                      const uint8_t* solid, Grid g, double hOverDt, const float* U, const float* V, const float* W,
                      uint16_t* coef, double* b64, Reduce red, double* out, hipStream_t s) {
     if (!numActive) return;
-    hipLaunchKernelGGL(assemble_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, nbrSlot, cellType, solid, g,
+    FLIP2_KERNEL(assemble_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, nbrSlot, cellType, solid, g,
                        hOverDt, U, V, W, coef, b64, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 2, numActive, out, s);
 }
 void launch_refine_start(uint32_t numActive, const double* r64, float* r, float* e, const uint16_t* coef, Reduce red,
                          double* out, hipStream_t s) {
-    hipLaunchKernelGGL(refine_start_kernel, dim3(numActive), dim3(kWG), 0, s, r64, r, e, coef, red.partials);
+    FLIP2_KERNEL(refine_start_kernel, dim3(numActive), dim3(kWG), 0, s, r64, r, e, coef, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
 void launch_jacobi_init(uint32_t numActive, const float* r, const uint16_t* coef, float* z, float* p, Reduce red,
                         double* out, hipStream_t s) {
-    hipLaunchKernelGGL(jacobi_init_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, z, p, red.partials);
+    FLIP2_KERNEL(jacobi_init_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, z, p, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
 void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
                        Reduce red, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(matvec_dot_kernel, dim3(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials);
+    FLIP2_KERNEL(matvec_dot_kernel, dim3(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
 void launch_update(uint32_t numActive, const double* rzOld, const double* pq, const float* p, const float* q,
                    const uint16_t* coef, float* e, float* r, float* z, bool jacobi, Reduce red, double* out,
                    hipStream_t s) {
-    hipLaunchKernelGGL(update_kernel, dim3(numActive), dim3(kWG), 0, s, rzOld, pq, p, q, coef, e, r, z,
+    FLIP2_KERNEL(update_kernel, dim3(numActive), dim3(kWG), 0, s, rzOld, pq, p, q, coef, e, r, z,
                        jacobi ? 1 : 0, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 2, numActive, out, s);
 }
 void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(dot_kernel, dim3(numActive), dim3(kWG), 0, s, a, b, red.partials);
+    FLIP2_KERNEL(dot_kernel, dim3(numActive), dim3(kWG), 0, s, a, b, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
 void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
                       bool first, hipStream_t s) {
-    hipLaunchKernelGGL(direction_kernel, dim3(numActive), dim3(kWG), 0, s, rzNew, rzOld, z, p, first ? 1 : 0);
+    FLIP2_KERNEL(direction_kernel, dim3(numActive), dim3(kWG), 0, s, rzNew, rzOld, z, p, first ? 1 : 0);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot,
                       const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s) {
-    hipLaunchKernelGGL(presmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, nbrSlot, cellType, omega, r1,
+    FLIP2_KERNEL(presmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, nbrSlot, cellType, omega, r1,
                        pairs);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const uint16_t* coef, const int32_t* nbrSlot,
                        float omega, float* z, hipStream_t s) {
-    hipLaunchKernelGGL(postsmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, y, coef, nbrSlot, omega, z);
+    FLIP2_KERNEL(postsmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, y, coef, nbrSlot, omega, z);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& d, Reduce red, double* out,
@@ -1074,7 +1074,7 @@ void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& 
     if (d.count == 0) return;
     switch (d.count) {
 #define FLEX_CASE(N) \
-    case N: hipLaunchKernelGGL(flex_dots_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, red.partials); break;
+    case N: FLIP2_KERNEL(flex_dots_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, red.partials); break;
         FLEX_CASE(1) FLEX_CASE(2) FLEX_CASE(3) FLEX_CASE(4) FLEX_CASE(5) FLEX_CASE(6) FLEX_CASE(7) FLEX_CASE(8)
         FLEX_CASE(9) FLEX_CASE(10) FLEX_CASE(11) FLEX_CASE(12) FLEX_CASE(13) FLEX_CASE(14) FLEX_CASE(15) FLEX_CASE(16)
 #undef FLEX_CASE
@@ -1091,7 +1091,7 @@ void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirecti
     switch (d.count) {
 #define FLEX_CASE(N)                                                                                              \
     case N:                                                                                                       \
-        hipLaunchKernelGGL(flex_direction_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, zq, r, p, red.partials); \
+        FLIP2_KERNEL(flex_direction_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, zq, r, p, red.partials); \
         break;
         FLEX_CASE(0) FLEX_CASE(1) FLEX_CASE(2) FLEX_CASE(3) FLEX_CASE(4) FLEX_CASE(5) FLEX_CASE(6) FLEX_CASE(7)
         FLEX_CASE(8) FLEX_CASE(9) FLEX_CASE(10) FLEX_CASE(11) FLEX_CASE(12) FLEX_CASE(13) FLEX_CASE(14) FLEX_CASE(15)
@@ -1103,33 +1103,33 @@ void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirecti
 }
 
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s) {
-    hipLaunchKernelGGL(accumulate_kernel, dim3(numActive), dim3(kWG), 0, s, e, x64, coef);
+    FLIP2_KERNEL(accumulate_kernel, dim3(numActive), dim3(kWG), 0, s, e, x64, coef);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(true_residual_kernel, dim3(numActive), dim3(kWG), 0, s, b64, x64, coef, nbrSlot, r64,
+    FLIP2_KERNEL(true_residual_kernel, dim3(numActive), dim3(kWG), 0, s, b64, x64, coef, nbrSlot, r64,
                        red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
 void launch_fill_f32(float* a, uint32_t count, float v, hipStream_t s) {
     if (!count) return;
-    hipLaunchKernelGGL(fill_f32_kernel, g1(count, 256), dim3(256), 0, s, a, count, v);
+    FLIP2_KERNEL(fill_f32_kernel, g1(count, 256), dim3(256), 0, s, a, count, v);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_fill_f64(double* a, uint32_t count, double v, hipStream_t s) {
     if (!count) return;
-    hipLaunchKernelGGL(fill_f64_kernel, g1(count, 256), dim3(256), 0, s, a, count, v);
+    FLIP2_KERNEL(fill_f64_kernel, g1(count, 256), dim3(256), 0, s, a, count, v);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, hipStream_t s) {
-    hipLaunchKernelGGL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
+    FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_mic_apply(uint32_t numActive, const float* r, const uint16_t* coef, const float* precon, float* z,
                       Reduce red, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(mic_apply_kernel, dim3(numActive), dim3(64), 0, s, r, coef, precon, z, red.partials);
+    FLIP2_KERNEL(mic_apply_kernel, dim3(numActive), dim3(64), 0, s, r, coef, precon, z, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
@@ -1137,32 +1137,32 @@ void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, co
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,
                             float* U, float* V, float* W, hipStream_t s) {
     if (!numActive) return;
-    hipLaunchKernelGGL(velocity_update_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, nbrSlot, cellType,
+    FLIP2_KERNEL(velocity_update_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, nbrSlot, cellType,
                        solid, g, x64, dtOverH, U, V, W);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_prepare_faces(float* U, float* V, float* W, float* U0, float* V0, float* W0, const uint8_t* solid, Grid g,
                           float gdt, SolidBox box, hipStream_t s) {
     const uint32_t cells = g.sizeX * g.sizeY * g.sizeZ;
-    hipLaunchKernelGGL(prepare_faces_kernel, g1(cells, 256), dim3(256), 0, s, U, V, W, U0, V0, W0, solid, g, gdt, box,
+    FLIP2_KERNEL(prepare_faces_kernel, g1(cells, 256), dim3(256), 0, s, U, V, W, U0, V0, W0, solid, g, gdt, box,
                        cells);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_rasterize_solid(const uint8_t* staticMask, SolidBox box, Grid g, uint8_t* solid, hipStream_t s) {
     const uint32_t cells = g.sizeX * g.sizeY * g.sizeZ;
-    hipLaunchKernelGGL(rasterize_solid_kernel, g1(cells, 256), dim3(256), 0, s, staticMask, box, g, solid, cells);
+    FLIP2_KERNEL(rasterize_solid_kernel, g1(cells, 256), dim3(256), 0, s, staticMask, box, g, solid, cells);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
                      const float* V0, const float* W0, Grid g, float flip, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(g2p_flip_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, U0, V0, W0, g, flip);
+    FLIP2_KERNEL(g2p_flip_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, U0, V0, W0, g, flip);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
                        Grid g, float dt, float margin, SolidBox box, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(advect_rk2_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
+    FLIP2_KERNEL(advect_rk2_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
     FLIP2_LAUNCH_CHECK();
 }
 

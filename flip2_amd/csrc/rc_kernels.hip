@@ -747,33 +747,33 @@ inline dim3 grid1d(uint32_t n, uint32_t bs) { return dim3(ceil_div(n ? n : 1, bs
 
 void launch_root_cell(double* px, double* py, double* pz, uint32_t n, Grid g, uint32_t* keys, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(root_cell_kernel, grid1d(n, 256), dim3(256), 0, s, px, py, pz, n, g, keys);
+    FLIP2_KERNEL(root_cell_kernel, grid1d(n, 256), dim3(256), 0, s, px, py, pz, n, g, keys);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_gather3(const uint32_t* perm, uint32_t n, const double* ax, const double* ay, const double* az,
                     double* bx, double* by, double* bz, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(gather3_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, ax, ay, az, bx, by, bz);
+    FLIP2_KERNEL(gather3_kernel, grid1d(n, 256), dim3(256), 0, s, perm, n, ax, ay, az, bx, by, bz);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_map_nodes(const uint32_t* uniq, uint32_t n, uint32_t* first, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(map_nodes_kernel, grid1d(n, 256), dim3(256), 0, s, uniq, n, first);
+    FLIP2_KERNEL(map_nodes_kernel, grid1d(n, 256), dim3(256), 0, s, uniq, n, first);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_fill_u32(uint32_t* a, uint32_t count, uint32_t value, hipStream_t s) {
     if (!count) return;
-    hipLaunchKernelGGL(fill_u32_kernel, grid1d(count, 256), dim3(256), 0, s, a, count, value);
+    FLIP2_KERNEL(fill_u32_kernel, grid1d(count, 256), dim3(256), 0, s, a, count, value);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_row_table(uint32_t B, const uint32_t* first, const uint32_t* keys, uint32_t* rows, uint32_t sizeX,
                       hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(row_table_kernel, grid1d(B, 256), dim3(256), 0, s, B, first, keys, rows, sizeX);
+    FLIP2_KERNEL(row_table_kernel, grid1d(B, 256), dim3(256), 0, s, B, first, keys, rows, sizeX);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -781,7 +781,7 @@ void launch_mark_voxels(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, co
                         const double* px, const double* py, const double* pz, uint64_t* usedMask,
                         uint64_t* solidMask, uint32_t* counts, uint32_t* oobEvents, hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(mark_voxels_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, usedMask,
+    FLIP2_KERNEL(mark_voxels_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, usedMask,
                        solidMask, counts, oobEvents);
     FLIP2_LAUNCH_CHECK();
 }
@@ -789,7 +789,7 @@ void launch_mark_voxels(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, co
 void launch_compact_voxels(uint32_t B, const uint64_t* usedMask, const uint64_t* solidMask, const uint32_t* nodeOff,
                            uint32_t* voxelIDs, int8_t* solids, hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(compact_voxels_kernel, dim3(B), dim3(64), 0, s, usedMask, solidMask, nodeOff, voxelIDs,
+    FLIP2_KERNEL(compact_voxels_kernel, dim3(B), dim3(64), 0, s, usedMask, solidMask, nodeOff, voxelIDs,
                        solids);
     FLIP2_LAUNCH_CHECK();
 }
@@ -799,14 +799,14 @@ void launch_p2g(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, const uint
                 const uint32_t* nodeOff, const uint32_t* voxelIDs, const int8_t* solids, double* Ux, double* Uy,
                 double* Uz, hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(p2g_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, vx, vy, vz, nodeOff,
+    FLIP2_KERNEL(p2g_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, vx, vy, vz, nodeOff,
                        voxelIDs, solids, Ux, Uy, Uz);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_gravity(uint32_t V, const int8_t* solids, double* Uy, double gdt, bool remove, hipStream_t s) {
     if (!V) return;
-    hipLaunchKernelGGL(gravity_kernel, grid1d(V, 256), dim3(256), 0, s, V, solids, Uy, gdt, remove ? 1 : 0);
+    FLIP2_KERNEL(gravity_kernel, grid1d(V, 256), dim3(256), 0, s, V, solids, Uy, gdt, remove ? 1 : 0);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -814,7 +814,7 @@ void launch_divu(uint32_t B, Grid g, const uint32_t* keys, const uint32_t* first
                  const uint32_t* nodeOff, const uint32_t* voxelIDs, const double* Ux, const double* Uy,
                  const double* Uz, double* divU, hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(divu_kernel, dim3(B), dim3(256), 0, s, B, g, keys, first, rows, nodeOff, voxelIDs, Ux, Uy,
+    FLIP2_KERNEL(divu_kernel, dim3(B), dim3(256), 0, s, B, g, keys, first, rows, nodeOff, voxelIDs, Ux, Uy,
                        Uz, divU);
     FLIP2_LAUNCH_CHECK();
 }
@@ -823,7 +823,7 @@ void launch_sor(uint32_t B, const uint32_t* nodeOff, const uint32_t* voxelIDs, c
                 const double* divU, double* p, double* residuals, double scale, double w, uint32_t iters,
                 hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(sor_kernel, dim3(B), dim3(256), 0, s, nodeOff, voxelIDs, solids, divU, p, residuals, scale,
+    FLIP2_KERNEL(sor_kernel, dim3(B), dim3(256), 0, s, nodeOff, voxelIDs, solids, divU, p, residuals, scale,
                        w, iters);
     FLIP2_LAUNCH_CHECK();
 }
@@ -834,14 +834,14 @@ size_t sampled_absmax_tmp_doubles(uint32_t n, int count) {
 
 void launch_sampled_absmax(Ptr3 arrays, int count, uint32_t n, double* tmp, double* result, hipStream_t s) {
     FLIP2_REQUIRE(count >= 1 && count <= 3, kErrArgument, "sampled max: 1..3 arrays");
-    hipLaunchKernelGGL(sampled_absmax_kernel, dim3(count), dim3(256), 0, s, arrays, n, tmp, result);
+    FLIP2_KERNEL(sampled_absmax_kernel, dim3(count), dim3(256), 0, s, arrays, n, tmp, result);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_velocity_update(uint32_t B, const uint32_t* nodeOff, const uint32_t* voxelIDs, const int8_t* solids,
                             const double* p, double* Ux, double* Uy, double* Uz, double scale, hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(velocity_update_kernel, dim3(B), dim3(256), 0, s, nodeOff, voxelIDs, solids, p, Ux, Uy, Uz,
+    FLIP2_KERNEL(velocity_update_kernel, dim3(B), dim3(256), 0, s, nodeOff, voxelIDs, solids, p, Ux, Uy, Uz,
                        scale);
     FLIP2_LAUNCH_CHECK();
 }
@@ -850,7 +850,7 @@ void launch_g2p(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, const uint
                 const double* py, const double* pz, double* vx, double* vy, double* vz, const uint32_t* nodeOff,
                 const uint32_t* voxelIDs, const double* Ux, const double* Uy, const double* Uz, hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(g2p_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, vx, vy, vz, nodeOff,
+    FLIP2_KERNEL(g2p_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, vx, vy, vz, nodeOff,
                        voxelIDs, Ux, Uy, Uz);
     FLIP2_LAUNCH_CHECK();
 }
@@ -858,7 +858,7 @@ void launch_g2p(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, const uint
 void launch_advect(uint32_t n, double dt, double* px, double* py, double* pz, const double* vx, const double* vy,
                    const double* vz, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(advect_kernel, grid1d(n, 256), dim3(256), 0, s, n, dt, px, py, pz, vx, vy, vz);
+    FLIP2_KERNEL(advect_kernel, grid1d(n, 256), dim3(256), 0, s, n, dt, px, py, pz, vx, vy, vz);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -866,14 +866,14 @@ void launch_move_solids(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, co
                         double* py, double* pz, const uint32_t* nodeOff, const uint32_t* voxelIDs,
                         const int8_t* solids, uint64_t counter, uint32_t* relocated, hipStream_t s) {
     if (!B) return;
-    hipLaunchKernelGGL(move_solids_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, nodeOff,
+    FLIP2_KERNEL(move_solids_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, nodeOff,
                        voxelIDs, solids, counter, relocated);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_philox_kat(const uint32_t* in, uint32_t count, uint32_t* out, hipStream_t s) {
     if (!count) return;
-    hipLaunchKernelGGL(philox_kat_kernel, grid1d(count, 64), dim3(64), 0, s, in, count, out);
+    FLIP2_KERNEL(philox_kat_kernel, grid1d(count, 64), dim3(64), 0, s, in, count, out);
     FLIP2_LAUNCH_CHECK();
 }
 

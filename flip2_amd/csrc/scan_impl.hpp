@@ -223,10 +223,10 @@ void scan_launch(Input input, uint32_t* out, uint32_t n, bool exclusive, bool ve
     FLIP2_REQUIRE(wsBytes >= scan_workspace_bytes(n), kErrArgument, "scan workspace too small");
     uint64_t* status = reinterpret_cast<uint64_t*>(ws);
     uint32_t* counter = reinterpret_cast<uint32_t*>(status + tiles);
-    hipLaunchKernelGGL(scan_clear_kernel<0>, dim3(ceil_div(tiles + 1, 256)), dim3(256), 0, stream, status,
+    FLIP2_KERNEL(scan_clear_kernel<0>, dim3(ceil_div(tiles + 1, 256)), dim3(256), 0, stream, status,
                        tiles, counter);
     FLIP2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(scan_kernel<Input>, dim3(tiles), dim3(kScanThreads), 0, stream, input, out, n,
+    FLIP2_KERNEL(scan_kernel<Input>, dim3(tiles), dim3(kScanThreads), 0, stream, input, out, n,
                        exclusive ? 1 : 0, vec ? 1 : 0, status, counter, totalDev);
     FLIP2_LAUNCH_CHECK();
 }

@@ -318,13 +318,13 @@ void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* ke
     const uint32_t tiles = ceil_div(n, (uint32_t)items * kSortThreads);
     const size_t passStride = (size_t)tiles * kRadix;
     const uint32_t headWords = (uint32_t)(L.statusOff / 4);
-    hipLaunchKernelGGL(sort_clear_kernel, dim3(ceil_div(headWords, 256)), dim3(256), 0, stream,
+    FLIP2_KERNEL(sort_clear_kernel, dim3(ceil_div(headWords, 256)), dim3(256), 0, stream,
                        reinterpret_cast<uint32_t*>(w), headWords);
     FLIP2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(sort_histogram_kernel, dim3(ceil_div(n, kHistChunk)), dim3(kHistThreads), 0, stream, keysIn, n,
+    FLIP2_KERNEL(sort_histogram_kernel, dim3(ceil_div(n, kHistChunk)), dim3(kHistThreads), 0, stream, keysIn, n,
                        beginBit, endBit, L.passes, hist, status, (uint32_t)(L.passes * passStride));
     FLIP2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(sort_offsets_kernel, dim3(1), dim3(64 * kMaxPasses), 0, stream, hist, L.passes, offsets);
+    FLIP2_KERNEL(sort_offsets_kernel, dim3(1), dim3(64 * kMaxPasses), 0, stream, hist, L.passes, offsets);
     FLIP2_LAUNCH_CHECK();
     if (passEvents) FLIP2_HIP(hipEventRecord(passEvents[0], stream));
 
@@ -340,13 +340,13 @@ void sort_pairs_u32(const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* ke
         const uint32_t mask = (1u << bits) - 1u;
         uint32_t* st = status + (size_t)p * passStride;
         if (items == 32)
-            hipLaunchKernelGGL(sort_pass_kernel<32>, dim3(ceil_div(n, 32u * kSortThreads)), dim3(kSortThreads), 0,
+            FLIP2_KERNEL(sort_pass_kernel<32>, dim3(ceil_div(n, 32u * kSortThreads)), dim3(kSortThreads), 0,
                                stream, srcK, srcV, dstK, dstV, n, shift, mask, offsets + p * kRadix, st, counters + p);
         else if (items == 24)
-            hipLaunchKernelGGL(sort_pass_kernel<24>, dim3(ceil_div(n, 24u * kSortThreads)), dim3(kSortThreads), 0,
+            FLIP2_KERNEL(sort_pass_kernel<24>, dim3(ceil_div(n, 24u * kSortThreads)), dim3(kSortThreads), 0,
                                stream, srcK, srcV, dstK, dstV, n, shift, mask, offsets + p * kRadix, st, counters + p);
         else
-            hipLaunchKernelGGL(sort_pass_kernel<16>, dim3(ceil_div(n, 16u * kSortThreads)), dim3(kSortThreads), 0,
+            FLIP2_KERNEL(sort_pass_kernel<16>, dim3(ceil_div(n, 16u * kSortThreads)), dim3(kSortThreads), 0,
                                stream, srcK, srcV, dstK, dstV, n, shift, mask, offsets + p * kRadix, st, counters + p);
         FLIP2_LAUNCH_CHECK();
         if (passEvents) FLIP2_HIP(hipEventRecord(passEvents[p + 1], stream));

@@ -291,13 +291,13 @@ __global__ void dense_postsmooth_kernel(const uint16_t* __restrict__ coefD, cons
 }
 
 void launch_dense_presmooth(const uint16_t* coefD, const float* r, float omega, float* r1, int n, hipStream_t s) {
-    hipLaunchKernelGGL(dense_presmooth_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, r, omega, r1, n);
+    FLIP2_KERNEL(dense_presmooth_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, r, omega, r1, n);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_dense_postsmooth(const uint16_t* coefD, const float* r, const float* y, float omega, float* z, int n,
                              hipStream_t s) {
-    hipLaunchKernelGGL(dense_postsmooth_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, r, y, omega, z,
+    FLIP2_KERNEL(dense_postsmooth_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, r, y, omega, z,
                        n);
     FLIP2_LAUNCH_CHECK();
 }
@@ -307,12 +307,12 @@ void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, cons
     const long long nvox = (long long)n * n * n;
     const uint32_t cells = numActive * 512u;
     if (clear)
-        hipLaunchKernelGGL(clear_input_kernel, g1(nvox, 256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(in0c), mask,
+        FLIP2_KERNEL(clear_input_kernel, g1(nvox, 256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(in0c), mask,
                            nvox);
     const dim3 rg = red_grid(cells);
-    hipLaunchKernelGGL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
-    hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red);
-    hipLaunchKernelGGL(scatter_input_kernel, g1(cells ? cells : 1, 256), dim3(256), 0, s, activeBricks, cellType, r,
+    FLIP2_KERNEL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
+    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red);
+    FLIP2_KERNEL(scatter_input_kernel, g1(cells ? cells : 1, 256), dim3(256), 0, s, activeBricks, cellType, r,
                        cells, n, in0c, mask, scale, red);
     FLIP2_LAUNCH_CHECK();
 }
@@ -321,13 +321,13 @@ void launch_input_scale(uint32_t numActive, const uint8_t* cellType, const float
                         hipStream_t s) {
     const uint32_t cells = numActive * 512u;
     const dim3 rg = red_grid(cells);
-    hipLaunchKernelGGL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
-    hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red, scale);
+    FLIP2_KERNEL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
+    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red, scale);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_pairs_scale(const double* pairs, uint32_t count, double* red, float* scale, hipStream_t s) {
-    hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, pairs, (int)count, red, scale);
+    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, pairs, (int)count, red, scale);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -349,7 +349,7 @@ void launch_dense_vector(uint32_t numActive, const uint32_t* activeBricks, const
                          float* vD, uint16_t* coefD, hipStream_t s) {
     const uint32_t cells = numActive * 512u;
     if (!cells) return;
-    hipLaunchKernelGGL(dense_vector_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, coef, v, cells, n, vD, coefD);
+    FLIP2_KERNEL(dense_vector_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, coef, v, cells, n, vD, coefD);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -357,34 +357,34 @@ void launch_dense_system(uint32_t numActive, const uint32_t* activeBricks, const
                          const double* x, int n, float* bD, float* xD, uint16_t* coefD, hipStream_t s) {
     const uint32_t cells = numActive * 512u;
     if (!cells) return;
-    hipLaunchKernelGGL(dense_system_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, coef, b, x, cells, n, bD, xD,
+    FLIP2_KERNEL(dense_system_kernel, g1(cells, 256), dim3(256), 0, s, activeBricks, coef, b, x, cells, n, bD, xD,
                        coefD);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_dense_apply_A(const uint16_t* coefD, const float* x, float* y, int n, hipStream_t s) {
-    hipLaunchKernelGGL(dense_apply_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, x, y, n);
+    FLIP2_KERNEL(dense_apply_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, x, y, n);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_residual(const uint16_t* coefD, const float* z, const float* bD, float* res, int n, double* red,
                      hipStream_t s) {
-    hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
-    hipLaunchKernelGGL(residual_kernel, red_grid((long long)n * n * n), dim3(256), 0, s, coefD, z, bD, res, n, red);
+    FLIP2_KERNEL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
+    FLIP2_KERNEL(residual_kernel, red_grid((long long)n * n * n), dim3(256), 0, s, coefD, z, bD, res, n, red);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_loss_grad(const uint16_t* coefD, const float* res, const double* red, float* dOut, int n, hipStream_t s) {
-    hipLaunchKernelGGL(loss_grad_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, res, red, dOut, n);
+    FLIP2_KERNEL(loss_grad_kernel, g1((long long)n * n * n, 256), dim3(256), 0, s, coefD, res, red, dOut, n);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_dense_input(const float* bD, const uint16_t* coefD, int n, uint16_t* in0, uint16_t* in0c, uint8_t* mask,
                         float* scale, double* red, hipStream_t s) {
     const long long nvox = (long long)n * n * n;
-    hipLaunchKernelGGL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
-    hipLaunchKernelGGL(dense_norm_kernel, red_grid(nvox), dim3(256), 0, s, bD, coefD, nvox, red);
-    hipLaunchKernelGGL(dense_input_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, in0, in0c, mask, scale,
+    FLIP2_KERNEL(zero_red_kernel, dim3(1), dim3(64), 0, s, red, 2);
+    FLIP2_KERNEL(dense_norm_kernel, red_grid(nvox), dim3(256), 0, s, bD, coefD, nvox, red);
+    FLIP2_KERNEL(dense_input_kernel, g1(nvox, 256), dim3(256), 0, s, bD, coefD, nvox, in0, in0c, mask, scale,
                        red);
     FLIP2_LAUNCH_CHECK();
 }
@@ -411,6 +411,10 @@ struct Rng {
 
 UNet::UNet(int n, hipStream_t stream) : n_(n), n1_(n / 2), n2_(n / 4), stream_(stream) {
     FLIP2_REQUIRE(n >= 8 && n % 8 == 0, kErrArgument, "U-Net grid size must be a positive multiple of 8");
+    // the largest activation (the level-0 decoder input, 48 channels per
+    // voxel) must count its elements in 32 bits: n <= 440
+    FLIP2_REQUIRE((uint64_t)n * n * n * 48u <= 0xFFFFFFFFull, kErrRange,
+                  "U-Net grid size too large (at most 440: 48 n^3 activations must fit 32-bit counts)");
     const int spec[10][3] = {{16, 16, 0}, {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2},
                              {64, 64, 2}, {96, 32, 1}, {32, 32, 1}, {48, 16, 0}, {16, 16, 0}};
     size_t off = 0;

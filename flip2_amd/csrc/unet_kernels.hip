@@ -1317,7 +1317,7 @@ void launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, u
     const uint32_t tiles = grid.x * grid.y * grid.z;
     const int split = tiles >= 2048u ? 1 : (tiles >= 1024u ? 2 : 4);
 #define CONV_LAUNCH(CI, CO, SP)                                                                                     \
-    hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, 0, 0, SP>), dim3(grid.x * SP, grid.y, grid.z), dim3(256), 0, s, X, \
+    FLIP2_KERNEL((conv3_fwd_kernel<CI, CO, 0, 0, SP>), dim3(grid.x * SP, grid.y, grid.z), dim3(256), 0, s, X, \
                        (const uint16_t*)nullptr, W, bias, Y, (float*)nullptr, n, relu ? 1 : 0, noHead, cm)
 #define CONV_CASE(CI, CO)                                                                                   \
     if (cin == CI && cout == CO) {                                                                          \
@@ -1348,10 +1348,10 @@ void launch_conv3_fwd_upcat(const uint16_t* Xskip, const uint16_t* Xcoarse, cons
     FLIP2_REQUIRE(n % 2 == 0, kErrArgument, "conv3 with upsampled input: even grid size");
     const dim3 grid = conv_grid(n);
     if (ca == 64 && cb == 32 && cout == 32) {
-        hipLaunchKernelGGL((conv3_fwd_kernel<96, 32, 0, 64>), grid, dim3(256), 0, s, Xskip, Xcoarse, W, bias, Y,
+        FLIP2_KERNEL((conv3_fwd_kernel<96, 32, 0, 64>), grid, dim3(256), 0, s, Xskip, Xcoarse, W, bias, Y,
                            (float*)nullptr, n, relu ? 1 : 0, HeadArgs{}, cm);
     } else if (ca == 32 && cb == 16 && cout == 16) {
-        hipLaunchKernelGGL((conv3_fwd_kernel<48, 16, 0, 32>), grid, dim3(256), 0, s, Xskip, Xcoarse, W, bias, Y,
+        FLIP2_KERNEL((conv3_fwd_kernel<48, 16, 0, 32>), grid, dim3(256), 0, s, Xskip, Xcoarse, W, bias, Y,
                            (float*)nullptr, n, relu ? 1 : 0, HeadArgs{}, cm);
     } else {
         throw Error(kErrArgument, "conv3 with upsampled input: unsupported channel counts");
@@ -1362,7 +1362,7 @@ void launch_conv3_fwd_upcat(const uint16_t* Xskip, const uint16_t* Xcoarse, cons
 void launch_conv3_in2(const uint16_t* X2, const uint16_t* W, const float* bias, uint16_t* Y, int n, bool relu,
                       const ConvMask& cm, hipStream_t s) {
     const dim3 grid = conv_grid(n);
-    hipLaunchKernelGGL(conv3_in2_kernel, grid, dim3(256), 0, s, X2, W, bias, Y, n, relu ? 1 : 0, cm);
+    FLIP2_KERNEL(conv3_in2_kernel, grid, dim3(256), 0, s, X2, W, bias, Y, n, relu ? 1 : 0, cm);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -1370,14 +1370,14 @@ void launch_conv3_in2_list(const uint16_t* X2, const SlotInput& slots, const uin
                            uint16_t* Y, int n, bool relu, const uint8_t* vmask, const TileList& tl, hipStream_t s) {
     FLIP2_REQUIRE(!slots.r || n % 8 == 0, kErrArgument, "slot input: n must be a multiple of the 8^3 bricks");
     // four workgroups per CU (the register budget's limit)
-    hipLaunchKernelGGL(conv3_in2_list_kernel, dim3(256 * 4), dim3(256), 0, s, X2, slots, W, bias, Y, n, relu ? 1 : 0,
+    FLIP2_KERNEL(conv3_in2_list_kernel, dim3(256 * 4), dim3(256), 0, s, X2, slots, W, bias, Y, n, relu ? 1 : 0,
                        vmask, tl);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_tile_lists(const uint8_t* cur, const uint8_t* prev, uint32_t tiles, uint32_t* compute, uint32_t* clear,
                        uint32_t* counts, hipStream_t s) {
-    hipLaunchKernelGGL(tile_lists_kernel, dim3(1), dim3(1024), 0, s, cur, prev, tiles, compute, clear, counts);
+    FLIP2_KERNEL(tile_lists_kernel, dim3(1), dim3(1024), 0, s, cur, prev, tiles, compute, clear, counts);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -1396,7 +1396,7 @@ void launch_conv3_list(const uint16_t* X, const uint16_t* Xcoarse, const uint16_
     const int r = relu ? 1 : 0;
 #define LIST_CASE(CI, CA_, CO, SP)                                                                               \
     if (cin == CI && ca == CA_ && cout == CO && split == SP) {                                                   \
-        hipLaunchKernelGGL((conv3_list_kernel<CI, CO, CA_, SP>), grid, block, 0, s, X, Xcoarse, W, bias, Y, pooled, n, r, \
+        FLIP2_KERNEL((conv3_list_kernel<CI, CO, CA_, SP>), grid, block, 0, s, X, Xcoarse, W, bias, Y, pooled, n, r, \
                            head, vmask, tl);                                                                      \
         FLIP2_LAUNCH_CHECK();                                                                                    \
         return;                                                                                                  \
@@ -1419,14 +1419,14 @@ uint32_t conv_tiles(int n) {
 
 void launch_mask_pool(const uint8_t* m, int n, uint8_t* out, hipStream_t s) {
     const long long nc = n / 2;
-    hipLaunchKernelGGL(mask_pool_kernel, g1(nc * nc * nc, 256), dim3(256), 0, s, m, n, out);
+    FLIP2_KERNEL(mask_pool_kernel, g1(nc * nc * nc, 256), dim3(256), 0, s, m, n, out);
     FLIP2_LAUNCH_CHECK();
 }
 
 void launch_tile_flags(const uint8_t* m, int n, uint8_t* flags, hipStream_t s) {
     const dim3 g = conv_grid(n);
     const uint32_t tiles = g.x * g.y * g.z;
-    hipLaunchKernelGGL(tile_flags_kernel, g1(tiles, 256), dim3(256), 0, s, m, n, g.x, g.y, tiles, flags);
+    FLIP2_KERNEL(tile_flags_kernel, g1(tiles, 256), dim3(256), 0, s, m, n, g.x, g.y, tiles, flags);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -1435,7 +1435,7 @@ void launch_conv3_fwd_head(const uint16_t* X, const uint16_t* W, const float* bi
     FLIP2_REQUIRE(cin == 16 && head.out && head.w && head.b && head.mask && head.scale, kErrArgument,
                   "conv3 with fused head: 16 input channels and all head arguments");
     const dim3 grid = conv_grid(n);
-    hipLaunchKernelGGL((conv3_fwd_kernel<16, 16, 0>), grid, dim3(256), 0, s, X, (const uint16_t*)nullptr, W, bias, Y,
+    FLIP2_KERNEL((conv3_fwd_kernel<16, 16, 0>), grid, dim3(256), 0, s, X, (const uint16_t*)nullptr, W, bias, Y,
                        (float*)nullptr, n, relu ? 1 : 0, head, cm);
     FLIP2_LAUNCH_CHECK();
 }
@@ -1444,14 +1444,14 @@ void launch_conv3_dgrad(const uint16_t* dPb, const float* W32, uint16_t* WT, flo
                         hipStream_t s) {
     const long long nvox = (long long)n * n * n;
     const int nw = 27 * cin * cout;
-    hipLaunchKernelGGL(transpose_weights_kernel, g1(nw, 256), dim3(256), 0, s, W32, WT, cin, cout);
+    FLIP2_KERNEL(transpose_weights_kernel, g1(nw, 256), dim3(256), 0, s, W32, WT, cin, cout);
     FLIP2_LAUNCH_CHECK();
     (void)nvox;
     const dim3 grid = conv_grid(n);
     // the forward kernel with the roles of the channel counts swapped
 #define DGRAD_CASE(CI, CO)                                                                                      \
     if (cin == CI && cout == CO) {                                                                              \
-        hipLaunchKernelGGL((conv3_fwd_kernel<CO, CI, 1>), grid, dim3(256), 0, s, dPb, (const uint16_t*)nullptr, WT,  \
+        FLIP2_KERNEL((conv3_fwd_kernel<CO, CI, 1>), grid, dim3(256), 0, s, dPb, (const uint16_t*)nullptr, WT,  \
                            (const float*)nullptr, (uint16_t*)nullptr, dX, n, 0, HeadArgs{}, ConvMask{});       \
         FLIP2_LAUNCH_CHECK();                                                                                   \
         return;                                                                                                 \
@@ -1472,7 +1472,7 @@ void launch_conv3_wgrad(const uint16_t* dPb, const float* dP, const uint16_t* X,
     const long long nvox = (long long)n * n * n;
     // bias: column sums of dP
     const int bblocks = (int)std::min<long long>(1024, (nvox + 255) / 256);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(bblocks), dim3(256), 0, s, dP, nvox, cout, dB);
+    FLIP2_KERNEL(bias_grad_kernel, dim3(bblocks), dim3(256), 0, s, dP, nvox, cout, dB);
     FLIP2_LAUNCH_CHECK();
     // chunks: enough workgroups to fill the GPU, few enough that the atomics
     // (27 * cout * cin per chunk) stay around 4M
@@ -1483,7 +1483,7 @@ void launch_conv3_wgrad(const uint16_t* dPb, const float* dP, const uint16_t* X,
     chunks = (nvox + chunk - 1) / chunk;
 #define WGRAD_CASE(CI, CO)                                                                                      \
     if (cin == CI && cout == CO) {                                                                              \
-        hipLaunchKernelGGL((conv3_wgrad_mfma_kernel<CI, CO>), dim3((unsigned)chunks, 27), dim3(256), 0, s, dPb, X, \
+        FLIP2_KERNEL((conv3_wgrad_mfma_kernel<CI, CO>), dim3((unsigned)chunks, 27), dim3(256), 0, s, dPb, X, \
                            dW, n, chunk);                                                                       \
         FLIP2_LAUNCH_CHECK();                                                                                   \
         return;                                                                                                 \
@@ -1500,57 +1500,57 @@ void launch_conv3_wgrad(const uint16_t* dPb, const float* dP, const uint16_t* X,
 }
 
 void launch_relu_mask_bf16(float* g, const uint16_t* act, uint16_t* gb, long long count, hipStream_t s) {
-    hipLaunchKernelGGL(relu_mask_bf16_kernel, g1(count, 256), dim3(256), 0, s, g, act, gb, count);
+    FLIP2_KERNEL(relu_mask_bf16_kernel, g1(count, 256), dim3(256), 0, s, g, act, gb, count);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_avgpool(const uint16_t* X, uint16_t* P, int n, int C, hipStream_t s) {
     const long long total = (long long)(n / 2) * (n / 2) * (n / 2) * C;
-    hipLaunchKernelGGL(avgpool_kernel, g1(total, 256), dim3(256), 0, s, X, P, n, C);
+    FLIP2_KERNEL(avgpool_kernel, g1(total, 256), dim3(256), 0, s, X, P, n, C);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_avgpool_bwd(const float* dP, float* dX, int n, int C, hipStream_t s) {
     const long long total = (long long)n * n * n * C;
-    hipLaunchKernelGGL(avgpool_bwd_kernel, g1(total, 256), dim3(256), 0, s, dP, dX, n, C);
+    FLIP2_KERNEL(avgpool_bwd_kernel, g1(total, 256), dim3(256), 0, s, dP, dX, n, C);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_upcat(const uint16_t* A, const uint16_t* B, uint16_t* U, int n, int Ca, int Cb, hipStream_t s) {
     const long long total = (long long)n * n * n * (Ca + Cb);
-    hipLaunchKernelGGL(upcat_kernel, g1(total, 256), dim3(256), 0, s, A, B, U, n, Ca, Cb);
+    FLIP2_KERNEL(upcat_kernel, g1(total, 256), dim3(256), 0, s, A, B, U, n, Ca, Cb);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_upcat_bwd(const float* dU, float* dA, float* dB, int n, int Ca, int Cb, hipStream_t s) {
     const long long total = (long long)(n / 2) * (n / 2) * (n / 2) * Ca + (long long)n * n * n * Cb;
-    hipLaunchKernelGGL(upcat_bwd_kernel, g1(total, 256), dim3(256), 0, s, dU, dA, dB, n, Ca, Cb);
+    FLIP2_KERNEL(upcat_bwd_kernel, g1(total, 256), dim3(256), 0, s, dU, dA, dB, n, Ca, Cb);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_head_fwd(const uint16_t* X, const float* w, const float* b, const uint8_t* mask, const float* scale,
                      float* out, long long nvox, int C, hipStream_t s) {
-    hipLaunchKernelGGL(head_fwd_kernel, g1(nvox, 256), dim3(256), 0, s, X, w, b, mask, scale, out, nvox, C);
+    FLIP2_KERNEL(head_fwd_kernel, g1(nvox, 256), dim3(256), 0, s, X, w, b, mask, scale, out, nvox, C);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_head_bwd(const float* dOut, const uint16_t* X, const float* w, const uint8_t* mask, const float* scale,
                      float* dX, float* dw, float* db, long long nvox, int C, hipStream_t s) {
     FLIP2_REQUIRE(C <= 16, kErrArgument, "head: at most 16 channels");
-    hipLaunchKernelGGL(head_bwd_kernel, g1(nvox, 256), dim3(256), 0, s, dOut, X, w, mask, scale, dX, dw, db, nvox, C);
+    FLIP2_KERNEL(head_bwd_kernel, g1(nvox, 256), dim3(256), 0, s, dOut, X, w, mask, scale, dX, dw, db, nvox, C);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wbf, int count, float lr, float b1, float b2,
                  float eps, int t, hipStream_t s) {
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
-    hipLaunchKernelGGL(adam_kernel, g1(count, 256), dim3(256), 0, s, w, g, m, v, wbf, count, lr, b1, b2, eps, c1, c2);
+    FLIP2_KERNEL(adam_kernel, g1(count, 256), dim3(256), 0, s, w, g, m, v, wbf, count, lr, b1, b2, eps, c1, c2);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_to_bf16(const float* a, uint16_t* b, long long count, hipStream_t s) {
-    hipLaunchKernelGGL(to_bf16_kernel, g1(count, 256), dim3(256), 0, s, a, b, count);
+    FLIP2_KERNEL(to_bf16_kernel, g1(count, 256), dim3(256), 0, s, a, b, count);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_bf16_to_f32(const uint16_t* a, float* b, long long count, hipStream_t s) {
-    hipLaunchKernelGGL(bf16_to_f32_kernel, g1(count, 256), dim3(256), 0, s, a, b, count);
+    FLIP2_KERNEL(bf16_to_f32_kernel, g1(count, 256), dim3(256), 0, s, a, b, count);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_zero_f32(float* a, long long count, hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(zero_f32_kernel, g1(count, 256), dim3(256), 0, s, a, count);
+    FLIP2_KERNEL(zero_f32_kernel, g1(count, 256), dim3(256), 0, s, a, count);
     FLIP2_LAUNCH_CHECK();
 }
 

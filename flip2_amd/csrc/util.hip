@@ -21,13 +21,13 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
     const size_t vec = aligned ? bytes / 16 : 0;
     if (vec) {
         const size_t blocks = std::min<size_t>((vec + 255) / 256, 256u * 1024u);
-        hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+        FLIP2_KERNEL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
                            reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), vec);
         FLIP2_LAUNCH_CHECK();
     }
     const size_t rest = bytes - vec * 16;
     if (rest) {
-        hipLaunchKernelGGL(copy1_kernel, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, s,
+        FLIP2_KERNEL(copy1_kernel, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, s,
                            reinterpret_cast<const uint8_t*>(src) + vec * 16, reinterpret_cast<uint8_t*>(dst) + vec * 16,
                            rest);
         FLIP2_LAUNCH_CHECK();

@@ -304,6 +304,19 @@ int flip2_fluid_last_solve(flip2_fluid* f, flip2_solve_stats* stats);
  * "cellType","coef","b","p". */
 int flip2_fluid_get_array(flip2_fluid* f, const char* name, void* host, size_t capacity, size_t* count);
 
+/* Per-kernel timing with HIP events on the simulation's stream, for the
+ * following flip2_fluid_step / _step_frame / _stage / _solve_with calls:
+ * each launch of a kernel whose name is in the comma-separated list
+ * `kernels` (NULL: every kernel) is bracketed by two events.  Launches
+ * captured into a HIP graph are not timed.  flip2_fluid_timing returns, for
+ * timed kernel k (in name order), its name, the summed milliseconds and the
+ * launch count (FLIP2_ERR_RANGE past the last one); enabling resets the
+ * totals.  The reference has no timing (it prints host wall times only,
+ * particles.cu:550). */
+int flip2_fluid_enable_timing(flip2_fluid* f, const char* kernels);
+int flip2_fluid_disable_timing(flip2_fluid* f);
+int flip2_fluid_timing(flip2_fluid* f, int k, char* name, size_t name_len, double* total_ms, uint64_t* launches);
+
 /* ---------------------------------------------------------------------- */
 /* Learned preconditioner: 3-level 3D U-Net (16/32/64 channels, 3^3       */
 /* convolutions, BF16 weights/activations, FP32 accumulation on MFMA) on  */

@@ -327,3 +327,80 @@ void orc_ns_advect_rk2(uint32_t n, double* x, double* y, double* z, const double
         x[p] = xn[0]; y[p] = xn[1]; z[p] = xn[2];
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* One whole serial step of the north-star method, in the library's stage   */
+/* order: transfer (bin, 1-bit LSD sort, gather, counts, scan, P2G) ->      */
+/* walls + gravity -> assembly -> FP64 Jacobi-PCG to tol -> pressure        */
+/* gradient -> FLIP G2P -> RK2.  The particle arrays are replaced by their  */
+/* sorted, updated state.  Scratch is owned by the orc_ns_work object.      */
+/* ------------------------------------------------------------------------ */
+struct orc_ns_work {
+    uint32_t n, cells, keyBits;
+    uint32_t *keys, *perm, *counts, *start;
+    float* sorted;
+    double *F[6], *b, *p, *vel[3], *pos[3];
+    uint8_t* type;
+    uint16_t* coef;
+};
+
+orc_ns_work* orc_ns_work_create(uint32_t n, orc_grid g) {
+    orc_ns_work* w = calloc(1, sizeof(orc_ns_work));
+    if (!w) return NULL;
+    w->n = n;
+    w->cells = g.sizeX * g.sizeY * g.sizeZ;
+    w->keyBits = 1;
+    while (w->keyBits < 32 && ((w->cells - 1) >> w->keyBits)) ++w->keyBits;
+    size_t nn = n ? n : 1, cc = (size_t)w->cells + 1;
+    w->keys = malloc(nn * 4);
+    w->perm = malloc(nn * 4);
+    w->sorted = malloc(6 * nn * 4);
+    w->counts = malloc(cc * 4);
+    w->start = malloc(cc * 4);
+    for (int c = 0; c < 6; ++c) w->F[c] = malloc(cc * 8);
+    w->b = malloc(cc * 8);
+    w->p = malloc(cc * 8);
+    w->type = malloc(cc);
+    w->coef = malloc(cc * 2);
+    for (int c = 0; c < 3; ++c) {
+        w->vel[c] = malloc(nn * 8);
+        w->pos[c] = malloc(nn * 8);
+    }
+    return w;
+}
+
+void orc_ns_work_destroy(orc_ns_work* w) {
+    if (!w) return;
+    free(w->keys); free(w->perm); free(w->sorted); free(w->counts); free(w->start);
+    for (int c = 0; c < 6; ++c) free(w->F[c]);
+    free(w->b); free(w->p); free(w->type); free(w->coef);
+    for (int c = 0; c < 3; ++c) { free(w->vel[c]); free(w->pos[c]); }
+    free(w);
+}
+
+uint32_t orc_ns_step(orc_ns_work* w, float* x, float* y, float* z, float* vx, float* vy, float* vz, orc_grid g,
+                     const uint8_t* solid, double dt, double gravity, double flip, double tol, uint32_t maxIt,
+                     double margin, double* relOut) {
+    const uint32_t n = w->n;
+    orc_ns_transfer_step(x, y, z, vx, vy, vz, n, g, w->keyBits, w->keys, w->perm, w->sorted, w->counts, w->start,
+                         w->F[0], w->F[1], w->F[2]);
+    orc_ns_prepare_faces(w->F[0], w->F[1], w->F[2], w->F[3], w->F[4], w->F[5], solid, g, gravity * dt);
+    const double h = (double)g.cellSize;
+    orc_ns_assemble(w->counts, solid, g, h / dt, w->F[0], w->F[1], w->F[2], w->type, w->coef, w->b);
+    uint32_t it = orc_ns_pcg_jacobi(w->coef, g, w->b, w->p, tol, maxIt, relOut);
+    orc_ns_velocity_update(w->type, solid, g, w->p, dt / h, w->F[0], w->F[1], w->F[2]);
+    const float* s = w->sorted;
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) w->vel[c][i] = (double)s[(size_t)(3 + c) * n + i];
+    orc_ns_g2p_flip(n, s, s + n, s + 2 * (size_t)n, w->vel[0], w->vel[1], w->vel[2], w->F[0], w->F[1], w->F[2],
+                    w->F[3], w->F[4], w->F[5], g, flip);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) w->pos[c][i] = (double)s[(size_t)c * n + i];
+    orc_ns_advect_rk2(n, w->pos[0], w->pos[1], w->pos[2], w->F[0], w->F[1], w->F[2], solid, g, dt, margin);
+    float* out[6] = {x, y, z, vx, vy, vz};
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) out[c][i] = (float)w->pos[c][i];
+        for (int c = 0; c < 3; ++c) out[3 + c][i] = (float)w->vel[c][i];
+    }
+    return it;
+}

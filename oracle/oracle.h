@@ -159,6 +159,17 @@ void orc_ns_g2p_flip(uint32_t n, const float* x, const float* y, const float* z,
 void orc_ns_advect_rk2(uint32_t n, double* x, double* y, double* z, const double* U, const double* V, const double* W,
                        const uint8_t* solid, orc_grid g, double dt, double margin);
 
+/* One whole serial step of the north-star method (transfer, walls and
+ * gravity, assembly, FP64 Jacobi-PCG to tol, pressure gradient, FLIP G2P,
+ * RK2); the FP32 particle arrays are replaced by the sorted, updated state.
+ * Returns the PCG iteration count; *relOut the final relative residual. */
+typedef struct orc_ns_work orc_ns_work;
+orc_ns_work* orc_ns_work_create(uint32_t n, orc_grid g);
+void orc_ns_work_destroy(orc_ns_work* w);
+uint32_t orc_ns_step(orc_ns_work* w, float* x, float* y, float* z, float* vx, float* vy, float* vz, orc_grid g,
+                     const uint8_t* solid, double dt, double gravity, double flip, double tol, uint32_t maxIt,
+                     double margin, double* relOut);
+
 /* Resize voxel arrays to V (used by stage tests that inject state). */
 void orc_resize_voxels(orc_particles* s, uint32_t V);
 void orc_resize_nodes(orc_particles* s, uint32_t B);

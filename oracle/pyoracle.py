@@ -108,6 +108,12 @@ def lib() -> ctypes.CDLL:
         l.orc_ns_g2p_flip.argtypes = [ctypes.c_uint32] + [vp] * 12 + [OrcGrid, ctypes.c_double]
         l.orc_ns_advect_rk2.argtypes = [ctypes.c_uint32] + [vp] * 7 + [OrcGrid, ctypes.c_double, ctypes.c_double]
         l.orc_ns_transfer_step.argtypes = [vp] * 6 + [ctypes.c_uint32, OrcGrid, ctypes.c_uint32] + [vp] * 8
+        l.orc_ns_work_create.argtypes = [ctypes.c_uint32, OrcGrid]
+        l.orc_ns_work_create.restype = vp
+        l.orc_ns_work_destroy.argtypes = [vp]
+        l.orc_ns_step.argtypes = [vp] + [vp] * 6 + [OrcGrid, vp] + [ctypes.c_double] * 4 + [ctypes.c_uint32,
+                                                                                        ctypes.c_double, vp]
+        l.orc_ns_step.restype = ctypes.c_uint32
         _lib = l
     return _lib
 
@@ -196,6 +202,45 @@ class NsTransferStep:
         lib().orc_ns_transfer_step(*(_p(v) for v in self.a), self.n, self.g, self.key_bits, _p(self.keys),
                                    _p(self.perm), _p(self.sorted), _p(self.counts), _p(self.start),
                                    *(_p(o) for o in self.out))
+
+
+class NsStep:
+    """Whole serial FP64 step of the north-star method (orc_ns_step): the
+    particle state (FP32 arrays held here) is replaced by the sorted, updated
+    state after each run()."""
+
+    def __init__(self, x, y, z, vx, vy, vz, grid, solid=None, dt=1.0 / 240.0, gravity=-9.81, flip=0.95, tol=1e-6,
+                 max_it=100000, margin=1e-3):
+        self.a = [np.array(v, dtype=np.float32, copy=True) for v in (x, y, z, vx, vy, vz)]
+        self.g = OrcGrid(*grid)
+        self.solid = None if solid is None else np.ascontiguousarray(solid, np.uint8)
+        self.dt, self.gravity, self.flip, self.tol, self.max_it = dt, gravity, flip, tol, max_it
+        self.margin = margin * self.g.cellSize
+        self.w = lib().orc_ns_work_create(self.a[0].size, self.g)
+        if not self.w:
+            raise MemoryError("orc_ns_work_create failed")
+        self.iterations, self.rel = 0, 0.0
+
+    def run(self):
+        rel = ctypes.c_double()
+        self.iterations = lib().orc_ns_step(self.w, *(_p(v) for v in self.a), self.g,
+                                            None if self.solid is None else _p(self.solid), self.dt, self.gravity,
+                                            self.flip, self.tol, self.max_it, self.margin, ctypes.byref(rel))
+        self.rel = rel.value
+
+    def state(self):
+        return [v.copy() for v in self.a]
+
+    def close(self):
+        if getattr(self, "w", None):
+            lib().orc_ns_work_destroy(self.w)
+            self.w = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _f64(a):

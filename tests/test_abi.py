@@ -2,6 +2,8 @@
 import ctypes
 import re
 
+import pytest
+
 import flip2_amd
 
 
@@ -34,3 +36,37 @@ def test_errors_are_reported_not_fatal():
     rc = lib.flip2_particles_create(10, None)
     assert rc == -1
     assert b"NULL" in lib.flip2_last_error()
+
+
+def _config(**kw):
+    c = flip2_amd.FluidConfigC()
+    flip2_amd.lib().flip2_fluid_default_config(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+@pytest.mark.parametrize("field,value", [
+    ("max_iterations", 0), ("max_iterations", 0xFFFFFFFF), ("max_iterations", (1 << 24) + 1),
+    ("check_every", 1 << 20), ("dt", 0.0), ("dt", float("nan")), ("tolerance", -1.0), ("flip", 1.5),
+    ("preconditioner", 7), ("margin", 0.75),
+])
+def test_fluid_config_refused_before_any_device_call(field, value):
+    # the configuration is validated before the simulation touches the device,
+    # so a bad value is refused the same way with or without a GPU
+    lib = flip2_amd.lib()
+    h = ctypes.c_void_p()
+    rc = lib.flip2_fluid_create(16, ctypes.byref(_config(**{field: value})), ctypes.byref(h))
+    assert rc == -1  # FLIP2_ERR_ARGUMENT
+    assert b"config" in lib.flip2_last_error()
+    assert not h.value
+
+
+@pytest.mark.parametrize("n", [448, 512])
+def test_unet_size_beyond_32bit_counts_refused(n):
+    lib = flip2_amd.lib()
+    h = ctypes.c_void_p()
+    rc = lib.flip2_unet_create(n, ctypes.byref(h))
+    assert rc == -5  # FLIP2_ERR_RANGE
+    assert b"440" in lib.flip2_last_error()
+    assert not h.value

@@ -14,22 +14,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from flip2_amd import scenes
-
-RTOL = 1e-5
-
-
-def rel_err(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
-
-
-def dense_from_slots(sim, arr, cells):
-    ab = sim.get("activeBricks").astype(np.int64)
-    out = np.zeros(cells, dtype=arr.dtype)
-    idx = (ab[:, None] * 512 + np.arange(512)[None, :]).reshape(-1)
-    out[idx] = arr[:idx.size]
-    return out
+from tests.fluid_checks import dense_from_slots, rel_err, staged_step_vs_oracle
 
 
 def dam_break_sim(flip2, n=32, precond="jacobi", seed=1):
@@ -42,69 +27,9 @@ def dam_break_sim(flip2, n=32, precond="jacobi", seed=1):
 
 
 def test_stages_match_oracle(flip2, oracle):
-    n = 32
-    sim, grid, (x, y, z) = dam_break_sim(flip2, n)
-    cells = n ** 3
-    dt, h = 1.0 / 240.0, 1.0 / n
-    # transfer: P2G of the unsorted input
-    sim.stage("transfer")
-    P = [sim.get(c).astype(np.float64) for c in ("U", "V", "W")]
-    Po = oracle.ns_p2g(x, y, z, np.zeros_like(x), np.zeros_like(x), np.zeros_like(x), grid)
-    for a, b in zip(P, Po):
-        assert np.max(np.abs(a - b)) <= 1e-6  # all-zero velocities: exact zero field
-    # prepare: walls + gravity, saved field
-    sim.stage("prepare")
-    F, F0 = oracle.ns_prepare_faces(*P, grid, -9.81 * dt)
-    err = {}  # measured error per field (printed; pytest -s shows it)
-    for name, ref in zip(("U", "V", "W", "U0", "V0", "W0"), F + F0):
-        e = rel_err(sim.get(name), ref) if np.max(np.abs(ref)) > 0.0 else 0.0
-        err["faces " + name] = e
-        assert e <= RTOL, name
-    # system
-    sim.stage("system")
-    counts = sim.get("counts")
-    Fg = [sim.get(c).astype(np.float64) for c in ("U", "V", "W")]
-    ty, coef, b = oracle.ns_assemble(counts, grid, h / dt, *Fg)
-    assert np.array_equal(dense_from_slots(sim, sim.get("coef"), cells), coef)
-    err["rhs b"] = rel_err(dense_from_slots(sim, sim.get("b"), cells), b)
-    assert err["rhs b"] <= RTOL
-    # solve: FP64 true residual of the GPU pressure
-    sim.stage("solve")
-    st = sim.last_solve()
-    assert st["converged"], st
-    p = dense_from_slots(sim, sim.get("p"), cells)
-    r = b - oracle.ns_apply_A(coef, grid, p)
-    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-6
-    pref, _, rel = oracle.ns_pcg_jacobi(coef, grid, b, tol=1e-12)
-    assert rel <= 1e-12
-    err["pressure"] = rel_err(p, pref)
+    sim, grid, _ = dam_break_sim(flip2, 32)
+    err, _ = staged_step_vs_oracle(sim, grid, oracle)
     assert err["pressure"] <= 2e-3  # residual 1e-6 x condition number of the 32^3 Poisson operator
-    # project
-    sim.stage("project")
-    Fp = oracle.ns_velocity_update(ty, grid, p, dt / h, *Fg)
-    for name, ref in zip(("U", "V", "W"), Fp):
-        err["projected " + name] = rel_err(sim.get(name), ref)
-        assert err["projected " + name] <= RTOL, name
-    # divergence of the projected field, relative to before
-    _, _, b_after = oracle.ns_assemble(counts, grid, h / dt, *[sim.get(c).astype(np.float64) for c in ("U", "V", "W")])
-    assert np.linalg.norm(b_after) <= 2e-6 * np.linalg.norm(b)
-    # g2p (state is in sorted order after the transfer)
-    xs, ys, zs, vxs, vys, vzs = sim.particles()
-    Fn = [sim.get(c) for c in ("U", "V", "W", "U0", "V0", "W0")]
-    vo = oracle.ns_g2p_flip(xs, ys, zs, vxs, vys, vzs, *Fn, grid, 0.95)
-    sim.stage("g2p")
-    _, _, _, vx2, vy2, vz2 = sim.particles()
-    for c, a, ref in zip("xyz", (vx2, vy2, vz2), vo):
-        err["particle v" + c] = rel_err(a, ref)
-        assert err["particle v" + c] <= RTOL
-    # advect
-    xo = oracle.ns_advect_rk2(xs, ys, zs, *Fn[:3], grid, dt, 1e-3 * h)
-    sim.stage("advect")
-    x2, y2, z2, _, _, _ = sim.particles()
-    for c, a, ref in zip("xyz", (x2, y2, z2), xo):
-        err["particle " + c] = rel_err(a, ref)
-        assert err["particle " + c] <= RTOL
-    print("max-norm relative error vs the FP64 oracle:", {k: float(f"{v:.2e}") for k, v in err.items()})
 
 
 @pytest.mark.parametrize("precond", ["jacobi", "mic0"])
