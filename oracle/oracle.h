@@ -130,6 +130,12 @@ void orc_advect_particles(orc_particles* s);            /* particles.cu:751-760 
 void orc_move_solid_particles_only(orc_particles* s);   /* particles.cu:674-740 (kernel only) */
 void orc_move_solid_particles(orc_particles* s);        /* particles.cu:762-769 */
 void orc_initialize(orc_particles* s);                  /* particles.cu:805-812 */
+/* The dt-halving retry control of pressureSolve (particles.cu:528-549) on
+ * caller-supplied solve and halve steps. */
+typedef double (*orc_solve_cb)(void* ctx, double dt);
+typedef void (*orc_halve_cb)(void* ctx, double oldDt, double newDt);
+uint32_t orc_pressure_retry(double* dt, double threshold, orc_solve_cb solve, orc_halve_cb halve, void* ctx,
+                            double* last);
 void orc_solve_frame(orc_particles* s, double fps);     /* particles.cu:784-803 */
 
 /* ------------------------------------------------------------------------ */

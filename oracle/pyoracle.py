@@ -58,6 +58,31 @@ class OrcParticles(ctypes.Structure):
     ]
 
 
+SOLVE_CB = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
+HALVE_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, ctypes.c_double)
+
+
+def pressure_retry(dt, threshold, residuals):
+    """Drive the oracle's pressureSolve retry control with a scripted
+    sequence of terminating residuals; returns (solves, final dt, the dt of
+    each solve, the (old, new) dt of each halving, final residual)."""
+    seq = list(residuals)
+    seen, halvings = [], []
+
+    def solve(_ctx, d):
+        seen.append(d)
+        return seq[len(seen) - 1]
+
+    def halve(_ctx, old, new):
+        halvings.append((old, new))
+
+    cs, ch = SOLVE_CB(solve), HALVE_CB(halve)
+    d = ctypes.c_double(dt)
+    last = ctypes.c_double()
+    n = lib().orc_pressure_retry(ctypes.byref(d), threshold, cs, ch, None, ctypes.byref(last))
+    return n, d.value, seen, halvings, last.value
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -108,6 +133,9 @@ def lib() -> ctypes.CDLL:
         l.orc_ns_g2p_flip.argtypes = [ctypes.c_uint32] + [vp] * 12 + [OrcGrid, ctypes.c_double]
         l.orc_ns_advect_rk2.argtypes = [ctypes.c_uint32] + [vp] * 7 + [OrcGrid, ctypes.c_double, ctypes.c_double]
         l.orc_ns_transfer_step.argtypes = [vp] * 6 + [ctypes.c_uint32, OrcGrid, ctypes.c_uint32] + [vp] * 8
+        l.orc_pressure_retry.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_double, SOLVE_CB, HALVE_CB, vp,
+                                         ctypes.POINTER(ctypes.c_double)]
+        l.orc_pressure_retry.restype = ctypes.c_uint32
         l.orc_ns_work_create.argtypes = [ctypes.c_uint32, OrcGrid]
         l.orc_ns_work_create.restype = vp
         l.orc_ns_work_destroy.argtypes = [vp]

@@ -724,32 +724,55 @@ double orc_gs_solve(orc_particles* s, double dt, double threshold, uint32_t maxI
     return maxResidual;
 }
 
+/* The retry loop of Particles::pressureSolve (particles.cu:528-549): solve;
+ * while the terminating residual keeps decreasing and is not below the
+ * threshold, halve dt (the callback undoes gravity at the old dt, clears p,
+ * applies gravity at the new dt and recomputes divU) and solve again.
+ * Returns the number of solves; *last receives the final residual. */
+uint32_t orc_pressure_retry(double* dt, double threshold, orc_solve_cb solve, orc_halve_cb halve, void* ctx,
+                            double* last) {
+    double previousTerminatingResidual = 100.0, terminatingResidual;
+    uint32_t attempts = 0;
+    for (;;) {
+        terminatingResidual = solve(ctx, *dt);
+        attempts += 1;
+        if (!(previousTerminatingResidual - terminatingResidual > 0)) break;
+        if (terminatingResidual < threshold) break;
+        const double old = *dt;
+        *dt /= 2.0;
+        halve(ctx, old, *dt);
+        previousTerminatingResidual = terminatingResidual;
+    }
+    if (last) *last = terminatingResidual;
+    return attempts;
+}
+
+static double rc_retry_solve(void* ctx, double dt) {
+    orc_particles* s = (orc_particles*)ctx;
+    uint32_t its = 0;
+    const double r = orc_gs_solve(s, dt, s->sorThreshold, s->sorMaxIterations, &its);
+    s->lastSorIterations += its;
+    return r;
+}
+
+static void rc_retry_halve(void* ctx, double oldDt, double newDt) {
+    orc_particles* s = (orc_particles*)ctx;
+    orc_remove_gravity(s, oldDt);
+    for (uint32_t i = 0; i < s->numUsedVoxels; ++i) s->p[i] = 0.0;
+    orc_apply_gravity(s, newDt);
+    orc_calc_div_u(s);
+}
+
 /* Particles::pressureSolve (particles.cu:507-555). */
 void orc_pressure_solve(orc_particles* s) {
-    const double threshold = s->sorThreshold;
-    const uint32_t maxIterations = s->sorMaxIterations;
-    double previousTerminatingResidual = 100.0, terminatingResidual;
-    uint32_t its = 0;
     s->lastSorIterations = 0;
-    s->lastSolveAttempts = 0;
     s->dt = orc_get_courant_dt(s);
     if (s->frameDt - s->elapsedTimeThisFrame < s->dt) s->dt = s->frameDt - s->elapsedTimeThisFrame;
     orc_apply_gravity(s, s->dt);
     orc_calc_div_u(s);
-    for (;;) {
-        terminatingResidual = orc_gs_solve(s, s->dt, threshold, maxIterations, &its);
-        s->lastSorIterations += its;
-        s->lastSolveAttempts += 1;
-        if (!(previousTerminatingResidual - terminatingResidual > 0)) break;
-        if (terminatingResidual < threshold) break;
-        orc_remove_gravity(s, s->dt);
-        s->dt /= 2.0;
-        for (uint32_t i = 0; i < s->numUsedVoxels; ++i) s->p[i] = 0.0;
-        orc_apply_gravity(s, s->dt);
-        previousTerminatingResidual = terminatingResidual;
-        orc_calc_div_u(s);
-    }
-    s->lastResidual = terminatingResidual;
+    double last = 0.0;
+    s->lastSolveAttempts = orc_pressure_retry(&s->dt, s->sorThreshold, rc_retry_solve, rc_retry_halve, s, &last);
+    s->lastResidual = last;
     s->elapsedTime += s->dt;
     s->elapsedTimeThisFrame += s->dt;
 }
