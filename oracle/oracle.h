@@ -13,9 +13,49 @@
  *   - "ns_*" (ns_oracle.c): the north-star method (trilinear MAC P2G,
  *     Jacobi / brick-MIC(0) PCG, FLIP 0.95, RK2, conv3d U-Net) in FP64.
  *
- * Pinning (what checks each function against something other than itself) is
- * listed per function in tests/test_oracle_*.py.  Functions with no such
- * check are marked "parity unpinned" below.
+ * Pins: what checks each function against something other than itself
+ * (tests/test_oracle_primitives.py = P, test_oracle_rc.py = R,
+ * test_oracle_rc_brute.py = B, test_oracle_ns.py = N, test_golden.py = G):
+ *   orc_find_grid_cell         P: hand cases at cell faces +-1 ulp, clamping, float narrowing
+ *   orc_scan_u32               P: numpy cumsum mod 2^32
+ *   orc_radix_sort_u32/_pairs  P: numpy stable argsort; partial bit ranges
+ *   orc_sampled_absmax         P: hand-computed sampled maxima (Q12)
+ *   orc_philox4x32_10          P: the published known-answer vectors
+ *   orc_set_domain, orc_align_particles_to_grid
+ *                              P, B: FP32 narrowing decides the offset-domain cases
+ *   orc_sort_particles         B: positions gathered by the stable order, velocities kept (Q4)
+ *   orc_generate_voxels        B: brute force over every voxel and particle of
+ *                                 the reference's marking and compaction (Q6-Q9,
+ *                                 Q24), segments, the hand-built row table
+ *   orc_particle_vel_to_voxels R: single-particle hat weights
+ *   orc_get_courant_dt         B: closed form over a sampled maximum
+ *   orc_apply/remove_gravity   B: closed form on non-solid voxels
+ *   orc_calc_div_u             R: linear field; B: brute force across nodes (Q17)
+ *   orc_gs_iterations          R: converged iteration = dense solve
+ *   orc_gs_solve               B: checks every 1024 iterations (Q14)
+ *   orc_pressure_retry/_solve  B: scripted residual sequences (Q20)
+ *   orc_update_voxel_velocities R: linear pressure closed form
+ *   orc_voxel_vels_to_particles R: constant field; B: exact-zero voxels take no weight (Q21)
+ *   orc_advect_particles       R: Euler closed form
+ *   orc_move_solid_particles_only  parity unpinned: the reference's relocation
+ *                                 draws from curand XORWOW (particles.cu:681,
+ *                                 735-737), replaced by Philox4x32-10 here
+ *   orc_move_solid_particles   G2P part as above; relocation unpinned
+ *   orc_initialize, orc_solve_frame
+ *                              composition of the pinned stages in the order of
+ *                              particles.cu:784-812; G: fixtures stop drift
+ *   orc_ns_tiled_index, orc_ns_bin  N: bijection and hand cases, clamping
+ *   orc_ns_p2g                 N: uniform field, staggered hat weights, partition
+ *                                 of unity, face point, wall-truncated weights
+ *   orc_ns_prepare_faces       N: closed form with solids
+ *   orc_ns_assemble, orc_ns_apply_A, orc_ns_pcg_jacobi
+ *                              N: scipy sparse operator from the cell types,
+ *                                 numpy MAC divergence, scipy spsolve
+ *   orc_ns_velocity_update     N: linear pressure closed form, free surface
+ *   orc_ns_g2p_flip            N: linear field exact; FLIP 0 / 1 / 0.95 blends
+ *   orc_ns_advect_rk2          N: uniform field, midpoint closed form, clamp, solids
+ *   orc_ns_transfer_step, orc_ns_step
+ *                              N: equal to the stages run one by one
  *
  * Build: oracle/Makefile -> oracle/liboracle.so (gcc -O2 -ffp-contract=off).
  * FMA contraction is disabled so every product/sum is an individually rounded

@@ -366,3 +366,28 @@ def test_serial_step_equals_its_stages(oracle):
     got = st.state()
     for g, e in zip(got, list(xo) + list(v)):
         assert np.array_equal(g, e.astype(np.float32))
+
+
+# ------------------------------------------------------ binning, tiled index
+def test_tiled_index_is_the_brick_order(oracle):
+    n = 16
+    grid = grid_of(n)
+    i, j, k = ijk(n)
+    t = oracle.ns_tiled_index(i, j, k, grid)
+    assert np.array_equal(np.sort(t), np.arange(n ** 3))   # a bijection
+    # hand case: cell (9, 2, 13) is in brick (1, 0, 1) of 2 x 2 x 2 bricks ->
+    # brick (1 * 2 + 0) * 2 + 1 = 5, local (1, 2, 5)
+    assert oracle.ns_tiled_index(9, 2, 13, grid) == 5 * 512 + 5 * 64 + 2 * 8 + 1
+    assert oracle.lib().orc_ns_tiled_index(9, 2, 13, oracle.OrcGrid(*grid)) == 5 * 512 + 5 * 64 + 2 * 8 + 1
+
+
+def test_ns_bin_cells_and_clamping(oracle):
+    n = 16
+    grid = grid_of(n, origin=(-1.0, 0.5, 0.0), h=0.125)
+    x = np.array([-1.0, -0.99, 0.999, -5.0, 9.0], np.float32)
+    y = np.array([0.5, 0.5 + 0.125 * 3.5, 2.49, 0.0, 0.6], np.float32)
+    z = np.array([0.0, 1.0, 1.99, 0.3, -2.0], np.float32)
+    keys = oracle.ns_bin(x, y, z, grid)
+    cell = [(0, 0, 0), (0, 3, 8), (15, 15, 15), (0, 0, 2), (15, 0, 0)]  # clamped into [0, n - 1]
+    for k, (a, b, c) in zip(keys, cell):
+        assert k == oracle.ns_tiled_index(a, b, c, grid)

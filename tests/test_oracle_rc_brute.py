@@ -356,3 +356,72 @@ def test_pressure_retry_control(oracle, residuals, solves, halvings):
     assert dt == dt0 / 2 ** halvings
     assert halv == [(dt0 / 2 ** k, dt0 / 2 ** (k + 1)) for k in range(halvings)]
     assert last == residuals[solves - 1]
+
+
+# ------------------------------------------------- sort, segments, dt, gravity
+def test_sort_gathers_positions_only_and_segments(oracle):
+    # sortParticles (particles.cu:52-78) permutes px, py, pz by the stable key
+    # order and leaves the velocities in place (Q4); the head flags' inclusive
+    # scan numbers the nodes from 1 and the node table holds each node's first
+    # particle (particleToGridFunctions.cu:303-345)
+    grid = (0.0, 0.0, 0.0, 4, 4, 4, 0.25)
+    pos = jitter_cells([(3, 1, 2), (0, 0, 0), (3, 1, 2), (1, 2, 3), (0, 0, 0)], 0.25, 3, seed=4, lo=0.1, hi=0.9)
+    vel = np.arange(pos.size, dtype=np.float64).reshape(-1, 3)
+    s = run_scene(oracle, pos, grid, vel)
+    keys0 = (np.floor(pos[:, 0] / 0.25) + 4 * np.floor(pos[:, 1] / 0.25) + 16 * np.floor(pos[:, 2] / 0.25))
+    order = np.argsort(keys0, kind="stable")
+    assert np.array_equal(s.get("reorderedGridIndices"), order.astype(np.uint32))
+    for name, c in (("px", 0), ("py", 1), ("pz", 2)):
+        assert np.array_equal(s.get(name), pos[order, c])
+    for name, c in (("vx", 0), ("vy", 1), ("vz", 2)):
+        assert np.array_equal(s.get(name), vel[:, c])
+    k = keys0[order]
+    heads = np.concatenate([[True], k[1:] != k[:-1]])
+    assert np.array_equal(s.get("uniqueGridNodeIndices"), np.cumsum(heads).astype(np.uint32))
+    assert np.array_equal(s.get("gridNodeIndicesToFirstParticleIndex"), np.flatnonzero(heads).astype(np.uint32))
+
+
+def test_courant_dt_and_gravity_closed_forms(oracle):
+    grid = (0.0, 0.0, 0.0, 4, 4, 4, 0.3)
+    # the node at x = 0 holds low-wall (solid) voxels
+    pos = jitter_cells([(1, 1, 1), (2, 1, 1), (0, 1, 1)], 0.3, 300, seed=6, lo=0.2, hi=0.8)
+    n = pos.shape[0]
+    vel = np.zeros((n, 3))
+    vel[0, 0] = -1.5          # sampled (index 0)
+    vel[256, 1] = 4.0         # sampled (index 256), the largest magnitude
+    vel[100, 2] = -50.0       # not sampled (Q12)
+    s = run_scene(oracle, pos, grid, vel)
+    voxel = float(np.float32(0.3)) / (8 - 2 * 2)   # cellSize / (numVoxels1D - 2 floor(radius))
+    assert s.getCourantDt() == 0.7 * (voxel / 4.0)
+    # gravity: Uy += (double)(-9.8f) dt on non-solid voxels only
+    nv = s.get("voxelsUy").size
+    rng = np.random.default_rng(1)
+    u = rng.normal(size=nv)
+    s.set("voxelsUy", u)
+    sol = s.get("solids") != 0
+    assert sol.any() and not sol.all()
+    dt = 0.0123
+    s.applyGravity(dt)
+    g = float(np.float32(-9.8)) * dt
+    got = s.get("voxelsUy")
+    assert np.array_equal(got[~sol], u[~sol] + g) and np.array_equal(got[sol], u[sol])
+    s.removeGravity(dt)
+    assert np.array_equal(s.get("voxelsUy")[~sol], (u[~sol] + g) - g)
+
+
+def test_sor_checks_every_1024_iterations(oracle):
+    # cudaGSiteration (voxelSolveFunctions.cu:391-412) checks the sampled
+    # residual at it % 1024 == 0 only: a solve stops after 1 + 1024 k
+    # iterations (or at the cap); with divU = 0 the first check ends it
+    grid = (0.0, 0.0, 0.0, 4, 4, 4, 0.25)
+    pos = jitter_cells([(1, 1, 1), (2, 1, 1), (1, 2, 1)], 0.25, 8, seed=2, lo=0.2, hi=0.8)
+    s = run_scene(oracle, pos, grid)
+    nv = s.get("divU").size
+    s.set("divU", np.zeros(nv))
+    r, it = s.gsSolve(0.01)
+    assert it == 1 and r == 0.0
+    rng = np.random.default_rng(3)
+    s.set("divU", rng.normal(size=nv))
+    s.set("p", np.zeros(nv))
+    r, it = s.gsSolve(0.01, threshold=1e-12, maxIterations=5000)
+    assert it % 1024 == 1 or it == 5000, it
