@@ -49,7 +49,8 @@ FluidSim::FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device)
     FLIP2_REQUIRE(g.sizeX % 8 == 0 && g.sizeY % 8 == 0 && g.sizeZ % 8 == 0 && g.sizeX && g.sizeY && g.sizeZ,
                   kErrArgument, "grid sizes must be positive multiples of 8");
     FLIP2_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    pipe_.reset(new P2GPipeline(maxParticles, g));
+    dm_.setStream(stream_);
+    pipe_.reset(new P2GPipeline(maxParticles, g, &dm_));
     numCells_ = g.sizeX * g.sizeY * g.sizeZ;
     numBricks_ = numCells_ / 512u;
     rec_.resizeAsync(2 * maxParticles, stream_);
@@ -69,6 +70,7 @@ FluidSim::~FluidSim() {
     dropLearnedGraph();
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
+        dm_.setStream(nullptr);  // drained: the arrays' releases need no further waits
         (void)hipStreamDestroy(stream_);
     }
 }

@@ -157,26 +157,26 @@ private:
     void readHistory(uint32_t first, uint32_t count, double* host);
 
     FluidConfig cfg_;
-    DeviceManager dm_;
+    DeviceManager dm_;  // the pool of every device array below (declared first: outlives them)
     hipStream_t stream_ = nullptr;
     uint32_t maxParticles_, n_ = 0, numCells_, numBricks_;
     uint32_t numActive_ = 0;
     std::unique_ptr<P2GPipeline> pipe_;
-    DeviceVec<float4> rec_;  // particle records (2 x float4 per particle)
-    DeviceVec<float> U_, V_, W_, U0_, V0_, W0_;
-    DeviceVec<uint8_t> solid_, solidStatic_;  // combined mask, static part
+    DeviceVec<float4> rec_{dm_};  // particle records (2 x float4 per particle)
+    DeviceVec<float> U_{dm_}, V_{dm_}, W_{dm_}, U0_{dm_}, V0_{dm_}, W0_{dm_};
+    DeviceVec<uint8_t> solid_{dm_}, solidStatic_{dm_};  // combined mask, static part
     bool hasSolid_ = false, hasStatic_ = false;
     ns::SolidBox box_{};
     void rebuildSolid();
     // active bricks
-    DeviceVec<uint32_t> brickFlag_, brickExcl_, activeBricks_;
-    DeviceVec<int32_t> brickSlot_, nbrSlot_;
-    DeviceVec<uint8_t> cellType_;
-    DeviceVec<uint16_t> coef_;
-    DeviceVec<double> b64_, x64_, r64_;
-    DeviceVec<float> r_, z_, p_, q_, e_, precon_, aux0_, aux1_;
-    DeviceVec<float> pRing_, qRing_;  // flexible CG directions, learnedDirections + 1 slots
-    DeviceVec<double> zq_;
+    DeviceVec<uint32_t> brickFlag_{dm_}, brickExcl_{dm_}, activeBricks_{dm_};
+    DeviceVec<int32_t> brickSlot_{dm_}, nbrSlot_{dm_};
+    DeviceVec<uint8_t> cellType_{dm_};
+    DeviceVec<uint16_t> coef_{dm_};
+    DeviceVec<double> b64_{dm_}, x64_{dm_}, r64_{dm_};
+    DeviceVec<float> r_{dm_}, z_{dm_}, p_{dm_}, q_{dm_}, e_{dm_}, precon_{dm_}, aux0_{dm_}, aux1_{dm_};
+    DeviceVec<float> pRing_{dm_}, qRing_{dm_};  // flexible CG directions, learnedDirections + 1 slots
+    DeviceVec<double> zq_{dm_};
     uint32_t ringSlots_ = 0;
     void ensureRing(uint32_t slots);
     void applyLearned(uint32_t na);
@@ -185,9 +185,9 @@ private:
     bool learnedFresh_ = true;  // the next learned apply starts a new system
     hipGraph_t learnedGraph_ = nullptr;
     hipGraphExec_t learnedGraphExec_ = nullptr;
-    DeviceVec<double> partials_, hist_, scal_;
-    DeviceVec<uint32_t> total_;
-    DeviceVec<uint8_t> ws_;
+    DeviceVec<double> partials_{dm_}, hist_{dm_}, scal_{dm_};
+    DeviceVec<uint32_t> total_{dm_};
+    DeviceVec<uint8_t> ws_{dm_};
     uint32_t slotCapacity_ = 0;
     SolveStats stats_;
     PrecondFn learned_;

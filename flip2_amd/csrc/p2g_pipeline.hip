@@ -5,7 +5,8 @@
 
 namespace flip2 {
 
-P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g) : grid_(g), maxParticles_(maxParticles) {
+P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g, DeviceManager* pool)
+    : grid_(g), maxParticles_(maxParticles) {
     FLIP2_REQUIRE(g.sizeX % 8 == 0 && g.sizeY % 8 == 0 && g.sizeZ % 8 == 0 && g.sizeX && g.sizeY && g.sizeZ,
                   kErrArgument, "grid sizes must be positive multiples of 8");
     FLIP2_REQUIRE(g.cellSize > 0.0f, kErrArgument, "cell size must be positive");
@@ -13,6 +14,9 @@ P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g) : grid_(g), maxPa
     FLIP2_REQUIRE(cells < (1ull << 31), kErrRange, "grid has more than 2^31 cells");
     FLIP2_REQUIRE(maxParticles > 0 && maxParticles < (1u << 30), kErrRange, "particle count must be in [1, 2^30)");
     numCells_ = (uint32_t)cells;
+    for (auto* v : {&keys_, &keysSorted_, &perm_, &keysAlt_, &permAlt_, &counts_, &cellStart_, &stats_}) v->setPool(pool);
+    for (auto* v : {&packed_, &sorted_}) v->setPool(pool);
+    ws_.setPool(pool);
     keyBits_ = std::max<uint32_t>(1, bits_for(numCells_ - 1));
     keys_.resize(maxParticles); keys_.host().clear(); keys_.host().shrink_to_fit();
     keysSorted_.resizeAsync(maxParticles, nullptr);

@@ -13,7 +13,9 @@ namespace flip2 {
 
 class P2GPipeline {
 public:
-    P2GPipeline(uint32_t maxParticles, const Grid& g);
+    // pool: allocator of the pipeline's arrays (null: hipMalloc); it must
+    // outlive the pipeline.
+    P2GPipeline(uint32_t maxParticles, const Grid& g, DeviceManager* pool = nullptr);
     ~P2GPipeline();
     P2GPipeline(const P2GPipeline&) = delete;
     P2GPipeline& operator=(const P2GPipeline&) = delete;

@@ -71,9 +71,10 @@ public:
     void setSolverLimits(double threshold, uint32_t maxIterations);
 
     RcStats stats();
-    uint32_t size() const { return size_; }
-    const Grid& grid() const { return grid_; }
-    hipStream_t stream() const { return stream_; }
+    uint32_t numParticles() const { return size; }
+    const Grid& getGrid() const { return grid; }
+    hipStream_t getStream() const { return stream; }
+    DeviceManager& deviceManager() { return dm_; }
     void synchronize();
 
 private:
@@ -84,35 +85,53 @@ private:
     double solveScale(double dt) const;
     void ensureAbsmaxTmp(uint32_t n);
 
+    // Every device array below is allocated from this pool (declared first:
+    // it outlives them).
     DeviceManager dm_;
-    hipStream_t stream_ = nullptr;
-    uint32_t size_;
-    Grid grid_{};
-    uint32_t yDimRows_ = 0;
 
-    DeviceVec<double> px_, py_, pz_, vx_, vy_, vz_;
-    DeviceVec<uint32_t> gridCell_, reorderedGridIndices_, uniqueGridNodeIndices_;
-    DeviceVec<uint32_t> gridNodeIndicesToFirstParticleIndex_, yDimNumUsedGridNodes_, nodeIndexUsedVoxels_;
-    DeviceVec<uint32_t> voxelIDsUsed_;
-    DeviceVec<int8_t> solids_;
-    DeviceVec<double> voxelsUx_, voxelsUy_, voxelsUz_, divU_, p_, residuals_;
+    // The reference's members, with its names and types (particles.hu:47-92),
+    // so a harness written against them (the friend ParticleSystemTester of
+    // engine/testing.h) reads and writes the same state.
+    DeviceVec<double> px{dm_}, py{dm_}, pz{dm_};
+    DeviceVec<double> vx{dm_}, vy{dm_}, vz{dm_};
+    DeviceVec<uint32_t> gridCell{dm_};
+    DeviceVec<char> solids{dm_};
+    DeviceVec<double> p{dm_};
+    DeviceVec<double> residuals{dm_};
+    DeviceVec<uint32_t> yDimNumUsedGridNodes{dm_};
+    DeviceVec<uint32_t> reorderedGridIndices{dm_};
+    DeviceVec<uint32_t> uniqueGridNodeIndices{dm_};
+    DeviceVec<uint32_t> gridNodeIndicesToFirstParticleIndex{dm_};
+    DeviceVec<uint32_t> nodeIndexUsedVoxels{dm_};
+    DeviceVec<uint32_t> voxelIDsUsed{dm_};
+    DeviceVec<double> divU{dm_};
+    DeviceVec<double> voxelsUx{dm_}, voxelsUy{dm_}, voxelsUz{dm_};
+
+    uint32_t numUsedGridNodes = 0;
+    uint32_t numUsedVoxels = 0;
+    uint32_t refinementLevel = kRcRefinement;
+    uint32_t numVoxelsPerNode = kRcVoxelsPerNode;
+    uint32_t numVoxels1D = kRcVoxels1D;
+    uint32_t size;
+    double radius = kRcRadius;
+    Grid grid{};
+    double frameDt;
+    double dt = 0.0;
+    double elapsedTime = 0.0;
+    double elapsedTimeThisFrame = 0.0;
+    hipStream_t stream = nullptr;
 
     // scratch
-    DeviceVec<uint32_t> sortedKeys_, keysAlt_, permAlt_;
-    DeviceVec<double> tmpX_, tmpY_, tmpZ_;
-    DeviceVec<uint64_t> usedMask_, solidMask_;
-    DeviceVec<uint32_t> nodeCounts_;
-    DeviceVec<uint8_t> workspace_;
-    DeviceVec<double> absmaxTmp_;
-    DeviceVec<double> scalarsD_;   // [0..2] sampled maxima
-    DeviceVec<uint32_t> scalarsU_; // [0] scan total, [1] out-of-tile events, [2] relocations
+    uint32_t yDimRows_ = 0;
+    DeviceVec<uint32_t> sortedKeys_{dm_}, keysAlt_{dm_}, permAlt_{dm_};
+    DeviceVec<double> tmpX_{dm_}, tmpY_{dm_}, tmpZ_{dm_};
+    DeviceVec<uint64_t> usedMask_{dm_}, solidMask_{dm_};
+    DeviceVec<uint32_t> nodeCounts_{dm_};
+    DeviceVec<uint8_t> workspace_{dm_};
+    DeviceVec<double> absmaxTmp_{dm_};
+    DeviceVec<double> scalarsD_{dm_};   // [0..2] sampled maxima
+    DeviceVec<uint32_t> scalarsU_{dm_}; // [0] scan total, [1] out-of-tile events, [2] relocations
 
-    uint32_t numUsedGridNodes_ = 0;
-    uint32_t numUsedVoxels_ = 0;
-    double frameDt_;
-    double dt_ = 0.0;
-    double elapsedTime_ = 0.0;
-    double elapsedTimeThisFrame_ = 0.0;
     uint32_t lastSorIterations_ = 0;
     uint32_t lastSolveAttempts_ = 0;
     double lastResidual_ = 0.0;
@@ -120,6 +139,8 @@ private:
     uint64_t moveSolidsCounter_ = 0;
     double sorThreshold_;
     uint32_t sorMaxIterations_;
+
+    friend class ParticleSystemTester;
 };
 
 }  // namespace flip2

@@ -231,7 +231,7 @@ mark_voxels_kernel(uint32_t B, uint32_t n, Grid g, const uint32_t* __restrict__ 
 __global__ void __launch_bounds__(64)
 compact_voxels_kernel(const uint64_t* __restrict__ usedMask, const uint64_t* __restrict__ solidMask,
                       const uint32_t* __restrict__ nodeOff, uint32_t* __restrict__ voxelIDs,
-                      int8_t* __restrict__ solids) {
+                      char* __restrict__ solids) {
     const uint32_t b = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t start = (b == 0) ? 0u : nodeOff[b - 1];
@@ -243,7 +243,7 @@ compact_voxels_kernel(const uint64_t* __restrict__ usedMask, const uint64_t* __r
         if ((m >> lane) & 1ull) {
             const uint32_t r = prefix + (uint32_t)__popcll(m & lt);
             voxelIDs[start + r] = w * 64 + lane;
-            solids[start + r] = (int8_t)((sm >> lane) & 1ull);
+            solids[start + r] = (char)((sm >> lane) & 1ull);
         }
         prefix += (uint32_t)__popcll(m);
     }
@@ -257,7 +257,7 @@ p2g_kernel(uint32_t B, uint32_t n, Grid g, const uint32_t* __restrict__ keys, co
            const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
            const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
            const uint32_t* __restrict__ nodeOff, const uint32_t* __restrict__ voxelIDs,
-           const int8_t* __restrict__ solids, double* __restrict__ Ux, double* __restrict__ Uy,
+           const char* __restrict__ solids, double* __restrict__ Ux, double* __restrict__ Uy,
            double* __restrict__ Uz) {
     __shared__ double s_u[3][NV];
     __shared__ double s_w[3][NV];
@@ -311,7 +311,7 @@ p2g_kernel(uint32_t B, uint32_t n, Grid g, const uint32_t* __restrict__ keys, co
 }
 
 // applyGravityKernel / removeGravityKernel (voxelSolveFunctions.cu:160-175).
-__global__ void gravity_kernel(uint32_t V, const int8_t* __restrict__ solids, double* __restrict__ Uy, double gdt,
+__global__ void gravity_kernel(uint32_t V, const char* __restrict__ solids, double* __restrict__ Uy, double gdt,
                                int remove) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V || solids[i]) return;
@@ -444,7 +444,7 @@ __device__ inline bool is_red(uint32_t id) {
 // one of the last (black) half-sweep: non-updated voxels hold 0.
 __global__ void __launch_bounds__(256)
 sor_kernel(const uint32_t* __restrict__ nodeOff, const uint32_t* __restrict__ voxelIDs,
-           const int8_t* __restrict__ solids, const double* __restrict__ divU, double* __restrict__ p,
+           const char* __restrict__ solids, const double* __restrict__ divU, double* __restrict__ p,
            double* __restrict__ residuals, double scale, double w, uint32_t iters) {
     __shared__ double s_p[NV];
     __shared__ double s_d[NV];
@@ -566,7 +566,7 @@ sampled_absmax_kernel(Ptr3 arrays, uint32_t n, double* __restrict__ tmp, double*
 // pressureToAcceleration (voxelSolveFunctions.cu:415-479), three components.
 __global__ void __launch_bounds__(256)
 velocity_update_kernel(const uint32_t* __restrict__ nodeOff, const uint32_t* __restrict__ voxelIDs,
-                       const int8_t* __restrict__ solids, const double* __restrict__ p, double* __restrict__ Ux,
+                       const char* __restrict__ solids, const double* __restrict__ p, double* __restrict__ Ux,
                        double* __restrict__ Uy, double* __restrict__ Uz, double scale) {
     __shared__ double s_p[NV];
     __shared__ uint8_t s_sol[NV];
@@ -689,7 +689,7 @@ __global__ void __launch_bounds__(64)
 move_solids_kernel(uint32_t B, uint32_t n, Grid g, const uint32_t* __restrict__ keys,
                    const uint32_t* __restrict__ first, double* __restrict__ px, double* __restrict__ py,
                    double* __restrict__ pz, const uint32_t* __restrict__ nodeOff,
-                   const uint32_t* __restrict__ voxelIDs, const int8_t* __restrict__ solids, uint64_t counter,
+                   const uint32_t* __restrict__ voxelIDs, const char* __restrict__ solids, uint64_t counter,
                    uint32_t* __restrict__ relocated) {
     __shared__ uint8_t s_sol[NV];
     const uint32_t b = blockIdx.x;
@@ -787,7 +787,7 @@ void launch_mark_voxels(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, co
 }
 
 void launch_compact_voxels(uint32_t B, const uint64_t* usedMask, const uint64_t* solidMask, const uint32_t* nodeOff,
-                           uint32_t* voxelIDs, int8_t* solids, hipStream_t s) {
+                           uint32_t* voxelIDs, char* solids, hipStream_t s) {
     if (!B) return;
     FLIP2_KERNEL(compact_voxels_kernel, dim3(B), dim3(64), 0, s, usedMask, solidMask, nodeOff, voxelIDs,
                        solids);
@@ -796,7 +796,7 @@ void launch_compact_voxels(uint32_t B, const uint64_t* usedMask, const uint64_t*
 
 void launch_p2g(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, const uint32_t* first, const double* px,
                 const double* py, const double* pz, const double* vx, const double* vy, const double* vz,
-                const uint32_t* nodeOff, const uint32_t* voxelIDs, const int8_t* solids, double* Ux, double* Uy,
+                const uint32_t* nodeOff, const uint32_t* voxelIDs, const char* solids, double* Ux, double* Uy,
                 double* Uz, hipStream_t s) {
     if (!B) return;
     FLIP2_KERNEL(p2g_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, vx, vy, vz, nodeOff,
@@ -804,7 +804,7 @@ void launch_p2g(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, const uint
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_gravity(uint32_t V, const int8_t* solids, double* Uy, double gdt, bool remove, hipStream_t s) {
+void launch_gravity(uint32_t V, const char* solids, double* Uy, double gdt, bool remove, hipStream_t s) {
     if (!V) return;
     FLIP2_KERNEL(gravity_kernel, grid1d(V, 256), dim3(256), 0, s, V, solids, Uy, gdt, remove ? 1 : 0);
     FLIP2_LAUNCH_CHECK();
@@ -819,7 +819,7 @@ void launch_divu(uint32_t B, Grid g, const uint32_t* keys, const uint32_t* first
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_sor(uint32_t B, const uint32_t* nodeOff, const uint32_t* voxelIDs, const int8_t* solids,
+void launch_sor(uint32_t B, const uint32_t* nodeOff, const uint32_t* voxelIDs, const char* solids,
                 const double* divU, double* p, double* residuals, double scale, double w, uint32_t iters,
                 hipStream_t s) {
     if (!B) return;
@@ -838,7 +838,7 @@ void launch_sampled_absmax(Ptr3 arrays, int count, uint32_t n, double* tmp, doub
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_velocity_update(uint32_t B, const uint32_t* nodeOff, const uint32_t* voxelIDs, const int8_t* solids,
+void launch_velocity_update(uint32_t B, const uint32_t* nodeOff, const uint32_t* voxelIDs, const char* solids,
                             const double* p, double* Ux, double* Uy, double* Uz, double scale, hipStream_t s) {
     if (!B) return;
     FLIP2_KERNEL(velocity_update_kernel, dim3(B), dim3(256), 0, s, nodeOff, voxelIDs, solids, p, Ux, Uy, Uz,
@@ -864,7 +864,7 @@ void launch_advect(uint32_t n, double dt, double* px, double* py, double* pz, co
 
 void launch_move_solids(uint32_t B, uint32_t n, Grid g, const uint32_t* keys, const uint32_t* first, double* px,
                         double* py, double* pz, const uint32_t* nodeOff, const uint32_t* voxelIDs,
-                        const int8_t* solids, uint64_t counter, uint32_t* relocated, hipStream_t s) {
+                        const char* solids, uint64_t counter, uint32_t* relocated, hipStream_t s) {
     if (!B) return;
     FLIP2_KERNEL(move_solids_kernel, dim3(B), dim3(64), 0, s, B, n, g, keys, first, px, py, pz, nodeOff,
                        voxelIDs, solids, counter, relocated);

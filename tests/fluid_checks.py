@@ -95,7 +95,7 @@ def staged_step_vs_oracle(sim, grid, oracle, dt=1.0 / 240.0, gravity=-9.81, flip
     _, _, b_after = oracle.ns_assemble(counts, grid, h / dt, *[sim.get(c).astype(np.float64) for c in "UVW"],
                                        solid=solid)
     err["divergence after / before"] = float(np.linalg.norm(b_after) / np.linalg.norm(b))
-    assert err["divergence after / before"] <= 2e-6
+    assert err["divergence after / before"] <= 2e-6, err
     # g2p (state in sorted order after the transfer)
     xs, ys, zs, vxs, vys, vzs = sim.particles()
     Fn = [sim.get(c) for c in ("U", "V", "W", "U0", "V0", "W0")]
