@@ -274,7 +274,7 @@ void FluidSim::ensureSlots(uint32_t numActive) {
     cellType_.resizeAsync(cells, stream_);
     coef_.resizeAsync(cells, stream_);
     for (auto* a : {&b64_, &x64_, &r64_}) a->resizeAsync(cells, stream_);
-    for (auto* a : {&r_, &z_, &p_, &q_, &e_, &precon_, &aux0_, &aux1_}) a->resizeAsync(cells, stream_);
+    for (auto* a : {&r_, &z_, &p_, &p2_, &q_, &e_, &precon_, &aux0_, &aux1_}) a->resizeAsync(cells, stream_);
     partials_.resizeAsync(cap * (uint32_t)ns::kMaxFlexDirections + 64u, stream_);
     slotCapacity_ = cap;
     ringSlots_ = 0;
@@ -470,32 +470,6 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     for (uint32_t refine = 0; refine < 12 && total < cap; ++refine) {
         const double innerTol = std::max(0.5 * cfg_.tolerance / rel, 2e-5);
         const double ref = rel * st.bNorm;  // norm of the FP32 right-hand side of this refinement
-        ns::launch_refine_start(na, r64_.devPtr(), r_.devPtr(), e_.devPtr(), coef_.devPtr(), reduce(),
-                                scal_.devPtr() + 1, stream_);
-        if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
-        if (jac) {
-            ns::launch_jacobi_init(na, r_.devPtr(), coef_.devPtr(), z_.devPtr(), p_.devPtr(), reduce(), H, stream_);
-        } else if (flex && itCarry == 0) {
-            // p_0 = z_0; H[0] = p_0 . r_0 (the step-length numerator)
-            applyLearned(na);
-            ns::FlexDirections none{};
-            ns::launch_flex_direction(na, z_.devPtr(), none, zq_.devPtr(), r_.devPtr(), ringP(0), reduce(), H,
-                                      stream_);
-        } else if (flex) {
-            // the apply and direction of iteration itCarry, on the replaced
-            // residual: p = z - sum_i (z . q_i / p_i . q_i) p_i
-            applyLearned(na);
-            const ns::FlexDirections d = kept(itCarry - 1, H);
-            ns::launch_flex_dots(na, z_.devPtr(), d, reduce(), zq_.devPtr(), stream_);
-            ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(itCarry), reduce(),
-                                      H + 4u * itCarry, stream_);
-        } else {
-            ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), H, stream_);
-            ns::launch_direction(na, H, H, z_.devPtr(), p_.devPtr(), true, stream_);
-        }
-        const uint32_t it0 = flex ? itCarry : 0u;
-        uint32_t it = it0;
-        bool innerDone = false;
         // the residual norms of iterations first + 1 .. first + count (read
         // back from the history) reached the inner tolerance
         auto converged = [&](uint32_t first, uint32_t count) {
@@ -506,29 +480,83 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             }
             return false;
         };
-        while (!innerDone && total + (it - it0) < cap) {
-            const uint32_t batch = std::min(every, cap - total - (it - it0));
-            const uint32_t first = it;
-            for (uint32_t k = 0; k < batch; ++k, ++it) {
-                double* hk = H + 4u * it;
-                double* hn = H + 4u * (it + 1);
-                float* p = flex ? ringP(it) : p_.devPtr();
-                float* q = flex ? ringQ(it) : q_.devPtr();
-                ns::launch_matvec_dot(na, p, coef_.devPtr(), nbrSlot_.devPtr(), q, reduce(), hk + 2, stream_);
-                ns::launch_update(na, hk, hk + 2, p, q, coef_.devPtr(), e_.devPtr(), r_.devPtr(), z_.devPtr(), jac,
-                                  reduce(), hn, stream_);
-                if (flex && k + 1 == batch && converged(first, batch)) {
-                    // checked before the apply, the costly part of a
-                    // flexible-CG iteration: after convergence it is not needed
-                    innerDone = true;
+        if (!flex) {
+            // Jacobi / MIC(0): two kernels per iteration (direction + matvec,
+            // update + preconditioner), p alternating between two buffers;
+            // the refinement starts with r = (float) r64, e = 0, z = M^-1 r
+            ns::CgFused cg{};
+            cg.coef = coef_.devPtr();
+            cg.nbrSlot = nbrSlot_.devPtr();
+            cg.precon = jac ? nullptr : precon_.devPtr();
+            cg.q = q_.devPtr();
+            cg.e = e_.devPtr();
+            cg.r = r_.devPtr();
+            cg.z = z_.devPtr();
+            cg.na = na;
+            cg.groupsA = ns::cg_groups(na, false);
+            cg.groupsB = ns::cg_groups(na, !jac);
+            cg.partA = partials_.devPtr();
+            cg.partB = partials_.devPtr() + cg.groupsA;
+            cg.hist = H;
+            float* pbuf[2] = {p_.devPtr(), p2_.devPtr()};
+            ns::launch_cg_update_precond(cg, 0, true, !jac, r64_.devPtr(), nullptr, stream_);
+            if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
+            uint32_t it = 0;
+            bool innerDone = false;
+            while (!innerDone && total + it < cap) {
+                const uint32_t batch = std::min(every, cap - total - it);
+                const uint32_t first = it;
+                for (uint32_t k = 0; k < batch; ++k, ++it) {
+                    float* pNew = pbuf[it & 1u];
+                    ns::launch_cg_direction_matvec(cg, it, it == 0, pbuf[(it + 1u) & 1u], pNew, stream_);
+                    ns::launch_cg_update_precond(cg, it, false, !jac, nullptr, pNew, stream_);
                     if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
-                    ++it;
-                    break;
                 }
-                if (prec == kPrecMIC0) {
-                    ns::launch_mic_apply(na, r_.devPtr(), coef_.devPtr(), precon_.devPtr(), z_.devPtr(), reduce(), hn,
-                                         stream_);
-                } else if (flex) {
+                ns::launch_cg_publish(cg, it, stream_);  // entry `it`: rr after the batch's last update
+                innerDone = converged(first, batch);
+            }
+            total += it;
+        } else {
+            // flexible CG with the learned preconditioner
+            ns::launch_refine_start(na, r64_.devPtr(), r_.devPtr(), e_.devPtr(), coef_.devPtr(), reduce(),
+                                    scal_.devPtr() + 1, stream_);
+            if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
+            if (itCarry == 0) {
+                // p_0 = z_0; H[0] = p_0 . r_0 (the step-length numerator)
+                applyLearned(na);
+                ns::FlexDirections none{};
+                ns::launch_flex_direction(na, z_.devPtr(), none, zq_.devPtr(), r_.devPtr(), ringP(0), reduce(), H,
+                                          stream_);
+            } else {
+                // the apply and direction of iteration itCarry, on the replaced
+                // residual: p = z - sum_i (z . q_i / p_i . q_i) p_i
+                applyLearned(na);
+                const ns::FlexDirections d = kept(itCarry - 1, H);
+                ns::launch_flex_dots(na, z_.devPtr(), d, reduce(), zq_.devPtr(), stream_);
+                ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(itCarry), reduce(),
+                                          H + 4u * itCarry, stream_);
+            }
+            const uint32_t it0 = itCarry;
+            uint32_t it = it0;
+            bool innerDone = false;
+            while (!innerDone && total + (it - it0) < cap) {
+                const uint32_t batch = std::min(every, cap - total - (it - it0));
+                const uint32_t first = it;
+                for (uint32_t k = 0; k < batch; ++k, ++it) {
+                    double* hk = H + 4u * it;
+                    double* hn = H + 4u * (it + 1);
+                    ns::launch_matvec_dot(na, ringP(it), coef_.devPtr(), nbrSlot_.devPtr(), ringQ(it), reduce(),
+                                          hk + 2, stream_);
+                    ns::launch_update(na, hk, hk + 2, ringP(it), ringQ(it), coef_.devPtr(), e_.devPtr(), r_.devPtr(),
+                                      z_.devPtr(), false, reduce(), hn, stream_);
+                    if (k + 1 == batch && converged(first, batch)) {
+                        // checked before the apply, the costly part of a
+                        // flexible-CG iteration: after convergence it is not needed
+                        innerDone = true;
+                        if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
+                        ++it;
+                        break;
+                    }
                     // p = z - sum_i (z . q_i / p_i . q_i) p_i over the kept
                     // directions; hn[0] = p . r
                     applyLearned(na);
@@ -536,14 +564,12 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                     ns::launch_flex_dots(na, z_.devPtr(), d, reduce(), zq_.devPtr(), stream_);
                     ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(it + 1), reduce(),
                                               hn, stream_);
+                    if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
                 }
-                if (!flex) ns::launch_direction(na, hn, hk, z_.devPtr(), p_.devPtr(), false, stream_);
-                if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
             }
-            if (!flex && converged(first, batch)) innerDone = true;
+            total += it - it0;
+            itCarry = it;
         }
-        total += it - it0;
-        if (flex) itCarry = it;
         ns::launch_accumulate(na, e_.devPtr(), x64_.devPtr(), coef_.devPtr(), stream_);
         ns::launch_true_residual(na, b64_.devPtr(), x64_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), r64_.devPtr(),
                                  reduce(), scal_.devPtr() + 2, stream_);

@@ -174,7 +174,7 @@ private:
     DeviceVec<uint8_t> cellType_{dm_};
     DeviceVec<uint16_t> coef_{dm_};
     DeviceVec<double> b64_{dm_}, x64_{dm_}, r64_{dm_};
-    DeviceVec<float> r_{dm_}, z_{dm_}, p_{dm_}, q_{dm_}, e_{dm_}, precon_{dm_}, aux0_{dm_}, aux1_{dm_};
+    DeviceVec<float> r_{dm_}, z_{dm_}, p_{dm_}, p2_{dm_}, q_{dm_}, e_{dm_}, precon_{dm_}, aux0_{dm_}, aux1_{dm_};
     DeviceVec<float> pRing_{dm_}, qRing_{dm_};  // flexible CG directions, learnedDirections + 1 slots
     DeviceVec<double> zq_{dm_};
     uint32_t ringSlots_ = 0;

@@ -15,8 +15,10 @@
 // Mixed precision: the operator, the preconditioners and the Krylov vectors
 // of the inner solve are FP32; every dot product accumulates in FP64; the
 // right-hand side, the solution and the true residual b - A x are FP64
-// (iterative refinement).  Reductions end in the last workgroup of the
-// kernel that produced the partials, in a fixed order (deterministic).
+// (iterative refinement).  Reductions are deterministic: every workgroup
+// stores its partial; the two PCG kernels of an iteration sum the other's
+// partials at their start, other kernels are followed by a one-workgroup
+// summing kernel, always in workgroup order.
 #include "ns_solver.hpp"
 
 namespace flip2 {
@@ -325,24 +327,6 @@ refine_start_kernel(const double* __restrict__ r64, float* __restrict__ r, float
     store_partials(&t, 1, partials);
 }
 
-// Jacobi z = r / diag, p = z (first iteration), rz into out[0].
-__global__ void __launch_bounds__(kWG)
-jacobi_init_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, float* __restrict__ z,
-                   float* __restrict__ p, double* __restrict__ partials) {
-    __shared__ double s_tmp[16];
-    const uint32_t base = blockIdx.x * 512u;
-    double rz = 0.0;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
-        const uint16_t cf = coef[base + l];
-        const float zv = cf ? r[base + l] / (float)(cf & 7u) : 0.f;
-        z[base + l] = zv;
-        p[base + l] = zv;
-        rz += (double)r[base + l] * (double)zv;
-    }
-    const double t = block_sum(rz, s_tmp);
-    store_partials(&t, 1, partials);
-}
-
 // q = A p; pq into out[0].
 __global__ void __launch_bounds__(kWG)
 matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
@@ -389,17 +373,6 @@ update_kernel(const double* __restrict__ rzOld, const double* __restrict__ pq, c
     vals[0] = block_sum(rz, s_tmp);
     vals[1] = block_sum(rr, s_tmp);
     store_partials(vals, 2, partials);
-}
-
-// rz = r . z (after a separate preconditioner apply).
-__global__ void __launch_bounds__(kWG)
-dot_kernel(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ partials) {
-    __shared__ double s_tmp[16];
-    const uint32_t base = blockIdx.x * 512u;
-    double s = 0.0;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) s += (double)a[base + l] * (double)b[base + l];
-    const double t = block_sum(s, s_tmp);
-    store_partials(&t, 1, partials);
 }
 
 // Damped-Jacobi smoothing around an external (learned) correction:
@@ -466,15 +439,6 @@ postsmooth_kernel(const float* __restrict__ r, const float* __restrict__ y, cons
         const float xv = sx[l];
         z[v] = xv + omega * (r[v] - apply_A_with(slot, l, cf, nbrSlot, xs)) / (float)(cf & 7u);
     }
-}
-
-// p = z + beta p, beta = rzNew / rzOld (p = z when first != 0).
-__global__ void __launch_bounds__(kWG)
-direction_kernel(const double* __restrict__ rzNew, const double* __restrict__ rzOld, const float* __restrict__ z,
-                 float* __restrict__ p, int first) {
-    const float beta = (!first && *rzOld != 0.0) ? (float)(*rzNew / *rzOld) : 0.f;
-    const uint32_t base = blockIdx.x * 512u;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) p[base + l] = fmaf(beta, p[base + l], z[base + l]);
 }
 
 template <int N>  // kept directions (d.count == N)
@@ -657,72 +621,257 @@ mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon)
     for (uint32_t l = threadIdx.x; l < 512u; l += 64u) precon[base + l] = s_pc[l];
 }
 
-// z = M^-1 r with the brick MIC(0) factors (forward and backward plane
-// sweeps); rz into out[0].
-__global__ void __launch_bounds__(64)
-mic_apply_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, const float* __restrict__ precon,
-                 float* __restrict__ z, double* __restrict__ partials) {
-    __shared__ float s_q[512];
-    __shared__ float s_pc[512];
-    __shared__ uint16_t s_cf[512];
-    __shared__ double s_tmp[16];
-    const uint32_t base = blockIdx.x * 512u;
-    for (uint32_t l = threadIdx.x; l < 512u; l += 64u) {
-        s_cf[l] = coef[base + l];
-        s_pc[l] = precon[base + l];
-        s_q[l] = 0.f;
-    }
+// ---------------------------------------------------------------------------
+// PCG in two kernels per iteration (Jacobi and brick-local MIC(0)).
+//
+//   direction+matvec (iteration k):  beta = rz_k / rz_{k-1};  p = z + beta p';
+//                                    q = A p;  partial p.q
+//   update+precondition:             alpha = rz_k / p.q;  e += alpha p;
+//                                    r -= alpha q;  z = M^-1 r;  partials r.z, r.r
+// Each kernel runs a bounded number of workgroups (each walks a contiguous
+// range of slots) and starts by summing the other kernel's partials itself,
+// in a fixed order that every workgroup repeats identically, so no separate
+// reduction launch is needed and the sums stay deterministic.  Workgroup 0
+// also records the sums in the iteration history (rz, rr, pq per iteration).
+// p alternates between two buffers (the kernel reads the previous direction
+// at neighbour cells owned by other workgroups while writing the new one).
+// ---------------------------------------------------------------------------
+namespace {
+
+// FP64 sum over the workgroup, result in every thread.
+__device__ inline double block_sum_all(double v, double* s_tmp) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();
-    const uint32_t stride[3] = {1u, 8u, 64u};
-    for (int s = 0; s <= 21; ++s) {
-        const int beg = c_planes.start[s], end = c_planes.start[s + 1];
-        for (int t = beg + (int)threadIdx.x; t < end; t += 64) {
-            const uint32_t l = c_planes.cells[t];
-            const uint16_t cf = s_cf[l];
-            float q = 0.f;
-            if (cf) {
-                float acc = r[base + l];
-                const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    if (lc[ax] == 0u) continue;
-                    const uint32_t m = l - stride[ax];
-                    acc -= a_plus(s_cf[m], ax, m) * s_pc[m] * s_q[m];
-                }
-                q = acc * s_pc[l];
-            }
-            s_q[l] = q;
+    if (lane == 0) s_tmp[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_tmp[w];
+    return r;
+}
+
+// Sum of count partials, identical in every workgroup (fixed order).
+__device__ inline double sum_partials(const double* __restrict__ part, uint32_t count, double* s_tmp) {
+    double a = 0.0;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) a += part[i];
+    return block_sum_all(a, s_tmp);
+}
+
+// The slot range [beg, end) of workgroup w of `groups`: each XCD (blocks are
+// dealt to the 8 XCDs round-robin) takes a contiguous eighth, so neighbouring
+// slots share an L2.
+__device__ inline void slot_range(uint32_t na, uint32_t groups, uint32_t& beg, uint32_t& end) {
+    const uint32_t w = blockIdx.x;
+    const uint32_t perXcd = (groups + 7u) >> 3;
+    const uint32_t order = (w & 7u) * perXcd + (w >> 3);
+    const uint32_t slotsPer = (na + groups - 1u) / groups;
+    beg = min(na, order * slotsPer);
+    end = min(na, beg + slotsPer);
+}
+
+}  // namespace
+
+// p = z + beta p_old (p = z when first), q = A p; partial p.q per workgroup.
+__global__ void __launch_bounds__(kWG)
+cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __restrict__ pOld,
+                           float* __restrict__ pNew) {
+    __shared__ double s_tmp[16];
+    __shared__ float s_p[512];
+    const double rz = sum_partials(c.partB, c.groupsB, s_tmp);
+    const double rr = sum_partials(c.partB + c.groupsB, c.groupsB, s_tmp);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.hist[4u * it] = rz;
+        c.hist[4u * it + 1u] = rr;
+    }
+    float beta = 0.f;
+    if (!first) {
+        const double rzOld = c.hist[4u * (it - 1u)];
+        beta = rzOld != 0.0 ? (float)(rz / rzOld) : 0.f;
+    }
+    uint32_t beg, end;
+    slot_range(c.na, c.groupsA, beg, end);
+    double pq = 0.0;
+    for (uint32_t slot = beg; slot < end; ++slot) {
+        const uint32_t base = slot * 512u;
+        for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
+            const uint32_t v = base + l;
+            s_p[l] = c.coef[v] ? (first ? c.z[v] : fmaf(beta, pOld[v], c.z[v])) : 0.f;
         }
         __syncthreads();
-    }
-    for (int s = 21; s >= 0; --s) {
-        const int beg = c_planes.start[s], end = c_planes.start[s + 1];
-        for (int t = beg + (int)threadIdx.x; t < end; t += 64) {
-            const uint32_t l = c_planes.cells[t];
-            const uint16_t cf = s_cf[l];
-            float zz = 0.f;
-            if (cf) {
-                float acc = s_q[l];
-                const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    if (lc[ax] == 7u) continue;
-                    const uint32_t m = l + stride[ax];
-                    acc -= a_plus(cf, ax, l) * s_pc[l] * s_q[m];
-                }
-                zz = acc * s_pc[l];
-            }
-            s_q[l] = zz;  // z overwrites q in place (plane s+1.. already final)
+        auto px = [&](uint32_t i) {
+            if ((i >> 9) == slot) return s_p[i & 511u];
+            return first ? c.z[i] : fmaf(beta, pOld[i], c.z[i]);  // a fluid neighbour in another slot
+        };
+        for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
+            const uint32_t v = base + l;
+            const uint16_t cf = c.coef[v];
+            const float qv = apply_A_with(slot, l, cf, c.nbrSlot, px);
+            const float pv = s_p[l];
+            pNew[v] = pv;
+            c.q[v] = qv;
+            pq += (double)pv * (double)qv;
         }
-        __syncthreads();
+        __syncthreads();  // s_p is rewritten for the next slot
     }
-    double rz = 0.0;
-    for (uint32_t l = threadIdx.x; l < 512u; l += 64u) {
-        z[base + l] = s_q[l];
-        rz += (double)r[base + l] * (double)s_q[l];
+    const double t = block_sum(pq, s_tmp);
+    if (threadIdx.x == 0) c.partA[blockIdx.x] = t;
+}
+
+// e += alpha p, r -= alpha q (skipped when init: r = (float) r64, e = 0),
+// z = M^-1 r (Jacobi, or brick MIC(0) with one wave per slot); partials
+// r.z and r.r per workgroup.
+template <bool kMic>
+__global__ void __launch_bounds__(kWG)
+cg_update_precond_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64,
+                         const float* __restrict__ p) {
+    __shared__ double s_tmp[16];
+    __shared__ float s_q[kMic ? 4 : 1][512];
+    __shared__ float s_pc[kMic ? 4 : 1][512];
+    __shared__ uint16_t s_cf[kMic ? 4 : 1][512];
+    float alpha = 0.f;
+    if (!init) {
+        const double pq = sum_partials(c.partA, c.groupsA, s_tmp);
+        if (blockIdx.x == 0 && threadIdx.x == 0) c.hist[4u * it + 2u] = pq;
+        const double rz = c.hist[4u * it];
+        alpha = pq != 0.0 ? (float)(rz / pq) : 0.f;
     }
-    const double t = block_sum(rz, s_tmp);
-    store_partials(&t, 1, partials);
+    // e, r update of one cell; returns the new r
+    auto update = [&](uint32_t v, uint16_t cf) -> float {
+        if (!cf) {
+            if (init) {
+                c.r[v] = 0.f;
+                c.e[v] = 0.f;
+            }
+            return 0.f;
+        }
+        if (init) {
+            const float rv = (float)r64[v];
+            c.r[v] = rv;
+            c.e[v] = 0.f;
+            return rv;
+        }
+        c.e[v] = fmaf(alpha, p[v], c.e[v]);
+        const float rv = fmaf(-alpha, c.q[v], c.r[v]);
+        c.r[v] = rv;
+        return rv;
+    };
+    uint32_t beg, end;
+    slot_range(c.na, c.groupsB, beg, end);
+    double rz = 0.0, rr = 0.0;
+    if (!kMic) {
+        for (uint32_t slot = beg; slot < end; ++slot) {
+            for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
+                const uint32_t v = slot * 512u + l;
+                const uint16_t cf = c.coef[v];
+                const float rv = update(v, cf);
+                const float zv = cf ? rv / (float)(cf & 7u) : 0.f;
+                c.z[v] = zv;
+                rr += (double)rv * (double)rv;
+                rz += (double)rv * (double)zv;
+            }
+        }
+    } else {
+        // one wave per slot; the four waves step through the planes together
+        const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        float* sq = s_q[kMic ? wave : 0];
+        float* spc = s_pc[kMic ? wave : 0];
+        uint16_t* scf = s_cf[kMic ? wave : 0];
+        const uint32_t stride[3] = {1u, 8u, 64u};
+        for (uint32_t s0 = beg; s0 < end; s0 += 4u) {
+            const uint32_t slot = s0 + wave;
+            const bool mine = slot < end;
+            const uint32_t base = slot * 512u;
+            float rk[8];  // this lane's r values (cells lane + 64 k)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t l = lane + 64u * (uint32_t)k;
+                uint16_t cf = 0;
+                float rv = 0.f, pc = 0.f;
+                if (mine) {
+                    cf = c.coef[base + l];
+                    pc = c.precon[base + l];
+                    rv = update(base + l, cf);
+                    rr += (double)rv * (double)rv;
+                }
+                rk[k] = rv;
+                scf[l] = cf;
+                spc[l] = pc;
+                sq[l] = rv;  // r, replaced by the forward sweep's q plane by plane
+            }
+            __syncthreads();
+            for (int s = 0; s <= 21; ++s) {
+                const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
+                for (int t = pb + (int)lane; t < pe; t += 64) {
+                    const uint32_t l = c_planes.cells[t];
+                    const uint16_t cf = scf[l];
+                    float qv = 0.f;
+                    if (cf) {
+                        float acc = sq[l];
+                        const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
+#pragma unroll
+                        for (int ax = 0; ax < 3; ++ax) {
+                            if (lc[ax] == 0u) continue;
+                            const uint32_t m = l - stride[ax];
+                            acc -= a_plus(scf[m], ax, m) * spc[m] * sq[m];
+                        }
+                        qv = acc * spc[l];
+                    }
+                    sq[l] = qv;
+                }
+                __syncthreads();
+            }
+            for (int s = 21; s >= 0; --s) {
+                const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
+                for (int t = pb + (int)lane; t < pe; t += 64) {
+                    const uint32_t l = c_planes.cells[t];
+                    const uint16_t cf = scf[l];
+                    float zz = 0.f;
+                    if (cf) {
+                        float acc = sq[l];
+                        const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
+#pragma unroll
+                        for (int ax = 0; ax < 3; ++ax) {
+                            if (lc[ax] == 7u) continue;
+                            const uint32_t m = l + stride[ax];
+                            acc -= a_plus(cf, ax, l) * spc[l] * sq[m];
+                        }
+                        zz = acc * spc[l];
+                    }
+                    sq[l] = zz;
+                }
+                __syncthreads();
+            }
+            if (mine) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t l = lane + 64u * (uint32_t)k;
+                    const float zv = sq[l];
+                    c.z[base + l] = zv;
+                    rz += (double)rk[k] * (double)zv;
+                }
+            }
+            __syncthreads();  // the LDS planes are rewritten for the next slots
+        }
+    }
+    const double t0 = block_sum(rz, s_tmp);
+    const double t1 = block_sum(rr, s_tmp);
+    if (threadIdx.x == 0) {
+        c.partB[blockIdx.x] = t0;
+        c.partB[c.groupsB + blockIdx.x] = t1;
+    }
+}
+
+// History entry `it` (rz, rr) from the update kernel's partials (the next
+// direction kernel writes the same sums; this publishes them before it runs).
+__global__ void __launch_bounds__(kWG)
+cg_publish_kernel(CgFused c, uint32_t it) {
+    __shared__ double s_tmp[16];
+    const double rz = sum_partials(c.partB, c.groupsB, s_tmp);
+    const double rr = sum_partials(c.partB + c.groupsB, c.groupsB, s_tmp);
+    if (threadIdx.x == 0) {
+        c.hist[4u * it] = rz;
+        c.hist[4u * it + 1u] = rr;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1027,12 +1176,6 @@ void launch_refine_start(uint32_t numActive, const double* r64, float* r, float*
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
-void launch_jacobi_init(uint32_t numActive, const float* r, const uint16_t* coef, float* z, float* p, Reduce red,
-                        double* out, hipStream_t s) {
-    FLIP2_KERNEL(jacobi_init_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, z, p, red.partials);
-    FLIP2_LAUNCH_CHECK();
-    reduce_to(red, 1, numActive, out, s);
-}
 void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
                        Reduce red, double* out, hipStream_t s) {
     FLIP2_KERNEL(matvec_dot_kernel, dim3(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials);
@@ -1046,16 +1189,6 @@ void launch_update(uint32_t numActive, const double* rzOld, const double* pq, co
                        jacobi ? 1 : 0, red.partials);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 2, numActive, out, s);
-}
-void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, double* out, hipStream_t s) {
-    FLIP2_KERNEL(dot_kernel, dim3(numActive), dim3(kWG), 0, s, a, b, red.partials);
-    FLIP2_LAUNCH_CHECK();
-    reduce_to(red, 1, numActive, out, s);
-}
-void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
-                      bool first, hipStream_t s) {
-    FLIP2_KERNEL(direction_kernel, dim3(numActive), dim3(kWG), 0, s, rzNew, rzOld, z, p, first ? 1 : 0);
-    FLIP2_LAUNCH_CHECK();
 }
 void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot,
                       const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s) {
@@ -1127,11 +1260,28 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
     FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
     FLIP2_LAUNCH_CHECK();
 }
-void launch_mic_apply(uint32_t numActive, const float* r, const uint16_t* coef, const float* precon, float* z,
-                      Reduce red, double* out, hipStream_t s) {
-    FLIP2_KERNEL(mic_apply_kernel, dim3(numActive), dim3(64), 0, s, r, coef, precon, z, red.partials);
+uint32_t cg_groups(uint32_t numActive, bool micUpdate) {
+    if (!numActive) return 0;
+    const uint32_t per = micUpdate ? 4u : 1u;  // slots per workgroup step
+    const uint32_t k = ceil_div(numActive, per * kCgMaxGroups);
+    return ceil_div(numActive, per * k);
+}
+void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
+                                hipStream_t s) {
+    FLIP2_KERNEL(cg_direction_matvec_kernel, dim3(c.groupsA), dim3(kWG), 0, s, c, it, first ? 1 : 0, pOld, pNew);
     FLIP2_LAUNCH_CHECK();
-    reduce_to(red, 1, numActive, out, s);
+}
+void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
+                              hipStream_t s) {
+    if (mic)
+        FLIP2_KERNEL(cg_update_precond_kernel<true>, dim3(c.groupsB), dim3(kWG), 0, s, c, it, init ? 1 : 0, r64, p);
+    else
+        FLIP2_KERNEL(cg_update_precond_kernel<false>, dim3(c.groupsB), dim3(kWG), 0, s, c, it, init ? 1 : 0, r64, p);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_cg_publish(const CgFused& c, uint32_t it, hipStream_t s) {
+    FLIP2_KERNEL(cg_publish_kernel, dim3(1), dim3(kWG), 0, s, c, it);
+    FLIP2_LAUNCH_CHECK();
 }
 void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,

@@ -35,16 +35,11 @@ void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int
                      uint16_t* coef, double* b64, Reduce red, double* out, hipStream_t s);
 void launch_refine_start(uint32_t numActive, const double* r64, float* r, float* e, const uint16_t* coef, Reduce red,
                          double* out, hipStream_t s);
-void launch_jacobi_init(uint32_t numActive, const float* r, const uint16_t* coef, float* z, float* p, Reduce red,
-                        double* out, hipStream_t s);
 void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
                        Reduce red, double* out, hipStream_t s);
 void launch_update(uint32_t numActive, const double* rzOld, const double* pq, const float* p, const float* q,
                    const uint16_t* coef, float* e, float* r, float* z, bool jacobi, Reduce red, double* out,
                    hipStream_t s);
-void launch_dot(uint32_t numActive, const float* a, const float* b, Reduce red, double* out, hipStream_t s);
-void launch_direction(uint32_t numActive, const double* rzNew, const double* rzOld, const float* z, float* p,
-                      bool first, hipStream_t s);
 // Flexible CG with the last `count` directions kept (Notay's truncated FCG):
 // p_i, q_i = A p_i and pq_i = p_i . q_i (device scalar) of each kept direction.
 constexpr int kMaxFlexDirections = 16;
@@ -60,14 +55,43 @@ void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& 
 // p = z - sum_i (zq[i] / pq_i) p_i (p must not be one of the p_i); pr = p . r.
 void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirections& d, const double* zq,
                            const float* r, float* p, Reduce red, double* pr, hipStream_t s);
+// PCG (Jacobi or brick MIC(0)) in two kernels per iteration, each summing
+// the other's workgroup partials itself (ns_solver.hip).  History entry k
+// (4 doubles at hist + 4k): rz_k, rr_k, (p.q)_k.
+constexpr uint32_t kCgMaxGroups = 1024;
+struct CgFused {
+    const uint16_t* coef;
+    const int32_t* nbrSlot;
+    const float* precon;  // MIC(0) factors (null for Jacobi)
+    float* q;
+    float* e;
+    float* r;
+    float* z;
+    double* partA;  // groupsA partials (p.q) of the direction kernel
+    double* partB;  // 2 groupsB partials (r.z, then r.r) of the update kernel
+    double* hist;
+    uint32_t na, groupsA, groupsB;
+};
+// Workgroups of the two kernels for numActive slots (micUpdate: the update
+// kernel's MIC(0) form, four slots per workgroup step).
+uint32_t cg_groups(uint32_t numActive, bool micUpdate);
+// Iteration `it`: p = z (first) or z + (rz_it / rz_{it-1}) pOld into pNew;
+// q = A pNew; history rz_it, rr_it.
+void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
+                                hipStream_t s);
+// Iteration `it`: alpha = rz_it / (p.q); e += alpha p; r -= alpha q; z = M^-1 r;
+// history (p.q)_it.  init: instead r = (float) r64 (0 off the fluid), e = 0
+// (p unused), the start of a refinement.
+void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
+                              hipStream_t s);
+// History entry `it` (rz, rr) from the last update kernel's partials.
+void launch_cg_publish(const CgFused& c, uint32_t it, hipStream_t s);
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s);
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s);
 void launch_fill_f32(float* a, uint32_t count, float v, hipStream_t s);
 void launch_fill_f64(double* a, uint32_t count, double v, hipStream_t s);
 void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, hipStream_t s);
-void launch_mic_apply(uint32_t numActive, const float* r, const uint16_t* coef, const float* precon, float* z,
-                      Reduce red, double* out, hipStream_t s);
 void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,
                             float* U, float* V, float* W, hipStream_t s);

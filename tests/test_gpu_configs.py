@@ -57,6 +57,7 @@ def solve_and_project(sim, grid, oracle, solid=None, solver=None):
     (what storing an exact projection in FP32 faces leaves)."""
     h, dt = grid[6], 1.0 / 240.0
     cells = grid[3] * grid[4] * grid[5]
+    print("solve_and_project: reading faces", flush=True)
     counts = sim.get("counts")
     Fg = [sim.get(c).astype(np.float64) for c in "UVW"]
     ty, _, b = oracle.ns_assemble(counts, grid, h / dt, *Fg, solid=solid)
@@ -65,11 +66,13 @@ def solve_and_project(sim, grid, oracle, solid=None, solver=None):
         st = sim.last_solve()
     else:
         st = sim.solve_with(solver)
+    print(f"solve_and_project: solved ({st['iterations']} iterations); exact projection", flush=True)
     p = dense_from_slots(sim, sim.get("p"), cells)
     exact = [a.astype(np.float32) for a in oracle.ns_velocity_update(ty, grid, p, dt / h, *Fg)]
     del Fg, p
     _, _, b_exact = oracle.ns_assemble(counts, grid, h / dt, *exact, solid=solid)
     del exact
+    print("solve_and_project: projecting on the device", flush=True)
     sim.stage("project")
     _, _, b_after = oracle.ns_assemble(counts, grid, h / dt, *[sim.get(c) for c in "UVW"], solid=solid)
     nb = float(np.linalg.norm(b))
