@@ -493,8 +493,8 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             cg.r = r_.devPtr();
             cg.z = z_.devPtr();
             cg.na = na;
-            cg.groupsA = ns::cg_groups(na, false);
-            cg.groupsB = ns::cg_groups(na, !jac);
+            ns::cg_groups(na, false, cg.groupsA, cg.slotsA);
+            ns::cg_groups(na, !jac, cg.groupsB, cg.slotsB);
             cg.partA = partials_.devPtr();
             cg.partB = partials_.devPtr() + cg.groupsA;
             cg.hist = H;

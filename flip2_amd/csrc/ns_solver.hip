@@ -660,11 +660,11 @@ __device__ inline double sum_partials(const double* __restrict__ part, uint32_t 
 // The slot range [beg, end) of workgroup w of `groups`: each XCD (blocks are
 // dealt to the 8 XCDs round-robin) takes a contiguous eighth, so neighbouring
 // slots share an L2.
-__device__ inline void slot_range(uint32_t na, uint32_t groups, uint32_t& beg, uint32_t& end) {
+// groups is at most 8 or a multiple of 8 (so workgroup -> order is a
+// bijection onto [0, groups)); groups * slotsPer >= na.
+__device__ inline void slot_range(uint32_t na, uint32_t groups, uint32_t slotsPer, uint32_t& beg, uint32_t& end) {
     const uint32_t w = blockIdx.x;
-    const uint32_t perXcd = (groups + 7u) >> 3;
-    const uint32_t order = (w & 7u) * perXcd + (w >> 3);
-    const uint32_t slotsPer = (na + groups - 1u) / groups;
+    const uint32_t order = groups <= 8u ? w : (w & 7u) * (groups >> 3) + (w >> 3);
     beg = min(na, order * slotsPer);
     end = min(na, beg + slotsPer);
 }
@@ -689,7 +689,7 @@ cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __res
         beta = rzOld != 0.0 ? (float)(rz / rzOld) : 0.f;
     }
     uint32_t beg, end;
-    slot_range(c.na, c.groupsA, beg, end);
+    slot_range(c.na, c.groupsA, c.slotsA, beg, end);
     double pq = 0.0;
     for (uint32_t slot = beg; slot < end; ++slot) {
         const uint32_t base = slot * 512u;
@@ -756,7 +756,7 @@ cg_update_precond_kernel(CgFused c, uint32_t it, int init, const double* __restr
         return rv;
     };
     uint32_t beg, end;
-    slot_range(c.na, c.groupsB, beg, end);
+    slot_range(c.na, c.groupsB, c.slotsB, beg, end);
     double rz = 0.0, rr = 0.0;
     if (!kMic) {
         for (uint32_t slot = beg; slot < end; ++slot) {
@@ -1260,11 +1260,13 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
     FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
     FLIP2_LAUNCH_CHECK();
 }
-uint32_t cg_groups(uint32_t numActive, bool micUpdate) {
-    if (!numActive) return 0;
+void cg_groups(uint32_t numActive, bool micUpdate, uint32_t& groups, uint32_t& slotsPer) {
+    groups = slotsPer = 0;
+    if (!numActive) return;
     const uint32_t per = micUpdate ? 4u : 1u;  // slots per workgroup step
-    const uint32_t k = ceil_div(numActive, per * kCgMaxGroups);
-    return ceil_div(numActive, per * k);
+    slotsPer = per * ceil_div(numActive, per * kCgMaxGroups);
+    groups = ceil_div(numActive, slotsPer);
+    if (groups > 8u) groups = (groups + 7u) & ~7u;  // whole rows of the 8 XCDs
 }
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s) {

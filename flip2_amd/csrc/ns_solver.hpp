@@ -70,11 +70,12 @@ struct CgFused {
     double* partA;  // groupsA partials (p.q) of the direction kernel
     double* partB;  // 2 groupsB partials (r.z, then r.r) of the update kernel
     double* hist;
-    uint32_t na, groupsA, groupsB;
+    uint32_t na, groupsA, groupsB, slotsA, slotsB;
 };
-// Workgroups of the two kernels for numActive slots (micUpdate: the update
-// kernel's MIC(0) form, four slots per workgroup step).
-uint32_t cg_groups(uint32_t numActive, bool micUpdate);
+// Workgroups of a kernel for numActive slots and the slots each walks
+// (micUpdate: the update kernel's MIC(0) form, four slots per workgroup
+// step); groups is at most 8 or a multiple of 8, at most kCgMaxGroups.
+void cg_groups(uint32_t numActive, bool micUpdate, uint32_t& groups, uint32_t& slotsPer);
 // Iteration `it`: p = z (first) or z + (rz_it / rz_{it-1}) pOld into pNew;
 // q = A pNew; history rz_it, rr_it.
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,

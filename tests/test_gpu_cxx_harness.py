@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "build", "engine", "flip2_tester")
 
 
-@pytest.mark.parametrize("n,cells,seed", [(20000, 24, 7), (3000, 16, 11), (60000, 32, 3)])
+@pytest.mark.parametrize("n,cells,seed", [(20000, 24, 7), (3000, 16, 11), (40000, 28, 3)])
 def test_cxx_harness(n, cells, seed):
     assert os.path.exists(EXE), "build/engine/flip2_tester is not built"
     r = subprocess.run([EXE, str(n), str(cells), str(seed)], capture_output=True, text=True, timeout=170)
