@@ -178,6 +178,22 @@ uint32_t orc_pressure_retry(double* dt, double threshold, orc_solve_cb solve, or
                             double* last);
 void orc_solve_frame(orc_particles* s, double fps);     /* particles.cu:784-803 */
 
+/* Sub-cell lists (cudaFindSubCell, particleToGridFunctions.cu:69-262) and
+ * per-voxel particle lists (perVoxelParticleListFunctions.cu:6-189): code
+ * the reference builds but never calls from its step.  touched* receive the
+ * inclusive scans of the per-particle counts; list* may be NULL (counts
+ * only).  Per-voxel lists skip entries outside [0, numVoxelsPerNode). */
+void orc_find_sub_cell(const double* px, const double* py, const double* pz, uint32_t n, orc_grid g,
+                       const uint32_t* gridPosition, uint32_t refinementLevel, double radius, uint32_t* touchedX,
+                       uint32_t* touchedY, uint32_t* touchedZ, uint32_t* listX, uint32_t* listY, uint32_t* listZ);
+void orc_sum_particles_per_node(uint32_t B, uint32_t n, const uint32_t* firstParticle, const uint32_t* touchedIncl,
+                                const uint32_t* subCellsDim, uint32_t numVoxelsPerNode, uint32_t* numNonZeroVoxels,
+                                uint32_t* numParticlesInVoxelLists);
+void orc_particle_list_create(uint32_t B, uint32_t n, uint32_t numVoxelsPerNode, const uint32_t* numUsedVoxelsPerNode,
+                              const uint32_t* firstParticleListIndexPerNode, const uint32_t* touchedIncl,
+                              const uint32_t* subCellsDim, const uint32_t* firstParticle, uint32_t* voxelIDs,
+                              uint32_t* listStart, uint32_t* particleLists);
+
 /* ------------------------------------------------------------------------ */
 /* North-star method (ns_oracle.c)                                          */
 /* ------------------------------------------------------------------------ */

@@ -120,6 +120,10 @@ def lib() -> ctypes.CDLL:
         l.orc_solve_frame.argtypes = [P, ctypes.c_double]
         l.orc_resize_voxels.argtypes = [P, ctypes.c_uint32]
         l.orc_resize_nodes.argtypes = [P, ctypes.c_uint32]
+        l.orc_find_sub_cell.argtypes = [vp] * 3 + [ctypes.c_uint32, OrcGrid, vp, ctypes.c_uint32,
+                                                    ctypes.c_double] + [vp] * 6
+        l.orc_sum_particles_per_node.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, ctypes.c_uint32, vp, vp]
+        l.orc_particle_list_create.argtypes = [ctypes.c_uint32] * 3 + [vp] * 8
         l.orc_ns_tiled_index.argtypes = [ctypes.c_uint32] * 3 + [OrcGrid]
         l.orc_ns_tiled_index.restype = ctypes.c_uint32
         l.orc_ns_bin.argtypes = [vp, vp, vp, ctypes.c_uint32, OrcGrid, vp]
@@ -148,6 +152,54 @@ def lib() -> ctypes.CDLL:
 
 def _p(a: np.ndarray) -> int:
     return a.ctypes.data
+
+
+def find_sub_cell(px, py, pz, grid, keys, refinement_level=1, radius=2.0):
+    """cudaFindSubCell: (inclusive scans of the per-particle counts, lists)
+    for the x, y and z velocity components."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (px, py, pz)]
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = k.size
+    t = [np.empty(n, np.uint32) for _ in range(3)]
+    g = OrcGrid(*grid)
+    lib().orc_find_sub_cell(*(_p(v) for v in a), n, g, _p(k), refinement_level, radius, *(_p(v) for v in t),
+                            None, None, None)
+    lists = [np.empty(int(v[-1]) if n else 0, np.uint32) for v in t]
+    if n:
+        t = [np.empty(n, np.uint32) for _ in range(3)]
+        lib().orc_find_sub_cell(*(_p(v) for v in a), n, g, _p(k), refinement_level, radius, *(_p(v) for v in t),
+                                *(_p(v) for v in lists))
+    return t, lists
+
+
+def sum_particles_per_node(first, touched_incl, sub_cells, n, num_voxels_per_node=512):
+    """cudaSumParticlesPerNodeAndWriteNumUsedVoxels: inclusive scans over the
+    nodes of (distinct voxels, list entries) of each node's particles."""
+    f = np.ascontiguousarray(first, np.uint32)
+    t = np.ascontiguousarray(touched_incl, np.uint32)
+    sc = np.ascontiguousarray(sub_cells, np.uint32)
+    nz = np.empty(f.size, np.uint32)
+    np_ = np.empty(f.size, np.uint32)
+    lib().orc_sum_particles_per_node(f.size, n, _p(f), _p(t), _p(sc) if sc.size else None, num_voxels_per_node,
+                                     _p(nz), _p(np_))
+    return nz, np_
+
+
+def particle_list_create(first, touched_incl, sub_cells, n, used_incl, lists_incl, num_voxels_per_node=512):
+    """cudaParticleListCreate: (voxelIDs, per-voxel list starts, lists)."""
+    f = np.ascontiguousarray(first, np.uint32)
+    t = np.ascontiguousarray(touched_incl, np.uint32)
+    sc = np.ascontiguousarray(sub_cells, np.uint32)
+    u = np.ascontiguousarray(used_incl, np.uint32)
+    li = np.ascontiguousarray(lists_incl, np.uint32)
+    V = int(u[-1]) if u.size else 0
+    L = int(li[-1]) if li.size else 0
+    vid = np.empty(max(V, 1), np.uint32)
+    start = np.empty(max(V, 1), np.uint32)
+    lists = np.empty(max(L, 1), np.uint32)
+    lib().orc_particle_list_create(f.size, n, num_voxels_per_node, _p(u), _p(li), _p(t), _p(sc) if sc.size else None,
+                                   _p(f), _p(vid), _p(start), _p(lists))
+    return vid[:V], start[:V], lists[:L]
 
 
 def find_grid_cell(px, py, pz, grid):

@@ -940,3 +940,160 @@ void orc_solve_frame(orc_particles* s, double fps) {
         orc_initialize(s);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Sub-cell lists and per-voxel particle lists                              */
+/* ------------------------------------------------------------------------ */
+
+/* The sub-cell geometry shared by subCellCreateNumSubCellsTouchedEachDimension
+ * and subCellCreateLists (particleToGridFunctions.cu:69-139, 141-205): the
+ * particle's offset inside its root cell plus the apron, in the reference's
+ * mixed float/double arithmetic -- apronCells = floorf(radius); sub-cell
+ * width cellSize / (2.0f * 2^level) in float; gridID * cellSize a float
+ * product; the sub-cell index floorf of the double quotient narrowed to
+ * float; distances through sqrtf of the double sum narrowed to float.  The
+ * checks run over x in [s - apron, s + apron) with unsigned bounds (an
+ * index below the apron wraps and the loop is empty).  Visits the touched
+ * sub-cells of one particle in the reference's loop order (x, then y, then
+ * z) and calls emit(dim, sub-cell index). */
+typedef void (*orc_sub_cell_emit)(void* ctx, int dim, uint32_t subCell);
+
+static void sub_cell_visit(double px, double py, double pz, orc_grid g, uint32_t key, uint32_t refinementLevel,
+                           double radius, orc_sub_cell_emit emit, void* ctx) {
+    const uint32_t xySize = g.sizeX * g.sizeY;
+    const uint32_t mod = key % xySize;
+    const uint32_t gz = key / xySize, gy = mod / g.sizeX, gx = mod % g.sizeX;
+    const uint32_t apron = (uint32_t)floorf((float)radius);
+    const double scw = (double)(g.cellSize / (2.0f * (float)(1u << refinementLevel)));
+    const double ox = ((px - (double)g.negX) - (double)((float)gx * g.cellSize)) + (double)apron * scw;
+    const double oy = ((py - (double)g.negY) - (double)((float)gy * g.cellSize)) + (double)apron * scw;
+    const double oz = ((pz - (double)g.negZ) - (double)((float)gz * g.cellSize)) + (double)apron * scw;
+    const uint32_t sx = sat_u32((double)floorf((float)(ox / scw)));
+    const uint32_t sy = sat_u32((double)floorf((float)(oy / scw)));
+    const uint32_t sz = sat_u32((double)floorf((float)(oz / scw)));
+    const double half = scw / 2.0;
+    const double rad = radius * scw;
+    const uint32_t nv = 2u * apron + (2u << refinementLevel);
+    for (uint32_t x = sx - apron; x < sx + apron; ++x)
+        for (uint32_t y = sy - apron; y < sy + apron; ++y)
+            for (uint32_t z = sz - apron; z < sz + apron; ++z) {
+                const double dx = ox - (double)x * scw, dy = oy - (double)y * scw, dz = oz - (double)z * scw;
+                const double xc = (double)sqrtf((float)(dx * dx + (dy + half) * (dy + half) + (dz + half) * (dz + half)));
+                const double yc = (double)sqrtf((float)((dx + half) * (dx + half) + dy * dy + (dz + half) * (dz + half)));
+                const double zc = (double)sqrtf((float)((dx + half) * (dx + half) + (dy + half) * (dy + half) + dz * dz));
+                const uint32_t id = x + y * nv + z * nv * nv;
+                if (xc < rad) emit(ctx, 0, id);
+                if (yc < rad) emit(ctx, 1, id);
+                if (zc < rad) emit(ctx, 2, id);
+            }
+}
+
+typedef struct {
+    uint32_t count[3];
+    uint32_t* list[3];
+    uint32_t at[3];
+} sub_cell_ctx;
+
+static void sub_cell_count(void* c, int dim, uint32_t id) {
+    (void)id;
+    ((sub_cell_ctx*)c)->count[dim]++;
+}
+static void sub_cell_write(void* c, int dim, uint32_t id) {
+    sub_cell_ctx* s = (sub_cell_ctx*)c;
+    s->list[dim][s->at[dim]++] = id;
+}
+
+/* cudaFindSubCell (particleToGridFunctions.cu:221-262): per-particle counts
+ * of touched sub-cells per velocity component, their inclusive scans
+ * (touched*), and the concatenated lists (list* may be NULL: counts only). */
+void orc_find_sub_cell(const double* px, const double* py, const double* pz, uint32_t n, orc_grid g,
+                       const uint32_t* gridPosition, uint32_t refinementLevel, double radius, uint32_t* touchedX,
+                       uint32_t* touchedY, uint32_t* touchedZ, uint32_t* listX, uint32_t* listY, uint32_t* listZ) {
+    uint32_t acc[3] = {0, 0, 0};
+    uint32_t* touched[3] = {touchedX, touchedY, touchedZ};
+    for (uint32_t i = 0; i < n; ++i) {
+        sub_cell_ctx c = {{0, 0, 0}, {listX, listY, listZ}, {acc[0], acc[1], acc[2]}};
+        if (listX && listY && listZ) sub_cell_visit(px[i], py[i], pz[i], g, gridPosition[i], refinementLevel, radius,
+                                                    sub_cell_write, &c);
+        else sub_cell_visit(px[i], py[i], pz[i], g, gridPosition[i], refinementLevel, radius, sub_cell_count, &c);
+        for (int d = 0; d < 3; ++d) {
+            acc[d] = (listX && listY && listZ) ? c.at[d] : acc[d] + c.count[d];
+            touched[d][i] = acc[d];
+        }
+    }
+}
+
+/* Entry range of particle i in a concatenated list (inclusive scan). */
+static uint32_t entry_begin(const uint32_t* incl, uint32_t i) { return i ? incl[i - 1] : 0u; }
+
+/* sumParticlesPerNode + cudaSumParticlesPerNodeAndWriteNumUsedVoxels
+ * (perVoxelParticleListFunctions.cu:6-81): per node, the number of list
+ * entries of its particles (numParticlesInVoxelLists) and of distinct
+ * voxels among them (numNonZeroVoxels), both then inclusive-scanned over the
+ * nodes.  Entries outside [0, numVoxelsPerNode) index past the reference's
+ * shared array (undefined there) and are skipped here. */
+void orc_sum_particles_per_node(uint32_t B, uint32_t n, const uint32_t* firstParticle, const uint32_t* touchedIncl,
+                                const uint32_t* subCellsDim, uint32_t numVoxelsPerNode, uint32_t* numNonZeroVoxels,
+                                uint32_t* numParticlesInVoxelLists) {
+    uint32_t* cnt = (uint32_t*)xcalloc(numVoxelsPerNode, sizeof(uint32_t));
+    uint32_t accV = 0, accP = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        memset(cnt, 0, numVoxelsPerNode * sizeof(uint32_t));
+        const uint32_t pe = b + 1 < B ? firstParticle[b + 1] : n;
+        for (uint32_t i = firstParticle[b]; i < pe; ++i)
+            for (uint32_t j = entry_begin(touchedIncl, i); j < touchedIncl[i]; ++j)
+                if (subCellsDim[j] < numVoxelsPerNode) cnt[subCellsDim[j]]++;
+        uint32_t sum = 0, nz = 0;
+        for (uint32_t v = 0; v < numVoxelsPerNode; ++v) {
+            sum += cnt[v];
+            nz += cnt[v] != 0;
+        }
+        accV += nz;
+        accP += sum;
+        numNonZeroVoxels[b] = accV;
+        numParticlesInVoxelLists[b] = accP;
+    }
+    free(cnt);
+}
+
+/* createParticleListStartIndices / cudaParticleListCreate
+ * (perVoxelParticleListFunctions.cu:85-189): per node, the used voxels in
+ * ascending voxel order (voxelIDs, node-local ids, at the node's offset
+ * numUsedVoxelsPerNode[b-1]), each with the start of its particle list
+ * (perVoxelParticleListStartIndices: the node's list offset
+ * firstParticleListIndexPerNode[b-1] plus the exclusive count of entries
+ * of lower voxels) and the list itself: the indices of the node's
+ * particles that touch the voxel, in ascending particle order. */
+void orc_particle_list_create(uint32_t B, uint32_t n, uint32_t numVoxelsPerNode, const uint32_t* numUsedVoxelsPerNode,
+                              const uint32_t* firstParticleListIndexPerNode, const uint32_t* touchedIncl,
+                              const uint32_t* subCellsDim, const uint32_t* firstParticle, uint32_t* voxelIDs,
+                              uint32_t* listStart, uint32_t* particleLists) {
+    uint32_t* cnt = (uint32_t*)xcalloc(numVoxelsPerNode, sizeof(uint32_t));
+    uint32_t* off = (uint32_t*)xcalloc(numVoxelsPerNode, sizeof(uint32_t));
+    for (uint32_t b = 0; b < B; ++b) {
+        memset(cnt, 0, numVoxelsPerNode * sizeof(uint32_t));
+        const uint32_t p0 = firstParticle[b], pe = b + 1 < B ? firstParticle[b + 1] : n;
+        for (uint32_t i = p0; i < pe; ++i)
+            for (uint32_t j = entry_begin(touchedIncl, i); j < touchedIncl[i]; ++j)
+                if (subCellsDim[j] < numVoxelsPerNode) cnt[subCellsDim[j]]++;
+        const uint32_t vBase = b ? numUsedVoxelsPerNode[b - 1] : 0u;
+        const uint32_t lBase = b ? firstParticleListIndexPerNode[b - 1] : 0u;
+        uint32_t used = 0, acc = 0;
+        for (uint32_t v = 0; v < numVoxelsPerNode; ++v) {
+            off[v] = acc;
+            if (cnt[v]) {
+                voxelIDs[vBase + used] = v;
+                listStart[vBase + used] = lBase + acc;
+                ++used;
+            }
+            acc += cnt[v];
+        }
+        for (uint32_t i = p0; i < pe; ++i)
+            for (uint32_t j = entry_begin(touchedIncl, i); j < touchedIncl[i]; ++j) {
+                const uint32_t v = subCellsDim[j];
+                if (v < numVoxelsPerNode) particleLists[lBase + off[v]++] = i;
+            }
+    }
+    free(cnt);
+    free(off);
+}
