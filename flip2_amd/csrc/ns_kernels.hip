@@ -159,8 +159,29 @@ __global__ void run_length_kernel(const uint32_t* __restrict__ keys, uint32_t n,
     const uint32_t k = min(keys[i], numCells);
     if (k >= numCells) return;
     if (i > 0 && min(keys[i - 1], numCells) == k) return;
+    // end of the run: a short walk covers the usual few particles per cell;
+    // a longer run is bracketed by doubling steps and bisected, so even one
+    // run of all n particles costs O(log n)
     uint32_t e = i + 1;
-    while (e < n && min(keys[e], numCells) == k) ++e;
+    const uint32_t walkEnd = min(n, i + 16u);
+    while (e < walkEnd && min(keys[e], numCells) == k) ++e;
+    if (e == walkEnd && e < n && min(keys[e], numCells) == k) {
+        uint32_t lo = e, step = 16u;  // keys[lo] == k
+        uint32_t hi = lo;
+        while (true) {
+            hi = (n - lo > step) ? lo + step : n;
+            if (hi == n || min(keys[hi], numCells) != k) break;
+            lo = hi;
+            step = step <= 0x40000000u ? step * 2u : step;
+        }
+        // keys[lo] == k; hi == n or keys[hi] != k: the run ends in (lo, hi]
+        while (hi - lo > 1u) {
+            const uint32_t mid = lo + (hi - lo) / 2u;
+            if (min(keys[mid], numCells) == k) lo = mid;
+            else hi = mid;
+        }
+        e = hi;
+    }
     counts[k] = e - i;
 }
 

@@ -69,3 +69,11 @@ def test_transfer_few_particles(flip2, oracle):
 def test_transfer_microbench_size(flip2, oracle):
     # 2^24 particles on 256^3: integer outputs bit-exact, faces vs FP64 oracle
     run_case(flip2, oracle, 1 << 24, 256, seed=2)
+
+
+def test_transfer_one_crowded_cell(flip2, oracle):
+    # every particle in one cell: one run of 100k keys (its end found by
+    # doubling steps and bisection, not a walk) plus runs of 15..17 keys
+    run_case(flip2, oracle, 100_000, 16, seed=10, lo=0.5, hi=0.5 + 0.9 / 16)
+    for n in (15, 16, 17, 33):
+        run_case(flip2, oracle, n, 16, seed=20 + n, lo=0.5, hi=0.5 + 0.9 / 16)
