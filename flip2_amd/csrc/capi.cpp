@@ -3,6 +3,7 @@
 #include "../../include/flip2.h"
 
 #include <cstring>
+#include <functional>
 #include <iterator>
 #include <mutex>
 #include <string>
@@ -196,6 +197,12 @@ void to_stats(const flip2::SolveStats& s, flip2_solve_stats* o) {
 #define NEED(p) FLIP2_REQUIRE((p) != nullptr, flip2::kErrArgument, #p " is NULL")
 
 }  // namespace
+
+namespace flip2 {
+// The C ABI's error convention (status code, message for flip2_last_error)
+// for entry points defined in other translation units.
+int capi_guarded(const std::function<void()>& f) { return guarded(f); }
+}  // namespace flip2
 
 extern "C" {
 

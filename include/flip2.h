@@ -96,6 +96,20 @@ int flip2_sort_pairs_u32(const uint32_t* d_keys_in, const uint32_t* d_vals_in, u
                          uint32_t* d_vals_out, uint32_t* d_keys_alt, uint32_t* d_vals_alt, uint32_t n,
                          uint32_t begin_bit, uint32_t end_bit, void* d_ws, size_t ws_bytes, void* stream);
 
+/* flip2_sort_pairs_u32 without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that forbid hipMalloc).  The
+ * workspace is a fixed device array of FLIP2_NOALLOC_SORT_SCRATCH_BYTES per
+ * GPU, held until the process ends (enough for 2^25 keys over 32 bits);
+ * larger requests return FLIP2_ERR_RANGE.  Same arguments and results as
+ * flip2_sort_pairs_u32 otherwise (caller-provided key/value and ping-pong
+ * buffers).  Call it with the stream's device current.  Calls that overlap on
+ * one GPU share the array and race: order them on one stream. */
+#define FLIP2_NOALLOC_SORT_SCRATCH_BYTES (34u << 20)
+int flip2_sort_pairs_u32_noalloc(const uint32_t* d_keys_in, const uint32_t* d_vals_in, uint32_t* d_keys_out,
+                                 uint32_t* d_vals_out, uint32_t* d_keys_alt, uint32_t* d_vals_alt, uint32_t n,
+                                 uint32_t begin_bit, uint32_t end_bit, void* stream);
+
 /* Clamp FP64 positions into the domain (written back) and compute u32 cell
  * keys x + y*sx + z*sx*sy.  Replaces cudaFindGridCell / rootCell
  * (particleToGridFunctions.cu:19-45, 216-219; particleToGridFunctions.hu:10). */
