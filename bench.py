@@ -47,8 +47,8 @@ NET_PATH = os.path.join(ROOT, "flip2_amd", "data", "c4_unet_16_32_64.npy")
 # solution and right-hand side FP64).
 SOLVER_BYTES_PER_FLUID_CELL = {
     "cg_direction_matvec_kernel": 4 + 4 + 2 + 4 + 4,  # z, p_old, coef -> p, q
-    "cg_update_precond_kernel": 4 + 4 + 4 + 4 + 2 + 4 + 4 + 4,  # p, q, e, r, coef -> e, r, z (Jacobi)
-    "cg_update_precond_mic": 4 + 4 + 4 + 4 + 2 + 4 + 4 + 4 + 4,  # ... and the MIC(0) factors
+    "cg_update_jacobi_kernel": 4 + 4 + 4 + 4 + 2 + 4 + 4 + 4,  # p, q, e, r, coef -> e, r, z
+    "cg_update_mic_kernel": 4 + 4 + 4 + 4 + 2 + 4 + 4 + 4 + 4,  # ... and the MIC(0) factors
     "matvec_dot_kernel": 4 + 2 + 4,              # p, coef -> q
     "update_kernel": 4 + 4 + 2 + 4 + 4 + 4 + 4,  # p, q, coef, e, r -> e, r
     "refine_start_kernel": 8 + 2 + 4 + 4,        # r64, coef -> r, e
@@ -60,8 +60,6 @@ SOLVER_BYTES_PER_FLUID_CELL = {
 def kernel_alg_bytes(name, n, cells, fluid_cells, jacobi=False):
     """Algorithmic bytes of one launch of `name` in a FLIP step (n particles,
     `cells` grid cells, `fluid_cells` fluid cells); None if not modelled."""
-    if name == "cg_update_precond_kernel" and not jacobi:
-        name = "cg_update_precond_mic"
     if name in SOLVER_BYTES_PER_FLUID_CELL:
         return SOLVER_BYTES_PER_FLUID_CELL[name] * fluid_cells
     per = {

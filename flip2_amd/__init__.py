@@ -141,6 +141,9 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_scan_u32": [_vp, _vp, _u32, ctypes.c_int, _vp, _vp, _sz, _vp],
         "flip2_sort_pairs_u32": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _sz, _vp],
         "flip2_find_grid_cell": [_vp, _vp, _vp, _u32, GridC, _vp, _vp],
+        "flip2_find_sub_cell": [_vp, _vp, _vp, _u32, GridC, _vp, _u32, _f64] + [_vp] * 9,
+        "flip2_sum_particles_per_node": [_u32, _u32] + [_vp] * 6,
+        "flip2_particle_list_create": [_u32, _u32] + [_vp] * 9,
         "flip2_philox4x32_10": [_vp, _u32, _vp],
         "flip2_particles_create": [_u32, ctypes.POINTER(_vp)],
         "flip2_particles_destroy": [_vp],
@@ -323,6 +326,52 @@ def find_grid_cell(px, py, pz, grid: Sequence) -> Tuple[np.ndarray, np.ndarray, 
     check(lib().flip2_find_grid_cell(d[0].ptr, d[1].ptr, d[2].ptr, n, g, dk.ptr, None))
     synchronize()
     return tuple(b.to_array(np.float64, n) for b in d) + (dk.to_array(np.uint32, n),)
+
+
+def find_sub_cell(px, py, pz, grid: Sequence, keys, refinement_level: int = 1, radius: float = 2.0):
+    """cudaFindSubCell on the GPU: ([inclusive scans of the per-particle
+    counts] * 3, [sub-cell lists] * 3) for the x, y, z components."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (px, py, pz)]
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = k.size
+    d = [DeviceBuffer.from_array(a) for a in arrs]
+    dk = DeviceBuffer.from_array(k)
+    dt = [DeviceBuffer(4 * n) for _ in range(3)]
+    totals = np.zeros(3, np.uint32)
+    g = GridC(*grid)
+    check(lib().flip2_find_sub_cell(*(b.ptr for b in d), n, g, dk.ptr, refinement_level, radius,
+                                    *(b.ptr for b in dt), None, None, None, None, _ptr(totals), None))
+    caps = totals.copy()
+    dl = [DeviceBuffer(4 * max(int(t), 1)) for t in totals]
+    check(lib().flip2_find_sub_cell(*(b.ptr for b in d), n, g, dk.ptr, refinement_level, radius,
+                                    *(b.ptr for b in dt), *(b.ptr for b in dl), _ptr(caps), _ptr(totals), None))
+    return ([b.to_array(np.uint32, n) for b in dt], [b.to_array(np.uint32, int(t)) for b, t in zip(dl, totals)])
+
+
+def sum_particles_per_node(first, touched_incl, sub_cells, n: int):
+    """cudaSumParticlesPerNodeAndWriteNumUsedVoxels on the GPU: inclusive scans
+    over the nodes of (distinct voxels, list entries)."""
+    f = np.ascontiguousarray(first, np.uint32)
+    B = f.size
+    df, dt = DeviceBuffer.from_array(f), DeviceBuffer.from_array(np.ascontiguousarray(touched_incl, np.uint32))
+    dl = DeviceBuffer.from_array(np.ascontiguousarray(sub_cells, np.uint32))
+    dnz, dne = DeviceBuffer(4 * B), DeviceBuffer(4 * B)
+    check(lib().flip2_sum_particles_per_node(B, n, df.ptr, dt.ptr, dl.ptr, dnz.ptr, dne.ptr, None))
+    return dnz.to_array(np.uint32, B), dne.to_array(np.uint32, B)
+
+
+def particle_list_create(first, touched_incl, sub_cells, n: int, nonzero_incl, entries_incl):
+    """cudaParticleListCreate on the GPU: (voxel ids, list starts, lists)."""
+    f = np.ascontiguousarray(first, np.uint32)
+    B = f.size
+    nz = np.ascontiguousarray(nonzero_incl, np.uint32)
+    ne = np.ascontiguousarray(entries_incl, np.uint32)
+    V, L = (int(nz[-1]), int(ne[-1])) if B else (0, 0)
+    bufs = [DeviceBuffer.from_array(a) for a in (nz, ne, np.ascontiguousarray(touched_incl, np.uint32),
+                                                 np.ascontiguousarray(sub_cells, np.uint32), f)]
+    dv, ds, dp = DeviceBuffer(4 * max(V, 1)), DeviceBuffer(4 * max(V, 1)), DeviceBuffer(4 * max(L, 1))
+    check(lib().flip2_particle_list_create(B, n, *(b.ptr for b in bufs), dv.ptr, ds.ptr, dp.ptr, None))
+    return dv.to_array(np.uint32, V), ds.to_array(np.uint32, V), dp.to_array(np.uint32, L)
 
 
 def philox4x32_10(ctr_key: np.ndarray) -> np.ndarray:

@@ -2,6 +2,7 @@
 // library's kernels.
 #include "algorithms.hpp"
 
+#include "lists.hpp"
 #include "rc_kernels.hpp"
 #include "scan_impl.hpp"
 #include "sort.hpp"
@@ -215,6 +216,61 @@ void removeGravity(const DeviceVec<char>& solids, DeviceVec<double>& voxelsUz, d
     FLIP2_REQUIRE(voxelsUz.size() >= solids.size(), kErrRange, "removeGravity: velocity array shorter than solids");
     if (solids.size()) rc::launch_gravity(solids.size(), solids.devPtr(), voxelsUz.devPtr(), (double)(-9.8f) * dt,
                                           true, stream);
+}
+
+void cudaFindSubCell(double* px, double* py, double* pz, uint32_t numParticles, Grid grid, uint32_t* gridPosition,
+                     uint32_t* subCellsTouchedX, uint32_t* subCellsTouchedY, uint32_t* subCellsTouchedZ,
+                     DeviceVec<uint32_t>& subCellPositionX, DeviceVec<uint32_t>& subCellPositionY,
+                     DeviceVec<uint32_t>& subCellPositionZ, uint32_t numRefinementLevels, double radius,
+                     hipStream_t stream) {
+    FLIP2_REQUIRE(numRefinementLevels < 16 && radius >= 0.0 && radius < 64.0, kErrArgument,
+                  "findSubCell: refinement level or radius out of range");
+    if (numParticles == 0) return;
+    rc::launch_sub_cell_counts(px, py, pz, numParticles, grid, gridPosition, numRefinementLevels, radius,
+                               subCellsTouchedX, subCellsTouchedY, subCellsTouchedZ, stream);
+    uint32_t* touched[3] = {subCellsTouchedX, subCellsTouchedY, subCellsTouchedZ};
+    uint32_t total[3];
+    for (int d = 0; d < 3; ++d) {
+        cudaParallelPrefixSum(numParticles, touched[d], stream);
+        FLIP2_HIP(hipMemcpyAsync(&total[d], touched[d] + numParticles - 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 stream));
+    }
+    FLIP2_HIP(hipStreamSynchronize(stream));
+    subCellPositionX.resizeAsync(total[0], stream);
+    subCellPositionY.resizeAsync(total[1], stream);
+    subCellPositionZ.resizeAsync(total[2], stream);
+    rc::launch_sub_cell_lists(px, py, pz, numParticles, grid, gridPosition, numRefinementLevels, radius,
+                              subCellsTouchedX, subCellsTouchedY, subCellsTouchedZ, subCellPositionX.devPtr(),
+                              subCellPositionY.devPtr(), subCellPositionZ.devPtr(), stream);
+}
+
+void cudaSumParticlesPerNodeAndWriteNumUsedVoxels(uint32_t numGridNodes, uint32_t numParticles,
+                                                  uint32_t* gridNodeIndicesToFirstParticleIndex,
+                                                  uint32_t* subCellsTouchedPerParticle, uint32_t* subCellsDim,
+                                                  uint32_t* numNonZeroVoxels, uint32_t* numParticlesInVoxelLists,
+                                                  uint32_t numVoxelsPerNode, hipStream_t stream) {
+    FLIP2_REQUIRE(numVoxelsPerNode == kRcVoxelsPerNode, kErrArgument, "per-voxel lists: 512 voxels per node only");
+    if (numGridNodes == 0) return;
+    rc::launch_particles_per_node(numGridNodes, numParticles, gridNodeIndicesToFirstParticleIndex,
+                                  subCellsTouchedPerParticle, subCellsDim, numNonZeroVoxels, numParticlesInVoxelLists,
+                                  stream);
+    cudaParallelPrefixSum(numGridNodes, numNonZeroVoxels, stream);
+    cudaParallelPrefixSum(numGridNodes, numParticlesInVoxelLists, stream);
+    FLIP2_HIP(hipStreamSynchronize(stream));
+}
+
+void cudaParticleListCreate(uint32_t totalNumberVoxelsDimension, uint32_t numGridNodes, uint32_t numParticles,
+                            uint32_t numVoxelsPerNode, uint32_t* voxelIDs, uint32_t* perVoxelParticleListStartIndices,
+                            uint32_t* numUsedVoxelsPerNode, uint32_t* firstParticleListIndexPerNode,
+                            uint32_t* subCellsTouchedPerParticle, uint32_t* subCellsDim,
+                            uint32_t* firstParticleInNodeIndex, uint32_t* particleLists, hipStream_t stream) {
+    (void)totalNumberVoxelsDimension;
+    FLIP2_REQUIRE(numVoxelsPerNode == kRcVoxelsPerNode, kErrArgument, "per-voxel lists: 512 voxels per node only");
+    if (numGridNodes == 0) return;
+    rc::launch_particle_lists(numGridNodes, numParticles, numUsedVoxelsPerNode, firstParticleListIndexPerNode,
+                              subCellsTouchedPerParticle, subCellsDim, firstParticleInNodeIndex, voxelIDs,
+                              perVoxelParticleListStartIndices, particleLists, stream);
+    FLIP2_HIP(hipStreamSynchronize(stream));
 }
 
 template <>

@@ -95,6 +95,40 @@ void cudaRadixSortUint(uint32_t numElements, uint32_t* inArray, uint32_t* outArr
                        uint32_t* front, uint32_t* back, hipStream_t frontStream, hipStream_t backStream,
                        uint32_t*& reorderedIndicesRelativeToOriginal);
 
+// particleToGridFunctions.hu:12 -- sub-cells of each particle's root cell
+// (with the apron) whose staggered x / y / z velocity point lies within
+// radius sub-cells: per-particle counts, inclusive-scanned into
+// subCellsTouched* (numParticles each), and the concatenated lists of
+// node-local voxel ids resized into subCellPosition*.  The reference never
+// calls it from its step.
+void cudaFindSubCell(double* px, double* py, double* pz, uint32_t numParticles, Grid grid, uint32_t* gridPosition,
+                     uint32_t* subCellsTouchedX, uint32_t* subCellsTouchedY, uint32_t* subCellsTouchedZ,
+                     DeviceVec<uint32_t>& subCellPositionX, DeviceVec<uint32_t>& subCellPositionY,
+                     DeviceVec<uint32_t>& subCellPositionZ, uint32_t numRefinementLevels, double radius,
+                     hipStream_t stream);
+
+// perVoxelParticleListFunctions.hu:8 -- per node, the list entries
+// (numParticlesInVoxelLists) and distinct voxels (numNonZeroVoxels) of its
+// particles' sub-cell lists, both inclusive-scanned over the nodes.
+// Entries outside [0, numVoxelsPerNode) are skipped (the reference indexes
+// past its shared array with them).
+void cudaSumParticlesPerNodeAndWriteNumUsedVoxels(uint32_t numGridNodes, uint32_t numParticles,
+                                                  uint32_t* gridNodeIndicesToFirstParticleIndex,
+                                                  uint32_t* subCellsTouchedPerParticle, uint32_t* subCellsDim,
+                                                  uint32_t* numNonZeroVoxels, uint32_t* numParticlesInVoxelLists,
+                                                  uint32_t numVoxelsPerNode, hipStream_t stream);
+
+// perVoxelParticleListFunctions.hu:13 -- per node, its used voxels in
+// ascending order (voxelIDs at numUsedVoxelsPerNode[b-1]), the start of
+// each one's particle list (perVoxelParticleListStartIndices) and the lists
+// (particleLists: the node's particles touching the voxel, ascending).
+// totalNumberVoxelsDimension is unused, as in the reference.
+void cudaParticleListCreate(uint32_t totalNumberVoxelsDimension, uint32_t numGridNodes, uint32_t numParticles,
+                            uint32_t numVoxelsPerNode, uint32_t* voxelIDs, uint32_t* perVoxelParticleListStartIndices,
+                            uint32_t* numUsedVoxelsPerNode, uint32_t* firstParticleListIndexPerNode,
+                            uint32_t* subCellsTouchedPerParticle, uint32_t* subCellsDim,
+                            uint32_t* firstParticleInNodeIndex, uint32_t* particleLists, hipStream_t stream);
+
 namespace rc {
 // The SOR loop of cudaGSiteration on raw arrays (shared with
 // Particles::gsSolve): B nodes, V voxels, scale = dt / (density h_sub^2).

@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "fluid.hpp"
+#include "lists.hpp"
 #include "unet.hpp"
 #include "ns_kernels.hpp"
 #include "p2g_pipeline.hpp"
@@ -349,6 +350,78 @@ int flip2_sort_pairs_u32(const uint32_t* d_keys_in, const uint32_t* d_vals_in, u
             flip2::sort_pairs_u32(d_keys_in, d_vals_in, d_keys_out, d_vals_out, d_keys_alt, d_vals_alt, n, begin_bit,
                                   end_bit, d_ws, ws_bytes, as_stream(stream));
         }
+    });
+}
+
+int flip2_find_sub_cell(const double* d_px, const double* d_py, const double* d_pz, uint32_t n, flip2_grid grid,
+                        const uint32_t* d_keys, uint32_t refinement_level, double radius, uint32_t* d_touched_x,
+                        uint32_t* d_touched_y, uint32_t* d_touched_z, uint32_t* d_list_x, uint32_t* d_list_y,
+                        uint32_t* d_list_z, const uint32_t* capacities, uint32_t* totals, void* stream) {
+    return guarded([&] {
+        NEED(totals);
+        totals[0] = totals[1] = totals[2] = 0;
+        if (n == 0) return;
+        NEED(d_px); NEED(d_py); NEED(d_pz); NEED(d_keys); NEED(d_touched_x); NEED(d_touched_y); NEED(d_touched_z);
+        FLIP2_REQUIRE(grid.sizeX && grid.sizeY && grid.sizeZ && grid.cellSize > 0.0f, flip2::kErrArgument,
+                      "invalid grid");
+        FLIP2_REQUIRE(refinement_level < 16 && radius >= 0.0 && radius < 64.0, flip2::kErrArgument,
+                      "refinement level or radius out of range");
+        const hipStream_t st = as_stream(stream);
+        flip2::rc::launch_sub_cell_counts(d_px, d_py, d_pz, n, to_grid(grid), d_keys, refinement_level, radius,
+                                          d_touched_x, d_touched_y, d_touched_z, st);
+        uint32_t* t[3] = {d_touched_x, d_touched_y, d_touched_z};
+        const size_t ws = flip2::scan_workspace_bytes(n);
+        void* w = nullptr;
+        FLIP2_HIP(hipMallocAsync(&w, ws, st));
+        for (int d = 0; d < 3; ++d) {
+            flip2::scan_u32(t[d], t[d], n, false, w, ws, nullptr, st);
+            FLIP2_HIP(hipMemcpyAsync(&totals[d], t[d] + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        }
+        FLIP2_HIP(hipFreeAsync(w, st));
+        FLIP2_HIP(hipStreamSynchronize(st));
+        if (!d_list_x && !d_list_y && !d_list_z) return;
+        NEED(d_list_x); NEED(d_list_y); NEED(d_list_z); NEED(capacities);
+        for (int d = 0; d < 3; ++d)
+            FLIP2_REQUIRE(capacities[d] >= totals[d], flip2::kErrRange, "sub-cell list capacity too small");
+        flip2::rc::launch_sub_cell_lists(d_px, d_py, d_pz, n, to_grid(grid), d_keys, refinement_level, radius,
+                                         d_touched_x, d_touched_y, d_touched_z, d_list_x, d_list_y, d_list_z, st);
+        FLIP2_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int flip2_sum_particles_per_node(uint32_t num_nodes, uint32_t n, const uint32_t* d_first, const uint32_t* d_touched,
+                                 const uint32_t* d_list, uint32_t* d_num_nonzero_voxels, uint32_t* d_num_entries,
+                                 void* stream) {
+    return guarded([&] {
+        if (num_nodes == 0) return;
+        NEED(d_first); NEED(d_touched); NEED(d_num_nonzero_voxels); NEED(d_num_entries);
+        FLIP2_REQUIRE(n > 0, flip2::kErrArgument, "nodes without particles");
+        const hipStream_t st = as_stream(stream);
+        flip2::rc::launch_particles_per_node(num_nodes, n, d_first, d_touched, d_list, d_num_nonzero_voxels,
+                                             d_num_entries, st);
+        const size_t ws = flip2::scan_workspace_bytes(num_nodes);
+        void* w = nullptr;
+        FLIP2_HIP(hipMallocAsync(&w, ws, st));
+        flip2::scan_u32(d_num_nonzero_voxels, d_num_nonzero_voxels, num_nodes, false, w, ws, nullptr, st);
+        flip2::scan_u32(d_num_entries, d_num_entries, num_nodes, false, w, ws, nullptr, st);
+        FLIP2_HIP(hipFreeAsync(w, st));
+        FLIP2_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int flip2_particle_list_create(uint32_t num_nodes, uint32_t n, const uint32_t* d_num_nonzero_voxels,
+                               const uint32_t* d_num_entries, const uint32_t* d_touched, const uint32_t* d_list,
+                               const uint32_t* d_first, uint32_t* d_voxel_ids, uint32_t* d_list_start,
+                               uint32_t* d_particle_lists, void* stream) {
+    return guarded([&] {
+        if (num_nodes == 0) return;
+        NEED(d_num_nonzero_voxels); NEED(d_num_entries); NEED(d_touched); NEED(d_first); NEED(d_voxel_ids);
+        NEED(d_list_start); NEED(d_particle_lists);
+        FLIP2_REQUIRE(n > 0, flip2::kErrArgument, "nodes without particles");
+        const hipStream_t st = as_stream(stream);
+        flip2::rc::launch_particle_lists(num_nodes, n, d_num_nonzero_voxels, d_num_entries, d_touched, d_list,
+                                         d_first, d_voxel_ids, d_list_start, d_particle_lists, st);
+        FLIP2_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -493,10 +493,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             cg.r = r_.devPtr();
             cg.z = z_.devPtr();
             cg.na = na;
-            ns::cg_groups(na, false, cg.groupsA, cg.slotsA);
-            ns::cg_groups(na, !jac, cg.groupsB, cg.slotsB);
-            cg.partA = partials_.devPtr();
-            cg.partB = partials_.devPtr() + cg.groupsA;
+            cg.partials = partials_.devPtr();
             cg.hist = H;
             float* pbuf[2] = {p_.devPtr(), p2_.devPtr()};
             ns::launch_cg_update_precond(cg, 0, true, !jac, r64_.devPtr(), nullptr, stream_);
@@ -512,7 +509,6 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                     ns::launch_cg_update_precond(cg, it, false, !jac, nullptr, pNew, stream_);
                     if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
                 }
-                ns::launch_cg_publish(cg, it, stream_);  // entry `it`: rr after the batch's last update
                 innerDone = converged(first, batch);
             }
             total += it;

@@ -622,255 +622,194 @@ mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon)
 }
 
 // ---------------------------------------------------------------------------
-// PCG in two kernels per iteration (Jacobi and brick-local MIC(0)).
+// PCG with two main kernels per iteration (Jacobi and brick-local MIC(0)).
 //
 //   direction+matvec (iteration k):  beta = rz_k / rz_{k-1};  p = z + beta p';
 //                                    q = A p;  partial p.q
-//   update+precondition:             alpha = rz_k / p.q;  e += alpha p;
+//   update+precondition:             alpha = rz_k / (p.q)_k;  e += alpha p;
 //                                    r -= alpha q;  z = M^-1 r;  partials r.z, r.r
-// Each kernel runs a bounded number of workgroups (each walks a contiguous
-// range of slots) and starts by summing the other kernel's partials itself,
-// in a fixed order that every workgroup repeats identically, so no separate
-// reduction launch is needed and the sums stay deterministic.  Workgroup 0
-// also records the sums in the iteration history (rz, rr, pq per iteration).
-// p alternates between two buffers (the kernel reads the previous direction
-// at neighbour cells owned by other workgroups while writing the new one).
+// One workgroup (direction+matvec, Jacobi update) or one wave (MIC(0)
+// update: the brick's forward and backward plane sweeps run in LDS) per
+// slot, so every slot's memory latency is in flight at once; each main
+// kernel is followed by a one-workgroup kernel that sums its partials in
+// slot order into the iteration history (rz, rr, pq per iteration:
+// deterministic), which the next kernel reads.  p alternates between two
+// buffers: the matvec reads the previous direction at neighbour cells owned
+// by other workgroups while writing the new one.
 // ---------------------------------------------------------------------------
-namespace {
 
-// FP64 sum over the workgroup, result in every thread.
-__device__ inline double block_sum_all(double v, double* s_tmp) {
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) s_tmp[wave] = v;
-    __syncthreads();
-    double r = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_tmp[w];
-    return r;
-}
-
-// Sum of count partials, identical in every workgroup (fixed order).
-__device__ inline double sum_partials(const double* __restrict__ part, uint32_t count, double* s_tmp) {
-    double a = 0.0;
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) a += part[i];
-    return block_sum_all(a, s_tmp);
-}
-
-// The slot range [beg, end) of workgroup w of `groups`: each XCD (blocks are
-// dealt to the 8 XCDs round-robin) takes a contiguous eighth, so neighbouring
-// slots share an L2.
-// groups is at most 8 or a multiple of 8 (so workgroup -> order is a
-// bijection onto [0, groups)); groups * slotsPer >= na.
-__device__ inline void slot_range(uint32_t na, uint32_t groups, uint32_t slotsPer, uint32_t& beg, uint32_t& end) {
-    const uint32_t w = blockIdx.x;
-    const uint32_t order = groups <= 8u ? w : (w & 7u) * (groups >> 3) + (w >> 3);
-    beg = min(na, order * slotsPer);
-    end = min(na, beg + slotsPer);
-}
-
-}  // namespace
-
-// p = z + beta p_old (p = z when first), q = A p; partial p.q per workgroup.
+// p = z + beta p_old (p = z when first), q = A p; partial p.q per slot.
 __global__ void __launch_bounds__(kWG)
 cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __restrict__ pOld,
                            float* __restrict__ pNew) {
     __shared__ double s_tmp[16];
     __shared__ float s_p[512];
-    const double rz = sum_partials(c.partB, c.groupsB, s_tmp);
-    const double rr = sum_partials(c.partB + c.groupsB, c.groupsB, s_tmp);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c.hist[4u * it] = rz;
-        c.hist[4u * it + 1u] = rr;
-    }
+    const uint32_t slot = blockIdx.x, base = slot * 512u;
     float beta = 0.f;
     if (!first) {
-        const double rzOld = c.hist[4u * (it - 1u)];
+        const double rz = c.hist[4u * it], rzOld = c.hist[4u * (it - 1u)];
         beta = rzOld != 0.0 ? (float)(rz / rzOld) : 0.f;
     }
-    uint32_t beg, end;
-    slot_range(c.na, c.groupsA, c.slotsA, beg, end);
+    for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
+        const uint32_t v = base + l;
+        s_p[l] = c.coef[v] ? (first ? c.z[v] : fmaf(beta, pOld[v], c.z[v])) : 0.f;
+    }
+    __syncthreads();
+    auto px = [&](uint32_t i) {
+        if ((i >> 9) == slot) return s_p[i & 511u];
+        return first ? c.z[i] : fmaf(beta, pOld[i], c.z[i]);  // a fluid neighbour in another slot
+    };
     double pq = 0.0;
-    for (uint32_t slot = beg; slot < end; ++slot) {
-        const uint32_t base = slot * 512u;
-        for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
-            const uint32_t v = base + l;
-            s_p[l] = c.coef[v] ? (first ? c.z[v] : fmaf(beta, pOld[v], c.z[v])) : 0.f;
-        }
-        __syncthreads();
-        auto px = [&](uint32_t i) {
-            if ((i >> 9) == slot) return s_p[i & 511u];
-            return first ? c.z[i] : fmaf(beta, pOld[i], c.z[i]);  // a fluid neighbour in another slot
-        };
-        for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
-            const uint32_t v = base + l;
-            const uint16_t cf = c.coef[v];
-            const float qv = apply_A_with(slot, l, cf, c.nbrSlot, px);
-            const float pv = s_p[l];
-            pNew[v] = pv;
-            c.q[v] = qv;
-            pq += (double)pv * (double)qv;
-        }
-        __syncthreads();  // s_p is rewritten for the next slot
+    for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
+        const uint32_t v = base + l;
+        const float qv = apply_A_with(slot, l, c.coef[v], c.nbrSlot, px);
+        const float pv = s_p[l];
+        pNew[v] = pv;
+        c.q[v] = qv;
+        pq += (double)pv * (double)qv;
     }
     const double t = block_sum(pq, s_tmp);
-    if (threadIdx.x == 0) c.partA[blockIdx.x] = t;
+    if (threadIdx.x == 0) c.partials[slot] = t;
 }
 
-// e += alpha p, r -= alpha q (skipped when init: r = (float) r64, e = 0),
-// z = M^-1 r (Jacobi, or brick MIC(0) with one wave per slot); partials
-// r.z and r.r per workgroup.
-template <bool kMic>
-__global__ void __launch_bounds__(kWG)
-cg_update_precond_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64,
-                         const float* __restrict__ p) {
-    __shared__ double s_tmp[16];
-    __shared__ float s_q[kMic ? 4 : 1][512];
-    __shared__ float s_pc[kMic ? 4 : 1][512];
-    __shared__ uint16_t s_cf[kMic ? 4 : 1][512];
-    float alpha = 0.f;
-    if (!init) {
-        const double pq = sum_partials(c.partA, c.groupsA, s_tmp);
-        if (blockIdx.x == 0 && threadIdx.x == 0) c.hist[4u * it + 2u] = pq;
-        const double rz = c.hist[4u * it];
-        alpha = pq != 0.0 ? (float)(rz / pq) : 0.f;
-    }
-    // e, r update of one cell; returns the new r
-    auto update = [&](uint32_t v, uint16_t cf) -> float {
-        if (!cf) {
-            if (init) {
-                c.r[v] = 0.f;
-                c.e[v] = 0.f;
-            }
-            return 0.f;
-        }
+namespace {
+// e, r update of one cell (init: r = (float) r64, e = 0); returns the new r.
+__device__ inline float cg_update_cell(const CgFused& c, uint32_t v, uint16_t cf, int init, float alpha,
+                                       const double* __restrict__ r64, const float* __restrict__ p) {
+    if (!cf) {
         if (init) {
-            const float rv = (float)r64[v];
-            c.r[v] = rv;
+            c.r[v] = 0.f;
             c.e[v] = 0.f;
-            return rv;
         }
-        c.e[v] = fmaf(alpha, p[v], c.e[v]);
-        const float rv = fmaf(-alpha, c.q[v], c.r[v]);
+        return 0.f;
+    }
+    if (init) {
+        const float rv = (float)r64[v];
         c.r[v] = rv;
+        c.e[v] = 0.f;
         return rv;
-    };
-    uint32_t beg, end;
-    slot_range(c.na, c.groupsB, c.slotsB, beg, end);
+    }
+    c.e[v] = fmaf(alpha, p[v], c.e[v]);
+    const float rv = fmaf(-alpha, c.q[v], c.r[v]);
+    c.r[v] = rv;
+    return rv;
+}
+
+__device__ inline float cg_alpha(const CgFused& c, uint32_t it, int init) {
+    if (init) return 0.f;
+    const double pq = c.hist[4u * it + 2u], rz = c.hist[4u * it];
+    return pq != 0.0 ? (float)(rz / pq) : 0.f;
+}
+}  // namespace
+
+// Jacobi: e += alpha p, r -= alpha q (init: r = (float) r64, e = 0),
+// z = r / diag; partials r.z and r.r per slot.
+__global__ void __launch_bounds__(kWG)
+cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64,
+                        const float* __restrict__ p) {
+    __shared__ double s_tmp[16];
+    const float alpha = cg_alpha(c, it, init);
+    const uint32_t slot = blockIdx.x;
     double rz = 0.0, rr = 0.0;
-    if (!kMic) {
-        for (uint32_t slot = beg; slot < end; ++slot) {
-            for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
-                const uint32_t v = slot * 512u + l;
-                const uint16_t cf = c.coef[v];
-                const float rv = update(v, cf);
-                const float zv = cf ? rv / (float)(cf & 7u) : 0.f;
-                c.z[v] = zv;
-                rr += (double)rv * (double)rv;
-                rz += (double)rv * (double)zv;
-            }
-        }
-    } else {
-        // one wave per slot; the four waves step through the planes together
-        const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-        float* sq = s_q[kMic ? wave : 0];
-        float* spc = s_pc[kMic ? wave : 0];
-        uint16_t* scf = s_cf[kMic ? wave : 0];
-        const uint32_t stride[3] = {1u, 8u, 64u};
-        for (uint32_t s0 = beg; s0 < end; s0 += 4u) {
-            const uint32_t slot = s0 + wave;
-            const bool mine = slot < end;
-            const uint32_t base = slot * 512u;
-            float rk[8];  // this lane's r values (cells lane + 64 k)
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t l = lane + 64u * (uint32_t)k;
-                uint16_t cf = 0;
-                float rv = 0.f, pc = 0.f;
-                if (mine) {
-                    cf = c.coef[base + l];
-                    pc = c.precon[base + l];
-                    rv = update(base + l, cf);
-                    rr += (double)rv * (double)rv;
-                }
-                rk[k] = rv;
-                scf[l] = cf;
-                spc[l] = pc;
-                sq[l] = rv;  // r, replaced by the forward sweep's q plane by plane
-            }
-            __syncthreads();
-            for (int s = 0; s <= 21; ++s) {
-                const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
-                for (int t = pb + (int)lane; t < pe; t += 64) {
-                    const uint32_t l = c_planes.cells[t];
-                    const uint16_t cf = scf[l];
-                    float qv = 0.f;
-                    if (cf) {
-                        float acc = sq[l];
-                        const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
-#pragma unroll
-                        for (int ax = 0; ax < 3; ++ax) {
-                            if (lc[ax] == 0u) continue;
-                            const uint32_t m = l - stride[ax];
-                            acc -= a_plus(scf[m], ax, m) * spc[m] * sq[m];
-                        }
-                        qv = acc * spc[l];
-                    }
-                    sq[l] = qv;
-                }
-                __syncthreads();
-            }
-            for (int s = 21; s >= 0; --s) {
-                const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
-                for (int t = pb + (int)lane; t < pe; t += 64) {
-                    const uint32_t l = c_planes.cells[t];
-                    const uint16_t cf = scf[l];
-                    float zz = 0.f;
-                    if (cf) {
-                        float acc = sq[l];
-                        const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
-#pragma unroll
-                        for (int ax = 0; ax < 3; ++ax) {
-                            if (lc[ax] == 7u) continue;
-                            const uint32_t m = l + stride[ax];
-                            acc -= a_plus(cf, ax, l) * spc[l] * sq[m];
-                        }
-                        zz = acc * spc[l];
-                    }
-                    sq[l] = zz;
-                }
-                __syncthreads();
-            }
-            if (mine) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const uint32_t l = lane + 64u * (uint32_t)k;
-                    const float zv = sq[l];
-                    c.z[base + l] = zv;
-                    rz += (double)rk[k] * (double)zv;
-                }
-            }
-            __syncthreads();  // the LDS planes are rewritten for the next slots
-        }
+    for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
+        const uint32_t v = slot * 512u + l;
+        const uint16_t cf = c.coef[v];
+        const float rv = cg_update_cell(c, v, cf, init, alpha, r64, p);
+        const float zv = cf ? rv / (float)(cf & 7u) : 0.f;
+        c.z[v] = zv;
+        rr += (double)rv * (double)rv;
+        rz += (double)rv * (double)zv;
     }
     const double t0 = block_sum(rz, s_tmp);
     const double t1 = block_sum(rr, s_tmp);
     if (threadIdx.x == 0) {
-        c.partB[blockIdx.x] = t0;
-        c.partB[c.groupsB + blockIdx.x] = t1;
+        c.partials[slot] = t0;
+        c.partials[c.na + slot] = t1;
     }
 }
 
-// History entry `it` (rz, rr) from the update kernel's partials (the next
-// direction kernel writes the same sums; this publishes them before it runs).
-__global__ void __launch_bounds__(kWG)
-cg_publish_kernel(CgFused c, uint32_t it) {
-    __shared__ double s_tmp[16];
-    const double rz = sum_partials(c.partB, c.groupsB, s_tmp);
-    const double rr = sum_partials(c.partB + c.groupsB, c.groupsB, s_tmp);
-    if (threadIdx.x == 0) {
-        c.hist[4u * it] = rz;
-        c.hist[4u * it + 1u] = rr;
+// MIC(0): the same update, then z = M^-1 r by the brick's forward and
+// backward sweeps over its 22 anti-diagonal planes, all in LDS; one wave
+// per slot.
+__global__ void __launch_bounds__(64)
+cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64, const float* __restrict__ p) {
+    __shared__ float sq[512];
+    __shared__ float spc[512];
+    __shared__ uint16_t scf[512];
+    const float alpha = cg_alpha(c, it, init);
+    const uint32_t lane = threadIdx.x, slot = blockIdx.x, base = slot * 512u;
+    const uint32_t stride[3] = {1u, 8u, 64u};
+    double rz = 0.0, rr = 0.0;
+    float rk[8];  // this lane's r values (cells lane + 64 k)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t l = lane + 64u * (uint32_t)k;
+        const uint16_t cf = c.coef[base + l];
+        const float pc = c.precon[base + l];
+        const float rv = cg_update_cell(c, base + l, cf, init, alpha, r64, p);
+        rr += (double)rv * (double)rv;
+        rk[k] = rv;
+        scf[l] = cf;
+        spc[l] = pc;
+        sq[l] = rv;  // r, replaced by the forward sweep's q plane by plane
+    }
+    __syncthreads();
+    for (int s = 0; s <= 21; ++s) {
+        const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
+        for (int t = pb + (int)lane; t < pe; t += 64) {
+            const uint32_t l = c_planes.cells[t];
+            const uint16_t cf = scf[l];
+            float qv = 0.f;
+            if (cf) {
+                float acc = sq[l];
+                const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    if (lc[ax] == 0u) continue;
+                    const uint32_t m = l - stride[ax];
+                    acc -= a_plus(scf[m], ax, m) * spc[m] * sq[m];
+                }
+                qv = acc * spc[l];
+            }
+            sq[l] = qv;
+        }
+        __syncthreads();
+    }
+    for (int s = 21; s >= 0; --s) {
+        const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
+        for (int t = pb + (int)lane; t < pe; t += 64) {
+            const uint32_t l = c_planes.cells[t];
+            const uint16_t cf = scf[l];
+            float zz = 0.f;
+            if (cf) {
+                float acc = sq[l];
+                const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    if (lc[ax] == 7u) continue;
+                    const uint32_t m = l + stride[ax];
+                    acc -= a_plus(cf, ax, l) * spc[l] * sq[m];
+                }
+                zz = acc * spc[l];
+            }
+            sq[l] = zz;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t l = lane + 64u * (uint32_t)k;
+        const float zv = sq[l];
+        c.z[base + l] = zv;
+        rz += (double)rk[k] * (double)zv;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        rz += __shfl_xor(rz, d, 64);
+        rr += __shfl_xor(rr, d, 64);
+    }
+    if (lane == 0) {
+        c.partials[slot] = rz;
+        c.partials[c.na + slot] = rr;
     }
 }
 
@@ -1260,30 +1199,20 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
     FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
     FLIP2_LAUNCH_CHECK();
 }
-void cg_groups(uint32_t numActive, bool micUpdate, uint32_t& groups, uint32_t& slotsPer) {
-    groups = slotsPer = 0;
-    if (!numActive) return;
-    const uint32_t per = micUpdate ? 4u : 1u;  // slots per workgroup step
-    slotsPer = per * ceil_div(numActive, per * kCgMaxGroups);
-    groups = ceil_div(numActive, slotsPer);
-    if (groups > 8u) groups = (groups + 7u) & ~7u;  // whole rows of the 8 XCDs
-}
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s) {
-    FLIP2_KERNEL(cg_direction_matvec_kernel, dim3(c.groupsA), dim3(kWG), 0, s, c, it, first ? 1 : 0, pOld, pNew);
+    FLIP2_KERNEL(cg_direction_matvec_kernel, dim3(c.na), dim3(kWG), 0, s, c, it, first ? 1 : 0, pOld, pNew);
     FLIP2_LAUNCH_CHECK();
+    reduce_to(Reduce{c.partials}, 1, c.na, c.hist + 4u * it + 2u, s);  // (p.q)_it
 }
 void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
                               hipStream_t s) {
     if (mic)
-        FLIP2_KERNEL(cg_update_precond_kernel<true>, dim3(c.groupsB), dim3(kWG), 0, s, c, it, init ? 1 : 0, r64, p);
+        FLIP2_KERNEL(cg_update_mic_kernel, dim3(c.na), dim3(64), 0, s, c, it, init ? 1 : 0, r64, p);
     else
-        FLIP2_KERNEL(cg_update_precond_kernel<false>, dim3(c.groupsB), dim3(kWG), 0, s, c, it, init ? 1 : 0, r64, p);
+        FLIP2_KERNEL(cg_update_jacobi_kernel, dim3(c.na), dim3(kWG), 0, s, c, it, init ? 1 : 0, r64, p);
     FLIP2_LAUNCH_CHECK();
-}
-void launch_cg_publish(const CgFused& c, uint32_t it, hipStream_t s) {
-    FLIP2_KERNEL(cg_publish_kernel, dim3(1), dim3(kWG), 0, s, c, it);
-    FLIP2_LAUNCH_CHECK();
+    reduce_to(Reduce{c.partials}, 2, c.na, c.hist + 4u * (init ? it : it + 1u), s);  // rz, rr of the next entry
 }
 void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,

@@ -55,10 +55,10 @@ void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& 
 // p = z - sum_i (zq[i] / pq_i) p_i (p must not be one of the p_i); pr = p . r.
 void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirections& d, const double* zq,
                            const float* r, float* p, Reduce red, double* pr, hipStream_t s);
-// PCG (Jacobi or brick MIC(0)) in two kernels per iteration, each summing
-// the other's workgroup partials itself (ns_solver.hip).  History entry k
-// (4 doubles at hist + 4k): rz_k, rr_k, (p.q)_k.
-constexpr uint32_t kCgMaxGroups = 1024;
+// PCG (Jacobi or brick MIC(0)) with two main kernels per iteration, each
+// followed by a one-workgroup fixed-order sum of its per-slot partials into
+// the iteration history (ns_solver.hip).  History entry k (4 doubles at
+// hist + 4k): rz_k, rr_k, (p.q)_k.
 struct CgFused {
     const uint16_t* coef;
     const int32_t* nbrSlot;
@@ -67,26 +67,20 @@ struct CgFused {
     float* e;
     float* r;
     float* z;
-    double* partA;  // groupsA partials (p.q) of the direction kernel
-    double* partB;  // 2 groupsB partials (r.z, then r.r) of the update kernel
+    double* partials;  // 2 na doubles
     double* hist;
-    uint32_t na, groupsA, groupsB, slotsA, slotsB;
+    uint32_t na;
 };
-// Workgroups of a kernel for numActive slots and the slots each walks
-// (micUpdate: the update kernel's MIC(0) form, four slots per workgroup
-// step); groups is at most 8 or a multiple of 8, at most kCgMaxGroups.
-void cg_groups(uint32_t numActive, bool micUpdate, uint32_t& groups, uint32_t& slotsPer);
 // Iteration `it`: p = z (first) or z + (rz_it / rz_{it-1}) pOld into pNew;
-// q = A pNew; history rz_it, rr_it.
+// q = A pNew; history (p.q)_it.
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s);
-// Iteration `it`: alpha = rz_it / (p.q); e += alpha p; r -= alpha q; z = M^-1 r;
-// history (p.q)_it.  init: instead r = (float) r64 (0 off the fluid), e = 0
-// (p unused), the start of a refinement.
+// Iteration `it`: alpha = rz_it / (p.q)_it; e += alpha p; r -= alpha q;
+// z = M^-1 r; history rz, rr of entry it + 1.  init: instead r = (float) r64
+// (0 off the fluid), e = 0 (p unused), history entry `it` -- the start of a
+// refinement.
 void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
                               hipStream_t s);
-// History entry `it` (rz, rr) from the last update kernel's partials.
-void launch_cg_publish(const CgFused& c, uint32_t it, hipStream_t s);
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s);
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s);

@@ -116,6 +116,36 @@ int flip2_sort_pairs_u32_noalloc(const uint32_t* d_keys_in, const uint32_t* d_va
 int flip2_find_grid_cell(double* d_px, double* d_py, double* d_pz, uint32_t n, flip2_grid grid, uint32_t* d_keys,
                          void* stream);
 
+/* Sub-cell lists (cudaFindSubCell, particleToGridFunctions.cu:69-262,
+ * particleToGridFunctions.hu:12): for each particle (sorted keys and clamped
+ * positions of the reference-compatible step), the sub-cells of its root
+ * cell plus apron whose staggered x / y / z velocity point lies within
+ * `radius` sub-cells.  d_touched_x/y/z (n each) receive the inclusive scans
+ * of the per-particle counts and totals[3] (host) their last elements; when
+ * d_list_x/y/z are non-NULL (capacities >= totals) the concatenated lists
+ * of node-local voxel ids are written too.  The reference builds these but
+ * never calls them from its step. */
+int flip2_find_sub_cell(const double* d_px, const double* d_py, const double* d_pz, uint32_t n, flip2_grid grid,
+                        const uint32_t* d_keys, uint32_t refinement_level, double radius, uint32_t* d_touched_x,
+                        uint32_t* d_touched_y, uint32_t* d_touched_z, uint32_t* d_list_x, uint32_t* d_list_y,
+                        uint32_t* d_list_z, const uint32_t* capacities, uint32_t* totals, void* stream);
+/* Per-voxel particle lists of one component's sub-cell list
+ * (perVoxelParticleListFunctions.cu:6-189, .hu:8-16).  Per node (b <
+ * num_nodes, particles [d_first[b], d_first[b+1])): d_num_nonzero_voxels and
+ * d_num_entries receive the inclusive scans over the nodes of its distinct
+ * voxels and list entries (cudaSumParticlesPerNodeAndWriteNumUsedVoxels);
+ * flip2_particle_list_create then writes the used voxels (ascending, at the
+ * node's offset), each one's list start and the lists (the node's particles
+ * touching the voxel, ascending) (cudaParticleListCreate).  512 voxels per
+ * node; list entries outside [0, 512) are skipped. */
+int flip2_sum_particles_per_node(uint32_t num_nodes, uint32_t n, const uint32_t* d_first, const uint32_t* d_touched,
+                                 const uint32_t* d_list, uint32_t* d_num_nonzero_voxels, uint32_t* d_num_entries,
+                                 void* stream);
+int flip2_particle_list_create(uint32_t num_nodes, uint32_t n, const uint32_t* d_num_nonzero_voxels,
+                               const uint32_t* d_num_entries, const uint32_t* d_touched, const uint32_t* d_list,
+                               const uint32_t* d_first, uint32_t* d_voxel_ids, uint32_t* d_list_start,
+                               uint32_t* d_particle_lists, void* stream);
+
 /* Philox4x32-10 on the device: in = count x (c0,c1,c2,c3,k0,k1), out = count x 4
  * (host arrays).  Known-answer check of the generator used by moveSolids. */
 int flip2_philox4x32_10(const uint32_t* in, uint32_t count, uint32_t* out);

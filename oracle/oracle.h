@@ -56,6 +56,14 @@
  *   orc_ns_advect_rk2          N: uniform field, midpoint closed form, clamp, solids
  *   orc_ns_transfer_step, orc_ns_step
  *                              N: equal to the stages run one by one
+ *   orc_find_sub_cell          L: exact lattice geometry (integer distances),
+ *                                 closed-form count at a sub-cell centre
+ *   orc_sum_particles_per_node, orc_particle_list_create
+ *                              L: numpy bincount / unique / lexsort of the
+ *                                 (node, voxel, particle) incidences
+ *   (L = tests/test_oracle_lists.py.)  Every entry point also runs under
+ *   AddressSanitizer and UndefinedBehaviorSanitizer (sanitize_check.c,
+ *   tests/test_oracle_sanitized.py).
  *
  * Build: oracle/Makefile -> oracle/liboracle.so (gcc -O2 -ffp-contract=off).
  * FMA contraction is disabled so every product/sum is an individually rounded

@@ -8,11 +8,13 @@ moving +-2 m/s, 8 particles per cell, seed 4, mixed-precision MIC(0)-PCG):
     particle velocities and positions; integer counts and coefficients equal;
     FP64 true residual <= 1e-6; pressure against a 1e-12 solve of the same
     system, the error the 1e-6 residual permits, stated per field);
-  - at 512^3 (30.36M particles) eight whole steps (contact at step 6)
+  - at 512^3 (30.36M particles; a particle crosses 4.3 cells per step, so
+    the fronts advance more slowly than the 64^3 run's) eight whole steps
     converge to the FP64 1e-6 residual, the projection of the ninth leaves
     no more divergence than the exact FP64 projection with the device's
     pressure stored in FP32 faces leaves (x1.25; oracle operator on the
-    device's faces), and the particles stay finite and in the box.
+    device's faces), the particles stay finite and in the box and the
+    spheres move towards each other.
 Dam break with a sphere obstacle at 128^3 with the committed network: the
 learned flexible CG (Jacobi-PCG fallback on) reaches the 1e-6 FP64 residual
 in fewer iterations than Jacobi-PCG, and its projection leaves no more
@@ -87,12 +89,17 @@ def test_colliding_spheres_512_steps(flip2, oracle):
     sim = flip2.FluidSim(x.size, grid, preconditioner="mic0")
     sim.set_particles(x, y, z, *vel)
     del x, y, z, vel
+    left0 = None
     for s in range(8):
         sim.step()
         st = sim.last_solve()
-        print(f"step {s + 1}: {st}")
+        print(f"step {s + 1}: {st}", flush=True)
         assert st["converged"] and st["rel_residual"] <= 1e-6, st
-    assert st["iterations"] > 0 and st["b_norm"] > 1.0, st
+        if s == 0:
+            px0 = sim.particles()[0]
+            left0 = float(np.mean(px0[px0 < 0.5]))
+            del px0
+    assert st["iterations"] > 0, st
     for s in ("transfer", "prepare", "system"):
         sim.stage(s)
     st, div, floor = solve_and_project(sim, grid, oracle)
@@ -110,6 +117,7 @@ def test_colliding_spheres_512_steps(flip2, oracle):
     # the spheres move towards each other
     left = px < 0.5
     assert float(np.mean(vx[left])) > 0.5 and float(np.mean(vx[~left])) < -0.5
+    assert float(np.mean(px[left])) > left0 + 0.01
 
 
 def test_dam_break_sphere_128_learned(flip2, oracle):
