@@ -824,6 +824,7 @@ int flip2_fluid_stage(flip2_fluid* f, const char* stage) {
         else if (s == "project") f->impl.project();
         else if (s == "g2p") f->impl.gridToParticles();
         else if (s == "advect") f->impl.advect();
+        else if (s == "g2p_advect") f->impl.gridToParticlesAndAdvect();
         else throw flip2::Error(flip2::kErrArgument, "unknown stage " + s);
         f->impl.synchronize();
     });

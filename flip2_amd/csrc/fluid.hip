@@ -602,14 +602,19 @@ void FluidSim::advect() {
                           box_, stream_);
 }
 
+void FluidSim::gridToParticlesAndAdvect() {
+    ns::launch_g2p_advect(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(),
+                          W0_.devPtr(), hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.flip, cfg_.dt,
+                          cfg_.margin * cfg_.grid.cellSize, box_, stream_);
+}
+
 void FluidSim::step() {
     transfer();
     prepareFaces();
     buildSystem();
     solve();
     project();
-    gridToParticles();
-    advect();
+    gridToParticlesAndAdvect();
 }
 
 uint32_t FluidSim::stepFrame(double frameDt, double cfl, uint32_t maxSubsteps) {
@@ -635,8 +640,7 @@ uint32_t FluidSim::stepFrame(double frameDt, double cfl, uint32_t maxSubsteps) {
             buildSystem();
             solve();
             project();
-            gridToParticles();
-            advect();
+            gridToParticlesAndAdvect();
             t += dt;
             ++k;
         }

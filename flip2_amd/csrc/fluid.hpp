@@ -113,6 +113,7 @@ public:
     void project();
     void gridToParticles();
     void advect();
+    void gridToParticlesAndAdvect();  // gridToParticles then advect, one kernel
 
     // Solve A x = b for the current system with a caller-given preconditioner
     // choice (used for preconditioner comparisons on one system).

@@ -61,6 +61,27 @@ __device__ inline void store_partials(const double* mine, int count, double* par
         for (int q = 0; q < count; ++q) partials[(size_t)q * gridDim.x + blockIdx.x] = mine[q];
 }
 
+// Orders the LDS accesses of the lanes of one wave (a one-wave workgroup's
+// barrier): the wave's LDS operations execute in issue order, so the fences
+// only keep the compiler from moving accesses across this point, and no
+// instruction waits for a store to complete.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Slot of this workgroup in an XCD-aware order, for kernels that read
+// neighbouring slots: the grid has 8 ceil(na / 8) workgroups, dealt to the
+// 8 XCDs round robin, and XCD x takes the x-th contiguous eighth of the
+// slots, so a slot's neighbours (nearby slot numbers: slots follow the
+// brick order) are loaded into the same L2.  Padding workgroups get na.
+__device__ inline uint32_t xcd_slot(uint32_t na) {
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t s = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    return s < na ? s : na;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -298,16 +319,30 @@ __device__ inline double apply_A64(uint32_t slot, uint32_t l, uint16_t cf, const
 
 }  // namespace
 
-// r32 = (float) r64; e = 0; rr into scal[out].
-// out[q] = sum over workgroups of partials[q * blocks + b], fixed order:
-// workgroup q sums quantity q, thread t the partials t, t + 256, ...
-__global__ void __launch_bounds__(kWG)
+// out[q] = sum over workgroups of partials[q * blocks + b], in a fixed order:
+// workgroup q sums quantity q; thread t adds the partials t + T j (T threads)
+// into eight accumulators (j mod 8), whose loads are all issued before the
+// adds.
+constexpr int kReduceThreads = 1024;
+__global__ void __launch_bounds__(kReduceThreads)
 reduce_partials_kernel(const double* __restrict__ partials, uint32_t blocks, double* __restrict__ out) {
-    __shared__ double s_tmp[16];
+    __shared__ double s_tmp[kReduceThreads / 64];
     const double* p = partials + (size_t)blockIdx.x * blocks;
-    double acc = 0.0;
-    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) acc += p[b];
-    const double t = block_sum(acc, s_tmp);
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+    for (uint32_t b0 = threadIdx.x; b0 < blocks; b0 += 8u * blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t b = b0 + (uint32_t)k * blockDim.x;
+            v[k] = b < blocks ? p[b] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += v[k];
+    }
+    const double a = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    const double t = block_sum(a, s_tmp);
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
@@ -330,9 +365,10 @@ refine_start_kernel(const double* __restrict__ r64, float* __restrict__ r, float
 // q = A p; pq into out[0].
 __global__ void __launch_bounds__(kWG)
 matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
-                  float* __restrict__ q, double* __restrict__ partials) {
+                  float* __restrict__ q, double* __restrict__ partials, uint32_t na) {
     __shared__ double s_tmp[16];
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = xcd_slot(na);
+    if (slot >= na) return;
     double pq = 0.0;
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
         const uint32_t v = slot * 512u + l;
@@ -341,7 +377,7 @@ matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef
         pq += (double)p[v] * (double)qv;
     }
     const double t = block_sum(pq, s_tmp);
-    store_partials(&t, 1, partials);
+    if (threadIdx.x == 0) partials[slot] = t;
 }
 
 // alpha = rz / pq; e += alpha p; r -= alpha q; with Jacobi also z = r/diag
@@ -516,9 +552,10 @@ accumulate_kernel(const float* __restrict__ e, double* __restrict__ x64, const u
 __global__ void __launch_bounds__(kWG)
 true_residual_kernel(const double* __restrict__ b64, const double* __restrict__ x64,
                      const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
-                     double* __restrict__ r64, double* __restrict__ partials) {
+                     double* __restrict__ r64, double* __restrict__ partials, uint32_t na) {
     __shared__ double s_tmp[16];
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = xcd_slot(na);
+    if (slot >= na) return;
     double rr = 0.0;
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
         const uint32_t v = slot * 512u + l;
@@ -529,7 +566,7 @@ true_residual_kernel(const double* __restrict__ b64, const double* __restrict__ 
         rr += rv * rv;
     }
     const double t = block_sum(rr, s_tmp);
-    store_partials(&t, 1, partials);
+    if (threadIdx.x == 0) partials[slot] = t;
 }
 
 __global__ void fill_f32_kernel(float* __restrict__ a, uint32_t count, float v) {
@@ -548,6 +585,10 @@ __global__ void fill_f64_kernel(double* __restrict__ a, uint32_t count, double v
 struct PlaneTable {
     uint16_t cells[512];
     uint16_t start[23];
+    // per plane and lane: the lane's cell c = cells[start[s] + lane] as c + 1
+    // (its entry in the MIC sweeps' padded LDS array, less 64), or 0 for a
+    // lane past the plane's end (the pad entry 64)
+    uint16_t lane_cell[22][64];
 };
 
 constexpr PlaneTable make_planes() {
@@ -562,6 +603,9 @@ constexpr PlaneTable make_planes() {
             }
     }
     t.start[22] = (uint16_t)n;
+    for (int s = 0; s <= 21; ++s)
+        for (int l = 0; l < 64; ++l)
+            t.lane_cell[s][l] = t.start[s] + l < t.start[s + 1] ? (uint16_t)(t.cells[t.start[s] + l] + 1) : (uint16_t)0;
     return t;
 }
 
@@ -628,9 +672,8 @@ mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon)
 //                                    q = A p;  partial p.q
 //   update+precondition:             alpha = rz_k / (p.q)_k;  e += alpha p;
 //                                    r -= alpha q;  z = M^-1 r;  partials r.z, r.r
-// One workgroup (direction+matvec, Jacobi update) or one wave (MIC(0)
-// update: the brick's forward and backward plane sweeps run in LDS) per
-// slot, so every slot's memory latency is in flight at once; each main
+// One wave per slot (the MIC(0) update's forward and backward plane sweeps
+// run in LDS), so every slot's memory latency is in flight at once; each main
 // kernel is followed by a one-workgroup kernel that sums its partials in
 // slot order into the iteration history (rz, rr, pq per iteration:
 // deterministic), which the next kernel reads.  p alternates between two
@@ -639,60 +682,135 @@ mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon)
 // ---------------------------------------------------------------------------
 
 // p = z + beta p_old (p = z when first), q = A p; partial p.q per slot.
-__global__ void __launch_bounds__(kWG)
+// One wave per slot, lane = column (i, j) of the brick holding the eight
+// cells k = 0..7 (cell l = 64 k + lane, so every load is one coalesced row
+// per k).  All inputs -- the column's coefficients, z and p_old, and the same
+// for the cells of the neighbouring slots it touches -- are loaded up front;
+// the new p goes to LDS for the x and y neighbours, z neighbours are the
+// lane's own registers.
+__global__ void __launch_bounds__(64)
 cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __restrict__ pOld,
                            float* __restrict__ pNew) {
-    __shared__ double s_tmp[16];
     __shared__ float s_p[512];
-    const uint32_t slot = blockIdx.x, base = slot * 512u;
+    const uint32_t lane = threadIdx.x, slot = xcd_slot(c.na), base = slot * 512u;
+    if (slot >= c.na) return;
+    const uint32_t i = lane & 7u, j = lane >> 3;
     float beta = 0.f;
     if (!first) {
         const double rz = c.hist[4u * it], rzOld = c.hist[4u * (it - 1u)];
         beta = rzOld != 0.0 ? (float)(rz / rzOld) : 0.f;
     }
-    for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
-        const uint32_t v = base + l;
-        s_p[l] = c.coef[v] ? (first ? c.z[v] : fmaf(beta, pOld[v], c.z[v])) : 0.f;
+    int32_t nb[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) nb[d] = c.nbrSlot[slot * 6u + d];
+    // neighbour slot and its cell index for the column's x and y sides
+    const int32_t xs = i == 0u ? nb[0] : (i == 7u ? nb[1] : -1);
+    const int32_t ys = j == 0u ? nb[2] : (j == 7u ? nb[3] : -1);
+    const uint32_t xl = (i == 0u ? 7u : 0u) + 8u * j, yl = i + (j == 0u ? 56u : 0u);
+    uint16_t cf[8];
+    float zv[8], po[8], xz[8], xp[8], yz[8], yp[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t v = base + 64u * (uint32_t)k + lane;
+        cf[k] = c.coef[v];
+        zv[k] = c.z[v];
+        po[k] = first ? 0.f : pOld[v];
+        xz[k] = xp[k] = yz[k] = yp[k] = 0.f;
+        if (xs >= 0) {
+            const uint32_t w = (uint32_t)xs * 512u + 64u * (uint32_t)k + xl;
+            xz[k] = c.z[w];
+            xp[k] = first ? 0.f : pOld[w];
+        }
+        if (ys >= 0) {
+            const uint32_t w = (uint32_t)ys * 512u + 64u * (uint32_t)k + yl;
+            yz[k] = c.z[w];
+            yp[k] = first ? 0.f : pOld[w];
+        }
+    }
+    float zlo = 0.f, plo = 0.f, zhi = 0.f, phi = 0.f;  // the z-side neighbours of cells k = 0 and 7
+    if (nb[4] >= 0) {
+        const uint32_t w = (uint32_t)nb[4] * 512u + 448u + lane;
+        zlo = c.z[w];
+        plo = first ? 0.f : pOld[w];
+    }
+    if (nb[5] >= 0) {
+        const uint32_t w = (uint32_t)nb[5] * 512u + lane;
+        zhi = c.z[w];
+        phi = first ? 0.f : pOld[w];
+    }
+    auto dir = [&](float zz, float pp) { return first ? zz : fmaf(beta, pp, zz); };
+    float pv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        pv[k] = cf[k] ? dir(zv[k], po[k]) : 0.f;
+        s_p[64u * (uint32_t)k + lane] = pv[k];
     }
     __syncthreads();
-    auto px = [&](uint32_t i) {
-        if ((i >> 9) == slot) return s_p[i & 511u];
-        return first ? c.z[i] : fmaf(beta, pOld[i], c.z[i]);  // a fluid neighbour in another slot
-    };
     double pq = 0.0;
-    for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
-        const uint32_t v = base + l;
-        const float qv = apply_A_with(slot, l, c.coef[v], c.nbrSlot, px);
-        const float pv = s_p[l];
-        pNew[v] = pv;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t l = 64u * (uint32_t)k + lane, v = base + l;
+        float qv = 0.f;
+        if (cf[k]) {
+            const uint32_t mask = cf[k] >> 3;
+            float sum = (float)(cf[k] & 7u) * pv[k];
+            // neighbours in the order of the mask bits: -x, +x, -y, +y, -z, +z
+            if (mask & 1u) sum -= i > 0u ? s_p[l - 1u] : dir(xz[k], xp[k]);
+            if (mask & 2u) sum -= i < 7u ? s_p[l + 1u] : dir(xz[k], xp[k]);
+            if (mask & 4u) sum -= j > 0u ? s_p[l - 8u] : dir(yz[k], yp[k]);
+            if (mask & 8u) sum -= j < 7u ? s_p[l + 8u] : dir(yz[k], yp[k]);
+            if (mask & 16u) sum -= k > 0 ? pv[k > 0 ? k - 1 : 0] : dir(zlo, plo);
+            if (mask & 32u) sum -= k < 7 ? pv[k < 7 ? k + 1 : 7] : dir(zhi, phi);
+            qv = sum;
+        }
+        pNew[v] = pv[k];
         c.q[v] = qv;
-        pq += (double)pv * (double)qv;
+        pq += (double)pv[k] * (double)qv;
     }
-    const double t = block_sum(pq, s_tmp);
-    if (threadIdx.x == 0) c.partials[slot] = t;
+    for (int d = 32; d >= 1; d >>= 1) pq += __shfl_xor(pq, d, 64);
+    if (lane == 0) c.partials[slot] = pq;
 }
 
 namespace {
-// e, r update of one cell (init: r = (float) r64, e = 0); returns the new r.
-__device__ inline float cg_update_cell(const CgFused& c, uint32_t v, uint16_t cf, int init, float alpha,
-                                       const double* __restrict__ r64, const float* __restrict__ p) {
-    if (!cf) {
+// e, r update of the eight cells base + lane + 64 k of a slot: e += alpha p,
+// r -= alpha q (init: r = (float) r64, e = 0); cells outside the fluid keep
+// e = r = 0.  Every load is issued before the first use (no load waits on a
+// branch), and e, r are stored for every cell.  cf, rk: the cells'
+// coefficients and new r.
+__device__ inline void cg_update_cells(const CgFused& c, uint32_t base, uint32_t lane, int init, float alpha,
+                                       const double* __restrict__ r64, const float* __restrict__ p, uint16_t cf[8],
+                                       float rk[8]) {
+    float pv[8], ev[8], qv[8], rv[8];
+    double r0[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t v = base + 64u * (uint32_t)k + lane;
+        cf[k] = c.coef[v];
         if (init) {
-            c.r[v] = 0.f;
-            c.e[v] = 0.f;
+            r0[k] = r64[v];
+        } else {
+            pv[k] = p[v];
+            ev[k] = c.e[v];
+            qv[k] = c.q[v];
+            rv[k] = c.r[v];
         }
-        return 0.f;
     }
-    if (init) {
-        const float rv = (float)r64[v];
-        c.r[v] = rv;
-        c.e[v] = 0.f;
-        return rv;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t v = base + 64u * (uint32_t)k + lane;
+        float rn, en;
+        if (init) {
+            rn = (float)r0[k];
+            en = 0.f;
+        } else {
+            en = fmaf(alpha, pv[k], ev[k]);
+            rn = fmaf(-alpha, qv[k], rv[k]);
+        }
+        if (!cf[k]) rn = en = 0.f;
+        c.e[v] = en;
+        c.r[v] = rn;
+        rk[k] = rn;
     }
-    c.e[v] = fmaf(alpha, p[v], c.e[v]);
-    const float rv = fmaf(-alpha, c.q[v], c.r[v]);
-    c.r[v] = rv;
-    return rv;
 }
 
 __device__ inline float cg_alpha(const CgFused& c, uint32_t it, int init) {
@@ -703,103 +821,125 @@ __device__ inline float cg_alpha(const CgFused& c, uint32_t it, int init) {
 }  // namespace
 
 // Jacobi: e += alpha p, r -= alpha q (init: r = (float) r64, e = 0),
-// z = r / diag; partials r.z and r.r per slot.
-__global__ void __launch_bounds__(kWG)
+// z = r / diag; partials r.z and r.r per slot.  One wave per slot, each lane
+// the eight cells lane + 64 k.
+__global__ void __launch_bounds__(64)
 cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64,
                         const float* __restrict__ p) {
-    __shared__ double s_tmp[16];
     const float alpha = cg_alpha(c, it, init);
-    const uint32_t slot = blockIdx.x;
+    const uint32_t lane = threadIdx.x, slot = xcd_slot(c.na);
+    if (slot >= c.na) return;
     double rz = 0.0, rr = 0.0;
-    for (uint32_t l = threadIdx.x; l < 512u; l += kWG) {
-        const uint32_t v = slot * 512u + l;
-        const uint16_t cf = c.coef[v];
-        const float rv = cg_update_cell(c, v, cf, init, alpha, r64, p);
-        const float zv = cf ? rv / (float)(cf & 7u) : 0.f;
+    uint16_t cf[8];
+    float rk[8];
+    cg_update_cells(c, slot * 512u, lane, init, alpha, r64, p, cf, rk);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t v = slot * 512u + 64u * (uint32_t)k + lane;
+        const float rv = rk[k];
+        const float zv = cf[k] ? rv / (float)(cf[k] & 7u) : 0.f;
         c.z[v] = zv;
         rr += (double)rv * (double)rv;
         rz += (double)rv * (double)zv;
     }
-    const double t0 = block_sum(rz, s_tmp);
-    const double t1 = block_sum(rr, s_tmp);
-    if (threadIdx.x == 0) {
-        c.partials[slot] = t0;
-        c.partials[c.na + slot] = t1;
+    for (int d = 32; d >= 1; d >>= 1) {
+        rz += __shfl_xor(rz, d, 64);
+        rr += __shfl_xor(rr, d, 64);
     }
+    if (lane == 0) {
+        c.partials[slot] = rz;
+        c.partials[c.na + slot] = rr;
+    }
+}
+
+// v where bit `bit` of w is set, else +0 (bit for bit: an AND with 0 or ~0)
+__device__ inline float keep_if_bit(float v, uint32_t w, int bit) {
+    const int m = (int)(w << (31 - bit)) >> 31;
+    return __int_as_float(__float_as_int(v) & m);
 }
 
 // MIC(0): the same update, then z = M^-1 r by the brick's forward and
 // backward sweeps over its 22 anti-diagonal planes, all in LDS; one wave
-// per slot.
+// per slot.  LDS holds (value, precondition factor) and the coefficient
+// word of cell c at entry 65 + c, with 65 zero entries below and 64 above,
+// so a cell's six neighbours are at fixed offsets (-1, -8, -64, +1, +8, +64)
+// and every read of a plane is issued at once, without a branch: a lane
+// with no cell on the plane works on the zero entry 64.  The coefficient
+// words keep only couplings inside the brick, so a neighbour offset that
+// wraps to another row or plane meets a cleared bit.  The coupling
+// A(c, m) = -1 of fluid neighbours is the matching bit of c's own word; the
+// terms (A pc_m) q_m and (A pc_c) z_m are formed as -(pc q) (negation is
+// exact), i.e. added where the bit is set.  The wave's LDS operations
+// execute in order, so a plane's reads see the previous plane's writes
+// without waiting for them (wave_lds_sync).
 __global__ void __launch_bounds__(64)
 cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64, const float* __restrict__ p) {
-    __shared__ float sq[512];
-    __shared__ float spc[512];
-    __shared__ uint16_t scf[512];
+    constexpr uint32_t kPad = 65, kEntries = 65 + 512 + 64;
+    __shared__ float2 s_a[kEntries];
+    __shared__ uint16_t s_c[kEntries];
     const float alpha = cg_alpha(c, it, init);
-    const uint32_t lane = threadIdx.x, slot = blockIdx.x, base = slot * 512u;
-    const uint32_t stride[3] = {1u, 8u, 64u};
+    const uint32_t lane = threadIdx.x, slot = xcd_slot(c.na), base = slot * 512u;
+    if (slot >= c.na) return;
     double rz = 0.0, rr = 0.0;
+    float pc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pc[k] = c.precon[base + lane + 64u * (uint32_t)k];
+    uint16_t cf[8];
     float rk[8];  // this lane's r values (cells lane + 64 k)
+    cg_update_cells(c, base, lane, init, alpha, r64, p, cf, rk);
+    for (uint32_t e = lane; e < kPad; e += 64u) {
+        s_a[e] = make_float2(0.f, 0.f);
+        s_c[e] = 0;
+    }
+    s_a[kPad + 512u + lane] = make_float2(0.f, 0.f);
+    s_c[kPad + 512u + lane] = 0;
+    const uint32_t i = lane & 7u, j = lane >> 3;
+    // in-brick coupling bits of the mask (-x, +x, -y, +y, -z, +z at bits 3..8)
+    const uint32_t keepXY = ~((i == 0u ? 1u << 3 : 0u) | (i == 7u ? 1u << 4 : 0u) |
+                              (j == 0u ? 1u << 5 : 0u) | (j == 7u ? 1u << 6 : 0u));
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const uint32_t l = lane + 64u * (uint32_t)k;
-        const uint16_t cf = c.coef[base + l];
-        const float pc = c.precon[base + l];
-        const float rv = cg_update_cell(c, base + l, cf, init, alpha, r64, p);
+        const float rv = rk[k];
         rr += (double)rv * (double)rv;
-        rk[k] = rv;
-        scf[l] = cf;
-        spc[l] = pc;
-        sq[l] = rv;  // r, replaced by the forward sweep's q plane by plane
+        const uint32_t keep = keepXY & ~((k == 0 ? 1u << 7 : 0u) | (k == 7 ? 1u << 8 : 0u));
+        s_c[kPad + l] = (uint16_t)(cf[k] & keep);
+        s_a[kPad + l] = make_float2(rv, pc[k]);  // r, replaced by the forward sweep's q plane by plane
     }
-    __syncthreads();
-    for (int s = 0; s <= 21; ++s) {
-        const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
-        for (int t = pb + (int)lane; t < pe; t += 64) {
-            const uint32_t l = c_planes.cells[t];
-            const uint16_t cf = scf[l];
-            float qv = 0.f;
-            if (cf) {
-                float acc = sq[l];
-                const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
+    // this lane's cell of each plane, fetched once so the sweeps wait only on LDS
+    uint32_t cellOf[22];
 #pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    if (lc[ax] == 0u) continue;
-                    const uint32_t m = l - stride[ax];
-                    acc -= a_plus(scf[m], ax, m) * spc[m] * sq[m];
-                }
-                qv = acc * spc[l];
-            }
-            sq[l] = qv;
-        }
-        __syncthreads();
+    for (int s = 0; s < 22; ++s) cellOf[s] = c_planes.lane_cell[s][lane];
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 0; s < 22; ++s) {
+        const uint32_t b = cellOf[s];  // own entry b + 64
+        const float2 own = s_a[b + 64u], mx = s_a[b + 63u], my = s_a[b + 56u], mz = s_a[b];
+        const uint32_t w = s_c[b + 64u];
+        float acc = own.x;
+        acc += keep_if_bit(mx.y * mx.x, w, 3);
+        acc += keep_if_bit(my.y * my.x, w, 5);
+        acc += keep_if_bit(mz.y * mz.x, w, 7);
+        s_a[b + 64u].x = acc * own.y;  // 0 off the fluid (pc = 0), and on the zero entry
+        wave_lds_sync();
     }
+#pragma unroll
     for (int s = 21; s >= 0; --s) {
-        const int pb = c_planes.start[s], pe = c_planes.start[s + 1];
-        for (int t = pb + (int)lane; t < pe; t += 64) {
-            const uint32_t l = c_planes.cells[t];
-            const uint16_t cf = scf[l];
-            float zz = 0.f;
-            if (cf) {
-                float acc = sq[l];
-                const uint32_t lc[3] = {l & 7u, (l >> 3) & 7u, l >> 6};
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    if (lc[ax] == 7u) continue;
-                    const uint32_t m = l + stride[ax];
-                    acc -= a_plus(cf, ax, l) * spc[l] * sq[m];
-                }
-                zz = acc * spc[l];
-            }
-            sq[l] = zz;
-        }
-        __syncthreads();
+        const uint32_t b = cellOf[s];
+        const float2 own = s_a[b + 64u];
+        const float ux = s_a[b + 65u].x, uy = s_a[b + 72u].x, uz = s_a[b + 128u].x;
+        const uint32_t w = s_c[b + 64u];
+        float acc = own.x;
+        acc += keep_if_bit(own.y * ux, w, 4);
+        acc += keep_if_bit(own.y * uy, w, 6);
+        acc += keep_if_bit(own.y * uz, w, 8);
+        s_a[b + 64u].x = acc * own.y;
+        wave_lds_sync();
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const uint32_t l = lane + 64u * (uint32_t)k;
-        const float zv = sq[l];
+        const float zv = s_a[kPad + l].x;
         c.z[base + l] = zv;
         rz += (double)rk[k] * (double)zv;
     }
@@ -925,9 +1065,18 @@ __global__ void prepare_faces_kernel(float* __restrict__ U, float* __restrict__ 
 // ---------------------------------------------------------------------------
 namespace {
 
-// Trilinear interpolation of a MAC component at q (cell units); faces
-// outside the stored range read 0 (walls).
-__device__ inline float interp(const float* F, int comp, const double q[3], Grid g, uint32_t nbX, uint32_t nbY) {
+// Trilinear stencil of a MAC component at q (cell units): the eight corner
+// faces in (a, b, e) order and their weights.  A corner outside the stored
+// range reads 0 (walls): it gets weight 0 and the index of a stored face, so
+// the eight loads need no branch and are all in flight together; adding
+// 0 * (finite value) leaves the running sum unchanged, so the sums equal
+// those that skip such corners.
+struct Stencil {
+    uint32_t idx[8];
+    float w[8];
+};
+
+__device__ inline void mac_stencil(int comp, const double q[3], Grid g, uint32_t nbX, uint32_t nbY, Stencil& st) {
     int base[3];
     float t[3];
 #pragma unroll
@@ -937,46 +1086,61 @@ __device__ inline float interp(const float* F, int comp, const double q[3], Grid
         base[d] = (int)f;
         t[d] = (float)(v - f);
     }
-    float acc = 0.f;
+    const int size[3] = {(int)g.sizeX, (int)g.sizeY, (int)g.sizeZ};
+    uint32_t cl[3][2];
+    bool in[3][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            const int v = base[d] + o;
+            in[d][o] = v >= 0 && v < size[d];
+            cl[d][o] = (uint32_t)min(max(v, 0), size[d] - 1);
+        }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                const int i = base[0] + a, j = base[1] + b, k = base[2] + e;
-                if (i < 0 || j < 0 || k < 0 || i >= (int)g.sizeX || j >= (int)g.sizeY || k >= (int)g.sizeZ) continue;
+                const int c = a * 4 + b * 2 + e;
                 const float w = (a ? t[0] : 1.f - t[0]) * (b ? t[1] : 1.f - t[1]) * (e ? t[2] : 1.f - t[2]);
-                acc = fmaf(w, F[tiled((uint32_t)i, (uint32_t)j, (uint32_t)k, nbX, nbY)], acc);
+                st.w[c] = (in[0][a] && in[1][b] && in[2][e]) ? w : 0.f;
+                st.idx[c] = tiled(cl[0][a], cl[1][b], cl[2][e], nbX, nbY);
             }
+}
+
+__device__ inline float interp(const float* F, int comp, const double q[3], Grid g, uint32_t nbX, uint32_t nbY) {
+    Stencil st;
+    mac_stencil(comp, q, g, nbX, nbY, st);
+    float v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = F[st.idx[c]];
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc = fmaf(st.w[c], v[c], acc);
     return acc;
 }
 
-__device__ inline float interp_delta(const float* F, const float* F0, int comp, const double q[3], Grid g,
-                                     uint32_t nbX, uint32_t nbY) {
-    int base[3];
-    float t[3];
+// interp(F) and interp(F - F0) on one stencil
+__device__ inline void interp_pic_delta(const float* F, const float* F0, int comp, const double q[3], Grid g,
+                                        uint32_t nbX, uint32_t nbY, float& pic, float& delta) {
+    Stencil st;
+    mac_stencil(comp, q, g, nbX, nbY, st);
+    float v[8], v0[8];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double v = (d == comp) ? q[d] : q[d] - 0.5;
-        const double f = floor(v);
-        base[d] = (int)f;
-        t[d] = (float)(v - f);
+    for (int c = 0; c < 8; ++c) {
+        v[c] = F[st.idx[c]];
+        v0[c] = F0[st.idx[c]];
     }
-    float acc = 0.f;
+    float a = 0.f, d = 0.f;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int i = base[0] + a, j = base[1] + b, k = base[2] + e;
-                if (i < 0 || j < 0 || k < 0 || i >= (int)g.sizeX || j >= (int)g.sizeY || k >= (int)g.sizeZ) continue;
-                const float w = (a ? t[0] : 1.f - t[0]) * (b ? t[1] : 1.f - t[1]) * (e ? t[2] : 1.f - t[2]);
-                const uint32_t idx = tiled((uint32_t)i, (uint32_t)j, (uint32_t)k, nbX, nbY);
-                acc = fmaf(w, F[idx] - F0[idx], acc);
-            }
-    return acc;
+    for (int c = 0; c < 8; ++c) {
+        a = fmaf(st.w[c], v[c], a);
+        d = fmaf(st.w[c], v[c] - v0[c], d);
+    }
+    pic = a;
+    delta = d;
 }
 
 __device__ inline void to_cell_units(float x, float y, float z, Grid g, double q[3]) {
@@ -989,11 +1153,11 @@ __device__ inline void to_cell_units(float x, float y, float z, Grid g, double q
 }  // namespace
 
 // v_p = flip * (v_p + interp(u - u0)) + (1 - flip) * interp(u)
-__global__ void g2p_flip_kernel(uint32_t n, float4* __restrict__ rec, const float* __restrict__ U,
+__global__ void __launch_bounds__(256) g2p_flip_kernel(uint32_t n, float4* __restrict__ rec, const float* __restrict__ U,
                                 const float* __restrict__ V, const float* __restrict__ W,
                                 const float* __restrict__ U0, const float* __restrict__ V0,
                                 const float* __restrict__ W0, Grid g, float flip) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t p = xcd_slot((n + 255u) >> 8) * 256u + threadIdx.x;  // particles sorted by brick
     if (p >= n) return;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
     float4 a = rec[2 * p], b = rec[2 * p + 1];
@@ -1004,8 +1168,8 @@ __global__ void g2p_flip_kernel(uint32_t n, float4* __restrict__ rec, const floa
     float vel[3] = {a.w, b.x, b.y};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float pic = interp(F[c], c, q, g, nbX, nbY);
-        const float dlt = interp_delta(F[c], F0[c], c, q, g, nbX, nbY);
+        float pic, dlt;
+        interp_pic_delta(F[c], F0[c], c, q, g, nbX, nbY, pic, dlt);
         vel[c] = flip * (vel[c] + dlt) + (1.f - flip) * pic;
     }
     a.w = vel[0];
@@ -1015,35 +1179,29 @@ __global__ void g2p_flip_kernel(uint32_t n, float4* __restrict__ rec, const floa
     rec[2 * p + 1] = b;
 }
 
-// Midpoint (RK2) advection through the grid velocity, then clamping into
-// [lo, hi] per axis and out of solid cells: a particle that ends inside the
-// solid box is pushed out through the box face nearest to it, one in another
-// solid cell returns to its start point.
-__global__ void advect_rk2_kernel(uint32_t n, float4* __restrict__ rec, const float* __restrict__ U,
-                                  const float* __restrict__ V, const float* __restrict__ W,
-                                  const uint8_t* __restrict__ solid, Grid g, float dt, float margin, SolidBox box) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+namespace {
+
+// Midpoint (RK2) advection of x0 through the grid velocity, u1 being the
+// velocity at x0; then clamping into [lo, hi] per axis and out of solid
+// cells: a particle that ends inside the solid box is pushed out through the
+// box face nearest to it, one in another solid cell returns to its start
+// point.
+__device__ inline void advect_from(const float x0[3], const float u1[3], const float* U, const float* V,
+                                   const float* W, const uint8_t* solid, Grid g, float dt, float margin,
+                                   const SolidBox& box, float xn[3]) {
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
-    float4 a = rec[2 * p];
-    const float x0[3] = {a.x, a.y, a.z};
-    double q[3];
-    to_cell_units(x0[0], x0[1], x0[2], g, q);
     const float* F[3] = {U, V, W};
-    float u1[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) u1[c] = interp(F[c], c, q, g, nbX, nbY);
     const float xm[3] = {fmaf(0.5f * dt, u1[0], x0[0]), fmaf(0.5f * dt, u1[1], x0[1]), fmaf(0.5f * dt, u1[2], x0[2])};
+    double q[3];
     to_cell_units(xm[0], xm[1], xm[2], g, q);
-    float xn[3];
     const float lo[3] = {g.negX + margin, g.negY + margin, g.negZ + margin};
     const float hi[3] = {g.negX + (float)g.sizeX * g.cellSize - margin, g.negY + (float)g.sizeY * g.cellSize - margin,
                          g.negZ + (float)g.sizeZ * g.cellSize - margin};
+    float u2[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const float u2 = interp(F[c], c, q, g, nbX, nbY);
-        xn[c] = fminf(fmaxf(fmaf(dt, u2, x0[c]), lo[c]), hi[c]);
-    }
+    for (int c = 0; c < 3; ++c) u2[c] = interp(F[c], c, q, g, nbX, nbY);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xn[c] = fminf(fmaxf(fmaf(dt, u2[c], x0[c]), lo[c]), hi[c]);
     if (box.active && xn[0] >= box.lo[0] && xn[0] < box.hi[0] && xn[1] >= box.lo[1] && xn[1] < box.hi[1] &&
         xn[2] >= box.lo[2] && xn[2] < box.hi[2]) {
         int best = -1;
@@ -1060,13 +1218,68 @@ __global__ void advect_rk2_kernel(uint32_t n, float4* __restrict__ rec, const fl
         to_cell_units(xn[0], xn[1], xn[2], g, q);
         const int ci = (int)floor(q[0]), cj = (int)floor(q[1]), ck = (int)floor(q[2]);
         if (cell_solid(ci, cj, ck, g, solid, nbX, nbY) && !in_box(ci, cj, ck, g, box)) {
-            xn[0] = x0[0]; xn[1] = x0[1]; xn[2] = x0[2];
+            xn[0] = x0[0];
+            xn[1] = x0[1];
+            xn[2] = x0[2];
         }
     }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) advect_rk2_kernel(uint32_t n, float4* __restrict__ rec, const float* __restrict__ U,
+                                  const float* __restrict__ V, const float* __restrict__ W,
+                                  const uint8_t* __restrict__ solid, Grid g, float dt, float margin, SolidBox box) {
+    const uint32_t p = xcd_slot((n + 255u) >> 8) * 256u + threadIdx.x;  // particles sorted by brick
+    if (p >= n) return;
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
+    float4 a = rec[2 * p];
+    const float x0[3] = {a.x, a.y, a.z};
+    double q[3];
+    to_cell_units(x0[0], x0[1], x0[2], g, q);
+    const float* F[3] = {U, V, W};
+    float u1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u1[c] = interp(F[c], c, q, g, nbX, nbY);
+    float xn[3];
+    advect_from(x0, u1, U, V, W, solid, g, dt, margin, box, xn);
     a.x = xn[0];
     a.y = xn[1];
     a.z = xn[2];
     rec[2 * p] = a;
+}
+
+// G2P followed by RK2 advection in one pass over the records: the same
+// results as g2p_flip_kernel then advect_rk2_kernel (advection reads the
+// grid velocity and the position, which G2P leaves unchanged; its first
+// interpolation is G2P's PIC term).
+__global__ void __launch_bounds__(256) g2p_advect_kernel(uint32_t n, float4* __restrict__ rec, const float* __restrict__ U,
+                                  const float* __restrict__ V, const float* __restrict__ W,
+                                  const float* __restrict__ U0, const float* __restrict__ V0,
+                                  const float* __restrict__ W0, const uint8_t* __restrict__ solid, Grid g, float flip,
+                                  float dt, float margin, SolidBox box) {
+    const uint32_t p = xcd_slot((n + 255u) >> 8) * 256u + threadIdx.x;  // particles sorted by brick
+    if (p >= n) return;
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
+    float4 a = rec[2 * p], b = rec[2 * p + 1];
+    const float x0[3] = {a.x, a.y, a.z};
+    double q[3];
+    to_cell_units(x0[0], x0[1], x0[2], g, q);
+    const float* F[3] = {U, V, W};
+    const float* F0[3] = {U0, V0, W0};
+    float vel[3] = {a.w, b.x, b.y}, u1[3];
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        float dlt;
+        interp_pic_delta(F[c], F0[c], c, q, g, nbX, nbY, u1[c], dlt);
+        vel[c] = flip * (vel[c] + dlt) + (1.f - flip) * u1[c];
+    }
+    float xn[3];
+    advect_from(x0, u1, U, V, W, solid, g, dt, margin, box, xn);
+    rec[2 * p] = make_float4(xn[0], xn[1], xn[2], vel[0]);
+    b.x = vel[1];
+    b.y = vel[2];
+    rec[2 * p + 1] = b;
 }
 
 // ---------------------------------------------------------------------------
@@ -1074,9 +1287,13 @@ __global__ void advect_rk2_kernel(uint32_t n, float4* __restrict__ rec, const fl
 // ---------------------------------------------------------------------------
 namespace {
 inline dim3 g1(uint32_t n, uint32_t bs) { return dim3(ceil_div(n ? n : 1, bs)); }
+// grid of a kernel that maps workgroups to slots with xcd_slot
+inline dim3 xcd_grid(uint32_t na) { return dim3(8u * ceil_div(na ? na : 1, 8)); }
 // out[0..count) = sums of the workgroup partials of the kernel just launched
 void reduce_to(const Reduce& red, int count, uint32_t blocks, double* out, hipStream_t s) {
-    FLIP2_KERNEL(reduce_partials_kernel, dim3(count), dim3(kWG), 0, s, red.partials, blocks, out);
+    // one load per thread per 8 partials: a small sum needs fewer threads
+    const int threads = blocks <= 2048u ? 256 : kReduceThreads;
+    FLIP2_KERNEL(reduce_partials_kernel, dim3(count), dim3(threads), 0, s, red.partials, blocks, out);
     FLIP2_LAUNCH_CHECK();
 }
 }  // namespace
@@ -1117,7 +1334,8 @@ void launch_refine_start(uint32_t numActive, const double* r64, float* r, float*
 }
 void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
                        Reduce red, double* out, hipStream_t s) {
-    FLIP2_KERNEL(matvec_dot_kernel, dim3(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials);
+    FLIP2_KERNEL(matvec_dot_kernel, xcd_grid(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials,
+                 numActive);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
@@ -1180,8 +1398,8 @@ void launch_accumulate(uint32_t numActive, const float* e, double* x64, const ui
 }
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s) {
-    FLIP2_KERNEL(true_residual_kernel, dim3(numActive), dim3(kWG), 0, s, b64, x64, coef, nbrSlot, r64,
-                       red.partials);
+    FLIP2_KERNEL(true_residual_kernel, xcd_grid(numActive), dim3(kWG), 0, s, b64, x64, coef, nbrSlot, r64,
+                 red.partials, numActive);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
@@ -1201,16 +1419,16 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
 }
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s) {
-    FLIP2_KERNEL(cg_direction_matvec_kernel, dim3(c.na), dim3(kWG), 0, s, c, it, first ? 1 : 0, pOld, pNew);
+    FLIP2_KERNEL(cg_direction_matvec_kernel, xcd_grid(c.na), dim3(64), 0, s, c, it, first ? 1 : 0, pOld, pNew);
     FLIP2_LAUNCH_CHECK();
     reduce_to(Reduce{c.partials}, 1, c.na, c.hist + 4u * it + 2u, s);  // (p.q)_it
 }
 void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
                               hipStream_t s) {
     if (mic)
-        FLIP2_KERNEL(cg_update_mic_kernel, dim3(c.na), dim3(64), 0, s, c, it, init ? 1 : 0, r64, p);
+        FLIP2_KERNEL(cg_update_mic_kernel, xcd_grid(c.na), dim3(64), 0, s, c, it, init ? 1 : 0, r64, p);
     else
-        FLIP2_KERNEL(cg_update_jacobi_kernel, dim3(c.na), dim3(kWG), 0, s, c, it, init ? 1 : 0, r64, p);
+        FLIP2_KERNEL(cg_update_jacobi_kernel, xcd_grid(c.na), dim3(64), 0, s, c, it, init ? 1 : 0, r64, p);
     FLIP2_LAUNCH_CHECK();
     reduce_to(Reduce{c.partials}, 2, c.na, c.hist + 4u * (init ? it : it + 1u), s);  // rz, rr of the next entry
 }
@@ -1237,13 +1455,21 @@ void launch_rasterize_solid(const uint8_t* staticMask, SolidBox box, Grid g, uin
 void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
                      const float* V0, const float* W0, Grid g, float flip, hipStream_t s) {
     if (!n) return;
-    FLIP2_KERNEL(g2p_flip_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, U0, V0, W0, g, flip);
+    FLIP2_KERNEL(g2p_flip_kernel, xcd_grid(ceil_div(n, 256)), dim3(256), 0, s, n, rec, U, V, W, U0, V0, W0, g, flip);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
                        Grid g, float dt, float margin, SolidBox box, hipStream_t s) {
     if (!n) return;
-    FLIP2_KERNEL(advect_rk2_kernel, g1(n, 256), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
+    FLIP2_KERNEL(advect_rk2_kernel, xcd_grid(ceil_div(n, 256)), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_g2p_advect(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
+                       const float* V0, const float* W0, const uint8_t* solid, Grid g, float flip, float dt,
+                       float margin, SolidBox box, hipStream_t s) {
+    if (!n) return;
+    FLIP2_KERNEL(g2p_advect_kernel, xcd_grid(ceil_div(n, 256)), dim3(256), 0, s, n, rec, U, V, W, U0, V0, W0, solid, g, flip, dt,
+                 margin, box);
     FLIP2_LAUNCH_CHECK();
 }
 

@@ -107,6 +107,10 @@ void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, co
                      const float* V0, const float* W0, Grid g, float flip, hipStream_t s);
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
                        Grid g, float dt, float margin, SolidBox box, hipStream_t s);
+// launch_g2p_flip then launch_advect_rk2 (same results) in one kernel
+void launch_g2p_advect(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
+                       const float* V0, const float* W0, const uint8_t* solid, Grid g, float flip, float dt,
+                       float margin, SolidBox box, hipStream_t s);
 
 }  // namespace ns
 }  // namespace flip2

@@ -130,6 +130,36 @@ def test_moving_solid_box_boundary(flip2, oracle):
     assert np.all(sim.get("U")[idx][1:8, :, :] == 0.0)  # no particles there: P2G gives 0
 
 
+@pytest.mark.parametrize("box", [False, True])
+def test_fused_g2p_advect_bit_identical(flip2, box):
+    """The step's fused G2P + RK2 kernel gives the records of the g2p stage
+    followed by the advect stage, bit for bit (with and without a moving
+    solid box, whose push-out and return-to-start branches it shares)."""
+    n = 32
+    h = 1.0 / n
+    x, y, z = scenes.dam_break(n, seed=21)
+    x = np.append(x, np.float32(0.2)); y = np.append(y, np.float32(0.1)); z = np.append(z, np.float32(0.5))
+    rng = np.random.default_rng(5)
+    v = [(rng.standard_normal(x.size) * 0.5).astype(np.float32) for _ in range(3)]
+    grid = (0.0, 0.0, 0.0, n, n, n, h)
+    out = []
+    for fused in (False, True):
+        sim = flip2.FluidSim(x.size, grid, preconditioner="mic0")
+        sim.set_particles(x, y, z, *v)
+        if box:
+            sim.set_solid_box((0.0, 0.0, 0.0), (0.25, 1.0, 1.0), (0.5, 0.0, 0.0))
+        for s in ("transfer", "prepare", "system", "solve", "project"):
+            sim.stage(s)
+        if fused:
+            sim.stage("g2p_advect")
+        else:
+            sim.stage("g2p")
+            sim.stage("advect")
+        out.append(sim.particles())
+    for name, a, b in zip(("x", "y", "z", "vx", "vy", "vz"), *out):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+
+
 def test_snapshot_resume_bit_identical(flip2, tmp_path):
     """A run saved mid-way and resumed in a fresh simulation (default
     configuration: the snapshot restores the settings, the static solid and
