@@ -818,6 +818,7 @@ int flip2_fluid_stage(flip2_fluid* f, const char* stage) {
         const flip2::KernelTimer::Install timed(f->activeTimer());
         const std::string s(stage);
         if (s == "transfer") f->impl.transfer();
+        else if (s == "transfer_prepare") f->impl.transferAndPrepare();
         else if (s == "prepare") f->impl.prepareFaces();
         else if (s == "system") f->impl.buildSystem();
         else if (s == "solve") f->impl.solve();

@@ -359,6 +359,13 @@ void FluidSim::transfer() {
     rec_.swapDevice(pipe_->sortedVec());
 }
 
+void FluidSim::transferAndPrepare() {
+    const ns::FacePrep prep{U0_.devPtr(), V0_.devPtr(), W0_.devPtr(), hasSolid_ ? solid_.devPtr() : nullptr, box_,
+                            cfg_.gravity * cfg_.dt};
+    pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_, &prep);
+    rec_.swapDevice(pipe_->sortedVec());
+}
+
 void FluidSim::prepareFaces() {
     ns::launch_prepare_faces(U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
                              hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.gravity * cfg_.dt, box_, stream_);
@@ -609,8 +616,7 @@ void FluidSim::gridToParticlesAndAdvect() {
 }
 
 void FluidSim::step() {
-    transfer();
-    prepareFaces();
+    transferAndPrepare();
     buildSystem();
     solve();
     project();

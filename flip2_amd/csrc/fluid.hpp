@@ -108,6 +108,7 @@ public:
     // Stages (the step is these in order)
     void transfer();
     void prepareFaces();
+    void transferAndPrepare();  // transfer then prepareFaces, fused (the same results)
     void buildSystem();
     SolveStats solve();
     void project();

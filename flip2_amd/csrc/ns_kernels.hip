@@ -25,6 +25,8 @@
 // neighbouring cells' particles in P2G touch one memory segment per particle.
 #include "ns_kernels.hpp"
 
+#include "face_prep.hpp"
+
 namespace flip2 {
 namespace ns {
 
@@ -281,40 +283,26 @@ constexpr int kShellRuns = 222;
 // one per thread; each scatters trilinear weights into the brick's owned
 // faces (the low faces of its 512 cells, per component) held in LDS as
 // 64-bit fixed-point sums.  Each owned face is written once:
-// u = sum(w v) / sum(w), 0 where no particle reaches it.  Workgroups are
-// dealt to the 8 XCDs round-robin; each XCD gets a contiguous range of
-// bricks, in Z-order when the brick grid is a power-of-two cube, so the
-// shell cells a workgroup reads were loaded by its neighbours into the same
-// L2.
-__global__ void __launch_bounds__(256)
-p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart, Grid g,
-                 uint32_t numBricks, int morton, const uint32_t* __restrict__ stats, float* __restrict__ U,
-                 float* __restrict__ V, float* __restrict__ W, float* __restrict__ WU, float* __restrict__ WV,
-                 float* __restrict__ WW) {
-    __shared__ unsigned long long s_w[3][512];
-    __shared__ unsigned long long s_m[3][512];
-    __shared__ uint32_t s_first[kShellRuns + 1];
-    __shared__ uint32_t s_off[kShellRuns + 2];
-    __shared__ uint32_t s_wave[4];
+// u = sum(w v) / sum(w), 0 where no particle reaches it.
+struct P2GShared {
+    unsigned long long w[3][512];
+    unsigned long long m[3][512];
+    uint32_t first[kShellRuns + 1];
+    uint32_t off[kShellRuns + 2];
+    uint32_t wave[4];
+};
+
+// Scatter of brick bb (index `brick`) into sh.w, sh.m (cleared here) by the
+// 256 threads of the workgroup; scaleW, scaleM: the fixed-point scales.
+// Ends with the sums complete and visible to the workgroup.
+__device__ inline void p2g_scatter_brick(P2GShared& sh, const int bb[3], uint32_t brick, const float4* __restrict__ rec,
+                                         const uint32_t* __restrict__ cellStart, Grid g, uint32_t nbX, uint32_t nbY,
+                                         uint32_t nbZ, const uint32_t* __restrict__ stats, float& scaleW,
+                                         float& scaleM) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t perXcd = (numBricks + 7u) >> 3;
-    const uint32_t order = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
-    if (order >= numBricks) return;
-    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
-    int bb[3];
-    if (morton) {
-        bb[0] = (int)compact3(order);
-        bb[1] = (int)compact3(order >> 1);
-        bb[2] = (int)compact3(order >> 2);
-    } else {
-        bb[0] = (int)(order % nbX);
-        bb[1] = (int)((order / nbX) % nbY);
-        bb[2] = (int)(order / (nbX * nbY));
-    }
-    const uint32_t brick = ((uint32_t)bb[2] * nbY + (uint32_t)bb[1]) * nbX + (uint32_t)bb[0];
     for (uint32_t t = tid; t < 1536u; t += 256u) {
-        (&s_w[0][0])[t] = 0ull;
-        (&s_m[0][0])[t] = 0ull;
+        (&sh.w[0][0])[t] = 0ull;
+        (&sh.m[0][0])[t] = 0ull;
     }
     // run table and its exclusive scan
     uint32_t first = 0, cnt = 0;
@@ -325,7 +313,7 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
             first = cellStart[c0];
             cnt = cellStart[c1] - first;
         }
-        s_first[tid] = first;
+        sh.first[tid] = first;
     }
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint32_t incl = cnt;
@@ -334,13 +322,13 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
         const uint32_t o = __shfl_up(incl, d, 64);
         if (lane >= (uint32_t)d) incl += o;
     }
-    if (lane == 63u) s_wave[wave] = incl;
+    if (lane == 63u) sh.wave[wave] = incl;
     __syncthreads();
     uint32_t base = 0;
-    for (uint32_t w = 0; w < wave; ++w) base += s_wave[w];
-    const uint32_t total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-    if (tid <= (uint32_t)kShellRuns) s_off[tid] = base + incl - cnt;
-    if (tid == 0) s_off[kShellRuns + 1] = total;
+    for (uint32_t w = 0; w < wave; ++w) base += sh.wave[w];
+    const uint32_t total = sh.wave[0] + sh.wave[1] + sh.wave[2] + sh.wave[3];
+    if (tid <= (uint32_t)kShellRuns) sh.off[tid] = base + incl - cnt;
+    if (tid == 0) sh.off[kShellRuns + 1] = total;
     __syncthreads();
 
     // fixed-point scales: a face's weight sum is at most the number of
@@ -348,9 +336,9 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
     const float bound = (float)(total > 0u ? total : 1u);
     const float vmax = __uint_as_float(stats[0]);
     const int ew = ilogbf(bound) + 1;
-    const float scaleW = ldexpf(1.0f, 62 - ew);
+    scaleW = ldexpf(1.0f, 62 - ew);
     const int em = vmax > 0.f ? ilogbf(bound * vmax) + 1 : ew;
-    const float scaleM = ldexpf(1.0f, min(62 - em, 126));
+    scaleM = ldexpf(1.0f, min(62 - em, 126));
 
     const double inv = 1.0 / (double)g.cellSize;
     const double org[3] = {(double)g.negX + (double)(bb[0] * 8) * (double)g.cellSize,
@@ -375,7 +363,7 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
         int b = kShellRuns + 1;  // last run with s_off <= lBeg
         while (b - run > 1) {
             const int m = (run + b) >> 1;
-            if (s_off[m] <= lBeg) run = m;
+            if (sh.off[m] <= lBeg) run = m;
             else b = m;
         }
     }
@@ -385,8 +373,8 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
         for (uint32_t k = 0; k < kPre; ++k) {
             if (i0 + k >= lCnt) break;
             const uint32_t sIdx = lBeg + i0 + k;
-            while (s_off[run + 1] <= sIdx) ++run;  // s_off[kShellRuns + 1] = total > sIdx
-            const uint32_t p = s_first[run] + (sIdx - s_off[run]);
+            while (sh.off[run + 1] <= sIdx) ++run;  // sh.off[kShellRuns + 1] = total > sIdx
+            const uint32_t p = sh.first[run] + (sIdx - sh.off[run]);
             pr0[k] = rec[2 * p];
             pr1[k] = rec[2 * p + 1];
         }
@@ -427,8 +415,8 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
                             if ((unsigned)lk >= 8u) continue;
                             const float wt = wab * (bz ? t1[2] : 1.f - t1[2]);
                             const int node = lk * 64 + lj * 8 + li;
-                            atomicAdd(&s_w[c][node], (unsigned long long)__float2ull_rn(wt * scaleW));
-                            atomicAdd(&s_m[c][node], (unsigned long long)__float2ll_rn(wt * mv));
+                            atomicAdd(&sh.w[c][node], (unsigned long long)__float2ull_rn(wt * scaleW));
+                            atomicAdd(&sh.m[c][node], (unsigned long long)__float2ll_rn(wt * mv));
                         }
                     }
                 }
@@ -436,17 +424,162 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
         }
     }
     __syncthreads();
+}
+
+// The face value of component c at cell t of the scattered brick.
+__device__ inline float p2g_face(const P2GShared& sh, int c, uint32_t t, double iw, double im, double& ws) {
+    ws = (double)sh.w[c][t] * iw;
+    const double ms = (double)(long long)sh.m[c][t] * im;
+    return ws > 0.0 ? (float)(ms / ws) : 0.f;
+}
+
+// Bricks are dealt to the 8 XCDs round-robin by workgroup; each XCD gets a
+// contiguous range of bricks, in Z-order when the brick grid is a
+// power-of-two cube, so the shell cells a workgroup reads were loaded by its
+// neighbours into the same L2.
+__global__ void __launch_bounds__(256)
+p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart, Grid g,
+                 uint32_t numBricks, int morton, const uint32_t* __restrict__ stats, float* __restrict__ U,
+                 float* __restrict__ V, float* __restrict__ W, float* __restrict__ WU, float* __restrict__ WV,
+                 float* __restrict__ WW) {
+    __shared__ P2GShared sh;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t perXcd = (numBricks + 7u) >> 3;
+    const uint32_t order = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
+    if (order >= numBricks) return;
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
+    int bb[3];
+    if (morton) {
+        bb[0] = (int)compact3(order);
+        bb[1] = (int)compact3(order >> 1);
+        bb[2] = (int)compact3(order >> 2);
+    } else {
+        bb[0] = (int)(order % nbX);
+        bb[1] = (int)((order / nbX) % nbY);
+        bb[2] = (int)(order / (nbX * nbY));
+    }
+    const uint32_t brick = ((uint32_t)bb[2] * nbY + (uint32_t)bb[1]) * nbX + (uint32_t)bb[0];
+    float scaleW, scaleM;
+    p2g_scatter_brick(sh, bb, brick, rec, cellStart, g, nbX, nbY, nbZ, stats, scaleW, scaleM);
     float* out[3] = {U, V, W};
     float* outW[3] = {WU, WV, WW};
     const double iw = 1.0 / (double)scaleW, im = 1.0 / (double)scaleM;
     for (uint32_t t = tid; t < 512u; t += 256u) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double ws = (double)s_w[c][t] * iw;
-            const double ms = (double)(long long)s_m[c][t] * im;
-            out[c][brick * 512u + t] = ws > 0.0 ? (float)(ms / ws) : 0.f;
+            double ws;
+            out[c][brick * 512u + t] = p2g_face(sh, c, t, iw, im, ws);
             if (outW[c]) outW[c][brick * 512u + t] = (float)ws;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transfer fused with the prepare stage (walls, solid box, gravity): only
+// bricks that particles reach are scattered; every other brick's faces are
+// the prepared values of a zero transfer.
+// ---------------------------------------------------------------------------
+
+// occ[b] = 1 if brick b holds a particle (cellStart is the exclusive scan of
+// the per-cell counts in brick-tiled cell order).
+__global__ void brick_occupancy_kernel(const uint32_t* __restrict__ cellStart, uint32_t numBricks,
+                                       uint8_t* __restrict__ occ) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= numBricks) return;
+    occ[b] = cellStart[(b + 1u) * 512u] != cellStart[b * 512u] ? 1 : 0;
+}
+
+// touched[b] = 1 if brick b or one of its 26 neighbours holds a particle (a
+// superset of the bricks whose faces a particle reaches: the P2G shell is one
+// cell deep); the touched bricks are appended to list (order: by wave, each
+// wave's bricks ascending; the transfer's result does not depend on it).
+__global__ void touched_bricks_kernel(const uint8_t* __restrict__ occ, uint32_t nbX, uint32_t nbY, uint32_t nbZ,
+                                      uint8_t* __restrict__ touched, uint32_t* __restrict__ list,
+                                      uint32_t* __restrict__ count) {
+    const uint32_t numBricks = nbX * nbY * nbZ;
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (b < numBricks) {
+        const int bx = (int)(b % nbX), by = (int)((b / nbX) % nbY), bz = (int)(b / (nbX * nbY));
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int x = bx + dx, y = by + dy, z = bz + dz;
+                    if (x < 0 || y < 0 || z < 0 || x >= (int)nbX || y >= (int)nbY || z >= (int)nbZ) continue;
+                    hit |= occ[((uint32_t)z * nbY + (uint32_t)y) * nbX + (uint32_t)x] != 0;
+                }
+        touched[b] = hit ? 1 : 0;
+    }
+    const unsigned long long mask = __ballot(hit);
+    if (!mask) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, 0, 64);
+    if (hit) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = b;
+}
+
+// Faces of the untouched bricks: prepared values of a zero transfer.
+__global__ void __launch_bounds__(256)
+empty_faces_kernel(const uint8_t* __restrict__ touched, uint32_t numCells, Grid g, const uint8_t* __restrict__ solid,
+                   SolidBox box, float gdt, float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
+                   float* __restrict__ U0, float* __restrict__ V0, float* __restrict__ W0) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= numCells || touched[c >> 9]) return;
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
+    const uint32_t l = c & 511u, b = c >> 9;
+    const int gc[3] = {(int)((b % nbX) * 8u + (l & 7u)), (int)(((b / nbX) % nbY) * 8u + ((l >> 3) & 7u)),
+                       (int)((b / (nbX * nbY)) * 8u + (l >> 6))};
+    const float in[3] = {0.f, 0.f, 0.f};
+    float saved[3], out[3];
+    prepare_cell_faces(in, gc, g, solid, box, gdt, nbX, nbY, saved, out);
+    U0[c] = saved[0];
+    V0[c] = saved[1];
+    W0[c] = saved[2];
+    U[c] = out[0];
+    V[c] = out[1];
+    W[c] = out[2];
+}
+
+// Scatter + prepare of the touched bricks; a fixed grid of workgroups loops
+// over the list, XCD x taking the x-th contiguous eighth of it (neighbouring
+// bricks in the same L2).
+__global__ void __launch_bounds__(256)
+p2g_prepare_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                   const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart, Grid g,
+                   const uint32_t* __restrict__ stats, const uint8_t* __restrict__ solid, SolidBox box, float gdt,
+                   float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, float* __restrict__ U0,
+                   float* __restrict__ V0, float* __restrict__ W0) {
+    __shared__ P2GShared sh;
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
+    const uint32_t total = *count;
+    const uint32_t perXcdGroups = gridDim.x >> 3, xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
+    const uint32_t lo = (uint32_t)(((uint64_t)total * xcd) >> 3), hi = (uint32_t)(((uint64_t)total * (xcd + 1u)) >> 3);
+    float* F[3] = {U, V, W};
+    float* F0[3] = {U0, V0, W0};
+    for (uint32_t e = lo + rank; e < hi; e += perXcdGroups) {
+        const uint32_t brick = list[e];
+        const int bb[3] = {(int)(brick % nbX), (int)((brick / nbX) % nbY), (int)(brick / (nbX * nbY))};
+        float scaleW, scaleM;
+        p2g_scatter_brick(sh, bb, brick, rec, cellStart, g, nbX, nbY, nbZ, stats, scaleW, scaleM);
+        const double iw = 1.0 / (double)scaleW, im = 1.0 / (double)scaleM;
+        for (uint32_t t = threadIdx.x; t < 512u; t += 256u) {
+            float in[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double ws;
+                in[c] = p2g_face(sh, c, t, iw, im, ws);
+            }
+            const int gc[3] = {bb[0] * 8 + (int)(t & 7u), bb[1] * 8 + (int)((t >> 3) & 7u), bb[2] * 8 + (int)(t >> 6)};
+            float saved[3], out[3];
+            prepare_cell_faces(in, gc, g, solid, box, gdt, nbX, nbY, saved, out);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                F0[c][brick * 512u + t] = saved[c];
+                F[c][brick * 512u + t] = out[c];
+            }
+        }
+        __syncthreads();  // the next brick clears the sums
     }
 }
 
@@ -522,6 +655,31 @@ void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, cons
     const uint32_t blocks = ((numBricks + 7u) >> 3) * 8u;
     FLIP2_KERNEL(p2g_brick_kernel, dim3(blocks), dim3(256), 0, s, sortedRec, cellStart, g, numBricks, morton,
                        stats, U, V, W, WU, WV, WW);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_p2g_prepare(const float4* sortedRec, const uint32_t* cellStart, Grid g, const uint32_t* stats,
+                        const FacePrep& prep, float* U, float* V, float* W, TouchedBricks t, hipStream_t s) {
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
+    const uint32_t numBricks = nbX * nbY * nbZ;
+    if (!numBricks) return;
+    FLIP2_REQUIRE(prep.U0 && prep.V0 && prep.W0 && t.occ && t.touched && t.list && t.count, kErrArgument,
+                  "transfer + prepare: null array");
+    FLIP2_KERNEL(zero_u32_kernel, dim3(1), dim3(64), 0, s, t.count, 1u);
+    FLIP2_LAUNCH_CHECK();
+    FLIP2_KERNEL(brick_occupancy_kernel, grid1d(numBricks, 256), dim3(256), 0, s, cellStart, numBricks, t.occ);
+    FLIP2_LAUNCH_CHECK();
+    FLIP2_KERNEL(touched_bricks_kernel, grid1d(numBricks, 256), dim3(256), 0, s, t.occ, nbX, nbY, nbZ, t.touched,
+                 t.list, t.count);
+    FLIP2_LAUNCH_CHECK();
+    const uint32_t numCells = numBricks * 512u;
+    FLIP2_KERNEL(empty_faces_kernel, grid1d(numCells, 256), dim3(256), 0, s, t.touched, numCells, g, prep.solid,
+                 prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0);
+    FLIP2_LAUNCH_CHECK();
+    // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
+    const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
+    FLIP2_KERNEL(p2g_prepare_kernel, dim3(groups), dim3(256), 0, s, t.list, t.count, sortedRec, cellStart, g, stats,
+                 prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0);
     FLIP2_LAUNCH_CHECK();
 }
 

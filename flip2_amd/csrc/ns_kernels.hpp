@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "ns_solver.hpp"
 
 namespace flip2 {
 namespace ns {
@@ -39,6 +40,25 @@ void launch_p2g_stats(const float4* sortedRec, const uint32_t* cellStart, uint32
 // (weight sums) may be null.
 void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, const uint32_t* stats, float* U, float* V,
                 float* W, float* WU, float* WV, float* WW, hipStream_t s);
+
+// The prepare stage's inputs and saved field, for the fused transfer.
+struct FacePrep {
+    float *U0, *V0, *W0;  // pre-projection field (written)
+    const uint8_t* solid;  // brick-tiled solid cells, may be null
+    SolidBox box;
+    float gdt;  // gravity * dt, added to v faces
+};
+// Work arrays of the fused transfer: occ, touched (one byte per brick),
+// list (one word per brick), count (one word).
+struct TouchedBricks {
+    uint8_t *occ, *touched;
+    uint32_t *list, *count;
+};
+// launch_p2g followed by launch_prepare_faces (the same U, V, W, U0, V0, W0),
+// with the transfer done only for bricks within one brick of a particle and
+// the others given the prepared faces of a zero transfer.
+void launch_p2g_prepare(const float4* sortedRec, const uint32_t* cellStart, Grid g, const uint32_t* stats,
+                        const FacePrep& prep, float* U, float* V, float* W, TouchedBricks t, hipStream_t s);
 
 }  // namespace ns
 }  // namespace flip2

@@ -21,6 +21,8 @@
 // summing kernel, always in workgroup order.
 #include "ns_solver.hpp"
 
+#include "face_prep.hpp"
+
 namespace flip2 {
 namespace ns {
 
@@ -1001,23 +1003,6 @@ velocity_update_kernel(const uint32_t* __restrict__ activeBricks, const int32_t*
 // ---------------------------------------------------------------------------
 // Body force, FLIP state and boundary faces
 // ---------------------------------------------------------------------------
-__device__ inline bool cell_solid(int i, int j, int k, Grid g, const uint8_t* solid, uint32_t nbX, uint32_t nbY) {
-    if (i < 0 || j < 0 || k < 0 || i >= (int)g.sizeX || j >= (int)g.sizeY || k >= (int)g.sizeZ) return true;
-    return solid && solid[tiled((uint32_t)i, (uint32_t)j, (uint32_t)k, nbX, nbY)];
-}
-
-__device__ inline bool in_box(int i, int j, int k, Grid g, const SolidBox& box) {
-    if (!box.active) return false;
-    const float c[3] = {g.negX + ((float)i + 0.5f) * g.cellSize, g.negY + ((float)j + 0.5f) * g.cellSize,
-                        g.negZ + ((float)k + 0.5f) * g.cellSize};
-    return c[0] >= box.lo[0] && c[0] < box.hi[0] && c[1] >= box.lo[1] && c[1] < box.hi[1] && c[2] >= box.lo[2] &&
-           c[2] < box.hi[2];
-}
-
-__device__ inline bool in_domain(int i, int j, int k, Grid g) {
-    return i >= 0 && j >= 0 && k >= 0 && i < (int)g.sizeX && j < (int)g.sizeY && k < (int)g.sizeZ;
-}
-
 __global__ void rasterize_solid_kernel(const uint8_t* __restrict__ staticMask, SolidBox box, Grid g,
                                        uint8_t* __restrict__ solid, uint32_t numCells) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1038,26 +1023,17 @@ __global__ void prepare_faces_kernel(float* __restrict__ U, float* __restrict__ 
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= numCells) return;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
-    const uint32_t b = c >> 9, l = c & 511u;
     int gc[3];
-    cell_coords(b, l, nbX, nbY, gc);
-    const bool here = cell_solid(gc[0], gc[1], gc[2], g, solid, nbX, nbY);
-    float* F[3] = {U, V, W};
-    float* F0[3] = {U0, V0, W0};
-    for (int axis = 0; axis < 3; ++axis) {
-        int m[3] = {gc[0], gc[1], gc[2]};
-        m[axis] -= 1;
-        const bool nbSolid = cell_solid(m[0], m[1], m[2], g, solid, nbX, nbY);
-        const bool wall = here || nbSolid;
-        float vs = 0.f;  // solid velocity of this face
-        if (wall && in_domain(m[0], m[1], m[2], g) &&
-            ((here && in_box(gc[0], gc[1], gc[2], g, box)) || (nbSolid && in_box(m[0], m[1], m[2], g, box))))
-            vs = box.vel[axis];
-        float v = wall ? vs : F[axis][c];
-        F0[axis][c] = v;
-        if (axis == 1 && !wall) v += gdt;
-        F[axis][c] = v;
-    }
+    cell_coords(c >> 9, c & 511u, nbX, nbY, gc);
+    const float in[3] = {U[c], V[c], W[c]};
+    float saved[3], out[3];
+    prepare_cell_faces(in, gc, g, solid, box, gdt, nbX, nbY, saved, out);
+    U0[c] = saved[0];
+    V0[c] = saved[1];
+    W0[c] = saved[2];
+    U[c] = out[0];
+    V[c] = out[1];
+    W[c] = out[2];
 }
 
 // ---------------------------------------------------------------------------

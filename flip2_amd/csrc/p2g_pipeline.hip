@@ -16,6 +16,8 @@ P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g, DeviceManager* po
     numCells_ = (uint32_t)cells;
     for (auto* v : {&keys_, &keysSorted_, &perm_, &keysAlt_, &permAlt_, &counts_, &cellStart_, &stats_}) v->setPool(pool);
     for (auto* v : {&packed_, &sorted_}) v->setPool(pool);
+    for (auto* v : {&occ_, &touched_}) v->setPool(pool);
+    touchedList_.setPool(pool);
     ws_.setPool(pool);
     keyBits_ = std::max<uint32_t>(1, bits_for(numCells_ - 1));
     keys_.resize(maxParticles); keys_.host().clear(); keys_.host().shrink_to_fit();
@@ -37,16 +39,17 @@ void P2GPipeline::run(uint32_t n, const float* x, const float* y, const float* z
                       const float* vz, float* U, float* V, float* W, hipStream_t s) {
     FLIP2_REQUIRE(n == 0 || (x && y && z && vx && vy && vz), kErrArgument, "null particle array");
     const ns::Float6 src{{x, y, z, vx, vy, vz}};
-    runImpl(n, &src, nullptr, U, V, W, s);
+    runImpl(n, &src, nullptr, U, V, W, s, nullptr);
 }
 
-void P2GPipeline::runRecords(uint32_t n, const float4* rec, float* U, float* V, float* W, hipStream_t s) {
+void P2GPipeline::runRecords(uint32_t n, const float4* rec, float* U, float* V, float* W, hipStream_t s,
+                             const ns::FacePrep* prep) {
     FLIP2_REQUIRE(n == 0 || rec, kErrArgument, "null particle records");
-    runImpl(n, nullptr, rec, U, V, W, s);
+    runImpl(n, nullptr, rec, U, V, W, s, prep);
 }
 
 void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W,
-                          hipStream_t s) {
+                          hipStream_t s, const ns::FacePrep* prep) {
     FLIP2_REQUIRE(n <= maxParticles_, kErrRange, "more particles than the pipeline was created for");
     FLIP2_REQUIRE(U && V && W, kErrArgument, "null output array");
     lastN_ = n;
@@ -73,8 +76,21 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
     if (timed) FLIP2_HIP(hipEventRecord(ev[4 + passes], s));
     scan_u32(counts_.devPtr(), cellStart_.devPtr(), numCells_ + 1, true, ws_.devPtr(), ws_.size(), nullptr, s);
     if (timed) FLIP2_HIP(hipEventRecord(ev[5 + passes], s));
-    ns::launch_p2g(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), U, V, W, nullptr, nullptr, nullptr,
-                   s);
+    if (prep) {
+        const uint32_t numBricks = numCells_ >> 9;
+        if (occ_.size() < numBricks) {
+            occ_.resizeAsync(numBricks, s);
+            touched_.resizeAsync(numBricks, s);
+            touchedList_.resizeAsync(numBricks + 1, s);
+        }
+        ns::launch_p2g_prepare(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), *prep, U, V, W,
+                               {occ_.devPtr(), touched_.devPtr(), touchedList_.devPtr(),
+                                touchedList_.devPtr() + numBricks},
+                               s);
+    } else {
+        ns::launch_p2g(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), U, V, W, nullptr, nullptr,
+                       nullptr, s);
+    }
     if (timed) {
         FLIP2_HIP(hipEventRecord(ev[6 + passes], s));
         ++pending_;

@@ -24,8 +24,11 @@ public:
     // sorted state, sort permutation and per-cell start offsets.
     void run(uint32_t n, const float* x, const float* y, const float* z, const float* vx, const float* vy,
              const float* vz, float* U, float* V, float* W, hipStream_t s);
-    // Same, from particle records (2 x float4 per particle).
-    void runRecords(uint32_t n, const float4* rec, float* U, float* V, float* W, hipStream_t s);
+    // Same, from particle records (2 x float4 per particle).  With prep, the
+    // transfer is followed by the prepare stage in the same kernels
+    // (ns::launch_p2g_prepare): U, V, W then hold the prepared faces.
+    void runRecords(uint32_t n, const float4* rec, float* U, float* V, float* W, hipStream_t s,
+                    const ns::FacePrep* prep = nullptr);
     size_t get(const std::string& name, void* host, size_t capacity, hipStream_t s);
     // Per-kernel timing of subsequent runs (HIP events on the run's stream):
     // ms[k] accumulates over runs for kernels "bin", "sort_setup" (clear +
@@ -64,7 +67,10 @@ private:
     std::vector<std::string> timingNames_;
     std::vector<double> timingMs_;
     int timedRuns_ = 0;
-    void runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W, hipStream_t s);
+    DeviceVec<uint8_t> occ_, touched_;  // fused transfer + prepare: per-brick flags
+    DeviceVec<uint32_t> touchedList_;    // and the touched bricks (list, then count)
+    void runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W, hipStream_t s,
+                 const ns::FacePrep* prep);
 };
 
 }  // namespace flip2

@@ -339,8 +339,8 @@ int flip2_fluid_step(flip2_fluid* f);
  * sampled maximum (getCourantDt, particles.cu:493-498; cudaVec.cu:173-233). */
 int flip2_fluid_step_frame(flip2_fluid* f, double frame_dt, double cfl, uint32_t max_substeps, uint32_t* substeps);
 /* One stage: "transfer", "prepare", "system", "solve", "project", "g2p",
- * "advect", or "g2p_advect" (g2p then advect in one kernel, as the step
- * runs them; the same results). */
+ * "advect", or the fused pairs the step runs, with the same results:
+ * "transfer_prepare" (transfer then prepare) and "g2p_advect". */
 int flip2_fluid_stage(flip2_fluid* f, const char* stage);
 /* Solve the current system with the given preconditioner (the system must
  * have been built by the "system" stage). */

@@ -130,6 +130,45 @@ def test_moving_solid_box_boundary(flip2, oracle):
     assert np.all(sim.get("U")[idx][1:8, :, :] == 0.0)  # no particles there: P2G gives 0
 
 
+@pytest.mark.parametrize("solid", ["none", "box", "static"])
+def test_fused_transfer_prepare_bit_identical(flip2, oracle, solid):
+    """The step's fused transfer + prepare (scatter only in bricks within one
+    brick of a particle, prepared zero faces elsewhere) gives the faces,
+    saved faces and sorted records of the transfer stage followed by the
+    prepare stage, bit for bit, on a 64^3 grid whose particles leave most
+    bricks untouched, with a moving solid box or a static solid mask."""
+    n = 64
+    h = 1.0 / n
+    rng = np.random.default_rng(8)
+    blob = rng.random((20000, 3)) * np.array([0.3, 0.25, 0.3]) + np.array([0.1, 0.0, 0.4])
+    edge = rng.random((500, 3)) * np.array([1.0, 0.02, 0.02]) + np.array([0.0, 0.97, 0.97])  # along the top corner
+    pos = np.concatenate([blob, edge]).astype(np.float32)
+    x, y, z = (np.ascontiguousarray(pos[:, d]) for d in range(3))
+    v = [(rng.standard_normal(x.size) * 0.5).astype(np.float32) for _ in range(3)]
+    grid = (0.0, 0.0, 0.0, n, n, n, h)
+    names = ("U", "V", "W", "U0", "V0", "W0")
+    out = []
+    for fused in (False, True):
+        sim = flip2.FluidSim(x.size, grid)
+        sim.set_particles(x, y, z, *v)
+        if solid == "box":
+            sim.set_solid_box((0.0, 0.0, 0.0), (0.25, 0.5, 1.0), (0.5, 0.0, 0.25))
+        elif solid == "static":
+            m = np.zeros((n, n, n), bool)
+            m[20:40, 0:6, 10:50] = True
+            m[60:, :, :] = True
+            from tests.fluid_checks import tiled_mask
+            sim.set_solid(tiled_mask(m.transpose(2, 1, 0), grid, oracle))
+        if fused:
+            sim.stage("transfer_prepare")
+        else:
+            sim.stage("transfer")
+            sim.stage("prepare")
+        out.append([sim.get(k) for k in names] + list(sim.particles()))
+    for name, a, b in zip(names + ("x", "y", "z", "vx", "vy", "vz"), *out):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+
+
 @pytest.mark.parametrize("box", [False, True])
 def test_fused_g2p_advect_bit_identical(flip2, box):
     """The step's fused G2P + RK2 kernel gives the records of the g2p stage
