@@ -603,7 +603,8 @@ class FluidSim:
         check(lib().flip2_fluid_disable_timing(self._h))
 
     def timing(self) -> dict:
-        """{kernel: (total ms, launches)} since enable_timing."""
+        """{kernel: (total ms, launches)} since enable_timing; the instances of
+        a kernel template (name<...>) are counted under the kernel's name."""
         out, k = {}, 0
         while True:
             name = ctypes.create_string_buffer(128)
@@ -611,7 +612,9 @@ class FluidSim:
             rc = lib().flip2_fluid_timing(self._h, k, name, 128, ctypes.byref(ms), ctypes.byref(cnt))
             if rc != 0:
                 break
-            out[name.value.decode()] = (ms.value, int(cnt.value))
+            key = name.value.decode().split("<", 1)[0]
+            prev = out.get(key, (0.0, 0))
+            out[key] = (prev[0] + ms.value, prev[1] + int(cnt.value))
             k += 1
         return out
 

@@ -501,6 +501,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             cg.z = z_.devPtr();
             cg.na = na;
             cg.partials = partials_.devPtr();
+            FLIP2_REQUIRE(ns::cg_partials_needed(na) <= partials_.size(), kErrState, "PCG partials buffer too small");
             cg.hist = H;
             float* pbuf[2] = {p_.devPtr(), p2_.devPtr()};
             ns::launch_cg_update_precond(cg, 0, true, !jac, r64_.devPtr(), nullptr, stream_);
@@ -516,6 +517,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                     ns::launch_cg_update_precond(cg, it, false, !jac, nullptr, pNew, stream_);
                     if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
                 }
+                ns::launch_cg_history(cg, it, stream_);  // entry `it`: the batch's last residual
                 innerDone = converged(first, batch);
             }
             total += it;

@@ -668,40 +668,112 @@ mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon)
 }
 
 // ---------------------------------------------------------------------------
-// PCG with two main kernels per iteration (Jacobi and brick-local MIC(0)).
+// PCG with two kernels per iteration (Jacobi and brick-local MIC(0)).
 //
-//   direction+matvec (iteration k):  beta = rz_k / rz_{k-1};  p = z + beta p';
-//                                    q = A p;  partial p.q
-//   update+precondition:             alpha = rz_k / (p.q)_k;  e += alpha p;
+//   direction+matvec (iteration k):  rz_k, rr_k = sums of the update's
+//                                    partials;  beta = rz_k / rz_{k-1};
+//                                    p = z + beta p';  q = A p;  partial p.q
+//   update+precondition:             (p.q)_k = sum of the matvec's partials;
+//                                    alpha = rz_k / (p.q)_k;  e += alpha p;
 //                                    r -= alpha q;  z = M^-1 r;  partials r.z, r.r
-// One wave per slot (the MIC(0) update's forward and backward plane sweeps
-// run in LDS), so every slot's memory latency is in flight at once; each main
-// kernel is followed by a one-workgroup kernel that sums its partials in
-// slot order into the iteration history (rz, rr, pq per iteration:
-// deterministic), which the next kernel reads.  p alternates between two
-// buffers: the matvec reads the previous direction at neighbour cells owned
-// by other workgroups while writing the new one.
+//
+// A workgroup is four waves, one slot each (the MIC(0) sweeps run in the
+// wave's own LDS), so every slot's memory latency is in flight at once.
+// Each kernel starts by summing the previous kernel's per-workgroup partials
+// itself (group_sum: the same fixed order in every workgroup, so every
+// workgroup -- and the end-of-batch history kernel -- gets the same double);
+// workgroup 0 records the sums in the iteration history (rz, rr, pq per
+// iteration) for the next iterations and the host's convergence checks.  No
+// reduction kernel runs between the two.  p alternates between two buffers:
+// the matvec reads the previous direction at neighbour cells owned by other
+// workgroups while writing the new one.
 // ---------------------------------------------------------------------------
+namespace {
 
-// p = z + beta p_old (p = z when first), q = A p; partial p.q per slot.
-// One wave per slot, lane = column (i, j) of the brick holding the eight
-// cells k = 0..7 (cell l = 64 k + lane, so every load is one coalesced row
-// per k).  All inputs -- the column's coefficients, z and p_old, and the same
-// for the cells of the neighbouring slots it touches -- are loaded up front;
-// the new p goes to LDS for the x and y neighbours, z neighbours are the
-// lane's own registers.
-__global__ void __launch_bounds__(64)
+// Slots (waves) per workgroup: 4 below kCgGroupedBelow slots, where the
+// kernels sum the partials themselves and launches dominate; 1 above, with a
+// one-workgroup reduction kernel after each main kernel (more workgroups in
+// flight, no partial reads at each workgroup's start).
+constexpr uint32_t kCgGroupedBelow = 4096;
+
+// Workgroup index in the XCD-aware order over ng groups (see xcd_slot).
+__device__ inline uint32_t cg_group(uint32_t ng) { return xcd_slot(ng); }
+
+// Fixed-order sums of a[0..n) and b[0..n) (b may be null) by the 256
+// threads of a workgroup: thread t adds a[t + 256 j] in j order (loads issued
+// four at a time before the adds), then the waves' butterfly sums, then the
+// four wave sums in order.  Every workgroup computes the same doubles.
+__device__ inline void group_sum(const double* __restrict__ a, const double* __restrict__ b, uint32_t n,
+                                 double* s_red, double& sa, double& sb) {
+    const uint32_t tid = threadIdx.x;
+    double x = 0.0, y = 0.0;
+    for (uint32_t i0 = tid; i0 < n; i0 += 4u * 256u) {
+        double va[4], vb[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t idx = i0 + 256u * (uint32_t)k;
+            va[k] = idx < n ? a[idx] : 0.0;
+            vb[k] = (b && idx < n) ? b[idx] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x += va[k];
+            y += vb[k];
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        x += __shfl_xor(x, d, 64);
+        y += __shfl_xor(y, d, 64);
+    }
+    if ((tid & 63u) == 0) {
+        s_red[tid >> 6] = x;
+        s_red[4 + (tid >> 6)] = y;
+    }
+    __syncthreads();
+    sa = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
+    sb = ((s_red[4] + s_red[5]) + s_red[6]) + s_red[7];
+    __syncthreads();
+}
+
+// The workgroup's partial: its W waves' values in wave order.
+template <uint32_t W>
+__device__ inline void group_partial(double mine, double* s_red, double* out) {
+    if constexpr (W == 1) {
+        if (threadIdx.x == 0) *out = mine;
+    } else {
+        if ((threadIdx.x & 63u) == 0) s_red[threadIdx.x >> 6] = mine;
+        __syncthreads();
+        if (threadIdx.x == 0) *out = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
+    }
+}
+
+}  // namespace
+
+// Iteration `it`: rz_it, rr_it from the update's partials (rz at
+// partials[na_g..], rr at partials[2 na_g..], na_g = number of groups);
+// p = z + beta p_old (p = z when first), q = A p; partial p.q per group at
+// partials[0..).  Wave = slot, lane = column (i, j) of the brick holding the
+// eight cells k = 0..7 (cell l = 64 k + lane, so every load is one coalesced
+// row per k).  All inputs -- the column's coefficients, z and p_old, and the
+// same for the cells of the neighbouring slots it touches -- are loaded
+// before the partial sums are read; the new p goes to LDS for the x and y
+// neighbours, z neighbours are the lane's own registers.
+template <uint32_t W>
+__global__ void __launch_bounds__(64 * W)
 cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __restrict__ pOld,
                            float* __restrict__ pNew) {
-    __shared__ float s_p[512];
-    const uint32_t lane = threadIdx.x, slot = xcd_slot(c.na), base = slot * 512u;
-    if (slot >= c.na) return;
+    __shared__ float s_pw[W][512];
+    __shared__ double s_red[8];
+    const uint32_t ng = (c.na + W - 1u) / W;
+    const uint32_t grp = cg_group(ng);
+    if (grp >= ng) return;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t slotRaw = grp * W + wave;
+    const bool live = slotRaw < c.na;
+    const uint32_t slot = live ? slotRaw : c.na - 1u;  // a padding wave mirrors the last slot and stores nothing
+    const uint32_t base = slot * 512u;
+    float* s_p = s_pw[wave];
     const uint32_t i = lane & 7u, j = lane >> 3;
-    float beta = 0.f;
-    if (!first) {
-        const double rz = c.hist[4u * it], rzOld = c.hist[4u * (it - 1u)];
-        beta = rzOld != 0.0 ? (float)(rz / rzOld) : 0.f;
-    }
     int32_t nb[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) nb[d] = c.nbrSlot[slot * 6u + d];
@@ -740,6 +812,22 @@ cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __res
         zhi = c.z[w];
         phi = first ? 0.f : pOld[w];
     }
+    double rz;
+    if constexpr (W > 1) {
+        double rr;
+        group_sum(c.partials + ng, c.partials + 2u * ng, ng, s_red, rz, rr);
+        if (grp == 0 && threadIdx.x == 0) {
+            c.hist[4u * it] = rz;
+            c.hist[4u * it + 1u] = rr;
+        }
+    } else {
+        rz = c.hist[4u * it];  // recorded by the reduction after the update
+    }
+    float beta = 0.f;
+    if (!first) {
+        const double rzOld = c.hist[4u * (it - 1u)];
+        beta = rzOld != 0.0 ? (float)(rz / rzOld) : 0.f;
+    }
     auto dir = [&](float zz, float pp) { return first ? zz : fmaf(beta, pp, zz); };
     float pv[8];
 #pragma unroll
@@ -747,7 +835,7 @@ cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __res
         pv[k] = cf[k] ? dir(zv[k], po[k]) : 0.f;
         s_p[64u * (uint32_t)k + lane] = pv[k];
     }
-    __syncthreads();
+    wave_lds_sync();
     double pq = 0.0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -765,93 +853,83 @@ cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __res
             if (mask & 32u) sum -= k < 7 ? pv[k < 7 ? k + 1 : 7] : dir(zhi, phi);
             qv = sum;
         }
-        pNew[v] = pv[k];
-        c.q[v] = qv;
+        if (live) {
+            pNew[v] = pv[k];
+            c.q[v] = qv;
+        }
         pq += (double)pv[k] * (double)qv;
     }
     for (int d = 32; d >= 1; d >>= 1) pq += __shfl_xor(pq, d, 64);
-    if (lane == 0) c.partials[slot] = pq;
+    group_partial<W>(live ? pq : 0.0, s_red, c.partials + grp);
 }
 
 namespace {
 // e, r update of the eight cells base + lane + 64 k of a slot: e += alpha p,
 // r -= alpha q (init: r = (float) r64, e = 0); cells outside the fluid keep
 // e = r = 0.  Every load is issued before the first use (no load waits on a
-// branch), and e, r are stored for every cell.  cf, rk: the cells'
-// coefficients and new r.
-__device__ inline void cg_update_cells(const CgFused& c, uint32_t base, uint32_t lane, int init, float alpha,
-                                       const double* __restrict__ r64, const float* __restrict__ p, uint16_t cf[8],
-                                       float rk[8]) {
+// branch), and e, r are stored for every cell of a live slot.  cf, rk: the
+// cells' coefficients and new r.  The loads come first (cg_load_cells), then
+// the step length is formed from the partial sums, then cg_store_cells.
+struct CellLoads {
+    uint16_t cf[8];
     float pv[8], ev[8], qv[8], rv[8];
     double r0[8];
+};
+__device__ inline void cg_load_cells(const CgFused& c, uint32_t base, uint32_t lane, int init,
+                                     const double* __restrict__ r64, const float* __restrict__ p, CellLoads& L) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const uint32_t v = base + 64u * (uint32_t)k + lane;
-        cf[k] = c.coef[v];
+        L.cf[k] = c.coef[v];
         if (init) {
-            r0[k] = r64[v];
+            L.r0[k] = r64[v];
         } else {
-            pv[k] = p[v];
-            ev[k] = c.e[v];
-            qv[k] = c.q[v];
-            rv[k] = c.r[v];
+            L.pv[k] = p[v];
+            L.ev[k] = c.e[v];
+            L.qv[k] = c.q[v];
+            L.rv[k] = c.r[v];
         }
     }
+}
+__device__ inline void cg_store_cells(const CgFused& c, uint32_t base, uint32_t lane, int init, float alpha, bool live,
+                                      const CellLoads& L, float rk[8]) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const uint32_t v = base + 64u * (uint32_t)k + lane;
         float rn, en;
         if (init) {
-            rn = (float)r0[k];
+            rn = (float)L.r0[k];
             en = 0.f;
         } else {
-            en = fmaf(alpha, pv[k], ev[k]);
-            rn = fmaf(-alpha, qv[k], rv[k]);
+            en = fmaf(alpha, L.pv[k], L.ev[k]);
+            rn = fmaf(-alpha, L.qv[k], L.rv[k]);
         }
-        if (!cf[k]) rn = en = 0.f;
-        c.e[v] = en;
-        c.r[v] = rn;
+        if (!L.cf[k]) rn = en = 0.f;
+        if (live) {
+            c.e[v] = en;
+            c.r[v] = rn;
+        }
         rk[k] = rn;
     }
 }
 
-__device__ inline float cg_alpha(const CgFused& c, uint32_t it, int init) {
+// alpha of iteration it from (p.q)_it and rz_it (recorded by the matvec
+// kernel or the reduction after the update); 0 for the refinement's start.
+// With W > 1 the workgroup sums (p.q)_it itself and workgroup 0 records it.
+template <uint32_t W>
+__device__ inline float cg_alpha_from(const CgFused& c, uint32_t it, int init, uint32_t ng, uint32_t grp,
+                                      double* s_red) {
     if (init) return 0.f;
-    const double pq = c.hist[4u * it + 2u], rz = c.hist[4u * it];
+    double pq;
+    if constexpr (W > 1) {
+        double unused;
+        group_sum(c.partials, nullptr, ng, s_red, pq, unused);
+        if (grp == 0 && threadIdx.x == 0) c.hist[4u * it + 2u] = pq;
+    } else {
+        pq = c.hist[4u * it + 2u];  // recorded by the reduction after the matvec
+    }
+    const double rz = c.hist[4u * it];
     return pq != 0.0 ? (float)(rz / pq) : 0.f;
-}
-}  // namespace
-
-// Jacobi: e += alpha p, r -= alpha q (init: r = (float) r64, e = 0),
-// z = r / diag; partials r.z and r.r per slot.  One wave per slot, each lane
-// the eight cells lane + 64 k.
-__global__ void __launch_bounds__(64)
-cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64,
-                        const float* __restrict__ p) {
-    const float alpha = cg_alpha(c, it, init);
-    const uint32_t lane = threadIdx.x, slot = xcd_slot(c.na);
-    if (slot >= c.na) return;
-    double rz = 0.0, rr = 0.0;
-    uint16_t cf[8];
-    float rk[8];
-    cg_update_cells(c, slot * 512u, lane, init, alpha, r64, p, cf, rk);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t v = slot * 512u + 64u * (uint32_t)k + lane;
-        const float rv = rk[k];
-        const float zv = cf[k] ? rv / (float)(cf[k] & 7u) : 0.f;
-        c.z[v] = zv;
-        rr += (double)rv * (double)rv;
-        rz += (double)rv * (double)zv;
-    }
-    for (int d = 32; d >= 1; d >>= 1) {
-        rz += __shfl_xor(rz, d, 64);
-        rr += __shfl_xor(rr, d, 64);
-    }
-    if (lane == 0) {
-        c.partials[slot] = rz;
-        c.partials[c.na + slot] = rr;
-    }
 }
 
 // v where bit `bit` of w is set, else +0 (bit for bit: an AND with 0 or ~0)
@@ -859,36 +937,97 @@ __device__ inline float keep_if_bit(float v, uint32_t w, int bit) {
     const int m = (int)(w << (31 - bit)) >> 31;
     return __int_as_float(__float_as_int(v) & m);
 }
+}  // namespace
+
+// Jacobi: e += alpha p, r -= alpha q (init: r = (float) r64, e = 0),
+// z = r / diag; partials r.z and r.r per group.  Wave = slot, each lane the
+// eight cells lane + 64 k.
+template <uint32_t W>
+__global__ void __launch_bounds__(64 * W)
+cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64,
+                        const float* __restrict__ p) {
+    __shared__ double s_red[8];
+    const uint32_t ng = (c.na + W - 1u) / W;
+    const uint32_t grp = cg_group(ng);
+    if (grp >= ng) return;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t slotRaw = grp * W + wave;
+    const bool live = slotRaw < c.na;
+    const uint32_t slot = live ? slotRaw : c.na - 1u;
+    CellLoads L;
+    cg_load_cells(c, slot * 512u, lane, init, r64, p, L);
+    const float alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
+    float rk[8];
+    cg_store_cells(c, slot * 512u, lane, init, alpha, live, L, rk);
+    double rz = 0.0, rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t v = slot * 512u + 64u * (uint32_t)k + lane;
+        const float rv = rk[k];
+        const float zv = L.cf[k] ? rv / (float)(L.cf[k] & 7u) : 0.f;
+        if (live) c.z[v] = zv;
+        rr += (double)rv * (double)rv;
+        rz += (double)rv * (double)zv;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        rz += __shfl_xor(rz, d, 64);
+        rr += __shfl_xor(rr, d, 64);
+    }
+    group_partial<W>(live ? rz : 0.0, s_red, c.partials + ng + grp);
+    group_partial<W>(live ? rr : 0.0, s_red + 4, c.partials + 2u * ng + grp);
+}
 
 // MIC(0): the same update, then z = M^-1 r by the brick's forward and
-// backward sweeps over its 22 anti-diagonal planes, all in LDS; one wave
-// per slot.  LDS holds (value, precondition factor) and the coefficient
-// word of cell c at entry 65 + c, with 65 zero entries below and 64 above,
-// so a cell's six neighbours are at fixed offsets (-1, -8, -64, +1, +8, +64)
-// and every read of a plane is issued at once, without a branch: a lane
-// with no cell on the plane works on the zero entry 64.  The coefficient
-// words keep only couplings inside the brick, so a neighbour offset that
-// wraps to another row or plane meets a cleared bit.  The coupling
-// A(c, m) = -1 of fluid neighbours is the matching bit of c's own word; the
-// terms (A pc_m) q_m and (A pc_c) z_m are formed as -(pc q) (negation is
-// exact), i.e. added where the bit is set.  The wave's LDS operations
-// execute in order, so a plane's reads see the previous plane's writes
-// without waiting for them (wave_lds_sync).
-__global__ void __launch_bounds__(64)
+// backward sweeps over its 22 anti-diagonal planes, all in the wave's LDS.
+// LDS holds (value, precondition factor) and the coefficient word of cell c
+// at entry 65 + c, with 65 zero entries below and 64 above, so a cell's six
+// neighbours are at fixed offsets (-1, -8, -64, +1, +8, +64) and every read
+// of a plane is issued at once, without a branch: a lane with no cell on the
+// plane works on the zero entry 64.  The coefficient words keep only
+// couplings inside the brick, so a neighbour offset that wraps to another
+// row or plane meets a cleared bit.  The coupling A(c, m) = -1 of fluid
+// neighbours is the matching bit of c's own word; the terms (A pc_m) q_m and
+// (A pc_c) z_m are formed as -(pc q) (negation is exact), i.e. added where
+// the bit is set.  The wave's LDS operations execute in order, so a plane's
+// reads see the previous plane's writes without waiting for them
+// (wave_lds_sync).
+template <uint32_t W>
+__global__ void __launch_bounds__(64 * W)
 cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64, const float* __restrict__ p) {
     constexpr uint32_t kPad = 65, kEntries = 65 + 512 + 64;
-    __shared__ float2 s_a[kEntries];
-    __shared__ uint16_t s_c[kEntries];
-    const float alpha = cg_alpha(c, it, init);
-    const uint32_t lane = threadIdx.x, slot = xcd_slot(c.na), base = slot * 512u;
-    if (slot >= c.na) return;
-    double rz = 0.0, rr = 0.0;
+    __shared__ float2 s_aw[W][kEntries];
+    __shared__ uint16_t s_cw[W][kEntries];
+    __shared__ double s_red[8];
+    const uint32_t ng = (c.na + W - 1u) / W;
+    const uint32_t grp = cg_group(ng);
+    if (grp >= ng) return;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t slotRaw = grp * W + wave;
+    const bool live = slotRaw < c.na;
+    const uint32_t slot = live ? slotRaw : c.na - 1u;
+    const uint32_t base = slot * 512u;
+    float2* s_a = s_aw[wave];
+    uint16_t* s_c = s_cw[wave];
+    // this lane's cell of each plane (two per word), fetched once so the
+    // sweeps wait only on LDS
+    uint32_t cellOf2[11];
+#pragma unroll
+    for (int s = 0; s < 11; ++s)
+        cellOf2[s] = (uint32_t)c_planes.lane_cell[2 * s][lane] | ((uint32_t)c_planes.lane_cell[2 * s + 1][lane] << 16);
+    auto cellOf = [&](int s) { return (s & 1) ? cellOf2[s >> 1] >> 16 : cellOf2[s >> 1] & 0xffffu; };
+    // W > 1: the partial sum first (fewer registers live across it); W == 1:
+    // the slot's loads first (the step length is two history words)
+    float alpha = 0.f;
+    if constexpr (W > 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
     float pc[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) pc[k] = c.precon[base + lane + 64u * (uint32_t)k];
-    uint16_t cf[8];
+    CellLoads L;
+    cg_load_cells(c, base, lane, init, r64, p, L);
+    if constexpr (W == 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
     float rk[8];  // this lane's r values (cells lane + 64 k)
-    cg_update_cells(c, base, lane, init, alpha, r64, p, cf, rk);
+    cg_store_cells(c, base, lane, init, alpha, live, L, rk);
+    double rz = 0.0, rr = 0.0;
     for (uint32_t e = lane; e < kPad; e += 64u) {
         s_a[e] = make_float2(0.f, 0.f);
         s_c[e] = 0;
@@ -905,17 +1044,13 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
         const float rv = rk[k];
         rr += (double)rv * (double)rv;
         const uint32_t keep = keepXY & ~((k == 0 ? 1u << 7 : 0u) | (k == 7 ? 1u << 8 : 0u));
-        s_c[kPad + l] = (uint16_t)(cf[k] & keep);
+        s_c[kPad + l] = (uint16_t)(L.cf[k] & keep);
         s_a[kPad + l] = make_float2(rv, pc[k]);  // r, replaced by the forward sweep's q plane by plane
     }
-    // this lane's cell of each plane, fetched once so the sweeps wait only on LDS
-    uint32_t cellOf[22];
-#pragma unroll
-    for (int s = 0; s < 22; ++s) cellOf[s] = c_planes.lane_cell[s][lane];
     wave_lds_sync();
 #pragma unroll
     for (int s = 0; s < 22; ++s) {
-        const uint32_t b = cellOf[s];  // own entry b + 64
+        const uint32_t b = cellOf(s);  // own entry b + 64
         const float2 own = s_a[b + 64u], mx = s_a[b + 63u], my = s_a[b + 56u], mz = s_a[b];
         const uint32_t w = s_c[b + 64u];
         float acc = own.x;
@@ -927,7 +1062,7 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
     }
 #pragma unroll
     for (int s = 21; s >= 0; --s) {
-        const uint32_t b = cellOf[s];
+        const uint32_t b = cellOf(s);
         const float2 own = s_a[b + 64u];
         const float ux = s_a[b + 65u].x, uy = s_a[b + 72u].x, uz = s_a[b + 128u].x;
         const uint32_t w = s_c[b + 64u];
@@ -942,16 +1077,27 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
     for (int k = 0; k < 8; ++k) {
         const uint32_t l = lane + 64u * (uint32_t)k;
         const float zv = s_a[kPad + l].x;
-        c.z[base + l] = zv;
+        if (live) c.z[base + l] = zv;
         rz += (double)rk[k] * (double)zv;
     }
     for (int d = 32; d >= 1; d >>= 1) {
         rz += __shfl_xor(rz, d, 64);
         rr += __shfl_xor(rr, d, 64);
     }
-    if (lane == 0) {
-        c.partials[slot] = rz;
-        c.partials[c.na + slot] = rr;
+    group_partial<W>(live ? rz : 0.0, s_red, c.partials + ng + grp);
+    group_partial<W>(live ? rr : 0.0, s_red + 4, c.partials + 2u * ng + grp);
+}
+
+// hist[4 it], hist[4 it + 1] = rz, rr from the update's partials (the sums
+// the next matvec kernel forms), for the host's check at the end of a batch.
+__global__ void __launch_bounds__(256) cg_history_kernel(CgFused c, uint32_t it) {
+    __shared__ double s_red[8];
+    const uint32_t ng = (c.na + 3u) / 4u;  // the grouped kernels' workgroups
+    double rz, rr;
+    group_sum(c.partials + ng, c.partials + 2u * ng, ng, s_red, rz, rr);
+    if (threadIdx.x == 0) {
+        c.hist[4u * it] = rz;
+        c.hist[4u * it + 1u] = rr;
     }
 }
 
@@ -1393,20 +1539,46 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
     FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
     FLIP2_LAUNCH_CHECK();
 }
+uint32_t cg_partials_needed(uint32_t na) { return 3u * (na < kCgGroupedBelow ? (na + 3u) / 4u : na); }
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s) {
-    FLIP2_KERNEL(cg_direction_matvec_kernel, xcd_grid(c.na), dim3(64), 0, s, c, it, first ? 1 : 0, pOld, pNew);
-    FLIP2_LAUNCH_CHECK();
-    reduce_to(Reduce{c.partials}, 1, c.na, c.hist + 4u * it + 2u, s);  // (p.q)_it
+    FLIP2_REQUIRE(c.na > 0, kErrArgument, "PCG: no active slots");
+    if (c.na < kCgGroupedBelow) {
+        FLIP2_KERNEL(cg_direction_matvec_kernel<4>, xcd_grid((c.na + 3u) / 4u), dim3(256), 0, s, c, it, first ? 1 : 0,
+                     pOld, pNew);
+        FLIP2_LAUNCH_CHECK();
+    } else {
+        FLIP2_KERNEL(cg_direction_matvec_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, first ? 1 : 0, pOld,
+                     pNew);
+        FLIP2_LAUNCH_CHECK();
+        reduce_to(Reduce{c.partials}, 1, c.na, c.hist + 4u * it + 2u, s);  // (p.q)_it
+    }
 }
 void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
                               hipStream_t s) {
-    if (mic)
-        FLIP2_KERNEL(cg_update_mic_kernel, xcd_grid(c.na), dim3(64), 0, s, c, it, init ? 1 : 0, r64, p);
-    else
-        FLIP2_KERNEL(cg_update_jacobi_kernel, xcd_grid(c.na), dim3(64), 0, s, c, it, init ? 1 : 0, r64, p);
+    FLIP2_REQUIRE(c.na > 0, kErrArgument, "PCG: no active slots");
+    const int in = init ? 1 : 0;
+    if (c.na < kCgGroupedBelow) {
+        const dim3 grid = xcd_grid((c.na + 3u) / 4u);
+        if (mic)
+            FLIP2_KERNEL(cg_update_mic_kernel<4>, grid, dim3(256), 0, s, c, it, in, r64, p);
+        else
+            FLIP2_KERNEL(cg_update_jacobi_kernel<4>, grid, dim3(256), 0, s, c, it, in, r64, p);
+        FLIP2_LAUNCH_CHECK();
+    } else {
+        if (mic)
+            FLIP2_KERNEL(cg_update_mic_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, in, r64, p);
+        else
+            FLIP2_KERNEL(cg_update_jacobi_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, in, r64, p);
+        FLIP2_LAUNCH_CHECK();
+        // rz, rr of the next entry (partials at [na, 3 na))
+        reduce_to(Reduce{c.partials + c.na}, 2, c.na, c.hist + 4u * (init ? it : it + 1u), s);
+    }
+}
+void launch_cg_history(const CgFused& c, uint32_t it, hipStream_t s) {
+    if (c.na >= kCgGroupedBelow) return;  // the reduction after each update recorded it
+    FLIP2_KERNEL(cg_history_kernel, dim3(1), dim3(256), 0, s, c, it);
     FLIP2_LAUNCH_CHECK();
-    reduce_to(Reduce{c.partials}, 2, c.na, c.hist + 4u * (init ? it : it + 1u), s);  // rz, rr of the next entry
 }
 void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,

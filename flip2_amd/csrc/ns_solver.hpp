@@ -67,20 +67,27 @@ struct CgFused {
     float* e;
     float* r;
     float* z;
-    double* partials;  // 2 na doubles
+    double* partials;  // cg_partials_needed(na) doubles
     double* hist;
     uint32_t na;
 };
-// Iteration `it`: p = z (first) or z + (rz_it / rz_{it-1}) pOld into pNew;
-// q = A pNew; history (p.q)_it.
+// Doubles of CgFused::partials for na slots.
+uint32_t cg_partials_needed(uint32_t na);
+// Iteration `it`: history rz_it, rr_it (entry it) from the previous update's
+// partials; p = z (first) or z + (rz_it / rz_{it-1}) pOld into pNew;
+// q = A p; partials of p.q.
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s);
-// Iteration `it`: alpha = rz_it / (p.q)_it; e += alpha p; r -= alpha q;
-// z = M^-1 r; history rz, rr of entry it + 1.  init: instead r = (float) r64
-// (0 off the fluid), e = 0 (p unused), history entry `it` -- the start of a
-// refinement.
+// Iteration `it`: history (p.q)_it from the matvec's partials;
+// alpha = rz_it / (p.q)_it; e += alpha p; r -= alpha q; z = M^-1 r;
+// partials of rz, rr (history entry it + 1, recorded by the next matvec or
+// launch_cg_history).  init: instead r = (float) r64 (0 off the fluid),
+// e = 0 (p unused), partials of entry `it` -- the start of a refinement.
 void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
                               hipStream_t s);
+// history rz, rr of entry `it` from the last update's partials (the values
+// the next matvec would record), for a check at the end of a batch
+void launch_cg_history(const CgFused& c, uint32_t it, hipStream_t s);
 void launch_accumulate(uint32_t numActive, const float* e, double* x64, const uint16_t* coef, hipStream_t s);
 void launch_true_residual(uint32_t numActive, const double* b64, const double* x64, const uint16_t* coef,
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s);
