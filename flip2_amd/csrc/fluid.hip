@@ -61,6 +61,7 @@ FluidSim::FluidSim(uint32_t maxParticles, const FluidConfig& cfg, int device)
     activeBricks_.resizeAsync(numBricks_, stream_);
     total_.resizeAsync(4, stream_);
     scal_.resizeAsync(16, stream_);
+    stop_.resizeAsync(1, stream_);
     hist_.resizeAsync((uint32_t)history_entries(cfg), stream_);
     ws_.resizeAsync((uint32_t)scan_workspace_bytes(numBricks_ + 1), stream_);
     synchronize();
@@ -73,6 +74,26 @@ FluidSim::~FluidSim() {
         dm_.setStream(nullptr);  // drained: the arrays' releases need no further waits
         (void)hipStreamDestroy(stream_);
     }
+    for (auto& e : histEvent_)
+        if (e) (void)hipEventDestroy(e);
+    if (histPinned_) (void)hipHostFree(histPinned_);
+}
+
+// Pinned host buffer of at least `doubles` doubles (kept between solves).
+double* FluidSim::pinnedHistory(size_t doubles) {
+    if (doubles > histPinnedDoubles_) {
+        if (histPinned_) {
+            synchronize();  // no copy into the old buffer is pending
+            FLIP2_HIP(hipHostFree(histPinned_));
+            histPinned_ = nullptr;
+            histPinnedDoubles_ = 0;
+        }
+        FLIP2_HIP(hipHostMalloc((void**)&histPinned_, doubles * sizeof(double), hipHostMallocDefault));
+        histPinnedDoubles_ = doubles;
+    }
+    for (auto& e : histEvent_)
+        if (!e) FLIP2_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return histPinned_;
 }
 
 void FluidSim::synchronize() { FLIP2_HIP(hipStreamSynchronize(stream_)); }
@@ -503,24 +524,66 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             cg.partials = partials_.devPtr();
             FLIP2_REQUIRE(ns::cg_partials_needed(na) <= partials_.size(), kErrState, "PCG partials buffer too small");
             cg.hist = H;
+            cg.stop = stop_.devPtr();
+            const double thr = innerTol * ref;
+            cg.thr2 = thr * thr;
             float* pbuf[2] = {p_.devPtr(), p2_.devPtr()};
             ns::launch_cg_update_precond(cg, 0, true, !jac, r64_.devPtr(), nullptr, stream_);
             if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
-            uint32_t it = 0;
-            bool innerDone = false;
-            while (!innerDone && total + it < cap) {
-                const uint32_t batch = std::min(every, cap - total - it);
-                const uint32_t first = it;
-                for (uint32_t k = 0; k < batch; ++k, ++it) {
+            // Batches of `every` iterations are queued one batch ahead of the
+            // host's check of the previous one (read from pinned memory after
+            // its event), so the GPU does not wait for the host; the first
+            // entry with rr <= thr2 stops the refinement on the device, and
+            // the kernels queued after it return at once.
+            const uint32_t limit = cap - total;
+            const size_t stride = 4u * ((size_t)every + 1u);
+            double* pinned = pinnedHistory(2 * stride);
+            struct Batch {
+                uint32_t first, count;
+                int slot;
+            };
+            uint32_t queued = 0;
+            auto enqueue = [&](int slot) {
+                const Batch b{queued, std::min(every, limit - queued), slot};
+                for (uint32_t k = 0; k < b.count; ++k, ++queued) {
+                    const uint32_t it = queued;
                     float* pNew = pbuf[it & 1u];
                     ns::launch_cg_direction_matvec(cg, it, it == 0, pbuf[(it + 1u) & 1u], pNew, stream_);
                     ns::launch_cg_update_precond(cg, it, false, !jac, nullptr, pNew, stream_);
                     if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
                 }
-                ns::launch_cg_history(cg, it, stream_);  // entry `it`: the batch's last residual
-                innerDone = converged(first, batch);
+                ns::launch_cg_history(cg, queued, stream_);  // the batch's last entry
+                // entries first .. first + count (the first one again: entry 0
+                // of the refinement is checked with the first batch)
+                FLIP2_HIP(hipMemcpyAsync(pinned + slot * stride, H + 4u * b.first, 4u * (b.count + 1u) * sizeof(double),
+                                         hipMemcpyDeviceToHost, stream_));
+                FLIP2_HIP(hipEventRecord(histEvent_[slot], stream_));
+                return b;
+            };
+            uint32_t doneAt = UINT32_MAX;  // the converged entry
+            auto convergedIn = [&](const Batch& b) {
+                FLIP2_HIP(hipEventSynchronize(histEvent_[b.slot]));
+                const double* hb = pinned + b.slot * stride;
+                for (uint32_t k = 0; k <= b.count; ++k) {
+                    const double rr = hb[4 * k + 1];
+                    if (rr <= cg.thr2 || !(rr == rr)) {
+                        doneAt = b.first + k;
+                        return true;
+                    }
+                }
+                return false;
+            };
+            if (limit > 0) {
+                Batch cur = enqueue(0);
+                while (true) {
+                    const bool haveNext = queued < limit;
+                    Batch next{};
+                    if (haveNext) next = enqueue(cur.slot ^ 1);
+                    if (convergedIn(cur) || !haveNext) break;  // converged, or the cap
+                    cur = next;
+                }
             }
-            total += it;
+            total += doneAt != UINT32_MAX ? doneAt : queued;
         } else {
             // flexible CG with the learned preconditioner
             ns::launch_refine_start(na, r64_.devPtr(), r_.devPtr(), e_.devPtr(), coef_.devPtr(), reduce(),

@@ -188,6 +188,12 @@ private:
     hipGraph_t learnedGraph_ = nullptr;
     hipGraphExec_t learnedGraphExec_ = nullptr;
     DeviceVec<double> partials_{dm_}, hist_{dm_}, scal_{dm_};
+    DeviceVec<uint32_t> stop_{dm_};  // the fused PCG's convergence flag
+    // pinned host copies of two batches' history entries, and their events
+    double* histPinned_ = nullptr;
+    size_t histPinnedDoubles_ = 0;
+    hipEvent_t histEvent_[2] = {nullptr, nullptr};
+    double* pinnedHistory(size_t doubles);
     DeviceVec<uint32_t> total_{dm_};
     DeviceVec<uint8_t> ws_{dm_};
     uint32_t slotCapacity_ = 0;

@@ -772,6 +772,7 @@ cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __res
     const bool live = slotRaw < c.na;
     const uint32_t slot = live ? slotRaw : c.na - 1u;  // a padding wave mirrors the last slot and stores nothing
     const uint32_t base = slot * 512u;
+    if (*c.stop) return;  // converged earlier in this refinement: no loads either
     float* s_p = s_pw[wave];
     const uint32_t i = lane & 7u, j = lane >> 3;
     int32_t nb[6];
@@ -819,6 +820,10 @@ cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __res
         if (grp == 0 && threadIdx.x == 0) {
             c.hist[4u * it] = rz;
             c.hist[4u * it + 1u] = rr;
+        }
+        if (rr <= c.thr2 || !(rr == rr)) {  // the same decision in every workgroup
+            if (grp == 0 && threadIdx.x == 0) *c.stop = 1u;
+            return;
         }
     } else {
         rz = c.hist[4u * it];  // recorded by the reduction after the update
@@ -913,6 +918,16 @@ __device__ inline void cg_store_cells(const CgFused& c, uint32_t base, uint32_t 
     }
 }
 
+// An update runs unless the refinement has converged; the refinement's
+// start (init) clears the flag instead (workgroup 0).
+__device__ inline bool cg_proceed(const CgFused& c, int init, uint32_t grp) {
+    if (init) {
+        if (grp == 0 && threadIdx.x == 0) *c.stop = 0u;
+        return true;
+    }
+    return *c.stop == 0u;
+}
+
 // alpha of iteration it from (p.q)_it and rz_it (recorded by the matvec
 // kernel or the reduction after the update); 0 for the refinement's start.
 // With W > 1 the workgroup sums (p.q)_it itself and workgroup 0 records it.
@@ -954,6 +969,7 @@ cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restri
     const uint32_t slotRaw = grp * W + wave;
     const bool live = slotRaw < c.na;
     const uint32_t slot = live ? slotRaw : c.na - 1u;
+    if (!cg_proceed(c, init, grp)) return;
     CellLoads L;
     cg_load_cells(c, slot * 512u, lane, init, r64, p, L);
     const float alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
@@ -1017,6 +1033,7 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
     auto cellOf = [&](int s) { return (s & 1) ? cellOf2[s >> 1] >> 16 : cellOf2[s >> 1] & 0xffffu; };
     // W > 1: the partial sum first (fewer registers live across it); W == 1:
     // the slot's loads first (the step length is two history words)
+    if (!cg_proceed(c, init, grp)) return;
     float alpha = 0.f;
     if constexpr (W > 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
     float pc[8];
@@ -1092,12 +1109,44 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
 // the next matvec kernel forms), for the host's check at the end of a batch.
 __global__ void __launch_bounds__(256) cg_history_kernel(CgFused c, uint32_t it) {
     __shared__ double s_red[8];
+    if (*c.stop) return;
     const uint32_t ng = (c.na + 3u) / 4u;  // the grouped kernels' workgroups
     double rz, rr;
     group_sum(c.partials + ng, c.partials + 2u * ng, ng, s_red, rz, rr);
     if (threadIdx.x == 0) {
         c.hist[4u * it] = rz;
         c.hist[4u * it + 1u] = rr;
+        if (rr <= c.thr2 || !(rr == rr)) *c.stop = 1u;
+    }
+}
+
+// The one-slot-per-workgroup kernels' sums (as reduce_partials_kernel),
+// skipped once the refinement has converged; with checkRR, quantity 1 is an
+// entry's rr and sets the flag when it meets the tolerance.
+__global__ void __launch_bounds__(kReduceThreads)
+cg_reduce_kernel(const double* __restrict__ partials, uint32_t blocks, double* __restrict__ out, uint32_t* stop,
+                 double thr2, int checkRR) {
+    __shared__ double s_tmp[kReduceThreads / 64];
+    if (*stop) return;
+    const double* p = partials + (size_t)blockIdx.x * blocks;
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+    for (uint32_t b0 = threadIdx.x; b0 < blocks; b0 += 8u * blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t b = b0 + (uint32_t)k * blockDim.x;
+            v[k] = b < blocks ? p[b] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += v[k];
+    }
+    const double a = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    const double t = block_sum(a, s_tmp);
+    if (threadIdx.x == 0) {
+        out[blockIdx.x] = t;
+        if (checkRR && blockIdx.x == 1 && (t <= thr2 || !(t == t))) *stop = 1u;
     }
 }
 
@@ -1539,10 +1588,13 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
     FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
     FLIP2_LAUNCH_CHECK();
 }
+namespace {
+int cg_reduce_threads(uint32_t blocks) { return blocks <= 2048u ? 256 : kReduceThreads; }
+}  // namespace
 uint32_t cg_partials_needed(uint32_t na) { return 3u * (na < kCgGroupedBelow ? (na + 3u) / 4u : na); }
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s) {
-    FLIP2_REQUIRE(c.na > 0, kErrArgument, "PCG: no active slots");
+    FLIP2_REQUIRE(c.na > 0 && c.stop, kErrArgument, "PCG: no active slots or no convergence flag");
     if (c.na < kCgGroupedBelow) {
         FLIP2_KERNEL(cg_direction_matvec_kernel<4>, xcd_grid((c.na + 3u) / 4u), dim3(256), 0, s, c, it, first ? 1 : 0,
                      pOld, pNew);
@@ -1551,12 +1603,14 @@ void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const
         FLIP2_KERNEL(cg_direction_matvec_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, first ? 1 : 0, pOld,
                      pNew);
         FLIP2_LAUNCH_CHECK();
-        reduce_to(Reduce{c.partials}, 1, c.na, c.hist + 4u * it + 2u, s);  // (p.q)_it
+        FLIP2_KERNEL(cg_reduce_kernel, dim3(1), dim3(cg_reduce_threads(c.na)), 0, s, c.partials, c.na,
+                     c.hist + 4u * it + 2u, c.stop, c.thr2, 0);  // (p.q)_it
+        FLIP2_LAUNCH_CHECK();
     }
 }
 void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
                               hipStream_t s) {
-    FLIP2_REQUIRE(c.na > 0, kErrArgument, "PCG: no active slots");
+    FLIP2_REQUIRE(c.na > 0 && c.stop, kErrArgument, "PCG: no active slots or no convergence flag");
     const int in = init ? 1 : 0;
     if (c.na < kCgGroupedBelow) {
         const dim3 grid = xcd_grid((c.na + 3u) / 4u);
@@ -1572,7 +1626,9 @@ void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic
             FLIP2_KERNEL(cg_update_jacobi_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, in, r64, p);
         FLIP2_LAUNCH_CHECK();
         // rz, rr of the next entry (partials at [na, 3 na))
-        reduce_to(Reduce{c.partials + c.na}, 2, c.na, c.hist + 4u * (init ? it : it + 1u), s);
+        FLIP2_KERNEL(cg_reduce_kernel, dim3(2), dim3(cg_reduce_threads(c.na)), 0, s, c.partials + c.na, c.na,
+                     c.hist + 4u * (init ? it : it + 1u), c.stop, c.thr2, 1);
+        FLIP2_LAUNCH_CHECK();
     }
 }
 void launch_cg_history(const CgFused& c, uint32_t it, hipStream_t s) {

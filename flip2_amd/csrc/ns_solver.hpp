@@ -70,6 +70,12 @@ struct CgFused {
     double* partials;  // cg_partials_needed(na) doubles
     double* hist;
     uint32_t na;
+    // Convergence on the device: *stop is cleared by the refinement's start
+    // and set once an entry's rr <= thr2 (or is not a number); from then on
+    // every kernel of the refinement returns at once, so the host may queue
+    // iterations ahead of its checks.
+    uint32_t* stop;
+    double thr2;
 };
 // Doubles of CgFused::partials for na slots.
 uint32_t cg_partials_needed(uint32_t na);
