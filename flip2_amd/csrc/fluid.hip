@@ -280,6 +280,7 @@ void FluidSim::loadSnapshot(const std::string& path) {
 }
 
 void FluidSim::rebuildSolid() {
+    ++solidVersion_;
     hasSolid_ = hasStatic_ || box_.active;
     if (!hasSolid_) return;
     if (solid_.size() < numCells_) solid_.resizeAsync(numCells_, stream_);
@@ -376,18 +377,26 @@ void FluidSim::readHistory(uint32_t first, uint32_t count, double* host) {
 }
 
 void FluidSim::transfer() {
+    facesValid_ = false;
     pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_);
     rec_.swapDevice(pipe_->sortedVec());
 }
 
 void FluidSim::transferAndPrepare() {
+    const PrepKey key{cfg_.gravity * cfg_.dt, box_, hasSolid_, solidVersion_};
+    const bool same = facesValid_ && key.gdt == prepKey_.gdt && key.hasSolid == prepKey_.hasSolid &&
+                      key.solidVersion == prepKey_.solidVersion &&
+                      std::memcmp(&key.box, &prepKey_.box, sizeof(key.box)) == 0;
     const ns::FacePrep prep{U0_.devPtr(), V0_.devPtr(), W0_.devPtr(), hasSolid_ ? solid_.devPtr() : nullptr, box_,
-                            cfg_.gravity * cfg_.dt};
+                            key.gdt, same ? 0 : 1};
     pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_, &prep);
     rec_.swapDevice(pipe_->sortedVec());
+    facesValid_ = true;
+    prepKey_ = key;
 }
 
 void FluidSim::prepareFaces() {
+    facesValid_ = false;
     ns::launch_prepare_faces(U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
                              hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.gravity * cfg_.dt, box_, stream_);
 }

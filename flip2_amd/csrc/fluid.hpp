@@ -189,6 +189,19 @@ private:
     hipGraphExec_t learnedGraphExec_ = nullptr;
     DeviceVec<double> partials_{dm_}, hist_{dm_}, scal_{dm_};
     DeviceVec<uint32_t> stop_{dm_};  // the fused PCG's convergence flag
+    // The faces of bricks untouched by the last fused transfer hold the
+    // prepared zero-transfer values of prepKey_ (so the next fused transfer
+    // rewrites only the bricks that left the touched set).  Cleared by every
+    // other writer of the face arrays and by a change of solids.
+    struct PrepKey {
+        float gdt;
+        ns::SolidBox box;
+        bool hasSolid;
+        uint64_t solidVersion;
+    };
+    bool facesValid_ = false;
+    PrepKey prepKey_{};
+    uint64_t solidVersion_ = 0;
     // pinned host copies of two batches' history entries, and their events
     double* histPinned_ = nullptr;
     size_t histPinnedDoubles_ = 0;

@@ -491,16 +491,31 @@ __global__ void brick_occupancy_kernel(const uint32_t* __restrict__ cellStart, u
 
 // touched[b] = 1 if brick b or one of its 26 neighbours holds a particle (a
 // superset of the bricks whose faces a particle reaches: the P2G shell is one
-// cell deep); the touched bricks are appended to list (order: by wave, each
-// wave's bricks ascending; the transfer's result does not depend on it).
+// cell deep); the touched bricks are appended to list.  Thread t looks at
+// the t-th brick in Z-order when the brick grid is a power-of-two cube (else
+// in index order), and each wave appends its bricks in thread order, so the
+// list runs through space in compact stretches: the transfer kernel's
+// workgroups on one XCD then read their bricks' shells through one L2.  (The
+// transfer's result does not depend on the order.)
 __global__ void touched_bricks_kernel(const uint8_t* __restrict__ occ, uint32_t nbX, uint32_t nbY, uint32_t nbZ,
-                                      uint8_t* __restrict__ touched, uint32_t* __restrict__ list,
+                                      int morton, uint8_t* __restrict__ touched, uint32_t* __restrict__ list,
                                       uint32_t* __restrict__ count) {
     const uint32_t numBricks = nbX * nbY * nbZ;
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     bool hit = false;
-    if (b < numBricks) {
-        const int bx = (int)(b % nbX), by = (int)((b / nbX) % nbY), bz = (int)(b / (nbX * nbY));
+    uint32_t b = t;
+    if (t < numBricks) {
+        int bx, by, bz;
+        if (morton) {
+            bx = (int)compact3(t);
+            by = (int)compact3(t >> 1);
+            bz = (int)compact3(t >> 2);
+            b = ((uint32_t)bz * nbY + (uint32_t)by) * nbX + (uint32_t)bx;
+        } else {
+            bx = (int)(t % nbX);
+            by = (int)((t / nbX) % nbY);
+            bz = (int)(t / (nbX * nbY));
+        }
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
@@ -519,13 +534,15 @@ __global__ void touched_bricks_kernel(const uint8_t* __restrict__ occ, uint32_t 
     if (hit) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = b;
 }
 
-// Faces of the untouched bricks: prepared values of a zero transfer.
+// Faces of the untouched bricks: prepared values of a zero transfer (with
+// prevTouched, only for the bricks the previous run touched).
 __global__ void __launch_bounds__(256)
-empty_faces_kernel(const uint8_t* __restrict__ touched, uint32_t numCells, Grid g, const uint8_t* __restrict__ solid,
-                   SolidBox box, float gdt, float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
-                   float* __restrict__ U0, float* __restrict__ V0, float* __restrict__ W0) {
+empty_faces_kernel(const uint8_t* __restrict__ touched, const uint8_t* __restrict__ prevTouched, uint32_t numCells,
+                   Grid g, const uint8_t* __restrict__ solid, SolidBox box, float gdt, float* __restrict__ U,
+                   float* __restrict__ V, float* __restrict__ W, float* __restrict__ U0, float* __restrict__ V0,
+                   float* __restrict__ W0) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= numCells || touched[c >> 9]) return;
+    if (c >= numCells || touched[c >> 9] || (prevTouched && !prevTouched[c >> 9])) return;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
     const uint32_t l = c & 511u, b = c >> 9;
     const int gc[3] = {(int)((b % nbX) * 8u + (l & 7u)), (int)(((b / nbX) % nbY) * 8u + ((l >> 3) & 7u)),
@@ -542,8 +559,11 @@ empty_faces_kernel(const uint8_t* __restrict__ touched, uint32_t numCells, Grid 
 }
 
 // Scatter + prepare of the touched bricks; a fixed grid of workgroups loops
-// over the list, XCD x taking the x-th contiguous eighth of it (neighbouring
-// bricks in the same L2).
+// over the list, which is cut into chunks of 64 entries (a 4x4x4 block of
+// bricks in Z-order) dealt round robin to the 8 XCDs: a chunk's shells are
+// read through one L2, and every XCD gets a share of the dense and the
+// sparse parts of the scene (contiguous eighths of a Z-ordered list gave one
+// XCD each octant, i.e. most of a blob's particles to one or two XCDs).
 __global__ void __launch_bounds__(256)
 p2g_prepare_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
                    const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart, Grid g,
@@ -554,10 +574,12 @@ p2g_prepare_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
     const uint32_t total = *count;
     const uint32_t perXcdGroups = gridDim.x >> 3, xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
-    const uint32_t lo = (uint32_t)(((uint64_t)total * xcd) >> 3), hi = (uint32_t)(((uint64_t)total * (xcd + 1u)) >> 3);
     float* F[3] = {U, V, W};
     float* F0[3] = {U0, V0, W0};
-    for (uint32_t e = lo + rank; e < hi; e += perXcdGroups) {
+    for (uint32_t m = rank;; m += perXcdGroups) {
+        // the XCD's m-th entry: offset m % 64 of its (m / 64)-th chunk
+        const uint32_t e = ((m >> 6) * 8u + xcd) * 64u + (m & 63u);
+        if (e >= total) break;  // chunks of an XCD are in increasing order
         const uint32_t brick = list[e];
         const int bb[3] = {(int)(brick % nbX), (int)((brick / nbX) % nbY), (int)(brick / (nbX * nbY))};
         float scaleW, scaleM;
@@ -669,12 +691,16 @@ void launch_p2g_prepare(const float4* sortedRec, const uint32_t* cellStart, Grid
     FLIP2_LAUNCH_CHECK();
     FLIP2_KERNEL(brick_occupancy_kernel, grid1d(numBricks, 256), dim3(256), 0, s, cellStart, numBricks, t.occ);
     FLIP2_LAUNCH_CHECK();
-    FLIP2_KERNEL(touched_bricks_kernel, grid1d(numBricks, 256), dim3(256), 0, s, t.occ, nbX, nbY, nbZ, t.touched,
-                 t.list, t.count);
+    const bool pow2 = (nbX & (nbX - 1)) == 0;
+    const int morton = (nbX == nbY && nbY == nbZ && pow2 && nbX <= 1024) ? 1 : 0;
+    FLIP2_KERNEL(touched_bricks_kernel, grid1d(numBricks, 256), dim3(256), 0, s, t.occ, nbX, nbY, nbZ, morton,
+                 t.touched, t.list, t.count);
     FLIP2_LAUNCH_CHECK();
     const uint32_t numCells = numBricks * 512u;
-    FLIP2_KERNEL(empty_faces_kernel, grid1d(numCells, 256), dim3(256), 0, s, t.touched, numCells, g, prep.solid,
-                 prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0);
+    FLIP2_REQUIRE(prep.full || t.prevTouched, kErrArgument, "transfer + prepare: no previous touched flags");
+    FLIP2_KERNEL(empty_faces_kernel, grid1d(numCells, 256), dim3(256), 0, s, t.touched,
+                 prep.full ? nullptr : t.prevTouched, numCells, g, prep.solid, prep.box, prep.gdt, U, V, W, prep.U0,
+                 prep.V0, prep.W0);
     FLIP2_LAUNCH_CHECK();
     // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
     const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));

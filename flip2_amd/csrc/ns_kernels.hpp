@@ -47,12 +47,18 @@ struct FacePrep {
     const uint8_t* solid;  // brick-tiled solid cells, may be null
     SolidBox box;
     float gdt;  // gravity * dt, added to v faces
+    // 1: write the faces of every untouched brick; 0: only of those that the
+    // previous fused run touched -- the caller guarantees that the others
+    // still hold the prepared zero-transfer values of these same parameters
+    int full;
 };
 // Work arrays of the fused transfer: occ, touched (one byte per brick),
-// list (one word per brick), count (one word).
+// list (one word per brick), count (one word); prevTouched: the previous
+// fused run's touched flags (read when FacePrep::full is 0).
 struct TouchedBricks {
     uint8_t *occ, *touched;
     uint32_t *list, *count;
+    const uint8_t* prevTouched;
 };
 // launch_p2g followed by launch_prepare_faces (the same U, V, W, U0, V0, W0),
 // with the transfer done only for bricks within one brick of a particle and

@@ -1236,79 +1236,107 @@ __global__ void prepare_faces_kernel(float* __restrict__ U, float* __restrict__ 
 // ---------------------------------------------------------------------------
 namespace {
 
-// Trilinear stencil of a MAC component at q (cell units): the eight corner
-// faces in (a, b, e) order and their weights.  A corner outside the stored
-// range reads 0 (walls): it gets weight 0 and the index of a stored face, so
-// the eight loads need no branch and are all in flight together; adding
-// 0 * (finite value) leaves the running sum unchanged, so the sums equal
-// those that skip such corners.
-struct Stencil {
-    uint32_t idx[8];
-    float w[8];
+// Trilinear stencils of the MAC components at q (cell units), built per
+// axis.  The brick-tiled index is separable,
+//   ((kb nbY + jb) nbX + ib) 512 + (k & 7) 64 + (j & 7) 8 + (i & 7),
+// so a corner's index is the sum of one offset per axis, and its weight the
+// product of one weight per axis.  A corner outside [0, size) reads 0
+// (walls): its axis weight is 0 and its offset that of a stored face, so the
+// eight loads of a stencil need no branch and are in flight together;
+// adding 0 * (finite value) leaves a running sum unchanged, so the sums equal
+// those that skip such corners.  Component c is normal to axis c: it uses
+// the coordinate q[d] on axis c and q[d] - 0.5 on the other two, so one
+// position needs six axis stencils for all three components.
+struct AxisStencil {
+    uint32_t off[2];
+    float w[2];
 };
 
-__device__ inline void mac_stencil(int comp, const double q[3], Grid g, uint32_t nbX, uint32_t nbY, Stencil& st) {
-    int base[3];
-    float t[3];
+__device__ inline AxisStencil axis_stencil(double v, int size, uint32_t brickStride, uint32_t cellStride) {
+    const double f = floor(v);
+    const int base = (int)f;
+    const float t = (float)(v - f);
+    AxisStencil a;
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        const int c = base + o;
+        const uint32_t cc = (uint32_t)min(max(c, 0), size - 1);
+        a.off[o] = (cc >> 3) * brickStride + (cc & 7u) * cellStride;
+        a.w[o] = (c >= 0 && c < size) ? (o ? t : 1.f - t) : 0.f;
+    }
+    return a;
+}
+
+struct PosStencil {
+    AxisStencil ax[3][2];  // [axis][0]: at q[axis]; [axis][1]: at q[axis] - 0.5
+};
+
+__device__ inline void pos_stencil(const double q[3], Grid g, uint32_t nbX, uint32_t nbY, PosStencil& ps) {
+    const int size[3] = {(int)g.sizeX, (int)g.sizeY, (int)g.sizeZ};
+    const uint32_t bs[3] = {512u, nbX * 512u, nbX * nbY * 512u}, cs[3] = {1u, 8u, 64u};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double v = (d == comp) ? q[d] : q[d] - 0.5;
-        const double f = floor(v);
-        base[d] = (int)f;
-        t[d] = (float)(v - f);
+        ps.ax[d][0] = axis_stencil(q[d], size[d], bs[d], cs[d]);
+        ps.ax[d][1] = axis_stencil(q[d] - 0.5, size[d], bs[d], cs[d]);
     }
-    const int size[3] = {(int)g.sizeX, (int)g.sizeY, (int)g.sizeZ};
-    uint32_t cl[3][2];
-    bool in[3][2];
+}
+
+__device__ inline AxisStencil pick(const AxisStencil& a, const AxisStencil& b, bool first) {
+    AxisStencil r;
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int o = 0; o < 2; ++o) {
-            const int v = base[d] + o;
-            in[d][o] = v >= 0 && v < size[d];
-            cl[d][o] = (uint32_t)min(max(v, 0), size[d] - 1);
-        }
+    for (int o = 0; o < 2; ++o) {
+        r.off[o] = first ? a.off[o] : b.off[o];
+        r.w[o] = first ? a.w[o] : b.w[o];
+    }
+    return r;
+}
+
+// The eight corners of component c, (a, b, e) order: indices and weights
+// ((w_a w_b) w_e, the product order of the per-corner form).
+__device__ inline void component_corners(const PosStencil& ps, int c, uint32_t idx[8], float w[8]) {
+    const AxisStencil X = pick(ps.ax[0][0], ps.ax[0][1], c == 0);
+    const AxisStencil Y = pick(ps.ax[1][0], ps.ax[1][1], c == 1);
+    const AxisStencil Z = pick(ps.ax[2][0], ps.ax[2][1], c == 2);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                const int c = a * 4 + b * 2 + e;
-                const float w = (a ? t[0] : 1.f - t[0]) * (b ? t[1] : 1.f - t[1]) * (e ? t[2] : 1.f - t[2]);
-                st.w[c] = (in[0][a] && in[1][b] && in[2][e]) ? w : 0.f;
-                st.idx[c] = tiled(cl[0][a], cl[1][b], cl[2][e], nbX, nbY);
+                const int k = a * 4 + b * 2 + e;
+                idx[k] = X.off[a] + Y.off[b] + Z.off[e];
+                w[k] = X.w[a] * Y.w[b] * Z.w[e];
             }
 }
 
-__device__ inline float interp(const float* F, int comp, const double q[3], Grid g, uint32_t nbX, uint32_t nbY) {
-    Stencil st;
-    mac_stencil(comp, q, g, nbX, nbY, st);
-    float v[8];
+__device__ inline float interp(const float* F, int comp, const PosStencil& ps) {
+    uint32_t idx[8];
+    float w[8], v[8];
+    component_corners(ps, comp, idx, w);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = F[st.idx[c]];
+    for (int k = 0; k < 8; ++k) v[k] = F[idx[k]];
     float acc = 0.f;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc = fmaf(st.w[c], v[c], acc);
+    for (int k = 0; k < 8; ++k) acc = fmaf(w[k], v[k], acc);
     return acc;
 }
 
 // interp(F) and interp(F - F0) on one stencil
-__device__ inline void interp_pic_delta(const float* F, const float* F0, int comp, const double q[3], Grid g,
-                                        uint32_t nbX, uint32_t nbY, float& pic, float& delta) {
-    Stencil st;
-    mac_stencil(comp, q, g, nbX, nbY, st);
-    float v[8], v0[8];
+__device__ inline void interp_pic_delta(const float* F, const float* F0, int comp, const PosStencil& ps, float& pic,
+                                        float& delta) {
+    uint32_t idx[8];
+    float w[8], v[8], v0[8];
+    component_corners(ps, comp, idx, w);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        v[c] = F[st.idx[c]];
-        v0[c] = F0[st.idx[c]];
+    for (int k = 0; k < 8; ++k) {
+        v[k] = F[idx[k]];
+        v0[k] = F0[idx[k]];
     }
     float a = 0.f, d = 0.f;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        a = fmaf(st.w[c], v[c], a);
-        d = fmaf(st.w[c], v[c] - v0[c], d);
+    for (int k = 0; k < 8; ++k) {
+        a = fmaf(w[k], v[k], a);
+        d = fmaf(w[k], v[k] - v0[k], d);
     }
     pic = a;
     delta = d;
@@ -1334,13 +1362,15 @@ __global__ void __launch_bounds__(256) g2p_flip_kernel(uint32_t n, float4* __res
     float4 a = rec[2 * p], b = rec[2 * p + 1];
     double q[3];
     to_cell_units(a.x, a.y, a.z, g, q);
+    PosStencil ps;
+    pos_stencil(q, g, nbX, nbY, ps);
     const float* F[3] = {U, V, W};
     const float* F0[3] = {U0, V0, W0};
     float vel[3] = {a.w, b.x, b.y};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         float pic, dlt;
-        interp_pic_delta(F[c], F0[c], c, q, g, nbX, nbY, pic, dlt);
+        interp_pic_delta(F[c], F0[c], c, ps, pic, dlt);
         vel[c] = flip * (vel[c] + dlt) + (1.f - flip) * pic;
     }
     a.w = vel[0];
@@ -1365,12 +1395,14 @@ __device__ inline void advect_from(const float x0[3], const float u1[3], const f
     const float xm[3] = {fmaf(0.5f * dt, u1[0], x0[0]), fmaf(0.5f * dt, u1[1], x0[1]), fmaf(0.5f * dt, u1[2], x0[2])};
     double q[3];
     to_cell_units(xm[0], xm[1], xm[2], g, q);
+    PosStencil ps;
+    pos_stencil(q, g, nbX, nbY, ps);
     const float lo[3] = {g.negX + margin, g.negY + margin, g.negZ + margin};
     const float hi[3] = {g.negX + (float)g.sizeX * g.cellSize - margin, g.negY + (float)g.sizeY * g.cellSize - margin,
                          g.negZ + (float)g.sizeZ * g.cellSize - margin};
     float u2[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) u2[c] = interp(F[c], c, q, g, nbX, nbY);
+    for (int c = 0; c < 3; ++c) u2[c] = interp(F[c], c, ps);
 #pragma unroll
     for (int c = 0; c < 3; ++c) xn[c] = fminf(fmaxf(fmaf(dt, u2[c], x0[c]), lo[c]), hi[c]);
     if (box.active && xn[0] >= box.lo[0] && xn[0] < box.hi[0] && xn[1] >= box.lo[1] && xn[1] < box.hi[1] &&
@@ -1408,10 +1440,12 @@ __global__ void __launch_bounds__(256) advect_rk2_kernel(uint32_t n, float4* __r
     const float x0[3] = {a.x, a.y, a.z};
     double q[3];
     to_cell_units(x0[0], x0[1], x0[2], g, q);
+    PosStencil ps;
+    pos_stencil(q, g, nbX, nbY, ps);
     const float* F[3] = {U, V, W};
     float u1[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) u1[c] = interp(F[c], c, q, g, nbX, nbY);
+    for (int c = 0; c < 3; ++c) u1[c] = interp(F[c], c, ps);
     float xn[3];
     advect_from(x0, u1, U, V, W, solid, g, dt, margin, box, xn);
     a.x = xn[0];
@@ -1436,13 +1470,15 @@ __global__ void __launch_bounds__(256) g2p_advect_kernel(uint32_t n, float4* __r
     const float x0[3] = {a.x, a.y, a.z};
     double q[3];
     to_cell_units(x0[0], x0[1], x0[2], g, q);
+    PosStencil ps;
+    pos_stencil(q, g, nbX, nbY, ps);
     const float* F[3] = {U, V, W};
     const float* F0[3] = {U0, V0, W0};
     float vel[3] = {a.w, b.x, b.y}, u1[3];
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
         float dlt;
-        interp_pic_delta(F[c], F0[c], c, q, g, nbX, nbY, u1[c], dlt);
+        interp_pic_delta(F[c], F0[c], c, ps, u1[c], dlt);
         vel[c] = flip * (vel[c] + dlt) + (1.f - flip) * u1[c];
     }
     float xn[3];

@@ -16,7 +16,7 @@ P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g, DeviceManager* po
     numCells_ = (uint32_t)cells;
     for (auto* v : {&keys_, &keysSorted_, &perm_, &keysAlt_, &permAlt_, &counts_, &cellStart_, &stats_}) v->setPool(pool);
     for (auto* v : {&packed_, &sorted_}) v->setPool(pool);
-    for (auto* v : {&occ_, &touched_}) v->setPool(pool);
+    for (auto* v : {&occ_, &touched_, &prevTouched_}) v->setPool(pool);
     touchedList_.setPool(pool);
     ws_.setPool(pool);
     keyBits_ = std::max<uint32_t>(1, bits_for(numCells_ - 1));
@@ -81,12 +81,18 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
         if (occ_.size() < numBricks) {
             occ_.resizeAsync(numBricks, s);
             touched_.resizeAsync(numBricks, s);
+            prevTouched_.resizeAsync(numBricks, s);
             touchedList_.resizeAsync(numBricks + 1, s);
+            prevTouchedValid_ = false;
         }
-        ns::launch_p2g_prepare(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), *prep, U, V, W,
+        ns::FacePrep fp = *prep;
+        if (!prevTouchedValid_) fp.full = 1;
+        ns::launch_p2g_prepare(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), fp, U, V, W,
                                {occ_.devPtr(), touched_.devPtr(), touchedList_.devPtr(),
-                                touchedList_.devPtr() + numBricks},
+                                touchedList_.devPtr() + numBricks, prevTouched_.devPtr()},
                                s);
+        touched_.swapDevice(prevTouched_);  // this run's flags are the next run's previous ones
+        prevTouchedValid_ = true;
     } else {
         ns::launch_p2g(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), U, V, W, nullptr, nullptr,
                        nullptr, s);

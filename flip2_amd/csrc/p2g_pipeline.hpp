@@ -67,7 +67,8 @@ private:
     std::vector<std::string> timingNames_;
     std::vector<double> timingMs_;
     int timedRuns_ = 0;
-    DeviceVec<uint8_t> occ_, touched_;  // fused transfer + prepare: per-brick flags
+    DeviceVec<uint8_t> occ_, touched_, prevTouched_;  // fused transfer + prepare: per-brick flags
+    bool prevTouchedValid_ = false;  // prevTouched_ holds the last fused run's flags
     DeviceVec<uint32_t> touchedList_;    // and the touched bricks (list, then count)
     void runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W, hipStream_t s,
                  const ns::FacePrep* prep);

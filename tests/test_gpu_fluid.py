@@ -136,7 +136,9 @@ def test_fused_transfer_prepare_bit_identical(flip2, oracle, solid):
     brick of a particle, prepared zero faces elsewhere) gives the faces,
     saved faces and sorted records of the transfer stage followed by the
     prepare stage, bit for bit, on a 64^3 grid whose particles leave most
-    bricks untouched, with a moving solid box or a static solid mask."""
+    bricks untouched, with a moving solid box or a static solid mask; over
+    two steps, so the second fused transfer (which rewrites only the bricks
+    that left the touched set) is covered too."""
     n = 64
     h = 1.0 / n
     rng = np.random.default_rng(8)
@@ -145,6 +147,8 @@ def test_fused_transfer_prepare_bit_identical(flip2, oracle, solid):
     pos = np.concatenate([blob, edge]).astype(np.float32)
     x, y, z = (np.ascontiguousarray(pos[:, d]) for d in range(3))
     v = [(rng.standard_normal(x.size) * 0.5).astype(np.float32) for _ in range(3)]
+    v[1][-500:] = -50.0  # the edge particles leave their bricks within one step (12 cells)
+    v[2][-500:] = -50.0
     grid = (0.0, 0.0, 0.0, n, n, n, h)
     names = ("U", "V", "W", "U0", "V0", "W0")
     out = []
@@ -159,11 +163,22 @@ def test_fused_transfer_prepare_bit_identical(flip2, oracle, solid):
             m[60:, :, :] = True
             from tests.fluid_checks import tiled_mask
             sim.set_solid(tiled_mask(m.transpose(2, 1, 0), grid, oracle))
-        if fused:
-            sim.stage("transfer_prepare")
-        else:
-            sim.stage("transfer")
-            sim.stage("prepare")
+        # two steps: the second fused transfer rewrites only the bricks that
+        # left the touched set (the particles move between the two)
+        for step in range(2):
+            if step:
+                for s in ("system", "solve", "project"):
+                    sim.stage(s)
+                if fused:
+                    sim.stage("g2p_advect")
+                else:
+                    sim.stage("g2p")
+                    sim.stage("advect")
+            if fused:
+                sim.stage("transfer_prepare")
+            else:
+                sim.stage("transfer")
+                sim.stage("prepare")
         out.append([sim.get(k) for k in names] + list(sim.particles()))
     for name, a, b in zip(names + ("x", "y", "z", "vx", "vy", "vz"), *out):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
