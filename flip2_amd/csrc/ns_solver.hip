@@ -993,64 +993,31 @@ cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restri
     group_partial<W>(live ? rr : 0.0, s_red + 4, c.partials + 2u * ng + grp);
 }
 
-// MIC(0): the same update, then z = M^-1 r by the brick's forward and
-// backward sweeps over its 22 anti-diagonal planes, all in the wave's LDS.
-// LDS holds (value, precondition factor) and the coefficient word of cell c
-// at entry 65 + c, with 65 zero entries below and 64 above, so a cell's six
-// neighbours are at fixed offsets (-1, -8, -64, +1, +8, +64) and every read
-// of a plane is issued at once, without a branch: a lane with no cell on the
-// plane works on the zero entry 64.  The coefficient words keep only
-// couplings inside the brick, so a neighbour offset that wraps to another
-// row or plane meets a cleared bit.  The coupling A(c, m) = -1 of fluid
-// neighbours is the matching bit of c's own word; the terms (A pc_m) q_m and
-// (A pc_c) z_m are formed as -(pc q) (negation is exact), i.e. added where
-// the bit is set.  The wave's LDS operations execute in order, so a plane's
-// reads see the previous plane's writes without waiting for them
-// (wave_lds_sync).
-template <uint32_t W>
-__global__ void __launch_bounds__(64 * W)
-cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64, const float* __restrict__ p) {
-    constexpr uint32_t kPad = 65, kEntries = 65 + 512 + 64;
-    __shared__ float2 s_aw[W][kEntries];
-    __shared__ uint16_t s_cw[W][kEntries];
-    __shared__ double s_red[8];
-    const uint32_t ng = (c.na + W - 1u) / W;
-    const uint32_t grp = cg_group(ng);
-    if (grp >= ng) return;
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t slotRaw = grp * W + wave;
-    const bool live = slotRaw < c.na;
-    const uint32_t slot = live ? slotRaw : c.na - 1u;
-    const uint32_t base = slot * 512u;
-    float2* s_a = s_aw[wave];
-    uint16_t* s_c = s_cw[wave];
-    // this lane's cell of each plane (two per word), fetched once so the
-    // sweeps wait only on LDS
-    uint32_t cellOf2[11];
+// Pieces of the MIC(0) update's sweeps, per wave.  LDS layout: (value,
+// precondition factor) and the coefficient word of cell c at entry
+// kMicPad + c of kMicEntries, zero entries below and above.
+constexpr uint32_t kMicPad = 65, kMicEntries = 65 + 512 + 64;
+
+// This lane's cell of each plane (two per word), from the plane table.
+__device__ inline void mic_plane_cells(uint32_t lane, uint32_t cellOf2[11]) {
 #pragma unroll
     for (int s = 0; s < 11; ++s)
         cellOf2[s] = (uint32_t)c_planes.lane_cell[2 * s][lane] | ((uint32_t)c_planes.lane_cell[2 * s + 1][lane] << 16);
-    auto cellOf = [&](int s) { return (s & 1) ? cellOf2[s >> 1] >> 16 : cellOf2[s >> 1] & 0xffffu; };
-    // W > 1: the partial sum first (fewer registers live across it); W == 1:
-    // the slot's loads first (the step length is two history words)
-    if (!cg_proceed(c, init, grp)) return;
-    float alpha = 0.f;
-    if constexpr (W > 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
-    float pc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) pc[k] = c.precon[base + lane + 64u * (uint32_t)k];
-    CellLoads L;
-    cg_load_cells(c, base, lane, init, r64, p, L);
-    if constexpr (W == 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
-    float rk[8];  // this lane's r values (cells lane + 64 k)
-    cg_store_cells(c, base, lane, init, alpha, live, L, rk);
-    double rz = 0.0, rr = 0.0;
-    for (uint32_t e = lane; e < kPad; e += 64u) {
+}
+
+__device__ inline void mic_clear_pads(float2* s_a, uint16_t* s_c, uint32_t lane) {
+    for (uint32_t e = lane; e < kMicPad; e += 64u) {
         s_a[e] = make_float2(0.f, 0.f);
         s_c[e] = 0;
     }
-    s_a[kPad + 512u + lane] = make_float2(0.f, 0.f);
-    s_c[kPad + 512u + lane] = 0;
+    s_a[kMicPad + 512u + lane] = make_float2(0.f, 0.f);
+    s_c[kMicPad + 512u + lane] = 0;
+}
+
+// The slot's r (rk), factors and in-brick coupling words into LDS; rr += r.r
+// of this lane's cells.
+__device__ inline void mic_stage(float2* s_a, uint16_t* s_c, uint32_t lane, const uint16_t cf[8], const float rk[8],
+                                 const float pc[8], double& rr) {
     const uint32_t i = lane & 7u, j = lane >> 3;
     // in-brick coupling bits of the mask (-x, +x, -y, +y, -z, +z at bits 3..8)
     const uint32_t keepXY = ~((i == 0u ? 1u << 3 : 0u) | (i == 7u ? 1u << 4 : 0u) |
@@ -1061,10 +1028,14 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
         const float rv = rk[k];
         rr += (double)rv * (double)rv;
         const uint32_t keep = keepXY & ~((k == 0 ? 1u << 7 : 0u) | (k == 7 ? 1u << 8 : 0u));
-        s_c[kPad + l] = (uint16_t)(L.cf[k] & keep);
-        s_a[kPad + l] = make_float2(rv, pc[k]);  // r, replaced by the forward sweep's q plane by plane
+        s_c[kMicPad + l] = (uint16_t)(cf[k] & keep);
+        s_a[kMicPad + l] = make_float2(rv, pc[k]);  // r, replaced by the forward sweep's q plane by plane
     }
-    wave_lds_sync();
+}
+
+// Forward then backward sweep: z = M^-1 r in place of r.
+__device__ inline void mic_sweeps(float2* s_a, const uint16_t* s_c, const uint32_t cellOf2[11]) {
+    auto cellOf = [&](int s) { return (s & 1) ? cellOf2[s >> 1] >> 16 : cellOf2[s >> 1] & 0xffffu; };
 #pragma unroll
     for (int s = 0; s < 22; ++s) {
         const uint32_t b = cellOf(s);  // own entry b + 64
@@ -1090,10 +1061,64 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
         s_a[b + 64u].x = acc * own.y;
         wave_lds_sync();
     }
+}
+
+// MIC(0): the same update, then z = M^-1 r by the brick's forward and
+// backward sweeps over its 22 anti-diagonal planes, all in the wave's LDS.
+// LDS holds (value, precondition factor) and the coefficient word of cell c
+// at entry 65 + c, with 65 zero entries below and 64 above, so a cell's six
+// neighbours are at fixed offsets (-1, -8, -64, +1, +8, +64) and every read
+// of a plane is issued at once, without a branch: a lane with no cell on the
+// plane works on the zero entry 64.  The coefficient words keep only
+// couplings inside the brick, so a neighbour offset that wraps to another
+// row or plane meets a cleared bit.  The coupling A(c, m) = -1 of fluid
+// neighbours is the matching bit of c's own word; the terms (A pc_m) q_m and
+// (A pc_c) z_m are formed as -(pc q) (negation is exact), i.e. added where
+// the bit is set.  The wave's LDS operations execute in order, so a plane's
+// reads see the previous plane's writes without waiting for them
+// (wave_lds_sync).
+template <uint32_t W>
+__global__ void __launch_bounds__(64 * W)
+cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64, const float* __restrict__ p) {
+    __shared__ float2 s_aw[W][kMicEntries];
+    __shared__ uint16_t s_cw[W][kMicEntries];
+    __shared__ double s_red[8];
+    const uint32_t ng = (c.na + W - 1u) / W;
+    const uint32_t grp = cg_group(ng);
+    if (grp >= ng) return;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t slotRaw = grp * W + wave;
+    const bool live = slotRaw < c.na;
+    const uint32_t slot = live ? slotRaw : c.na - 1u;
+    const uint32_t base = slot * 512u;
+    float2* s_a = s_aw[wave];
+    uint16_t* s_c = s_cw[wave];
+    // this lane's cell of each plane (two per word), fetched once so the
+    // sweeps wait only on LDS
+    uint32_t cellOf2[11];
+    mic_plane_cells(lane, cellOf2);
+    // W > 1: the partial sum first (fewer registers live across it); W == 1:
+    // the slot's loads first (the step length is two history words)
+    if (!cg_proceed(c, init, grp)) return;
+    float alpha = 0.f;
+    if constexpr (W > 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
+    float pc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pc[k] = c.precon[base + lane + 64u * (uint32_t)k];
+    CellLoads L;
+    cg_load_cells(c, base, lane, init, r64, p, L);
+    if constexpr (W == 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
+    float rk[8];  // this lane's r values (cells lane + 64 k)
+    cg_store_cells(c, base, lane, init, alpha, live, L, rk);
+    double rz = 0.0, rr = 0.0;
+    mic_clear_pads(s_a, s_c, lane);
+    mic_stage(s_a, s_c, lane, L.cf, rk, pc, rr);
+    wave_lds_sync();
+    mic_sweeps(s_a, s_c, cellOf2);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const uint32_t l = lane + 64u * (uint32_t)k;
-        const float zv = s_a[kPad + l].x;
+        const float zv = s_a[kMicPad + l].x;
         if (live) c.z[base + l] = zv;
         rz += (double)rk[k] * (double)zv;
     }
