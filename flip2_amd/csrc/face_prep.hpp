@@ -23,8 +23,9 @@ __device__ inline bool cell_solid(int i, int j, int k, Grid g, const uint8_t* so
 // The cell's centre lies in the (active) box.
 __device__ inline bool in_box(int i, int j, int k, Grid g, const SolidBox& box) {
     if (!box.active) return false;
-    const float c[3] = {g.negX + ((float)i + 0.5f) * g.cellSize, g.negY + ((float)j + 0.5f) * g.cellSize,
-                        g.negZ + ((float)k + 0.5f) * g.cellSize};
+    // explicit fmas: the same rounding in every kernel that inlines this
+    const float c[3] = {fmaf((float)i + 0.5f, g.cellSize, g.negX), fmaf((float)j + 0.5f, g.cellSize, g.negY),
+                        fmaf((float)k + 0.5f, g.cellSize, g.negZ)};
     return c[0] >= box.lo[0] && c[0] < box.hi[0] && c[1] >= box.lo[1] && c[1] < box.hi[1] && c[2] >= box.lo[2] &&
            c[2] < box.hi[2];
 }

@@ -109,6 +109,7 @@ void FluidSim::setConfig(const FluidConfig& cfg) {
 
 void FluidSim::setParticles(uint32_t n, const float* x, const float* y, const float* z, const float* vx,
                             const float* vy, const float* vz) {
+    bricksFresh_ = false;
     FLIP2_REQUIRE(n <= maxParticles_, kErrRange, "more particles than the simulation was created for");
     n_ = n;
     std::vector<float4> r(2 * (size_t)n);
@@ -234,6 +235,7 @@ void FluidSim::saveSnapshot(const std::string& path) {
 }
 
 void FluidSim::loadSnapshot(const std::string& path) {
+    bricksFresh_ = false;
     SnapFile in(path, "rb");
     char magic[sizeof(kSnapMagic)];
     in.get(magic, sizeof(magic));
@@ -378,6 +380,7 @@ void FluidSim::readHistory(uint32_t first, uint32_t count, double* host) {
 
 void FluidSim::transfer() {
     facesValid_ = false;
+    bricksFresh_ = false;
     pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_);
     rec_.swapDevice(pipe_->sortedVec());
 }
@@ -393,6 +396,7 @@ void FluidSim::transferAndPrepare() {
     rec_.swapDevice(pipe_->sortedVec());
     facesValid_ = true;
     prepKey_ = key;
+    bricksFresh_ = true;
 }
 
 void FluidSim::prepareFaces() {
@@ -678,15 +682,24 @@ void FluidSim::gridToParticles() {
 }
 
 void FluidSim::advect() {
+    bricksFresh_ = false;
     ns::launch_advect_rk2(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(),
                           hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.dt, cfg_.margin * cfg_.grid.cellSize,
                           box_, stream_);
 }
 
 void FluidSim::gridToParticlesAndAdvect() {
-    ns::launch_g2p_advect(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(),
-                          W0_.devPtr(), hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.flip, cfg_.dt,
-                          cfg_.margin * cfg_.grid.cellSize, box_, stream_);
+    const uint8_t* solid = hasSolid_ ? solid_.devPtr() : nullptr;
+    if (bricksFresh_)
+        ns::launch_g2p_advect_bricks(pipe_->touchedList(), pipe_->touchedCount(), pipe_->cellStart(), rec_.devPtr(),
+                                     U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
+                                     solid, cfg_.grid, cfg_.flip, cfg_.dt, cfg_.margin * cfg_.grid.cellSize, box_,
+                                     stream_);
+    else
+        ns::launch_g2p_advect(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(),
+                              W0_.devPtr(), solid, cfg_.grid, cfg_.flip, cfg_.dt, cfg_.margin * cfg_.grid.cellSize,
+                              box_, stream_);
+    bricksFresh_ = false;  // the particles moved
 }
 
 void FluidSim::step() {

@@ -200,6 +200,10 @@ private:
         uint64_t solidVersion;
     };
     bool facesValid_ = false;
+    // the records are sorted by brick and the pipeline's touched list and
+    // cell offsets describe them (set by the fused transfer, cleared when the
+    // particles move or another transfer runs)
+    bool bricksFresh_ = false;
     PrepKey prepKey_{};
     uint64_t solidVersion_ = 0;
     // pinned host copies of two batches' history entries, and their events

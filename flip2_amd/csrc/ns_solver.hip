@@ -699,19 +699,22 @@ constexpr uint32_t kCgGroupedBelow = 4096;
 // Workgroup index in the XCD-aware order over ng groups (see xcd_slot).
 __device__ inline uint32_t cg_group(uint32_t ng) { return xcd_slot(ng); }
 
-// Fixed-order sums of a[0..n) and b[0..n) (b may be null) by the 256
-// threads of a workgroup: thread t adds a[t + 256 j] in j order (loads issued
-// four at a time before the adds), then the waves' butterfly sums, then the
-// four wave sums in order.  Every workgroup computes the same doubles.
+// Fixed-order sums of a[0..n) and b[0..n) (b may be null) by the 64 W
+// threads of a workgroup: thread t adds a[t + 64 W j] in j order (loads
+// issued four at a time before the adds), then the waves' butterfly sums,
+// then the W wave sums in order (s_red: 2 W doubles).  Every workgroup
+// computes the same doubles.
+template <uint32_t W>
 __device__ inline void group_sum(const double* __restrict__ a, const double* __restrict__ b, uint32_t n,
                                  double* s_red, double& sa, double& sb) {
+    constexpr uint32_t T = 64u * W;
     const uint32_t tid = threadIdx.x;
     double x = 0.0, y = 0.0;
-    for (uint32_t i0 = tid; i0 < n; i0 += 4u * 256u) {
+    for (uint32_t i0 = tid; i0 < n; i0 += 4u * T) {
         double va[4], vb[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint32_t idx = i0 + 256u * (uint32_t)k;
+            const uint32_t idx = i0 + T * (uint32_t)k;
             va[k] = idx < n ? a[idx] : 0.0;
             vb[k] = (b && idx < n) ? b[idx] : 0.0;
         }
@@ -727,15 +730,21 @@ __device__ inline void group_sum(const double* __restrict__ a, const double* __r
     }
     if ((tid & 63u) == 0) {
         s_red[tid >> 6] = x;
-        s_red[4 + (tid >> 6)] = y;
+        s_red[W + (tid >> 6)] = y;
     }
     __syncthreads();
-    sa = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
-    sb = ((s_red[4] + s_red[5]) + s_red[6]) + s_red[7];
+    sa = s_red[0];
+    sb = s_red[W];
+#pragma unroll
+    for (uint32_t w = 1; w < W; ++w) {
+        sa += s_red[w];
+        sb += s_red[W + w];
+    }
     __syncthreads();
 }
 
-// The workgroup's partial: its W waves' values in wave order.
+// The workgroup's partial: its W waves' values in wave order (s_red: W
+// doubles).
 template <uint32_t W>
 __device__ inline void group_partial(double mine, double* s_red, double* out) {
     if constexpr (W == 1) {
@@ -743,7 +752,12 @@ __device__ inline void group_partial(double mine, double* s_red, double* out) {
     } else {
         if ((threadIdx.x & 63u) == 0) s_red[threadIdx.x >> 6] = mine;
         __syncthreads();
-        if (threadIdx.x == 0) *out = ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
+        if (threadIdx.x == 0) {
+            double t = s_red[0];
+#pragma unroll
+            for (uint32_t w = 1; w < W; ++w) t += s_red[w];
+            *out = t;
+        }
     }
 }
 
@@ -763,7 +777,7 @@ __global__ void __launch_bounds__(64 * W)
 cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __restrict__ pOld,
                            float* __restrict__ pNew) {
     __shared__ float s_pw[W][512];
-    __shared__ double s_red[8];
+    __shared__ double s_red[2 * W];
     const uint32_t ng = (c.na + W - 1u) / W;
     const uint32_t grp = cg_group(ng);
     if (grp >= ng) return;
@@ -816,7 +830,7 @@ cg_direction_matvec_kernel(CgFused c, uint32_t it, int first, const float* __res
     double rz;
     if constexpr (W > 1) {
         double rr;
-        group_sum(c.partials + ng, c.partials + 2u * ng, ng, s_red, rz, rr);
+        group_sum<W>(c.partials + ng, c.partials + 2u * ng, ng, s_red, rz, rr);
         if (grp == 0 && threadIdx.x == 0) {
             c.hist[4u * it] = rz;
             c.hist[4u * it + 1u] = rr;
@@ -938,7 +952,7 @@ __device__ inline float cg_alpha_from(const CgFused& c, uint32_t it, int init, u
     double pq;
     if constexpr (W > 1) {
         double unused;
-        group_sum(c.partials, nullptr, ng, s_red, pq, unused);
+        group_sum<W>(c.partials, nullptr, ng, s_red, pq, unused);
         if (grp == 0 && threadIdx.x == 0) c.hist[4u * it + 2u] = pq;
     } else {
         pq = c.hist[4u * it + 2u];  // recorded by the reduction after the matvec
@@ -957,11 +971,12 @@ __device__ inline float keep_if_bit(float v, uint32_t w, int bit) {
 // Jacobi: e += alpha p, r -= alpha q (init: r = (float) r64, e = 0),
 // z = r / diag; partials r.z and r.r per group.  Wave = slot, each lane the
 // eight cells lane + 64 k.
-template <uint32_t W>
+template <uint32_t W, int INIT>
 __global__ void __launch_bounds__(64 * W)
-cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64,
+cg_update_jacobi_kernel(CgFused c, uint32_t it, const double* __restrict__ r64,
                         const float* __restrict__ p) {
-    __shared__ double s_red[8];
+    constexpr int init = INIT;  // the refinement's start
+    __shared__ double s_red[2 * W];
     const uint32_t ng = (c.na + W - 1u) / W;
     const uint32_t grp = cg_group(ng);
     if (grp >= ng) return;
@@ -990,7 +1005,7 @@ cg_update_jacobi_kernel(CgFused c, uint32_t it, int init, const double* __restri
         rr += __shfl_xor(rr, d, 64);
     }
     group_partial<W>(live ? rz : 0.0, s_red, c.partials + ng + grp);
-    group_partial<W>(live ? rr : 0.0, s_red + 4, c.partials + 2u * ng + grp);
+    group_partial<W>(live ? rr : 0.0, s_red + W, c.partials + 2u * ng + grp);
 }
 
 // Pieces of the MIC(0) update's sweeps, per wave.  LDS layout: (value,
@@ -1077,12 +1092,13 @@ __device__ inline void mic_sweeps(float2* s_a, const uint16_t* s_c, const uint32
 // the bit is set.  The wave's LDS operations execute in order, so a plane's
 // reads see the previous plane's writes without waiting for them
 // (wave_lds_sync).
-template <uint32_t W>
+template <uint32_t W, int INIT>
 __global__ void __launch_bounds__(64 * W)
-cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict__ r64, const float* __restrict__ p) {
+cg_update_mic_kernel(CgFused c, uint32_t it, const double* __restrict__ r64, const float* __restrict__ p) {
+    constexpr int init = INIT;  // the refinement's start
     __shared__ float2 s_aw[W][kMicEntries];
     __shared__ uint16_t s_cw[W][kMicEntries];
-    __shared__ double s_red[8];
+    __shared__ double s_red[2 * W];
     const uint32_t ng = (c.na + W - 1u) / W;
     const uint32_t grp = cg_group(ng);
     if (grp >= ng) return;
@@ -1093,13 +1109,15 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
     const uint32_t base = slot * 512u;
     float2* s_a = s_aw[wave];
     uint16_t* s_c = s_cw[wave];
-    // this lane's cell of each plane (two per word), fetched once so the
-    // sweeps wait only on LDS
-    uint32_t cellOf2[11];
-    mic_plane_cells(lane, cellOf2);
     // W > 1: the partial sum first (fewer registers live across it); W == 1:
     // the slot's loads first (the step length is two history words)
     if (!cg_proceed(c, init, grp)) return;
+    // this lane's cell of each plane (two per word).  W == 1 (large systems,
+    // every slot's wave in flight) fetches it first, with the slot's loads;
+    // W > 1 after the update, when the loaded values are dead (fewer
+    // registers: more waves per SIMD, which the small systems need more).
+    uint32_t cellOf2[11];
+    if constexpr (W == 1) mic_plane_cells(lane, cellOf2);
     float alpha = 0.f;
     if constexpr (W > 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
     float pc[8];
@@ -1110,6 +1128,7 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
     if constexpr (W == 1) alpha = cg_alpha_from<W>(c, it, init, ng, grp, s_red);
     float rk[8];  // this lane's r values (cells lane + 64 k)
     cg_store_cells(c, base, lane, init, alpha, live, L, rk);
+    if constexpr (W > 1) mic_plane_cells(lane, cellOf2);
     double rz = 0.0, rr = 0.0;
     mic_clear_pads(s_a, s_c, lane);
     mic_stage(s_a, s_c, lane, L.cf, rk, pc, rr);
@@ -1127,17 +1146,18 @@ cg_update_mic_kernel(CgFused c, uint32_t it, int init, const double* __restrict_
         rr += __shfl_xor(rr, d, 64);
     }
     group_partial<W>(live ? rz : 0.0, s_red, c.partials + ng + grp);
-    group_partial<W>(live ? rr : 0.0, s_red + 4, c.partials + 2u * ng + grp);
+    group_partial<W>(live ? rr : 0.0, s_red + W, c.partials + 2u * ng + grp);
 }
 
 // hist[4 it], hist[4 it + 1] = rz, rr from the update's partials (the sums
 // the next matvec kernel forms), for the host's check at the end of a batch.
-__global__ void __launch_bounds__(256) cg_history_kernel(CgFused c, uint32_t it) {
-    __shared__ double s_red[8];
+template <uint32_t W>
+__global__ void __launch_bounds__(64 * W) cg_history_kernel(CgFused c, uint32_t it) {
+    __shared__ double s_red[2 * W];
     if (*c.stop) return;
-    const uint32_t ng = (c.na + 3u) / 4u;  // the grouped kernels' workgroups
+    const uint32_t ng = (c.na + W - 1u) / W;  // the grouped kernels' workgroups
     double rz, rr;
-    group_sum(c.partials + ng, c.partials + 2u * ng, ng, s_red, rz, rr);
+    group_sum<W>(c.partials + ng, c.partials + 2u * ng, ng, s_red, rz, rr);
     if (threadIdx.x == 0) {
         c.hist[4u * it] = rz;
         c.hist[4u * it + 1u] = rr;
@@ -1367,6 +1387,13 @@ __device__ inline void interp_pic_delta(const float* F, const float* F0, int com
     delta = d;
 }
 
+// flip (v + delta) + (1 - flip) pic with one explicit fma: left to the
+// compiler, the contraction of the plain expression depends on the code
+// around it, and the kernels that share this update must round it alike.
+__device__ inline float flip_blend(float flip, float v, float delta, float pic) {
+    return fmaf(flip, v + delta, (1.f - flip) * pic);
+}
+
 __device__ inline void to_cell_units(float x, float y, float z, Grid g, double q[3]) {
     const double inv = 1.0 / (double)g.cellSize;
     q[0] = ((double)x - (double)g.negX) * inv;
@@ -1396,7 +1423,7 @@ __global__ void __launch_bounds__(256) g2p_flip_kernel(uint32_t n, float4* __res
     for (int c = 0; c < 3; ++c) {
         float pic, dlt;
         interp_pic_delta(F[c], F0[c], c, ps, pic, dlt);
-        vel[c] = flip * (vel[c] + dlt) + (1.f - flip) * pic;
+        vel[c] = flip_blend(flip, vel[c], dlt, pic);
     }
     a.w = vel[0];
     b.x = vel[1];
@@ -1412,22 +1439,24 @@ namespace {
 // cells: a particle that ends inside the solid box is pushed out through the
 // box face nearest to it, one in another solid cell returns to its start
 // point.
-__device__ inline void advect_from(const float x0[3], const float u1[3], const float* U, const float* V,
-                                   const float* W, const uint8_t* solid, Grid g, float dt, float margin,
-                                   const SolidBox& box, float xn[3]) {
-    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
-    const float* F[3] = {U, V, W};
+// The midpoint of the RK2 step and its position in cell units.
+__device__ inline void advect_midpoint(const float x0[3], const float u1[3], Grid g, float dt, double q[3]) {
     const float xm[3] = {fmaf(0.5f * dt, u1[0], x0[0]), fmaf(0.5f * dt, u1[1], x0[1]), fmaf(0.5f * dt, u1[2], x0[2])};
-    double q[3];
     to_cell_units(xm[0], xm[1], xm[2], g, q);
-    PosStencil ps;
-    pos_stencil(q, g, nbX, nbY, ps);
+}
+
+// x0 + dt u2 (u2: the velocity at the midpoint), clamped into [lo, hi] per
+// axis and out of solid cells: a particle that ends inside the solid box is
+// pushed out through the box face nearest to it, one in another solid cell
+// returns to its start point.
+__device__ inline void advect_finish(const float x0[3], const float u2[3], const uint8_t* solid, Grid g, float dt,
+                                     float margin, const SolidBox& box, float xn[3]) {
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
     const float lo[3] = {g.negX + margin, g.negY + margin, g.negZ + margin};
-    const float hi[3] = {g.negX + (float)g.sizeX * g.cellSize - margin, g.negY + (float)g.sizeY * g.cellSize - margin,
-                         g.negZ + (float)g.sizeZ * g.cellSize - margin};
-    float u2[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) u2[c] = interp(F[c], c, ps);
+    // explicit fmas: the same rounding in every kernel that inlines this
+    const float hi[3] = {fmaf((float)g.sizeX, g.cellSize, g.negX) - margin,
+                         fmaf((float)g.sizeY, g.cellSize, g.negY) - margin,
+                         fmaf((float)g.sizeZ, g.cellSize, g.negZ) - margin};
 #pragma unroll
     for (int c = 0; c < 3; ++c) xn[c] = fminf(fmaxf(fmaf(dt, u2[c], x0[c]), lo[c]), hi[c]);
     if (box.active && xn[0] >= box.lo[0] && xn[0] < box.hi[0] && xn[1] >= box.lo[1] && xn[1] < box.hi[1] &&
@@ -1443,6 +1472,7 @@ __device__ inline void advect_from(const float x0[3], const float u1[3], const f
         if (best >= 0) xn[best] = target;
     }
     if (solid) {
+        double q[3];
         to_cell_units(xn[0], xn[1], xn[2], g, q);
         const int ci = (int)floor(q[0]), cj = (int)floor(q[1]), ck = (int)floor(q[2]);
         if (cell_solid(ci, cj, ck, g, solid, nbX, nbY) && !in_box(ci, cj, ck, g, box)) {
@@ -1451,6 +1481,53 @@ __device__ inline void advect_from(const float x0[3], const float u1[3], const f
             xn[2] = x0[2];
         }
     }
+}
+
+// Midpoint (RK2) advection of x0 through the grid velocity, u1 being the
+// velocity at x0.
+__device__ inline void advect_from(const float x0[3], const float u1[3], const float* U, const float* V,
+                                   const float* W, const uint8_t* solid, Grid g, float dt, float margin,
+                                   const SolidBox& box, float xn[3]) {
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
+    const float* F[3] = {U, V, W};
+    double q[3];
+    advect_midpoint(x0, u1, g, dt, q);
+    PosStencil ps;
+    pos_stencil(q, g, nbX, nbY, ps);
+    float u2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u2[c] = interp(F[c], c, ps);
+    advect_finish(x0, u2, solid, g, dt, margin, box, xn);
+}
+
+// The stencils of a position with offsets into a 10 x 10 x 10 region of
+// cells (x fastest) whose first cell is grid cell org: the corners and
+// weights of pos_stencil (the same clamping, so the same face values);
+// false if a corner lies outside the region (its offset is then clamped
+// into the region, and the caller uses the grid instead).
+constexpr int kRegion = 10;
+__device__ inline bool region_stencil(const double q[3], Grid g, const int org[3], PosStencil& ps) {
+    const int size[3] = {(int)g.sizeX, (int)g.sizeY, (int)g.sizeZ};
+    const uint32_t st[3] = {1u, (uint32_t)kRegion, (uint32_t)(kRegion * kRegion)};
+    bool inside = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int sh = 0; sh < 2; ++sh) {
+            const double v = sh ? q[d] - 0.5 : q[d];
+            const double f = floor(v);
+            const int base = (int)f;
+            const float t = (float)(v - f);
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+                const int c = base + o;
+                const int r = min(max(c, 0), size[d] - 1) - org[d];
+                inside = inside && r >= 0 && r < kRegion;
+                ps.ax[d][sh].off[o] = (uint32_t)min(max(r, 0), kRegion - 1) * st[d];
+                ps.ax[d][sh].w[o] = (c >= 0 && c < size[d]) ? (o ? t : 1.f - t) : 0.f;
+            }
+        }
+    return inside;
 }
 
 }  // namespace
@@ -1504,7 +1581,7 @@ __global__ void __launch_bounds__(256) g2p_advect_kernel(uint32_t n, float4* __r
     for (int c = 0; c < 3; ++c) {
         float dlt;
         interp_pic_delta(F[c], F0[c], c, ps, u1[c], dlt);
-        vel[c] = flip * (vel[c] + dlt) + (1.f - flip) * u1[c];
+        vel[c] = flip_blend(flip, vel[c], dlt, u1[c]);
     }
     float xn[3];
     advect_from(x0, u1, U, V, W, solid, g, dt, margin, box, xn);
@@ -1512,6 +1589,83 @@ __global__ void __launch_bounds__(256) g2p_advect_kernel(uint32_t n, float4* __r
     b.x = vel[1];
     b.y = vel[2];
     rec[2 * p + 1] = b;
+}
+
+// G2P + RK2 by brick, for the step (the particles sorted by brick, the
+// touched-brick list of the fused transfer): a workgroup copies the faces
+// of a brick's cells and their one-cell shell (10^3 cells, the six fields,
+// 24 KB) into LDS once, then interpolates for the brick's particles from
+// LDS -- a particle's G2P stencil lies in its brick's region, and the RK2
+// midpoint's usually does (else it reads the grid) -- instead of gathering
+// every corner from memory.  The same results as g2p_advect_kernel (the
+// same weights, corners and summation order over the same face values).
+// Workgroups walk the list like p2g_prepare_kernel (64-entry chunks dealt
+// to the XCDs round robin).
+__global__ void __launch_bounds__(256)
+g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                        const uint32_t* __restrict__ cellStart, float4* __restrict__ rec, const float* __restrict__ U,
+                        const float* __restrict__ V, const float* __restrict__ W, const float* __restrict__ U0,
+                        const float* __restrict__ V0, const float* __restrict__ W0, const uint8_t* __restrict__ solid,
+                        Grid g, float flip, float dt, float margin, SolidBox box) {
+    __shared__ float sF[6][kRegion * kRegion * kRegion];
+    const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
+    const uint32_t total = *count;
+    const uint32_t perXcdGroups = gridDim.x >> 3, xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
+    const float* Fg[6] = {U, V, W, U0, V0, W0};
+    const float* F[3] = {U, V, W};
+    const float* F0[3] = {U0, V0, W0};
+    for (uint32_t m = rank;; m += perXcdGroups) {
+        const uint32_t e = ((m >> 6) * 8u + xcd) * 64u + (m & 63u);
+        if (e >= total) break;
+        const uint32_t brick = list[e];
+        const uint32_t p0 = cellStart[brick * 512u], p1 = cellStart[brick * 512u + 512u];
+        if (p0 == p1) continue;  // a touched brick without particles
+        const int org[3] = {(int)(brick % nbX) * 8 - 1, (int)((brick / nbX) % nbY) * 8 - 1,
+                            (int)(brick / (nbX * nbY)) * 8 - 1};
+        for (uint32_t t = threadIdx.x; t < (uint32_t)(kRegion * kRegion * kRegion); t += 256u) {
+            const int x = org[0] + (int)(t % kRegion), y = org[1] + (int)((t / kRegion) % kRegion),
+                      z = org[2] + (int)(t / (kRegion * kRegion));
+            const bool in = in_domain(x, y, z, g);
+            const uint32_t idx = in ? tiled((uint32_t)x, (uint32_t)y, (uint32_t)z, nbX, nbY) : 0u;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) sF[a][t] = in ? Fg[a][idx] : 0.f;
+        }
+        __syncthreads();
+        for (uint32_t p = p0 + threadIdx.x; p < p1; p += 256u) {
+            float4 a = rec[2 * p], b = rec[2 * p + 1];
+            const float x0[3] = {a.x, a.y, a.z};
+            double q[3];
+            to_cell_units(x0[0], x0[1], x0[2], g, q);
+            PosStencil ps;
+            const bool local = region_stencil(q, g, org, ps);
+            if (!local) pos_stencil(q, g, nbX, nbY, ps);
+            float vel[3] = {a.w, b.x, b.y}, u1[3];
+#pragma unroll 1
+            for (int c = 0; c < 3; ++c) {
+                float dlt;
+                if (local)
+                    interp_pic_delta(sF[c], sF[3 + c], c, ps, u1[c], dlt);
+                else
+                    interp_pic_delta(F[c], F0[c], c, ps, u1[c], dlt);
+                vel[c] = flip_blend(flip, vel[c], dlt, u1[c]);
+            }
+            double qm[3];
+            advect_midpoint(x0, u1, g, dt, qm);
+            PosStencil pm;
+            const bool localM = region_stencil(qm, g, org, pm);
+            if (!localM) pos_stencil(qm, g, nbX, nbY, pm);
+            float u2[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) u2[c] = localM ? interp(sF[c], c, pm) : interp(F[c], c, pm);
+            float xn[3];
+            advect_finish(x0, u2, solid, g, dt, margin, box, xn);
+            rec[2 * p] = make_float4(xn[0], xn[1], xn[2], vel[0]);
+            b.x = vel[1];
+            b.y = vel[2];
+            rec[2 * p + 1] = b;
+        }
+        __syncthreads();  // the next brick overwrites the region
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1652,49 +1806,78 @@ void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, 
 namespace {
 int cg_reduce_threads(uint32_t blocks) { return blocks <= 2048u ? 256 : kReduceThreads; }
 }  // namespace
-uint32_t cg_partials_needed(uint32_t na) { return 3u * (na < kCgGroupedBelow ? (na + 3u) / 4u : na); }
+namespace {
+// Waves (slots) per workgroup of the grouped shape for na slots; 0 for the
+// one-slot shape with reduction kernels (at 8042 slots, C3: 76 us per
+// iteration against 84 with four slots and 90 with eight per workgroup).
+uint32_t cg_group_waves(uint32_t na) { return na < kCgGroupedBelow ? 4u : 0u; }
+
+template <uint32_t W>
+void launch_matvec_grouped(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew, hipStream_t s) {
+    FLIP2_KERNEL(cg_direction_matvec_kernel<W>, xcd_grid((c.na + W - 1u) / W), dim3(64 * W), 0, s, c, it,
+                 first ? 1 : 0, pOld, pNew);
+    FLIP2_LAUNCH_CHECK();
+}
+template <uint32_t W>
+void launch_update_grouped(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
+                           hipStream_t s) {
+    const dim3 grid = xcd_grid((c.na + W - 1u) / W), block(64 * W);
+    if (mic && init)
+        FLIP2_KERNEL((cg_update_mic_kernel<W, 1>), grid, block, 0, s, c, it, r64, p);
+    else if (mic)
+        FLIP2_KERNEL((cg_update_mic_kernel<W, 0>), grid, block, 0, s, c, it, r64, p);
+    else if (init)
+        FLIP2_KERNEL((cg_update_jacobi_kernel<W, 1>), grid, block, 0, s, c, it, r64, p);
+    else
+        FLIP2_KERNEL((cg_update_jacobi_kernel<W, 0>), grid, block, 0, s, c, it, r64, p);
+    FLIP2_LAUNCH_CHECK();
+}
+}  // namespace
+
+uint32_t cg_partials_needed(uint32_t na) {
+    const uint32_t w = cg_group_waves(na);
+    return 3u * (w ? (na + w - 1u) / w : na);
+}
 void launch_cg_direction_matvec(const CgFused& c, uint32_t it, bool first, const float* pOld, float* pNew,
                                 hipStream_t s) {
     FLIP2_REQUIRE(c.na > 0 && c.stop, kErrArgument, "PCG: no active slots or no convergence flag");
-    if (c.na < kCgGroupedBelow) {
-        FLIP2_KERNEL(cg_direction_matvec_kernel<4>, xcd_grid((c.na + 3u) / 4u), dim3(256), 0, s, c, it, first ? 1 : 0,
-                     pOld, pNew);
-        FLIP2_LAUNCH_CHECK();
-    } else {
-        FLIP2_KERNEL(cg_direction_matvec_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, first ? 1 : 0, pOld,
-                     pNew);
-        FLIP2_LAUNCH_CHECK();
-        FLIP2_KERNEL(cg_reduce_kernel, dim3(1), dim3(cg_reduce_threads(c.na)), 0, s, c.partials, c.na,
-                     c.hist + 4u * it + 2u, c.stop, c.thr2, 0);  // (p.q)_it
-        FLIP2_LAUNCH_CHECK();
+    switch (cg_group_waves(c.na)) {
+    case 4: launch_matvec_grouped<4>(c, it, first, pOld, pNew, s); return;
+    default: break;
     }
+    FLIP2_KERNEL(cg_direction_matvec_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, first ? 1 : 0, pOld, pNew);
+    FLIP2_LAUNCH_CHECK();
+    FLIP2_KERNEL(cg_reduce_kernel, dim3(1), dim3(cg_reduce_threads(c.na)), 0, s, c.partials, c.na,
+                 c.hist + 4u * it + 2u, c.stop, c.thr2, 0);  // (p.q)_it
+    FLIP2_LAUNCH_CHECK();
 }
 void launch_cg_update_precond(const CgFused& c, uint32_t it, bool init, bool mic, const double* r64, const float* p,
                               hipStream_t s) {
     FLIP2_REQUIRE(c.na > 0 && c.stop, kErrArgument, "PCG: no active slots or no convergence flag");
-    const int in = init ? 1 : 0;
-    if (c.na < kCgGroupedBelow) {
-        const dim3 grid = xcd_grid((c.na + 3u) / 4u);
-        if (mic)
-            FLIP2_KERNEL(cg_update_mic_kernel<4>, grid, dim3(256), 0, s, c, it, in, r64, p);
-        else
-            FLIP2_KERNEL(cg_update_jacobi_kernel<4>, grid, dim3(256), 0, s, c, it, in, r64, p);
-        FLIP2_LAUNCH_CHECK();
-    } else {
-        if (mic)
-            FLIP2_KERNEL(cg_update_mic_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, in, r64, p);
-        else
-            FLIP2_KERNEL(cg_update_jacobi_kernel<1>, xcd_grid(c.na), dim3(64), 0, s, c, it, in, r64, p);
-        FLIP2_LAUNCH_CHECK();
-        // rz, rr of the next entry (partials at [na, 3 na))
-        FLIP2_KERNEL(cg_reduce_kernel, dim3(2), dim3(cg_reduce_threads(c.na)), 0, s, c.partials + c.na, c.na,
-                     c.hist + 4u * (init ? it : it + 1u), c.stop, c.thr2, 1);
-        FLIP2_LAUNCH_CHECK();
+    switch (cg_group_waves(c.na)) {
+    case 4: launch_update_grouped<4>(c, it, init, mic, r64, p, s); return;
+    default: break;
     }
+    const dim3 grid = xcd_grid(c.na);
+    if (mic && init)
+        FLIP2_KERNEL((cg_update_mic_kernel<1, 1>), grid, dim3(64), 0, s, c, it, r64, p);
+    else if (mic)
+        FLIP2_KERNEL((cg_update_mic_kernel<1, 0>), grid, dim3(64), 0, s, c, it, r64, p);
+    else if (init)
+        FLIP2_KERNEL((cg_update_jacobi_kernel<1, 1>), grid, dim3(64), 0, s, c, it, r64, p);
+    else
+        FLIP2_KERNEL((cg_update_jacobi_kernel<1, 0>), grid, dim3(64), 0, s, c, it, r64, p);
+    FLIP2_LAUNCH_CHECK();
+    // rz, rr of the next entry (partials at [na, 3 na))
+    FLIP2_KERNEL(cg_reduce_kernel, dim3(2), dim3(cg_reduce_threads(c.na)), 0, s, c.partials + c.na, c.na,
+                 c.hist + 4u * (init ? it : it + 1u), c.stop, c.thr2, 1);
+    FLIP2_LAUNCH_CHECK();
 }
 void launch_cg_history(const CgFused& c, uint32_t it, hipStream_t s) {
-    if (c.na >= kCgGroupedBelow) return;  // the reduction after each update recorded it
-    FLIP2_KERNEL(cg_history_kernel, dim3(1), dim3(256), 0, s, c, it);
+    switch (cg_group_waves(c.na)) {
+    case 4: FLIP2_KERNEL(cg_history_kernel<4>, dim3(1), dim3(256), 0, s, c, it); break;
+    default: return;  // the reduction after each update recorded it
+    }
     FLIP2_LAUNCH_CHECK();
 }
 void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
@@ -1727,6 +1910,17 @@ void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, 
                        Grid g, float dt, float margin, SolidBox box, hipStream_t s) {
     if (!n) return;
     FLIP2_KERNEL(advect_rk2_kernel, xcd_grid(ceil_div(n, 256)), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* cellStart, float4* rec,
+                              const float* U, const float* V, const float* W, const float* U0, const float* V0,
+                              const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
+                              SolidBox box, hipStream_t s) {
+    const uint32_t numBricks = (g.sizeX >> 3) * (g.sizeY >> 3) * (g.sizeZ >> 3);
+    // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
+    const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
+    FLIP2_KERNEL(g2p_advect_brick_kernel, dim3(groups), dim3(256), 0, s, list, count, cellStart, rec, U, V, W, U0, V0,
+                 W0, solid, g, flip, dt, margin, box);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_g2p_advect(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,

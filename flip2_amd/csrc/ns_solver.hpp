@@ -120,6 +120,13 @@ void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, co
                      const float* V0, const float* W0, Grid g, float flip, hipStream_t s);
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
                        Grid g, float dt, float margin, SolidBox box, hipStream_t s);
+// launch_g2p_advect by brick over the fused transfer's touched-brick list
+// (list, count: entries and their number on the device; cellStart: the
+// transfer's per-cell offsets of the sorted records); the same results.
+void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* cellStart, float4* rec,
+                              const float* U, const float* V, const float* W, const float* U0, const float* V0,
+                              const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
+                              SolidBox box, hipStream_t s);
 // launch_g2p_flip then launch_advect_rk2 (same results) in one kernel
 void launch_g2p_advect(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
                        const float* V0, const float* W0, const uint8_t* solid, Grid g, float flip, float dt,

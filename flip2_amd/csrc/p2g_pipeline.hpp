@@ -43,6 +43,10 @@ public:
     int timedRuns() const { return timedRuns_; }
     void resetTiming();
     const uint32_t* cellStart() const { return cellStart_.devPtr(); }
+    // The last fused run's touched bricks: list and (device) count
+    // (valid after a runRecords with FacePrep).
+    const uint32_t* touchedList() const { return touchedList_.devPtr(); }
+    const uint32_t* touchedCount() const { return touchedList_.devPtr() + (numCells_ >> 9); }
     const uint32_t* counts() const { return counts_.devPtr(); }
     // max |velocity component| of the last run's particles (float bits)
     const uint32_t* maxSpeedBits() const { return stats_.devPtr(); }

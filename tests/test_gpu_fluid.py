@@ -186,15 +186,19 @@ def test_fused_transfer_prepare_bit_identical(flip2, oracle, solid):
 
 @pytest.mark.parametrize("box", [False, True])
 def test_fused_g2p_advect_bit_identical(flip2, box):
-    """The step's fused G2P + RK2 kernel gives the records of the g2p stage
-    followed by the advect stage, bit for bit (with and without a moving
-    solid box, whose push-out and return-to-start branches it shares)."""
+    """The step's fused G2P + RK2 (by brick, from LDS copies of the faces,
+    after the fused transfer) gives the records of the g2p stage followed by
+    the advect stage, bit for bit (with and without a moving solid box,
+    whose push-out and return-to-start branches it shares; the edge
+    particles' fast motion takes some RK2 midpoints outside their brick's
+    region, to the grid fallback)."""
     n = 32
     h = 1.0 / n
     x, y, z = scenes.dam_break(n, seed=21)
     x = np.append(x, np.float32(0.2)); y = np.append(y, np.float32(0.1)); z = np.append(z, np.float32(0.5))
     rng = np.random.default_rng(5)
     v = [(rng.standard_normal(x.size) * 0.5).astype(np.float32) for _ in range(3)]
+    v[0][:2000] = 60.0  # a fast stretch: its grid velocity moves RK2 midpoints past the brick's region
     grid = (0.0, 0.0, 0.0, n, n, n, h)
     out = []
     for fused in (False, True):
@@ -202,7 +206,9 @@ def test_fused_g2p_advect_bit_identical(flip2, box):
         sim.set_particles(x, y, z, *v)
         if box:
             sim.set_solid_box((0.0, 0.0, 0.0), (0.25, 1.0, 1.0), (0.5, 0.0, 0.0))
-        for s in ("transfer", "prepare", "system", "solve", "project"):
+        # the fused path as the step runs it: after the fused transfer the
+        # G2P + RK2 kernel works by brick from LDS copies of the faces
+        for s in (("transfer_prepare",) if fused else ("transfer", "prepare")) + ("system", "solve", "project"):
             sim.stage(s)
         if fused:
             sim.stage("g2p_advect")
