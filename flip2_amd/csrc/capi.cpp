@@ -1050,9 +1050,10 @@ int flip2_fluid_set_unet(flip2_fluid* f, flip2_unet* net, uint32_t fallback_iter
         flip2::UNet* u = &net->net;
         flip2::FluidSim* s = &sim;
         sim.setLearnedPreconditioner(
-            [u, s](uint32_t na, const float* r, float* z, bool newSystem, const double* pairs, hipStream_t st) {
+            [u, s](uint32_t na, const float* r, float* z, bool newSystem, const double* pairs, hipStream_t st,
+                   const uint32_t* stop) {
                 u->applyActive(na, s->activeBricksDev(), s->brickSlotDev(), s->cellTypeDev(), r, z, s->gridN(),
-                               newSystem, pairs, st);
+                               newSystem, pairs, st, stop);
             });
     });
 }

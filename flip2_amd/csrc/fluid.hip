@@ -12,6 +12,30 @@
 namespace flip2 {
 
 namespace {
+// The residual trend of a refinement's checked history entries: the
+// per-iteration rate of rr between the last two checked iterations, used to
+// guess whether the batch in flight reaches the threshold.  Such a batch is
+// checked before the next one is queued: work queued behind a converged
+// batch returns at once on the device but still costs its launches.
+struct RrTrend {
+    double last = -1.0, rate = 1.0;
+    uint32_t lastIt = 0;
+    void see(double rr, uint32_t it) {
+        if (!(rr > 0.0)) return;  // zero or NaN: the batch check decides
+        if (last > 0.0 && it > lastIt) rate = std::pow(rr / last, 1.0 / (double)(it - lastIt));
+        last = rr;
+        lastIt = it;
+    }
+    // rr extrapolated to iteration `it` lies within a factor 4 of thr2 (no
+    // when the trend is unknown or not decreasing)
+    bool likely(uint32_t it, double thr2) const {
+        if (!(last > 0.0) || it <= lastIt || !(rate < 1.0)) return false;
+        return last * std::pow(rate, (double)(it - lastIt)) <= 4.0 * thr2;
+    }
+};
+}  // namespace
+
+namespace {
 // Largest accepted inner-iteration cap and residual-check interval: the
 // solver keeps 4 doubles of history per inner iteration on the device and
 // reads (check interval + 1) x 4 of them back per check.
@@ -317,16 +341,17 @@ void FluidSim::ensureRing(uint32_t slots) {
 // a damped-Jacobi pre-/post-smoothing step.
 void FluidSim::applyLearnedEager(uint32_t na, bool newSystem) {
     const float w = cfg_.learnedSmoothing;
+    const uint32_t* stop = stop_.devPtr();
     if (w > 0.f) {
         // the partial sums of the network input's scale come with the smoothing
         double* pairs = partials_.devPtr();
         ns::launch_presmooth(na, r_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(), w, aux0_.devPtr(),
-                             pairs, stream_);
-        learned_(na, aux0_.devPtr(), aux1_.devPtr(), newSystem, pairs, stream_);
+                             pairs, stream_, stop);
+        learned_(na, aux0_.devPtr(), aux1_.devPtr(), newSystem, pairs, stream_, stop);
         ns::launch_postsmooth(na, r_.devPtr(), aux1_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), w, z_.devPtr(),
-                              stream_);
+                              stream_, stop);
     } else {
-        learned_(na, r_.devPtr(), z_.devPtr(), newSystem, nullptr, stream_);
+        learned_(na, r_.devPtr(), z_.devPtr(), newSystem, nullptr, stream_, stop);
     }
 }
 
@@ -370,12 +395,6 @@ double FluidSim::readScalar(const double* dev) {
     FLIP2_HIP(hipMemcpyAsync(&v, dev, sizeof(v), hipMemcpyDeviceToHost, stream_));
     synchronize();
     return v;
-}
-
-void FluidSim::readHistory(uint32_t first, uint32_t count, double* host) {
-    FLIP2_HIP(hipMemcpyAsync(host, hist_.devPtr() + 4u * first, 4u * count * sizeof(double), hipMemcpyDeviceToHost,
-                             stream_));
-    synchronize();
 }
 
 void FluidSim::transfer() {
@@ -500,7 +519,6 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     uint32_t total = 0;
     const uint32_t cap = std::min(capIn, cfg_.maxIterations);
     const uint32_t every = std::max(1u, flex ? cfg_.learnedCheckEvery : cfg_.checkEvery);
-    std::vector<double> h(4 * ((size_t)every + 1));
     double* H = hist_.devPtr();
     // Flexible CG does not restart at a refinement: the recursive residual is
     // replaced by the true one (FP64, rounded to FP32), the correction so far
@@ -511,16 +529,6 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     for (uint32_t refine = 0; refine < 12 && total < cap; ++refine) {
         const double innerTol = std::max(0.5 * cfg_.tolerance / rel, 2e-5);
         const double ref = rel * st.bNorm;  // norm of the FP32 right-hand side of this refinement
-        // the residual norms of iterations first + 1 .. first + count (read
-        // back from the history) reached the inner tolerance
-        auto converged = [&](uint32_t first, uint32_t count) {
-            readHistory(first + 1, count, h.data());
-            for (uint32_t k = 0; k < count; ++k) {
-                const double rr = h[4 * k + 1];
-                if (!(rr == rr) || std::sqrt(std::max(rr, 0.0)) <= innerTol * ref) return true;
-            }
-            return false;
-        };
         if (!flex) {
             // Jacobi / MIC(0): two kernels per iteration (direction + matvec,
             // update + preconditioner), p alternating between two buffers;
@@ -574,6 +582,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                 return b;
             };
             uint32_t doneAt = UINT32_MAX;  // the converged entry
+            RrTrend trend;
             auto convergedIn = [&](const Batch& b) {
                 FLIP2_HIP(hipEventSynchronize(histEvent_[b.slot]));
                 const double* hb = pinned + b.slot * stride;
@@ -583,6 +592,7 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                         doneAt = b.first + k;
                         return true;
                     }
+                    trend.see(rr, b.first + k);
                 }
                 return false;
             };
@@ -590,6 +600,11 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
                 Batch cur = enqueue(0);
                 while (true) {
                     const bool haveNext = queued < limit;
+                    if (haveNext && trend.likely(cur.first + cur.count, cg.thr2)) {
+                        if (convergedIn(cur)) break;  // checked before more is queued
+                        cur = enqueue(cur.slot ^ 1);
+                        continue;
+                    }
                     Batch next{};
                     if (haveNext) next = enqueue(cur.slot ^ 1);
                     if (convergedIn(cur) || !haveNext) break;  // converged, or the cap
@@ -598,58 +613,101 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
             }
             total += doneAt != UINT32_MAX ? doneAt : queued;
         } else {
-            // flexible CG with the learned preconditioner
+            // flexible CG with the learned preconditioner.  Its kernels (the
+            // apply's included) take the converged flag: the update's sum of
+            // rr sets it when rr <= thr2, and every kernel queued after that
+            // returns at once, so iterations are queued a batch ahead of the
+            // host's check as in the PCG branch.  refine_start clears the flag.
+            ns::Reduce fr = reduce();
+            fr.stop = stop_.devPtr();
+            const double thr = innerTol * ref;
+            fr.thr2 = thr * thr;
             ns::launch_refine_start(na, r64_.devPtr(), r_.devPtr(), e_.devPtr(), coef_.devPtr(), reduce(),
-                                    scal_.devPtr() + 1, stream_);
+                                    scal_.devPtr() + 1, stream_, stop_.devPtr());
             if (refine == 0 && captureAt(0)) capture_(na, r_.devPtr(), stream_);
             if (itCarry == 0) {
                 // p_0 = z_0; H[0] = p_0 . r_0 (the step-length numerator)
                 applyLearned(na);
                 ns::FlexDirections none{};
-                ns::launch_flex_direction(na, z_.devPtr(), none, zq_.devPtr(), r_.devPtr(), ringP(0), reduce(), H,
-                                          stream_);
+                ns::launch_flex_direction(na, z_.devPtr(), none, zq_.devPtr(), r_.devPtr(), ringP(0), fr, H, stream_);
             } else {
                 // the apply and direction of iteration itCarry, on the replaced
                 // residual: p = z - sum_i (z . q_i / p_i . q_i) p_i
                 applyLearned(na);
                 const ns::FlexDirections d = kept(itCarry - 1, H);
-                ns::launch_flex_dots(na, z_.devPtr(), d, reduce(), zq_.devPtr(), stream_);
-                ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(itCarry), reduce(),
+                ns::launch_flex_dots(na, z_.devPtr(), d, fr, zq_.devPtr(), stream_);
+                ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(itCarry), fr,
                                           H + 4u * itCarry, stream_);
             }
             const uint32_t it0 = itCarry;
-            uint32_t it = it0;
-            bool innerDone = false;
-            while (!innerDone && total + (it - it0) < cap) {
-                const uint32_t batch = std::min(every, cap - total - (it - it0));
-                const uint32_t first = it;
-                for (uint32_t k = 0; k < batch; ++k, ++it) {
+            const uint32_t limit = cap - total;
+            const size_t stride = 4u * (size_t)every;
+            double* pinned = pinnedHistory(2 * stride);
+            struct Batch {
+                uint32_t first, count;
+                int slot;
+            };
+            uint32_t queued = 0;  // iterations of this refinement queued so far
+            auto enqueue = [&](int slot) {
+                const Batch b{queued, std::min(every, limit - queued), slot};
+                for (uint32_t k = 0; k < b.count; ++k) {
+                    const uint32_t it = it0 + queued;
                     double* hk = H + 4u * it;
                     double* hn = H + 4u * (it + 1);
-                    ns::launch_matvec_dot(na, ringP(it), coef_.devPtr(), nbrSlot_.devPtr(), ringQ(it), reduce(),
-                                          hk + 2, stream_);
+                    ns::launch_matvec_dot(na, ringP(it), coef_.devPtr(), nbrSlot_.devPtr(), ringQ(it), fr, hk + 2,
+                                          stream_);
                     ns::launch_update(na, hk, hk + 2, ringP(it), ringQ(it), coef_.devPtr(), e_.devPtr(), r_.devPtr(),
-                                      z_.devPtr(), false, reduce(), hn, stream_);
-                    if (k + 1 == batch && converged(first, batch)) {
-                        // checked before the apply, the costly part of a
-                        // flexible-CG iteration: after convergence it is not needed
-                        innerDone = true;
-                        if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
-                        ++it;
-                        break;
-                    }
+                                      z_.devPtr(), false, fr, hn, stream_);
+                    ++queued;
+                    if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
+                    if (queued == limit) break;  // no direction after the cap's last iteration
                     // p = z - sum_i (z . q_i / p_i . q_i) p_i over the kept
                     // directions; hn[0] = p . r
                     applyLearned(na);
                     const ns::FlexDirections d = kept(it, H);
-                    ns::launch_flex_dots(na, z_.devPtr(), d, reduce(), zq_.devPtr(), stream_);
-                    ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(it + 1), reduce(),
-                                              hn, stream_);
-                    if (refine == 0 && captureAt(it + 1)) capture_(na, r_.devPtr(), stream_);
+                    ns::launch_flex_dots(na, z_.devPtr(), d, fr, zq_.devPtr(), stream_);
+                    ns::launch_flex_direction(na, z_.devPtr(), d, zq_.devPtr(), r_.devPtr(), ringP(it + 1), fr, hn,
+                                              stream_);
+                }
+                // the batch's residual entries it0 + first + 1 .. it0 + first + count
+                FLIP2_HIP(hipMemcpyAsync(pinned + slot * stride, H + 4u * (it0 + b.first + 1u),
+                                         4u * b.count * sizeof(double), hipMemcpyDeviceToHost, stream_));
+                FLIP2_HIP(hipEventRecord(histEvent_[slot], stream_));
+                return b;
+            };
+            uint32_t done = UINT32_MAX;  // iterations up to the converged entry
+            RrTrend trend;
+            auto convergedIn = [&](const Batch& b) {
+                FLIP2_HIP(hipEventSynchronize(histEvent_[b.slot]));
+                const double* hb = pinned + b.slot * stride;
+                for (uint32_t k = 0; k < b.count; ++k) {
+                    const double rr = hb[4 * k + 1];
+                    if (rr <= fr.thr2 || !(rr == rr)) {
+                        done = b.first + k + 1;
+                        return true;
+                    }
+                    trend.see(rr, b.first + k + 1);
+                }
+                return false;
+            };
+            if (limit > 0) {
+                Batch cur = enqueue(0);
+                while (true) {
+                    const bool haveNext = queued < limit;
+                    if (haveNext && trend.likely(cur.first + cur.count, fr.thr2)) {
+                        if (convergedIn(cur)) break;  // checked before more is queued
+                        cur = enqueue(cur.slot ^ 1);
+                        continue;
+                    }
+                    Batch next{};
+                    if (haveNext) next = enqueue(cur.slot ^ 1);
+                    if (convergedIn(cur) || !haveNext) break;  // converged, or the cap
+                    cur = next;
                 }
             }
-            total += it - it0;
-            itCarry = it;
+            if (done == UINT32_MAX) done = queued;
+            total += done;
+            itCarry = it0 + done;
         }
         ns::launch_accumulate(na, e_.devPtr(), x64_.devPtr(), coef_.devPtr(), stream_);
         ns::launch_true_residual(na, b64_.devPtr(), x64_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), r64_.devPtr(),

@@ -66,8 +66,11 @@ struct SolveStats {
 // applies of a solve launch the same kernels on the same buffers, so the
 // solver may replay them from a HIP graph.  normPairs: null, or per slot
 // (sum of r^2, count) over the fluid cells, already computed by the solver.
+// stop: the solver's converged flag; while it is set the apply's kernels
+// return at once (the solver queues iterations ahead of its convergence
+// check).
 using PrecondFn = std::function<void(uint32_t numActive, const float* r, float* z, bool newSystem,
-                                     const double* normPairs, hipStream_t s)>;
+                                     const double* normPairs, hipStream_t s, const uint32_t* stop)>;
 // Receives the residual r_k = b - A x_k (FP32, active-brick layout) of a
 // solve's first refinement after k inner iterations (training data).
 using CaptureFn = std::function<void(uint32_t numActive, const float* r, hipStream_t s)>;
@@ -156,7 +159,6 @@ private:
     void ensureSlots(uint32_t numActive);
     ns::Reduce reduce() { return ns::Reduce{partials_.devPtr()}; }
     double readScalar(const double* dev);
-    void readHistory(uint32_t first, uint32_t count, double* host);
 
     FluidConfig cfg_;
     DeviceManager dm_;  // the pool of every device array below (declared first: outlives them)

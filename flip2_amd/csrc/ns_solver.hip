@@ -350,8 +350,9 @@ reduce_partials_kernel(const double* __restrict__ partials, uint32_t blocks, dou
 
 __global__ void __launch_bounds__(kWG)
 refine_start_kernel(const double* __restrict__ r64, float* __restrict__ r, float* __restrict__ e,
-                    const uint16_t* __restrict__ coef, double* __restrict__ partials) {
+                    const uint16_t* __restrict__ coef, double* __restrict__ partials, uint32_t* resetStop) {
     __shared__ double s_tmp[16];
+    if (resetStop && blockIdx.x == 0 && threadIdx.x == 0) *resetStop = 0u;
     const uint32_t base = blockIdx.x * 512u;
     double rr = 0.0;
     for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
@@ -367,8 +368,9 @@ refine_start_kernel(const double* __restrict__ r64, float* __restrict__ r, float
 // q = A p; pq into out[0].
 __global__ void __launch_bounds__(kWG)
 matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
-                  float* __restrict__ q, double* __restrict__ partials, uint32_t na) {
+                  float* __restrict__ q, double* __restrict__ partials, uint32_t na, const uint32_t* stop) {
     __shared__ double s_tmp[16];
+    if (stop && *stop) return;
     const uint32_t slot = xcd_slot(na);
     if (slot >= na) return;
     double pq = 0.0;
@@ -387,8 +389,10 @@ matvec_dot_kernel(const float* __restrict__ p, const uint16_t* __restrict__ coef
 __global__ void __launch_bounds__(kWG)
 update_kernel(const double* __restrict__ rzOld, const double* __restrict__ pq, const float* __restrict__ p,
               const float* __restrict__ q, const uint16_t* __restrict__ coef, float* __restrict__ e,
-              float* __restrict__ r, float* __restrict__ z, int jacobi, double* __restrict__ partials /* [rz_new, rr_new] per workgroup */) {
+              float* __restrict__ r, float* __restrict__ z, int jacobi, double* __restrict__ partials /* [rz_new, rr_new] per workgroup */,
+              const uint32_t* stop) {
     __shared__ double s_tmp[16];
+    if (stop && *stop) return;
     const double den = *pq;
     const float alpha = (den != 0.0) ? (float)(*rzOld / den) : 0.f;
     const uint32_t base = blockIdx.x * 512u;
@@ -423,9 +427,10 @@ update_kernel(const double* __restrict__ rzOld, const double* __restrict__ pq, c
 __global__ void __launch_bounds__(kWG)
 presmooth_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
                  const uint8_t* __restrict__ cellType, float omega, float* __restrict__ r1,
-                 double* __restrict__ pairs) {
+                 double* __restrict__ pairs, const uint32_t* stop) {
     __shared__ float sx[512];
     __shared__ double s_tmp[16];
+    if (stop && *stop) return;
     const uint32_t slot = blockIdx.x;
     auto jac = [&](uint32_t i) {
         const uint16_t c = coef[i];
@@ -457,8 +462,9 @@ presmooth_kernel(const float* __restrict__ r, const uint16_t* __restrict__ coef,
 
 __global__ void __launch_bounds__(kWG)
 postsmooth_kernel(const float* __restrict__ r, const float* __restrict__ y, const uint16_t* __restrict__ coef,
-                  const int32_t* __restrict__ nbrSlot, float omega, float* __restrict__ z) {
+                  const int32_t* __restrict__ nbrSlot, float omega, float* __restrict__ z, const uint32_t* stop) {
     __shared__ float sx[512];
+    if (stop && *stop) return;
     const uint32_t slot = blockIdx.x;
     auto xf = [&](uint32_t i) {
         const uint16_t c = coef[i];
@@ -481,7 +487,9 @@ postsmooth_kernel(const float* __restrict__ r, const float* __restrict__ y, cons
 
 template <int N>  // kept directions (d.count == N)
 __global__ void __launch_bounds__(kWG)
-flex_dots_kernel(const float* __restrict__ z, FlexDirections d, double* __restrict__ partials) {
+flex_dots_kernel(const float* __restrict__ z, FlexDirections d, double* __restrict__ partials,
+                 const uint32_t* stop) {
+    if (stop && *stop) return;
     const uint32_t base = blockIdx.x * 512u;
     double acc[N];
 #pragma unroll
@@ -517,8 +525,10 @@ flex_dots_kernel(const float* __restrict__ z, FlexDirections d, double* __restri
 template <int N>  // kept directions (d.count == N, 0 for p = z)
 __global__ void __launch_bounds__(kWG)
 flex_direction_kernel(const float* __restrict__ z, FlexDirections d, const double* __restrict__ zq,
-                      const float* __restrict__ r, float* __restrict__ p, double* __restrict__ partials) {
+                      const float* __restrict__ r, float* __restrict__ p, double* __restrict__ partials,
+                      const uint32_t* stop) {
     __shared__ double s_tmp[16];
+    if (stop && *stop) return;
     float beta[N > 0 ? N : 1];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -1676,10 +1686,16 @@ inline dim3 g1(uint32_t n, uint32_t bs) { return dim3(ceil_div(n ? n : 1, bs)); 
 // grid of a kernel that maps workgroups to slots with xcd_slot
 inline dim3 xcd_grid(uint32_t na) { return dim3(8u * ceil_div(na ? na : 1, 8)); }
 // out[0..count) = sums of the workgroup partials of the kernel just launched
-void reduce_to(const Reduce& red, int count, uint32_t blocks, double* out, hipStream_t s) {
+// With red.stop: skipped once converged (cg_reduce_kernel); checkRR: sum 1
+// is a residual norm that sets the flag.
+void reduce_to(const Reduce& red, int count, uint32_t blocks, double* out, hipStream_t s, bool checkRR = false) {
     // one load per thread per 8 partials: a small sum needs fewer threads
     const int threads = blocks <= 2048u ? 256 : kReduceThreads;
-    FLIP2_KERNEL(reduce_partials_kernel, dim3(count), dim3(threads), 0, s, red.partials, blocks, out);
+    if (red.stop)
+        FLIP2_KERNEL(cg_reduce_kernel, dim3(count), dim3(threads), 0, s, red.partials, blocks, out, red.stop, red.thr2,
+                     checkRR ? 1 : 0);
+    else
+        FLIP2_KERNEL(reduce_partials_kernel, dim3(count), dim3(threads), 0, s, red.partials, blocks, out);
     FLIP2_LAUNCH_CHECK();
 }
 }  // namespace
@@ -1713,15 +1729,16 @@ void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int
     reduce_to(red, 2, numActive, out, s);
 }
 void launch_refine_start(uint32_t numActive, const double* r64, float* r, float* e, const uint16_t* coef, Reduce red,
-                         double* out, hipStream_t s) {
-    FLIP2_KERNEL(refine_start_kernel, dim3(numActive), dim3(kWG), 0, s, r64, r, e, coef, red.partials);
+                         double* out, hipStream_t s, uint32_t* resetStop) {
+    FLIP2_KERNEL(refine_start_kernel, dim3(numActive), dim3(kWG), 0, s, r64, r, e, coef, red.partials, resetStop);
     FLIP2_LAUNCH_CHECK();
+    red.stop = nullptr;  // the refinement's first sum is always taken
     reduce_to(red, 1, numActive, out, s);
 }
 void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
                        Reduce red, double* out, hipStream_t s) {
     FLIP2_KERNEL(matvec_dot_kernel, xcd_grid(numActive), dim3(kWG), 0, s, p, coef, nbrSlot, q, red.partials,
-                 numActive);
+                 numActive, red.stop);
     FLIP2_LAUNCH_CHECK();
     reduce_to(red, 1, numActive, out, s);
 }
@@ -1729,19 +1746,20 @@ void launch_update(uint32_t numActive, const double* rzOld, const double* pq, co
                    const uint16_t* coef, float* e, float* r, float* z, bool jacobi, Reduce red, double* out,
                    hipStream_t s) {
     FLIP2_KERNEL(update_kernel, dim3(numActive), dim3(kWG), 0, s, rzOld, pq, p, q, coef, e, r, z,
-                       jacobi ? 1 : 0, red.partials);
+                       jacobi ? 1 : 0, red.partials, red.stop);
     FLIP2_LAUNCH_CHECK();
-    reduce_to(red, 2, numActive, out, s);
+    reduce_to(red, 2, numActive, out, s, true);
 }
 void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot,
-                      const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s) {
+                      const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s,
+                      const uint32_t* stop) {
     FLIP2_KERNEL(presmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, coef, nbrSlot, cellType, omega, r1,
-                       pairs);
+                       pairs, stop);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const uint16_t* coef, const int32_t* nbrSlot,
-                       float omega, float* z, hipStream_t s) {
-    FLIP2_KERNEL(postsmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, y, coef, nbrSlot, omega, z);
+                       float omega, float* z, hipStream_t s, const uint32_t* stop) {
+    FLIP2_KERNEL(postsmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, y, coef, nbrSlot, omega, z, stop);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& d, Reduce red, double* out,
@@ -1750,7 +1768,7 @@ void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& 
     if (d.count == 0) return;
     switch (d.count) {
 #define FLEX_CASE(N) \
-    case N: FLIP2_KERNEL(flex_dots_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, red.partials); break;
+    case N: FLIP2_KERNEL(flex_dots_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, red.partials, red.stop); break;
         FLEX_CASE(1) FLEX_CASE(2) FLEX_CASE(3) FLEX_CASE(4) FLEX_CASE(5) FLEX_CASE(6) FLEX_CASE(7) FLEX_CASE(8)
         FLEX_CASE(9) FLEX_CASE(10) FLEX_CASE(11) FLEX_CASE(12) FLEX_CASE(13) FLEX_CASE(14) FLEX_CASE(15) FLEX_CASE(16)
 #undef FLEX_CASE
@@ -1767,7 +1785,8 @@ void launch_flex_direction(uint32_t numActive, const float* z, const FlexDirecti
     switch (d.count) {
 #define FLEX_CASE(N)                                                                                              \
     case N:                                                                                                       \
-        FLIP2_KERNEL(flex_direction_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, zq, r, p, red.partials); \
+        FLIP2_KERNEL(flex_direction_kernel<N>, dim3(numActive), dim3(kWG), 0, s, z, d, zq, r, p, red.partials, \
+                     red.stop);                                                                                   \
         break;
         FLEX_CASE(0) FLEX_CASE(1) FLEX_CASE(2) FLEX_CASE(3) FLEX_CASE(4) FLEX_CASE(5) FLEX_CASE(6) FLEX_CASE(7)
         FLEX_CASE(8) FLEX_CASE(9) FLEX_CASE(10) FLEX_CASE(11) FLEX_CASE(12) FLEX_CASE(13) FLEX_CASE(14) FLEX_CASE(15)

@@ -14,6 +14,11 @@ constexpr uint8_t kSolid = 2;
 // quantity (a follow-up kernel sums them in workgroup order).
 struct Reduce {
     double* partials;
+    // Flexible CG's device-side convergence (null: off): the kernels return
+    // at once while *stop is set, the sums skip as cg_reduce_kernel does, and
+    // the update's residual sum sets *stop when it meets thr2 (or is NaN).
+    uint32_t* stop = nullptr;
+    double thr2 = 0.0;
 };
 
 // A moving solid box (e.g. a wave paddle): cells whose centre lies in
@@ -33,8 +38,10 @@ void launch_brick_setup(uint32_t numActive, const uint32_t* activeBricks, const 
 void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot, const uint8_t* cellType,
                      const uint8_t* solid, Grid g, double hOverDt, const float* U, const float* V, const float* W,
                      uint16_t* coef, double* b64, Reduce red, double* out, hipStream_t s);
+// r = (float) r64 on the fluid cells, e = 0, ||r||^2 into out[0]; clears
+// *resetStop (if given) for the refinement it starts.
 void launch_refine_start(uint32_t numActive, const double* r64, float* r, float* e, const uint16_t* coef, Reduce red,
-                         double* out, hipStream_t s);
+                         double* out, hipStream_t s, uint32_t* resetStop = nullptr);
 void launch_matvec_dot(uint32_t numActive, const float* p, const uint16_t* coef, const int32_t* nbrSlot, float* q,
                        Reduce red, double* out, hipStream_t s);
 void launch_update(uint32_t numActive, const double* rzOld, const double* pq, const float* p, const float* q,
@@ -107,10 +114,12 @@ void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, co
 // r1 = r - A (w D^-1 r);  z = x + w D^-1 (r - A x) with x = w D^-1 r + y.
 // pairs != null (2 numActive doubles): per slot (sum of r1^2, count) over the
 // cells with cellType 1, the input scale of the learned correction.
+// stop (if given): the kernels return at once while *stop is set.
 void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, const int32_t* nbrSlot,
-                      const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s);
+                      const uint8_t* cellType, float omega, float* r1, double* pairs, hipStream_t s,
+                      const uint32_t* stop = nullptr);
 void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const uint16_t* coef, const int32_t* nbrSlot,
-                       float omega, float* z, hipStream_t s);
+                       float omega, float* z, hipStream_t s, const uint32_t* stop = nullptr);
 void launch_prepare_faces(float* U, float* V, float* W, float* U0, float* V0, float* W0, const uint8_t* solid, Grid g,
                           float gdt, SolidBox box, hipStream_t s);
 // solid = staticMask | box (brick-tiled bytes); staticMask may be null.

@@ -96,22 +96,51 @@ __device__ inline void block_pair_partial(double a, double c, double* partials) 
 
 __device__ inline float rms_scale(const double* red);
 
-// One wave: red[0..1] = sums of the pairs, lane l taking pairs l, l + 64, ...;
+// red[0..1] = sums of the pairs (one workgroup of up to 256 threads, a
+// fixed order: thread t takes pairs t, t + blockDim, ... four at a time);
 // scale != null: also *scale = RMS scale of the network input (rms_scale).
-__global__ void sum_pairs_kernel(const double* __restrict__ partials, int blocks, double* __restrict__ red,
-                                 float* __restrict__ scale = nullptr) {
-    double a = 0.0, c = 0.0;
-    for (int b = (int)threadIdx.x; b < blocks; b += 64) {
-        a += partials[2 * b];
-        c += partials[2 * b + 1];
+// stop != null and set: returns at once (work queued after the solver's
+// convergence).
+__global__ void __launch_bounds__(256) sum_pairs_kernel(const double* __restrict__ partials, int blocks,
+                                                        double* __restrict__ red, float* __restrict__ scale = nullptr,
+                                                        const uint32_t* __restrict__ stop = nullptr) {
+    __shared__ double s_a[4], s_c[4];
+    if (stop && *stop) return;
+    const int nt = (int)blockDim.x;
+    double a4[4] = {0.0, 0.0, 0.0, 0.0}, c4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = (int)threadIdx.x; b0 < blocks; b0 += 4 * nt) {
+        double va[4], vc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int b = b0 + k * nt;
+            va[k] = b < blocks ? partials[2 * b] : 0.0;
+            vc[k] = b < blocks ? partials[2 * b + 1] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            a4[k] += va[k];
+            c4[k] += vc[k];
+        }
     }
+    double a = (a4[0] + a4[1]) + (a4[2] + a4[3]), c = (c4[0] + c4[1]) + (c4[2] + c4[3]);
     for (int d = 32; d >= 1; d >>= 1) {
         a += __shfl_xor(a, d, 64);
         c += __shfl_xor(c, d, 64);
     }
+    const int wave = (int)threadIdx.x >> 6, waves = (nt + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_a[wave] = a;
+        s_c[wave] = c;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        red[0] = a;
-        red[1] = c;
+        double ta = 0.0, tc = 0.0;
+        for (int w = 0; w < waves; ++w) {
+            ta += s_a[w];
+            tc += s_c[w];
+        }
+        red[0] = ta;
+        red[1] = tc;
         if (scale) scale[0] = rms_scale(red);
     }
 }
@@ -311,7 +340,7 @@ void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, cons
                            nvox);
     const dim3 rg = red_grid(cells);
     FLIP2_KERNEL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
-    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red);
+    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(256), 0, s, red + 8, (int)rg.x, red, nullptr, nullptr);
     FLIP2_KERNEL(scatter_input_kernel, g1(cells ? cells : 1, 256), dim3(256), 0, s, activeBricks, cellType, r,
                        cells, n, in0c, mask, scale, red);
     FLIP2_LAUNCH_CHECK();
@@ -322,12 +351,13 @@ void launch_input_scale(uint32_t numActive, const uint8_t* cellType, const float
     const uint32_t cells = numActive * 512u;
     const dim3 rg = red_grid(cells);
     FLIP2_KERNEL(active_norm_kernel, rg, dim3(256), 0, s, cellType, r, cells, red + 8);
-    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, red + 8, (int)rg.x, red, scale);
+    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(256), 0, s, red + 8, (int)rg.x, red, scale, nullptr);
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_pairs_scale(const double* pairs, uint32_t count, double* red, float* scale, hipStream_t s) {
-    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(64), 0, s, pairs, (int)count, red, scale);
+void launch_pairs_scale(const double* pairs, uint32_t count, double* red, float* scale, hipStream_t s,
+                        const uint32_t* stop) {
+    FLIP2_KERNEL(sum_pairs_kernel, dim3(1), dim3(256), 0, s, pairs, (int)count, red, scale, stop);
     FLIP2_LAUNCH_CHECK();
 }
 
@@ -662,7 +692,7 @@ void UNet::adamStep(float lr, hipStream_t s) {
 
 void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
                        const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem,
-                       const double* normPairs, hipStream_t s) {
+                       const double* normPairs, hipStream_t s, const uint32_t* stop) {
     FLIP2_REQUIRE(gridN == n_ && n_ % 8 == 0, kErrArgument, "U-Net grid size differs from the simulation grid");
     if (newSystem) {
         // the dense fluid mask (level masks, tile lists, the head's mask)
@@ -671,21 +701,25 @@ void UNet::applyActive(uint32_t numActive, const uint32_t* activeBricks, const i
         setInputMasks(s, true);
         nn::launch_zero_f32(z, (long long)numActive * 512, s);
     } else if (normPairs) {
-        nn::launch_pairs_scale(normPairs, numActive, red_.devPtr(), scale_.devPtr(), s);
+        nn::launch_pairs_scale(normPairs, numActive, red_.devPtr(), scale_.devPtr(), s, stop);
     } else {
         nn::launch_input_scale(numActive, cellType, r, red_.devPtr(), scale_.devPtr(), s);
     }
     slotIn_ = nn::SlotInput{r, cellType, brickSlot, scale_.devPtr()};
     slotOut_ = z;
+    for (auto& t : tl_) t.stop = stop;
+    auto reset = [&] {
+        slotIn_ = nn::SlotInput{};
+        slotOut_ = nullptr;
+        for (auto& t : tl_) t.stop = nullptr;
+    };
     try {
         forward(s, false);
     } catch (...) {
-        slotIn_ = nn::SlotInput{};
-        slotOut_ = nullptr;
+        reset();
         throw;
     }
-    slotIn_ = nn::SlotInput{};
-    slotOut_ = nullptr;
+    reset();
 }
 
 }  // namespace flip2

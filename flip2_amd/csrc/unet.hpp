@@ -65,9 +65,11 @@ public:
     // from the previous apply's (masks and tile flags are rebuilt, z is
     // zeroed); otherwise only r changed.  z is written at fluid cells only.
     // normPairs: null, or the per-slot (sum of r^2, fluid count) pairs of r.
+    // stop: null, or a flag that makes the apply's kernels (other than those
+    // of a newSystem apply's mask rebuild) return at once while it is set.
     void applyActive(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
                      const uint8_t* cellType, const float* r, float* z, int gridN, bool newSystem,
-                     const double* normPairs, hipStream_t s);
+                     const double* normPairs, hipStream_t s, const uint32_t* stop = nullptr);
 
     const std::vector<UNetLayer>& layers() const { return layers_; }
 
@@ -116,7 +118,8 @@ void launch_scatter_input(uint32_t numActive, const uint32_t* activeBricks, cons
 void launch_input_scale(uint32_t numActive, const uint8_t* cellType, const float* r, double* red, float* scale,
                         hipStream_t s);
 // The same from `count` precomputed (sum of r^2, count) pairs.
-void launch_pairs_scale(const double* pairs, uint32_t count, double* red, float* scale, hipStream_t s);
+void launch_pairs_scale(const double* pairs, uint32_t count, double* red, float* scale, hipStream_t s,
+                        const uint32_t* stop = nullptr);
 
 // Dense copies of a solver system: b, solution x, operator coefficients
 // (u16 diag | mask << 3, brick-tiled order in, dense [z][y][x] out).

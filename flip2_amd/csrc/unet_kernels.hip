@@ -20,11 +20,19 @@ namespace nn {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ inline uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+// Round to BF16, nearest even: gfx950's v_cvt_pk_bf16_f32 (two values per
+// instruction when the compiler pairs them, as in f2bf2).
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t f2bf2(float lo, float hi) {
+    const v2bf v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
+}
+// Four values as BF16, packed low to high; all zero unless keep (a mask,
+// not a branch around the conversions).
+__device__ inline uint2 f2bf4(float a, float b, float c, float d, bool keep) {
+    const uint32_t m = keep ? 0xffffffffu : 0u;
+    return make_uint2(f2bf2(a, b) & m, f2bf2(c, d) & m);
 }
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -218,6 +226,18 @@ __device__ inline int pair_tap(int q, int dy, int half) {
     return sc < 9 ? (sc % 3) + 3 * dy + 9 * (sc / 3) : 27;
 }
 
+// Zero that the compiler cannot prove wave-uniform.  A load from a uniform
+// address yields a uniform value, which the compiler moves into a scalar
+// register right after the load -- with a wait for every vector memory
+// operation issued before it, the prefetch of the next step included.
+// Adding this zero to the index keeps the value per lane, so the wait falls
+// where the value is first used.
+__device__ inline uint32_t lane_zero() {
+    uint32_t z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
 // Inputs of the epilogue (MODE 0) for lane (col, kq): the bias of its
 // channels and the masks of its four voxels (bx + col, by + j, bz + wave).
 // Loaded before the MFMAs, so their latency overlaps them and a prefetch
@@ -323,21 +343,16 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e
             const int y = by + j;
             const bool inside = x < n && y < n && z < n;
             const long long v = ((long long)z * n + y) * n + x;
-            float yv[4];
-            uint16_t o[4];
+            float t[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float t = acc[0][j][r] + e.bias[0][r];
-                if (relu) t = fmaxf(t, 0.f);
-                o[r] = e.live(j) ? f2bf(t) : (uint16_t)0;
-                yv[r] = bf2f(o[r]);
+                t[r] = acc[0][j][r] + e.bias[0][r];
+                if (relu) t[r] = fmaxf(t[r], 0.f);
             }
-            if (Y && inside) {
-                uint2 packed;
-                packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-                packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-                *reinterpret_cast<uint2*>(Y + v * COUT + 4 * kq) = packed;
-            }
+            const uint2 packed = f2bf4(t[0], t[1], t[2], t[3], e.live(j));
+            const float yv[4] = {__uint_as_float(packed.x << 16), __uint_as_float(packed.x & 0xffff0000u),
+                                 __uint_as_float(packed.y << 16), __uint_as_float(packed.y & 0xffff0000u)};
+            if (Y && inside) *reinterpret_cast<uint2*>(Y + v * COUT + 4 * kq) = packed;
             float ch[16];
 #pragma unroll
             for (int r = 0; r < 4; ++r) ch[r] = yv[r];
@@ -368,16 +383,13 @@ __device__ inline void conv_epilogue(const v4f (&acc)[MB][4], const EpiIn<MB>& e
                     make_float4(acc[m][j][0], acc[m][j][1], acc[m][j][2], acc[m][j][3]);
                 continue;
             }
-            uint16_t o[4];
+            float yv[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float yv = acc[m][j][r] + e.bias[m][r];
-                if (relu) yv = fmaxf(yv, 0.f);
-                o[r] = e.live(j) ? f2bf(yv) : (uint16_t)0;
+                yv[r] = acc[m][j][r] + e.bias[m][r];
+                if (relu) yv[r] = fmaxf(yv[r], 0.f);
             }
-            uint2 packed;
-            packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-            packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+            const uint2 packed = f2bf4(yv[0], yv[1], yv[2], yv[3], e.live(j));
             *reinterpret_cast<uint2*>(Y + v * COUT + co) = packed;
         }
     }
@@ -453,17 +465,13 @@ __device__ inline void pool_tile(const v4f (&acc)[4], const EpiIn<1>& e, int rel
                                  int col, int kq) {
 #pragma unroll
     for (int j = 0; j < kTY; ++j) {
-        uint16_t o[4];
+        float t[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            float t = acc[j][r] + e.bias[0][r];
-            if (relu) t = fmaxf(t, 0.f);
-            o[r] = e.live(j) ? f2bf(t) : (uint16_t)0;
+            t[r] = acc[j][r] + e.bias[0][r];
+            if (relu) t[r] = fmaxf(t[r], 0.f);
         }
-        uint2 packed;
-        packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-        packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-        *reinterpret_cast<uint2*>(&sPool[wave][j][col][4 * kq]) = packed;
+        *reinterpret_cast<uint2*>(&sPool[wave][j][col][4 * kq]) = f2bf4(t[0], t[1], t[2], t[3], e.live(j));
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -487,7 +495,7 @@ __device__ inline void pool_tile(const v4f (&acc)[4], const EpiIn<1>& e, int rel
             uint32_t o[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                o[q] = (uint32_t)f2bf(sum[2 * q] * 0.125f) | ((uint32_t)f2bf(sum[2 * q + 1] * 0.125f) << 16);
+                o[q] = f2bf2(sum[2 * q] * 0.125f, sum[2 * q + 1] * 0.125f);
             *reinterpret_cast<uint4*>(P + (((long long)Z * nc + Yc) * nc + X) * PC + c0 + 8 * h) =
                 make_uint4(o[0], o[1], o[2], o[3]);
         }
@@ -510,6 +518,7 @@ __global__ void __launch_bounds__(256)
 conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Xc, const uint16_t* __restrict__ Wt,
                   const float* __restrict__ bias, uint16_t* __restrict__ Y, uint16_t* __restrict__ P, int n, int relu,
                   HeadArgs head, const uint8_t* __restrict__ vmask, TileList tl) {
+    if (tl.stop && *tl.stop) return;
     constexpr int MB = COUT / 16 / SPLIT;
     constexpr int STAGES = CIN / 16;
     static_assert(MB == 1 && 16 * SPLIT == COUT, "one 16-channel output block per workgroup");
@@ -568,26 +577,72 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
         by = (int)((t % tilesXY) / tilesX) * kTY;
         bz = (int)(t / tilesXY) * kTZ;
     };
-    uint4 pre[kStageLoads], preW[kWLoads];
-    // loads of step (unit, stage) into pre / preW
-    auto load_step = [&](uint32_t unit, int bx, int by, int bz, int stage) {
+    // the weight fragments as four named registers: as an array they were
+    // kept in scratch memory
+    uint4 pre[kStageLoads], preW0, preW1, preW2, preW3;
+    // Per-thread geometry of the prefetch loads, fixed for the kernel.
+    // Element t = threadIdx.x + 256 k of a stage is 8-channel half t & 1 of
+    // halo voxel t >> 1.  hOff: that voxel's offset in the grid from the
+    // tile's first halo voxel (bx - 1, by - 1, bz - 1); hOffC: the offset of
+    // its parent in the half-resolution grid from the parent of (bx, by, bz)
+    // (tile origins are even, so both are the same for every tile).  A
+    // thread's last element may lie past the halo (t >= 2 kHalo, never
+    // stored to LDS): it is clamped to the last element, so its load stays
+    // in bounds.  wOff: the offset of the thread's weight fragments within
+    // a (stage, channel group) block, clamped the same way.
+    constexpr int CB = CIN - CA;
+    const int nc = n >> 1;
+    const int half = (int)threadIdx.x & 1;
+    int hOff[kStageLoads], hOffC[CA > 0 ? kStageLoads : 1];
 #pragma unroll
-        for (int k = 0; k < kStageLoads; ++k) {
-            const int t = (int)threadIdx.x + 256 * k;
-            const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, bx, by, bz, 16 * stage, t) : nullptr;
-            pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
-        }
-        if (kFixedW) return;
-        const int co0 = (int)(unit % SPLIT) * 16;
+    for (int k = 0; k < kStageLoads; ++k) {
+        const int hv = min((int)threadIdx.x + 256 * k, 2 * kHalo - 1) >> 1;
+        const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+        hOff[k] = (hz * n + hy) * n + hx;
+        if (CA > 0) hOffC[k] = (((hz + 1) / 2 - 1) * nc + ((hy + 1) / 2 - 1)) * nc + ((hx + 1) / 2 - 1);
+    }
+    static_assert(kWLoads == 4, "four weight fragments per thread and stage");
+    int wOff[kWLoads] = {0, 0, 0, 0};
+    if (!kFixedW) {
 #pragma unroll
         for (int k = 0; k < kWLoads; ++k) {
-            const int t = (int)threadIdx.x + 256 * k;
+            const int t = min((int)threadIdx.x + 256 * k, kWStage - 1);
             const int tap = t >> 5, chunk = (t >> 4) & 1, co = t & 15;
-            const uint4* src = t < kWStage ? reinterpret_cast<const uint4*>(Wt + (size_t)(tap * COUT + co0 + co) * CIN +
-                                                                            16 * stage + 8 * chunk)
-                                           : nullptr;
-            preW[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+            wOff[k] = (tap * COUT + co) * CIN + 8 * chunk;
         }
+    }
+    // loads of step (unit, stage) into pre / preW0..3
+    auto load_step = [&](uint32_t unit, int bx, int by, int bz, int stage) {
+        const int c0 = 16 * stage;
+        // the halo lies inside the grid: no per-element bounds
+        const bool interior =
+            bx >= 1 && by >= 1 && bz >= 1 && bx + kTX < n && by + kTY < n && bz + kTZ < n;
+        if (interior && CA > 0 && c0 < CA) {
+            const uint32_t base = (uint32_t)(((bz >> 1) * nc + (by >> 1)) * nc + (bx >> 1));
+#pragma unroll
+            for (int k = 0; k < kStageLoads; ++k)
+                pre[k] = *reinterpret_cast<const uint4*>(Xc + (size_t)(base + (uint32_t)hOffC[k]) * CA + c0 +
+                                                         8 * half);
+        } else if (interior) {
+            const uint32_t base = (uint32_t)(((bz - 1) * n + (by - 1)) * n + (bx - 1));
+#pragma unroll
+            for (int k = 0; k < kStageLoads; ++k)
+                pre[k] = *reinterpret_cast<const uint4*>(X + (size_t)(base + (uint32_t)hOff[k]) * CB + (c0 - CA) +
+                                                         8 * half);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kStageLoads; ++k) {
+                const int t = (int)threadIdx.x + 256 * k;
+                const uint4* src = t < kHalo * 2 ? stage_src<CIN, CA>(X, Xc, n, bx, by, bz, c0, t) : nullptr;
+                pre[k] = src ? *src : make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        if (kFixedW) return;
+        const uint16_t* w = Wt + (size_t)((int)(unit % SPLIT) * 16 * CIN + c0);
+        preW0 = *reinterpret_cast<const uint4*>(w + wOff[0]);
+        preW1 = *reinterpret_cast<const uint4*>(w + wOff[1]);
+        preW2 = *reinterpret_cast<const uint4*>(w + wOff[2]);
+        preW3 = *reinterpret_cast<const uint4*>(w + wOff[3]);
     };
     auto store_step = [&](int b) {
 #pragma unroll
@@ -596,11 +651,11 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
             if (t < kHalo * 2) sIn[b][t & 1][t >> 1] = pre[k];
         }
         if (kFixedW) return;
-#pragma unroll
-        for (int k = 0; k < kWLoads; ++k) {
-            const int t = (int)threadIdx.x + 256 * k;
-            if (t < kWStage) sW[b][t] = preW[k];
-        }
+        const int t = (int)threadIdx.x;  // t + 512 < kWStage <= t + 768 + 96
+        sW[b][t] = preW0;
+        sW[b][t + 256] = preW1;
+        sW[b][t + 512] = preW2;
+        if (t + 768 < kWStage) sW[b][t + 768] = preW3;
     };
     v8bf aw[kFixedW ? 15 : 1];
     if (kFixedW) {
@@ -640,21 +695,28 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
         }
         const bool more = nu < end;
         const int m0 = (int)(u % SPLIT);
+        const bool newUnit = more && nu != u;
+        const uint32_t after = nu + perXcd;
+        // the tile id two units ahead, issued before this step's other loads
+        // and kept per lane: it is first used after the prefetch has been
+        // waited for, so it adds no wait of its own
+        const uint32_t tileAfter = tl.compute[(newUnit && after < end ? after / SPLIT : 0u) + lane_zero()];
         // loaded at every step (only the last stage of a unit uses them): a
         // conditional load would be tested, and waited for, on the spot
         const EpiIn<1> epi = epi_load<COUT, 1>(bias, vmask, head, n, bx, by, bz, m0, wave, col, kq);
+        // The next step is loaded (and stored into LDS below) also after
+        // the last one -- tile 0 of the list, in bounds and never used --
+        // so that no path through the loop leaves a prefetch register
+        // pending: the compiler's wait analysis does not see that both
+        // places test the same condition, and would otherwise wait at the
+        // top of every step for the previous step's output stores.
         int nbx = bx, nby = by, nbz = bz;
-        if (more) {
-            if (nu != u) {
-                nbx = (int)(tileNext % tilesX) * kTX;
-                nby = (int)((tileNext % tilesXY) / tilesX) * kTY;
-                nbz = (int)(tileNext / tilesXY) * kTZ;
-            }
-            load_step(nu, nbx, nby, nbz, ns);
+        if (nu != u) {
+            nbx = (int)(tileNext % tilesX) * kTX;
+            nby = (int)((tileNext % tilesXY) / tilesX) * kTY;
+            nbz = (int)(tileNext / tilesXY) * kTZ;
         }
-        const bool newUnit = more && nu != u;
-        const uint32_t after = nu + perXcd;
-        const uint32_t tileAfter = tl.compute[newUnit && after < end ? after / SPLIT : 0u];
+        load_step(nu, nbx, nby, nbz, ns);
         if constexpr (kFixedW) {
             conv_stage_rows(sIn[buf], [&](int q, int dy) { return aw[q * 3 + dy]; }, wave, col, kq, acc[0]);
         } else {
@@ -672,7 +734,7 @@ conv3_list_kernel(const uint16_t* __restrict__ X, const uint16_t* __restrict__ X
         // The prefetched step goes into the other LDS buffer (free since the
         // last barrier) before the epilogue's global stores are issued: the
         // wait for the prefetch then does not also wait for those stores.
-        if (more) store_step(buf ^ 1);
+        store_step(buf ^ 1);
         if (stage == STAGES - 1) {
             conv_epilogue<COUT, 0, 1>(acc, epi, hw, Y, nullptr, n, relu, head, bx, by, bz, m0, wave, col, kq);
             if (P) pool_tile(acc[0], epi, relu, sPool, P, n, COUT, 16 * m0, bx, by, bz, wave, col, kq);
@@ -755,17 +817,13 @@ conv3_in2_kernel(const uint16_t* __restrict__ X2, const uint16_t* __restrict__ W
         const long long v = ((long long)z * n + y) * n + x;
         const bool live = !cm.voxel || cm.voxel[v];
         const int co = 4 * kq;
-        uint16_t o[4];
+        float yv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            float yv = acc[j][r] + bias[co + r];
-            if (relu) yv = fmaxf(yv, 0.f);
-            o[r] = live ? f2bf(yv) : (uint16_t)0;
+            yv[r] = acc[j][r] + bias[co + r];
+            if (relu) yv[r] = fmaxf(yv[r], 0.f);
         }
-        uint2 packed;
-        packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-        packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-        *reinterpret_cast<uint2*>(Y + v * 16 + co) = packed;
+        *reinterpret_cast<uint2*>(Y + v * 16 + co) = f2bf4(yv[0], yv[1], yv[2], yv[3], live);
     }
 }
 
@@ -782,6 +840,7 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
                       const float* __restrict__ bias, uint16_t* __restrict__ Y, int n, int relu,
                       const uint8_t* __restrict__ vmask, TileList tl) {
     __shared__ uint32_t sIn[2][kHalo];
+    if (tl.stop && *tl.stop) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
@@ -885,14 +944,15 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
     for (;;) {
         const uint32_t nu = u + perXcd;
         const bool more = nu < end;
+        // the tile id two tiles ahead, first (see lane_zero)
+        const uint32_t tileAfter = tl.compute[(more && nu + perXcd < end ? nu + perXcd : 0u) + lane_zero()];
         // this tile's voxel masks before the next tile's loads
         const EpiIn<1> epi = epi_load<16, 1>(bias, vmask, HeadArgs{}, n, bx, by, bz, 0, wave, col, kq);
+        // loaded and stored also after the last tile (tile 0 of the list,
+        // never used): see conv3_list_kernel
         int nbx = bx, nby = by, nbz = bz;
-        if (more) {
-            tile_origin(tileNext, nbx, nby, nbz);
-            load_tile(nbx, nby, nbz);
-        }
-        const uint32_t tileAfter = tl.compute[more && nu + perXcd < end ? nu + perXcd : 0u];
+        tile_origin(tileNext, nbx, nby, nbz);
+        load_tile(nbx, nby, nbz);
         v4f acc[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -912,24 +972,20 @@ conv3_in2_list_kernel(const uint16_t* __restrict__ X2, SlotInput si, const uint1
             }
         }
         // next tile into the other buffer before this tile's output stores
-        if (more) store_tile(buf ^ 1);
+        store_tile(buf ^ 1);
         const int z = bz + wave, x = bx + col;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int y = by + j;
             if (x >= n || y >= n || z >= n) continue;
             const long long v = ((long long)z * n + y) * n + x;
-            uint16_t o[4];
+            float yv[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float yv = acc[j][r] + epi.bias[0][r];
-                if (relu) yv = fmaxf(yv, 0.f);
-                o[r] = epi.live(j) ? f2bf(yv) : (uint16_t)0;
+                yv[r] = acc[j][r] + epi.bias[0][r];
+                if (relu) yv[r] = fmaxf(yv[r], 0.f);
             }
-            uint2 packed;
-            packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-            packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-            *reinterpret_cast<uint2*>(Y + v * 16 + 4 * kq) = packed;
+            *reinterpret_cast<uint2*>(Y + v * 16 + 4 * kq) = f2bf4(yv[0], yv[1], yv[2], yv[3], epi.live(j));
         }
         if (!more) break;
         __syncthreads();

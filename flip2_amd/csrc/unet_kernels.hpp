@@ -26,6 +26,9 @@ struct TileList {
     const uint32_t* clear;
     const uint32_t* numClear;
     uint32_t tilesX, tilesY;
+    // null, or a flag that makes the list convolutions return at once
+    // (the solver's converged flag: work queued after convergence)
+    const uint32_t* stop = nullptr;
 };
 // Y = act(conv3x3x3(X, W) + bias), MFMA BF16 with FP32 accumulation.
 // W: [27][cout][cin] BF16 (tap = (dz*3 + dy)*3 + dx, offsets d-1).
