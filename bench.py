@@ -256,7 +256,8 @@ def fluid_steps(cfg, steps, warmup, grid_c3=512, roofline=False):
     return out
 
 
-def learned_vs_pcg(frames=200, compare=8, grid_n=128, fallback=0, flex=8, net_path=NET_PATH, no_graph=False):
+def learned_vs_pcg(frames=200, compare=8, grid_n=128, fallback=0, flex=8, net_path=NET_PATH, no_graph=False,
+                   smoother="ic0"):
     """Learned FCG vs Jacobi-PCG and MIC(0)-PCG on the 128^3 dam break with a
     sphere: ms per pressure solve to the 1e-6 FP64 residual on the same
     systems, the `compare` frames after `frames` Jacobi-PCG steps."""
@@ -276,6 +277,7 @@ def learned_vs_pcg(frames=200, compare=8, grid_n=128, fallback=0, flex=8, net_pa
         sim.step()
     sim.set_unet(net, fallback_iterations=fallback)
     sim.set_flex_directions(flex)
+    sim.set_learned_smoother(smoother)
     if no_graph:
         sim.set_learned_graph(False)
     res = compare_solvers(sim, compare)
@@ -380,6 +382,8 @@ def main():
     ap.add_argument("--compare-frames", type=int, default=8)
     ap.add_argument("--fallback", type=int, default=0, help="learned: Jacobi-PCG fallback iteration cap (0: none)")
     ap.add_argument("--flex", type=int, default=8, help="learned: earlier directions flexible CG keeps (1..16)")
+    ap.add_argument("--smoother", choices=["jacobi", "ic0"], default="ic0",
+                    help="learned: smoothing step around the network")
     ap.add_argument("--no-graph", action="store_true",
                     help="learned: launch the preconditioner's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--load-net", default=NET_PATH, help="c4/c5: the network to use")
@@ -420,7 +424,7 @@ def main():
         if args.train:
             return train_c4(args)
         r = learned_vs_pcg(args.frames, args.compare_frames, args.grid_c4, args.fallback, args.flex, args.load_net,
-                           args.no_graph)
+                           args.no_graph, args.smoother)
         print(json.dumps(r), flush=True)
         return
     if args.config == "c5":
@@ -452,7 +456,7 @@ def main():
         c1 = fluid_steps("c1", 10, 2, roofline=True)
         result["c1"] = {k: c1[k] for k in ("workload", "particles", "value", "unit", "ms_per_step", "solve",
                                            "roofline")}
-        result["learned_vs_pcg"] = learned_vs_pcg()
+        result["learned_vs_pcg"] = learned_vs_pcg(smoother=args.smoother)
         result["learned_vs_pcg"]["metric"] = "ms/frame pressure solve @1e-6 rel. residual, learned vs PCG"
         result["transfer_c2"] = transfer_microbench()
     if not args.no_cpu_baseline:
@@ -517,6 +521,7 @@ def c5_run(args):
     net.set_params(np.load(args.load_net, allow_pickle=False))
     sim.set_unet(net, fallback_iterations=args.fallback)
     sim.set_flex_directions(args.flex)
+    sim.set_learned_smoother(args.smoother)
     if args.no_graph:
         sim.set_learned_graph(False)
     print(f"[bench] c5: {x.size} particles", file=sys.stderr, flush=True)
@@ -563,6 +568,7 @@ def train_c4(args):
         np.save(args.save_net, net.params(), allow_pickle=False)
     sim.set_unet(net, fallback_iterations=args.fallback)
     sim.set_flex_directions(args.flex)
+    sim.set_learned_smoother(args.smoother)
     out = {"config": "c4-train", "frames_captured": nf, "epochs": args.epochs, "capture_s": round(capture_s, 1),
            "train_s": round(train_s, 1), "final_loss": round(float(losses[-1]), 5) if losses else None}
     out.update(compare_solvers(sim, args.compare_frames))

@@ -211,6 +211,7 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_set_flex_directions": [_vp, _u32],
         "flip2_fluid_set_flex_check_every": [_vp, _u32],
         "flip2_fluid_set_learned_graph": [_vp, ctypes.c_int],
+        "flip2_fluid_set_learned_smoother": [_vp, ctypes.c_int],
         "flip2_fluid_enable_timing": [_vp, ctypes.c_char_p],
         "flip2_fluid_disable_timing": [_vp],
         "flip2_fluid_timing": [_vp, ctypes.c_int, ctypes.c_char_p, _sz, ctypes.POINTER(_f64),
@@ -580,6 +581,14 @@ class FluidSim:
         """Attach `net` as the "learned" preconditioner (None detaches)."""
         check(lib().flip2_fluid_set_unet(self._h, net._h if net is not None else None, int(fallback_iterations)))
         self._unet = net
+
+    def set_learned_smoother(self, smoother: str) -> None:
+        """Smoothing around the learned correction: "jacobi" (damped, the
+        network's weight) or "ic0" (brick-local incomplete Cholesky)."""
+        kinds = {"jacobi": 0, "ic0": 1}
+        if smoother not in kinds:
+            raise ValueError(f"unknown smoother {smoother!r} (jacobi or ic0)")
+        check(lib().flip2_fluid_set_learned_smoother(self._h, kinds[smoother]))
 
     def set_learned_graph(self, enable: bool) -> None:
         """Replay the learned preconditioner from a HIP graph (default) or not."""

@@ -739,6 +739,7 @@ int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg) {
         c.learnedDirections = f->impl.config().learnedDirections;
         c.learnedCheckEvery = f->impl.config().learnedCheckEvery;
         c.learnedGraph = f->impl.config().learnedGraph;
+        c.learnedSmoother = f->impl.config().learnedSmoother;
         f->impl.setConfig(c);
     });
 }
@@ -1023,6 +1024,14 @@ int flip2_fluid_set_flex_check_every(flip2_fluid* f, uint32_t iterations) {
                       "flexible CG checks its residual every 1 to 1024 iterations");
         flip2::FluidConfig c = f->impl.config();
         c.learnedCheckEvery = iterations;
+        f->impl.setConfig(c);
+    });
+}
+int flip2_fluid_set_learned_smoother(flip2_fluid* f, int smoother) {
+    return guarded([&] {
+        NEED(f);
+        flip2::FluidConfig c = f->impl.config();
+        c.learnedSmoother = smoother;
         f->impl.setConfig(c);
     });
 }

@@ -61,6 +61,8 @@ void validate_config(const FluidConfig& c) {
                   "config: the flexible-CG check interval must lie in [1, 65536]");
     FLIP2_REQUIRE(std::isfinite(c.margin) && c.margin >= 0.0f && c.margin < 0.5f, kErrArgument,
                   "config: margin must lie in [0, 0.5)");
+    FLIP2_REQUIRE(c.learnedSmoother == 0 || c.learnedSmoother == 1, kErrArgument,
+                  "config: the learned smoother is 0 (damped Jacobi) or 1 (IC(0))");
 }
 
 // Device history entries the solver needs for a configuration.
@@ -322,7 +324,7 @@ void FluidSim::ensureSlots(uint32_t numActive) {
     cellType_.resizeAsync(cells, stream_);
     coef_.resizeAsync(cells, stream_);
     for (auto* a : {&b64_, &x64_, &r64_}) a->resizeAsync(cells, stream_);
-    for (auto* a : {&r_, &z_, &p_, &p2_, &q_, &e_, &precon_, &aux0_, &aux1_}) a->resizeAsync(cells, stream_);
+    for (auto* a : {&r_, &z_, &p_, &p2_, &q_, &e_, &precon_, &aux0_, &aux1_, &aux2_}) a->resizeAsync(cells, stream_);
     partials_.resizeAsync(cap * (uint32_t)ns::kMaxFlexDirections + 64u, stream_);
     slotCapacity_ = cap;
     ringSlots_ = 0;
@@ -342,7 +344,21 @@ void FluidSim::ensureRing(uint32_t slots) {
 void FluidSim::applyLearnedEager(uint32_t na, bool newSystem) {
     const float w = cfg_.learnedSmoothing;
     const uint32_t* stop = stop_.devPtr();
-    if (w > 0.f) {
+    if (w > 0.f && cfg_.learnedSmoother == 1) {
+        // IC(0) smoothing: t = M^-1 r; r1 = r - A t; y = N(r1);
+        // z = (t + y) + M^-1 (r - A (t + y)).  aux2: t, aux0: r1 and then
+        // the post-smoothing residual, aux1: y.
+        double* pairs = partials_.devPtr();
+        ns::launch_mic_smooth(na, r_.devPtr(), nullptr, nullptr, coef_.devPtr(), precon_.devPtr(), aux2_.devPtr(),
+                              stream_, stop);
+        ns::launch_smooth_residual(na, r_.devPtr(), aux2_.devPtr(), nullptr, coef_.devPtr(), nbrSlot_.devPtr(),
+                                   cellType_.devPtr(), aux0_.devPtr(), pairs, stream_, stop);
+        learned_(na, aux0_.devPtr(), aux1_.devPtr(), newSystem, pairs, stream_, stop);
+        ns::launch_smooth_residual(na, r_.devPtr(), aux2_.devPtr(), aux1_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(),
+                                   cellType_.devPtr(), aux0_.devPtr(), nullptr, stream_, stop);
+        ns::launch_mic_smooth(na, aux0_.devPtr(), aux2_.devPtr(), aux1_.devPtr(), coef_.devPtr(), precon_.devPtr(),
+                              z_.devPtr(), stream_, stop);
+    } else if (w > 0.f) {
         // the partial sums of the network input's scale come with the smoothing
         double* pairs = partials_.devPtr();
         ns::launch_presmooth(na, r_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(), w, aux0_.devPtr(),
@@ -492,6 +508,8 @@ SolveStats FluidSim::solveOnce(int prec, uint32_t capIn) {
     FLIP2_REQUIRE(prec != kPrecLearned || learned_, kErrState, "no learned preconditioner installed");
     learnedFresh_ = true;
     if (prec == kPrecMIC0) ns::launch_mic_factor(na, coef_.devPtr(), precon_.devPtr(), stream_);
+    if (prec == kPrecLearned && cfg_.learnedSmoothing > 0.f && cfg_.learnedSmoother == 1)
+        ns::launch_mic_factor(na, coef_.devPtr(), precon_.devPtr(), stream_, false);
     const bool jac = prec == kPrecJacobi;
     const bool flex = prec == kPrecLearned;
     // Flexible CG: direction of inner iteration j in ring slot j % slots

@@ -636,9 +636,10 @@ __device__ inline float a_plus(uint16_t cf, int axis, uint32_t l) {
 }  // namespace
 
 // MIC(0) factor of each brick's diagonal block (Bridson's modified
-// incomplete Cholesky, tau = 0.97, safety 0.25): precon = 1/sqrt(e).
+// incomplete Cholesky, tau = 0.97 for the PCG, safety 0.25): precon =
+// 1/sqrt(e).  tau = 0 gives the unmodified IC(0) factor.
 __global__ void __launch_bounds__(64)
-mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon) {
+mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon, float tau) {
     __shared__ float s_pc[512];
     __shared__ uint16_t s_cf[512];
     const uint32_t base = blockIdx.x * 512u;
@@ -664,7 +665,7 @@ mic_factor_kernel(const uint16_t* __restrict__ coef, float* __restrict__ precon)
                     const float pm = s_pc[m];
                     e -= (a * pm) * (a * pm);
                     const float o1 = a_plus(cm, (ax + 1) % 3, m), o2 = a_plus(cm, (ax + 2) % 3, m);
-                    e -= kMicTau * (a * (o1 + o2) * pm * pm);
+                    e -= tau * (a * (o1 + o2) * pm * pm);
                 }
                 if (e < kMicSigma * diag) e = diag;
                 pc = 1.0f / sqrtf(e);
@@ -1157,6 +1158,85 @@ cg_update_mic_kernel(CgFused c, uint32_t it, const double* __restrict__ r64, con
     }
     group_partial<W>(live ? rz : 0.0, s_red, c.partials + ng + grp);
     group_partial<W>(live ? rr : 0.0, s_red + W, c.partials + 2u * ng + grp);
+}
+
+// Smoothing by a brick-local incomplete Cholesky factor around the learned
+// correction (the unmodified IC(0) factor: with the MIC(0) one, whose
+// M^-1 A has eigenvalues from 1 up to O(1/h), the undamped step grows
+// errors), one wave per slot with the MIC(0) update's sweeps: out = a + b +
+// M^-1 in on the fluid cells (a, b may be null), 0 elsewhere.  Returns at
+// once while *stop is set.
+__global__ void __launch_bounds__(64)
+mic_smooth_kernel(const float* __restrict__ in, const float* __restrict__ a, const float* __restrict__ b,
+                  const uint16_t* __restrict__ coef, const float* __restrict__ precon, float* __restrict__ out,
+                  uint32_t na, const uint32_t* __restrict__ stop) {
+    __shared__ float2 s_a[kMicEntries];
+    __shared__ uint16_t s_c[kMicEntries];
+    if (stop && *stop) return;
+    const uint32_t slot = xcd_slot(na);
+    if (slot >= na) return;
+    const uint32_t lane = threadIdx.x, base = slot * 512u;
+    uint16_t cf[8];
+    float rk[8], pc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t v = base + lane + 64u * (uint32_t)k;
+        cf[k] = coef[v];
+        rk[k] = in[v];
+        pc[k] = precon[v];
+    }
+    uint32_t cellOf2[11];
+    mic_plane_cells(lane, cellOf2);
+    double rr = 0.0;  // (not needed here)
+    mic_clear_pads(s_a, s_c, lane);
+    mic_stage(s_a, s_c, lane, cf, rk, pc, rr);
+    wave_lds_sync();
+    mic_sweeps(s_a, s_c, cellOf2);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t l = lane + 64u * (uint32_t)k, v = base + l;
+        float z = s_a[kMicPad + l].x;
+        if (a) z += a[v];
+        if (b) z += b[v];
+        out[v] = cf[k] ? z : 0.f;
+    }
+}
+
+// r1 = r - A (a + b) on the fluid cells (b may be null), 0 elsewhere; the
+// slot's own operand staged in LDS as in presmooth_kernel, whose pairs
+// (sum of r1^2, count over cellType 1) it also writes when pairs != null.
+__global__ void __launch_bounds__(kWG)
+smooth_residual_kernel(const float* __restrict__ r, const float* __restrict__ a, const float* __restrict__ b,
+                       const uint16_t* __restrict__ coef, const int32_t* __restrict__ nbrSlot,
+                       const uint8_t* __restrict__ cellType, float* __restrict__ r1, double* __restrict__ pairs,
+                       const uint32_t* __restrict__ stop) {
+    __shared__ float sx[512];
+    __shared__ double s_tmp[16];
+    if (stop && *stop) return;
+    const uint32_t slot = blockIdx.x;
+    auto xf = [&](uint32_t i) { return coef[i] ? a[i] + (b ? b[i] : 0.f) : 0.f; };
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) sx[l] = xf(slot * 512u + l);
+    __syncthreads();
+    auto xs = [&](uint32_t i) { return (i >> 9) == slot ? sx[i & 511u] : xf(i); };
+    double ss = 0.0, cnt = 0.0;
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+        const uint32_t v = slot * 512u + l;
+        const uint16_t cf = coef[v];
+        const float rv = cf ? r[v] - apply_A_with(slot, l, cf, nbrSlot, xs) : 0.f;
+        r1[v] = rv;
+        if (pairs && cellType[v] == 1) {
+            ss += (double)rv * (double)rv;
+            cnt += 1.0;
+        }
+    }
+    if (pairs) {
+        const double sa = block_sum(ss, s_tmp);
+        const double sc = block_sum(cnt, s_tmp);
+        if (threadIdx.x == 0) {
+            pairs[2 * blockIdx.x] = sa;
+            pairs[2 * blockIdx.x + 1] = sc;
+        }
+    }
 }
 
 // hist[4 it], hist[4 it + 1] = rz, rr from the update's partials (the sums
@@ -1762,6 +1842,21 @@ void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const
     FLIP2_KERNEL(postsmooth_kernel, dim3(numActive), dim3(kWG), 0, s, r, y, coef, nbrSlot, omega, z, stop);
     FLIP2_LAUNCH_CHECK();
 }
+void launch_mic_smooth(uint32_t numActive, const float* in, const float* a, const float* b, const uint16_t* coef,
+                       const float* precon, float* out, hipStream_t s, const uint32_t* stop) {
+    if (!numActive) return;
+    FLIP2_KERNEL(mic_smooth_kernel, xcd_grid(numActive), dim3(64), 0, s, in, a, b, coef, precon, out, numActive,
+                 stop);
+    FLIP2_LAUNCH_CHECK();
+}
+void launch_smooth_residual(uint32_t numActive, const float* r, const float* a, const float* b, const uint16_t* coef,
+                            const int32_t* nbrSlot, const uint8_t* cellType, float* r1, double* pairs, hipStream_t s,
+                            const uint32_t* stop) {
+    if (!numActive) return;
+    FLIP2_KERNEL(smooth_residual_kernel, dim3(numActive), dim3(kWG), 0, s, r, a, b, coef, nbrSlot, cellType, r1,
+                 pairs, stop);
+    FLIP2_LAUNCH_CHECK();
+}
 void launch_flex_dots(uint32_t numActive, const float* z, const FlexDirections& d, Reduce red, double* out,
                       hipStream_t s) {
     FLIP2_REQUIRE(d.count >= 0 && d.count <= kMaxFlexDirections, kErrArgument, "flexible CG: too many directions");
@@ -1818,8 +1913,8 @@ void launch_fill_f64(double* a, uint32_t count, double v, hipStream_t s) {
     FLIP2_KERNEL(fill_f64_kernel, g1(count, 256), dim3(256), 0, s, a, count, v);
     FLIP2_LAUNCH_CHECK();
 }
-void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, hipStream_t s) {
-    FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon);
+void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, hipStream_t s, bool modified) {
+    FLIP2_KERNEL(mic_factor_kernel, dim3(numActive), dim3(64), 0, s, coef, precon, modified ? kMicTau : 0.0f);
     FLIP2_LAUNCH_CHECK();
 }
 namespace {

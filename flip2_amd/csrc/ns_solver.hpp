@@ -106,7 +106,9 @@ void launch_true_residual(uint32_t numActive, const double* b64, const double* x
                           const int32_t* nbrSlot, double* r64, Reduce red, double* out, hipStream_t s);
 void launch_fill_f32(float* a, uint32_t count, float v, hipStream_t s);
 void launch_fill_f64(double* a, uint32_t count, double v, hipStream_t s);
-void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, hipStream_t s);
+// modified: the MIC(0) factor (the PCG preconditioner); false: IC(0)
+// (the learned preconditioner's smoother).
+void launch_mic_factor(uint32_t numActive, const uint16_t* coef, float* precon, hipStream_t s, bool modified = true);
 void launch_velocity_update(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot,
                             const uint8_t* cellType, const uint8_t* solid, Grid g, const double* x64, float dtOverH,
                             float* U, float* V, float* W, hipStream_t s);
@@ -120,6 +122,15 @@ void launch_presmooth(uint32_t numActive, const float* r, const uint16_t* coef, 
                       const uint32_t* stop = nullptr);
 void launch_postsmooth(uint32_t numActive, const float* r, const float* y, const uint16_t* coef, const int32_t* nbrSlot,
                        float omega, float* z, hipStream_t s, const uint32_t* stop = nullptr);
+// IC(0) smoothing around the learned correction: out = a + b + M^-1 in on
+// the fluid cells (a, b may be null; M the brick-local factors of
+// launch_mic_factor with modified = false), and r1 = r - A (a + b) (b may be
+// null) with the network input's pairs as launch_presmooth.
+void launch_mic_smooth(uint32_t numActive, const float* in, const float* a, const float* b, const uint16_t* coef,
+                       const float* precon, float* out, hipStream_t s, const uint32_t* stop = nullptr);
+void launch_smooth_residual(uint32_t numActive, const float* r, const float* a, const float* b, const uint16_t* coef,
+                            const int32_t* nbrSlot, const uint8_t* cellType, float* r1, double* pairs, hipStream_t s,
+                            const uint32_t* stop = nullptr);
 void launch_prepare_faces(float* U, float* V, float* W, float* U0, float* V0, float* W0, const uint8_t* solid, Grid g,
                           float gdt, SolidBox box, hipStream_t s);
 // solid = staticMask | box (brick-tiled bytes); staticMask may be null.

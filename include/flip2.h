@@ -419,6 +419,14 @@ int flip2_fluid_set_flex_directions(flip2_fluid* f, uint32_t directions);
  * flexible CG; the other solvers check every config.check_every
  * iterations.  No counterpart in the reference. */
 int flip2_fluid_set_flex_check_every(flip2_fluid* f, uint32_t iterations);
+/* The smoothing step around the learned correction (when the network has
+ * a smoothing weight): FLIP2_SMOOTH_JACOBI (0) damped Jacobi with that
+ * weight, FLIP2_SMOOTH_IC0 (1, default) a brick-local incomplete Cholesky
+ * factor (t = M^-1 r, network on r - A t, then z = x + M^-1 (r - A x) with
+ * x the corrected t).  No counterpart in the reference. */
+#define FLIP2_SMOOTH_JACOBI 0
+#define FLIP2_SMOOTH_IC0 1
+int flip2_fluid_set_learned_smoother(flip2_fluid* f, int smoother);
 /* enable != 0 (default): the learned preconditioner's kernels are captured
  * once per solve into a HIP graph and replayed; 0 launches them one by one.
  * Same results either way.  No counterpart in the reference. */

@@ -153,9 +153,9 @@ def test_loss_grad_is_repeatable(flip2):
 def test_training_and_learned_fcg(flip2):
     """Train on the residuals r_k = b - A x_k that Jacobi-PCG produces in its
     first iterations (k = 0, 2, ..., 64) over 24 dam-break steps; the trained
-    network, wrapped in damped-Jacobi smoothing, preconditions flexible CG
-    to the 1e-6 FP64 residual in fewer iterations than Jacobi-PCG on the
-    next step's system."""
+    network, wrapped in damped-Jacobi smoothing and in brick-local IC(0)
+    smoothing, preconditions flexible CG to the 1e-6 FP64 residual in fewer
+    iterations than Jacobi-PCG on the next step's system."""
     n = 32
     x, y, z = scenes.dam_break(n, seed=1)
     zero = np.zeros_like(x)
@@ -179,6 +179,7 @@ def test_training_and_learned_fcg(flip2):
     for st in ("transfer", "prepare", "system"):
         sim.stage(st)
     sim.set_unet(net)
+    sim.set_learned_smoother("jacobi")
     jac = sim.solve_with("jacobi")
     lrn = sim.solve_with("learned")
     assert jac["converged"] and lrn["converged"], (jac, lrn)
@@ -200,6 +201,16 @@ def test_training_and_learned_fcg(flip2):
     sim.set_flex_check_every(1)
     assert batched["converged"] and batched["rel_residual"] <= 1e-6, batched
     assert batched["iterations"] >= lrn["iterations"], (batched, lrn)
+    # IC(0) smoothing (the default): converges in about as many iterations
+    # or fewer, and the graph replays match eager launches bit for bit too
+    sim.set_learned_smoother("ic0")
+    ic = sim.solve_with("learned")
+    assert ic["converged"] and ic["rel_residual"] <= 1e-6, ic
+    assert ic["iterations"] <= lrn["iterations"] + 2, (ic, lrn)
+    sim.set_learned_graph(False)
+    ic_eager = sim.solve_with("learned")
+    sim.set_learned_graph(True)
+    assert ic_eager["iterations"] == ic["iterations"] and ic_eager["rel_residual"] == ic["rel_residual"], (ic_eager, ic)
     # the fallback: a cap below what FCG needs hands the solve to Jacobi-PCG
     sim.set_unet(net, fallback_iterations=2)
     fb = sim.solve_with("learned")
