@@ -213,6 +213,7 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_set_learned_graph": [_vp, ctypes.c_int],
         "flip2_fluid_set_learned_smoother": [_vp, ctypes.c_int],
         "flip2_fluid_enable_timing": [_vp, ctypes.c_char_p],
+        "flip2_fluid_set_timing_sample": [_vp, ctypes.c_uint32],
         "flip2_fluid_disable_timing": [_vp],
         "flip2_fluid_timing": [_vp, ctypes.c_int, ctypes.c_char_p, _sz, ctypes.POINTER(_f64),
                                ctypes.POINTER(ctypes.c_uint64)],
@@ -602,11 +603,14 @@ class FluidSim:
         """Iterations (1..1024) between residual checks in flexible CG."""
         check(lib().flip2_fluid_set_flex_check_every(self._h, int(iterations)))
 
-    def enable_timing(self, kernels: Optional[Sequence[str]] = None) -> None:
+    def enable_timing(self, kernels: Optional[Sequence[str]] = None, sample_every: int = 1) -> None:
         """Time every launch (or those of the named kernels) of the following
-        steps, stages and solves with HIP events on the simulation's stream."""
+        steps, stages and solves with HIP events on the simulation's stream;
+        sample_every > 1 records one launch in that many of each kernel."""
         arg = ",".join(kernels).encode() if kernels else None
         check(lib().flip2_fluid_enable_timing(self._h, arg))
+        if sample_every != 1:
+            check(lib().flip2_fluid_set_timing_sample(self._h, int(sample_every)))
 
     def disable_timing(self) -> None:
         check(lib().flip2_fluid_disable_timing(self._h))

@@ -1084,7 +1084,15 @@ int flip2_fluid_enable_timing(flip2_fluid* f, const char* kernels) {
             }
         }
         f->timer.setFilter(names);
+        f->timer.setSampleEvery(1);
         f->timing = true;
+    });
+}
+int flip2_fluid_set_timing_sample(flip2_fluid* f, uint32_t every) {
+    return guarded([&] {
+        NEED(f);
+        FLIP2_REQUIRE(every >= 1, flip2::kErrArgument, "timing: sample every >= 1 launches");
+        f->timer.setSampleEvery(every);
     });
 }
 int flip2_fluid_disable_timing(flip2_fluid* f) {

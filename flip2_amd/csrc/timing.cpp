@@ -25,12 +25,15 @@ KernelTimer::~KernelTimer() {
 void KernelTimer::setFilter(const std::vector<std::string>& names) {
     filter_.clear();
     filter_.insert(names.begin(), names.end());
+    seen_.clear();
 }
 
-bool KernelTimer::wants(const char* name) const {
+bool KernelTimer::wants(const char* name) {
     // nested scopes (a group around its kernels) record the outer one only
     if (open_) return false;
-    return filter_.empty() || filter_.count(name) != 0;
+    if (!filter_.empty() && filter_.count(name) == 0) return false;
+    if (every_ == 1u) return true;
+    return (seen_[name]++ % every_) == 0u;
 }
 
 hipEvent_t KernelTimer::take() {
@@ -77,6 +80,7 @@ void KernelTimer::collect() {
 void KernelTimer::reset() {
     collect();
     totals_.clear();
+    seen_.clear();
 }
 
 }  // namespace flip2

@@ -27,7 +27,12 @@ public:
 
     // Only the listed names are recorded (an empty list records all).
     void setFilter(const std::vector<std::string>& names);
-    bool wants(const char* name) const;
+    // Record one launch in every `every` of each name (1: all).  The events
+    // around a launch cost the stream a few microseconds, so a timed run of
+    // many short launches samples them.
+    void setSampleEvery(uint32_t every) { every_ = every ? every : 1u; seen_.clear(); }
+    // Whether the next launch named `name` is recorded (counts it).
+    bool wants(const char* name);
     // Record the start / stop event of one launch named `name` on stream s.
     void start(const char* name, hipStream_t s);
     void stop(hipStream_t s);
@@ -60,6 +65,8 @@ private:
     std::vector<Pending> pending_;
     std::set<std::string> filter_;
     std::map<std::string, Entry> totals_;
+    std::map<std::string, uint64_t> seen_;
+    uint32_t every_ = 1;
     bool open_ = false;
 };
 

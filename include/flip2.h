@@ -360,6 +360,10 @@ int flip2_fluid_get_array(flip2_fluid* f, const char* name, void* host, size_t c
  * totals.  The reference has no timing (it prints host wall times only,
  * particles.cu:550). */
 int flip2_fluid_enable_timing(flip2_fluid* f, const char* kernels);
+/* After flip2_fluid_enable_timing: record only the first of every `every`
+ * launches of each timed kernel (the events cost the stream a few
+ * microseconds per recorded launch).  No counterpart in the reference. */
+int flip2_fluid_set_timing_sample(flip2_fluid* f, uint32_t every);
 int flip2_fluid_disable_timing(flip2_fluid* f);
 int flip2_fluid_timing(flip2_fluid* f, int k, char* name, size_t name_len, double* total_ms, uint64_t* launches);
 

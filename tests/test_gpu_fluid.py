@@ -260,6 +260,30 @@ def test_snapshot_resume_bit_identical(flip2, tmp_path):
         other.load_snapshot(cut)
 
 
+def test_kernel_timing_sample(flip2):
+    """The per-kernel event timer records every launch, or with a sampling
+    stride one in every `sample_every` launches of each kernel (the same
+    deterministic step on two simulations: the same launch count, divided)."""
+    n = 32
+    x, y, z = scenes.dam_break(n, seed=3)
+    zero = np.zeros_like(x)
+    grid = (0.0, 0.0, 0.0, n, n, n, 1.0 / n)
+    counts = []
+    for every in (1, 4):
+        sim = flip2.FluidSim(x.size, grid, preconditioner="mic0")
+        sim.set_particles(x, y, z, zero, zero, zero)
+        sim.step()
+        sim.enable_timing(["cg_update_mic_kernel"], sample_every=every)
+        sim.step()
+        t = sim.timing()
+        sim.disable_timing()
+        assert set(t) == {"cg_update_mic_kernel"}, t
+        ms, launches = t["cg_update_mic_kernel"]
+        assert ms > 0.0
+        counts.append(launches)
+    assert counts[0] > 8 and counts[1] == (counts[0] + 3) // 4, counts
+
+
 def test_step_frame_cfl_substeps(flip2):
     """Adaptive step: with a fast initial velocity the frame is split into
     substeps of dt = min(remaining, cfl h / max|v_c|) (exact maximum over
