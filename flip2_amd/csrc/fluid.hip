@@ -442,7 +442,14 @@ void FluidSim::prepareFaces() {
 
 void FluidSim::buildSystem() {
     const uint8_t* solid = hasSolid_ ? solid_.devPtr() : nullptr;
-    ns::launch_classify_bricks(pipe_->counts(), solid, numBricks_, brickFlag_.devPtr(), stream_);
+    // after a fused transfer the pipeline holds the cell offsets of its
+    // occupied bricks only (no dense per-cell counts)
+    const bool sparse = pipe_->sparseCells();
+    if (sparse)
+        ns::launch_classify_bricks_sparse(pipe_->touchedList(), pipe_->touchedCount(), pipe_->occupancy(),
+                                          pipe_->cellStart(), solid, numBricks_, brickFlag_.devPtr(), stream_);
+    else
+        ns::launch_classify_bricks(pipe_->counts(), solid, numBricks_, brickFlag_.devPtr(), stream_);
     scan_u32(brickFlag_.devPtr(), brickExcl_.devPtr(), numBricks_, true, ws_.devPtr(), ws_.size(), total_.devPtr(),
              stream_);
     uint32_t numActive = 0;
@@ -452,8 +459,9 @@ void FluidSim::buildSystem() {
     ensureSlots(std::max<uint32_t>(numActive, 1));
     ns::launch_build_slots(brickFlag_.devPtr(), brickExcl_.devPtr(), numBricks_, brickSlot_.devPtr(),
                            activeBricks_.devPtr(), stream_);
-    ns::launch_brick_setup(numActive, activeBricks_.devPtr(), brickSlot_.devPtr(), pipe_->counts(), solid, cfg_.grid,
-                           nbrSlot_.devPtr(), cellType_.devPtr(), stream_);
+    ns::launch_brick_setup(numActive, activeBricks_.devPtr(), brickSlot_.devPtr(), sparse ? nullptr : pipe_->counts(),
+                           solid, cfg_.grid, nbrSlot_.devPtr(), cellType_.devPtr(), stream_,
+                           sparse ? pipe_->cellStart() : nullptr);
     const double hOverDt = (double)cfg_.grid.cellSize / (double)cfg_.dt;
     ns::launch_assemble(numActive, activeBricks_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(), solid, cfg_.grid,
                         hOverDt, U_.devPtr(), V_.devPtr(), W_.devPtr(), coef_.devPtr(), b64_.devPtr(), reduce(),
@@ -767,7 +775,7 @@ void FluidSim::advect() {
 void FluidSim::gridToParticlesAndAdvect() {
     const uint8_t* solid = hasSolid_ ? solid_.devPtr() : nullptr;
     if (bricksFresh_)
-        ns::launch_g2p_advect_bricks(pipe_->touchedList(), pipe_->touchedCount(), pipe_->cellStart(), rec_.devPtr(),
+        ns::launch_g2p_advect_bricks(pipe_->touchedList(), pipe_->touchedCount(), pipe_->brickStart(), rec_.devPtr(),
                                      U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
                                      solid, cfg_.grid, cfg_.flip, cfg_.dt, cfg_.margin * cfg_.grid.cellSize, box_,
                                      stream_);

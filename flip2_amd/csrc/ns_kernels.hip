@@ -295,10 +295,13 @@ struct P2GShared {
 // Scatter of brick bb (index `brick`) into sh.w, sh.m (cleared here) by the
 // 256 threads of the workgroup; scaleW, scaleM: the fixed-point scales.
 // Ends with the sums complete and visible to the workgroup.
+// occ (may be null): brick occupancy flags; a run in a brick without
+// particles is empty without reading cellStart (which then need only hold
+// the cells of occupied bricks).
 __device__ inline void p2g_scatter_brick(P2GShared& sh, const int bb[3], uint32_t brick, const float4* __restrict__ rec,
                                          const uint32_t* __restrict__ cellStart, Grid g, uint32_t nbX, uint32_t nbY,
                                          uint32_t nbZ, const uint32_t* __restrict__ stats, float& scaleW,
-                                         float& scaleM) {
+                                         float& scaleM, const uint8_t* __restrict__ occ = nullptr) {
     const uint32_t tid = threadIdx.x;
     for (uint32_t t = tid; t < 1536u; t += 256u) {
         (&sh.w[0][0])[t] = 0ull;
@@ -308,7 +311,8 @@ __device__ inline void p2g_scatter_brick(P2GShared& sh, const int bb[3], uint32_
     uint32_t first = 0, cnt = 0;
     if (tid <= (uint32_t)kShellRuns) {
         uint32_t c0 = brick * 512u, c1 = c0 + 512u;
-        const bool ok = tid == 0 || shell_run((int)tid - 1, bb, nbX, nbY, nbZ, c0, c1);
+        bool ok = tid == 0 || shell_run((int)tid - 1, bb, nbX, nbY, nbZ, c0, c1);
+        if (ok && occ) ok = occ[c0 >> 9] != 0;
         if (ok) {
             first = cellStart[c0];
             cnt = cellStart[c1] - first;
@@ -480,13 +484,73 @@ p2g_brick_kernel(const float4* __restrict__ rec, const uint32_t* __restrict__ ce
 // the prepared values of a zero transfer.
 // ---------------------------------------------------------------------------
 
-// occ[b] = 1 if brick b holds a particle (cellStart is the exclusive scan of
-// the per-cell counts in brick-tiled cell order).
-__global__ void brick_occupancy_kernel(const uint32_t* __restrict__ cellStart, uint32_t numBricks,
-                                       uint8_t* __restrict__ occ) {
+// First index of a key >= v in the sorted keys[0, n).
+__device__ inline uint32_t lower_bound_u32(const uint32_t* __restrict__ keys, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Brick starts of the sorted cell keys (brick-tiled: brick = key >> 9):
+// brickStart[b] = first particle of brick b or later, brickStart[numBricks]
+// = n; occ[b] = 1 if brick b holds a particle.  One thread per brick, two
+// binary searches.
+__global__ void brick_starts_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t numBricks,
+                                    uint32_t* __restrict__ brickStart, uint8_t* __restrict__ occ) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= numBricks) return;
-    occ[b] = cellStart[(b + 1u) * 512u] != cellStart[b * 512u] ? 1 : 0;
+    const uint32_t s0 = lower_bound_u32(keys, n, b * 512u);
+    const uint32_t s1 = b + 1u < numBricks ? lower_bound_u32(keys, n, (b + 1u) * 512u) : n;
+    brickStart[b] = s0;
+    if (b + 1u == numBricks) brickStart[numBricks] = n;
+    occ[b] = s1 > s0 ? 1 : 0;
+}
+
+// cellStart of the occupied bricks of the touched list: the cell counts of a
+// brick's particles (a histogram in LDS over its range of the sorted keys),
+// exclusive-scanned and offset by the brick's start; entry (b + 1) * 512 is
+// the next brick's start, which that brick (if occupied) writes too, with
+// the same value.  The other bricks' entries are left as they are: readers
+// test occupancy first.
+__global__ void __launch_bounds__(256)
+sparse_cell_starts_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                          const uint8_t* __restrict__ occ, const uint32_t* __restrict__ brickStart,
+                          const uint32_t* __restrict__ keys, uint32_t* __restrict__ cellStart) {
+    __shared__ uint32_t hist[512];
+    __shared__ uint32_t waveSum[4];
+    const uint32_t total = *count;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
+        const uint32_t b = list[e];
+        if (!occ[b]) continue;  // the same for the whole workgroup
+        const uint32_t p0 = brickStart[b], p1 = brickStart[b + 1u];
+        hist[tid] = 0u;
+        hist[tid + 256u] = 0u;
+        __syncthreads();
+        for (uint32_t i = p0 + tid; i < p1; i += 256u) atomicAdd(&hist[keys[i] & 511u], 1u);
+        __syncthreads();
+        // exclusive scan of the 512 counts: thread t holds cells 2t, 2t + 1
+        const uint32_t c0 = hist[2u * tid], c1 = hist[2u * tid + 1u];
+        uint32_t incl = c0 + c1;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d, 64);
+            if (lane >= (uint32_t)d) incl += o;
+        }
+        if (lane == 63u) waveSum[wave] = incl;
+        __syncthreads();
+        uint32_t base = p0;
+        for (uint32_t w = 0; w < wave; ++w) base += waveSum[w];
+        const uint32_t ex = base + incl - c0 - c1;
+        cellStart[b * 512u + 2u * tid] = ex;
+        cellStart[b * 512u + 2u * tid + 1u] = ex + c0;
+        if (tid == 0) cellStart[b * 512u + 512u] = p1;
+        __syncthreads();  // hist and waveSum are reused by the next brick
+    }
 }
 
 // touched[b] = 1 if brick b or one of its 26 neighbours holds a particle (a
@@ -569,7 +633,7 @@ p2g_prepare_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict
                    const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart, Grid g,
                    const uint32_t* __restrict__ stats, const uint8_t* __restrict__ solid, SolidBox box, float gdt,
                    float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, float* __restrict__ U0,
-                   float* __restrict__ V0, float* __restrict__ W0) {
+                   float* __restrict__ V0, float* __restrict__ W0, const uint8_t* __restrict__ occ) {
     __shared__ P2GShared sh;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
     const uint32_t total = *count;
@@ -583,7 +647,7 @@ p2g_prepare_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict
         const uint32_t brick = list[e];
         const int bb[3] = {(int)(brick % nbX), (int)((brick / nbX) % nbY), (int)(brick / (nbX * nbY))};
         float scaleW, scaleM;
-        p2g_scatter_brick(sh, bb, brick, rec, cellStart, g, nbX, nbY, nbZ, stats, scaleW, scaleM);
+        p2g_scatter_brick(sh, bb, brick, rec, cellStart, g, nbX, nbY, nbZ, stats, scaleW, scaleM, occ);
         const double iw = 1.0 / (double)scaleW, im = 1.0 / (double)scaleM;
         for (uint32_t t = threadIdx.x; t < 512u; t += 256u) {
             float in[3];
@@ -680,21 +744,27 @@ void launch_p2g(const float4* sortedRec, const uint32_t* cellStart, Grid g, cons
     FLIP2_LAUNCH_CHECK();
 }
 
-void launch_p2g_prepare(const float4* sortedRec, const uint32_t* cellStart, Grid g, const uint32_t* stats,
-                        const FacePrep& prep, float* U, float* V, float* W, TouchedBricks t, hipStream_t s) {
+void launch_p2g_prepare(const float4* sortedRec, const uint32_t* sortedKeys, uint32_t n, uint32_t* cellStart,
+                        uint32_t* brickStart, Grid g, const uint32_t* stats, const FacePrep& prep, float* U, float* V,
+                        float* W, TouchedBricks t, hipStream_t s) {
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
     const uint32_t numBricks = nbX * nbY * nbZ;
     if (!numBricks) return;
-    FLIP2_REQUIRE(prep.U0 && prep.V0 && prep.W0 && t.occ && t.touched && t.list && t.count, kErrArgument,
-                  "transfer + prepare: null array");
+    FLIP2_REQUIRE(prep.U0 && prep.V0 && prep.W0 && t.occ && t.touched && t.list && t.count && brickStart &&
+                      cellStart && (sortedKeys || n == 0),
+                  kErrArgument, "transfer + prepare: null array");
     FLIP2_KERNEL(zero_u32_kernel, dim3(1), dim3(64), 0, s, t.count, 1u);
     FLIP2_LAUNCH_CHECK();
-    FLIP2_KERNEL(brick_occupancy_kernel, grid1d(numBricks, 256), dim3(256), 0, s, cellStart, numBricks, t.occ);
+    FLIP2_KERNEL(brick_starts_kernel, grid1d(numBricks, 256), dim3(256), 0, s, sortedKeys, n, numBricks, brickStart,
+                 t.occ);
     FLIP2_LAUNCH_CHECK();
     const bool pow2 = (nbX & (nbX - 1)) == 0;
     const int morton = (nbX == nbY && nbY == nbZ && pow2 && nbX <= 1024) ? 1 : 0;
     FLIP2_KERNEL(touched_bricks_kernel, grid1d(numBricks, 256), dim3(256), 0, s, t.occ, nbX, nbY, nbZ, morton,
                  t.touched, t.list, t.count);
+    FLIP2_LAUNCH_CHECK();
+    FLIP2_KERNEL(sparse_cell_starts_kernel, dim3(std::min<uint32_t>(numBricks, 2048u)), dim3(256), 0, s, t.list,
+                 t.count, t.occ, brickStart, sortedKeys, cellStart);
     FLIP2_LAUNCH_CHECK();
     const uint32_t numCells = numBricks * 512u;
     FLIP2_REQUIRE(prep.full || t.prevTouched, kErrArgument, "transfer + prepare: no previous touched flags");
@@ -705,7 +775,7 @@ void launch_p2g_prepare(const float4* sortedRec, const uint32_t* cellStart, Grid
     // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
     const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
     FLIP2_KERNEL(p2g_prepare_kernel, dim3(groups), dim3(256), 0, s, t.list, t.count, sortedRec, cellStart, g, stats,
-                 prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0);
+                 prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0, t.occ);
     FLIP2_LAUNCH_CHECK();
 }
 

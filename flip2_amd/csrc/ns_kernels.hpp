@@ -62,9 +62,14 @@ struct TouchedBricks {
 };
 // launch_p2g followed by launch_prepare_faces (the same U, V, W, U0, V0, W0),
 // with the transfer done only for bricks within one brick of a particle and
-// the others given the prepared faces of a zero transfer.
-void launch_p2g_prepare(const float4* sortedRec, const uint32_t* cellStart, Grid g, const uint32_t* stats,
-                        const FacePrep& prep, float* U, float* V, float* W, TouchedBricks t, hipStream_t s);
+// the others given the prepared faces of a zero transfer.  Takes the sorted
+// keys instead of the dense per-cell offsets and writes brickStart
+// (numBricks + 1: first particle of each brick, n at the end), t.occ and
+// cellStart of the occupied bricks only (entries b * 512 .. b * 512 + 512);
+// the other entries of cellStart keep what they held.
+void launch_p2g_prepare(const float4* sortedRec, const uint32_t* sortedKeys, uint32_t n, uint32_t* cellStart,
+                        uint32_t* brickStart, Grid g, const uint32_t* stats, const FacePrep& prep, float* U, float* V,
+                        float* W, TouchedBricks t, hipStream_t s);
 
 }  // namespace ns
 }  // namespace flip2

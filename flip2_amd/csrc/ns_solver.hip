@@ -104,6 +104,27 @@ classify_bricks_kernel(const uint32_t* __restrict__ counts, const uint8_t* __res
     if (threadIdx.x == 0) flag[b] = m ? 1u : 0u;
 }
 
+// The same from the cell offsets of a fused transfer, which exist for the
+// occupied bricks only: flag (cleared beforehand) is set for the occupied
+// bricks of the touched list that have such a cell.
+__global__ void __launch_bounds__(64)
+classify_bricks_sparse_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                              const uint8_t* __restrict__ occ, const uint32_t* __restrict__ cellStart,
+                              const uint8_t* __restrict__ solid, uint32_t* __restrict__ flag) {
+    const uint32_t total = *count;
+    for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
+        const uint32_t b = list[e];
+        if (!occ[b]) continue;
+        bool any = false;
+        for (uint32_t t = threadIdx.x; t < 512u; t += 64u) {
+            const uint32_t c = b * 512u + t;
+            any |= (cellStart[c + 1u] > cellStart[c]) && !(solid && solid[c]);
+        }
+        const uint64_t m = __ballot(any);
+        if (threadIdx.x == 0) flag[b] = m ? 1u : 0u;
+    }
+}
+
 __global__ void build_slots_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ excl,
                                    uint32_t numBricks, int32_t* __restrict__ brickSlot,
                                    uint32_t* __restrict__ activeBricks) {
@@ -120,8 +141,9 @@ __global__ void build_slots_kernel(const uint32_t* __restrict__ flag, const uint
 // Neighbour slots and cell types of each active brick.
 __global__ void __launch_bounds__(kWG)
 brick_setup_kernel(const uint32_t* __restrict__ activeBricks, const int32_t* __restrict__ brickSlot,
-                   const uint32_t* __restrict__ counts, const uint8_t* __restrict__ solid, Grid g,
-                   int32_t* __restrict__ nbrSlot, uint8_t* __restrict__ cellType) {
+                   const uint32_t* __restrict__ counts, const uint32_t* __restrict__ cellStart,
+                   const uint8_t* __restrict__ solid, Grid g, int32_t* __restrict__ nbrSlot,
+                   uint8_t* __restrict__ cellType) {
     const uint32_t slot = blockIdx.x;
     const uint32_t b = activeBricks[slot];
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
@@ -141,7 +163,7 @@ brick_setup_kernel(const uint32_t* __restrict__ activeBricks, const int32_t* __r
         const uint32_t c = b * 512u + t;
         uint8_t ty = kAir;
         if (solid && solid[c]) ty = kSolid;
-        else if (counts[c] > 0u) ty = kFluid;
+        else if (counts ? counts[c] > 0u : cellStart[c + 1u] > cellStart[c]) ty = kFluid;
         cellType[slot * 512u + t] = ty;
     }
 }
@@ -581,6 +603,10 @@ true_residual_kernel(const double* __restrict__ b64, const double* __restrict__ 
     if (threadIdx.x == 0) partials[slot] = t;
 }
 
+__global__ void fill_u32_kernel(uint32_t* __restrict__ a, uint32_t count, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a[i] = v;
+}
 __global__ void fill_f32_kernel(float* __restrict__ a, uint32_t count, float v) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) a[i] = v;
@@ -1682,7 +1708,7 @@ __global__ void __launch_bounds__(256) g2p_advect_kernel(uint32_t n, float4* __r
 }
 
 // G2P + RK2 by brick, for the step (the particles sorted by brick, the
-// touched-brick list of the fused transfer): a workgroup copies the faces
+// touched-brick list and brick starts of the fused transfer): a workgroup copies the faces
 // of a brick's cells and their one-cell shell (10^3 cells, the six fields,
 // 24 KB) into LDS once, then interpolates for the brick's particles from
 // LDS -- a particle's G2P stencil lies in its brick's region, and the RK2
@@ -1693,7 +1719,7 @@ __global__ void __launch_bounds__(256) g2p_advect_kernel(uint32_t n, float4* __r
 // to the XCDs round robin).
 __global__ void __launch_bounds__(256)
 g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-                        const uint32_t* __restrict__ cellStart, float4* __restrict__ rec, const float* __restrict__ U,
+                        const uint32_t* __restrict__ brickStart, float4* __restrict__ rec, const float* __restrict__ U,
                         const float* __restrict__ V, const float* __restrict__ W, const float* __restrict__ U0,
                         const float* __restrict__ V0, const float* __restrict__ W0, const uint8_t* __restrict__ solid,
                         Grid g, float flip, float dt, float margin, SolidBox box) {
@@ -1708,7 +1734,7 @@ g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __res
         const uint32_t e = ((m >> 6) * 8u + xcd) * 64u + (m & 63u);
         if (e >= total) break;
         const uint32_t brick = list[e];
-        const uint32_t p0 = cellStart[brick * 512u], p1 = cellStart[brick * 512u + 512u];
+        const uint32_t p0 = brickStart[brick], p1 = brickStart[brick + 1u];
         if (p0 == p1) continue;  // a touched brick without particles
         const int org[3] = {(int)(brick % nbX) * 8 - 1, (int)((brick / nbX) % nbY) * 8 - 1,
                             (int)(brick / (nbX * nbY)) * 8 - 1};
@@ -1785,6 +1811,15 @@ void launch_classify_bricks(const uint32_t* counts, const uint8_t* solid, uint32
     FLIP2_KERNEL(classify_bricks_kernel, dim3(numBricks), dim3(64), 0, s, counts, solid, flag);
     FLIP2_LAUNCH_CHECK();
 }
+void launch_classify_bricks_sparse(const uint32_t* list, const uint32_t* count, const uint8_t* occ,
+                                   const uint32_t* cellStart, const uint8_t* solid, uint32_t numBricks, uint32_t* flag,
+                                   hipStream_t s) {
+    FLIP2_KERNEL(fill_u32_kernel, g1(numBricks, 256), dim3(256), 0, s, flag, numBricks, 0u);
+    FLIP2_LAUNCH_CHECK();
+    FLIP2_KERNEL(classify_bricks_sparse_kernel, dim3(std::min<uint32_t>(numBricks, 4096u)), dim3(64), 0, s, list,
+                 count, occ, cellStart, solid, flag);
+    FLIP2_LAUNCH_CHECK();
+}
 void launch_build_slots(const uint32_t* flag, const uint32_t* excl, uint32_t numBricks, int32_t* brickSlot,
                         uint32_t* activeBricks, hipStream_t s) {
     FLIP2_KERNEL(build_slots_kernel, g1(numBricks, 256), dim3(256), 0, s, flag, excl, numBricks, brickSlot,
@@ -1793,10 +1828,11 @@ void launch_build_slots(const uint32_t* flag, const uint32_t* excl, uint32_t num
 }
 void launch_brick_setup(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
                         const uint32_t* counts, const uint8_t* solid, Grid g, int32_t* nbrSlot, uint8_t* cellType,
-                        hipStream_t s) {
+                        hipStream_t s, const uint32_t* cellStart) {
     if (!numActive) return;
-    FLIP2_KERNEL(brick_setup_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, brickSlot, counts, solid, g,
-                       nbrSlot, cellType);
+    FLIP2_REQUIRE(counts || cellStart, kErrArgument, "brick setup: no cell counts or offsets");
+    FLIP2_KERNEL(brick_setup_kernel, dim3(numActive), dim3(kWG), 0, s, activeBricks, brickSlot,
+                 counts ? counts : nullptr, counts ? nullptr : cellStart, solid, g, nbrSlot, cellType);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot, const uint8_t* cellType,
@@ -2026,14 +2062,14 @@ void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, 
     FLIP2_KERNEL(advect_rk2_kernel, xcd_grid(ceil_div(n, 256)), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
     FLIP2_LAUNCH_CHECK();
 }
-void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* cellStart, float4* rec,
+void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* brickStart, float4* rec,
                               const float* U, const float* V, const float* W, const float* U0, const float* V0,
                               const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
                               SolidBox box, hipStream_t s) {
     const uint32_t numBricks = (g.sizeX >> 3) * (g.sizeY >> 3) * (g.sizeZ >> 3);
     // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
     const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
-    FLIP2_KERNEL(g2p_advect_brick_kernel, dim3(groups), dim3(256), 0, s, list, count, cellStart, rec, U, V, W, U0, V0,
+    FLIP2_KERNEL(g2p_advect_brick_kernel, dim3(groups), dim3(256), 0, s, list, count, brickStart, rec, U, V, W, U0, V0,
                  W0, solid, g, flip, dt, margin, box);
     FLIP2_LAUNCH_CHECK();
 }

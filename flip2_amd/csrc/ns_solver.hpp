@@ -30,11 +30,17 @@ struct SolidBox {
 
 void launch_classify_bricks(const uint32_t* counts, const uint8_t* solid, uint32_t numBricks, uint32_t* flag,
                             hipStream_t s);
+// The same after a fused transfer, whose cell offsets cover the occupied
+// bricks of its touched list only (flag is cleared first).
+void launch_classify_bricks_sparse(const uint32_t* list, const uint32_t* count, const uint8_t* occ,
+                                   const uint32_t* cellStart, const uint8_t* solid, uint32_t numBricks, uint32_t* flag,
+                                   hipStream_t s);
 void launch_build_slots(const uint32_t* flag, const uint32_t* excl, uint32_t numBricks, int32_t* brickSlot,
                         uint32_t* activeBricks, hipStream_t s);
+// Cell types from counts, or (counts null) from the cell offsets cellStart.
 void launch_brick_setup(uint32_t numActive, const uint32_t* activeBricks, const int32_t* brickSlot,
                         const uint32_t* counts, const uint8_t* solid, Grid g, int32_t* nbrSlot, uint8_t* cellType,
-                        hipStream_t s);
+                        hipStream_t s, const uint32_t* cellStart = nullptr);
 void launch_assemble(uint32_t numActive, const uint32_t* activeBricks, const int32_t* nbrSlot, const uint8_t* cellType,
                      const uint8_t* solid, Grid g, double hOverDt, const float* U, const float* V, const float* W,
                      uint16_t* coef, double* b64, Reduce red, double* out, hipStream_t s);
@@ -141,9 +147,10 @@ void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, co
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
                        Grid g, float dt, float margin, SolidBox box, hipStream_t s);
 // launch_g2p_advect by brick over the fused transfer's touched-brick list
-// (list, count: entries and their number on the device; cellStart: the
-// transfer's per-cell offsets of the sorted records); the same results.
-void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* cellStart, float4* rec,
+// (list, count: entries and their number on the device; brickStart: the
+// transfer's first sorted record of each brick, numBricks + 1 entries); the
+// same results.
+void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* brickStart, float4* rec,
                               const float* U, const float* V, const float* W, const float* U0, const float* V0,
                               const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
                               SolidBox box, hipStream_t s);

@@ -18,6 +18,7 @@ P2GPipeline::P2GPipeline(uint32_t maxParticles, const Grid& g, DeviceManager* po
     for (auto* v : {&packed_, &sorted_}) v->setPool(pool);
     for (auto* v : {&occ_, &touched_, &prevTouched_}) v->setPool(pool);
     touchedList_.setPool(pool);
+    brickStart_.setPool(pool);
     ws_.setPool(pool);
     keyBits_ = std::max<uint32_t>(1, bits_for(numCells_ - 1));
     keys_.resize(maxParticles); keys_.host().clear(); keys_.host().shrink_to_fit();
@@ -72,9 +73,15 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
         ns::launch_gather_rec(perm_.devPtr(), n, records, sorted_.devPtr(), stats_.devPtr(), s);
     }
     if (timed) FLIP2_HIP(hipEventRecord(ev[3 + passes], s));
-    ns::launch_cell_counts(keysSorted_.devPtr(), n, numCells_, counts_.devPtr(), s);
+    // The fused run (prep) needs the cell offsets of the occupied bricks
+    // only, which it derives from the sorted keys itself; the dense per-cell
+    // counts and their scan over every cell of the grid are for the plain
+    // transfer.
+    sparseCells_ = prep != nullptr;
+    if (!prep) ns::launch_cell_counts(keysSorted_.devPtr(), n, numCells_, counts_.devPtr(), s);
     if (timed) FLIP2_HIP(hipEventRecord(ev[4 + passes], s));
-    scan_u32(counts_.devPtr(), cellStart_.devPtr(), numCells_ + 1, true, ws_.devPtr(), ws_.size(), nullptr, s);
+    if (!prep)
+        scan_u32(counts_.devPtr(), cellStart_.devPtr(), numCells_ + 1, true, ws_.devPtr(), ws_.size(), nullptr, s);
     if (timed) FLIP2_HIP(hipEventRecord(ev[5 + passes], s));
     if (prep) {
         const uint32_t numBricks = numCells_ >> 9;
@@ -83,11 +90,13 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
             touched_.resizeAsync(numBricks, s);
             prevTouched_.resizeAsync(numBricks, s);
             touchedList_.resizeAsync(numBricks + 1, s);
+            brickStart_.resizeAsync(numBricks + 1, s);
             prevTouchedValid_ = false;
         }
         ns::FacePrep fp = *prep;
         if (!prevTouchedValid_) fp.full = 1;
-        ns::launch_p2g_prepare(sorted_.devPtr(), cellStart_.devPtr(), grid_, stats_.devPtr(), fp, U, V, W,
+        ns::launch_p2g_prepare(sorted_.devPtr(), keysSorted_.devPtr(), n, cellStart_.devPtr(), brickStart_.devPtr(),
+                               grid_, stats_.devPtr(), fp, U, V, W,
                                {occ_.devPtr(), touched_.devPtr(), touchedList_.devPtr(),
                                 touchedList_.devPtr() + numBricks, prevTouched_.devPtr()},
                                s);
@@ -162,6 +171,10 @@ size_t P2GPipeline::get(const std::string& name, void* host, size_t capacity, hi
     if (name == "keys") return out(keys_, lastN_);
     if (name == "sortedKeys") return out(keysSorted_, lastN_);
     if (name == "perm") return out(perm_, lastN_);
+    if (name == "counts" || name == "cellStart")
+        FLIP2_REQUIRE(!sparseCells_, kErrState,
+                      "counts / cellStart: the last run was a fused transfer, which keeps the offsets of the "
+                      "occupied bricks only");
     if (name == "counts") return out(counts_, numCells_ + 1);
     if (name == "cellStart") return out(cellStart_, numCells_ + 1);
     static const char* names[6] = {"x", "y", "z", "vx", "vy", "vz"};

@@ -47,7 +47,14 @@ public:
     // (valid after a runRecords with FacePrep).
     const uint32_t* touchedList() const { return touchedList_.devPtr(); }
     const uint32_t* touchedCount() const { return touchedList_.devPtr() + (numCells_ >> 9); }
+    // Per-cell counts: written by a plain run only (sparseCells() false).
     const uint32_t* counts() const { return counts_.devPtr(); }
+    // The last run was a fused one (runRecords with FacePrep): cellStart
+    // holds the cells of the occupied bricks only (occupancy(): one flag per
+    // brick), and brickStart() the first particle of every brick.
+    bool sparseCells() const { return sparseCells_; }
+    const uint8_t* occupancy() const { return occ_.devPtr(); }
+    const uint32_t* brickStart() const { return brickStart_.devPtr(); }
     // max |velocity component| of the last run's particles (float bits)
     const uint32_t* maxSpeedBits() const { return stats_.devPtr(); }
     // Sorted particle records of the last run; a caller may swap them into
@@ -74,6 +81,8 @@ private:
     DeviceVec<uint8_t> occ_, touched_, prevTouched_;  // fused transfer + prepare: per-brick flags
     bool prevTouchedValid_ = false;  // prevTouched_ holds the last fused run's flags
     DeviceVec<uint32_t> touchedList_;    // and the touched bricks (list, then count)
+    DeviceVec<uint32_t> brickStart_;     // first particle of each brick (fused runs)
+    bool sparseCells_ = false;
     void runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W, hipStream_t s,
                  const ns::FacePrep* prep);
 };
