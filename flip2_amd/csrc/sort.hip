@@ -99,10 +99,22 @@ sort_histogram_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t be
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             if (i + e >= n) break;
+            const uint64_t act = __ballot(true);
+            const uint32_t lead = (uint32_t)__ffsll((unsigned long long)act) - 1u;
             for (uint32_t p = 0; p < passes; ++p) {
                 const uint32_t shift = beginBit + 8u * p;
                 const uint32_t bits = min(8u, endBit - shift);
-                atomicAdd(&s_hist[p * kRadix + digit_of(key[4 * k + e], shift, (1u << bits) - 1u)], 1u);
+                const uint32_t d = digit_of(key[4 * k + e], shift, (1u << bits) - 1u);
+                // Keys that arrive nearly sorted (a FLIP step's particles,
+                // sorted the step before) share their upper digits across a
+                // wave, and 64 atomics on one LDS word serialise: a wave
+                // whose digits agree adds its count with one atomic.
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                if (__ballot(d == d0) == act) {
+                    if ((threadIdx.x & 63u) == lead) atomicAdd(&s_hist[p * kRadix + d0], (uint32_t)__popcll(act));
+                } else {
+                    atomicAdd(&s_hist[p * kRadix + d], 1u);
+                }
             }
         }
     }
