@@ -265,7 +265,7 @@ def fluid_steps(cfg, steps, warmup, grid_c3=512, roofline=False):
 
 
 def learned_vs_pcg(frames=200, compare=8, grid_n=128, fallback=0, flex=8, net_path=NET_PATH, no_graph=False,
-                   smoother="ic0"):
+                   smoother="ic0", smooth_steps=1):
     """Learned FCG vs Jacobi-PCG and MIC(0)-PCG on the 128^3 dam break with a
     sphere: ms per pressure solve to the 1e-6 FP64 residual on the same
     systems, the `compare` frames after `frames` Jacobi-PCG steps."""
@@ -286,6 +286,7 @@ def learned_vs_pcg(frames=200, compare=8, grid_n=128, fallback=0, flex=8, net_pa
     sim.set_unet(net, fallback_iterations=fallback)
     sim.set_flex_directions(flex)
     sim.set_learned_smoother(smoother)
+    sim.set_learned_smooth_steps(smooth_steps)
     if no_graph:
         sim.set_learned_graph(False)
     res = compare_solvers(sim, compare)
@@ -392,6 +393,7 @@ def main():
     ap.add_argument("--flex", type=int, default=8, help="learned: earlier directions flexible CG keeps (1..16)")
     ap.add_argument("--smoother", choices=["jacobi", "ic0"], default="ic0",
                     help="learned: smoothing step around the network")
+    ap.add_argument("--smooth-steps", type=int, default=1, help="learned: IC(0) smoothing steps (1 or 2)")
     ap.add_argument("--no-graph", action="store_true",
                     help="learned: launch the preconditioner's kernels one by one instead of replaying a HIP graph")
     ap.add_argument("--load-net", default=NET_PATH, help="c4/c5: the network to use")
@@ -432,7 +434,7 @@ def main():
         if args.train:
             return train_c4(args)
         r = learned_vs_pcg(args.frames, args.compare_frames, args.grid_c4, args.fallback, args.flex, args.load_net,
-                           args.no_graph, args.smoother)
+                           args.no_graph, args.smoother, args.smooth_steps)
         print(json.dumps(r), flush=True)
         return
     if args.config == "c5":
@@ -530,6 +532,7 @@ def c5_run(args):
     sim.set_unet(net, fallback_iterations=args.fallback)
     sim.set_flex_directions(args.flex)
     sim.set_learned_smoother(args.smoother)
+    sim.set_learned_smooth_steps(args.smooth_steps)
     if args.no_graph:
         sim.set_learned_graph(False)
     print(f"[bench] c5: {x.size} particles", file=sys.stderr, flush=True)

@@ -212,6 +212,7 @@ def _declare(l: ctypes.CDLL) -> None:
         "flip2_fluid_set_flex_check_every": [_vp, _u32],
         "flip2_fluid_set_learned_graph": [_vp, ctypes.c_int],
         "flip2_fluid_set_learned_smoother": [_vp, ctypes.c_int],
+        "flip2_fluid_set_learned_smooth_steps": [_vp, ctypes.c_uint32],
         "flip2_fluid_enable_timing": [_vp, ctypes.c_char_p],
         "flip2_fluid_set_timing_sample": [_vp, ctypes.c_uint32],
         "flip2_fluid_disable_timing": [_vp],
@@ -590,6 +591,10 @@ class FluidSim:
         if smoother not in kinds:
             raise ValueError(f"unknown smoother {smoother!r} (jacobi or ic0)")
         check(lib().flip2_fluid_set_learned_smoother(self._h, kinds[smoother]))
+
+    def set_learned_smooth_steps(self, steps: int) -> None:
+        """IC(0) smoothing steps (1 or 2) before and after the network."""
+        check(lib().flip2_fluid_set_learned_smooth_steps(self._h, int(steps)))
 
     def set_learned_graph(self, enable: bool) -> None:
         """Replay the learned preconditioner from a HIP graph (default) or not."""

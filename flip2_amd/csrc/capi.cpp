@@ -740,6 +740,7 @@ int flip2_fluid_set_config(flip2_fluid* f, const flip2_fluid_config* cfg) {
         c.learnedCheckEvery = f->impl.config().learnedCheckEvery;
         c.learnedGraph = f->impl.config().learnedGraph;
         c.learnedSmoother = f->impl.config().learnedSmoother;
+        c.learnedSmoothSteps = f->impl.config().learnedSmoothSteps;
         f->impl.setConfig(c);
     });
 }
@@ -1032,6 +1033,14 @@ int flip2_fluid_set_learned_smoother(flip2_fluid* f, int smoother) {
         NEED(f);
         flip2::FluidConfig c = f->impl.config();
         c.learnedSmoother = smoother;
+        f->impl.setConfig(c);
+    });
+}
+int flip2_fluid_set_learned_smooth_steps(flip2_fluid* f, uint32_t steps) {
+    return guarded([&] {
+        NEED(f);
+        flip2::FluidConfig c = f->impl.config();
+        c.learnedSmoothSteps = steps;
         f->impl.setConfig(c);
     });
 }

@@ -63,6 +63,8 @@ void validate_config(const FluidConfig& c) {
                   "config: margin must lie in [0, 0.5)");
     FLIP2_REQUIRE(c.learnedSmoother == 0 || c.learnedSmoother == 1, kErrArgument,
                   "config: the learned smoother is 0 (damped Jacobi) or 1 (IC(0))");
+    FLIP2_REQUIRE(c.learnedSmoothSteps == 1 || c.learnedSmoothSteps == 2, kErrArgument,
+                  "config: 1 or 2 IC(0) smoothing steps around the learned correction");
 }
 
 // Device history entries the solver needs for a configuration.
@@ -324,7 +326,8 @@ void FluidSim::ensureSlots(uint32_t numActive) {
     cellType_.resizeAsync(cells, stream_);
     coef_.resizeAsync(cells, stream_);
     for (auto* a : {&b64_, &x64_, &r64_}) a->resizeAsync(cells, stream_);
-    for (auto* a : {&r_, &z_, &p_, &p2_, &q_, &e_, &precon_, &aux0_, &aux1_, &aux2_}) a->resizeAsync(cells, stream_);
+    for (auto* a : {&r_, &z_, &p_, &p2_, &q_, &e_, &precon_, &aux0_, &aux1_, &aux2_, &aux3_})
+        a->resizeAsync(cells, stream_);
     partials_.resizeAsync(cap * (uint32_t)ns::kMaxFlexDirections + 64u, stream_);
     slotCapacity_ = cap;
     ringSlots_ = 0;
@@ -345,19 +348,37 @@ void FluidSim::applyLearnedEager(uint32_t na, bool newSystem) {
     const float w = cfg_.learnedSmoothing;
     const uint32_t* stop = stop_.devPtr();
     if (w > 0.f && cfg_.learnedSmoother == 1) {
-        // IC(0) smoothing: t = M^-1 r; r1 = r - A t; y = N(r1);
-        // z = (t + y) + M^-1 (r - A (t + y)).  aux2: t, aux0: r1 and then
-        // the post-smoothing residual, aux1: y.
+        // IC(0) smoothing: t = M^-1 r (then, for two steps, t += M^-1 (r -
+        // A t)); r1 = r - A t; y = N(r1); x = (t + y) + M^-1 (r - A (t + y))
+        // (then x += M^-1 (r - A x)); z = x.  aux2 / aux3: t and x, aux0:
+        // the residuals, aux1: y.
         double* pairs = partials_.devPtr();
-        ns::launch_mic_smooth(na, r_.devPtr(), nullptr, nullptr, coef_.devPtr(), precon_.devPtr(), aux2_.devPtr(),
-                              stream_, stop);
-        ns::launch_smooth_residual(na, r_.devPtr(), aux2_.devPtr(), nullptr, coef_.devPtr(), nbrSlot_.devPtr(),
-                                   cellType_.devPtr(), aux0_.devPtr(), pairs, stream_, stop);
+        const bool two = cfg_.learnedSmoothSteps >= 2;
+        auto residual = [&](const float* a, const float* b, double* pr) {
+            ns::launch_smooth_residual(na, r_.devPtr(), a, b, coef_.devPtr(), nbrSlot_.devPtr(), cellType_.devPtr(),
+                                       aux0_.devPtr(), pr, stream_, stop);
+        };
+        auto smooth = [&](const float* in, const float* a, const float* b, float* out) {
+            ns::launch_mic_smooth(na, in, a, b, coef_.devPtr(), precon_.devPtr(), out, stream_, stop);
+        };
+        smooth(r_.devPtr(), nullptr, nullptr, aux2_.devPtr());
+        float* t = aux2_.devPtr();
+        if (two) {
+            residual(t, nullptr, nullptr);
+            smooth(aux0_.devPtr(), t, nullptr, aux3_.devPtr());
+            t = aux3_.devPtr();
+        }
+        residual(t, nullptr, pairs);
         learned_(na, aux0_.devPtr(), aux1_.devPtr(), newSystem, pairs, stream_, stop);
-        ns::launch_smooth_residual(na, r_.devPtr(), aux2_.devPtr(), aux1_.devPtr(), coef_.devPtr(), nbrSlot_.devPtr(),
-                                   cellType_.devPtr(), aux0_.devPtr(), nullptr, stream_, stop);
-        ns::launch_mic_smooth(na, aux0_.devPtr(), aux2_.devPtr(), aux1_.devPtr(), coef_.devPtr(), precon_.devPtr(),
-                              z_.devPtr(), stream_, stop);
+        residual(t, aux1_.devPtr(), nullptr);
+        if (!two) {
+            smooth(aux0_.devPtr(), t, aux1_.devPtr(), z_.devPtr());
+        } else {
+            float* x = t == aux2_.devPtr() ? aux3_.devPtr() : aux2_.devPtr();
+            smooth(aux0_.devPtr(), t, aux1_.devPtr(), x);
+            residual(x, nullptr, nullptr);
+            smooth(aux0_.devPtr(), x, nullptr, z_.devPtr());
+        }
     } else if (w > 0.f) {
         // the partial sums of the network input's scale come with the smoothing
         double* pairs = partials_.devPtr();

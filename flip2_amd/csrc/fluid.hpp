@@ -41,6 +41,8 @@ struct FluidConfig {
     // flexible-CG iterations: 14.8 against 16.4 at 128^3, 68.5 against
     // 103.5 at 256^3 with the committed network).
     int learnedSmoother = 1;
+    // IC(0) smoothing steps before and after the network (1 or 2).
+    uint32_t learnedSmoothSteps = 1;
     // Flexible CG keeps this many earlier directions for the learned
     // preconditioner (1 = FCG(1); at most ns::kMaxFlexDirections).
     uint32_t learnedDirections = 8;
@@ -184,7 +186,7 @@ private:
     DeviceVec<uint16_t> coef_{dm_};
     DeviceVec<double> b64_{dm_}, x64_{dm_}, r64_{dm_};
     DeviceVec<float> r_{dm_}, z_{dm_}, p_{dm_}, p2_{dm_}, q_{dm_}, e_{dm_}, precon_{dm_}, aux0_{dm_}, aux1_{dm_},
-        aux2_{dm_};
+        aux2_{dm_}, aux3_{dm_};
     DeviceVec<float> pRing_{dm_}, qRing_{dm_};  // flexible CG directions, learnedDirections + 1 slots
     DeviceVec<double> zq_{dm_};
     uint32_t ringSlots_ = 0;

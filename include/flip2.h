@@ -431,6 +431,8 @@ int flip2_fluid_set_flex_check_every(flip2_fluid* f, uint32_t iterations);
 #define FLIP2_SMOOTH_JACOBI 0
 #define FLIP2_SMOOTH_IC0 1
 int flip2_fluid_set_learned_smoother(flip2_fluid* f, int smoother);
+/* IC(0) smoothing steps (1, default, or 2) before and after the network. */
+int flip2_fluid_set_learned_smooth_steps(flip2_fluid* f, uint32_t steps);
 /* enable != 0 (default): the learned preconditioner's kernels are captured
  * once per solve into a HIP graph and replayed; 0 launches them one by one.
  * Same results either way.  No counterpart in the reference. */
