@@ -601,25 +601,29 @@ __global__ void touched_bricks_kernel(const uint8_t* __restrict__ occ, uint32_t 
 // Faces of the untouched bricks: prepared values of a zero transfer (with
 // prevTouched, only for the bricks the previous run touched).
 __global__ void __launch_bounds__(256)
-empty_faces_kernel(const uint8_t* __restrict__ touched, const uint8_t* __restrict__ prevTouched, uint32_t numCells,
+empty_faces_kernel(const uint8_t* __restrict__ touched, const uint8_t* __restrict__ prevTouched, uint32_t numBricks,
                    Grid g, const uint8_t* __restrict__ solid, SolidBox box, float gdt, float* __restrict__ U,
                    float* __restrict__ V, float* __restrict__ W, float* __restrict__ U0, float* __restrict__ V0,
                    float* __restrict__ W0) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= numCells || touched[c >> 9] || (prevTouched && !prevTouched[c >> 9])) return;
+    // one workgroup per brick: the flags are tested once, and the bricks
+    // with nothing to write (most of them in steady state) exit at once
+    const uint32_t b = blockIdx.x;
+    if (b >= numBricks || touched[b] || (prevTouched && !prevTouched[b])) return;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
-    const uint32_t l = c & 511u, b = c >> 9;
-    const int gc[3] = {(int)((b % nbX) * 8u + (l & 7u)), (int)(((b / nbX) % nbY) * 8u + ((l >> 3) & 7u)),
-                       (int)((b / (nbX * nbY)) * 8u + (l >> 6))};
-    const float in[3] = {0.f, 0.f, 0.f};
-    float saved[3], out[3];
-    prepare_cell_faces(in, gc, g, solid, box, gdt, nbX, nbY, saved, out);
-    U0[c] = saved[0];
-    V0[c] = saved[1];
-    W0[c] = saved[2];
-    U[c] = out[0];
-    V[c] = out[1];
-    W[c] = out[2];
+    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+        const uint32_t c = b * 512u + l;
+        const int gc[3] = {(int)((b % nbX) * 8u + (l & 7u)), (int)(((b / nbX) % nbY) * 8u + ((l >> 3) & 7u)),
+                           (int)((b / (nbX * nbY)) * 8u + (l >> 6))};
+        const float in[3] = {0.f, 0.f, 0.f};
+        float saved[3], out[3];
+        prepare_cell_faces(in, gc, g, solid, box, gdt, nbX, nbY, saved, out);
+        U0[c] = saved[0];
+        V0[c] = saved[1];
+        W0[c] = saved[2];
+        U[c] = out[0];
+        V[c] = out[1];
+        W[c] = out[2];
+    }
 }
 
 // Scatter + prepare of the touched bricks; a fixed grid of workgroups loops
@@ -766,11 +770,9 @@ void launch_p2g_prepare(const float4* sortedRec, const uint32_t* sortedKeys, uin
     FLIP2_KERNEL(sparse_cell_starts_kernel, dim3(std::min<uint32_t>(numBricks, 2048u)), dim3(256), 0, s, t.list,
                  t.count, t.occ, brickStart, sortedKeys, cellStart);
     FLIP2_LAUNCH_CHECK();
-    const uint32_t numCells = numBricks * 512u;
     FLIP2_REQUIRE(prep.full || t.prevTouched, kErrArgument, "transfer + prepare: no previous touched flags");
-    FLIP2_KERNEL(empty_faces_kernel, grid1d(numCells, 256), dim3(256), 0, s, t.touched,
-                 prep.full ? nullptr : t.prevTouched, numCells, g, prep.solid, prep.box, prep.gdt, U, V, W, prep.U0,
-                 prep.V0, prep.W0);
+    FLIP2_KERNEL(empty_faces_kernel, dim3(numBricks), dim3(256), 0, s, t.touched, prep.full ? nullptr : t.prevTouched,
+                 numBricks, g, prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0);
     FLIP2_LAUNCH_CHECK();
     // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
     const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
