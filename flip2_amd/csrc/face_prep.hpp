@@ -14,6 +14,22 @@ __device__ inline uint32_t face_tiled(uint32_t i, uint32_t j, uint32_t k, uint32
     return (((k >> 3) * nbY + (j >> 3)) * nbX + (i >> 3)) * 512u + (k & 7u) * 64u + (j & 7u) * 8u + (i & 7u);
 }
 
+__device__ inline uint32_t clamp_cell(double q, uint32_t n) {
+    if (!(q > 0.0)) return 0u;
+    const double f = floor(q);
+    return f >= (double)(n - 1) ? n - 1 : (uint32_t)f;
+}
+
+// Brick-tiled index of the cell holding an FP32 position (clamped into the
+// grid): the particle's sort key.
+__device__ inline uint32_t cell_key(float x, float y, float z, const Grid& g) {
+    const double inv = 1.0 / (double)g.cellSize;
+    const uint32_t i = clamp_cell(((double)x - (double)g.negX) * inv, g.sizeX);
+    const uint32_t j = clamp_cell(((double)y - (double)g.negY) * inv, g.sizeY);
+    const uint32_t k = clamp_cell(((double)z - (double)g.negZ) * inv, g.sizeZ);
+    return face_tiled(i, j, k, g.sizeX >> 3, g.sizeY >> 3);
+}
+
 // Cells outside the domain count as solid.
 __device__ inline bool cell_solid(int i, int j, int k, Grid g, const uint8_t* solid, uint32_t nbX, uint32_t nbY) {
     if (i < 0 || j < 0 || k < 0 || i >= (int)g.sizeX || j >= (int)g.sizeY || k >= (int)g.sizeZ) return true;

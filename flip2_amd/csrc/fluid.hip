@@ -138,6 +138,7 @@ void FluidSim::setConfig(const FluidConfig& cfg) {
 void FluidSim::setParticles(uint32_t n, const float* x, const float* y, const float* z, const float* vx,
                             const float* vy, const float* vz) {
     bricksFresh_ = false;
+    keysFresh_ = false;
     FLIP2_REQUIRE(n <= maxParticles_, kErrRange, "more particles than the simulation was created for");
     n_ = n;
     std::vector<float4> r(2 * (size_t)n);
@@ -264,6 +265,7 @@ void FluidSim::saveSnapshot(const std::string& path) {
 
 void FluidSim::loadSnapshot(const std::string& path) {
     bricksFresh_ = false;
+    keysFresh_ = false;
     SnapFile in(path, "rb");
     char magic[sizeof(kSnapMagic)];
     in.get(magic, sizeof(magic));
@@ -437,6 +439,7 @@ double FluidSim::readScalar(const double* dev) {
 void FluidSim::transfer() {
     facesValid_ = false;
     bricksFresh_ = false;
+    keysFresh_ = false;
     pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_);
     rec_.swapDevice(pipe_->sortedVec());
 }
@@ -448,7 +451,8 @@ void FluidSim::transferAndPrepare() {
                       std::memcmp(&key.box, &prepKey_.box, sizeof(key.box)) == 0;
     const ns::FacePrep prep{U0_.devPtr(), V0_.devPtr(), W0_.devPtr(), hasSolid_ ? solid_.devPtr() : nullptr, box_,
                             key.gdt, same ? 0 : 1};
-    pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_, &prep);
+    pipe_->runRecords(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), stream_, &prep, keysFresh_);
+    keysFresh_ = false;
     rec_.swapDevice(pipe_->sortedVec());
     facesValid_ = true;
     prepKey_ = key;
@@ -794,6 +798,7 @@ void FluidSim::gridToParticles() {
 
 void FluidSim::advect() {
     bricksFresh_ = false;
+    keysFresh_ = false;
     ns::launch_advect_rk2(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(),
                           hasSolid_ ? solid_.devPtr() : nullptr, cfg_.grid, cfg_.dt, cfg_.margin * cfg_.grid.cellSize,
                           box_, stream_);
@@ -801,11 +806,12 @@ void FluidSim::advect() {
 
 void FluidSim::gridToParticlesAndAdvect() {
     const uint8_t* solid = hasSolid_ ? solid_.devPtr() : nullptr;
+    keysFresh_ = bricksFresh_;  // the brick kernel also writes the next transfer's keys
     if (bricksFresh_)
         ns::launch_g2p_advect_bricks(pipe_->touchedList(), pipe_->touchedCount(), pipe_->brickStart(), rec_.devPtr(),
                                      U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
                                      solid, cfg_.grid, cfg_.flip, cfg_.dt, cfg_.margin * cfg_.grid.cellSize, box_,
-                                     stream_);
+                                     stream_, pipe_->keys());
     else
         ns::launch_g2p_advect(n_, rec_.devPtr(), U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(),
                               W0_.devPtr(), solid, cfg_.grid, cfg_.flip, cfg_.dt, cfg_.margin * cfg_.grid.cellSize,

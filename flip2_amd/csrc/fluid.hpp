@@ -214,6 +214,10 @@ private:
     // cell offsets describe them (set by the fused transfer, cleared when the
     // particles move or another transfer runs)
     bool bricksFresh_ = false;
+    // the pipeline's key array holds the cell key of every record (written
+    // by the brick G2P + RK2 kernel; cleared when the records change
+    // otherwise), so the next fused transfer skips its binning pass
+    bool keysFresh_ = false;
     PrepKey prepKey_{};
     uint64_t solidVersion_ = 0;
     // pinned host copies of two batches' history entries, and their events

@@ -30,23 +30,8 @@
 namespace flip2 {
 namespace ns {
 
-__device__ inline uint32_t tiled_index(uint32_t i, uint32_t j, uint32_t k, uint32_t nbX, uint32_t nbY) {
-    return (((k >> 3) * nbY + (j >> 3)) * nbX + (i >> 3)) * 512u + (k & 7u) * 64u + (j & 7u) * 8u + (i & 7u);
-}
-
-__device__ inline uint32_t clamp_cell(double q, uint32_t n) {
-    if (!(q > 0.0)) return 0u;
-    const double f = floor(q);
-    return f >= (double)(n - 1) ? n - 1 : (uint32_t)f;
-}
-
-__device__ inline uint32_t cell_key(float x, float y, float z, const Grid& g) {
-    const double inv = 1.0 / (double)g.cellSize;
-    const uint32_t i = clamp_cell(((double)x - (double)g.negX) * inv, g.sizeX);
-    const uint32_t j = clamp_cell(((double)y - (double)g.negY) * inv, g.sizeY);
-    const uint32_t k = clamp_cell(((double)z - (double)g.negZ) * inv, g.sizeZ);
-    return tiled_index(i, j, k, g.sizeX >> 3, g.sizeY >> 3);
-}
+// cell_key, clamp_cell: face_prep.hpp (shared with the G2P + RK2 kernel,
+// which writes the next step's keys).
 
 // Cell keys of FP32 positions (clamped into the grid).
 __global__ void bin_kernel(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
@@ -693,6 +678,11 @@ void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec
 void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint32_t* stats, hipStream_t s) {
     if (!n) return;
     FLIP2_KERNEL(bin_rec_kernel, grid1d(n, 256), dim3(256), 0, s, rec, n, g, keys, stats);
+    FLIP2_LAUNCH_CHECK();
+}
+
+void launch_clear_stats(uint32_t* stats, hipStream_t s) {
+    FLIP2_KERNEL(zero_u32_kernel, dim3(1), dim3(64), 0, s, stats, 1u);
     FLIP2_LAUNCH_CHECK();
 }
 

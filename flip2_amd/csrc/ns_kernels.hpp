@@ -22,6 +22,9 @@ void launch_bin(const float* x, const float* y, const float* z, uint32_t n, Grid
 // zero it, the gather records max |velocity component| (float bits) in it.
 void launch_bin_pack(Float6 src, uint32_t n, Grid g, uint32_t* keys, float4* rec, uint32_t* stats, hipStream_t s);
 void launch_bin_rec(const float4* rec, uint32_t n, Grid g, uint32_t* keys, uint32_t* stats, hipStream_t s);
+// stats[0] = 0 (the binning kernels' reset of the speed statistic, for a run
+// whose keys were computed elsewhere).
+void launch_clear_stats(uint32_t* stats, hipStream_t s);
 // dst[i] = src[perm[i]] (records)
 void launch_gather_rec(const uint32_t* perm, uint32_t n, const float4* src, float4* dst, uint32_t* stats,
                        hipStream_t s);

@@ -1722,7 +1722,7 @@ g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __res
                         const uint32_t* __restrict__ brickStart, float4* __restrict__ rec, const float* __restrict__ U,
                         const float* __restrict__ V, const float* __restrict__ W, const float* __restrict__ U0,
                         const float* __restrict__ V0, const float* __restrict__ W0, const uint8_t* __restrict__ solid,
-                        Grid g, float flip, float dt, float margin, SolidBox box) {
+                        Grid g, float flip, float dt, float margin, SolidBox box, uint32_t* __restrict__ keys) {
     __shared__ float sF[6][kRegion * kRegion * kRegion];
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
     const uint32_t total = *count;
@@ -1779,6 +1779,7 @@ g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __res
             b.x = vel[1];
             b.y = vel[2];
             rec[2 * p + 1] = b;
+            if (keys) keys[p] = cell_key(xn[0], xn[1], xn[2], g);  // the next transfer's sort key
         }
         __syncthreads();  // the next brick overwrites the region
     }
@@ -2065,12 +2066,12 @@ void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, 
 void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* brickStart, float4* rec,
                               const float* U, const float* V, const float* W, const float* U0, const float* V0,
                               const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
-                              SolidBox box, hipStream_t s) {
+                              SolidBox box, hipStream_t s, uint32_t* keys) {
     const uint32_t numBricks = (g.sizeX >> 3) * (g.sizeY >> 3) * (g.sizeZ >> 3);
     // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
     const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
     FLIP2_KERNEL(g2p_advect_brick_kernel, dim3(groups), dim3(256), 0, s, list, count, brickStart, rec, U, V, W, U0, V0,
-                 W0, solid, g, flip, dt, margin, box);
+                 W0, solid, g, flip, dt, margin, box, keys);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_g2p_advect(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,

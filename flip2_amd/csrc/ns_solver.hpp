@@ -149,11 +149,12 @@ void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, 
 // launch_g2p_advect by brick over the fused transfer's touched-brick list
 // (list, count: entries and their number on the device; brickStart: the
 // transfer's first sorted record of each brick, numBricks + 1 entries); the
-// same results.
+// same results.  keys (may be null): also the cell key of each particle's
+// new position (what the next transfer's binning computes).
 void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* brickStart, float4* rec,
                               const float* U, const float* V, const float* W, const float* U0, const float* V0,
                               const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
-                              SolidBox box, hipStream_t s);
+                              SolidBox box, hipStream_t s, uint32_t* keys = nullptr);
 // launch_g2p_flip then launch_advect_rk2 (same results) in one kernel
 void launch_g2p_advect(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,
                        const float* V0, const float* W0, const uint8_t* solid, Grid g, float flip, float dt,

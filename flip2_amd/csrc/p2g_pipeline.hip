@@ -44,13 +44,13 @@ void P2GPipeline::run(uint32_t n, const float* x, const float* y, const float* z
 }
 
 void P2GPipeline::runRecords(uint32_t n, const float4* rec, float* U, float* V, float* W, hipStream_t s,
-                             const ns::FacePrep* prep) {
+                             const ns::FacePrep* prep, bool keysReady) {
     FLIP2_REQUIRE(n == 0 || rec, kErrArgument, "null particle records");
-    runImpl(n, nullptr, rec, U, V, W, s, prep);
+    runImpl(n, nullptr, rec, U, V, W, s, prep, keysReady);
 }
 
 void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W,
-                          hipStream_t s, const ns::FacePrep* prep) {
+                          hipStream_t s, const ns::FacePrep* prep, bool keysReady) {
     FLIP2_REQUIRE(n <= maxParticles_, kErrRange, "more particles than the pipeline was created for");
     FLIP2_REQUIRE(U && V && W, kErrArgument, "null output array");
     lastN_ = n;
@@ -63,6 +63,8 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
     if (soa) {
         ns::launch_bin_pack(*soa, n, grid_, keys_.devPtr(), packed_.devPtr(), stats_.devPtr(), s);
         records = packed_.devPtr();
+    } else if (keysReady) {
+        ns::launch_clear_stats(stats_.devPtr(), s);  // what the binning pass also does
     } else {
         ns::launch_bin_rec(rec, n, grid_, keys_.devPtr(), stats_.devPtr(), s);
     }

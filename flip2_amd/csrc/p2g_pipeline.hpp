@@ -27,8 +27,12 @@ public:
     // Same, from particle records (2 x float4 per particle).  With prep, the
     // transfer is followed by the prepare stage in the same kernels
     // (ns::launch_p2g_prepare): U, V, W then hold the prepared faces.
+    // keysReady: keys() already holds the cell key of every record (written
+    // by the step's G2P + RK2 kernel), so the binning pass is skipped.
     void runRecords(uint32_t n, const float4* rec, float* U, float* V, float* W, hipStream_t s,
-                    const ns::FacePrep* prep = nullptr);
+                    const ns::FacePrep* prep = nullptr, bool keysReady = false);
+    // The unsorted cell keys of the next run (see runRecords' keysReady).
+    uint32_t* keys() { return keys_.devPtr(); }
     size_t get(const std::string& name, void* host, size_t capacity, hipStream_t s);
     // Per-kernel timing of subsequent runs (HIP events on the run's stream):
     // ms[k] accumulates over runs for kernels "bin", "sort_setup" (clear +
@@ -84,7 +88,7 @@ private:
     DeviceVec<uint32_t> brickStart_;     // first particle of each brick (fused runs)
     bool sparseCells_ = false;
     void runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, float* U, float* V, float* W, hipStream_t s,
-                 const ns::FacePrep* prep);
+                 const ns::FacePrep* prep, bool keysReady = false);
 };
 
 }  // namespace flip2
