@@ -46,6 +46,20 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
         if (!(cond)) throw ::flip2::Error((code), std::string(msg));    \
     } while (0)
 
+// Workgroups of `kernel` (block threads, no dynamic LDS) that the current
+// device's CUs hold at once, rounded down to a multiple of 8 (one per XCD),
+// at least 8: the grid of a kernel whose workgroups loop over a fixed share
+// of the work, where a workgroup that waits for a free CU would run its
+// whole share after the first ones finish.
+template <typename Kernel>
+inline uint32_t resident_groups(Kernel kernel, int block) {
+    int dev = 0, cus = 0, perCu = 0;
+    FLIP2_HIP(hipGetDevice(&dev));
+    FLIP2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    FLIP2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, block, 0));
+    return std::max(8u, (uint32_t)(cus * perCu) & ~7u);
+}
+
 // Domain description with the reference's field order and types
 // (engine/grid.hpp:8-38): FP32 corner and cell size, u32 cell counts.
 struct Grid {

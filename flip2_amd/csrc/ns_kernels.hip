@@ -764,8 +764,8 @@ void launch_p2g_prepare(const float4* sortedRec, const uint32_t* sortedKeys, uin
     FLIP2_KERNEL(empty_faces_kernel, dim3(numBricks), dim3(256), 0, s, t.touched, prep.full ? nullptr : t.prevTouched,
                  numBricks, g, prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0);
     FLIP2_LAUNCH_CHECK();
-    // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
-    const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
+    // as many workgroups as the CUs hold at once (registers and LDS), 8 per XCD row
+    const uint32_t groups = std::min<uint32_t>(resident_groups(p2g_prepare_kernel, 256), 8u * ((numBricks + 7u) >> 3));
     FLIP2_KERNEL(p2g_prepare_kernel, dim3(groups), dim3(256), 0, s, t.list, t.count, sortedRec, cellStart, g, stats,
                  prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0, t.occ);
     FLIP2_LAUNCH_CHECK();

@@ -1747,8 +1747,18 @@ g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __res
             for (int a = 0; a < 6; ++a) sF[a][t] = in ? Fg[a][idx] : 0.f;
         }
         __syncthreads();
+        // each iteration loads the next one's records before its own work
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+        if (p0 + threadIdx.x < p1) {
+            a = rec[2 * (p0 + threadIdx.x)];
+            b = rec[2 * (p0 + threadIdx.x) + 1];
+        }
         for (uint32_t p = p0 + threadIdx.x; p < p1; p += 256u) {
-            float4 a = rec[2 * p], b = rec[2 * p + 1];
+            float4 an = a, bn = b;
+            if (p + 256u < p1) {
+                an = rec[2 * (p + 256u)];
+                bn = rec[2 * (p + 256u) + 1];
+            }
             const float x0[3] = {a.x, a.y, a.z};
             double q[3];
             to_cell_units(x0[0], x0[1], x0[2], g, q);
@@ -1780,6 +1790,8 @@ g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __res
             b.y = vel[2];
             rec[2 * p + 1] = b;
             if (keys) keys[p] = cell_key(xn[0], xn[1], xn[2], g);  // the next transfer's sort key
+            a = an;
+            b = bn;
         }
         __syncthreads();  // the next brick overwrites the region
     }
@@ -2068,8 +2080,9 @@ void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const
                               const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
                               SolidBox box, hipStream_t s, uint32_t* keys) {
     const uint32_t numBricks = (g.sizeX >> 3) * (g.sizeY >> 3) * (g.sizeZ >> 3);
-    // at most 6 of these workgroups fit a CU (LDS); 8 per XCD row
-    const uint32_t groups = std::min<uint32_t>(1536u, 8u * ((numBricks + 7u) >> 3));
+    // as many workgroups as the CUs hold at once (registers and LDS), 8 per XCD row
+    const uint32_t groups =
+        std::min<uint32_t>(resident_groups(g2p_advect_brick_kernel, 256), 8u * ((numBricks + 7u) >> 3));
     FLIP2_KERNEL(g2p_advect_brick_kernel, dim3(groups), dim3(256), 0, s, list, count, brickStart, rec, U, V, W, U0, V0,
                  W0, solid, g, flip, dt, margin, box, keys);
     FLIP2_LAUNCH_CHECK();
