@@ -590,24 +590,36 @@ empty_faces_kernel(const uint8_t* __restrict__ touched, const uint8_t* __restric
                    Grid g, const uint8_t* __restrict__ solid, SolidBox box, float gdt, float* __restrict__ U,
                    float* __restrict__ V, float* __restrict__ W, float* __restrict__ U0, float* __restrict__ V0,
                    float* __restrict__ W0) {
-    // one workgroup per brick: the flags are tested once, and the bricks
-    // with nothing to write (most of them in steady state) exit at once
-    const uint32_t b = blockIdx.x;
-    if (b >= numBricks || touched[b] || (prevTouched && !prevTouched[b])) return;
+    // 64 bricks per workgroup: the first wave tests their flags once (a
+    // lane per brick), then the workgroup writes the bricks that have faces
+    // to write (few in steady state; a workgroup per brick spent the kernel
+    // dispatching workgroups that exit at once)
+    __shared__ unsigned long long s_mask;
+    const uint32_t b0 = blockIdx.x * 64u;
+    if (threadIdx.x < 64u) {
+        const uint32_t bt = b0 + threadIdx.x;
+        const bool hit = bt < numBricks && !touched[bt] && (!prevTouched || prevTouched[bt]);
+        const unsigned long long m = __ballot(hit);
+        if (threadIdx.x == 0) s_mask = m;
+    }
+    __syncthreads();
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
-    for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
-        const uint32_t c = b * 512u + l;
-        const int gc[3] = {(int)((b % nbX) * 8u + (l & 7u)), (int)(((b / nbX) % nbY) * 8u + ((l >> 3) & 7u)),
-                           (int)((b / (nbX * nbY)) * 8u + (l >> 6))};
-        const float in[3] = {0.f, 0.f, 0.f};
-        float saved[3], out[3];
-        prepare_cell_faces(in, gc, g, solid, box, gdt, nbX, nbY, saved, out);
-        U0[c] = saved[0];
-        V0[c] = saved[1];
-        W0[c] = saved[2];
-        U[c] = out[0];
-        V[c] = out[1];
-        W[c] = out[2];
+    for (unsigned long long m = s_mask; m; m &= m - 1ull) {
+        const uint32_t b = b0 + (uint32_t)__builtin_ctzll(m);
+        for (uint32_t l = threadIdx.x; l < 512u; l += blockDim.x) {
+            const uint32_t c = b * 512u + l;
+            const int gc[3] = {(int)((b % nbX) * 8u + (l & 7u)), (int)(((b / nbX) % nbY) * 8u + ((l >> 3) & 7u)),
+                               (int)((b / (nbX * nbY)) * 8u + (l >> 6))};
+            const float in[3] = {0.f, 0.f, 0.f};
+            float saved[3], out[3];
+            prepare_cell_faces(in, gc, g, solid, box, gdt, nbX, nbY, saved, out);
+            U0[c] = saved[0];
+            V0[c] = saved[1];
+            W0[c] = saved[2];
+            U[c] = out[0];
+            V[c] = out[1];
+            W[c] = out[2];
+        }
     }
 }
 
@@ -761,7 +773,7 @@ void launch_p2g_prepare(const float4* sortedRec, const uint32_t* sortedKeys, uin
                  t.count, t.occ, brickStart, sortedKeys, cellStart);
     FLIP2_LAUNCH_CHECK();
     FLIP2_REQUIRE(prep.full || t.prevTouched, kErrArgument, "transfer + prepare: no previous touched flags");
-    FLIP2_KERNEL(empty_faces_kernel, dim3(numBricks), dim3(256), 0, s, t.touched, prep.full ? nullptr : t.prevTouched,
+    FLIP2_KERNEL(empty_faces_kernel, dim3((numBricks + 63u) / 64u), dim3(256), 0, s, t.touched, prep.full ? nullptr : t.prevTouched,
                  numBricks, g, prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0);
     FLIP2_LAUNCH_CHECK();
     // as many workgroups as the CUs hold at once (registers and LDS), 8 per XCD row
