@@ -808,7 +808,8 @@ void FluidSim::gridToParticlesAndAdvect() {
     const uint8_t* solid = hasSolid_ ? solid_.devPtr() : nullptr;
     keysFresh_ = bricksFresh_;  // the brick kernel also writes the next transfer's keys
     if (bricksFresh_)
-        ns::launch_g2p_advect_bricks(pipe_->touchedList(), pipe_->touchedCount(), pipe_->brickStart(), rec_.devPtr(),
+        ns::launch_g2p_advect_bricks(pipe_->touchedList(), pipe_->touchedCount(), pipe_->listCounters(),
+                                     pipe_->brickStart(), rec_.devPtr(),
                                      U_.devPtr(), V_.devPtr(), W_.devPtr(), U0_.devPtr(), V0_.devPtr(), W0_.devPtr(),
                                      solid, cfg_.grid, cfg_.flip, cfg_.dt, cfg_.margin * cfg_.grid.cellSize, box_,
                                      stream_, pipe_->keys());

@@ -1715,27 +1715,38 @@ __global__ void __launch_bounds__(256) g2p_advect_kernel(uint32_t n, float4* __r
 // midpoint's usually does (else it reads the grid) -- instead of gathering
 // every corner from memory.  The same results as g2p_advect_kernel (the
 // same weights, corners and summation order over the same face values).
-// Workgroups walk the list like p2g_prepare_kernel (64-entry chunks dealt
-// to the XCDs round robin).
+// The list is cut into 64-entry chunks dealt to the XCDs round robin (as
+// p2g_prepare_kernel walks it); a workgroup takes the next entry of its
+// XCD's share from that XCD's counter (counters[xcd], zero at launch), one
+// entry ahead, so workgroups with light bricks take more of them.
 __global__ void __launch_bounds__(256)
 g2p_advect_brick_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-                        const uint32_t* __restrict__ brickStart, float4* __restrict__ rec, const float* __restrict__ U,
+                        uint32_t* __restrict__ counters, const uint32_t* __restrict__ brickStart, float4* __restrict__ rec, const float* __restrict__ U,
                         const float* __restrict__ V, const float* __restrict__ W, const float* __restrict__ U0,
                         const float* __restrict__ V0, const float* __restrict__ W0, const uint8_t* __restrict__ solid,
                         Grid g, float flip, float dt, float margin, SolidBox box, uint32_t* __restrict__ keys) {
     __shared__ float sF[6][kRegion * kRegion * kRegion];
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3;
     const uint32_t total = *count;
-    const uint32_t perXcdGroups = gridDim.x >> 3, xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
+    __shared__ uint32_t s_next;
+    const uint32_t xcd = blockIdx.x & 7u;  // workgroups are dealt to the XCDs round robin
     const float* Fg[6] = {U, V, W, U0, V0, W0};
     const float* F[3] = {U, V, W};
     const float* F0[3] = {U0, V0, W0};
-    for (uint32_t m = rank;; m += perXcdGroups) {
+    if (threadIdx.x == 0) s_next = atomicAdd(&counters[xcd], 1u);
+    __syncthreads();
+    for (uint32_t m = s_next;; m = s_next) {
+        // the XCD's m-th entry: offset m % 64 of its (m / 64)-th chunk
         const uint32_t e = ((m >> 6) * 8u + xcd) * 64u + (m & 63u);
-        if (e >= total) break;
+        if (e >= total) break;  // (the same for the whole workgroup) later entries are past it too
+        __syncthreads();        // every thread has read s_next
+        if (threadIdx.x == 0) s_next = atomicAdd(&counters[xcd], 1u);  // the next entry, during this brick
         const uint32_t brick = list[e];
         const uint32_t p0 = brickStart[brick], p1 = brickStart[brick + 1u];
-        if (p0 == p1) continue;  // a touched brick without particles
+        if (p0 == p1) {  // a touched brick without particles
+            __syncthreads();  // s_next is written
+            continue;
+        }
         const int org[3] = {(int)(brick % nbX) * 8 - 1, (int)((brick / nbX) % nbY) * 8 - 1,
                             (int)(brick / (nbX * nbY)) * 8 - 1};
         for (uint32_t t = threadIdx.x; t < (uint32_t)(kRegion * kRegion * kRegion); t += 256u) {
@@ -2075,7 +2086,8 @@ void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, 
     FLIP2_KERNEL(advect_rk2_kernel, xcd_grid(ceil_div(n, 256)), dim3(256), 0, s, n, rec, U, V, W, solid, g, dt, margin, box);
     FLIP2_LAUNCH_CHECK();
 }
-void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* brickStart, float4* rec,
+void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, uint32_t* counters,
+                              const uint32_t* brickStart, float4* rec,
                               const float* U, const float* V, const float* W, const float* U0, const float* V0,
                               const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
                               SolidBox box, hipStream_t s, uint32_t* keys) {
@@ -2083,8 +2095,10 @@ void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const
     // as many workgroups as the CUs hold at once (registers and LDS), 8 per XCD row
     const uint32_t groups =
         std::min<uint32_t>(resident_groups(g2p_advect_brick_kernel, 256), 8u * ((numBricks + 7u) >> 3));
-    FLIP2_KERNEL(g2p_advect_brick_kernel, dim3(groups), dim3(256), 0, s, list, count, brickStart, rec, U, V, W, U0, V0,
-                 W0, solid, g, flip, dt, margin, box, keys);
+    FLIP2_KERNEL(fill_u32_kernel, dim3(1), dim3(64), 0, s, counters, 8u, 0u);
+    FLIP2_LAUNCH_CHECK();
+    FLIP2_KERNEL(g2p_advect_brick_kernel, dim3(groups), dim3(256), 0, s, list, count, counters, brickStart, rec, U, V,
+                 W, U0, V0, W0, solid, g, flip, dt, margin, box, keys);
     FLIP2_LAUNCH_CHECK();
 }
 void launch_g2p_advect(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const float* U0,

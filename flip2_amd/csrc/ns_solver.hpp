@@ -147,11 +147,13 @@ void launch_g2p_flip(uint32_t n, float4* rec, const float* U, const float* V, co
 void launch_advect_rk2(uint32_t n, float4* rec, const float* U, const float* V, const float* W, const uint8_t* solid,
                        Grid g, float dt, float margin, SolidBox box, hipStream_t s);
 // launch_g2p_advect by brick over the fused transfer's touched-brick list
-// (list, count: entries and their number on the device; brickStart: the
+// (list, count: entries and their number on the device; counters: 8 device
+// words, cleared here, the work counters of the 8 XCDs; brickStart: the
 // transfer's first sorted record of each brick, numBricks + 1 entries); the
 // same results.  keys (may be null): also the cell key of each particle's
 // new position (what the next transfer's binning computes).
-void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, const uint32_t* brickStart, float4* rec,
+void launch_g2p_advect_bricks(const uint32_t* list, const uint32_t* count, uint32_t* counters,
+                              const uint32_t* brickStart, float4* rec,
                               const float* U, const float* V, const float* W, const float* U0, const float* V0,
                               const float* W0, const uint8_t* solid, Grid g, float flip, float dt, float margin,
                               SolidBox box, hipStream_t s, uint32_t* keys = nullptr);

@@ -91,7 +91,7 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
             occ_.resizeAsync(numBricks, s);
             touched_.resizeAsync(numBricks, s);
             prevTouched_.resizeAsync(numBricks, s);
-            touchedList_.resizeAsync(numBricks + 1, s);
+            touchedList_.resizeAsync(numBricks + 1 + 8, s);  // list, count, 8 work counters
             brickStart_.resizeAsync(numBricks + 1, s);
             prevTouchedValid_ = false;
         }

@@ -51,6 +51,9 @@ public:
     // (valid after a runRecords with FacePrep).
     const uint32_t* touchedList() const { return touchedList_.devPtr(); }
     const uint32_t* touchedCount() const { return touchedList_.devPtr() + (numCells_ >> 9); }
+    // 8 device counters after the count, for kernels that walk the touched
+    // list with a work counter per XCD (they clear them first)
+    uint32_t* listCounters() { return touchedList_.devPtr() + (numCells_ >> 9) + 1; }
     // Per-cell counts: written by a plain run only (sparseCells() false).
     const uint32_t* counts() const { return counts_.devPtr(); }
     // The last run was a fused one (runRecords with FacePrep): cellStart
