@@ -628,23 +628,32 @@ empty_faces_kernel(const uint8_t* __restrict__ touched, const uint8_t* __restric
 // bricks in Z-order) dealt round robin to the 8 XCDs: a chunk's shells are
 // read through one L2, and every XCD gets a share of the dense and the
 // sparse parts of the scene (contiguous eighths of a Z-ordered list gave one
-// XCD each octant, i.e. most of a blob's particles to one or two XCDs).
+// XCD each octant, i.e. most of a blob's particles to one or two XCDs).  A
+// workgroup takes the next entry of its XCD's share from that XCD's counter
+// (counters[xcd], zero at launch), one entry ahead, so workgroups with
+// light bricks take more of them.
 __global__ void __launch_bounds__(256)
 p2g_prepare_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
                    const float4* __restrict__ rec, const uint32_t* __restrict__ cellStart, Grid g,
                    const uint32_t* __restrict__ stats, const uint8_t* __restrict__ solid, SolidBox box, float gdt,
                    float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, float* __restrict__ U0,
-                   float* __restrict__ V0, float* __restrict__ W0, const uint8_t* __restrict__ occ) {
+                   float* __restrict__ V0, float* __restrict__ W0, const uint8_t* __restrict__ occ,
+                   uint32_t* __restrict__ counters) {
     __shared__ P2GShared sh;
+    __shared__ uint32_t s_next;
     const uint32_t nbX = g.sizeX >> 3, nbY = g.sizeY >> 3, nbZ = g.sizeZ >> 3;
     const uint32_t total = *count;
-    const uint32_t perXcdGroups = gridDim.x >> 3, xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
+    const uint32_t xcd = blockIdx.x & 7u;  // workgroups are dealt to the XCDs round robin
     float* F[3] = {U, V, W};
     float* F0[3] = {U0, V0, W0};
-    for (uint32_t m = rank;; m += perXcdGroups) {
+    if (threadIdx.x == 0) s_next = atomicAdd(&counters[xcd], 1u);
+    __syncthreads();
+    for (uint32_t m = s_next;; m = s_next) {
         // the XCD's m-th entry: offset m % 64 of its (m / 64)-th chunk
         const uint32_t e = ((m >> 6) * 8u + xcd) * 64u + (m & 63u);
-        if (e >= total) break;  // chunks of an XCD are in increasing order
+        if (e >= total) break;  // chunks of an XCD are in increasing order (the same for the workgroup)
+        __syncthreads();        // every thread has read s_next
+        if (threadIdx.x == 0) s_next = atomicAdd(&counters[xcd], 1u);  // the next entry, during this brick
         const uint32_t brick = list[e];
         const int bb[3] = {(int)(brick % nbX), (int)((brick / nbX) % nbY), (int)(brick / (nbX * nbY))};
         float scaleW, scaleM;
@@ -666,7 +675,7 @@ p2g_prepare_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict
                 F[c][brick * 512u + t] = out[c];
             }
         }
-        __syncthreads();  // the next brick clears the sums
+        __syncthreads();  // the next brick clears the sums; s_next is written
     }
 }
 
@@ -778,8 +787,10 @@ void launch_p2g_prepare(const float4* sortedRec, const uint32_t* sortedKeys, uin
     FLIP2_LAUNCH_CHECK();
     // as many workgroups as the CUs hold at once (registers and LDS), 8 per XCD row
     const uint32_t groups = std::min<uint32_t>(resident_groups(p2g_prepare_kernel, 256), 8u * ((numBricks + 7u) >> 3));
+    FLIP2_KERNEL(zero_u32_kernel, dim3(1), dim3(64), 0, s, t.counters, 8u);
+    FLIP2_LAUNCH_CHECK();
     FLIP2_KERNEL(p2g_prepare_kernel, dim3(groups), dim3(256), 0, s, t.list, t.count, sortedRec, cellStart, g, stats,
-                 prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0, t.occ);
+                 prep.solid, prep.box, prep.gdt, U, V, W, prep.U0, prep.V0, prep.W0, t.occ, t.counters);
     FLIP2_LAUNCH_CHECK();
 }
 

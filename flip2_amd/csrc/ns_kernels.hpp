@@ -62,6 +62,7 @@ struct TouchedBricks {
     uint8_t *occ, *touched;
     uint32_t *list, *count;
     const uint8_t* prevTouched;
+    uint32_t* counters;  // 8 words: the transfer's work counter per XCD (cleared by the launch)
 };
 // launch_p2g followed by launch_prepare_faces (the same U, V, W, U0, V0, W0),
 // with the transfer done only for bricks within one brick of a particle and

@@ -100,7 +100,7 @@ void P2GPipeline::runImpl(uint32_t n, const ns::Float6* soa, const float4* rec, 
         ns::launch_p2g_prepare(sorted_.devPtr(), keysSorted_.devPtr(), n, cellStart_.devPtr(), brickStart_.devPtr(),
                                grid_, stats_.devPtr(), fp, U, V, W,
                                {occ_.devPtr(), touched_.devPtr(), touchedList_.devPtr(),
-                                touchedList_.devPtr() + numBricks, prevTouched_.devPtr()},
+                                touchedList_.devPtr() + numBricks, prevTouched_.devPtr(), listCounters()},
                                s);
         touched_.swapDevice(prevTouched_);  // this run's flags are the next run's previous ones
         prevTouchedValid_ = true;
